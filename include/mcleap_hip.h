@@ -1,0 +1,193 @@
+/*
+ * mcleap_hip.h -- C ABI of libmcleap_hip.so, the MI355X (gfx950) engine for
+ * kinetic 2-D Delaunay maintenance of moving points.
+ *
+ * The reference (hporro/mcleap) is header-only C++/CUDA with no ABI; its
+ * callers use three C++ types.  Each entry point below replaces the
+ * reference member named in its comment (path:line in the reference tree).
+ * The C++ drop-in headers include/mcleap/{Host_Triangulation,
+ * Device_Triangulation,gridCount2d}.h wrap these calls with the reference's
+ * class and method names.
+ *
+ * Conventions
+ *  - Every function returns an int status (MCL_OK = 0).  No exception or
+ *    C++ type crosses the ABI.
+ *  - Device buffers are passed as plain device pointers (void*), sizes as int.
+ *    Positions are float2 pairs (x, y); "AoS" arrays use the reference record
+ *    layouts: Vertex{pos_index, he} = 2 ints, HalfEdge{next, v, t, op} =
+ *    4 ints, Triangle{he} = 1 int (Common_Triangulation.h:42-57).
+ *  - `stream` is a hipStream_t (NULL = the default stream).  Calls are
+ *    synchronous on return unless stated, like the reference (which calls
+ *    cudaDeviceSynchronize after every launch, e.g. Device_Triangulation.h:343).
+ *  - One handle per host thread.
+ */
+#ifndef MCLEAP_HIP_H
+#define MCLEAP_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+#define MCL_OK 0
+#define MCL_ERR_HIP 1          /* a HIP runtime call failed */
+#define MCL_ERR_INPUT 2        /* invalid argument / point outside the domain */
+#define MCL_ERR_DUPLICATE 3    /* duplicate point rejected at load */
+#define MCL_ERR_TOPOLOGY 4     /* mesh invariant broken / cannot build */
+#define MCL_ERR_CAPACITY 5     /* caller buffer too small (count still reported) */
+#define MCL_ERR_UNRESOLVED 6   /* untangling left inverted triangles */
+#define MCL_ERR_ROUND_CAP 7    /* flip rounds exceeded the cap */
+#define MCL_ERR_NO_DEVICE 8    /* no usable gfx950 device */
+
+/* --------------------------------------------------------------- statistics */
+typedef struct {
+    long long rounds;          /* Lawson flip rounds executed */
+    long long flips;           /* Lawson flips */
+    long long untangle_flips;  /* B/C-step flips (untangle) */
+    long long reverts;         /* vertex moves taken back because a triangle inverted */
+    long long iterations;      /* move/legalise iterations of the kinetic step */
+    long long stalled;         /* vertices that could not reach their target */
+    long long inverted;        /* inverted triangles seen after the first move */
+    long long relocated;       /* vertices removed and re-inserted at their targets */
+    long long pred_orient[4];  /* orientation decided at FP32 / FP64 / exact-small / exact-big */
+    long long pred_incircle[4];/* incircle decided at FP32 / FP64 / exact-small / exact-big */
+    double ms;                 /* wall time of the call, host clock */
+} mcl_stats_t;
+
+/* =========================================================================
+ * Host triangulation build  (HostTriangulation, Host_Triangulation.h:24-98)
+ * ========================================================================= */
+typedef struct mcl_hmesh mcl_hmesh_t;
+
+/* HostTriangulation() + addDelaunayPoints(points) (Host_Triangulation.h:107-137,
+ * 296-300).  periodic = 0: the reference layout with 4 corners at +-1e7
+ * (vertex ids 0..3) and input point perm[k] as vertex 4+k.  periodic = 1:
+ * unit-torus triangulation of points in [0,1)^2, no corners, vertex k = perm[k].
+ * Exact predicates; duplicates are rejected (MCL_ERR_DUPLICATE). */
+int mcl_hmesh_build(const float* xy, int n, int periodic, mcl_hmesh_t** out);
+/* sizes of m_pos, m_v, m_he, m_t (Host_Triangulation.h:27-30) */
+int mcl_hmesh_sizes(const mcl_hmesh_t* m, int* npos, int* nv, int* nhe, int* nt);
+/* copy out the AoS arrays (any pointer may be NULL); perm has n entries */
+int mcl_hmesh_export(const mcl_hmesh_t* m, float* pos, int* v2, int* he4, int* t1, int* perm);
+int mcl_hmesh_free(mcl_hmesh_t* m);
+
+/* =========================================================================
+ * Device triangulation  (DeviceTriangulation, Device_Triangulation.h:128-223)
+ * ========================================================================= */
+typedef struct mcl_tri mcl_tri_t;
+
+/* DeviceTriangulation(HostTriangulation*): allocateMemory + transferToDevice
+ * (Device_Triangulation.h:232-236, 255-285).  The AoS host arrays are stored
+ * on the device as SoA (he_next, he_v, he_t, he_op, v_he, t_he, float2 pos).
+ * periodic != 0 marks a torus mesh (minimum-image geometry). */
+int mcl_tri_create(const float* pos, int npos, const int* v2, int nv, const int* he4, int nhe,
+                   const int* t1, int nt, int periodic, mcl_tri_t** out);
+/* ~DeviceTriangulation / freeMemory (Device_Triangulation.h:238-241, 287-306) */
+int mcl_tri_destroy(mcl_tri_t* t);
+/* m_pos_size, m_v_size, m_he_size, m_t_size (Device_Triangulation.h:134-137) */
+int mcl_tri_sizes(const mcl_tri_t* t, int* npos, int* nv, int* nhe, int* nt);
+/* transferToHost (Device_Triangulation.h:245-253): SoA -> AoS; any may be NULL */
+int mcl_tri_download(mcl_tri_t* t, float* pos, int* v2, int* he4, int* t1);
+/* transferToDevice (Device_Triangulation.h:255-263) */
+int mcl_tri_upload(mcl_tri_t* t, const float* pos, const int* v2, const int* he4, const int* t1);
+/* public m_pos (device glm::vec2*, Device_Triangulation.h:139): float2 device pointer */
+void* mcl_tri_pos_ptr(mcl_tri_t* t);
+
+/* movePoints(glm::vec2* d): pos[i] += d[i] for non-corner vertices
+ * (Device_Triangulation.h:395-402, kernels.cuh:38-43); d is a device float2 array */
+int mcl_tri_move(mcl_tri_t* t, const void* d_disp, void* stream);
+/* integrateMovePoints(float* dt, glm::vec2* v): pos[i] += v[i]*dt[i]
+ * (Device_Triangulation.h:403-410, kernels.cuh:45-50) */
+int mcl_tri_integrate(mcl_tri_t* t, const void* d_dt, const void* d_vel, void* stream);
+
+/* delonize / delonize2 (Device_Triangulation.h:324-390): parallel Lawson flip
+ * rounds until no edge is illegal.  Requires a mesh with no inverted
+ * triangle.  Exact incircle; MCL_ERR_ROUND_CAP if max_rounds is hit
+ * (max_rounds <= 0: unlimited).  variant 0 = device-driven rounds with
+ * worklists and FP32-first predicates, variant 1 = one launch per round with
+ * a host round trip and an FP64-first predicate (the reference's loop shape). */
+int mcl_tri_delonize(mcl_tri_t* t, int variant, int max_rounds, mcl_stats_t* st, void* stream);
+/* untangle / untangle2 (Device_Triangulation.h:422-495): rounds of B/C-step
+ * flips of creased edges (kernels.cuh:302-362, Host_Triangulation.h:425-471)
+ * with exact orientation.  Returns MCL_ERR_UNRESOLVED when inverted
+ * triangles remain (the reference prints "couldn't fix everything" and
+ * returns false). */
+int mcl_tri_untangle(mcl_tri_t* t, int max_rounds, mcl_stats_t* st, void* stream);
+/* number of triangles that are not strictly counter-clockwise (exact) */
+int mcl_tri_count_inverted(mcl_tri_t* t, long long* out, void* stream);
+/* number of interior edges that fail the exact Delaunay test */
+int mcl_tri_count_non_delaunay(mcl_tri_t* t, long long* out, void* stream);
+
+/* ---------------------------------------------------- kinetic step driver */
+/* synthetic displacement fields of the benchmark workloads */
+#define MCL_MOTION_GAUSS 0     /* Gaussian noise sigma, clamped to [0, 1-2^-24]^2 */
+#define MCL_MOTION_SHEAR 1     /* torus: (a sin 2 pi y, a cos 2 pi x) + noise sigma, wrapped */
+
+typedef struct {
+    int motion;                /* MCL_MOTION_* */
+    float sigma;               /* Gaussian noise standard deviation */
+    float amp;                 /* shear-flow amplitude */
+    unsigned long long seed;   /* counter-based RNG key */
+    unsigned long long step;   /* step index (RNG counter) */
+    int variant;               /* 0 = device-driven (fast), 1 = plain (host round trips, FP64-first) */
+    int max_iterations;        /* move/legalise iterations before stalling vertices (<=0: 64) */
+} mcl_step_params_t;
+
+/* One kinetic step: displace every non-corner vertex, keep the mesh valid
+ * (moves that would invert a triangle are taken back and retried in halves),
+ * and flip back to the exact Delaunay triangulation.  Replaces the
+ * move + untangle2 + delonize2 sequence of moving_device_mesh.cu:119-155. */
+int mcl_tri_step(mcl_tri_t* t, const mcl_step_params_t* p, mcl_stats_t* st, void* stream);
+
+/* ------------------------------------------------------------- queries */
+/* oneRing<K>(int*) (layout 0, SoA: ring[(c+1)*nv + i], count in ring[i]) and
+ * oneRing2<K>(int*) (layout 1, AoS: ring[(c+1) + K*i], count in ring[K*i]),
+ * kernels.cuh:54-121: CW rotation from v.he, at most K-1 neighbours kept.
+ * d_ring is caller-allocated device memory of nv*K ints. */
+int mcl_tri_one_ring(mcl_tri_t* t, int K, int layout, void* d_ring, void* stream);
+/* one-ring as CSR (no cap): d_offsets nv+1 ints, d_nbrs cap ints; same CW
+ * order as above.  *total = number of entries (MCL_ERR_CAPACITY if > cap). */
+int mcl_tri_one_ring_csr(mcl_tri_t* t, void* d_offsets, void* d_nbrs, int cap, int* total,
+                         void* stream);
+/* closestNeighbors<K>(ring, out) (kernels.cuh:129-146) over a CSR one-ring */
+int mcl_tri_closest(mcl_tri_t* t, const void* d_offsets, const void* d_nbrs, void* d_out,
+                    void* stream);
+
+/* =========================================================================
+ * Uniform-grid FRNN  (GridCount2d, gridCount2d.h:23-48)
+ * ========================================================================= */
+typedef struct mcl_grid mcl_grid_t;
+#define MCL_GRID_ROWMAJOR 0    /* reference cell key (y*nx + x) % (nx*ny) */
+#define MCL_GRID_MORTON 1      /* Morton-ordered cell key */
+#define MCL_GRID_PERIODIC 2    /* flag: wrap cells, minimum-image distances */
+
+/* GridCount2d(numP, min, cell_size, num_cells) (gridCount2d.h:50-71) */
+int mcl_grid_create(int nump, float minx, float miny, float csx, float csy, int ncx, int ncy,
+                    int flags, mcl_grid_t** out);
+int mcl_grid_destroy(mcl_grid_t* g);
+/* update(pos[, vel]) (gridCount2d.h:119-135, 174-260): counting sort by cell;
+ * permutes the caller's device arrays in place (vel may be NULL).  Within a
+ * cell, points keep their previous relative order (the reference's atomic
+ * order is nondeterministic). */
+int mcl_grid_update(mcl_grid_t* g, void* d_pos, void* d_vel, void* stream);
+/* d_ind: new slot of old point i (gridCount2d.h:40, 146-153); perm: old id of slot s */
+int mcl_grid_index_ptrs(mcl_grid_t* g, void** d_ind, void** d_perm, void** d_cell_start,
+                        void** d_cell_count);
+/* apply_f_frnn(f, pos, rad) (gridCount2d.h:262-302) as CSR over sorted slots:
+ * for each slot i, every slot j != i in the 3x3 cells around i with
+ * r2 = dx*dx + dy*dy <= rad*rad (FP32, no contraction).  d_offsets has
+ * nump+1 ints, d_nbrs cap ints.  *total = pair count (MCL_ERR_CAPACITY if > cap). */
+int mcl_grid_frnn_csr(mcl_grid_t* g, const void* d_pos, float rad, void* d_offsets, void* d_nbrs,
+                      int cap, long long* total, void* stream);
+/* mean_num_particle_in_cell (gridCount2d.h:304-334) */
+int mcl_grid_mean_occupancy(mcl_grid_t* g, float* out, void* stream);
+
+/* ----------------------------------------------------------------- misc */
+/* library build string and the device it runs on */
+const char* mcl_version(void);
+int mcl_device_info(char* name, int len, int* cu_count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCLEAP_HIP_H */

@@ -1,0 +1,14 @@
+"""mcleap_amd: kinetic 2-D Delaunay maintenance of moving points on MI355X.
+
+The engine is libmcleap_hip.so (hand-written gfx950 HIP kernels behind the C
+ABI in include/mcleap_hip.h).  This package holds the build script and a
+Python mirror of the reference's host interface (HostTriangulation,
+DeviceTriangulation, GridCount2d) used by tests and benchmarks.
+"""
+from ._lib import GRID_MORTON, GRID_PERIODIC, GRID_ROWMAJOR, MOTION_GAUSS, MOTION_SHEAR, MclError, lib
+from .triangulation import DeviceTriangulation, GridCount2d, HostTriangulation
+
+__all__ = [
+    "HostTriangulation", "DeviceTriangulation", "GridCount2d", "MclError", "lib",
+    "MOTION_GAUSS", "MOTION_SHEAR", "GRID_ROWMAJOR", "GRID_MORTON", "GRID_PERIODIC",
+]

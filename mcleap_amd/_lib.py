@@ -1,0 +1,130 @@
+"""ctypes binding of libmcleap_hip.so (the C ABI declared in include/mcleap_hip.h).
+
+The product path has no CPU fallback: importing a GPU entry point without the
+built library, or calling one without a GPU, raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmcleap_hip.so")
+
+MCL_OK = 0
+ERRORS = {
+    1: "HIP runtime error",
+    2: "invalid input",
+    3: "duplicate point",
+    4: "topology error",
+    5: "capacity exceeded",
+    6: "untangling left inverted triangles",
+    7: "flip round cap reached",
+    8: "no usable GPU",
+}
+
+MOTION_GAUSS = 0
+MOTION_SHEAR = 1
+GRID_ROWMAJOR = 0
+GRID_MORTON = 1
+GRID_PERIODIC = 2
+
+
+class MclError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__("%s failed: %s (status %d)" % (what, ERRORS.get(code, "unknown"), code))
+        self.code = code
+
+
+class Stats(ctypes.Structure):
+    """mcl_stats_t"""
+    _fields_ = [
+        ("rounds", ctypes.c_longlong),
+        ("flips", ctypes.c_longlong),
+        ("untangle_flips", ctypes.c_longlong),
+        ("reverts", ctypes.c_longlong),
+        ("iterations", ctypes.c_longlong),
+        ("stalled", ctypes.c_longlong),
+        ("inverted", ctypes.c_longlong),
+        ("relocated", ctypes.c_longlong),
+        ("pred_orient", ctypes.c_longlong * 4),
+        ("pred_incircle", ctypes.c_longlong * 4),
+        ("ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["pred_orient"] = list(self.pred_orient)
+        d["pred_incircle"] = list(self.pred_incircle)
+        return d
+
+
+class StepParams(ctypes.Structure):
+    """mcl_step_params_t"""
+    _fields_ = [
+        ("motion", ctypes.c_int),
+        ("sigma", ctypes.c_float),
+        ("amp", ctypes.c_float),
+        ("seed", ctypes.c_ulonglong),
+        ("step", ctypes.c_ulonglong),
+        ("variant", ctypes.c_int),
+        ("max_iterations", ctypes.c_int),
+    ]
+
+
+_L = None
+_p = ctypes.c_void_p
+_i = ctypes.c_int
+_f = ctypes.c_float
+_pp = ctypes.POINTER(ctypes.c_void_p)
+
+SIGNATURES = {
+    "mcl_hmesh_build": [_p, _i, _i, _pp],
+    "mcl_hmesh_sizes": [_p, _p, _p, _p, _p],
+    "mcl_hmesh_export": [_p, _p, _p, _p, _p, _p],
+    "mcl_hmesh_free": [_p],
+    "mcl_tri_create": [_p, _i, _p, _i, _p, _i, _p, _i, _i, _pp],
+    "mcl_tri_destroy": [_p],
+    "mcl_tri_sizes": [_p, _p, _p, _p, _p],
+    "mcl_tri_download": [_p, _p, _p, _p, _p],
+    "mcl_tri_upload": [_p, _p, _p, _p, _p],
+    "mcl_tri_pos_ptr": [_p],
+    "mcl_tri_move": [_p, _p, _p],
+    "mcl_tri_integrate": [_p, _p, _p, _p],
+    "mcl_tri_delonize": [_p, _i, _i, _p, _p],
+    "mcl_tri_untangle": [_p, _i, _p, _p],
+    "mcl_tri_count_inverted": [_p, _p, _p],
+    "mcl_tri_count_non_delaunay": [_p, _p, _p],
+    "mcl_tri_step": [_p, _p, _p, _p],
+    "mcl_tri_one_ring": [_p, _i, _i, _p, _p],
+    "mcl_tri_one_ring_csr": [_p, _p, _p, _i, _p, _p],
+    "mcl_tri_closest": [_p, _p, _p, _p, _p],
+    "mcl_grid_create": [_i, _f, _f, _f, _f, _i, _i, _i, _pp],
+    "mcl_grid_destroy": [_p],
+    "mcl_grid_update": [_p, _p, _p, _p],
+    "mcl_grid_index_ptrs": [_p, _pp, _pp, _pp, _pp],
+    "mcl_grid_frnn_csr": [_p, _p, _f, _p, _p, _i, _p, _p],
+    "mcl_grid_mean_occupancy": [_p, _p, _p],
+    "mcl_version": [],
+    "mcl_device_info": [_p, _i, _p],
+}
+RESTYPES = {"mcl_tri_pos_ptr": _p, "mcl_version": ctypes.c_char_p}
+
+
+def lib():
+    """Load libmcleap_hip.so; raises if it has not been built."""
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libmcleap_hip.so is not built (run `python -m mcleap_amd.build`)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = RESTYPES.get(name, _i)
+        _L = L
+    return _L
+
+
+def check(rc, what):
+    if rc != MCL_OK:
+        raise MclError(rc, what)
+    return rc

@@ -1,0 +1,297 @@
+// mcl_grid.hip -- uniform-grid fixed-radius neighbours (GridCount2d,
+// gridCount2d.h:23-302) as a counting sort plus a CSR query.
+//
+// update():   key -> histogram (global atomics) -> exclusive scan ->
+//             slot assignment -> per-cell sort of the slots by previous
+//             index (deterministic; the reference's atomic order is not) ->
+//             gather into the sorted arrays -> copy back over the caller's
+//             buffers (the reference sorts the caller's array in place,
+//             gridCount2d.h:207-208, 256).
+// frnn_csr(): per sorted slot i, the 3x3 cells around i's cell; pairs with
+//             r2 = dx*dx + dy*dy <= rad*rad in FP32 without contraction
+//             (gridCount2d.h:284-290), j != i.  Two passes (count, scan,
+//             fill), so the CSR is deterministic.
+// Cell keys: MCL_GRID_ROWMAJOR reproduces calcHash / calc_hashxy
+// (gridCount2d.h:89-108) including the unsigned wrap and the query's
+// `hh.x > nx` bound (gridCount2d.h:278-281); MCL_GRID_MORTON orders cells
+// along a Morton curve (cells of a tile are contiguous);
+// MCL_GRID_PERIODIC wraps cells and distances on the unit torus.
+#include <cstdio>
+
+#include "mcl_geom.hpp"
+#include "mcl_internal.hpp"
+#include "mcl_scan.hpp"
+
+using namespace mcl;
+
+namespace {
+
+constexpr int kT = 256;
+
+struct GridParams {
+    float minx, miny, csx, csy;
+    int nx, ny, tot, nkeys;
+    int morton, periodic;
+};
+
+__host__ __device__ __forceinline__ uint32_t spread16(uint32_t x) {
+    x &= 0xFFFFu;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
+// calc_hashxy (gridCount2d.h:94-102): FP32 subtract, divide, truncate, C %
+__device__ __forceinline__ int2 cell_of(const GridParams& g, float2 p) {
+    float qx = __fsub_rn(p.x, g.minx), qy = __fsub_rn(p.y, g.miny);
+    int x = (int)__fdiv_rn(qx, g.csx), y = (int)__fdiv_rn(qy, g.csy);
+    x %= g.nx;
+    y %= g.ny;
+    if (g.periodic || g.morton) {
+        if (g.periodic) {
+            x = (x + g.nx) % g.nx;
+            y = (y + g.ny) % g.ny;
+        } else {
+            x = min(max(x, 0), g.nx - 1);
+            y = min(max(y, 0), g.ny - 1);
+        }
+    }
+    return make_int2(x, y);
+}
+__device__ __forceinline__ int key_of(const GridParams& g, int x, int y) {
+    if (g.morton) return (int)(spread16((uint32_t)x) | (spread16((uint32_t)y) << 1));
+    return (int)(((unsigned)y * (unsigned)g.nx + (unsigned)x) % (unsigned)g.tot);  // calcHash
+}
+
+__global__ void k_zero(int* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+__global__ void __launch_bounds__(kT) k_key_count(GridParams g, const float2* __restrict__ pos, int n,
+                                                  int* __restrict__ key, int* __restrict__ count) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int2 c = cell_of(g, pos[i]);
+        int k = key_of(g, c.x, c.y);
+        key[i] = k;
+        atomicAdd(&count[k], 1);
+    }
+}
+
+__global__ void __launch_bounds__(kT) k_slot(const int* __restrict__ key, int n, const int* __restrict__ start,
+                                             int* __restrict__ fill, int* __restrict__ perm) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int k = key[i];
+        perm[start[k] + atomicAdd(&fill[k], 1)] = i;
+    }
+}
+
+// order each cell's slots by previous index (insertion sort, cells are small)
+__global__ void __launch_bounds__(kT) k_cell_sort(const int* __restrict__ start, int nkeys, int* __restrict__ perm) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nkeys; c += gridDim.x * blockDim.x) {
+        int b = start[c], e = start[c + 1];
+        for (int i = b + 1; i < e; i++) {
+            int v = perm[i], j = i - 1;
+            while (j >= b && perm[j] > v) { perm[j + 1] = perm[j]; j--; }
+            perm[j + 1] = v;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kT) k_gather(const int* __restrict__ perm, int n, const float2* __restrict__ pos,
+                                               const float2* __restrict__ vel, float2* __restrict__ spos,
+                                               float2* __restrict__ svel, int* __restrict__ ind) {
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        int i = perm[s];
+        spos[s] = pos[i];
+        if (vel) svel[s] = vel[i];
+        ind[i] = s;
+    }
+}
+
+// pass 0 counts, pass 1 writes
+template <int kPass>
+__global__ void __launch_bounds__(kT) k_frnn(GridParams g, const float2* __restrict__ pos, int n,
+                                             const int* __restrict__ start, float rad2, int* __restrict__ cnt,
+                                             const int* __restrict__ off, int* __restrict__ nbr, int cap) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float2 pi = pos[i];
+        const int2 h = cell_of(g, pi);
+        int c = 0, w = kPass ? off[i] : 0;
+        for (int ddx = -1; ddx <= 1; ddx++) {
+            for (int ddy = -1; ddy <= 1; ddy++) {
+                int x = h.x + ddx, y = h.y + ddy;
+                if (g.periodic) {
+                    x = (x + g.nx) % g.nx;
+                    y = (y + g.ny) % g.ny;
+                } else if (g.morton) {
+                    if (x < 0 || x >= g.nx || y < 0 || y >= g.ny) continue;
+                } else {
+                    if (x > g.nx || x < 0) continue;  // gridCount2d.h:278-281 bound test
+                    if (y > g.ny || y < 0) continue;
+                }
+                const int k = key_of(g, x, y);
+                const int b = start[k], e = start[k + 1];
+                for (int j = b; j < e; j++) {
+                    if (j == i) continue;
+                    const float2 pj = pos[j];
+                    float dx = __fsub_rn(pj.x, pi.x), dy = __fsub_rn(pj.y, pi.y);
+                    if (g.periodic) { dx = min_image_f(dx); dy = min_image_f(dy); }
+                    const float r2 = __fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy));
+                    if (r2 <= rad2) {
+                        if (kPass && w < cap) nbr[w] = j;
+                        w++;
+                        c++;
+                    }
+                }
+            }
+        }
+        if (!kPass) cnt[i] = c;
+    }
+}
+
+__global__ void k_occupancy(const int* __restrict__ start, int nkeys, unsigned long long* out) {
+    unsigned long long nonempty = 0;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nkeys; c += gridDim.x * blockDim.x)
+        nonempty += (start[c + 1] > start[c]);
+    if (nonempty) atomicAdd(out, nonempty);
+}
+
+int blocks(long long n) {
+    long long b = (n + kT - 1) / kT;
+    return (int)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
+}
+
+}  // namespace
+
+struct mcl_grid {
+    GridParams g;
+    int nump = 0;
+    int* key = nullptr;
+    int* start = nullptr;  // nkeys + 1 (counts, then exclusive offsets)
+    int* fill = nullptr;
+    int* perm = nullptr;
+    int* ind = nullptr;
+    int* cnt = nullptr;    // nump + 1
+    int* scan_tmp = nullptr;
+    float2* spos = nullptr;
+    float2* svel = nullptr;
+    unsigned long long* scalar = nullptr;
+};
+
+static void grid_free(mcl_grid* G) {
+    void* p[] = {G->key, G->start, G->fill, G->perm, G->ind, G->cnt, G->scan_tmp, G->spos, G->svel, G->scalar};
+    for (void* q : p)
+        if (q) (void)hipFree(q);
+}
+
+extern "C" int mcl_grid_create(int nump, float minx, float miny, float csx, float csy, int ncx, int ncy, int flags,
+                               mcl_grid_t** out) {
+    if (!out || nump < 0 || ncx <= 0 || ncy <= 0 || !(csx > 0) || !(csy > 0)) return MCL_ERR_INPUT;
+    const int morton = (flags & 1) ? 1 : 0, periodic = (flags & MCL_GRID_PERIODIC) ? 1 : 0;
+    if (morton && (ncx > 65536 || ncy > 65536)) return MCL_ERR_INPUT;
+    if (periodic && (ncx < 3 || ncy < 3)) return MCL_ERR_INPUT;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MCL_ERR_NO_DEVICE;
+    mcl_grid* G = new mcl_grid();
+    G->nump = nump;
+    G->g = {minx, miny, csx, csy, ncx, ncy, ncx * ncy, ncx * ncy, morton, periodic};
+    if (morton) {
+        int side = 1;
+        while (side < ncx || side < ncy) side <<= 1;
+        G->g.nkeys = side * side;
+    }
+    const int nk = G->g.nkeys;
+    bool ok = hipMalloc(&G->key, sizeof(int) * (nump + 1)) == hipSuccess &&
+              hipMalloc(&G->start, sizeof(int) * (nk + 1)) == hipSuccess &&
+              hipMalloc(&G->fill, sizeof(int) * (nk + 1)) == hipSuccess &&
+              hipMalloc(&G->perm, sizeof(int) * (nump + 1)) == hipSuccess &&
+              hipMalloc(&G->ind, sizeof(int) * (nump + 1)) == hipSuccess &&
+              hipMalloc(&G->cnt, sizeof(int) * (nump + 1)) == hipSuccess &&
+              hipMalloc(&G->scan_tmp, sizeof(int) * (scan_scratch_ints((nk > nump ? nk : nump) + 1) + 16)) == hipSuccess &&
+              hipMalloc(&G->spos, sizeof(float2) * (nump + 1)) == hipSuccess &&
+              hipMalloc(&G->svel, sizeof(float2) * (nump + 1)) == hipSuccess &&
+              hipMalloc(&G->scalar, sizeof(unsigned long long) * 2) == hipSuccess;
+    if (!ok) { grid_free(G); delete G; return MCL_ERR_HIP; }
+    hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, 0, G->start, nk + 1);
+    if (hipDeviceSynchronize() != hipSuccess) { grid_free(G); delete G; return MCL_ERR_HIP; }
+    *out = G;
+    return MCL_OK;
+}
+
+extern "C" int mcl_grid_destroy(mcl_grid_t* G) {
+    if (!G) return MCL_OK;
+    (void)hipDeviceSynchronize();
+    grid_free(G);
+    delete G;
+    return MCL_OK;
+}
+
+extern "C" int mcl_grid_update(mcl_grid_t* G, void* d_pos, void* d_vel, void* stream) {
+    if (!G || !d_pos) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    const int n = G->nump, nk = G->g.nkeys;
+    float2* pos = (float2*)d_pos;
+    float2* vel = (float2*)d_vel;
+    hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, G->start, nk + 1);
+    hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, G->fill, nk + 1);
+    hipLaunchKernelGGL(k_key_count, dim3(blocks(n)), dim3(kT), 0, s, G->g, pos, n, G->key, G->start);
+    int rc = scan_exclusive(G->start, G->start, nk, G->scan_tmp, s);
+    if (rc != MCL_OK) return rc;
+    hipLaunchKernelGGL(k_slot, dim3(blocks(n)), dim3(kT), 0, s, G->key, n, G->start, G->fill, G->perm);
+    hipLaunchKernelGGL(k_cell_sort, dim3(blocks(nk)), dim3(kT), 0, s, G->start, nk, G->perm);
+    hipLaunchKernelGGL(k_gather, dim3(blocks(n)), dim3(kT), 0, s, G->perm, n, pos, vel, G->spos, G->svel, G->ind);
+    MCL_HIP(hipGetLastError());
+    MCL_HIP(hipMemcpyAsync(pos, G->spos, sizeof(float2) * n, hipMemcpyDeviceToDevice, s));
+    if (vel) MCL_HIP(hipMemcpyAsync(vel, G->svel, sizeof(float2) * n, hipMemcpyDeviceToDevice, s));
+    MCL_HIP(hipStreamSynchronize(s));
+    return MCL_OK;
+}
+
+extern "C" int mcl_grid_index_ptrs(mcl_grid_t* G, void** d_ind, void** d_perm, void** d_cell_start,
+                                   void** d_cell_count) {
+    if (!G) return MCL_ERR_INPUT;
+    if (d_ind) *d_ind = G->ind;
+    if (d_perm) *d_perm = G->perm;
+    if (d_cell_start) *d_cell_start = G->start;
+    if (d_cell_count) *d_cell_count = nullptr;  // counts are start[c+1] - start[c]
+    return MCL_OK;
+}
+
+extern "C" int mcl_grid_frnn_csr(mcl_grid_t* G, const void* d_pos, float rad, void* d_offsets, void* d_nbrs, int cap,
+                                 long long* total, void* stream) {
+    if (!G || !d_pos || !d_offsets) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    const int n = G->nump;
+    const float rad2 = rad * rad;
+    int* off = (int*)d_offsets;
+    hipLaunchKernelGGL(k_frnn<0>, dim3(blocks(n)), dim3(kT), 0, s, G->g, (const float2*)d_pos, n, G->start, rad2, off,
+                       (const int*)nullptr, (int*)nullptr, 0);
+    int rc = scan_exclusive(off, off, n, G->scan_tmp, s);
+    if (rc != MCL_OK) return rc;
+    int tot = 0;
+    MCL_HIP(hipMemcpyAsync(&tot, off + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    MCL_HIP(hipStreamSynchronize(s));
+    if (total) *total = tot;
+    if (d_nbrs && cap > 0) {
+        hipLaunchKernelGGL(k_frnn<1>, dim3(blocks(n)), dim3(kT), 0, s, G->g, (const float2*)d_pos, n, G->start, rad2,
+                           (int*)nullptr, (const int*)off, (int*)d_nbrs, cap);
+        MCL_HIP(hipGetLastError());
+        MCL_HIP(hipStreamSynchronize(s));
+    }
+    return tot > cap ? MCL_ERR_CAPACITY : MCL_OK;
+}
+
+extern "C" int mcl_grid_mean_occupancy(mcl_grid_t* G, float* out, void* stream) {
+    if (!G || !out) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, (int*)G->scalar, 2);
+    hipLaunchKernelGGL(k_occupancy, dim3(blocks(G->g.nkeys)), dim3(kT), 0, s, G->start, G->g.nkeys, G->scalar);
+    MCL_HIP(hipGetLastError());
+    unsigned long long ne = 0;
+    MCL_HIP(hipMemcpyAsync(&ne, G->scalar, sizeof(ne), hipMemcpyDeviceToHost, s));
+    MCL_HIP(hipStreamSynchronize(s));
+    *out = ne ? (float)G->nump / (float)ne : 0.0f;
+    return MCL_OK;
+}

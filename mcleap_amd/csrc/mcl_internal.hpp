@@ -1,0 +1,76 @@
+// mcl_internal.hpp -- declarations shared by the translation units of
+// libmcleap_hip.so.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/mcleap_hip.h"
+
+#define MCL_HIP(call)                                  \
+    do {                                               \
+        hipError_t _e = (call);                        \
+        if (_e != hipSuccess) return mcl::hip_fail(_e, #call, __FILE__, __LINE__); \
+    } while (0)
+
+namespace mcl {
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+// ---------------------------------------------------------------- host mesh
+struct HostMesh {
+    bool periodic = false;
+    int nv = 0;
+    std::vector<float> pos;   // 2 * nv
+    std::vector<int> vrec;    // 2 * nv  (pos_index, he)
+    std::vector<int> he;      // 4 * nhe (next, v, t, op)
+    std::vector<int> t;       // nt
+    std::vector<int> perm;    // n input indices in vertex order
+    long long flips = 0;
+};
+int build_box(const float* xy, int n, HostMesh& m);
+int build_torus(const float* xy, int n, HostMesh& m);
+std::vector<int> hilbert_order(const float* xy, int n);
+
+// ------------------------------------------------------------- device mesh
+// Structure-of-arrays half-edge mesh.  Half-edge h and its twin h^1 are
+// adjacent, so the pair (2e, 2e+1) of edge e is one 8-byte load in every
+// array.  Vertex records and triangle records are separate arrays.
+struct DMesh {
+    int nv = 0, nhe = 0, nt = 0, ne = 0;
+    int periodic = 0;
+    float2* pos = nullptr;
+    int* he_next = nullptr;
+    int* he_v = nullptr;
+    int* he_t = nullptr;
+    int* he_op = nullptr;
+    int* v_he = nullptr;
+    int* v_pidx = nullptr;
+    int* t_he = nullptr;
+};
+
+// statistics slots kept on the device (unsigned long long)
+enum StatSlot : int {
+    kStRounds = 0, kStFlips, kStUntangleFlips, kStReverts, kStIterations, kStStalled, kStInverted,
+    kStOrient0, kStOrient1, kStOrient2, kStOrient3,
+    kStIncircle0, kStIncircle1, kStIncircle2, kStIncircle3,
+    kStRelocated, kStRelocFailed,
+    kStCount
+};
+
+// device counters (int)
+enum CountSlot : int {
+    kCnCand = 0,      // 3 slots: candidate edge list sizes, by round % 3
+    kCnWants = 3,     // 3 slots: edges that want to flip, by round % 3
+    kCnBad = 6,       // 3 slots: inverted triangles found, by pass % 3
+    kCnRev = 9,       // 3 slots: vertices reverted in the pass
+    kCnMove = 12,     // vertices moving in this iteration
+    kCnPend = 13,     // vertices still short of their target
+    kCnErr = 14,      // error flags
+    kCnMisc = 15,
+    kCnTouched = 16,  // vertices of quads flipped by repair rounds
+    kCnCount = 32
+};
+
+}  // namespace mcl

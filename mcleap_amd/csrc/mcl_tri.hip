@@ -1,0 +1,1682 @@
+// mcl_tri.hip -- device triangulation: SoA half-edge mesh, kinetic moves,
+// validity checks, Lawson flip rounds, B/C untangling, one-ring gathers.
+//
+// Flip rounds (replacing flip_delaunay_kernel + delonize2,
+// kernels.cuh:274-300, Device_Triangulation.h:324-351):
+//   test kernel  -- for each candidate edge: exact incircle of its quad;
+//                   an illegal edge claims both triangles with a 64-bit
+//                   atomicMax of (round | hash | edge).  The round sits in
+//                   the high bits, so claims never need resetting.
+//   flip kernel  -- an edge that holds both claims flips (f2to2 semantics,
+//                   Common_Triangulation.h:69-127, on SoA) and queues the
+//                   four edges of its quad for the next round; a loser
+//                   re-queues itself.  Winners own disjoint triangle pairs,
+//                   so no flip reads or writes another winner's records.
+// Queues are deduplicated with per-edge round stamps.  Counters live in
+// device memory in round-indexed slots (round % 3), each slot cleared by the
+// kernel two steps before its next use, so no memset is needed between
+// launches (safe under graph capture).
+#include <chrono>
+#include <cstdlib>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "mcl_geom.hpp"
+#include "mcl_internal.hpp"
+#include "mcl_scan.hpp"
+
+using namespace mcl;
+
+namespace mcl {
+
+constexpr int kThreads = 256;
+
+// ------------------------------------------------------------------ Philox
+// Philox4x32-10 (Salmon et al., SC'11) keyed by the seed, counter = (id, step)
+struct U4 { uint32_t x, y, z, w; };
+__device__ __forceinline__ U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t hi0 = __umulhi(M0, c0), lo0 = M0 * c0;
+        uint32_t hi1 = __umulhi(M1, c2), lo1 = M1 * c2;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += W0; k1 += W1;
+    }
+    return {c0, c1, c2, c3};
+}
+// two standard normals by Box-Muller from one Philox draw
+__device__ __forceinline__ float2 gauss2(uint32_t id, unsigned long long step, unsigned long long seed) {
+    U4 r = philox(id, (uint32_t)step, (uint32_t)(step >> 32), 0x6d636c70u, (uint32_t)seed, (uint32_t)(seed >> 32));
+    float u1 = ((r.x >> 8) + 1u) * 5.9604644775390625e-08f;  // (0, 1]
+    float u2 = (r.y >> 8) * 5.9604644775390625e-08f;         // [0, 1)
+    float rad = sqrtf(-2.0f * logf(u1));
+    float s, c;
+    sincospif(2.0f * u2, &s, &c);
+    return make_float2(rad * c, rad * s);
+}
+
+__device__ __forceinline__ bool is_corner(float2 p) { return fabsf(p.x) >= 1.0e6f || fabsf(p.y) >= 1.0e6f; }
+
+__device__ __forceinline__ float wrap01(float x) {
+    float w = x - floorf(x);
+    return (w >= 1.0f || w < 0.0f) ? 0.0f : w;
+}
+__device__ __forceinline__ float clamp_box(float x) {
+    const float hi = 0.99999994039535522f;  // 1 - 2^-24
+    return fminf(fmaxf(x, 0.0f), hi) + 0.0f;
+}
+
+// ---------------------------------------------------- statistics helpers
+// per-thread counters reduced in LDS, one global atomic per block and slot
+template <int N>
+__device__ __forceinline__ void flush_stats(const unsigned (&loc)[N], unsigned long long* stats, int slot0) {
+    __shared__ unsigned acc[16];
+    __syncthreads();
+    if (threadIdx.x < N) acc[threadIdx.x] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < N; k++)
+        if (loc[k]) atomicAdd(&acc[k], loc[k]);
+    __syncthreads();
+    if (threadIdx.x < N && acc[threadIdx.x]) atomicAdd(&stats[slot0 + threadIdx.x], (unsigned long long)acc[threadIdx.x]);
+}
+
+__device__ __forceinline__ void tri_verts(const DMesh& m, int t, int& a, int& b, int& c) {
+    int h = m.t_he[t];
+    int2 vv = reinterpret_cast<const int2*>(m.he_v)[h >> 1];
+    a = (h & 1) ? vv.y : vv.x;
+    b = (h & 1) ? vv.x : vv.y;
+    c = m.he_op[h];
+}
+
+__device__ __forceinline__ unsigned long long claim_key(int round, int e) {
+    uint32_t h = (uint32_t)e * 0x9E3779B1u ^ (uint32_t)round * 0x85EBCA77u;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    return ((unsigned long long)(uint32_t)round << 36) | ((unsigned long long)(h & 0xFFu) << 28) |
+           (unsigned long long)(uint32_t)e;
+}
+
+__device__ __forceinline__ void flip_soa(const DMesh& m, int e, int& he2, int& he3, int& he4, int& he5) {
+    const int he0 = 2 * e, he1 = he0 + 1;
+    const int2 nx = reinterpret_cast<const int2*>(m.he_next)[e];
+    he2 = nx.x;
+    he4 = nx.y;
+    he3 = m.he_next[he2];
+    he5 = m.he_next[he4];
+    const int2 tt = reinterpret_cast<const int2*>(m.he_t)[e];
+    const int2 vv = reinterpret_cast<const int2*>(m.he_v)[e];
+    const int2 oo = reinterpret_cast<const int2*>(m.he_op)[e];
+    const int t0 = tt.x, t1 = tt.y, v0 = vv.x, v1 = vv.y, v2 = oo.x, v3 = oo.y;
+    m.t_he[t0] = he0;
+    m.t_he[t1] = he1;
+    m.v_he[v1] = he2;
+    m.v_he[v0] = he4;
+    // t0 = (v2 -> v3 -> v1): he0, he5, he2 ;  t1 = (v3 -> v2 -> v0): he1, he3, he4
+    reinterpret_cast<int2*>(m.he_next)[e] = make_int2(he5, he3);
+    m.he_next[he5] = he2;
+    m.he_next[he2] = he0;
+    m.he_next[he3] = he4;
+    m.he_next[he4] = he1;
+    reinterpret_cast<int2*>(m.he_v)[e] = make_int2(v2, v3);
+    m.he_t[he5] = t0;
+    m.he_t[he3] = t1;
+    reinterpret_cast<int2*>(m.he_op)[e] = make_int2(v1, v0);
+    m.he_op[he5] = v2;
+    m.he_op[he2] = v3;
+    m.he_op[he3] = v3;
+    m.he_op[he4] = v2;
+}
+
+// -------------------------------------------------------------- movement
+// synthetic displacement of every non-corner vertex; stores the accepted
+// position (acc), the target (tgt) and moves the vertex to its target
+__global__ void __launch_bounds__(kThreads) k_move_motion(DMesh m, float2* __restrict__ acc, float2* __restrict__ tgt,
+                                                          int* __restrict__ v_mv, int* __restrict__ v_fexp, int stamp,
+                                                          mcl_step_params_t p) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
+        float2 q = m.pos[v];
+        acc[v] = q;
+        if (is_corner(q)) { tgt[v] = q; continue; }
+        float2 g = gauss2((uint32_t)v, p.step, p.seed);
+        float2 t;
+        if (p.motion == MCL_MOTION_SHEAR) {
+            float dx = p.amp * sinpif(2.0f * q.y) + p.sigma * g.x;
+            float dy = p.amp * cospif(2.0f * q.x) + p.sigma * g.y;
+            t = make_float2(wrap01(q.x + dx), wrap01(q.y + dy));
+        } else {
+            t = make_float2(clamp_box(q.x + p.sigma * g.x), clamp_box(q.y + p.sigma * g.y));
+        }
+        tgt[v] = t;
+        m.pos[v] = t;
+        v_mv[v] = stamp;
+        v_fexp[v] = 0;
+    }
+}
+
+// movePoints(d): kernels.cuh:38-43 (corners stay fixed)
+__global__ void k_move_disp(DMesh m, const float2* __restrict__ d) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
+        float2 q = m.pos[v];
+        if (is_corner(q)) continue;
+        float2 dd = d[v];
+        m.pos[v] = make_float2(q.x + dd.x, q.y + dd.y);
+    }
+}
+// integrateMovePoints(dt, v): kernels.cuh:45-50
+__global__ void k_integrate(DMesh m, const float* __restrict__ dt, const float2* __restrict__ vel) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
+        float2 q = m.pos[v];
+        if (is_corner(q)) continue;
+        float2 w = vel[v];
+        float t = dt[v];
+        m.pos[v] = make_float2(q.x + w.x * t, q.y + w.y * t);
+    }
+}
+
+// ---------------------------------------------------- validity (orientation)
+// every triangle: strictly CCW?  non-CCW triangles are appended to `bad`
+template <bool kF32>
+__global__ void __launch_bounds__(kThreads) k_check_all(DMesh m, int* __restrict__ bad, int* __restrict__ cnt, int slot,
+                                                        unsigned long long* __restrict__ stats) {
+    unsigned loc[4] = {0, 0, 0, 0};
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m.nt; t += gridDim.x * blockDim.x) {
+        int a, b, c, st;
+        tri_verts(m, t, a, b, c);
+        int s = orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st);
+        loc[st]++;
+        if (s <= 0) {
+            int k = atomicAdd(&cnt[slot], 1);
+            if (bad) bad[k] = t;
+        }
+    }
+    flush_stats(loc, stats, kStOrient0);
+}
+
+// Taking back moves, phase 1 (positions and topology only read, so every
+// decision is race-free).  For each bad triangle:
+//  - made by a repair flip of this iteration: undo that flip;
+//  - otherwise: take back the one vertex, among those that moved in this
+//    iteration and are not back at their accepted position, with the largest
+//    displacement.
+// Every bad triangle has an action: an original triangle whose vertices are
+// all at accepted positions is valid (the iteration's starting state is).
+__global__ void __launch_bounds__(kThreads) k_revert_mark(DMesh m, const int* __restrict__ bad, int* __restrict__ cnt,
+                                                          int slot_bad, const float2* __restrict__ acc,
+                                                          const int* __restrict__ v_mv, int mv_stamp,
+                                                          const int* __restrict__ t_rep, const int* __restrict__ t_rep_e,
+                                                          int rep_stamp, int* __restrict__ e_undo, int undo_stamp,
+                                                          int* __restrict__ undo_list, int slot_undo,
+                                                          int* __restrict__ v_rev, int rev_stamp, int* __restrict__ rev_list,
+                                                          int slot_rev, int* __restrict__ stuck, int slot_stuck) {
+    const int n = cnt[slot_bad];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int t = bad[i];
+        if (t_rep[t] == rep_stamp) {
+            const int e = t_rep_e[t];
+            if (atomicExch(&e_undo[e], undo_stamp) != undo_stamp) undo_list[atomicAdd(&cnt[slot_undo], 1)] = e;
+            continue;
+        }
+        int vs[3];
+        tri_verts(m, t, vs[0], vs[1], vs[2]);
+        int best = -1;
+        float bd = -1.0f;
+        for (int k = 0; k < 3; k++) {
+            const int v = vs[k];
+            if (v_mv[v] != mv_stamp) continue;
+            const float2 q = m.pos[v], a = acc[v];
+            if (q.x == a.x && q.y == a.y) continue;
+            float dx = q.x - a.x, dy = q.y - a.y;
+            if (m.periodic) { dx = min_image_f(dx); dy = min_image_f(dy); }
+            const float d = dx * dx + dy * dy;
+            if (d > bd) { bd = d; best = v; }
+        }
+        if (best < 0) {
+            stuck[atomicAdd(&cnt[slot_stuck], 1)] = t;
+            continue;
+        }
+        if (atomicExch(&v_rev[best], rev_stamp) != rev_stamp) rev_list[atomicAdd(&cnt[slot_rev], 1)] = best;
+    }
+}
+
+// phase 2a: move the marked vertices back and halve their next attempt
+__global__ void __launch_bounds__(kThreads) k_revert_apply(DMesh m, const int* __restrict__ rev_list,
+                                                           const int* __restrict__ cnt, int slot_rev,
+                                                           const float2* __restrict__ acc, int* __restrict__ v_fexp,
+                                                           int* __restrict__ all_rev, int* __restrict__ cnt_w, int slot_all,
+                                                           unsigned long long* __restrict__ stats) {
+    const int n = cnt[slot_rev];
+    unsigned loc[1] = {0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int v = rev_list[i];
+        m.pos[v] = acc[v];
+        v_fexp[v] = min(v_fexp[v] + 1, 64);
+        all_rev[atomicAdd(&cnt_w[slot_all], 1)] = v;
+        loc[0]++;
+    }
+    flush_stats(loc, stats, kStReverts);
+}
+
+// phase 2b: undo marked repair flips (a repair's two triangles are never
+// touched by another repair of the same iteration, so each undo restores
+// them exactly) and queue the restored triangles that are still bad
+template <bool kF32>
+__global__ void __launch_bounds__(kThreads) k_undo_apply(DMesh m, const int* __restrict__ undo_list, int* __restrict__ cnt,
+                                                         int slot_undo, int* __restrict__ t_rep, int* __restrict__ bad_out,
+                                                         int slot_out, unsigned long long* __restrict__ stats) {
+    const int n = cnt[slot_undo];
+    unsigned loc[4] = {0, 0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int e = undo_list[i];
+        int h2, h3, h4, h5;
+        flip_soa(m, e, h2, h3, h4, h5);
+        const int2 tt = reinterpret_cast<const int2*>(m.he_t)[e];
+        t_rep[tt.x] = 0;
+        t_rep[tt.y] = 0;
+        const int ts[2] = {tt.x, tt.y};
+        for (int k = 0; k < 2; k++) {
+            int a, b, c, st;
+            tri_verts(m, ts[k], a, b, c);
+            int sg = orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st);
+            loc[st]++;
+            if (sg <= 0) bad_out[atomicAdd(&cnt[slot_out], 1)] = ts[k];
+        }
+    }
+    flush_stats(loc, stats, kStOrient0);
+}
+
+// Repair rounds: keep the triangles of `bad` that are still not strictly CCW
+// and queue their edges as candidates of round `round`.
+template <bool kF32>
+__global__ void __launch_bounds__(kThreads) k_filter_bad(DMesh m, const int* __restrict__ bad, int* __restrict__ cnt,
+                                                         int slot_in, int* __restrict__ bad_out, int slot_out,
+                                                         int* __restrict__ e_stamp, int round, int* __restrict__ cand,
+                                                         unsigned long long* __restrict__ stats) {
+    const int n = cnt[slot_in];
+    unsigned loc[4] = {0, 0, 0, 0};
+    const int slot_c = kCnCand + round % 3;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int t = bad[i];
+        int a, b, c, st;
+        tri_verts(m, t, a, b, c);
+        if (orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st) > 0) { loc[st]++; continue; }
+        loc[st]++;
+        bad_out[atomicAdd(&cnt[slot_out], 1)] = t;
+        const int h0 = m.t_he[t], h1 = m.he_next[h0], h2 = m.he_next[h1];
+        const int es[3] = {h0 >> 1, h1 >> 1, h2 >> 1};
+        for (int k = 0; k < 3; k++)
+            if (atomicExch(&e_stamp[es[k]], round) != round) cand[atomicAdd(&cnt[slot_c], 1)] = es[k];
+    }
+    flush_stats(loc, stats, kStOrient0);
+}
+
+// winners of a repair round flip; their two triangles are marked with the
+// iteration stamp and the edge (for a possible undo); the quad's vertices are
+// recorded so that later iterations re-queue the quad's edges for Lawson
+__global__ void __launch_bounds__(kThreads) k_repair_flip(DMesh m, int* __restrict__ cnt, int round,
+                                                          const int* __restrict__ wants,
+                                                          const unsigned long long* __restrict__ claim,
+                                                          int* __restrict__ t_rep, int* __restrict__ t_rep_e, int rep_stamp,
+                                                          int* __restrict__ v_touch, int* __restrict__ touched,
+                                                          int slot_touched, unsigned long long* __restrict__ stats) {
+    const int r3 = round % 3;
+    const int n = cnt[kCnWants + r3];
+    unsigned loc[1] = {0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int e = wants[i];
+        const int2 tt = reinterpret_cast<const int2*>(m.he_t)[e];
+        const unsigned long long key = claim_key(round, e);
+        if (claim[tt.x] == key && claim[tt.y] == key) {
+            const int2 vv = reinterpret_cast<const int2*>(m.he_v)[e];
+            const int2 oo = reinterpret_cast<const int2*>(m.he_op)[e];
+            int h2, h3, h4, h5;
+            flip_soa(m, e, h2, h3, h4, h5);
+            t_rep[tt.x] = rep_stamp; t_rep[tt.y] = rep_stamp;
+            t_rep_e[tt.x] = e; t_rep_e[tt.y] = e;
+            const int qv[4] = {vv.x, vv.y, oo.x, oo.y};
+            for (int k = 0; k < 4; k++)
+                if (atomicExch(&v_touch[qv[k]], rep_stamp) != rep_stamp) touched[atomicAdd(&cnt[slot_touched], 1)] = qv[k];
+            atomicAdd(&cnt[kCnErr], 1);
+            loc[0]++;
+        }
+    }
+    flush_stats(loc, stats, kStUntangleFlips);
+}
+
+
+// check the triangles around each listed vertex; append non-CCW ones
+template <bool kF32>
+__global__ void __launch_bounds__(kThreads) k_check_ring(DMesh m, const int* __restrict__ vlist, const int* __restrict__ cnt_in,
+                                                         int slot_in, int* __restrict__ t_stamp, int stamp, int* __restrict__ bad,
+                                                         int* __restrict__ cnt_out, int slot_out,
+                                                         unsigned long long* __restrict__ stats) {
+    const int n = cnt_in[slot_in];
+    unsigned loc[4] = {0, 0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int v = vlist[i];
+        int h0 = m.v_he[v], h = h0, guard = 0;
+        do {
+            int t = m.he_t[h];
+            if (t >= 0 && atomicExch(&t_stamp[t], stamp) != stamp) {
+                int a, b, c, st;
+                tri_verts(m, t, a, b, c);
+                int s = orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st);
+                loc[st]++;
+                if (s <= 0) bad[atomicAdd(&cnt_out[slot_out], 1)] = t;
+            }
+            h = m.he_next[h ^ 1];
+        } while (h != h0 && ++guard < 4096);
+    }
+    flush_stats(loc, stats, kStOrient0);
+}
+
+// next attempt for vertices short of their target.  fexp == 0: the whole
+// remaining displacement.  fexp >= 1 (the previous attempt was taken back):
+// 0.9 * 2^-(fexp-1) of the fraction at which the first triangle around the
+// vertex would flip orientation with its neighbours where they are now
+// (orientation is affine in the fraction, so the crossing is O0 / (O0 - O1)).
+// The attempt is checked exactly afterwards, so the estimate needs no care
+// beyond staying in (0, 1].
+__global__ void __launch_bounds__(kThreads) k_advance(DMesh m, const int* __restrict__ vlist, const int* __restrict__ cnt,
+                                                      int slot, float2* __restrict__ acc, float2* __restrict__ tgt,
+                                                      int* __restrict__ v_mv, int stamp, const int* __restrict__ v_fexp,
+                                                      int fexp_cap, unsigned long long* __restrict__ stats) {
+    const int n = cnt[slot];
+    unsigned loc[1] = {0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int v = vlist[i];
+        const float2 q = m.pos[v], t = tgt[v];
+        acc[v] = q;
+        const int fe = v_fexp[v];
+        if (fe >= fexp_cap) {  // give up: the vertex stays where it is this step
+            tgt[v] = q;
+            loc[0]++;
+            continue;
+        }
+        float dx = t.x - q.x, dy = t.y - q.y;
+        if (m.periodic) { dx = min_image_f(dx); dy = min_image_f(dy); }
+        float2 x;
+        if (fe == 0) {
+            x = t;
+        } else {
+            double fstar = 1.0;
+            const int h0 = m.v_he[v];
+            int h = h0, guard = 0;
+            do {
+                if (m.he_t[h] >= 0) {
+                    // triangle (v, a, b) with a = dest(h), b = op(h)
+                    const float2 pa = m.pos[m.he_v[h ^ 1]], pb = m.pos[m.he_op[h]];
+                    double ax = (double)pa.x - q.x, ay = (double)pa.y - q.y;
+                    double bx = (double)pb.x - q.x, by = (double)pb.y - q.y;
+                    if (m.periodic) {
+                        ax = ax > 0.5 ? ax - 1.0 : (ax < -0.5 ? ax + 1.0 : ax);
+                        ay = ay > 0.5 ? ay - 1.0 : (ay < -0.5 ? ay + 1.0 : ay);
+                        bx = bx > 0.5 ? bx - 1.0 : (bx < -0.5 ? bx + 1.0 : bx);
+                        by = by > 0.5 ? by - 1.0 : (by < -0.5 ? by + 1.0 : by);
+                    }
+                    const double o0 = ax * by - ay * bx;
+                    const double o1 = (ax - dx) * (by - dy) - (ay - dy) * (bx - dx);
+                    if (o1 <= 0.0 && o0 > 0.0) fstar = fmin(fstar, o0 / (o0 - o1));
+                }
+                h = m.he_next[h ^ 1];
+            } while (h != h0 && ++guard < 4096);
+            const float f = (float)(0.9 * fstar) * ldexpf(1.0f, -(fe - 1));
+            if (m.periodic) x = make_float2(wrap01(q.x + f * dx), wrap01(q.y + f * dy));
+            else x = make_float2(q.x + f * dx, q.y + f * dy);
+        }
+        m.pos[v] = x;
+        v_mv[v] = stamp;
+    }
+    flush_stats(loc, stats, kStStalled);
+}
+
+// edges whose Delaunay test can change after vertex v moved: the edges
+// incident to v and the edges opposite v in its triangles
+__global__ void __launch_bounds__(kThreads) k_enqueue_ring(DMesh m, const int* __restrict__ vlist, const int* __restrict__ cnt,
+                                                           int slot, const float2* __restrict__ acc, int* __restrict__ e_stamp,
+                                                           int estamp, int* __restrict__ cand, int* __restrict__ cnt_c,
+                                                           int slot_c, int require_moved) {
+    const int n = cnt[slot];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int v = vlist[i];
+        float2 q = m.pos[v], a = acc[v];
+        if (require_moved && q.x == a.x && q.y == a.y) continue;
+        int h0 = m.v_he[v], h = h0, guard = 0;
+        do {
+            int e1 = h >> 1, e2 = m.he_next[h] >> 1;
+            if (atomicExch(&e_stamp[e1], estamp) != estamp) cand[atomicAdd(&cnt_c[slot_c], 1)] = e1;
+            if (m.he_t[h] >= 0 && atomicExch(&e_stamp[e2], estamp) != estamp) cand[atomicAdd(&cnt_c[slot_c], 1)] = e2;
+            h = m.he_next[h ^ 1];
+        } while (h != h0 && ++guard < 4096);
+    }
+}
+
+// vertices of this iteration that did not reach their target stay pending;
+// a successful partial move doubles the next attempt's fraction
+__global__ void __launch_bounds__(kThreads) k_collect_pending(DMesh m, const int* __restrict__ vlist, const int* __restrict__ cnt,
+                                                              int slot, const float2* __restrict__ acc,
+                                                              const float2* __restrict__ tgt, int* __restrict__ v_fexp,
+                                                              int* __restrict__ pend, int* __restrict__ cnt_p, int slot_p) {
+    const int n = cnt[slot];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int v = vlist[i];
+        float2 q = m.pos[v], t = tgt[v], a = acc[v];
+        if (q.x == t.x && q.y == t.y) continue;
+        bool moved = !(q.x == a.x && q.y == a.y);
+        if (moved) v_fexp[v] = 0;  // after a successful partial move, try the whole rest
+        pend[atomicAdd(&cnt_p[slot_p], 1)] = v;
+    }
+}
+
+// ------------------------------------------------------------ relocation
+// Moving a vertex to a point its current star cannot reach without
+// inverting a triangle: remove it (flips inside its star down to degree 3,
+// then the three triangles merge), and insert it again at the target (1->3
+// split of the containing triangle, 2->4 if the target lies on an edge).
+// This is the remove-and-reinsert move that the reference declares as
+// moveFlipflop (Host_Triangulation.h:77, 408) and leaves unimplemented.
+// Every flip and split keeps all triangles strictly CCW (checked exactly),
+// so the mesh stays valid; Lawson rounds restore the Delaunay property.
+constexpr int kMaxStar = 48;
+
+// visibility walk from triangle t toward p; returns the containing triangle
+// (and in *on the half-edge of it that p lies on, or -1), -1 if the walk
+// leaves the mesh or runs too long, -2 if p coincides with a vertex
+template <bool kF32>
+__device__ int locate_point(const DMesh& m, float2 p, int t, int* on) {
+    for (int step = 0; step < 4096; step++) {
+        const int h0 = m.t_he[t];
+        const int hs[3] = {h0, m.he_next[h0], m.he_next[m.he_next[h0]]};
+        bool moved = false;
+        int zero = -1;
+        for (int k = 0; k < 3; k++) {
+            const int h = hs[(k + step) % 3];
+            const float2 a = m.pos[m.he_v[h]], b = m.pos[m.he_v[h ^ 1]];
+            if ((a.x == p.x && a.y == p.y) || (b.x == p.x && b.y == p.y)) return -2;
+            int st;
+            const int o = orient<kF32>(a, b, p, m.periodic, &st);
+            if (o < 0) {
+                const int nt = m.he_t[h ^ 1];
+                if (nt < 0) return -1;
+                t = nt;
+                moved = true;
+                break;
+            }
+            if (o == 0) zero = h;
+        }
+        if (!moved) {
+            *on = zero;
+            return t;
+        }
+    }
+    return -1;
+}
+
+// phase 1 (read-only): find where each listed vertex's target lies
+template <bool kF32>
+__global__ void __launch_bounds__(kThreads) k_reloc_locate(DMesh m, const int* __restrict__ vlist,
+                                                           const int* __restrict__ cnt, int slot,
+                                                           const float2* __restrict__ tgt, int* __restrict__ r_tt,
+                                                           int* __restrict__ r_tt2) {
+    const int n = cnt[slot];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int v = vlist[i];
+        const float2 t = tgt[v], q = m.pos[v];
+        if (t.x == q.x && t.y == q.y) { r_tt[v] = -3; continue; }
+        int on = -1;
+        const int tt = locate_point<kF32>(m, t, m.he_t[m.v_he[v]], &on);
+        r_tt[v] = tt;
+        r_tt2[v] = (tt >= 0 && on >= 0) ? m.he_t[on ^ 1] : -1;
+    }
+}
+
+// phase 2: claim the star triangles and the target triangle(s)
+__global__ void __launch_bounds__(kThreads) k_reloc_claim(DMesh m, const int* __restrict__ vlist,
+                                                          const int* __restrict__ cnt, int slot, int round,
+                                                          const int* __restrict__ r_tt, const int* __restrict__ r_tt2,
+                                                          unsigned long long* __restrict__ claim) {
+    const int n = cnt[slot];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int v = vlist[i];
+        const int tt = r_tt[v];
+        if (tt < 0) continue;
+        const unsigned long long key = claim_key(round, v);
+        const int h0 = m.v_he[v];
+        int h = h0, guard = 0;
+        do {
+            const int t = m.he_t[h];
+            if (t >= 0) atomicMax(&claim[t], key);
+            h = m.he_next[h ^ 1];
+        } while (h != h0 && ++guard < kMaxStar);
+        atomicMax(&claim[tt], key);
+        if (r_tt2[v] >= 0) atomicMax(&claim[r_tt2[v]], key);
+    }
+}
+
+__device__ __forceinline__ void set_he(const DMesh& m, int h, int nx, int v, int t, int o) {
+    m.he_next[h] = nx;
+    m.he_v[h] = v;
+    m.he_t[h] = t;
+    m.he_op[h] = o;
+}
+
+// containment of p in triangle X: 1 inside (on_edge = half-edge or -1), 0 outside
+template <bool kF32>
+__device__ __forceinline__ int tri_contains(const DMesh& m, int X, float2 p, int* on_edge) {
+    const int h0 = m.t_he[X];
+    const int hs[3] = {h0, m.he_next[h0], m.he_next[m.he_next[h0]]};
+    int zero = -1, st;
+    for (int k = 0; k < 3; k++) {
+        const int o = orient<kF32>(m.pos[m.he_v[hs[k]]], m.pos[m.he_v[hs[k] ^ 1]], p, m.periodic, &st);
+        if (o < 0) return 0;
+        if (o == 0) zero = hs[k];
+    }
+    *on_edge = zero;
+    return 1;
+}
+
+// phase 3: winners (all claims held) remove and re-insert their vertex
+template <bool kF32>
+__global__ void __launch_bounds__(kThreads) k_reloc_apply(DMesh m, const int* __restrict__ vlist,
+                                                          const int* __restrict__ cnt, int slot, int round,
+                                                          const int* __restrict__ r_tt, const int* __restrict__ r_tt2,
+                                                          const unsigned long long* __restrict__ claim,
+                                                          const float2* __restrict__ tgt, int* __restrict__ next_list,
+                                                          int* __restrict__ cnt_next, int slot_next,
+                                                          unsigned long long* __restrict__ stats) {
+    const int n = cnt[slot];
+    unsigned loc[2] = {0, 0};  // relocated, given up
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int v = vlist[i];
+        const int tt = r_tt[v], tt2 = r_tt2[v];
+        if (tt == -3) continue;              // already at its target
+        if (tt < 0) { loc[1]++; continue; }  // target on a vertex or walk lost: stays put
+        const unsigned long long key = claim_key(round, v);
+        int star[kMaxStar];
+        int deg = 0;
+        bool mine = claim[tt] == key && (tt2 < 0 || claim[tt2] == key);
+        {
+            const int h0 = m.v_he[v];
+            int h = h0;
+            do {
+                if (deg >= kMaxStar) { mine = false; break; }
+                const int t = m.he_t[h];
+                if (t < 0 || claim[t] != key) mine = false;
+                star[deg++] = t;
+                h = m.he_next[h ^ 1];
+            } while (h != h0);
+        }
+        if (!mine) {  // lost a claim (or star too large): retry next round
+            if (deg < kMaxStar) next_list[atomicAdd(&cnt_next[slot_next], 1)] = v;
+            else loc[1]++;
+            continue;
+        }
+        const int deg0 = deg;
+        const float2 pv = m.pos[v], pt = tgt[v];
+        int st;
+        // --- remove: flip an edge (v, a) whose two new triangles are strictly CCW
+        while (deg > 3) {
+            bool flipped = false;
+            const int h0 = m.v_he[v];
+            int h = h0, guard = 0;
+            do {
+                const int a = m.he_v[h ^ 1], b = m.he_op[h], c = m.he_op[h ^ 1];
+                const float2 pa = m.pos[a], pb = m.pos[b], pc = m.pos[c];
+                if (orient<kF32>(pb, pc, pa, m.periodic, &st) > 0 && orient<kF32>(pc, pb, pv, m.periodic, &st) > 0) {
+                    int h2, h3, h4, h5;
+                    flip_soa(m, h >> 1, h2, h3, h4, h5);  // keeps v.he an outgoing half-edge of v
+                    flipped = true;
+                    break;
+                }
+                h = m.he_next[h ^ 1];
+            } while (h != h0 && ++guard < kMaxStar);
+            if (!flipped) break;
+            deg--;
+        }
+        if (deg > 3) {  // degenerate link (collinear vertices): give up, mesh still valid
+            loc[1]++;
+            continue;
+        }
+        // --- merge the three triangles around v into one
+        int hv[3], av[3], tv[3], gv[3], ov[3];
+        {
+            int h = m.v_he[v];
+            for (int k = 0; k < 3; k++) {
+                hv[k] = h;
+                av[k] = m.he_v[h ^ 1];
+                tv[k] = m.he_t[h];
+                gv[k] = m.he_next[h];  // link half-edge a_k -> op(h_k)
+                ov[k] = m.he_op[h];
+                h = m.he_next[h ^ 1];
+            }
+        }
+        const int T0 = tv[0], F1 = tv[1], F2 = tv[2];
+        const int E0 = hv[0] >> 1, E1 = hv[1] >> 1, E2 = hv[2] >> 1;
+        const int vsum = av[0] + av[1] + av[2];
+        for (int k = 0; k < 3; k++) {
+            int j = 0;
+            for (int q = 0; q < 3; q++)
+                if (av[q] == ov[k]) j = q;
+            m.he_next[gv[k]] = gv[j];
+            m.he_t[gv[k]] = T0;
+            m.he_op[gv[k]] = vsum - av[k] - ov[k];
+            m.v_he[av[k]] = gv[k];
+        }
+        m.t_he[T0] = gv[0];
+        // --- the triangle that receives v: search the claimed live triangles
+        int cand[kMaxStar + 2], nc = 0;
+        for (int k = 0; k < deg0; k++)
+            if (star[k] != F1 && star[k] != F2) cand[nc++] = star[k];
+        bool tt_known = false, tt2_known = tt2 < 0;
+        for (int k = 0; k < nc; k++) { tt_known |= cand[k] == tt; tt2_known |= cand[k] == tt2; }
+        if (!tt_known && tt != F1 && tt != F2) cand[nc++] = tt;
+        if (!tt2_known && tt2 != F1 && tt2 != F2) cand[nc++] = tt2;
+        float2 pins = pt;
+        int X = -1, on = -1;
+        for (int pass = 0; pass < 2 && X < 0; pass++) {
+            for (int k = 0; k < nc && X < 0; k++) {
+                int oe = -1;
+                if (tri_contains<kF32>(m, cand[k], pins, &oe)) {
+                    if (oe >= 0) {  // on an edge: the other side must be ours and live
+                        const int tw = m.he_t[oe ^ 1];
+                        bool ours = false;
+                        for (int q = 0; q < nc; q++) ours |= cand[q] == tw;
+                        if (!ours) continue;
+                    }
+                    X = cand[k];
+                    on = oe;
+                }
+            }
+            if (X < 0) pins = pv;  // cannot place the target: put v back where it was
+        }
+        // the old position lies strictly inside the merged star polygon, so X >= 0 here
+        const int n0 = 2 * E0, n1 = 2 * E1, n2 = 2 * E2;
+        if (on < 0) {
+            const int x0 = m.t_he[X], x1 = m.he_next[x0], x2 = m.he_next[x1];
+            const int a = m.he_v[x0], b = m.he_v[x1], c = m.he_v[x2];
+            set_he(m, x0, n0, a, X, v); set_he(m, n0, n2 ^ 1, b, X, a); set_he(m, n2 ^ 1, x0, v, X, b);
+            set_he(m, x1, n1, b, F1, v); set_he(m, n1, n0 ^ 1, c, F1, b); set_he(m, n0 ^ 1, x1, v, F1, c);
+            set_he(m, x2, n2, c, F2, v); set_he(m, n2, n1 ^ 1, a, F2, c); set_he(m, n1 ^ 1, x2, v, F2, a);
+            m.t_he[X] = x0; m.t_he[F1] = x1; m.t_he[F2] = x2;
+            m.v_he[v] = n2 ^ 1;
+        } else {
+            const int he0 = on, he1 = on ^ 1;
+            const int he2 = m.he_next[he0], he3 = m.he_next[he2], he4 = m.he_next[he1], he5 = m.he_next[he4];
+            const int t0 = X, t1 = m.he_t[he1];
+            const int a = m.he_v[he0], b = m.he_v[he1], c = m.he_v[he3], d = m.he_v[he5];
+            set_he(m, he0, n1, a, t0, c); set_he(m, n1, he3, v, t0, a); set_he(m, he3, he0, c, t0, v);
+            set_he(m, n0, he2, v, F1, c); set_he(m, he2, n1 ^ 1, b, F1, v); set_he(m, n1 ^ 1, n0, c, F1, b);
+            set_he(m, he1, he4, v, t1, d); set_he(m, he4, n2 ^ 1, a, t1, v); set_he(m, n2 ^ 1, he1, d, t1, a);
+            set_he(m, n0 ^ 1, n2, b, F2, d); set_he(m, n2, he5, v, F2, b); set_he(m, he5, n0 ^ 1, d, F2, v);
+            m.t_he[t0] = he0; m.t_he[F1] = n0; m.t_he[t1] = he1; m.t_he[F2] = n0 ^ 1;
+            m.v_he[v] = n0; m.v_he[a] = he0; m.v_he[b] = he2;
+        }
+        m.pos[v] = pins;
+        if (pins.x == pt.x && pins.y == pt.y) loc[0]++;
+        else loc[1]++;
+    }
+    flush_stats(loc, stats, kStRelocated);
+}
+
+// ------------------------------------------------------------ flip rounds
+// kind 0 = Lawson (exact incircle), kind 1 = B/C untangle (reference steps)
+template <int kKind, bool kAll, bool kF32, bool kBig>
+__global__ void __launch_bounds__(kThreads) k_round_test(DMesh m, const int* __restrict__ cand, int* __restrict__ cnt,
+                                                         int round, int n_all, int* __restrict__ wants,
+                                                         unsigned long long* __restrict__ claim, int* __restrict__ e_stamp,
+                                                         int* __restrict__ cand_next, unsigned long long* __restrict__ stats,
+                                                         const int* __restrict__ t_rep = nullptr, int rep_stamp = 0) {
+    const int r3 = round % 3;
+    const int n = kAll ? n_all : cnt[kCnCand + r3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[kCnCand + (r3 + 2) % 3] = 0;
+    unsigned loc[4] = {0, 0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int e = kAll ? i : cand[i];
+        const int2 tt = reinterpret_cast<const int2*>(m.he_t)[e];
+        if (tt.x < 0 || tt.y < 0) continue;
+        const int2 vv = reinterpret_cast<const int2*>(m.he_v)[e];
+        const int2 oo = reinterpret_cast<const int2*>(m.he_op)[e];
+        bool want = false;
+        if (kKind == 0) {
+            int st;
+            // inCircle(org(2e), op(2e+1), org(2e+1), op(2e)) > 0 (kernels.cuh:287-292)
+            int s = incircle<kF32, kBig>(m.pos[vv.x], m.pos[oo.y], m.pos[vv.y], m.pos[oo.x], m.periodic, &st);
+            if (s == kNeedBig) {
+                if (atomicExch(&e_stamp[e], round + 1) != round + 1)
+                    cand_next[atomicAdd(&cnt[kCnCand + (r3 + 1) % 3], 1)] = e;
+                continue;
+            }
+            loc[st]++;
+            want = s > 0;
+        } else if (kKind == 2) {
+            if (t_rep[tt.x] == rep_stamp || t_rep[tt.y] == rep_stamp) continue;
+            // repair: one triangle not strictly CCW, the other strictly CCW, and
+            // both triangles after the flip strictly CCW (B-step of
+            // kernels.cuh:329-331 including its degenerate on-edge case)
+            int st;
+            float2 p0 = m.pos[vv.x], p1 = m.pos[vv.y], a0 = m.pos[oo.x], a1 = m.pos[oo.y];
+            bool up0 = orient<kF32>(p0, p1, a0, m.periodic, &st) > 0; loc[st]++;
+            bool up1 = orient<kF32>(p1, p0, a1, m.periodic, &st) > 0; loc[st]++;
+            if (up0 != up1) {
+                bool n0 = orient<kF32>(a0, a1, p1, m.periodic, &st) > 0; loc[st]++;
+                bool n1 = n0 && orient<kF32>(a1, a0, p0, m.periodic, &st) > 0;
+                want = n0 && n1;
+            }
+        } else {
+            // isCreased / B-step / C-step (kernels.cuh:308-338) with exact orientation;
+            // t0 = (org, dst, op0), t1 = (dst, org, op1)
+            int st;
+            float2 p0 = m.pos[vv.x], p1 = m.pos[vv.y], a0 = m.pos[oo.x], a1 = m.pos[oo.y];
+            int s0 = orient<kF32>(p0, p1, a0, m.periodic, &st); loc[st]++;
+            int s1 = orient<kF32>(p1, p0, a1, m.periodic, &st); loc[st]++;
+            bool ccw0 = s0 >= 0, ccw1 = s1 >= 0;
+            if (ccw0 != ccw1) {
+                // upright triangle U, inverted triangle I
+                float2 u0 = ccw0 ? p0 : p1, u1 = ccw0 ? p1 : p0, ua = ccw0 ? a0 : a1;
+                float2 op_inv = ccw0 ? a1 : a0;
+                float2 i0 = ccw0 ? p1 : p0, i1 = ccw0 ? p0 : p1, ia = ccw0 ? a1 : a0;
+                float2 op_up = ua;
+                // B: op of the inverted triangle strictly inside the upright one
+                bool inB = orient<kF32>(u0, u1, op_inv, m.periodic, &st) > 0 &&
+                           orient<kF32>(u1, ua, op_inv, m.periodic, &st) > 0 &&
+                           orient<kF32>(ua, u0, op_inv, m.periodic, &st) > 0;
+                bool inC = false;
+                if (!inB)  // C: op of the upright triangle strictly inside the inverted one
+                    inC = orient<kF32>(i0, i1, op_up, m.periodic, &st) < 0 &&
+                          orient<kF32>(i1, ia, op_up, m.periodic, &st) < 0 &&
+                          orient<kF32>(ia, i0, op_up, m.periodic, &st) < 0;
+                want = inB || inC;
+            }
+        }
+        if (want) {
+            unsigned long long key = claim_key(round, e);
+            atomicMax(&claim[tt.x], key);
+            atomicMax(&claim[tt.y], key);
+            wants[atomicAdd(&cnt[kCnWants + r3], 1)] = e;
+        }
+    }
+    flush_stats(loc, stats, kKind == 0 ? kStIncircle0 : kStOrient0);
+}
+
+__device__ __forceinline__ void enqueue_edge(int e, int stamp, int* e_stamp, int* cand_next, int* cnt, int slot) {
+    if (atomicExch(&e_stamp[e], stamp) != stamp) cand_next[atomicAdd(&cnt[slot], 1)] = e;
+}
+
+__global__ void __launch_bounds__(kThreads) k_round_flip(DMesh m, int* __restrict__ cnt, int round,
+                                                         const int* __restrict__ wants,
+                                                         const unsigned long long* __restrict__ claim,
+                                                         int* __restrict__ e_stamp, int* __restrict__ cand_next,
+                                                         unsigned long long* __restrict__ stats, int stat_slot) {
+    const int r3 = round % 3;
+    const int n = cnt[kCnWants + r3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[kCnWants + (r3 + 1) % 3] = 0;
+    const int slot = kCnCand + (r3 + 1) % 3, stamp = round + 1;
+    unsigned loc[1] = {0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int e = wants[i];
+        const int2 tt = reinterpret_cast<const int2*>(m.he_t)[e];
+        const unsigned long long key = claim_key(round, e);
+        if (claim[tt.x] == key && claim[tt.y] == key) {
+            int h2, h3, h4, h5;
+            flip_soa(m, e, h2, h3, h4, h5);
+            loc[0]++;
+            enqueue_edge(h2 >> 1, stamp, e_stamp, cand_next, cnt, slot);
+            enqueue_edge(h3 >> 1, stamp, e_stamp, cand_next, cnt, slot);
+            enqueue_edge(h4 >> 1, stamp, e_stamp, cand_next, cnt, slot);
+            enqueue_edge(h5 >> 1, stamp, e_stamp, cand_next, cnt, slot);
+        } else {
+            enqueue_edge(e, stamp, e_stamp, cand_next, cnt, slot);
+        }
+    }
+    flush_stats(loc, stats, stat_slot);
+}
+
+// all flip rounds of a small remaining queue inside one workgroup; uses the
+// big exact stage (the workgroup's few waves bound its scratch memory)
+template <int kKind>
+__global__ void __launch_bounds__(1024) k_round_tail(DMesh m, int* __restrict__ cnt, int round0, int max_rounds,
+                                                      int* __restrict__ cand0, int* __restrict__ cand1,
+                                                      int* __restrict__ wants, unsigned long long* __restrict__ claim,
+                                                      int* __restrict__ e_stamp, unsigned long long* __restrict__ stats,
+                                                      int* __restrict__ rounds_done) {
+    __shared__ int s_n, s_w, s_next;
+    int round = round0;
+    unsigned flips = 0, loc[4] = {0, 0, 0, 0};
+    for (int it = 0; it < max_rounds; it++, round++) {
+        int* cand = (round & 1) ? cand1 : cand0;
+        int* cand_next = (round & 1) ? cand0 : cand1;
+        if (threadIdx.x == 0) {
+            s_n = cnt[kCnCand + round % 3];
+            s_w = 0;
+            s_next = 0;
+        }
+        __syncthreads();
+        const int n = s_n;
+        if (n == 0) break;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const int e = cand[i];
+            const int2 tt = reinterpret_cast<const int2*>(m.he_t)[e];
+            if (tt.x < 0 || tt.y < 0) continue;
+            const int2 vv = reinterpret_cast<const int2*>(m.he_v)[e];
+            const int2 oo = reinterpret_cast<const int2*>(m.he_op)[e];
+            bool want;
+            int st;
+            if (kKind == 0) {
+                int s = incircle<true, true>(m.pos[vv.x], m.pos[oo.y], m.pos[vv.y], m.pos[oo.x], m.periodic, &st);
+                loc[st]++;
+                want = s > 0;
+            } else {
+                float2 p0 = m.pos[vv.x], p1 = m.pos[vv.y], a0 = m.pos[oo.x], a1 = m.pos[oo.y];
+                bool ccw0 = orient<true>(p0, p1, a0, m.periodic, &st) >= 0;
+                bool ccw1 = orient<true>(p1, p0, a1, m.periodic, &st) >= 0;
+                want = false;
+                if (ccw0 != ccw1) {
+                    float2 u0 = ccw0 ? p0 : p1, u1 = ccw0 ? p1 : p0, ua = ccw0 ? a0 : a1, op_inv = ccw0 ? a1 : a0;
+                    float2 i0 = ccw0 ? p1 : p0, i1 = ccw0 ? p0 : p1, ia = ccw0 ? a1 : a0;
+                    bool inB = orient<true>(u0, u1, op_inv, m.periodic, &st) > 0 &&
+                               orient<true>(u1, ua, op_inv, m.periodic, &st) > 0 &&
+                               orient<true>(ua, u0, op_inv, m.periodic, &st) > 0;
+                    bool inC = !inB && orient<true>(i0, i1, ua, m.periodic, &st) < 0 &&
+                               orient<true>(i1, ia, ua, m.periodic, &st) < 0 &&
+                               orient<true>(ia, i0, ua, m.periodic, &st) < 0;
+                    want = inB || inC;
+                }
+            }
+            if (want) {
+                unsigned long long key = claim_key(round, e);
+                atomicMax(&claim[tt.x], key);
+                atomicMax(&claim[tt.y], key);
+                wants[atomicAdd(&s_w, 1)] = e;
+            }
+        }
+        __syncthreads();
+        const int nw = s_w;
+        for (int i = threadIdx.x; i < nw; i += blockDim.x) {
+            const int e = wants[i];
+            const int2 tt = reinterpret_cast<const int2*>(m.he_t)[e];
+            const unsigned long long key = claim_key(round, e);
+            const int stamp = round + 1;
+            if (claim[tt.x] == key && claim[tt.y] == key) {
+                int h2, h3, h4, h5;
+                flip_soa(m, e, h2, h3, h4, h5);
+                flips++;
+                int es[4] = {h2 >> 1, h3 >> 1, h4 >> 1, h5 >> 1};
+                for (int k = 0; k < 4; k++)
+                    if (atomicExch(&e_stamp[es[k]], stamp) != stamp) cand_next[atomicAdd(&s_next, 1)] = es[k];
+            } else if (atomicExch(&e_stamp[e], stamp) != stamp) {
+                cand_next[atomicAdd(&s_next, 1)] = e;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) cnt[kCnCand + (round + 1) % 3] = s_next;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *rounds_done = round - round0;
+    unsigned f[1] = {flips};
+    flush_stats(f, stats, kKind == 0 ? kStFlips : kStUntangleFlips);
+    flush_stats(loc, stats, kKind == 0 ? kStIncircle0 : kStOrient0);
+}
+
+// exact non-Delaunay edge count (diagnostic)
+__global__ void __launch_bounds__(kThreads) k_count_nd(DMesh m, unsigned long long* out) {
+    unsigned loc = 0;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < m.ne; e += gridDim.x * blockDim.x) {
+        const int2 tt = reinterpret_cast<const int2*>(m.he_t)[e];
+        if (tt.x < 0 || tt.y < 0) continue;
+        const int2 vv = reinterpret_cast<const int2*>(m.he_v)[e];
+        const int2 oo = reinterpret_cast<const int2*>(m.he_op)[e];
+        int st;
+        if (incircle<true, true>(m.pos[vv.x], m.pos[oo.y], m.pos[vv.y], m.pos[oo.x], m.periodic, &st) > 0) loc++;
+    }
+    if (loc) atomicAdd(out, (unsigned long long)loc);
+}
+
+__global__ void k_set_ints(int* p, int n, int v) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_set_u64(unsigned long long* p, int n, unsigned long long v) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ---------------------------------------------------------------- one ring
+// out-degree by CW rotation he <- next(he^1) from v.he (kernels.cuh:67-76)
+__global__ void __launch_bounds__(kThreads) k_degree(DMesh m, int* __restrict__ deg) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
+        int h0 = m.v_he[v], h = h0, c = 0;
+        do {
+            c++;
+            h = m.he_next[h ^ 1];
+        } while (h != h0 && c < 65536);
+        deg[v] = c;
+    }
+}
+__global__ void __launch_bounds__(kThreads) k_ring_csr(DMesh m, const int* __restrict__ off, int* __restrict__ nbr, int cap) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
+        int h0 = m.v_he[v], h = h0, k = off[v], end = off[v + 1];
+        do {
+            if (k < cap && k < end) nbr[k] = m.he_v[h ^ 1];
+            k++;
+            h = m.he_next[h ^ 1];
+        } while (h != h0 && k < end);
+    }
+}
+// oneRing<K> (layout 0) / oneRing2<K> (layout 1): kernels.cuh:54-121
+__global__ void __launch_bounds__(kThreads) k_ring_legacy(DMesh m, int K, int layout, int* __restrict__ ring) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
+        int h0 = m.v_he[v], h = h0, c = 0;
+        do {
+            if (c + 2 > K) break;
+            int w = m.he_v[h ^ 1];
+            if (layout == 0) ring[(size_t)(c + 1) * m.nv + v] = w;
+            else ring[(size_t)(c + 1) + (size_t)K * v] = w;
+            c++;
+            h = m.he_next[h ^ 1];
+        } while (h != h0);
+        if (layout == 0) ring[v] = c;
+        else ring[(size_t)K * v] = c;
+    }
+}
+// closestNeighbors (kernels.cuh:129-146) over the CSR ring
+__global__ void __launch_bounds__(kThreads) k_closest(DMesh m, const int* __restrict__ off, const int* __restrict__ nbr,
+                                                      int* __restrict__ out) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
+        float2 p = m.pos[v];
+        int b = off[v], e = off[v + 1];
+        int best = b < e ? nbr[b] : -1;
+        float bd = 3.0e38f;
+        for (int k = b; k < e; k++) {
+            float2 q = m.pos[nbr[k]];
+            float dx = p.x - q.x, dy = p.y - q.y;
+            if (m.periodic) { dx = min_image_f(dx); dy = min_image_f(dy); }
+            float d = __fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy));
+            if (k == b || d < bd) { bd = d; best = nbr[k]; }
+        }
+        out[v] = best;
+    }
+}
+
+}  // namespace mcl
+
+// =========================================================================
+// host side
+// =========================================================================
+struct mcl_tri {
+    DMesh m;
+    // flip rounds
+    unsigned long long* claim = nullptr;  // nt
+    int* e_stamp = nullptr;               // ne
+    int* cand[2] = {nullptr, nullptr};    // ne each
+    int* wants = nullptr;                 // ne
+    int* cnt = nullptr;                   // kCnCount
+    int* rounds_done = nullptr;
+    unsigned long long* stats = nullptr;  // kStCount
+    unsigned long long* scalar = nullptr;
+    // kinetic step
+    float2* acc = nullptr;
+    float2* tgt = nullptr;
+    int* v_mv = nullptr;
+    int* v_rev = nullptr;
+    int* v_fexp = nullptr;
+    int* t_stamp = nullptr;
+    int* bad[2] = {nullptr, nullptr};  // nt each
+    int* rev = nullptr;                // nv: reverted in the current pass
+    int* allrev = nullptr;             // nv: reverted during the iteration
+    int* pend[2] = {nullptr, nullptr}; // nv each
+    int* v_touch = nullptr;            // nv: stamp of the last repair that touched the vertex
+    int* t_rep = nullptr;              // nt: stamp of the repair iteration that made the triangle
+    int* t_rep_e = nullptr;            // nt: edge of that repair flip
+    int* e_undo = nullptr;             // ne: undo dedupe stamps
+    int* undo = nullptr;               // ne: repair flips to undo in the current pass
+    int* stuck = nullptr;              // nt
+    int* touched = nullptr;            // nv: vertices of repair-flipped quads
+    int* r_tt = nullptr;               // nv: relocation target triangle
+    int* r_tt2 = nullptr;              // nv: second triangle when the target is on an edge
+    // one ring
+    int* deg = nullptr;
+    int* scan_tmp = nullptr;
+    int* h_cnt = nullptr;  // pinned
+    int round = 1;         // global round counter (claims, edge stamps)
+    int stamp = 1;         // global stamp for vertex / triangle marks
+    int grid_blocks = 1024;
+};
+
+namespace mcl {
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+    fprintf(stderr, "[mcleap_hip] %s failed: %s (%s:%d)\n", what, hipGetErrorString(e), file, line);
+    return MCL_ERR_HIP;
+}
+}  // namespace mcl
+
+static int blocks_for(long long n, int cap = 8192) {
+    long long b = (n + kThreads - 1) / kThreads;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (int)b;
+}
+
+template <class T>
+static int dalloc(T** p, size_t n) {
+    MCL_HIP(hipMalloc((void**)p, sizeof(T) * (n ? n : 1)));
+    return MCL_OK;
+}
+
+#define MCL_TRY(x)                 \
+    do {                           \
+        int _r = (x);              \
+        if (_r != MCL_OK) return _r; \
+    } while (0)
+
+static int tri_free(mcl_tri* t) {
+    void* ptrs[] = {t->m.pos, t->m.he_next, t->m.he_v, t->m.he_t, t->m.he_op, t->m.v_he, t->m.v_pidx, t->m.t_he,
+                    t->claim, t->e_stamp, t->cand[0], t->cand[1], t->wants, t->cnt, t->rounds_done, t->stats,
+                    t->scalar, t->acc, t->tgt, t->v_mv, t->v_rev, t->v_fexp, t->t_stamp, t->bad[0], t->bad[1],
+                    t->rev, t->allrev, t->pend[0], t->pend[1], t->deg, t->scan_tmp, t->v_touch, t->t_rep, t->t_rep_e, t->e_undo, t->undo, t->stuck, t->touched, t->r_tt, t->r_tt2};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (t->h_cnt) (void)hipHostFree(t->h_cnt);
+    return MCL_OK;
+}
+
+static int upload_aos(mcl_tri* t, const float* pos, const int* v2, const int* he4, const int* t1) {
+    DMesh& m = t->m;
+    std::vector<int> a(m.nhe), b(m.nhe), c(m.nhe), d(m.nhe);
+    if (he4) {
+        for (int h = 0; h < m.nhe; h++) {
+            a[h] = he4[4 * h]; b[h] = he4[4 * h + 1]; c[h] = he4[4 * h + 2]; d[h] = he4[4 * h + 3];
+        }
+        MCL_HIP(hipMemcpy(m.he_next, a.data(), sizeof(int) * m.nhe, hipMemcpyHostToDevice));
+        MCL_HIP(hipMemcpy(m.he_v, b.data(), sizeof(int) * m.nhe, hipMemcpyHostToDevice));
+        MCL_HIP(hipMemcpy(m.he_t, c.data(), sizeof(int) * m.nhe, hipMemcpyHostToDevice));
+        MCL_HIP(hipMemcpy(m.he_op, d.data(), sizeof(int) * m.nhe, hipMemcpyHostToDevice));
+    }
+    if (v2) {
+        std::vector<int> pi(m.nv), vh(m.nv);
+        for (int v = 0; v < m.nv; v++) { pi[v] = v2[2 * v]; vh[v] = v2[2 * v + 1]; }
+        MCL_HIP(hipMemcpy(m.v_pidx, pi.data(), sizeof(int) * m.nv, hipMemcpyHostToDevice));
+        MCL_HIP(hipMemcpy(m.v_he, vh.data(), sizeof(int) * m.nv, hipMemcpyHostToDevice));
+    }
+    if (t1) MCL_HIP(hipMemcpy(m.t_he, t1, sizeof(int) * m.nt, hipMemcpyHostToDevice));
+    if (pos) MCL_HIP(hipMemcpy(m.pos, pos, sizeof(float2) * m.nv, hipMemcpyHostToDevice));
+    MCL_HIP(hipDeviceSynchronize());
+    return MCL_OK;
+}
+
+extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv, const int* he4, int nhe,
+                              const int* t1, int nt, int periodic, mcl_tri_t** out) {
+    if (!out || !pos || !v2 || !he4 || !t1 || npos != nv || nhe <= 0 || (nhe & 1) || nt <= 0) return MCL_ERR_INPUT;
+    if (nhe / 2 >= (1 << 28)) return MCL_ERR_CAPACITY;  // claim keys hold 28-bit edge ids
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MCL_ERR_NO_DEVICE;
+    mcl_tri* t = new mcl_tri();
+    DMesh& m = t->m;
+    m.nv = nv; m.nhe = nhe; m.nt = nt; m.ne = nhe / 2; m.periodic = periodic ? 1 : 0;
+    int rc = MCL_OK;
+#define A(p, n) if (rc == MCL_OK) rc = dalloc(&(p), (size_t)(n))
+    A(m.pos, nv); A(m.he_next, nhe); A(m.he_v, nhe); A(m.he_t, nhe); A(m.he_op, nhe);
+    A(m.v_he, nv); A(m.v_pidx, nv); A(m.t_he, nt);
+    A(t->claim, nt); A(t->e_stamp, m.ne); A(t->cand[0], m.ne); A(t->cand[1], m.ne); A(t->wants, m.ne);
+    A(t->cnt, kCnCount); A(t->rounds_done, 1); A(t->stats, kStCount); A(t->scalar, 4);
+    A(t->acc, nv); A(t->tgt, nv); A(t->v_mv, nv); A(t->v_rev, nv); A(t->v_fexp, nv); A(t->t_stamp, nt);
+    A(t->bad[0], nt); A(t->bad[1], nt); A(t->rev, nv); A(t->allrev, nv); A(t->pend[0], nv); A(t->pend[1], nv);
+    A(t->deg, nv + 1); A(t->scan_tmp, scan_scratch_ints(nv + 1) + 16);
+    A(t->v_touch, nv); A(t->t_rep, nt); A(t->t_rep_e, nt); A(t->e_undo, m.ne); A(t->undo, m.ne);
+    A(t->stuck, nt); A(t->touched, nv); A(t->r_tt, nv); A(t->r_tt2, nv);
+#undef A
+    if (rc == MCL_OK && hipHostMalloc((void**)&t->h_cnt, sizeof(int) * kCnCount) != hipSuccess) rc = MCL_ERR_HIP;
+    if (rc == MCL_OK) rc = upload_aos(t, pos, v2, he4, t1);
+    if (rc == MCL_OK) {
+        hipLaunchKernelGGL(k_set_u64, dim3(blocks_for(nt)), dim3(kThreads), 0, 0, t->claim, nt, 0ull);
+        hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(m.ne)), dim3(kThreads), 0, 0, t->e_stamp, m.ne, 0);
+        hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(nv)), dim3(kThreads), 0, 0, t->v_mv, nv, 0);
+        hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(nv)), dim3(kThreads), 0, 0, t->v_rev, nv, 0);
+        hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(nv)), dim3(kThreads), 0, 0, t->v_fexp, nv, 0);
+        hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(nv)), dim3(kThreads), 0, 0, t->v_touch, nv, 0);
+        hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(nt)), dim3(kThreads), 0, 0, t->t_rep, nt, 0);
+        hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(m.ne)), dim3(kThreads), 0, 0, t->e_undo, m.ne, 0);
+        hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(nt)), dim3(kThreads), 0, 0, t->t_stamp, nt, 0);
+        hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, 0, t->cnt, (int)kCnCount, 0);
+        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, 0, t->stats, (int)kStCount, 0ull);
+        if (hipDeviceSynchronize() != hipSuccess) rc = MCL_ERR_HIP;
+    }
+    if (rc != MCL_OK) { tri_free(t); delete t; return rc; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) == hipSuccess) t->grid_blocks = prop.multiProcessorCount * 8;
+    *out = t;
+    return MCL_OK;
+}
+
+extern "C" int mcl_tri_destroy(mcl_tri_t* t) {
+    if (!t) return MCL_OK;
+    (void)hipDeviceSynchronize();
+    tri_free(t);
+    delete t;
+    return MCL_OK;
+}
+
+extern "C" int mcl_tri_sizes(const mcl_tri_t* t, int* npos, int* nv, int* nhe, int* nt) {
+    if (!t) return MCL_ERR_INPUT;
+    if (npos) *npos = t->m.nv;
+    if (nv) *nv = t->m.nv;
+    if (nhe) *nhe = t->m.nhe;
+    if (nt) *nt = t->m.nt;
+    return MCL_OK;
+}
+
+extern "C" void* mcl_tri_pos_ptr(mcl_tri_t* t) { return t ? (void*)t->m.pos : nullptr; }
+
+extern "C" int mcl_tri_upload(mcl_tri_t* t, const float* pos, const int* v2, const int* he4, const int* t1) {
+    if (!t) return MCL_ERR_INPUT;
+    return upload_aos(t, pos, v2, he4, t1);
+}
+
+extern "C" int mcl_tri_download(mcl_tri_t* t, float* pos, int* v2, int* he4, int* t1) {
+    if (!t) return MCL_ERR_INPUT;
+    DMesh& m = t->m;
+    MCL_HIP(hipDeviceSynchronize());
+    if (he4) {
+        std::vector<int> a(m.nhe), b(m.nhe), c(m.nhe), d(m.nhe);
+        MCL_HIP(hipMemcpy(a.data(), m.he_next, sizeof(int) * m.nhe, hipMemcpyDeviceToHost));
+        MCL_HIP(hipMemcpy(b.data(), m.he_v, sizeof(int) * m.nhe, hipMemcpyDeviceToHost));
+        MCL_HIP(hipMemcpy(c.data(), m.he_t, sizeof(int) * m.nhe, hipMemcpyDeviceToHost));
+        MCL_HIP(hipMemcpy(d.data(), m.he_op, sizeof(int) * m.nhe, hipMemcpyDeviceToHost));
+        for (int h = 0; h < m.nhe; h++) {
+            he4[4 * h] = a[h]; he4[4 * h + 1] = b[h]; he4[4 * h + 2] = c[h]; he4[4 * h + 3] = d[h];
+        }
+    }
+    if (v2) {
+        std::vector<int> pi(m.nv), vh(m.nv);
+        MCL_HIP(hipMemcpy(pi.data(), m.v_pidx, sizeof(int) * m.nv, hipMemcpyDeviceToHost));
+        MCL_HIP(hipMemcpy(vh.data(), m.v_he, sizeof(int) * m.nv, hipMemcpyDeviceToHost));
+        for (int v = 0; v < m.nv; v++) { v2[2 * v] = pi[v]; v2[2 * v + 1] = vh[v]; }
+    }
+    if (t1) MCL_HIP(hipMemcpy(t1, m.t_he, sizeof(int) * m.nt, hipMemcpyDeviceToHost));
+    if (pos) MCL_HIP(hipMemcpy(pos, m.pos, sizeof(float2) * m.nv, hipMemcpyDeviceToHost));
+    return MCL_OK;
+}
+
+extern "C" int mcl_tri_move(mcl_tri_t* t, const void* d_disp, void* stream) {
+    if (!t || !d_disp) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_move_disp, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, (const float2*)d_disp);
+    MCL_HIP(hipGetLastError());
+    MCL_HIP(hipStreamSynchronize(s));
+    return MCL_OK;
+}
+
+extern "C" int mcl_tri_integrate(mcl_tri_t* t, const void* d_dt, const void* d_vel, void* stream) {
+    if (!t || !d_dt || !d_vel) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_integrate, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, (const float*)d_dt,
+                       (const float2*)d_vel);
+    MCL_HIP(hipGetLastError());
+    MCL_HIP(hipStreamSynchronize(s));
+    return MCL_OK;
+}
+
+// ------------------------------------------------------------ round driver
+static int read_counts(mcl_tri* t, hipStream_t s) {
+    MCL_HIP(hipMemcpyAsync(t->h_cnt, t->cnt, sizeof(int) * kCnCount, hipMemcpyDeviceToHost, s));
+    MCL_HIP(hipStreamSynchronize(s));
+    return MCL_OK;
+}
+
+// zero the counter slots used by the first round (round % 3 = r3):
+// wants[r3] and cand[r3+1]; cand[r3] is set by the caller
+__global__ void k_prep_round(int* cnt, int r3, int ncand) {
+    if (threadIdx.x == 0) {
+        cnt[kCnWants + r3] = 0;
+        cnt[kCnCand + (r3 + 1) % 3] = 0;
+        if (ncand >= 0) cnt[kCnCand + r3] = ncand;
+    }
+}
+
+// Runs flip rounds (kind 0 = Lawson, 1 = B/C) starting from either all
+// edges (all = true) or the queue already in cand[round % 2] with count in
+// cnt[kCnCand + round % 3].  variant 0: full-grid rounds while the queue is
+// large, then one workgroup finishes the rest (no host round trips until
+// the end).  variant 1: one test + one flip launch per round with a host
+// read of the queue length in between (FP64-first predicates).
+template <int kKind>
+static int run_rounds(mcl_tri* t, bool all, int variant, int max_rounds, hipStream_t s, long long* rounds_out) {
+    DMesh& m = t->m;
+    const int cap = max_rounds > 0 ? max_rounds : 1 << 20;
+    int r = 0;
+    if (all) {
+        hipLaunchKernelGGL(k_prep_round, dim3(1), dim3(64), 0, s, t->cnt, t->round % 3, 0);
+    }
+    bool first = all;
+    long long done = 0;
+    if (variant == 1) {
+        bool need_tail = false;
+        for (; r < cap; r++) {
+            const int R = t->round;
+            int* cand = t->cand[R & 1];
+            int* cand_next = t->cand[(R + 1) & 1];
+            const int stat = kKind == 0 ? kStFlips : kStUntangleFlips;
+            if (first) {
+                hipLaunchKernelGGL((k_round_test<kKind, true, false, false>), dim3(blocks_for(m.ne)), dim3(kThreads), 0, s,
+                                   m, cand, t->cnt, R, m.ne, t->wants, t->claim, t->e_stamp, cand_next, t->stats);
+            } else {
+                int n = t->h_cnt[kCnCand + R % 3];
+                hipLaunchKernelGGL((k_round_test<kKind, false, false, false>), dim3(blocks_for(n)), dim3(kThreads), 0, s,
+                                   m, cand, t->cnt, R, 0, t->wants, t->claim, t->e_stamp, cand_next, t->stats);
+            }
+            MCL_TRY(read_counts(t, s));
+            int nw = t->h_cnt[kCnWants + R % 3];
+            hipLaunchKernelGGL(k_round_flip, dim3(blocks_for(nw)), dim3(kThreads), 0, s, m, t->cnt, R, t->wants, t->claim,
+                               t->e_stamp, cand_next, t->stats, stat);
+            MCL_HIP(hipGetLastError());
+            MCL_TRY(read_counts(t, s));
+            t->round++;
+            done++;
+            first = false;
+            if (t->h_cnt[kCnCand + (R + 1) % 3] == 0) { r++; break; }
+            if (nw == 0) { need_tail = true; r++; break; }  // only big-exact cases left
+        }
+        if (!need_tail) {
+            if (rounds_out) *rounds_out = done;
+            if (r >= cap && t->h_cnt[kCnCand + t->round % 3] != 0) return MCL_ERR_ROUND_CAP;
+            return MCL_OK;
+        }
+        const int R = t->round;
+        hipLaunchKernelGGL(k_round_tail<kKind>, dim3(1), dim3(1024), 0, s, m, t->cnt, R, cap - r + 1, t->cand[0],
+                           t->cand[1], t->wants, t->claim, t->e_stamp, t->stats, t->rounds_done);
+        MCL_HIP(hipGetLastError());
+        int tail_rounds = 0;
+        MCL_HIP(hipMemcpyAsync(&tail_rounds, t->rounds_done, sizeof(int), hipMemcpyDeviceToHost, s));
+        MCL_TRY(read_counts(t, s));
+        t->round = R + tail_rounds + 1;
+        done += tail_rounds;
+        if (rounds_out) *rounds_out = done;
+        if (t->h_cnt[kCnCand + (R + tail_rounds) % 3] != 0) return MCL_ERR_ROUND_CAP;
+        return MCL_OK;
+    }
+    // variant 0: a few full-grid rounds, then the single-workgroup tail
+    const int full_rounds = 3;
+    const int gb = t->grid_blocks;
+    for (int k = 0; k < full_rounds && r < cap; k++, r++) {
+        const int R = t->round;
+        int* cand = t->cand[R & 1];
+        int* cand_next = t->cand[(R + 1) & 1];
+        const int stat = kKind == 0 ? kStFlips : kStUntangleFlips;
+        if (first)
+            hipLaunchKernelGGL((k_round_test<kKind, true, true, false>), dim3(gb), dim3(kThreads), 0, s, m, cand, t->cnt, R,
+                               m.ne, t->wants, t->claim, t->e_stamp, cand_next, t->stats);
+        else
+            hipLaunchKernelGGL((k_round_test<kKind, false, true, false>), dim3(gb), dim3(kThreads), 0, s, m, cand, t->cnt, R, 0,
+                               t->wants, t->claim, t->e_stamp, cand_next, t->stats);
+        hipLaunchKernelGGL(k_round_flip, dim3(gb), dim3(kThreads), 0, s, m, t->cnt, R, t->wants, t->claim, t->e_stamp,
+                           cand_next, t->stats, stat);
+        t->round++;
+        done++;
+        first = false;
+    }
+    const int R = t->round;
+    const int tail_cap = cap - r;
+    if (tail_cap > 0) {
+        hipLaunchKernelGGL(k_round_tail<kKind>, dim3(1), dim3(1024), 0, s, m, t->cnt, R, tail_cap, t->cand[0], t->cand[1],
+                           t->wants, t->claim, t->e_stamp, t->stats, t->rounds_done);
+    }
+    MCL_HIP(hipGetLastError());
+    int tail_rounds = 0;
+    MCL_HIP(hipMemcpyAsync(&tail_rounds, t->rounds_done, sizeof(int), hipMemcpyDeviceToHost, s));
+    MCL_TRY(read_counts(t, s));
+    t->round = R + tail_rounds + 1;  // +1 keeps claim keys of the next call above the tail's
+    done += tail_rounds;
+    if (rounds_out) *rounds_out = done;
+    if (t->h_cnt[kCnCand + (R + tail_rounds) % 3] != 0) return MCL_ERR_ROUND_CAP;
+    return MCL_OK;
+}
+
+static void stats_begin(mcl_tri* t, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, t->stats, (int)kStCount, 0ull);
+}
+static int stats_end(mcl_tri* t, hipStream_t s, mcl_stats_t* st, std::chrono::steady_clock::time_point t0) {
+    if (!st) return MCL_OK;
+    unsigned long long h[kStCount];
+    MCL_HIP(hipMemcpyAsync(h, t->stats, sizeof(h), hipMemcpyDeviceToHost, s));
+    MCL_HIP(hipStreamSynchronize(s));
+    long long rounds = st->rounds;
+    memset(st, 0, sizeof(*st));
+    st->rounds = rounds;
+    st->flips = (long long)h[kStFlips];
+    st->untangle_flips = (long long)h[kStUntangleFlips];
+    st->reverts = (long long)h[kStReverts];
+    st->stalled = (long long)h[kStStalled] + (long long)h[kStRelocFailed];
+    st->relocated = (long long)h[kStRelocated];
+    st->inverted = (long long)h[kStInverted];
+    for (int k = 0; k < 4; k++) {
+        st->pred_orient[k] = (long long)h[kStOrient0 + k];
+        st->pred_incircle[k] = (long long)h[kStIncircle0 + k];
+    }
+    st->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return MCL_OK;
+}
+
+extern "C" int mcl_tri_delonize(mcl_tri_t* t, int variant, int max_rounds, mcl_stats_t* st, void* stream) {
+    if (!t) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    auto t0 = std::chrono::steady_clock::now();
+    stats_begin(t, s);
+    long long rounds = 0;
+    int rc = run_rounds<0>(t, true, variant, max_rounds, s, &rounds);
+    if (st) st->rounds = rounds;
+    int rc2 = stats_end(t, s, st, t0);
+    return rc != MCL_OK ? rc : rc2;
+}
+
+extern "C" int mcl_tri_count_inverted(mcl_tri_t* t, long long* out, void* stream) {
+    if (!t || !out) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_prep_round, dim3(1), dim3(64), 0, s, t->cnt, 0, -1);
+    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->cnt + kCnMisc, 1, 0);
+    hipLaunchKernelGGL((k_check_all<true>), dim3(t->grid_blocks), dim3(kThreads), 0, s, t->m, (int*)nullptr, t->cnt,
+                       (int)kCnMisc, t->stats);
+    MCL_HIP(hipGetLastError());
+    MCL_TRY(read_counts(t, s));
+    *out = t->h_cnt[kCnMisc];
+    return MCL_OK;
+}
+
+extern "C" int mcl_tri_count_non_delaunay(mcl_tri_t* t, long long* out, void* stream) {
+    if (!t || !out) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, t->scalar, 1, 0ull);
+    hipLaunchKernelGGL(k_count_nd, dim3(256), dim3(kThreads), 0, s, t->m, t->scalar);
+    MCL_HIP(hipGetLastError());
+    unsigned long long h = 0;
+    MCL_HIP(hipMemcpyAsync(&h, t->scalar, sizeof(h), hipMemcpyDeviceToHost, s));
+    MCL_HIP(hipStreamSynchronize(s));
+    *out = (long long)h;
+    return MCL_OK;
+}
+
+extern "C" int mcl_tri_untangle(mcl_tri_t* t, int max_rounds, mcl_stats_t* st, void* stream) {
+    if (!t) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    auto t0 = std::chrono::steady_clock::now();
+    stats_begin(t, s);
+    long long rounds = 0;
+    int rc = run_rounds<1>(t, true, 1, max_rounds, s, &rounds);
+    if (st) st->rounds = rounds;
+    MCL_TRY(stats_end(t, s, st, t0));
+    if (rc != MCL_OK && rc != MCL_ERR_ROUND_CAP) return rc;
+    long long inv = 0;
+    MCL_TRY(mcl_tri_count_inverted(t, &inv, stream));
+    if (st) st->inverted = inv;
+    return inv ? MCL_ERR_UNRESOLVED : rc;
+}
+
+// ------------------------------------------------------------- kinetic step
+// Step state machine (one iteration per pass of the outer loop):
+//   move      -- iteration 0 moves every non-corner vertex to its target;
+//                later iterations move only pending vertices, by 2^-fexp of
+//                their remaining displacement
+//   check     -- triangles (all, or around movers) that are not strictly CCW
+//   repair    -- rounds of flips of an edge between a bad and a good triangle
+//                whose two new triangles are both strictly CCW (the B-step
+//                of Host_Triangulation.h:445-450 plus its on-edge case).  A
+//                triangle made by a repair is not repaired again in the same
+//                iteration, so every repair can later be undone on its own.
+//   revert    -- each remaining bad triangle either undoes the repair that
+//                made it or sends its largest-displacement mover back to the
+//                accepted position; passes repeat until no triangle is bad.
+//                This terminates: with all repairs undone and all movers
+//                back, the mesh is the iteration's valid starting state.
+//   Lawson    -- flip rounds to the exact Delaunay triangulation
+//   pending   -- movers short of their target retry next iteration
+namespace {
+struct StepCtx {
+    mcl_tri* t;
+    hipStream_t s;
+    bool f32;
+};
+
+template <bool kF32>
+int check_list(mcl_tri* t, hipStream_t s, const int* vlist, int slot_list, int n, int* bad, int slot_bad) {
+    int ts = ++t->stamp;
+    hipLaunchKernelGGL((k_check_ring<kF32>), dim3(blocks_for(n)), dim3(kThreads), 0, s, t->m, vlist, t->cnt, slot_list,
+                       t->t_stamp, ts, bad, t->cnt, slot_bad, t->stats);
+    MCL_HIP(hipGetLastError());
+    return MCL_OK;
+}
+}  // namespace
+
+static int zero_slots(mcl_tri* t, hipStream_t s, int first, int n) {
+    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->cnt + first, n, 0);
+    MCL_HIP(hipGetLastError());
+    return MCL_OK;
+}
+
+// repair rounds on the bad list in bad[*cur] (count slot kCnBad + *cur);
+// returns with the remaining bad triangles in bad[*cur]
+static int repair_rounds(mcl_tri* t, hipStream_t s, bool f32, int* cur, int rep_stamp, int* nrep) {
+    DMesh& m = t->m;
+    *nrep = 0;
+    for (int r = 0; r < 64; r++) {
+        int nb = t->h_cnt[kCnBad + *cur];
+        if (nb == 0) break;
+        const int R = t->round, nxt = 1 - *cur;
+        hipLaunchKernelGGL(k_prep_round, dim3(1), dim3(64), 0, s, t->cnt, R % 3, 0);
+        MCL_TRY(zero_slots(t, s, kCnBad + nxt, 1));
+        if (f32)
+            hipLaunchKernelGGL((k_filter_bad<true>), dim3(blocks_for(nb)), dim3(kThreads), 0, s, m, t->bad[*cur], t->cnt,
+                               (int)kCnBad + *cur, t->bad[nxt], (int)kCnBad + nxt, t->e_stamp, R, t->cand[R & 1], t->stats);
+        else
+            hipLaunchKernelGGL((k_filter_bad<false>), dim3(blocks_for(nb)), dim3(kThreads), 0, s, m, t->bad[*cur], t->cnt,
+                               (int)kCnBad + *cur, t->bad[nxt], (int)kCnBad + nxt, t->e_stamp, R, t->cand[R & 1], t->stats);
+        MCL_TRY(read_counts(t, s));
+        *cur = nxt;
+        int nc = t->h_cnt[kCnCand + R % 3];
+        if (t->h_cnt[kCnBad + nxt] == 0 || nc == 0) break;
+        if (f32)
+            hipLaunchKernelGGL((k_round_test<2, false, true, false>), dim3(blocks_for(nc)), dim3(kThreads), 0, s, m,
+                               t->cand[R & 1], t->cnt, R, 0, t->wants, t->claim, t->e_stamp, t->cand[(R + 1) & 1], t->stats,
+                               (const int*)t->t_rep, rep_stamp);
+        else
+            hipLaunchKernelGGL((k_round_test<2, false, false, false>), dim3(blocks_for(nc)), dim3(kThreads), 0, s, m,
+                               t->cand[R & 1], t->cnt, R, 0, t->wants, t->claim, t->e_stamp, t->cand[(R + 1) & 1], t->stats,
+                               (const int*)t->t_rep, rep_stamp);
+        MCL_TRY(read_counts(t, s));
+        int nw = t->h_cnt[kCnWants + R % 3];
+        t->round++;
+        if (nw == 0) break;
+        MCL_TRY(zero_slots(t, s, kCnErr, 1));
+        hipLaunchKernelGGL(k_repair_flip, dim3(blocks_for(nw)), dim3(kThreads), 0, s, m, t->cnt, R, t->wants, t->claim,
+                           t->t_rep, t->t_rep_e, rep_stamp, t->v_touch, t->touched, (int)kCnTouched, t->stats);
+        MCL_HIP(hipGetLastError());
+        MCL_TRY(read_counts(t, s));
+        *nrep += t->h_cnt[kCnErr];
+    }
+    return MCL_OK;
+}
+
+// take back moves / undo repairs until no triangle is bad; starts from the
+// bad list in bad[*cur]
+static int revert_passes(mcl_tri* t, hipStream_t s, bool f32, int* cur, int mv_stamp, int rep_stamp, int* undos,
+                         bool* stuck) {
+    DMesh& m = t->m;
+    *stuck = false;
+    for (int pass = 0; pass < (1 << 20); pass++) {
+        int nb = t->h_cnt[kCnBad + *cur];
+        if (nb == 0) break;
+        const int rev_stamp = ++t->stamp, undo_stamp = rev_stamp;
+        MCL_TRY(zero_slots(t, s, kCnRev, 2));   // rev list, undo list
+        MCL_TRY(zero_slots(t, s, kCnMisc, 1));  // stuck
+        hipLaunchKernelGGL(k_revert_mark, dim3(blocks_for(nb)), dim3(kThreads), 0, s, m, t->bad[*cur], t->cnt,
+                           (int)kCnBad + *cur, t->acc, t->v_mv, mv_stamp, t->t_rep, t->t_rep_e, rep_stamp, t->e_undo,
+                           undo_stamp, t->undo, (int)kCnRev + 1, t->v_rev, rev_stamp, t->rev, (int)kCnRev, t->stuck,
+                           (int)kCnMisc);
+        MCL_TRY(read_counts(t, s));
+        if (t->h_cnt[kCnMisc] > 0) { *stuck = true; return MCL_OK; }
+        const int nr = t->h_cnt[kCnRev], nu = t->h_cnt[kCnRev + 1];
+        const int nxt = 1 - *cur;
+        MCL_TRY(zero_slots(t, s, kCnBad + nxt, 1));
+        if (nr > 0)
+            hipLaunchKernelGGL(k_revert_apply, dim3(blocks_for(nr)), dim3(kThreads), 0, s, m, t->rev, t->cnt, (int)kCnRev,
+                               t->acc, t->v_fexp, t->allrev, t->cnt, (int)kCnRev + 2, t->stats);
+        if (nu > 0) {
+            *undos += nu;
+            if (f32)
+                hipLaunchKernelGGL((k_undo_apply<true>), dim3(blocks_for(nu)), dim3(kThreads), 0, s, m, t->undo, t->cnt,
+                                   (int)kCnRev + 1, t->t_rep, t->bad[nxt], (int)kCnBad + nxt, t->stats);
+            else
+                hipLaunchKernelGGL((k_undo_apply<false>), dim3(blocks_for(nu)), dim3(kThreads), 0, s, m, t->undo, t->cnt,
+                                   (int)kCnRev + 1, t->t_rep, t->bad[nxt], (int)kCnBad + nxt, t->stats);
+        }
+        if (nr > 0) {
+            if (f32) MCL_TRY(check_list<true>(t, s, t->rev, kCnRev, nr, t->bad[nxt], kCnBad + nxt));
+            else MCL_TRY(check_list<false>(t, s, t->rev, kCnRev, nr, t->bad[nxt], kCnBad + nxt));
+        }
+        MCL_HIP(hipGetLastError());
+        MCL_TRY(read_counts(t, s));
+        *cur = nxt;
+        if (nr == 0 && nu == 0) { *stuck = true; return MCL_OK; }
+    }
+    return MCL_OK;
+}
+
+// relocation rounds for the vertices listed in pend[*slot] (count in
+// kCnPend); each round re-locates, claims and applies; losers retry
+static int relocate_rounds(mcl_tri* t, hipStream_t s, bool f32, int n, int* slot, int* rounds) {
+    DMesh& m = t->m;
+    *rounds = 0;
+    int cur = *slot;
+    for (int r = 0; r < 256 && n > 0; r++) {
+        const int R = t->round++;
+        const int nxt = 1 - cur;
+        hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->cnt + kCnPend, 1, n);
+        hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->cnt + kCnMove, 1, 0);
+        if (f32)
+            hipLaunchKernelGGL((k_reloc_locate<true>), dim3(blocks_for(n)), dim3(kThreads), 0, s, m, t->pend[cur], t->cnt,
+                               (int)kCnPend, t->tgt, t->r_tt, t->r_tt2);
+        else
+            hipLaunchKernelGGL((k_reloc_locate<false>), dim3(blocks_for(n)), dim3(kThreads), 0, s, m, t->pend[cur], t->cnt,
+                               (int)kCnPend, t->tgt, t->r_tt, t->r_tt2);
+        hipLaunchKernelGGL(k_reloc_claim, dim3(blocks_for(n)), dim3(kThreads), 0, s, m, t->pend[cur], t->cnt, (int)kCnPend,
+                           R, t->r_tt, t->r_tt2, t->claim);
+        if (f32)
+            hipLaunchKernelGGL((k_reloc_apply<true>), dim3(blocks_for(n)), dim3(kThreads), 0, s, m, t->pend[cur], t->cnt,
+                               (int)kCnPend, R, t->r_tt, t->r_tt2, t->claim, t->tgt, t->pend[nxt], t->cnt, (int)kCnMove,
+                               t->stats);
+        else
+            hipLaunchKernelGGL((k_reloc_apply<false>), dim3(blocks_for(n)), dim3(kThreads), 0, s, m, t->pend[cur], t->cnt,
+                               (int)kCnPend, R, t->r_tt, t->r_tt2, t->claim, t->tgt, t->pend[nxt], t->cnt, (int)kCnMove,
+                               t->stats);
+        MCL_HIP(hipGetLastError());
+        MCL_TRY(read_counts(t, s));
+        n = t->h_cnt[kCnMove];
+        cur = nxt;
+        (*rounds)++;
+    }
+    *slot = cur;
+    return n == 0 ? MCL_OK : MCL_ERR_ROUND_CAP;
+}
+
+// One kinetic step:
+//   move      -- every non-corner vertex goes to its target
+//   check     -- triangles that are not strictly CCW
+//   repair    -- rounds of flips of an edge between a bad and a good triangle
+//                whose two new triangles are both strictly CCW (the B-step
+//                of Host_Triangulation.h:445-450 plus its on-edge case).  A
+//                triangle made by a repair is not repaired again in the same
+//                step, so every repair can be undone on its own.
+//   revert    -- each remaining bad triangle either undoes the repair that
+//                made it or sends its largest-displacement vertex back to its
+//                previous position, until no triangle is bad (this
+//                terminates: with every repair undone and every vertex back,
+//                the mesh is the previous, valid, state)
+//   relocate  -- vertices sent back are removed from their stars and
+//                re-inserted at their targets
+//   Lawson    -- flip rounds to the exact Delaunay triangulation
+extern "C" int mcl_tri_step(mcl_tri_t* t, const mcl_step_params_t* p, mcl_stats_t* st, void* stream) {
+    if (!t || !p) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    DMesh& m = t->m;
+    auto t0 = std::chrono::steady_clock::now();
+    stats_begin(t, s);
+    const int variant = p->variant;
+    const int gb = t->grid_blocks;
+    const bool f32 = variant == 0;
+    static const bool trace = getenv("MCL_TRACE") != nullptr;
+
+    const int mv_stamp = ++t->stamp;
+    hipLaunchKernelGGL(k_move_motion, dim3(blocks_for(m.nv)), dim3(kThreads), 0, s, m, t->acc, t->tgt, t->v_mv,
+                       t->v_fexp, mv_stamp, *p);
+    MCL_TRY(zero_slots(t, s, kCnBad, 11));  // bad, rev, move, pend, err, misc, touched
+    if (f32)
+        hipLaunchKernelGGL((k_check_all<true>), dim3(gb), dim3(kThreads), 0, s, m, t->bad[0], t->cnt, (int)kCnBad, t->stats);
+    else
+        hipLaunchKernelGGL((k_check_all<false>), dim3(gb), dim3(kThreads), 0, s, m, t->bad[0], t->cnt, (int)kCnBad, t->stats);
+    MCL_HIP(hipGetLastError());
+    MCL_TRY(read_counts(t, s));
+    const long long inverted0 = t->h_cnt[kCnBad];
+    int cur = 0, nrep = 0, nundo = 0, reloc_rounds = 0;
+    bool stuck = false;
+    MCL_TRY(repair_rounds(t, s, f32, &cur, mv_stamp, &nrep));
+    const int bad_after = t->h_cnt[kCnBad + cur];
+    MCL_TRY(revert_passes(t, s, f32, &cur, mv_stamp, mv_stamp, &nundo, &stuck));
+    if (stuck) return MCL_ERR_TOPOLOGY;  // cannot happen: see k_revert_mark
+    const int nrev = t->h_cnt[kCnRev + 2];
+    int rc = MCL_OK;
+    if (nrev > 0) {
+        // relocate the vertices that were sent back (list allrev -> pend[0])
+        MCL_HIP(hipMemcpyAsync(t->pend[0], t->allrev, sizeof(int) * nrev, hipMemcpyDeviceToDevice, s));
+        int slot = 0;
+        rc = relocate_rounds(t, s, f32, nrev, &slot, &reloc_rounds);
+    }
+    long long rounds = 0;
+    if (rc == MCL_OK) rc = run_rounds<0>(t, true, variant, 0, s, &rounds);
+    if (trace)
+        fprintf(stderr, "[step] inverted %lld repairs %d bad_after_repair %d undone %d reverted %d reloc_rounds %d lawson %lld\n",
+                inverted0, nrep, bad_after, nundo, nrev, reloc_rounds, rounds);
+    if (st) st->rounds = rounds;
+    int rc2 = stats_end(t, s, st, t0);
+    if (st) {
+        st->iterations = 1 + reloc_rounds;
+        st->inverted = inverted0;
+    }
+    return rc != MCL_OK ? rc : rc2;
+}
+
+// ---------------------------------------------------------------- one ring
+extern "C" int mcl_tri_one_ring(mcl_tri_t* t, int K, int layout, void* d_ring, void* stream) {
+    if (!t || !d_ring || K < 2 || (layout != 0 && layout != 1)) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_ring_legacy, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, K, layout, (int*)d_ring);
+    MCL_HIP(hipGetLastError());
+    MCL_HIP(hipStreamSynchronize(s));
+    return MCL_OK;
+}
+
+extern "C" int mcl_tri_one_ring_csr(mcl_tri_t* t, void* d_offsets, void* d_nbrs, int cap, int* total, void* stream) {
+    if (!t || !d_offsets) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    int* off = (int*)d_offsets;
+    hipLaunchKernelGGL(k_degree, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, off);
+    MCL_TRY(scan_exclusive(off, off, t->m.nv, t->scan_tmp, s));
+    int tot = 0;
+    MCL_HIP(hipMemcpyAsync(&tot, off + t->m.nv, sizeof(int), hipMemcpyDeviceToHost, s));
+    MCL_HIP(hipStreamSynchronize(s));
+    if (total) *total = tot;
+    if (d_nbrs && cap > 0) {
+        hipLaunchKernelGGL(k_ring_csr, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, off, (int*)d_nbrs, cap);
+        MCL_HIP(hipGetLastError());
+        MCL_HIP(hipStreamSynchronize(s));
+    }
+    return tot > cap ? MCL_ERR_CAPACITY : MCL_OK;
+}
+
+extern "C" int mcl_tri_closest(mcl_tri_t* t, const void* d_offsets, const void* d_nbrs, void* d_out, void* stream) {
+    if (!t || !d_offsets || !d_nbrs || !d_out) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_closest, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, (const int*)d_offsets,
+                       (const int*)d_nbrs, (int*)d_out);
+    MCL_HIP(hipGetLastError());
+    MCL_HIP(hipStreamSynchronize(s));
+    return MCL_OK;
+}

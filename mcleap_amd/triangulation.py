@@ -1,0 +1,333 @@
+"""Python mirror of the reference's host-side interface, over the C ABI.
+
+Class and method names follow the reference's C++ types so tests and scripts
+read like the reference's own:
+  HostTriangulation    Host_Triangulation.h:24-98
+  DeviceTriangulation  Device_Triangulation.h:128-223
+  GridCount2d          gridCount2d.h:23-48
+Host arrays are numpy AoS records in the reference layouts
+(m_pos float32 (V,2), m_v int32 (V,2) = {pos_index, he},
+m_he int32 (HE,4) = {next, v, t, op}, m_t int32 (T,) = {he}).
+Device buffers are torch tensors on the GPU; every device call goes to
+libmcleap_hip.so and raises on failure (there is no CPU fallback).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+_vp = ctypes.c_void_p
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(_vp)
+
+
+def _dev_ptr(x):
+    """device pointer of a torch tensor (or an int address)"""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return _vp(x)
+    if not x.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not x.is_contiguous():
+        raise ValueError("expected a contiguous device tensor")
+    return _vp(x.data_ptr())
+
+
+class HostTriangulation:
+    """Host triangulation (Host_Triangulation.h:24-98).
+
+    The constructor holds the 4 corner vertices at +-1e7 and 2 triangles
+    (Host_Triangulation.h:107-137).  addDelaunayPoints builds the exact
+    Delaunay triangulation in libmcleap_hip (Hilbert insertion order) and,
+    like the reference, reorders the caller's point array in place so that
+    vertex 4 + k holds points[k].  periodic=True builds the unit-torus
+    triangulation (no corners; vertex k holds points[k]).
+    """
+
+    def __init__(self, periodic=False):
+        self.periodic = bool(periodic)
+        self._points = np.zeros((0, 2), np.float32)
+        self.perm = np.zeros(0, np.int32)
+        if self.periodic:
+            self.m_pos = np.zeros((0, 2), np.float32)
+            self.m_v = np.zeros((0, 2), np.int32)
+            self.m_he = np.zeros((0, 4), np.int32)
+            self.m_t = np.zeros(0, np.int32)
+        else:
+            self._build(np.zeros((0, 2), np.float32))
+
+    def _build(self, pts):
+        L = lib()
+        h = _vp()
+        pts = np.ascontiguousarray(pts, np.float32)
+        check(L.mcl_hmesh_build(_np_ptr(pts), len(pts), int(self.periodic), ctypes.byref(h)), "mcl_hmesh_build")
+        try:
+            sz = [ctypes.c_int() for _ in range(4)]
+            check(L.mcl_hmesh_sizes(h, *[ctypes.byref(s) for s in sz]), "mcl_hmesh_sizes")
+            npos, nv, nhe, nt = (s.value for s in sz)
+            self.m_pos = np.zeros((npos, 2), np.float32)
+            self.m_v = np.zeros((nv, 2), np.int32)
+            self.m_he = np.zeros((nhe, 4), np.int32)
+            self.m_t = np.zeros(nt, np.int32)
+            perm = np.zeros(len(pts), np.int32)
+            check(L.mcl_hmesh_export(h, _np_ptr(self.m_pos), _np_ptr(self.m_v), _np_ptr(self.m_he),
+                                     _np_ptr(self.m_t), _np_ptr(perm)), "mcl_hmesh_export")
+        finally:
+            L.mcl_hmesh_free(h)
+        self.perm = perm
+        self._points = pts[perm]
+
+    def addDelaunayPoints(self, points):
+        """Host_Triangulation.h:296-300.  Reorders `points` in place (numpy)."""
+        pts = np.asarray(points, np.float32).reshape(-1, 2)
+        allpts = np.concatenate([self._points, pts]) if len(self._points) else pts
+        self._build(allpts)
+        if isinstance(points, np.ndarray) and points.shape == pts.shape and len(self._points) == len(pts):
+            points[...] = self._points
+        return True
+
+    def num_points(self):
+        return len(self._points)
+
+
+class DeviceTriangulation:
+    """Device triangulation (Device_Triangulation.h:128-223), SoA on the GPU."""
+
+    blocksize = 256
+
+    def __init__(self, host):
+        self.h_triangulation = host
+        L = lib()
+        self._h = _vp()
+        check(L.mcl_tri_create(_np_ptr(np.ascontiguousarray(host.m_pos)), len(host.m_pos),
+                               _np_ptr(np.ascontiguousarray(host.m_v)), len(host.m_v),
+                               _np_ptr(np.ascontiguousarray(host.m_he)), len(host.m_he),
+                               _np_ptr(np.ascontiguousarray(host.m_t)), len(host.m_t),
+                               int(host.periodic), ctypes.byref(self._h)), "mcl_tri_create")
+        self.m_pos_size = len(host.m_pos)
+        self.m_v_size = len(host.m_v)
+        self.m_he_size = len(host.m_he)
+        self.m_t_size = len(host.m_t)
+        self.periodic = host.periodic
+        self.last_stats = None
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            try:
+                lib().mcl_tri_destroy(self._h)
+            except Exception:
+                pass
+            self._h = _vp()
+
+    @property
+    def m_pos(self):
+        """device address of the float2 positions (Device_Triangulation.h:139)"""
+        return lib().mcl_tri_pos_ptr(self._h)
+
+    def pos_tensor(self):
+        """the device positions as a torch view (no copy)"""
+        import torch
+        return _tensor_from_ptr(self.m_pos, (self.m_pos_size, 2), torch.float32)
+
+    # ------------------------------------------------------------ transfers
+    def transferToHost(self):
+        h = self.h_triangulation
+        check(lib().mcl_tri_download(self._h, _np_ptr(h.m_pos), _np_ptr(h.m_v), _np_ptr(h.m_he),
+                                     _np_ptr(h.m_t)), "mcl_tri_download")
+        return True
+
+    def transferToDevice(self):
+        h = self.h_triangulation
+        check(lib().mcl_tri_upload(self._h, _np_ptr(np.ascontiguousarray(h.m_pos)),
+                                   _np_ptr(np.ascontiguousarray(h.m_v)), _np_ptr(np.ascontiguousarray(h.m_he)),
+                                   _np_ptr(np.ascontiguousarray(h.m_t))), "mcl_tri_upload")
+        return True
+
+    def download_positions(self):
+        pos = np.zeros((self.m_pos_size, 2), np.float32)
+        check(lib().mcl_tri_download(self._h, _np_ptr(pos), None, None, None), "mcl_tri_download")
+        return pos
+
+    def download(self):
+        pos = np.zeros((self.m_pos_size, 2), np.float32)
+        v = np.zeros((self.m_v_size, 2), np.int32)
+        he = np.zeros((self.m_he_size, 4), np.int32)
+        t = np.zeros(self.m_t_size, np.int32)
+        check(lib().mcl_tri_download(self._h, _np_ptr(pos), _np_ptr(v), _np_ptr(he), _np_ptr(t)),
+              "mcl_tri_download")
+        return pos, v, he, t
+
+    # ---------------------------------------------------------- operations
+    def _stats_call(self, fn, *args, allow=()):
+        st = _lib.Stats()
+        rc = fn(self._h, *args, ctypes.byref(st), None)
+        self.last_stats = st.as_dict()
+        if rc not in allow:
+            check(rc, fn.__name__)
+        return rc
+
+    def delonize(self, max_rounds=0):
+        """Lawson rounds with one launch pair and a host read per round (the
+        reference's loop shape, Device_Triangulation.h:353-390)."""
+        self._stats_call(lib().mcl_tri_delonize, 1, int(max_rounds))
+        return True
+
+    def delonize2(self, max_rounds=0):
+        """Lawson rounds driven on the device (Device_Triangulation.h:324-351)."""
+        self._stats_call(lib().mcl_tri_delonize, 0, int(max_rounds))
+        return True
+
+    def untangle(self, max_rounds=0):
+        """B/C-step rounds (Device_Triangulation.h:422-459); False if
+        inverted triangles remain."""
+        rc = self._stats_call(lib().mcl_tri_untangle, int(max_rounds), allow=(0, 6))
+        return rc == 0
+
+    untangle2 = untangle  # Device_Triangulation.h:461-495
+
+    def movePoints(self, d):
+        """Device_Triangulation.h:395-402: d is a device (V,2) float32 tensor"""
+        check(lib().mcl_tri_move(self._h, _dev_ptr(d), None), "mcl_tri_move")
+        return True
+
+    def integrateMovePoints(self, dt, v):
+        """Device_Triangulation.h:403-410"""
+        check(lib().mcl_tri_integrate(self._h, _dev_ptr(dt), _dev_ptr(v), None), "mcl_tri_integrate")
+        return True
+
+    def step(self, motion, sigma, amp=0.0, seed=1, step=0, variant=0, max_iterations=0):
+        """one kinetic step (move, keep valid, restore Delaunay)"""
+        p = _lib.StepParams(int(motion), float(sigma), float(amp), int(seed), int(step), int(variant),
+                            int(max_iterations))
+        st = _lib.Stats()
+        check(lib().mcl_tri_step(self._h, ctypes.byref(p), ctypes.byref(st), None), "mcl_tri_step")
+        self.last_stats = st.as_dict()
+        return self.last_stats
+
+    def count_inverted(self):
+        out = ctypes.c_longlong()
+        check(lib().mcl_tri_count_inverted(self._h, ctypes.byref(out), None), "mcl_tri_count_inverted")
+        return out.value
+
+    def count_non_delaunay(self):
+        out = ctypes.c_longlong()
+        check(lib().mcl_tri_count_non_delaunay(self._h, ctypes.byref(out), None), "mcl_tri_count_non_delaunay")
+        return out.value
+
+    # ------------------------------------------------------------- queries
+    def oneRing(self, K, ring):
+        """oneRing<K>(int*): SoA ring[(c+1)*V + i], count in ring[i] (kernels.cuh:54-84)"""
+        check(lib().mcl_tri_one_ring(self._h, int(K), 0, _dev_ptr(ring), None), "mcl_tri_one_ring")
+        return True
+
+    def oneRing2(self, K, ring):
+        """oneRing2<K>(int*): AoS ring[(c+1) + K*i], count in ring[K*i] (kernels.cuh:91-121)"""
+        check(lib().mcl_tri_one_ring(self._h, int(K), 1, _dev_ptr(ring), None), "mcl_tri_one_ring")
+        return True
+
+    def one_ring_csr(self, offsets, nbrs=None):
+        """CSR one-ring (no cap); returns the entry count"""
+        tot = ctypes.c_int()
+        cap = 0 if nbrs is None else int(nbrs.numel())
+        rc = lib().mcl_tri_one_ring_csr(self._h, _dev_ptr(offsets), _dev_ptr(nbrs), cap, ctypes.byref(tot), None)
+        if nbrs is not None:
+            check(rc, "mcl_tri_one_ring_csr")
+        return tot.value
+
+    def closestNeighbors(self, offsets, nbrs, out):
+        check(lib().mcl_tri_closest(self._h, _dev_ptr(offsets), _dev_ptr(nbrs), _dev_ptr(out), None),
+              "mcl_tri_closest")
+        return True
+
+
+def _tensor_from_ptr(ptr, shape, dtype):
+    """wrap a device allocation owned by the library as a torch tensor"""
+    import torch
+
+    class _Holder:
+        def __init__(self, p, nbytes):
+            self.__cuda_array_interface__ = {
+                "shape": shape, "typestr": "<f4" if dtype == torch.float32 else "<i4",
+                "data": (p, False), "version": 2}
+
+    n = int(np.prod(shape))
+    return torch.as_tensor(_Holder(ptr, n * 4), device="cuda")
+
+
+class GridCount2d:
+    """Uniform-grid FRNN (gridCount2d.h:23-48)."""
+
+    blocksize = 256
+
+    def __init__(self, numP, min_xy, cell_size, num_cells, flags=_lib.GRID_ROWMAJOR):
+        self.numP = int(numP)
+        self._g = _vp()
+        check(lib().mcl_grid_create(self.numP, float(min_xy[0]), float(min_xy[1]), float(cell_size[0]),
+                                    float(cell_size[1]), int(num_cells[0]), int(num_cells[1]), int(flags),
+                                    ctypes.byref(self._g)), "mcl_grid_create")
+
+    def __del__(self):
+        if getattr(self, "_g", None) and self._g.value:
+            try:
+                lib().mcl_grid_destroy(self._g)
+            except Exception:
+                pass
+            self._g = _vp()
+
+    def update(self, pos, vel=None):
+        """sorts the device arrays in place by cell (gridCount2d.h:119-135)"""
+        check(lib().mcl_grid_update(self._g, _dev_ptr(pos), _dev_ptr(vel), None), "mcl_grid_update")
+
+    def index_ptrs(self):
+        ind, perm, start, cnt = _vp(), _vp(), _vp(), _vp()
+        check(lib().mcl_grid_index_ptrs(self._g, ctypes.byref(ind), ctypes.byref(perm), ctypes.byref(start),
+                                        ctypes.byref(cnt)), "mcl_grid_index_ptrs")
+        return ind.value, perm.value, start.value
+
+    def d_ind(self):
+        """new slot of each previous index (gridCount2d.h:40) as a torch int32 tensor"""
+        import torch
+        ind, _, _ = self.index_ptrs()
+        return _tensor_from_ptr(ind, (self.numP,), torch.int32).clone()
+
+    def perm(self):
+        import torch
+        _, perm, _ = self.index_ptrs()
+        return _tensor_from_ptr(perm, (self.numP,), torch.int32).clone()
+
+    def frnn_csr(self, pos, rad, offsets, nbrs=None):
+        """pairs within rad of each sorted slot, as CSR over slots; returns the pair count"""
+        tot = ctypes.c_longlong()
+        cap = 0 if nbrs is None else int(nbrs.numel())
+        rc = lib().mcl_grid_frnn_csr(self._g, _dev_ptr(pos), float(rad), _dev_ptr(offsets), _dev_ptr(nbrs), cap,
+                                     ctypes.byref(tot), None)
+        if nbrs is not None:
+            check(rc, "mcl_grid_frnn_csr")
+        return tot.value
+
+    def apply_f_frnn(self, f, pos, rad):
+        """calls f(i, j, d, dist) with vectors of all pairs (i, j) within rad,
+        i == j included (gridCount2d.h:262-302)"""
+        import torch
+        n = self.numP
+        off = torch.zeros(n + 1, dtype=torch.int32, device="cuda")
+        tot = self.frnn_csr(pos, rad, off)
+        nbr = torch.zeros(max(tot, 1), dtype=torch.int32, device="cuda")
+        self.frnn_csr(pos, rad, off, nbr)
+        counts = (off[1:] - off[:-1]).long()
+        i = torch.repeat_interleave(torch.arange(n, device="cuda"), counts)
+        j = nbr[:tot].long()
+        i = torch.cat([i, torch.arange(n, device="cuda")])
+        j = torch.cat([j, torch.arange(n, device="cuda")])
+        d = pos[j] - pos[i]
+        f(i, j, d, torch.sqrt((d.double() ** 2).sum(1)))
+
+    def mean_num_particle_in_cell(self):
+        out = ctypes.c_float()
+        check(lib().mcl_grid_mean_occupancy(self._g, ctypes.byref(out), None), "mcl_grid_mean_occupancy")
+        return out.value

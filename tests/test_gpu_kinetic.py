@@ -1,0 +1,125 @@
+"""GPU parity of the kinetic step: move -> keep valid -> Lawson flips.
+
+After every step the downloaded mesh must (1) satisfy the half-edge
+invariants with every triangle strictly CCW (exact orientation), (2) have no
+edge failing the exact incircle test, and (3) equal, as a canonical triangle
+set, the oracle's from-scratch exact Delaunay triangulation of the downloaded
+positions (the restated HostTriangulation with exact predicates; unique
+because of the perturbation tie rule).
+"""
+import numpy as np
+import pytest
+
+import mcleap_amd as M
+from tests.conftest import uniform_points
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_box(O, dt, tag):
+    pos, v, he, t = dt.download()
+    rc, msg = O.check_mesh(pos, v, he, t, periodic=False, require_ccw=True)
+    assert rc == 0, "%s: %s" % (tag, msg)
+    assert O.count_non_delaunay(pos, he) == 0, tag
+    ref = O.delaunay(pos[4:])
+    p2, v2, h2, t2 = ref.export()
+    assert np.array_equal(O.canonical_triangles(he, t), O.canonical_triangles(h2, t2)), tag
+    return pos
+
+
+def _check_torus(O, dt, tag, full=True):
+    pos, v, he, t = dt.download()
+    rc, msg = O.check_mesh(pos, v, he, t, periodic=True, require_ccw=True)
+    assert rc == 0, "%s: %s" % (tag, msg)
+    assert O.count_non_delaunay(pos, he, periodic=True) == 0, tag
+    if full:
+        tri = O.torus_delaunay(pos)
+        assert np.array_equal(O.canonical_triangles(he, t), O.canonical_from_list(tri)), tag
+    return pos
+
+
+def test_upload_download_roundtrip(oracle):
+    xy = uniform_points(2000, 1)
+    H = M.HostTriangulation()
+    H.addDelaunayPoints(xy)
+    dt = M.DeviceTriangulation(H)
+    pos, v, he, t = dt.download()
+    assert np.array_equal(pos, H.m_pos) and np.array_equal(v, H.m_v)
+    assert np.array_equal(he, H.m_he) and np.array_equal(t, H.m_t)
+    assert dt.count_inverted() == 0
+    assert dt.count_non_delaunay() == 0
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_box_steps_config1(oracle, variant):
+    """Config 1 shape (N=10,000, sigma=1e-3, clamped), checked after every step."""
+    O = oracle
+    xy = uniform_points(10000, 1)
+    H = M.HostTriangulation()
+    H.addDelaunayPoints(xy)
+    dt = M.DeviceTriangulation(H)
+    prev = dt.download_positions()
+    nsteps = 12 if variant == 0 else 6
+    for k in range(nsteps):
+        st = dt.step(M.MOTION_GAUSS, 1e-3, seed=1, step=k, variant=variant)
+        print("step", k, st)
+        pos = _check_box(O, dt, "variant %d step %d" % (variant, k))
+        moved = np.any(pos != prev, axis=1)
+        assert moved[4:].mean() > 0.99  # nearly every point moved
+        assert np.all(pos[4:] >= 0) and np.all(pos[4:] < 1)
+        # a vertex whose relocation meets a degenerate (collinear) link keeps
+        # its position for the step; the mesh is still exact for the positions
+        assert st["stalled"] <= 3
+        prev = pos
+
+
+def test_box_large_moves(oracle):
+    """moves of ~0.5 mean spacing invert many triangles; the retreat path must
+    keep the mesh valid and still reach the exact DT."""
+    O = oracle
+    xy = uniform_points(4000, 7)
+    H = M.HostTriangulation()
+    H.addDelaunayPoints(xy)
+    dt = M.DeviceTriangulation(H)
+    for k in range(4):
+        st = dt.step(M.MOTION_GAUSS, 0.5 / np.sqrt(4000), seed=3, step=k)
+        print(st)
+        assert st["inverted"] > 0 or k > 0
+        _check_box(O, dt, "large step %d" % k)
+
+
+def test_torus_steps(oracle):
+    """Config 2 shape on the torus at N=20,000 (shear flow + noise, wrapped)."""
+    O = oracle
+    xy = uniform_points(20000, 2)
+    H = M.HostTriangulation(periodic=True)
+    H.addDelaunayPoints(xy)
+    dt = M.DeviceTriangulation(H)
+    for k in range(6):
+        st = dt.step(M.MOTION_SHEAR, 2e-4, amp=1e-3, seed=2, step=k)
+        print(st)
+        _check_torus(O, dt, "torus step %d" % k)
+
+
+def test_delonize_from_perturbed(oracle):
+    """delonize/delonize2 on a valid but non-Delaunay mesh (positions nudged
+    below the inversion threshold) must reach the exact DT."""
+    O = oracle
+    xy = uniform_points(5000, 4)
+    H = M.HostTriangulation()
+    H.addDelaunayPoints(xy)
+    for variant in (0, 1):
+        dt = M.DeviceTriangulation(H)
+        import torch
+        d = torch.zeros((dt.m_pos_size, 2), dtype=torch.float32, device="cuda")
+        g = torch.Generator(device="cuda").manual_seed(5)
+        d[4:] = 1e-4 * torch.randn((dt.m_pos_size - 4, 2), generator=g, device="cuda")
+        dt.movePoints(d)
+        torch.cuda.synchronize()
+        if dt.count_inverted() != 0:
+            pytest.skip("perturbation inverted a triangle")
+        nd = dt.count_non_delaunay()
+        (dt.delonize2 if variant == 0 else dt.delonize)()
+        print(variant, nd, dt.last_stats)
+        assert dt.last_stats["flips"] > 0
+        _check_box(O, dt, "delonize variant %d" % variant)
