@@ -146,7 +146,9 @@ int mcl_tri_step(mcl_tri_t* t, const mcl_step_params_t* p, mcl_stats_t* st, void
  * d_ring is caller-allocated device memory of nv*K ints. */
 int mcl_tri_one_ring(mcl_tri_t* t, int K, int layout, void* d_ring, void* stream);
 /* one-ring as CSR (no cap): d_offsets nv+1 ints, d_nbrs cap ints; same CW
- * order as above.  *total = number of entries (MCL_ERR_CAPACITY if > cap). */
+ * order as above.  *total = number of entries (MCL_ERR_CAPACITY if > cap).
+ * With total == NULL and cap >= nhe the call is asynchronous (the entry
+ * count always equals the half-edge count). */
 int mcl_tri_one_ring_csr(mcl_tri_t* t, void* d_offsets, void* d_nbrs, int cap, int* total,
                          void* stream);
 /* closestNeighbors<K>(ring, out) (kernels.cuh:129-146) over a CSR one-ring */
@@ -176,11 +178,28 @@ int mcl_grid_index_ptrs(mcl_grid_t* g, void** d_ind, void** d_perm, void** d_cel
 /* apply_f_frnn(f, pos, rad) (gridCount2d.h:262-302) as CSR over sorted slots:
  * for each slot i, every slot j != i in the 3x3 cells around i with
  * r2 = dx*dx + dy*dy <= rad*rad (FP32, no contraction).  d_offsets has
- * nump+1 ints, d_nbrs cap ints.  *total = pair count (MCL_ERR_CAPACITY if > cap). */
+ * nump+1 ints, d_nbrs cap ints.  *total = pair count (MCL_ERR_CAPACITY if > cap).
+ * With total == NULL the call is asynchronous; entries past cap are dropped
+ * and d_offsets[nump] still holds the full count. */
 int mcl_grid_frnn_csr(mcl_grid_t* g, const void* d_pos, float rad, void* d_offsets, void* d_nbrs,
                       int cap, long long* total, void* stream);
+/* the counting sort of update() without touching the source: the sorted
+ * positions stay in the grid (mcl_grid_sorted_ptr); asynchronous */
+int mcl_grid_build(mcl_grid_t* g, const void* d_src_pos, void* stream);
+void* mcl_grid_sorted_ptr(mcl_grid_t* g);
 /* mean_num_particle_in_cell (gridCount2d.h:304-334) */
 int mcl_grid_mean_occupancy(mcl_grid_t* g, float* out, void* stream);
+
+/* ------------------------------------------------------- kernel timers */
+/* GPU time of the main kernels, from HIP event pairs recorded on the launch
+ * stream around every launch while enabled (no synchronisation is added).
+ * Triangulation kernels: 0 = first Lawson round over all edges,
+ * 1 = orientation check of all triangles, 2 = move.
+ * Grid kernels: 0 = FRNN count pass, 1 = FRNN fill pass, 2 = grid build. */
+int mcl_tri_kernel_timer(mcl_tri_t* t, int enable);
+int mcl_tri_kernel_timer_read(mcl_tri_t* t, int which, double* total_ms, long long* launches);
+int mcl_grid_kernel_timer(mcl_grid_t* g, int enable);
+int mcl_grid_kernel_timer_read(mcl_grid_t* g, int which, double* total_ms, long long* launches);
 
 /* ----------------------------------------------------------------- misc */
 /* library build string and the device it runs on */

@@ -103,10 +103,16 @@ SIGNATURES = {
     "mcl_grid_index_ptrs": [_p, _pp, _pp, _pp, _pp],
     "mcl_grid_frnn_csr": [_p, _p, _f, _p, _p, _i, _p, _p],
     "mcl_grid_mean_occupancy": [_p, _p, _p],
+    "mcl_grid_build": [_p, _p, _p],
+    "mcl_grid_sorted_ptr": [_p],
+    "mcl_tri_kernel_timer": [_p, _i],
+    "mcl_tri_kernel_timer_read": [_p, _i, _p, _p],
+    "mcl_grid_kernel_timer": [_p, _i],
+    "mcl_grid_kernel_timer_read": [_p, _i, _p, _p],
     "mcl_version": [],
     "mcl_device_info": [_p, _i, _p],
 }
-RESTYPES = {"mcl_tri_pos_ptr": _p, "mcl_version": ctypes.c_char_p}
+RESTYPES = {"mcl_tri_pos_ptr": _p, "mcl_grid_sorted_ptr": _p, "mcl_version": ctypes.c_char_p}
 
 
 def lib():

@@ -178,6 +178,7 @@ struct mcl_grid {
     float2* spos = nullptr;
     float2* svel = nullptr;
     unsigned long long* scalar = nullptr;
+    mcl::KTimer timers[3];  // FRNN count, FRNN fill, build
 };
 
 static void grid_free(mcl_grid* G) {
@@ -249,6 +250,30 @@ extern "C" int mcl_grid_update(mcl_grid_t* G, void* d_pos, void* d_vel, void* st
     return MCL_OK;
 }
 
+// same counting sort, but the source stays untouched: the sorted copy lives
+// in the grid (mcl_grid_sorted_ptr).  Asynchronous on `stream`.
+extern "C" int mcl_grid_build(mcl_grid_t* G, const void* d_src, void* stream) {
+    if (!G || !d_src) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    const int n = G->nump, nk = G->g.nkeys;
+    const float2* pos = (const float2*)d_src;
+    G->timers[2].mark(s);
+    hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, G->start, nk + 1);
+    hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, G->fill, nk + 1);
+    hipLaunchKernelGGL(k_key_count, dim3(blocks(n)), dim3(kT), 0, s, G->g, pos, n, G->key, G->start);
+    int rc = scan_exclusive(G->start, G->start, nk, G->scan_tmp, s);
+    if (rc != MCL_OK) return rc;
+    hipLaunchKernelGGL(k_slot, dim3(blocks(n)), dim3(kT), 0, s, G->key, n, G->start, G->fill, G->perm);
+    hipLaunchKernelGGL(k_cell_sort, dim3(blocks(nk)), dim3(kT), 0, s, G->start, nk, G->perm);
+    hipLaunchKernelGGL(k_gather, dim3(blocks(n)), dim3(kT), 0, s, G->perm, n, pos, (const float2*)nullptr, G->spos,
+                       G->svel, G->ind);
+    G->timers[2].mark(s);
+    MCL_HIP(hipGetLastError());
+    return MCL_OK;
+}
+
+extern "C" void* mcl_grid_sorted_ptr(mcl_grid_t* G) { return G ? (void*)G->spos : nullptr; }
+
 extern "C" int mcl_grid_index_ptrs(mcl_grid_t* G, void** d_ind, void** d_perm, void** d_cell_start,
                                    void** d_cell_count) {
     if (!G) return MCL_ERR_INPUT;
@@ -266,14 +291,25 @@ extern "C" int mcl_grid_frnn_csr(mcl_grid_t* G, const void* d_pos, float rad, vo
     const int n = G->nump;
     const float rad2 = rad * rad;
     int* off = (int*)d_offsets;
+    G->timers[0].mark(s);
     hipLaunchKernelGGL(k_frnn<0>, dim3(blocks(n)), dim3(kT), 0, s, G->g, (const float2*)d_pos, n, G->start, rad2, off,
                        (const int*)nullptr, (int*)nullptr, 0);
+    G->timers[0].mark(s);
     int rc = scan_exclusive(off, off, n, G->scan_tmp, s);
     if (rc != MCL_OK) return rc;
+    if (!total) {  // asynchronous form: the fill pass keeps writes below cap
+        G->timers[1].mark(s);
+        if (d_nbrs && cap > 0)
+            hipLaunchKernelGGL(k_frnn<1>, dim3(blocks(n)), dim3(kT), 0, s, G->g, (const float2*)d_pos, n, G->start, rad2,
+                               (int*)nullptr, (const int*)off, (int*)d_nbrs, cap);
+        G->timers[1].mark(s);
+        MCL_HIP(hipGetLastError());
+        return MCL_OK;
+    }
     int tot = 0;
     MCL_HIP(hipMemcpyAsync(&tot, off + n, sizeof(int), hipMemcpyDeviceToHost, s));
     MCL_HIP(hipStreamSynchronize(s));
-    if (total) *total = tot;
+    *total = tot;
     if (d_nbrs && cap > 0) {
         hipLaunchKernelGGL(k_frnn<1>, dim3(blocks(n)), dim3(kT), 0, s, G->g, (const float2*)d_pos, n, G->start, rad2,
                            (int*)nullptr, (const int*)off, (int*)d_nbrs, cap);
@@ -293,5 +329,17 @@ extern "C" int mcl_grid_mean_occupancy(mcl_grid_t* G, float* out, void* stream) 
     MCL_HIP(hipMemcpyAsync(&ne, G->scalar, sizeof(ne), hipMemcpyDeviceToHost, s));
     MCL_HIP(hipStreamSynchronize(s));
     *out = ne ? (float)G->nump / (float)ne : 0.0f;
+    return MCL_OK;
+}
+
+extern "C" int mcl_grid_kernel_timer(mcl_grid_t* G, int enable) {
+    if (!G) return MCL_ERR_INPUT;
+    for (auto& k : G->timers) k.reset(enable != 0);
+    return MCL_OK;
+}
+extern "C" int mcl_grid_kernel_timer_read(mcl_grid_t* G, int which, double* total_ms, long long* launches) {
+    if (!G || which < 0 || which > 2) return MCL_ERR_INPUT;
+    double ms = G->timers[which].total_ms(launches);
+    if (total_ms) *total_ms = ms;
     return MCL_OK;
 }

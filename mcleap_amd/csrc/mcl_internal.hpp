@@ -74,3 +74,42 @@ enum CountSlot : int {
 };
 
 }  // namespace mcl
+
+namespace mcl {
+// GPU time of one kernel class, measured with event pairs recorded on the
+// launch stream around each launch; read back after the timed region.
+struct KTimer {
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    void reset(bool enable) {
+        on = enable;
+        used = 0;
+    }
+    void mark(hipStream_t s) {
+        if (!on) return;
+        if (used == ev.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) { on = false; return; }
+            ev.push_back(e);
+        }
+        (void)hipEventRecord(ev[used++], s);
+    }
+    double total_ms(long long* launches) {
+        double tot = 0.0;
+        long long n = 0;
+        for (size_t i = 0; i + 1 < used; i += 2) {
+            float ms = 0.f;
+            if (hipEventSynchronize(ev[i + 1]) == hipSuccess && hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) {
+                tot += ms;
+                n++;
+            }
+        }
+        if (launches) *launches = n;
+        return tot;
+    }
+    ~KTimer() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+};
+}  // namespace mcl

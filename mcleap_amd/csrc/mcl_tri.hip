@@ -69,6 +69,21 @@ __device__ __forceinline__ float clamp_box(float x) {
     return fminf(fmaxf(x, 0.0f), hi) + 0.0f;
 }
 
+// ---------------------------------------------------- wave-aggregated append
+// slot in a list whose length is *ctr, for every lane that calls this at the
+// same point: the lanes active here (the exec mask) take consecutive slots
+// and one lane does a single atomic for the whole wave
+__device__ __forceinline__ int agg_inc(int* ctr) {
+    const unsigned long long active = __ballot(1);
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)active) - 1;
+    const int rank = __popcll(active & ((1ull << lane) - 1ull));
+    int base = 0;
+    if (lane == leader) base = atomicAdd(ctr, __popcll(active));
+    base = __shfl(base, leader);
+    return base + rank;
+}
+
 // ---------------------------------------------------- statistics helpers
 // per-thread counters reduced in LDS, one global atomic per block and slot
 template <int N>
@@ -190,7 +205,7 @@ __global__ void __launch_bounds__(kThreads) k_check_all(DMesh m, int* __restrict
         int s = orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st);
         loc[st]++;
         if (s <= 0) {
-            int k = atomicAdd(&cnt[slot], 1);
+            int k = agg_inc(&cnt[slot]);
             if (bad) bad[k] = t;
         }
     }
@@ -218,11 +233,20 @@ __global__ void __launch_bounds__(kThreads) k_revert_mark(DMesh m, const int* __
         const int t = bad[i];
         if (t_rep[t] == rep_stamp) {
             const int e = t_rep_e[t];
-            if (atomicExch(&e_undo[e], undo_stamp) != undo_stamp) undo_list[atomicAdd(&cnt[slot_undo], 1)] = e;
+            if (atomicExch(&e_undo[e], undo_stamp) != undo_stamp) undo_list[agg_inc(&cnt[slot_undo])] = e;
             continue;
         }
         int vs[3];
         tri_verts(m, t, vs[0], vs[1], vs[2]);
+        // the vertex that moved most relative to the other two (a common
+        // drift of all three does not invert a triangle)
+        float2 dsp[3];
+        for (int k = 0; k < 3; k++) {
+            const float2 q = m.pos[vs[k]], a = acc[vs[k]];
+            float dx = q.x - a.x, dy = q.y - a.y;
+            if (m.periodic) { dx = min_image_f(dx); dy = min_image_f(dy); }
+            dsp[k] = make_float2(dx, dy);
+        }
         int best = -1;
         float bd = -1.0f;
         for (int k = 0; k < 3; k++) {
@@ -230,32 +254,86 @@ __global__ void __launch_bounds__(kThreads) k_revert_mark(DMesh m, const int* __
             if (v_mv[v] != mv_stamp) continue;
             const float2 q = m.pos[v], a = acc[v];
             if (q.x == a.x && q.y == a.y) continue;
-            float dx = q.x - a.x, dy = q.y - a.y;
-            if (m.periodic) { dx = min_image_f(dx); dy = min_image_f(dy); }
-            const float d = dx * dx + dy * dy;
+            const float rx = dsp[k].x - 0.5f * (dsp[(k + 1) % 3].x + dsp[(k + 2) % 3].x);
+            const float ry = dsp[k].y - 0.5f * (dsp[(k + 1) % 3].y + dsp[(k + 2) % 3].y);
+            const float d = rx * rx + ry * ry;
             if (d > bd) { bd = d; best = v; }
         }
         if (best < 0) {
-            stuck[atomicAdd(&cnt[slot_stuck], 1)] = t;
+            stuck[agg_inc(&cnt[slot_stuck])] = t;
             continue;
         }
-        if (atomicExch(&v_rev[best], rev_stamp) != rev_stamp) rev_list[atomicAdd(&cnt[slot_rev], 1)] = best;
+        if (atomicExch(&v_rev[best], rev_stamp) != rev_stamp) rev_list[agg_inc(&cnt[slot_rev])] = best;
     }
 }
 
-// phase 2a: move the marked vertices back and halve their next attempt
+// all triangles around v strictly CCW with v at p (neighbours where they are now)
+template <bool kF32>
+__device__ __forceinline__ bool star_valid(const DMesh& m, int v, float2 p) {
+    const int h0 = m.v_he[v];
+    int h = h0, guard = 0, st;
+    do {
+        if (m.he_t[h] >= 0) {
+            const float2 a = m.pos[m.he_v[h ^ 1]], b = m.pos[m.he_op[h]];
+            if (orient<kF32>(p, a, b, m.periodic, &st) <= 0) return false;
+        }
+        h = m.he_next[h ^ 1];
+    } while (h != h0 && ++guard < 64);
+    return true;
+}
+
+__device__ __forceinline__ float2 shifted(const DMesh& m, float2 a, float dx, float dy) {
+    if (m.periodic) return make_float2(wrap01(a.x + dx), wrap01(a.y + dy));
+    return make_float2(a.x + dx, a.y + dy);
+}
+
+// phase 2a: move the marked vertices back.  The first time in a step a vertex
+// is sent back it tries positions consistent with its neighbours' motion --
+// its previous position shifted by their mean displacement, then the
+// centroid of its neighbours -- and keeps the first at which its whole star
+// is strictly CCW; otherwise (and on any later take-back) it returns to its
+// previous position.  Soft moves happen at most once per vertex and step, so
+// the passes still end with every taken-back vertex at its previous position
+// in the worst case, which is valid.
+template <bool kF32>
 __global__ void __launch_bounds__(kThreads) k_revert_apply(DMesh m, const int* __restrict__ rev_list,
                                                            const int* __restrict__ cnt, int slot_rev,
                                                            const float2* __restrict__ acc, int* __restrict__ v_fexp,
+                                                           int* __restrict__ v_soft, int soft_stamp,
                                                            int* __restrict__ all_rev, int* __restrict__ cnt_w, int slot_all,
                                                            unsigned long long* __restrict__ stats) {
     const int n = cnt[slot_rev];
     unsigned loc[1] = {0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int v = rev_list[i];
-        m.pos[v] = acc[v];
+        const float2 a = acc[v];
+        float2 dest = a;
+        const bool first = v_soft[v] != soft_stamp;
+        if (first) {
+            v_soft[v] = soft_stamp;
+            all_rev[agg_inc(&cnt_w[slot_all])] = v;
+            // neighbour displacement and neighbour centroid, relative to a
+            float sdx = 0.f, sdy = 0.f, scx = 0.f, scy = 0.f;
+            int deg = 0;
+            const int h0 = m.v_he[v];
+            int h = h0;
+            do {
+                const int u = m.he_v[h ^ 1];
+                const float2 pu = m.pos[u], au = acc[u];
+                float dx = pu.x - au.x, dy = pu.y - au.y, cx = pu.x - a.x, cy = pu.y - a.y;
+                if (m.periodic) { dx = min_image_f(dx); dy = min_image_f(dy); cx = min_image_f(cx); cy = min_image_f(cy); }
+                sdx += dx; sdy += dy; scx += cx; scy += cy;
+                deg++;
+                h = m.he_next[h ^ 1];
+            } while (h != h0 && deg < 64);
+            const float inv = 1.0f / (float)deg;
+            const float2 c1 = shifted(m, a, sdx * inv, sdy * inv);
+            const float2 c2 = shifted(m, a, scx * inv, scy * inv);
+            if (star_valid<kF32>(m, v, c1)) dest = c1;
+            else if (star_valid<kF32>(m, v, c2)) dest = c2;
+        }
+        m.pos[v] = dest;
         v_fexp[v] = min(v_fexp[v] + 1, 64);
-        all_rev[atomicAdd(&cnt_w[slot_all], 1)] = v;
         loc[0]++;
     }
     flush_stats(loc, stats, kStReverts);
@@ -283,7 +361,7 @@ __global__ void __launch_bounds__(kThreads) k_undo_apply(DMesh m, const int* __r
             tri_verts(m, ts[k], a, b, c);
             int sg = orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st);
             loc[st]++;
-            if (sg <= 0) bad_out[atomicAdd(&cnt[slot_out], 1)] = ts[k];
+            if (sg <= 0) bad_out[agg_inc(&cnt[slot_out])] = ts[k];
         }
     }
     flush_stats(loc, stats, kStOrient0);
@@ -305,11 +383,11 @@ __global__ void __launch_bounds__(kThreads) k_filter_bad(DMesh m, const int* __r
         tri_verts(m, t, a, b, c);
         if (orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st) > 0) { loc[st]++; continue; }
         loc[st]++;
-        bad_out[atomicAdd(&cnt[slot_out], 1)] = t;
+        bad_out[agg_inc(&cnt[slot_out])] = t;
         const int h0 = m.t_he[t], h1 = m.he_next[h0], h2 = m.he_next[h1];
         const int es[3] = {h0 >> 1, h1 >> 1, h2 >> 1};
         for (int k = 0; k < 3; k++)
-            if (atomicExch(&e_stamp[es[k]], round) != round) cand[atomicAdd(&cnt[slot_c], 1)] = es[k];
+            if (atomicExch(&e_stamp[es[k]], round) != round) cand[agg_inc(&cnt[slot_c])] = es[k];
     }
     flush_stats(loc, stats, kStOrient0);
 }
@@ -339,8 +417,8 @@ __global__ void __launch_bounds__(kThreads) k_repair_flip(DMesh m, int* __restri
             t_rep_e[tt.x] = e; t_rep_e[tt.y] = e;
             const int qv[4] = {vv.x, vv.y, oo.x, oo.y};
             for (int k = 0; k < 4; k++)
-                if (atomicExch(&v_touch[qv[k]], rep_stamp) != rep_stamp) touched[atomicAdd(&cnt[slot_touched], 1)] = qv[k];
-            atomicAdd(&cnt[kCnErr], 1);
+                if (atomicExch(&v_touch[qv[k]], rep_stamp) != rep_stamp) touched[agg_inc(&cnt[slot_touched])] = qv[k];
+            agg_inc(&cnt[kCnErr]);
             loc[0]++;
         }
     }
@@ -366,7 +444,7 @@ __global__ void __launch_bounds__(kThreads) k_check_ring(DMesh m, const int* __r
                 tri_verts(m, t, a, b, c);
                 int s = orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st);
                 loc[st]++;
-                if (s <= 0) bad[atomicAdd(&cnt_out[slot_out], 1)] = t;
+                if (s <= 0) bad[agg_inc(&cnt_out[slot_out])] = t;
             }
             h = m.he_next[h ^ 1];
         } while (h != h0 && ++guard < 4096);
@@ -448,8 +526,8 @@ __global__ void __launch_bounds__(kThreads) k_enqueue_ring(DMesh m, const int* _
         int h0 = m.v_he[v], h = h0, guard = 0;
         do {
             int e1 = h >> 1, e2 = m.he_next[h] >> 1;
-            if (atomicExch(&e_stamp[e1], estamp) != estamp) cand[atomicAdd(&cnt_c[slot_c], 1)] = e1;
-            if (m.he_t[h] >= 0 && atomicExch(&e_stamp[e2], estamp) != estamp) cand[atomicAdd(&cnt_c[slot_c], 1)] = e2;
+            if (atomicExch(&e_stamp[e1], estamp) != estamp) cand[agg_inc(&cnt_c[slot_c])] = e1;
+            if (m.he_t[h] >= 0 && atomicExch(&e_stamp[e2], estamp) != estamp) cand[agg_inc(&cnt_c[slot_c])] = e2;
             h = m.he_next[h ^ 1];
         } while (h != h0 && ++guard < 4096);
     }
@@ -468,7 +546,7 @@ __global__ void __launch_bounds__(kThreads) k_collect_pending(DMesh m, const int
         if (q.x == t.x && q.y == t.y) continue;
         bool moved = !(q.x == a.x && q.y == a.y);
         if (moved) v_fexp[v] = 0;  // after a successful partial move, try the whole rest
-        pend[atomicAdd(&cnt_p[slot_p], 1)] = v;
+        pend[agg_inc(&cnt_p[slot_p])] = v;
     }
 }
 
@@ -483,35 +561,44 @@ __global__ void __launch_bounds__(kThreads) k_collect_pending(DMesh m, const int
 // so the mesh stays valid; Lawson rounds restore the Delaunay property.
 constexpr int kMaxStar = 48;
 
-// visibility walk from triangle t toward p; returns the containing triangle
-// (and in *on the half-edge of it that p lies on, or -1), -1 if the walk
-// leaves the mesh or runs too long, -2 if p coincides with a vertex
+// Remembering stochastic visibility walk (Devillers, Pion, Teillaud, "Walking
+// in a triangulation", 2002) from triangle t toward p: at each triangle the
+// edges are tried from a random one, the edge just crossed is skipped, and
+// the walk crosses the first edge that has p strictly on its far side.  A
+// deterministic visibility walk can cycle forever in a triangulation that is
+// not Delaunay; the random start makes termination certain with probability
+// one.  Returns the containing triangle (and in *on the half-edge of it that
+// p lies on, or -1), -1 if the walk leaves the mesh or exceeds kWalkCap
+// triangles, -2 if p coincides with a vertex.
+constexpr int kWalkCap = 2048;
 template <bool kF32>
-__device__ int locate_point(const DMesh& m, float2 p, int t, int* on) {
-    for (int step = 0; step < 4096; step++) {
+__device__ int locate_point(const DMesh& m, float2 p, int t, int* on, unsigned seed) {
+    int came = -1;  // half-edge of t we entered through
+    unsigned r = seed * 0x9E3779B9u + 0x7F4A7C15u;
+    for (int step = 0; step < kWalkCap; step++) {
         const int h0 = m.t_he[t];
         const int hs[3] = {h0, m.he_next[h0], m.he_next[m.he_next[h0]]};
-        bool moved = false;
-        int zero = -1;
+        r ^= r << 13; r ^= r >> 17; r ^= r << 5;
+        const int k0 = (int)(r % 3u);
+        int zero = -1, cross = -1;
         for (int k = 0; k < 3; k++) {
-            const int h = hs[(k + step) % 3];
+            const int h = hs[(k0 + k) % 3];
             const float2 a = m.pos[m.he_v[h]], b = m.pos[m.he_v[h ^ 1]];
             if ((a.x == p.x && a.y == p.y) || (b.x == p.x && b.y == p.y)) return -2;
+            if (h == came) continue;  // p is on our side of the edge we crossed
             int st;
             const int o = orient<kF32>(a, b, p, m.periodic, &st);
-            if (o < 0) {
-                const int nt = m.he_t[h ^ 1];
-                if (nt < 0) return -1;
-                t = nt;
-                moved = true;
-                break;
-            }
+            if (o < 0) { cross = h; break; }
             if (o == 0) zero = h;
         }
-        if (!moved) {
+        if (cross < 0) {
             *on = zero;
             return t;
         }
+        const int nt = m.he_t[cross ^ 1];
+        if (nt < 0) return -1;
+        came = cross ^ 1;
+        t = nt;
     }
     return -1;
 }
@@ -521,14 +608,14 @@ template <bool kF32>
 __global__ void __launch_bounds__(kThreads) k_reloc_locate(DMesh m, const int* __restrict__ vlist,
                                                            const int* __restrict__ cnt, int slot,
                                                            const float2* __restrict__ tgt, int* __restrict__ r_tt,
-                                                           int* __restrict__ r_tt2) {
+                                                           int* __restrict__ r_tt2, int round) {
     const int n = cnt[slot];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int v = vlist[i];
         const float2 t = tgt[v], q = m.pos[v];
         if (t.x == q.x && t.y == q.y) { r_tt[v] = -3; continue; }
         int on = -1;
-        const int tt = locate_point<kF32>(m, t, m.he_t[m.v_he[v]], &on);
+        const int tt = locate_point<kF32>(m, t, m.he_t[m.v_he[v]], &on, (unsigned)v ^ ((unsigned)round << 20));
         r_tt[v] = tt;
         r_tt2[v] = (tt >= 0 && on >= 0) ? m.he_t[on ^ 1] : -1;
     }
@@ -611,7 +698,7 @@ __global__ void __launch_bounds__(kThreads) k_reloc_apply(DMesh m, const int* __
             } while (h != h0);
         }
         if (!mine) {  // lost a claim (or star too large): retry next round
-            if (deg < kMaxStar) next_list[atomicAdd(&cnt_next[slot_next], 1)] = v;
+            if (deg < kMaxStar) next_list[agg_inc(&cnt_next[slot_next])] = v;
             else loc[1]++;
             continue;
         }
@@ -747,7 +834,7 @@ __global__ void __launch_bounds__(kThreads) k_round_test(DMesh m, const int* __r
             int s = incircle<kF32, kBig>(m.pos[vv.x], m.pos[oo.y], m.pos[vv.y], m.pos[oo.x], m.periodic, &st);
             if (s == kNeedBig) {
                 if (atomicExch(&e_stamp[e], round + 1) != round + 1)
-                    cand_next[atomicAdd(&cnt[kCnCand + (r3 + 1) % 3], 1)] = e;
+                    cand_next[agg_inc(&cnt[kCnCand + (r3 + 1) % 3])] = e;
                 continue;
             }
             loc[st]++;
@@ -796,14 +883,14 @@ __global__ void __launch_bounds__(kThreads) k_round_test(DMesh m, const int* __r
             unsigned long long key = claim_key(round, e);
             atomicMax(&claim[tt.x], key);
             atomicMax(&claim[tt.y], key);
-            wants[atomicAdd(&cnt[kCnWants + r3], 1)] = e;
+            wants[agg_inc(&cnt[kCnWants + r3])] = e;
         }
     }
     flush_stats(loc, stats, kKind == 0 ? kStIncircle0 : kStOrient0);
 }
 
 __device__ __forceinline__ void enqueue_edge(int e, int stamp, int* e_stamp, int* cand_next, int* cnt, int slot) {
-    if (atomicExch(&e_stamp[e], stamp) != stamp) cand_next[atomicAdd(&cnt[slot], 1)] = e;
+    if (atomicExch(&e_stamp[e], stamp) != stamp) cand_next[agg_inc(&cnt[slot])] = e;
 }
 
 __global__ void __launch_bounds__(kThreads) k_round_flip(DMesh m, int* __restrict__ cnt, int round,
@@ -1033,12 +1120,15 @@ struct mcl_tri {
     int* undo = nullptr;               // ne: repair flips to undo in the current pass
     int* stuck = nullptr;              // nt
     int* touched = nullptr;            // nv: vertices of repair-flipped quads
+    int* v_soft = nullptr;             // nv: step stamp of a vertex's soft take-back
     int* r_tt = nullptr;               // nv: relocation target triangle
     int* r_tt2 = nullptr;              // nv: second triangle when the target is on an edge
     // one ring
     int* deg = nullptr;
     int* scan_tmp = nullptr;
     int* h_cnt = nullptr;  // pinned
+    KTimer timers[3];      // Lawson full round, check all, move
+    int tail_threshold = 8192;  // queue length below which one workgroup finishes the rounds
     int round = 1;         // global round counter (claims, edge stamps)
     int stamp = 1;         // global stamp for vertex / triangle marks
     int grid_blocks = 1024;
@@ -1074,7 +1164,7 @@ static int tri_free(mcl_tri* t) {
     void* ptrs[] = {t->m.pos, t->m.he_next, t->m.he_v, t->m.he_t, t->m.he_op, t->m.v_he, t->m.v_pidx, t->m.t_he,
                     t->claim, t->e_stamp, t->cand[0], t->cand[1], t->wants, t->cnt, t->rounds_done, t->stats,
                     t->scalar, t->acc, t->tgt, t->v_mv, t->v_rev, t->v_fexp, t->t_stamp, t->bad[0], t->bad[1],
-                    t->rev, t->allrev, t->pend[0], t->pend[1], t->deg, t->scan_tmp, t->v_touch, t->t_rep, t->t_rep_e, t->e_undo, t->undo, t->stuck, t->touched, t->r_tt, t->r_tt2};
+                    t->rev, t->allrev, t->pend[0], t->pend[1], t->deg, t->scan_tmp, t->v_touch, t->t_rep, t->t_rep_e, t->e_undo, t->undo, t->stuck, t->touched, t->r_tt, t->r_tt2, t->v_soft};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (t->h_cnt) (void)hipHostFree(t->h_cnt);
@@ -1124,7 +1214,7 @@ extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv,
     A(t->bad[0], nt); A(t->bad[1], nt); A(t->rev, nv); A(t->allrev, nv); A(t->pend[0], nv); A(t->pend[1], nv);
     A(t->deg, nv + 1); A(t->scan_tmp, scan_scratch_ints(nv + 1) + 16);
     A(t->v_touch, nv); A(t->t_rep, nt); A(t->t_rep_e, nt); A(t->e_undo, m.ne); A(t->undo, m.ne);
-    A(t->stuck, nt); A(t->touched, nv); A(t->r_tt, nv); A(t->r_tt2, nv);
+    A(t->stuck, nt); A(t->touched, nv); A(t->r_tt, nv); A(t->r_tt2, nv); A(t->v_soft, nv);
 #undef A
     if (rc == MCL_OK && hipHostMalloc((void**)&t->h_cnt, sizeof(int) * kCnCount) != hipSuccess) rc = MCL_ERR_HIP;
     if (rc == MCL_OK) rc = upload_aos(t, pos, v2, he4, t1);
@@ -1135,6 +1225,7 @@ extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv,
         hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(nv)), dim3(kThreads), 0, 0, t->v_rev, nv, 0);
         hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(nv)), dim3(kThreads), 0, 0, t->v_fexp, nv, 0);
         hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(nv)), dim3(kThreads), 0, 0, t->v_touch, nv, 0);
+        hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(nv)), dim3(kThreads), 0, 0, t->v_soft, nv, 0);
         hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(nt)), dim3(kThreads), 0, 0, t->t_rep, nt, 0);
         hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(m.ne)), dim3(kThreads), 0, 0, t->e_undo, m.ne, 0);
         hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(nt)), dim3(kThreads), 0, 0, t->t_stamp, nt, 0);
@@ -1243,7 +1334,7 @@ __global__ void k_prep_round(int* cnt, int r3, int ncand) {
 template <int kKind>
 static int run_rounds(mcl_tri* t, bool all, int variant, int max_rounds, hipStream_t s, long long* rounds_out) {
     DMesh& m = t->m;
-    const int cap = max_rounds > 0 ? max_rounds : 1 << 20;
+    const int cap = max_rounds > 0 ? max_rounds : 4096;
     int r = 0;
     if (all) {
         hipLaunchKernelGGL(k_prep_round, dim3(1), dim3(64), 0, s, t->cnt, t->round % 3, 0);
@@ -1295,17 +1386,24 @@ static int run_rounds(mcl_tri* t, bool all, int variant, int max_rounds, hipStre
         if (t->h_cnt[kCnCand + (R + tail_rounds) % 3] != 0) return MCL_ERR_ROUND_CAP;
         return MCL_OK;
     }
-    // variant 0: a few full-grid rounds, then the single-workgroup tail
-    const int full_rounds = 3;
+    // variant 0: full-grid rounds while the queue is large, then the
+    // single-workgroup tail finishes the small remainder
     const int gb = t->grid_blocks;
-    for (int k = 0; k < full_rounds && r < cap; k++, r++) {
+    for (int k = 0; r < cap; k++, r++) {
+        if (k > 0) {
+            MCL_TRY(read_counts(t, s));
+            if (t->h_cnt[kCnCand + t->round % 3] < t->tail_threshold) break;
+        }
         const int R = t->round;
         int* cand = t->cand[R & 1];
         int* cand_next = t->cand[(R + 1) & 1];
         const int stat = kKind == 0 ? kStFlips : kStUntangleFlips;
-        if (first)
+        if (first) {
+            t->timers[0].mark(s);
             hipLaunchKernelGGL((k_round_test<kKind, true, true, false>), dim3(gb), dim3(kThreads), 0, s, m, cand, t->cnt, R,
                                m.ne, t->wants, t->claim, t->e_stamp, cand_next, t->stats);
+            t->timers[0].mark(s);
+        }
         else
             hipLaunchKernelGGL((k_round_test<kKind, false, true, false>), dim3(gb), dim3(kThreads), 0, s, m, cand, t->cnt, R, 0,
                                t->wants, t->claim, t->e_stamp, cand_next, t->stats);
@@ -1501,7 +1599,7 @@ static int revert_passes(mcl_tri* t, hipStream_t s, bool f32, int* cur, int mv_s
                          bool* stuck) {
     DMesh& m = t->m;
     *stuck = false;
-    for (int pass = 0; pass < (1 << 20); pass++) {
+    for (int pass = 0; pass < 4096; pass++) {
         int nb = t->h_cnt[kCnBad + *cur];
         if (nb == 0) break;
         const int rev_stamp = ++t->stamp, undo_stamp = rev_stamp;
@@ -1516,9 +1614,16 @@ static int revert_passes(mcl_tri* t, hipStream_t s, bool f32, int* cur, int mv_s
         const int nr = t->h_cnt[kCnRev], nu = t->h_cnt[kCnRev + 1];
         const int nxt = 1 - *cur;
         MCL_TRY(zero_slots(t, s, kCnBad + nxt, 1));
-        if (nr > 0)
-            hipLaunchKernelGGL(k_revert_apply, dim3(blocks_for(nr)), dim3(kThreads), 0, s, m, t->rev, t->cnt, (int)kCnRev,
-                               t->acc, t->v_fexp, t->allrev, t->cnt, (int)kCnRev + 2, t->stats);
+        if (nr > 0) {
+            if (f32)
+                hipLaunchKernelGGL((k_revert_apply<true>), dim3(blocks_for(nr)), dim3(kThreads), 0, s, m, t->rev, t->cnt,
+                                   (int)kCnRev, t->acc, t->v_fexp, t->v_soft, mv_stamp, t->allrev, t->cnt,
+                                   (int)kCnRev + 2, t->stats);
+            else
+                hipLaunchKernelGGL((k_revert_apply<false>), dim3(blocks_for(nr)), dim3(kThreads), 0, s, m, t->rev, t->cnt,
+                                   (int)kCnRev, t->acc, t->v_fexp, t->v_soft, mv_stamp, t->allrev, t->cnt,
+                                   (int)kCnRev + 2, t->stats);
+        }
         if (nu > 0) {
             *undos += nu;
             if (f32)
@@ -1553,10 +1658,10 @@ static int relocate_rounds(mcl_tri* t, hipStream_t s, bool f32, int n, int* slot
         hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->cnt + kCnMove, 1, 0);
         if (f32)
             hipLaunchKernelGGL((k_reloc_locate<true>), dim3(blocks_for(n)), dim3(kThreads), 0, s, m, t->pend[cur], t->cnt,
-                               (int)kCnPend, t->tgt, t->r_tt, t->r_tt2);
+                               (int)kCnPend, t->tgt, t->r_tt, t->r_tt2, R);
         else
             hipLaunchKernelGGL((k_reloc_locate<false>), dim3(blocks_for(n)), dim3(kThreads), 0, s, m, t->pend[cur], t->cnt,
-                               (int)kCnPend, t->tgt, t->r_tt, t->r_tt2);
+                               (int)kCnPend, t->tgt, t->r_tt, t->r_tt2, R);
         hipLaunchKernelGGL(k_reloc_claim, dim3(blocks_for(n)), dim3(kThreads), 0, s, m, t->pend[cur], t->cnt, (int)kCnPend,
                            R, t->r_tt, t->r_tt2, t->claim);
         if (f32)
@@ -1605,13 +1710,17 @@ extern "C" int mcl_tri_step(mcl_tri_t* t, const mcl_step_params_t* p, mcl_stats_
     static const bool trace = getenv("MCL_TRACE") != nullptr;
 
     const int mv_stamp = ++t->stamp;
+    t->timers[2].mark(s);
     hipLaunchKernelGGL(k_move_motion, dim3(blocks_for(m.nv)), dim3(kThreads), 0, s, m, t->acc, t->tgt, t->v_mv,
                        t->v_fexp, mv_stamp, *p);
+    t->timers[2].mark(s);
     MCL_TRY(zero_slots(t, s, kCnBad, 11));  // bad, rev, move, pend, err, misc, touched
+    t->timers[1].mark(s);
     if (f32)
         hipLaunchKernelGGL((k_check_all<true>), dim3(gb), dim3(kThreads), 0, s, m, t->bad[0], t->cnt, (int)kCnBad, t->stats);
     else
         hipLaunchKernelGGL((k_check_all<false>), dim3(gb), dim3(kThreads), 0, s, m, t->bad[0], t->cnt, (int)kCnBad, t->stats);
+    t->timers[1].mark(s);
     MCL_HIP(hipGetLastError());
     MCL_TRY(read_counts(t, s));
     const long long inverted0 = t->h_cnt[kCnBad];
@@ -1659,6 +1768,12 @@ extern "C" int mcl_tri_one_ring_csr(mcl_tri_t* t, void* d_offsets, void* d_nbrs,
     int* off = (int*)d_offsets;
     hipLaunchKernelGGL(k_degree, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, off);
     MCL_TRY(scan_exclusive(off, off, t->m.nv, t->scan_tmp, s));
+    if (!total && d_nbrs && cap >= t->m.nhe) {
+        // every half-edge leaves exactly one vertex: the total is nhe, no read-back needed
+        hipLaunchKernelGGL(k_ring_csr, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, off, (int*)d_nbrs, cap);
+        MCL_HIP(hipGetLastError());
+        return MCL_OK;
+    }
     int tot = 0;
     MCL_HIP(hipMemcpyAsync(&tot, off + t->m.nv, sizeof(int), hipMemcpyDeviceToHost, s));
     MCL_HIP(hipStreamSynchronize(s));
@@ -1678,5 +1793,17 @@ extern "C" int mcl_tri_closest(mcl_tri_t* t, const void* d_offsets, const void* 
                        (const int*)d_nbrs, (int*)d_out);
     MCL_HIP(hipGetLastError());
     MCL_HIP(hipStreamSynchronize(s));
+    return MCL_OK;
+}
+
+extern "C" int mcl_tri_kernel_timer(mcl_tri_t* t, int enable) {
+    if (!t) return MCL_ERR_INPUT;
+    for (auto& k : t->timers) k.reset(enable != 0);
+    return MCL_OK;
+}
+extern "C" int mcl_tri_kernel_timer_read(mcl_tri_t* t, int which, double* total_ms, long long* launches) {
+    if (!t || which < 0 || which > 2) return MCL_ERR_INPUT;
+    double ms = t->timers[which].total_ms(launches);
+    if (total_ms) *total_ms = ms;
     return MCL_OK;
 }

@@ -244,6 +244,21 @@ class DeviceTriangulation:
               "mcl_tri_closest")
         return True
 
+    def one_ring_csr_async(self, offsets, nbrs):
+        """CSR one-ring without a host read-back (nbrs must hold m_he_size ints)"""
+        check(lib().mcl_tri_one_ring_csr(self._h, _dev_ptr(offsets), _dev_ptr(nbrs), int(nbrs.numel()), None, None),
+              "mcl_tri_one_ring_csr")
+
+    def kernel_timer(self, enable=True):
+        check(lib().mcl_tri_kernel_timer(self._h, int(enable)), "mcl_tri_kernel_timer")
+
+    def kernel_timer_read(self, which):
+        """(total GPU ms, launches) of kernel class `which` (0 Lawson full round, 1 check all, 2 move)"""
+        ms, n = ctypes.c_double(), ctypes.c_longlong()
+        check(lib().mcl_tri_kernel_timer_read(self._h, int(which), ctypes.byref(ms), ctypes.byref(n)),
+              "mcl_tri_kernel_timer_read")
+        return ms.value, n.value
+
 
 def _tensor_from_ptr(ptr, shape, dtype):
     """wrap a device allocation owned by the library as a torch tensor"""
@@ -326,6 +341,29 @@ class GridCount2d:
         j = torch.cat([j, torch.arange(n, device="cuda")])
         d = pos[j] - pos[i]
         f(i, j, d, torch.sqrt((d.double() ** 2).sum(1)))
+
+    def build(self, src_pos):
+        """counting sort of src_pos into the grid's own sorted copy (src untouched, async)"""
+        check(lib().mcl_grid_build(self._g, _dev_ptr(src_pos), None), "mcl_grid_build")
+
+    def sorted_ptr(self):
+        return lib().mcl_grid_sorted_ptr(self._g)
+
+    def frnn_csr_async(self, pos_ptr, rad, offsets, nbrs):
+        """CSR FRNN over sorted slots without a host read-back"""
+        check(lib().mcl_grid_frnn_csr(self._g, _vp(pos_ptr) if isinstance(pos_ptr, int) else _dev_ptr(pos_ptr),
+                                      float(rad), _dev_ptr(offsets), _dev_ptr(nbrs), int(nbrs.numel()), None, None),
+              "mcl_grid_frnn_csr")
+
+    def kernel_timer(self, enable=True):
+        check(lib().mcl_grid_kernel_timer(self._g, int(enable)), "mcl_grid_kernel_timer")
+
+    def kernel_timer_read(self, which):
+        """(total GPU ms, launches) of kernel class `which` (0 FRNN count, 1 FRNN fill, 2 build)"""
+        ms, n = ctypes.c_double(), ctypes.c_longlong()
+        check(lib().mcl_grid_kernel_timer_read(self._g, int(which), ctypes.byref(ms), ctypes.byref(n)),
+              "mcl_grid_kernel_timer_read")
+        return ms.value, n.value
 
     def mean_num_particle_in_cell(self):
         out = ctypes.c_float()

@@ -79,6 +79,8 @@ int orc_tri_delonize_worklist(orc_tri* t, long long* flips);
 /* Host_Triangulation.h:417-471: one pass of B/C steps; returns #flips */
 int orc_tri_untangle(orc_tri* t);
 int orc_tri_move_point(orc_tri* t, int i, float dx, float dy);
+/* the mesh lives on the unit torus: predicates use minimum-image differences */
+void orc_tri_set_periodic(orc_tri* t, int periodic);
 /* counts per Host diagnostics (moving_device_mesh.cu:212-242) with the
  * current predicate mode: [non-Delaunay edges, creased, inverted edges,
  * non-CCW triangles] */

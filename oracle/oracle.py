@@ -48,6 +48,7 @@ def lib():
             "orc_tri_delonize_worklist": (_i, [_p, _p]),
             "orc_tri_untangle": (_i, [_p]),
             "orc_tri_move_point": (_i, [_p, _i, _f, _f]),
+            "orc_tri_set_periodic": (None, [_p, _i]),
             "orc_tri_diagnostics": (None, [_p, _p]),
             "orc_tri_one_ring": (_i, [_p, _i, _p, _i]),
             "orc_tri_get_frnn": (_i, [_p, _i, _f, _p, _i]),
@@ -151,6 +152,9 @@ class HostTri:
         t = np.ascontiguousarray(t, np.int32).reshape(-1)
         lib().orc_tri_import(self.h, _ptr(pos), len(pos), _ptr(v), len(v), _ptr(he), len(he),
                              _ptr(t), len(t))
+
+    def set_periodic(self, periodic=True):
+        lib().orc_tri_set_periodic(self.h, int(periodic))
 
     def set_positions(self, pos):
         pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 2)

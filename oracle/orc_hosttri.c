@@ -26,6 +26,7 @@ struct orc_tri {
     orc_halfedge* he; int nhe, che;
     orc_triangle* t; int nt, ct;
     int mode;
+    int periodic;
 };
 
 #define GROW(arr, n, cap, type)                                             \
@@ -55,10 +56,19 @@ static orc_pt P(const orc_tri* T, int v) {
     orc_pt p = {T->pos[2 * v], T->pos[2 * v + 1], 0, 0};
     return p;
 }
+static void dv(const orc_tri* T, int v, double* out) {
+    out[0] = T->pos[2 * v];
+    out[1] = T->pos[2 * v + 1];
+}
 static int orient_v(const orc_tri* T, int a, int b, int c) {
     if (T->mode == ORC_PRED_REFERENCE)
         return orc_orient2d_ref(T->pos[2 * a], T->pos[2 * a + 1], T->pos[2 * b], T->pos[2 * b + 1],
                                 T->pos[2 * c], T->pos[2 * c + 1]);
+    if (T->periodic) {  /* unit torus, minimum-image differences */
+        double pa[2], pb[2], pc[2];
+        dv(T, a, pa); dv(T, b, pb); dv(T, c, pc);
+        return orc_orient2d_exact(pa, pb, pc, 1);
+    }
     orc_pt pa = P(T, a), pb = P(T, b), pc = P(T, c);
     return orc_orient_pt(&pa, &pb, &pc);
 }
@@ -73,9 +83,17 @@ static int incircle_v(const orc_tri* T, int a, int b, int c, int d) {
     if (T->mode == ORC_PRED_REFERENCE)
         return orc_incircle_ref(T->pos[2 * a], T->pos[2 * a + 1], T->pos[2 * b], T->pos[2 * b + 1],
                                 T->pos[2 * c], T->pos[2 * c + 1], T->pos[2 * d], T->pos[2 * d + 1]);
+    if (T->periodic) {
+        double pa[2], pb[2], pc[2], pd[2];
+        dv(T, a, pa); dv(T, b, pb); dv(T, c, pc); dv(T, d, pd);
+        return orc_incircle_sos(pa, pb, pc, pd, 1);
+    }
     orc_pt pa = P(T, a), pb = P(T, b), pc = P(T, c), pd = P(T, d);
     return orc_incircle_sos_pt(&pa, &pb, &pc, &pd);
 }
+
+/* the imported mesh lives on the unit torus (no corners; minimum image) */
+void orc_tri_set_periodic(orc_tri* T, int periodic) { T->periodic = periodic ? 1 : 0; }
 
 /* ----------------------------------------------- Common_Triangulation.h */
 /* f2to2 -- Common_Triangulation.h:69-127 */
@@ -537,6 +555,12 @@ void orc_tri_diagnostics(const orc_tri* T, long long* out) {
     }
     for (int t = 0; t < T->nt; t++) {
         int vs[3]; tri_verts(T, t, vs);
+        if (T->periodic) {
+            double a[2], b[2], c[2];
+            dv(T, vs[0], a); dv(T, vs[1], b); dv(T, vs[2], c);
+            if (orc_orient2d_exact(a, b, c, 1) <= 0) nccw++;
+            continue;
+        }
         orc_pt a = P(T, vs[0]), b = P(T, vs[1]), c = P(T, vs[2]);
         if (orc_orient_pt(&a, &b, &c) <= 0) nccw++;
     }
