@@ -109,6 +109,7 @@ SIGNATURES = {
     "mcl_tri_kernel_timer_read": [_p, _i, _p, _p],
     "mcl_grid_kernel_timer": [_p, _i],
     "mcl_grid_kernel_timer_read": [_p, _i, _p, _p],
+    "mcl_eval_predicates": [_i, _i, _p, _i, _p, _p, _p],
     "mcl_version": [],
     "mcl_device_info": [_p, _i, _p],
 }

@@ -1,0 +1,147 @@
+"""GPU grid FRNN (GridCount2d) against the oracle.
+
+- The reference's lattice tests (tests/grid2d_frnn.cu:62-203, 280-340) run
+  through update() + the FRNN query: counts equal the closed forms.
+- Its random test (mt19937(10), uniform(-50, 50), 10,000 points): neighbour
+  SETS keyed by original id equal the oracle's grid search and brute force
+  (the reference checks counts to +-1; the FP32 predicate here is the same
+  on both sides, so the sets are exact).
+- Config-1 shape (N = 10,000 in [0,1), r = 0.02) with row-major and Morton
+  keys, and the torus grid (periodic cells, minimum-image distances).
+- update() permutes the caller's positions (and velocities) by cell, stable
+  within a cell; d_ind maps old index -> new slot.
+- mean_num_particle_in_cell.
+"""
+import numpy as np
+import pytest
+
+import mcleap_amd as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _frnn_sets(g, pos_dev, rad):
+    """neighbour lists per sorted slot, mapped to original ids"""
+    import torch
+    n = g.numP
+    off = torch.zeros(n + 1, dtype=torch.int32, device="cuda")
+    tot = g.frnn_csr(pos_dev, rad, off)
+    nbr = torch.zeros(max(tot, 1), dtype=torch.int32, device="cuda")
+    assert g.frnn_csr(pos_dev, rad, off, nbr) == tot
+    off, nbr = off.cpu().numpy(), nbr.cpu().numpy()[:tot]
+    perm = g.perm().cpu().numpy()
+    out = {}
+    for s in range(n):
+        out[int(perm[s])] = sorted(int(perm[j]) for j in nbr[off[s]:off[s + 1]])
+    return out
+
+
+def _oracle_sets(counts, nbr):
+    return {i: sorted(nbr[i, :counts[i]].tolist()) for i in range(len(counts))}
+
+
+def _run_grid(xy, minxy, cs, nc, rad, flags):
+    import torch
+    g = M.GridCount2d(len(xy), minxy, cs, nc, flags)
+    pos = torch.from_numpy(xy.copy()).cuda()
+    g.update(pos)
+    return g, pos
+
+
+@pytest.mark.parametrize("side,step,rad,want_max", [(10, 1.0, 1.0, 4), (20, 0.5, 1.0, 12), (40, 0.25, 0.25, 4)])
+def test_reference_lattices(oracle, side, step, rad, want_max):
+    gi = np.stack(np.meshgrid(np.arange(side), np.arange(side), indexing="ij"), -1).reshape(-1, 2)
+    xy = (gi * step).astype(np.float32)
+    g, pos = _run_grid(xy, (0, 0), (1, 1), (10, 10), rad, M.GRID_ROWMAJOR)
+    sets = _frnn_sets(g, pos, rad)
+    counts, nbr = oracle.grid_frnn(xy, (0, 0), (1, 1), (10, 10), rad)
+    assert max(len(v) for v in sets.values()) == want_max
+    assert sets == _oracle_sets(counts, nbr)
+
+
+def test_reference_random_seed10(oracle):
+    O = oracle
+    xy = O.mt_uniform_doubles(10, -50.0, 50.0, 20000).reshape(-1, 2).astype(np.float32)
+    g, pos = _run_grid(xy, (-50, -50), (1, 1), (100, 100), 1.0, M.GRID_ROWMAJOR)
+    sets = _frnn_sets(g, pos, 1.0)
+    counts, nbr = O.grid_frnn(xy, (-50, -50), (1, 1), (100, 100), 1.0, cap=64)
+    assert sets == _oracle_sets(counts, nbr)
+    bc, _ = O.brute_frnn(xy, 1.0, cap=64)
+    assert np.array_equal(counts, bc)
+
+
+@pytest.mark.parametrize("flags", [M.GRID_ROWMAJOR, M.GRID_MORTON])
+def test_config1_box(oracle, flags):
+    rng = np.random.default_rng(1)
+    xy = rng.random((10000, 2)).astype(np.float32)
+    g, pos = _run_grid(xy, (0, 0), (0.02, 0.02), (50, 50), 0.02, flags)
+    sets = _frnn_sets(g, pos, 0.02)
+    counts, nbr = oracle.grid_frnn(xy, (0, 0), (0.02, 0.02), (50, 50), 0.02, cap=128)
+    assert sets == _oracle_sets(counts, nbr)
+    assert abs(np.mean(counts) - np.pi * 0.02 ** 2 * 10000) < 1.0
+
+
+def test_torus_grid(oracle):
+    rng = np.random.default_rng(2)
+    n = 1 << 14
+    xy = rng.random((n, 2)).astype(np.float32)
+    r = 2.0 / np.sqrt(n)
+    nc = int(np.floor(1.0 / r))
+    cs = 1.0 / nc
+    g, pos = _run_grid(xy, (0, 0), (cs, cs), (nc, nc), r, M.GRID_MORTON | M.GRID_PERIODIC)
+    sets = _frnn_sets(g, pos, r)
+    counts, nbr = oracle.grid_frnn(xy, (0, 0), (cs, cs), (nc, nc), r, periodic=True, cap=128)
+    assert sets == _oracle_sets(counts, nbr)
+    bc, _ = oracle.brute_frnn(xy, r, periodic=True, cap=128)
+    assert np.array_equal(counts, bc)
+
+
+def test_update_permutes_in_place(oracle):
+    import torch
+    rng = np.random.default_rng(3)
+    xy = rng.random((5000, 2)).astype(np.float32)
+    vel = rng.standard_normal((5000, 2)).astype(np.float32)
+    g = M.GridCount2d(len(xy), (0, 0), (0.05, 0.05), (20, 20), M.GRID_ROWMAJOR)
+    pos = torch.from_numpy(xy.copy()).cuda()
+    v = torch.from_numpy(vel.copy()).cuda()
+    g.update(pos, v)
+    h, perm_o, start = oracle.grid_sort(xy, (0, 0), (0.05, 0.05), (20, 20))
+    perm = g.perm().cpu().numpy()
+    assert np.array_equal(perm, perm_o)  # stable counting sort: same order as the oracle
+    assert np.array_equal(pos.cpu().numpy(), xy[perm])
+    assert np.array_equal(v.cpu().numpy(), vel[perm])
+    ind = g.d_ind().cpu().numpy()
+    assert np.array_equal(ind[perm], np.arange(len(xy)))
+    occ = g.mean_num_particle_in_cell()
+    nz = np.diff(start)
+    # gridCount2d.h:313-334: particles / non-empty cells
+    assert occ == np.float32(len(xy)) / np.float32((nz > 0).sum())
+
+
+def test_build_leaves_source_untouched(oracle):
+    import torch
+    rng = np.random.default_rng(4)
+    xy = rng.random((8000, 2)).astype(np.float32)
+    g = M.GridCount2d(len(xy), (0, 0), (0.025, 0.025), (40, 40), M.GRID_MORTON)
+    src = torch.from_numpy(xy.copy()).cuda()
+    g.build(src)
+    torch.cuda.synchronize()
+    assert np.array_equal(src.cpu().numpy(), xy)
+    sets = _frnn_sets(g, M.triangulation._tensor_from_ptr(g.sorted_ptr(), (len(xy), 2), torch.float32), 0.025)
+    counts, nbr = oracle.grid_frnn(xy, (0, 0), (0.025, 0.025), (40, 40), 0.025, cap=128)
+    assert sets == _oracle_sets(counts, nbr)
+
+
+def test_apply_f_frnn_includes_self_pairs(oracle):
+    import torch
+    rng = np.random.default_rng(5)
+    xy = rng.random((2000, 2)).astype(np.float32)
+    g, pos = _run_grid(xy, (0, 0), (0.05, 0.05), (20, 20), 0.05, M.GRID_ROWMAJOR)
+    seen = {}
+
+    def f(i, j, d, dist):
+        seen["i"], seen["j"], seen["dist"] = i.cpu().numpy(), j.cpu().numpy(), dist.cpu().numpy()
+
+    g.apply_f_frnn(f, pos, 0.05)
+    assert np.sum(seen["i"] == seen["j"]) == len(xy)
+    assert np.all(seen["dist"] <= 0.05 + 1e-7)

@@ -108,18 +108,21 @@ def test_delonize_from_perturbed(oracle):
     xy = uniform_points(5000, 4)
     H = M.HostTriangulation()
     H.addDelaunayPoints(xy)
+    import torch
     for variant in (0, 1):
-        dt = M.DeviceTriangulation(H)
-        import torch
-        d = torch.zeros((dt.m_pos_size, 2), dtype=torch.float32, device="cuda")
-        g = torch.Generator(device="cuda").manual_seed(5)
-        d[4:] = 1e-4 * torch.randn((dt.m_pos_size - 4, 2), generator=g, device="cuda")
-        dt.movePoints(d)
-        torch.cuda.synchronize()
-        if dt.count_inverted() != 0:
-            pytest.skip("perturbation inverted a triangle")
+        for sigma in (1e-4, 3e-5, 1e-5, 3e-6):
+            dt = M.DeviceTriangulation(H)
+            d = torch.zeros((dt.m_pos_size, 2), dtype=torch.float32, device="cuda")
+            g = torch.Generator(device="cuda").manual_seed(5)
+            d[4:] = sigma * torch.randn((dt.m_pos_size - 4, 2), generator=g, device="cuda")
+            dt.movePoints(d)
+            torch.cuda.synchronize()
+            if dt.count_inverted() == 0:
+                break
+        assert dt.count_inverted() == 0
         nd = dt.count_non_delaunay()
+        assert nd > 0
         (dt.delonize2 if variant == 0 else dt.delonize)()
-        print(variant, nd, dt.last_stats)
+        print(variant, sigma, nd, dt.last_stats)
         assert dt.last_stats["flips"] > 0
         _check_box(O, dt, "delonize variant %d" % variant)

@@ -1,0 +1,116 @@
+"""CPU tests of the product's host side and of the C-ABI library surface.
+
+- libmcleap_hip.so loads without a GPU and exports every function that
+  include/mcleap_hip.h declares (no compute calls are made here).
+- The host builder behind HostTriangulation.addDelaunayPoints (C++ in the
+  library, Host_Triangulation.h:296-300 semantics, exact predicates) gives
+  the same canonical triangle set as the oracle's restated incremental
+  Lawson build, in the box with the four +-1e7 corners and on the torus.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import mcleap_amd as M
+from tests.conftest import uniform_points
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    return sorted(set(re.findall(r"\b(mcl_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(M._lib.LIB_PATH)
+    names = _declared("mcleap_hip.h") + _declared("mcleap_hip_diag.h")
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the ctypes table binds exactly what the header declares
+    assert set(M._lib.SIGNATURES) <= set(names)
+
+
+def test_version_string():
+    v = M.lib().mcl_version()
+    assert v and b"gfx950" in v
+
+
+def test_corner_layout_matches_reference():
+    """Host_Triangulation.h:107-137: 4 corners, 2 triangles, 10 half-edges"""
+    H = M.HostTriangulation()
+    assert H.m_pos.shape == (4, 2) and H.m_he.shape == (10, 4) and H.m_t.shape == (2,)
+    assert set(map(tuple, H.m_pos.tolist())) == {(-1e7, -1e7), (1e7, -1e7), (1e7, 1e7), (-1e7, 1e7)}
+    # outer face: 4 half-edges with t = op = -1
+    outer = H.m_he[H.m_he[:, 2] < 0]
+    assert len(outer) == 4 and np.all(outer[:, 3] == -1)
+
+
+@pytest.mark.parametrize("n,seed", [(50, 1), (2000, 2), (20000, 3)])
+def test_host_builder_box_equals_oracle(oracle, n, seed):
+    O = oracle
+    xy = uniform_points(n, seed)
+    H = M.HostTriangulation()
+    H.addDelaunayPoints(xy)
+    assert H.m_pos.shape[0] == len(xy) + 4
+    # the caller's array was reordered in place to the vertex order
+    assert np.array_equal(H.m_pos[4:], xy)
+    rc, msg = O.check_mesh(H.m_pos, H.m_v, H.m_he, H.m_t, periodic=False, require_ccw=True)
+    assert rc == 0, msg
+    assert H.m_he.shape[0] == 10 + 6 * len(xy) and H.m_t.shape[0] == 2 + 2 * len(xy)
+    assert O.count_non_delaunay(H.m_pos, H.m_he) == 0
+    ref = O.delaunay(H.m_pos[4:])
+    _, _, h2, t2 = ref.export()
+    assert np.array_equal(O.canonical_triangles(H.m_he, H.m_t), O.canonical_triangles(h2, t2))
+
+
+@pytest.mark.parametrize("n,seed", [(300, 4), (5000, 5)])
+def test_host_builder_torus_equals_oracle(oracle, n, seed):
+    O = oracle
+    xy = uniform_points(n, seed)
+    H = M.HostTriangulation(periodic=True)
+    H.addDelaunayPoints(xy)
+    assert H.m_t.shape[0] == 2 * len(xy) and H.m_he.shape[0] == 6 * len(xy)
+    rc, msg = O.check_mesh(H.m_pos, H.m_v, H.m_he, H.m_t, periodic=True, require_ccw=True)
+    assert rc == 0, msg
+    assert O.count_non_delaunay(H.m_pos, H.m_he, periodic=True) == 0
+    tri = O.torus_delaunay(H.m_pos)
+    assert np.array_equal(O.canonical_triangles(H.m_he, H.m_t), O.canonical_from_list(tri))
+
+
+def test_host_builder_rejects_duplicates():
+    xy = uniform_points(100, 6)
+    xy = np.concatenate([xy, xy[:1]])
+    H = M.HostTriangulation()
+    with pytest.raises(M.MclError):
+        H.addDelaunayPoints(xy)
+
+
+def test_cocircular_lattice_is_unique_under_the_tie_rule(oracle):
+    """an exact integer lattice is fully degenerate (every square cocircular);
+    the symbolic perturbation still defines one triangulation, and both
+    builders (different insertion orders) must produce it"""
+    O = oracle
+    g = np.stack(np.meshgrid(np.arange(12), np.arange(12), indexing="ij"), -1).reshape(-1, 2) / 16.0
+    xy = g.astype(np.float32)
+    H = M.HostTriangulation()
+    H.addDelaunayPoints(xy)
+    ref = O.delaunay(H.m_pos[4:])
+    _, _, h2, t2 = ref.export()
+    assert np.array_equal(O.canonical_triangles(H.m_he, H.m_t), O.canonical_triangles(h2, t2))
+    # a shuffled insertion order gives the same set
+    perm = np.random.default_rng(0).permutation(len(xy))
+    ref2 = O.delaunay(H.m_pos[4:][perm])
+    _, _, h3, t3 = ref2.export()
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(perm))
+    c3 = O.canonical_triangles(h3, t3).astype(np.int64)
+    remap = np.concatenate([np.arange(4), 4 + perm])  # oracle vertex 4+k holds point perm[k]
+    c3 = O.canonical_from_list(remap[c3])
+    assert np.array_equal(O.canonical_triangles(H.m_he, H.m_t), c3)
