@@ -1,0 +1,138 @@
+// Device_Triangulation.h -- drop-in DeviceTriangulation (reference
+// include/Device_Triangulation.h:128-223) over libmcleap_hip.so.
+//
+// Same constructor (borrows a HostTriangulation, allocates and uploads),
+// public m_pos (device vec2*, the caller may read it in its own kernels and
+// copy from it) and sizes, and the same member functions returning bool.
+// The mesh lives on the device as SoA arrays owned by the library: the
+// reference's device m_v / m_he / m_t AoS pointers are not exposed (no
+// caller of the reference touches them); transferToHost fills the host
+// object's AoS vectors.  Every call is synchronous, like the reference.
+#pragma once
+
+#include "Host_Triangulation.h"
+
+struct DeviceTriangulation {
+    using vec2 = mcleap::vec2;
+    HostTriangulation* h_triangulation;
+    int m_pos_size = 0, m_v_size = 0, m_he_size = 0, m_t_size = 0;
+    vec2* m_pos = nullptr;  // device positions (float2), owned by the library
+    int blocksize = 256;
+    mcl_stats_t last_stats{};  // statistics of the last delonize/untangle/step
+
+    explicit DeviceTriangulation(HostTriangulation* h) : h_triangulation(h) {
+        mcleap::must(mcl_tri_create(reinterpret_cast<const float*>(h->m_pos.data()), (int)h->m_pos.size(),
+                                    reinterpret_cast<const int*>(h->m_v.data()), (int)h->m_v.size(),
+                                    reinterpret_cast<const int*>(h->m_he.data()), (int)h->m_he.size(),
+                                    reinterpret_cast<const int*>(h->m_t.data()), (int)h->m_t.size(), h->periodic, &t_),
+                     "DeviceTriangulation");
+        mcl_tri_sizes(t_, &m_pos_size, &m_v_size, &m_he_size, &m_t_size);
+        m_pos = static_cast<vec2*>(mcl_tri_pos_ptr(t_));
+    }
+    ~DeviceTriangulation() { mcl_tri_destroy(t_); }
+    DeviceTriangulation(const DeviceTriangulation&) = delete;
+    DeviceTriangulation& operator=(const DeviceTriangulation&) = delete;
+
+    // -------------------------------------------
+    // transfers (Device_Triangulation.h:245-263)
+    bool transferToHost() {
+        HostTriangulation* h = h_triangulation;
+        return mcleap::ok(mcl_tri_download(t_, reinterpret_cast<float*>(h->m_pos.data()),
+                                           reinterpret_cast<int*>(h->m_v.data()), reinterpret_cast<int*>(h->m_he.data()),
+                                           reinterpret_cast<int*>(h->m_t.data())),
+                          "transferToHost");
+    }
+    bool transferToDevice() {
+        HostTriangulation* h = h_triangulation;
+        return mcleap::ok(mcl_tri_upload(t_, reinterpret_cast<const float*>(h->m_pos.data()),
+                                         reinterpret_cast<const int*>(h->m_v.data()),
+                                         reinterpret_cast<const int*>(h->m_he.data()),
+                                         reinterpret_cast<const int*>(h->m_t.data())),
+                          "transferToDevice");
+    }
+    bool allocateMemory() { return true; }  // done by the constructor
+    bool freeMemory() { return true; }      // done by the destructor
+
+    // -------------------------------------------
+    // delonizing (Device_Triangulation.h:324-390)
+    bool delonize() { return mcleap::ok(mcl_tri_delonize(t_, 1, 0, &last_stats, nullptr), "delonize"); }
+    bool delonize2() { return mcleap::ok(mcl_tri_delonize(t_, 0, 0, &last_stats, nullptr), "delonize2"); }
+
+    // -------------------------------------------
+    // moving points (Device_Triangulation.h:395-410); d, dt, v are device arrays
+    bool movePoints(vec2* d) { return mcleap::ok(mcl_tri_move(t_, d, nullptr), "movePoints"); }
+    bool integrateMovePoints(float* dt, vec2* v) {
+        return mcleap::ok(mcl_tri_integrate(t_, dt, v, nullptr), "integrateMovePoints");
+    }
+    // declared and left unimplemented in the reference
+    bool moveFlipflop(int, vec2) { return false; }
+    bool moveFlipflopDelaunay(int, vec2) { return false; }
+    bool moveUntangling(int, vec2) { return false; }
+    bool moveUntanglingDelaunay(int, vec2) { return false; }
+
+    // -------------------------------------------
+    // untangling (Device_Triangulation.h:422-495): false when inverted triangles remain
+    bool untangle() { return mcl_tri_untangle(t_, 0, &last_stats, nullptr) == MCL_OK; }
+    bool untangle2() { return untangle(); }
+
+    // movePoints(d) + untangle + delonize in one call that always ends on a
+    // valid, exactly Delaunay mesh (d: device displacement per vertex)
+    bool moveAndLegalize(vec2* d, int variant = 0) {
+        return mcleap::ok(mcl_tri_move_legalize(t_, d, variant, &last_stats, nullptr), "moveAndLegalize");
+    }
+    // one kinetic step of a synthetic workload: move, keep the mesh valid,
+    // flip back to the exact Delaunay triangulation (the library's driver)
+    bool step(const mcl_step_params_t& p) { return mcleap::ok(mcl_tri_step(t_, &p, &last_stats, nullptr), "step"); }
+
+    // -------------------------------------------
+    // neighbourhood searching (Device_Triangulation.h:499-557); all buffers
+    // are caller-allocated device memory in the reference layouts
+    template <int maxRingSize>
+    bool oneRing(int* ring_neighbors) {
+        return mcleap::ok(mcl_tri_one_ring(t_, maxRingSize, 0, ring_neighbors, nullptr), "oneRing");
+    }
+    template <int maxRingSize>
+    bool oneRing2(int* ring_neighbors) {
+        return mcleap::ok(mcl_tri_one_ring(t_, maxRingSize, 1, ring_neighbors, nullptr), "oneRing2");
+    }
+    template <int maxRingSize>
+    bool closestNeighbors(int* ring_neighbors, int* closest_neighbors) {
+        return mcleap::ok(mcl_tri_closest_ring(t_, maxRingSize, 0, ring_neighbors, closest_neighbors, nullptr),
+                          "closestNeighbors");
+    }
+    template <int maxRingSize>
+    bool closestNeighbors2(int* ring_neighbors, int* closest_neighbors) {
+        return mcleap::ok(mcl_tri_closest_ring(t_, maxRingSize, 1, ring_neighbors, closest_neighbors, nullptr),
+                          "closestNeighbors2");
+    }
+    template <int maxRingSize, int maxFRNNSize>
+    bool getFRNN(float r, int* ring_neighbors, int* neighbors) {
+        int ov = 0;
+        return mcleap::ok(mcl_tri_frnn_graph(t_, r, maxRingSize, 0, ring_neighbors, maxFRNNSize, neighbors, &ov, nullptr),
+                          "getFRNN");
+    }
+    template <int maxRingSize, int maxFRNNSize>
+    bool getFRNN2(float r, int* ring_neighbors, int* neighbors) {
+        int ov = 0;
+        return mcleap::ok(mcl_tri_frnn_graph(t_, r, maxRingSize, 1, ring_neighbors, maxFRNNSize, neighbors, &ov, nullptr),
+                          "getFRNN2");
+    }
+
+    bool swapVertices(int, int) { return false; }  // experimental stub in the reference
+
+    // diagnostics used by the reference experiments (moving_device_mesh.cu:212-242)
+    long long countInverted() {
+        long long n = -1;
+        mcl_tri_count_inverted(t_, &n, nullptr);
+        return n;
+    }
+    long long countNonDelaunay() {
+        long long n = -1;
+        mcl_tri_count_non_delaunay(t_, &n, nullptr);
+        return n;
+    }
+    mcl_tri_t* handle() { return t_; }
+
+  private:
+    mcl_tri_t* t_ = nullptr;
+};

@@ -71,6 +71,30 @@ int mcl_hmesh_sizes(const mcl_hmesh_t* m, int* npos, int* nv, int* nhe, int* nt)
 int mcl_hmesh_export(const mcl_hmesh_t* m, float* pos, int* v2, int* he4, int* t1, int* perm);
 int mcl_hmesh_free(mcl_hmesh_t* m);
 
+/* Host-side operations on the caller's AoS arrays (pos: float pairs indexed
+ * by Vertex.pos_index; he4/t1/v2 as above), exact predicates, in place. */
+/* orient2d / inCircle (Helpers_Triangulation.h:86-151) with exact signs;
+ * incircle ties broken by the symbolic perturbation (never 0 unless the four
+ * points are collinear).  periodic != 0: minimum-image differences. */
+int mcl_orient2d(const float* a, const float* b, const float* c, int periodic);
+int mcl_incircle(const float* a, const float* b, const float* c, const float* d, int periodic);
+/* f2to2 of the edge of he_index (Common_Triangulation.h:69-127, the flip of
+ * delonizeEdge/untangleEdge); check != 0 refuses (MCL_ERR_TOPOLOGY) a flip
+ * that would leave a triangle not strictly CCW */
+int mcl_hmesh_flip(float* pos, int* he4, int nhe, int* t1, int nt, int* v2, int nv, int periodic,
+                   int he_index, int check);
+/* delonize / delonizeVertex (Host_Triangulation.h:305-319, 323-346,
+ * 368-395): Lawson flips from a work stack of edges (all edges, or the edges
+ * of the given half-edges `seeds`) until every interior edge passes the exact
+ * incircle test.  A reflex quad (possible only on a tangled mesh) is left
+ * unflipped and reported as MCL_ERR_UNRESOLVED. */
+int mcl_hmesh_delonize(float* pos, int* he4, int nhe, int* t1, int nt, int* v2, int nv, int periodic,
+                       const int* seeds, int nseeds, long long* flips);
+/* untangle (Host_Triangulation.h:417-471): one pass of the B/C steps over all
+ * edges with exact orientation; *flips = flips made */
+int mcl_hmesh_untangle(float* pos, int* he4, int nhe, int* t1, int nt, int* v2, int nv, int periodic,
+                       long long* flips);
+
 /* =========================================================================
  * Device triangulation  (DeviceTriangulation, Device_Triangulation.h:128-223)
  * ========================================================================= */
@@ -133,11 +157,20 @@ typedef struct {
     int max_iterations;        /* move/legalise iterations before stalling vertices (<=0: 64) */
 } mcl_step_params_t;
 
-/* One kinetic step: displace every non-corner vertex, keep the mesh valid
- * (moves that would invert a triangle are taken back and retried in halves),
- * and flip back to the exact Delaunay triangulation.  Replaces the
- * move + untangle2 + delonize2 sequence of moving_device_mesh.cu:119-155. */
+/* One kinetic step: displace every non-corner vertex by the synthetic motion,
+ * keep the mesh valid (repair flips; moves that would still invert a triangle
+ * are taken back, and those vertices are removed and re-inserted at their
+ * targets), and flip back to the exact Delaunay triangulation.  Replaces the
+ * move + untangle2 + delonize2 sequence of moving_device_mesh.cu:119-155.
+ * st->stalled counts vertices left short of their target (a target on an
+ * existing vertex, or a degenerate star). */
 int mcl_tri_step(mcl_tri_t* t, const mcl_step_params_t* p, mcl_stats_t* st, void* stream);
+/* movePoints(d) followed by untangle + delonize (moving_device_mesh.cu:119-155)
+ * as one call that always ends on a valid mesh that is exactly Delaunay:
+ * the kinetic step of mcl_tri_step with the caller's device displacement
+ * array d (float2 per vertex, corners ignored; on the torus the targets
+ * wrap into [0,1)).  Box targets must stay inside the corner square. */
+int mcl_tri_move_legalize(mcl_tri_t* t, const void* d_disp, int variant, mcl_stats_t* st, void* stream);
 
 /* ------------------------------------------------------------- queries */
 /* oneRing<K>(int*) (layout 0, SoA: ring[(c+1)*nv + i], count in ring[i]) and
@@ -154,6 +187,21 @@ int mcl_tri_one_ring_csr(mcl_tri_t* t, void* d_offsets, void* d_nbrs, int cap, i
 /* closestNeighbors<K>(ring, out) (kernels.cuh:129-146) over a CSR one-ring */
 int mcl_tri_closest(mcl_tri_t* t, const void* d_offsets, const void* d_nbrs, void* d_out,
                     void* stream);
+/* closestNeighbors<K> / closestNeighbors2<K> (Device_Triangulation.h:519-526,
+ * 537-544; kernels.cuh:129-163) over the ring written by mcl_tri_one_ring
+ * with the same K and layout (the AoS form reads ring[(c+1) + K*i], the
+ * layout oneRing2 writes). */
+int mcl_tri_closest_ring(mcl_tri_t* t, int K, int layout, const void* d_ring, void* d_out,
+                         void* stream);
+/* getFRNN<K,M>(r, ring, out) / getFRNN2<K,M> (Device_Triangulation.h:528-535,
+ * 546-553; kernels.cuh:167-272): the vertices reachable from each vertex
+ * through the Delaunay graph along vertices within r of it (FP32 squared
+ * distance <= r*r), from the ring of mcl_tri_one_ring (K, layout).  Output
+ * in the same layout with M slots per vertex: the count in slot 0, at most
+ * M-1 ids.  MCL_ERR_CAPACITY (with *overflow = number of vertices) when a
+ * walk needs more than 256 ids or 64 stacked vertices. */
+int mcl_tri_frnn_graph(mcl_tri_t* t, float r, int K, int layout, const void* d_ring, int M, void* d_out,
+                       int* overflow, void* stream);
 
 /* =========================================================================
  * Uniform-grid FRNN  (GridCount2d, gridCount2d.h:23-48)

@@ -110,6 +110,9 @@ SIGNATURES = {
     "mcl_grid_kernel_timer": [_p, _i],
     "mcl_grid_kernel_timer_read": [_p, _i, _p, _p],
     "mcl_eval_predicates": [_i, _i, _p, _i, _p, _p, _p],
+    "mcl_tri_closest_ring": [_p, _i, _i, _p, _p, _p],
+    "mcl_tri_move_legalize": [_p, _p, _i, _p, _p],
+    "mcl_tri_frnn_graph": [_p, _f, _i, _i, _p, _i, _p, ctypes.POINTER(_i), _p],
     "mcl_version": [],
     "mcl_device_info": [_p, _i, _p],
 }

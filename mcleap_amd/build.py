@@ -59,5 +59,19 @@ def build(verbose=False):
     return OUT
 
 
+def build_cpp_example(verbose=False):
+    """the drop-in C++ test program (tests/cpp/dropin_test.cpp) against the
+    headers in include/mcleap and the built library"""
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_test.cpp")
+    exe = os.path.join(ROOT, "tests", "cpp", "dropin_test")
+    cmd = [HIPCC] + FLAGS + ["-x", "hip", src, "-o", exe, "-L" + HERE, "-lmcleap_hip",
+                             "-Wl,-rpath,$ORIGIN/../../mcleap_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr[-8000:]))
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_cpp_example())

@@ -152,18 +152,23 @@ __device__ __forceinline__ void flip_soa(const DMesh& m, int e, int& he2, int& h
 // position (acc), the target (tgt) and moves the vertex to its target
 __global__ void __launch_bounds__(kThreads) k_move_motion(DMesh m, float2* __restrict__ acc, float2* __restrict__ tgt,
                                                           int* __restrict__ v_mv, int* __restrict__ v_fexp, int stamp,
-                                                          mcl_step_params_t p) {
+                                                          mcl_step_params_t p, const float2* __restrict__ disp) {
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
         float2 q = m.pos[v];
         acc[v] = q;
         if (is_corner(q)) { tgt[v] = q; continue; }
-        float2 g = gauss2((uint32_t)v, p.step, p.seed);
         float2 t;
-        if (p.motion == MCL_MOTION_SHEAR) {
+        if (disp) {  // caller displacement (movePoints semantics); the torus wraps
+            const float2 d = disp[v];
+            t = make_float2(q.x + d.x, q.y + d.y);
+            if (m.periodic) t = make_float2(wrap01(t.x), wrap01(t.y));
+        } else if (p.motion == MCL_MOTION_SHEAR) {
+            float2 g = gauss2((uint32_t)v, p.step, p.seed);
             float dx = p.amp * sinpif(2.0f * q.y) + p.sigma * g.x;
             float dy = p.amp * cospif(2.0f * q.x) + p.sigma * g.y;
             t = make_float2(wrap01(q.x + dx), wrap01(q.y + dy));
         } else {
+            float2 g = gauss2((uint32_t)v, p.step, p.seed);
             t = make_float2(clamp_box(q.x + p.sigma * g.x), clamp_box(q.y + p.sigma * g.y));
         }
         tgt[v] = t;
@@ -1086,6 +1091,81 @@ __global__ void __launch_bounds__(kThreads) k_closest(DMesh m, const int* __rest
     }
 }
 
+// ring entry c (0-based) of vertex v in the legacy layouts of k_ring_legacy
+__device__ __forceinline__ int ring_at(const int* ring, int layout, int stride, int nv, int v, int c) {
+    return layout == 0 ? ring[(size_t)(c + 1) * nv + v] : ring[(size_t)(c + 1) + (size_t)stride * v];
+}
+__device__ __forceinline__ int ring_count(const int* ring, int layout, int stride, int v) {
+    return layout == 0 ? ring[v] : ring[(size_t)stride * v];
+}
+
+// closestNeighbors<K> / closestNeighbors2<K> (kernels.cuh:129-163) over the
+// legacy SoA / AoS ring (AoS indexed as oneRing2 writes it)
+__global__ void __launch_bounds__(kThreads) k_closest_legacy(DMesh m, int K, int layout, const int* __restrict__ ring,
+                                                             int* __restrict__ out) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
+        const float2 p = m.pos[v];
+        const int cnt = ring_count(ring, layout, K, v);
+        int best = cnt > 0 ? ring_at(ring, layout, K, m.nv, v, 0) : -1;
+        float bd = 3.0e38f;
+        for (int c = 0; c < cnt; c++) {
+            const int w = ring_at(ring, layout, K, m.nv, v, c);
+            const float2 q = m.pos[w];
+            float dx = p.x - q.x, dy = p.y - q.y;
+            if (m.periodic) { dx = min_image_f(dx); dy = min_image_f(dy); }
+            const float d = __fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy));
+            if (c == 0 || d < bd) { bd = d; best = w; }
+        }
+        out[v] = best;
+    }
+}
+
+// getFRNN<K, M> / getFRNN2<K, M> (kernels.cuh:167-272): vertices reachable
+// from v through the Delaunay graph along vertices within r of v, found by a
+// depth-first walk over the ring; squared FP32 distance <= r*r.  Output in the
+// ring's layout with M slots per vertex: count in slot 0 (the full count),
+// at most M-1 ids kept.  The visited test scans the ids found so far, like
+// the reference; walks past kMaxFound ids stop growing the set (count stays
+// exact up to there) and are reported through *overflow.
+constexpr int kFrnnStack = 64, kMaxFound = 256;
+__global__ void __launch_bounds__(kThreads) k_frnn_graph(DMesh m, float rr, int K, int layout,
+                                                         const int* __restrict__ ring, int M, int* __restrict__ out,
+                                                         int* __restrict__ overflow) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
+        const float2 p = m.pos[v];
+        int stack[kFrnnStack];
+        int found[kMaxFound];
+        int ns = 0, nf = 0;
+        bool over = false;
+        stack[ns++] = v;
+        while (ns > 0) {
+            const int s = stack[--ns];
+            const int cnt = ring_count(ring, layout, K, s);
+            for (int c = 0; c < cnt; c++) {
+                const int w = ring_at(ring, layout, K, m.nv, s, c);
+                if (w == v) continue;
+                bool seen = false;
+                for (int k = 0; k < nf && !seen; k++) seen = found[k] == w;
+                if (seen) continue;
+                const float2 q = m.pos[w];
+                float dx = p.x - q.x, dy = p.y - q.y;
+                if (m.periodic) { dx = min_image_f(dx); dy = min_image_f(dy); }
+                if (__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)) > rr) continue;
+                if (nf == kMaxFound || ns == kFrnnStack) { over = true; continue; }
+                if (nf + 1 < M) {
+                    if (layout == 0) out[(size_t)(nf + 1) * m.nv + v] = w;
+                    else out[(size_t)(nf + 1) + (size_t)M * v] = w;
+                }
+                found[nf++] = w;
+                stack[ns++] = w;
+            }
+        }
+        if (layout == 0) out[v] = nf;
+        else out[(size_t)M * v] = nf;
+        if (over) atomicAdd(overflow, 1);
+    }
+}
+
 }  // namespace mcl
 
 // =========================================================================
@@ -1698,8 +1778,7 @@ static int relocate_rounds(mcl_tri* t, hipStream_t s, bool f32, int n, int* slot
 //   relocate  -- vertices sent back are removed from their stars and
 //                re-inserted at their targets
 //   Lawson    -- flip rounds to the exact Delaunay triangulation
-extern "C" int mcl_tri_step(mcl_tri_t* t, const mcl_step_params_t* p, mcl_stats_t* st, void* stream) {
-    if (!t || !p) return MCL_ERR_INPUT;
+static int step_impl(mcl_tri* t, const mcl_step_params_t* p, const float2* disp, mcl_stats_t* st, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     DMesh& m = t->m;
     auto t0 = std::chrono::steady_clock::now();
@@ -1712,7 +1791,7 @@ extern "C" int mcl_tri_step(mcl_tri_t* t, const mcl_step_params_t* p, mcl_stats_
     const int mv_stamp = ++t->stamp;
     t->timers[2].mark(s);
     hipLaunchKernelGGL(k_move_motion, dim3(blocks_for(m.nv)), dim3(kThreads), 0, s, m, t->acc, t->tgt, t->v_mv,
-                       t->v_fexp, mv_stamp, *p);
+                       t->v_fexp, mv_stamp, *p, disp);
     t->timers[2].mark(s);
     MCL_TRY(zero_slots(t, s, kCnBad, 11));  // bad, rev, move, pend, err, misc, touched
     t->timers[1].mark(s);
@@ -1750,6 +1829,18 @@ extern "C" int mcl_tri_step(mcl_tri_t* t, const mcl_step_params_t* p, mcl_stats_
         st->inverted = inverted0;
     }
     return rc != MCL_OK ? rc : rc2;
+}
+
+extern "C" int mcl_tri_step(mcl_tri_t* t, const mcl_step_params_t* p, mcl_stats_t* st, void* stream) {
+    if (!t || !p) return MCL_ERR_INPUT;
+    return step_impl(t, p, nullptr, st, stream);
+}
+
+extern "C" int mcl_tri_move_legalize(mcl_tri_t* t, const void* d_disp, int variant, mcl_stats_t* st, void* stream) {
+    if (!t || !d_disp || (variant != 0 && variant != 1)) return MCL_ERR_INPUT;
+    mcl_step_params_t p{};
+    p.variant = variant;
+    return step_impl(t, &p, (const float2*)d_disp, st, stream);
 }
 
 // ---------------------------------------------------------------- one ring
@@ -1794,6 +1885,31 @@ extern "C" int mcl_tri_closest(mcl_tri_t* t, const void* d_offsets, const void* 
     MCL_HIP(hipGetLastError());
     MCL_HIP(hipStreamSynchronize(s));
     return MCL_OK;
+}
+
+extern "C" int mcl_tri_closest_ring(mcl_tri_t* t, int K, int layout, const void* d_ring, void* d_out, void* stream) {
+    if (!t || !d_ring || !d_out || K < 2 || (layout != 0 && layout != 1)) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_closest_legacy, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, K, layout,
+                       (const int*)d_ring, (int*)d_out);
+    MCL_HIP(hipGetLastError());
+    MCL_HIP(hipStreamSynchronize(s));
+    return MCL_OK;
+}
+
+extern "C" int mcl_tri_frnn_graph(mcl_tri_t* t, float r, int K, int layout, const void* d_ring, int M, void* d_out,
+                                  int* overflow, void* stream) {
+    if (!t || !d_ring || !d_out || K < 2 || M < 1 || (layout != 0 && layout != 1) || !(r >= 0.0f)) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->cnt + kCnMisc, 1, 0);
+    hipLaunchKernelGGL(k_frnn_graph, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, r * r, K, layout,
+                       (const int*)d_ring, M, (int*)d_out, t->cnt + kCnMisc);
+    MCL_HIP(hipGetLastError());
+    int ov = 0;
+    MCL_HIP(hipMemcpyAsync(&ov, t->cnt + kCnMisc, sizeof(int), hipMemcpyDeviceToHost, s));
+    MCL_HIP(hipStreamSynchronize(s));
+    if (overflow) *overflow = ov;
+    return ov ? MCL_ERR_CAPACITY : MCL_OK;
 }
 
 extern "C" int mcl_tri_kernel_timer(mcl_tri_t* t, int enable) {

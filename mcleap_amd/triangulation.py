@@ -209,6 +209,15 @@ class DeviceTriangulation:
         self.last_stats = st.as_dict()
         return self.last_stats
 
+    def move_legalize(self, d, variant=0):
+        """movePoints(d) + untangle + delonize as one call that always ends
+        valid and exactly Delaunay (d: device (V,2) float32 tensor)"""
+        st = _lib.Stats()
+        check(lib().mcl_tri_move_legalize(self._h, _dev_ptr(d), int(variant), ctypes.byref(st), None),
+              "mcl_tri_move_legalize")
+        self.last_stats = st.as_dict()
+        return self.last_stats
+
     def count_inverted(self):
         out = ctypes.c_longlong()
         check(lib().mcl_tri_count_inverted(self._h, ctypes.byref(out), None), "mcl_tri_count_inverted")
@@ -242,6 +251,20 @@ class DeviceTriangulation:
     def closestNeighbors(self, offsets, nbrs, out):
         check(lib().mcl_tri_closest(self._h, _dev_ptr(offsets), _dev_ptr(nbrs), _dev_ptr(out), None),
               "mcl_tri_closest")
+        return True
+
+    def closestNeighbors_ring(self, K, ring, out, layout=0):
+        """closestNeighbors<K>(ring, out) (layout 0) / closestNeighbors2<K> (layout 1)
+        over the ring written by oneRing / oneRing2"""
+        check(lib().mcl_tri_closest_ring(self._h, int(K), int(layout), _dev_ptr(ring), _dev_ptr(out), None),
+              "mcl_tri_closest_ring")
+        return True
+
+    def getFRNN(self, K, Mx, r, ring, out, layout=0):
+        """getFRNN<K, M>(r, ring, out) (layout 0) / getFRNN2<K, M> (layout 1)"""
+        ov = ctypes.c_int()
+        check(lib().mcl_tri_frnn_graph(self._h, float(r), int(K), int(layout), _dev_ptr(ring), int(Mx), _dev_ptr(out),
+                                       ctypes.byref(ov), None), "mcl_tri_frnn_graph")
         return True
 
     def one_ring_csr_async(self, offsets, nbrs):

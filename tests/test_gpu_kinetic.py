@@ -126,3 +126,63 @@ def test_delonize_from_perturbed(oracle):
         print(variant, sigma, nd, dt.last_stats)
         assert dt.last_stats["flips"] > 0
         _check_box(O, dt, "delonize variant %d" % variant)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_move_legalize_caller_displacement(oracle, periodic):
+    """movePoints(d) + untangle + delonize as one call with a caller
+    displacement array: the mesh ends valid and equal to the oracle's DT of
+    the moved positions, every vertex at pos + d (wrapped on the torus)"""
+    import torch
+    O = oracle
+    n = 8000
+    xy = uniform_points(n, 31)
+    H = M.HostTriangulation(periodic=periodic)
+    H.addDelaunayPoints(xy)
+    dt = M.DeviceTriangulation(H)
+    g = torch.Generator(device="cuda").manual_seed(9)
+    for k in range(3):
+        p0 = dt.download_positions()
+        d = (0.4 / np.sqrt(n)) * torch.randn((dt.m_pos_size, 2), generator=g, device="cuda")
+        if not periodic:
+            # keep targets inside [0, 1): shrink moves that would leave it
+            q = torch.from_numpy(p0).cuda()
+            t = q + d
+            d = torch.where((t >= 0) & (t < 1), d, torch.zeros_like(d))
+            d[:4] = 0
+        st = dt.move_legalize(d)
+        print(st)
+        assert st["stalled"] == 0
+        pos = (_check_torus if periodic else _check_box)(O, dt, "move_legalize %d" % k)
+        want = p0 + d.cpu().numpy()
+        if periodic:
+            want = np.mod(want, np.float32(1.0))
+            want[want >= 1.0] = 0.0
+        sel = slice(None) if periodic else slice(4, None)
+        assert np.array_equal(pos[sel], want.astype(np.float32)[sel])
+
+
+def test_untangle_api(oracle):
+    """untangle (B/C rounds, exact orientation) never breaks the half-edge
+    structure, does not increase inversions, and reports whether it
+    resolved them all"""
+    import torch
+    O = oracle
+    xy = uniform_points(6000, 33)
+    H = M.HostTriangulation()
+    H.addDelaunayPoints(xy)
+    dt = M.DeviceTriangulation(H)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    d = torch.zeros((dt.m_pos_size, 2), dtype=torch.float32, device="cuda")
+    d[4:] = 2e-3 * torch.randn((dt.m_pos_size - 4, 2), generator=g, device="cuda")
+    dt.movePoints(d)
+    before = dt.count_inverted()
+    assert before > 0
+    ok = dt.untangle()
+    after = dt.count_inverted()
+    print(before, after, dt.last_stats)
+    assert after <= before and dt.last_stats["untangle_flips"] > 0
+    assert ok == (after == 0)
+    pos, v, he, t = dt.download()
+    rc, msg = O.check_mesh(pos, v, he, t, periodic=False, require_ccw=False)
+    assert rc == 0, msg

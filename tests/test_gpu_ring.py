@@ -111,3 +111,58 @@ def test_closest_neighbors_reference_case(oracle, n):
         d2[i] = np.inf
         best = np.flatnonzero(d2 == d2.min())
         assert got[i] in best, (i, got[i], best)
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_closest_over_legacy_ring(oracle, layout):
+    """closestNeighbors<K> over oneRing (SoA) and closestNeighbors2<K> over oneRing2 (AoS)"""
+    import torch
+    d = oracle.mt_uniform_floats(3, -1000.0, 1000.0, 2000)
+    H = M.HostTriangulation()
+    H.addDelaunayPoints(d.reshape(-1, 2))
+    dt = M.DeviceTriangulation(H)
+    pos = dt.download_positions()
+    nv, K = len(pos), 100
+    ring = torch.zeros(nv * K, dtype=torch.int32, device="cuda")
+    (dt.oneRing if layout == 0 else dt.oneRing2)(K, ring)
+    out = torch.zeros(nv, dtype=torch.int32, device="cuda")
+    dt.closestNeighbors_ring(K, ring, out, layout)
+    got = out.cpu().numpy()
+    for i in range(4, nv):
+        dx = (pos[i, 0] - pos[:, 0]).astype(np.float32)
+        dy = (pos[i, 1] - pos[:, 1]).astype(np.float32)
+        d2 = dx * dx + dy * dy
+        d2[i] = np.inf
+        assert got[i] in np.flatnonzero(d2 == d2.min())
+
+
+@pytest.mark.parametrize("radius", [5.0, 10.0, 20.0, 30.0, 40.0, 50.0])
+def test_graph_frnn_reference_case(oracle, radius):
+    """tests/test_frnn.cu: mt19937(3), 1000 points in [-1000, 1000)^2, K = 20,
+    M = 100; the count equals the brute-force count within r (the
+    reference's check) and the id set equals the oracle's getFRNN walk
+    (Host_Triangulation.h:525-560) on the same mesh"""
+    import torch
+    d = oracle.mt_uniform_floats(3, -1000.0, 1000.0, 2000)
+    H = M.HostTriangulation()
+    H.addDelaunayPoints(d.reshape(-1, 2))
+    dt = M.DeviceTriangulation(H)
+    pos, v, he, t = dt.download()
+    nv, K, Mx = len(pos), 20, 100
+    T = oracle.HostTri(oracle.HostTri.EXACT)
+    T.load(pos, v, he, t)
+    for layout in (0, 1):
+        ring = torch.zeros(nv * K, dtype=torch.int32, device="cuda")
+        (dt.oneRing if layout == 0 else dt.oneRing2)(K, ring)
+        out = torch.full((nv * Mx,), -1, dtype=torch.int32, device="cuda")
+        dt.getFRNN(K, Mx, radius, ring, out, layout)
+        o = out.cpu().numpy()
+        rr = np.float32(radius) * np.float32(radius)
+        for i in range(4, nv):
+            cnt = o[i] if layout == 0 else o[Mx * i]
+            ids = [o[(k + 1) * nv + i] if layout == 0 else o[(k + 1) + Mx * i] for k in range(min(cnt, Mx - 1))]
+            dx = (pos[i, 0] - pos[4:, 0]).astype(np.float32)
+            dy = (pos[i, 1] - pos[4:, 1]).astype(np.float32)
+            brute = int(((dx * dx + dy * dy) <= rr).sum()) - 1
+            assert cnt == brute, (layout, i, cnt, brute)
+            assert sorted(ids) == sorted(T.get_frnn(i, radius))
