@@ -56,6 +56,7 @@ enum StatSlot : int {
     kStOrient0, kStOrient1, kStOrient2, kStOrient3,
     kStIncircle0, kStIncircle1, kStIncircle2, kStIncircle3,
     kStRelocated, kStRelocFailed,
+    kStRelWalk, kStRelStar, kStRelStuck, kStRelPutBack,  // reasons of the failures
     kStCount
 };
 

@@ -225,20 +225,30 @@ __global__ void __launch_bounds__(kThreads) k_check_all(DMesh m, int* __restrict
 //    displacement.
 // Every bad triangle has an action: an original triangle whose vertices are
 // all at accepted positions is valid (the iteration's starting state is).
-__global__ void __launch_bounds__(kThreads) k_revert_mark(DMesh m, const int* __restrict__ bad, int* __restrict__ cnt,
-                                                          int slot_bad, const float2* __restrict__ acc,
-                                                          const int* __restrict__ v_mv, int mv_stamp,
-                                                          const int* __restrict__ t_rep, const int* __restrict__ t_rep_e,
-                                                          int rep_stamp, int* __restrict__ e_undo, int undo_stamp,
-                                                          int* __restrict__ undo_list, int slot_undo,
-                                                          int* __restrict__ v_rev, int rev_stamp, int* __restrict__ rev_list,
-                                                          int slot_rev, int* __restrict__ stuck, int slot_stuck) {
-    const int n = cnt[slot_bad];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+struct RevertArgs {
+    const float2* acc;
+    const int* v_mv;
+    int mv_stamp;
+    const int* t_rep;
+    const int* t_rep_e;
+    int rep_stamp;
+    int* e_undo;
+    int* undo_list;
+    int* v_rev;
+    int* rev_list;
+    int* stuck;
+};
+__device__ __forceinline__ void revert_mark_body(const DMesh& m, const int* __restrict__ bad, int n, const RevertArgs& A,
+                                                 int undo_stamp, int rev_stamp, int* c_undo, int* c_rev, int* c_stuck,
+                                                 int i0, int stride) {
+    const float2* __restrict__ acc = A.acc;
+    const int* __restrict__ v_mv = A.v_mv;
+    const int mv_stamp = A.mv_stamp;
+    for (int i = i0; i < n; i += stride) {
         const int t = bad[i];
-        if (t_rep[t] == rep_stamp) {
-            const int e = t_rep_e[t];
-            if (atomicExch(&e_undo[e], undo_stamp) != undo_stamp) undo_list[agg_inc(&cnt[slot_undo])] = e;
+        if (A.t_rep[t] == A.rep_stamp) {
+            const int e = A.t_rep_e[t];
+            if (atomicExch(&A.e_undo[e], undo_stamp) != undo_stamp) A.undo_list[agg_inc(c_undo)] = e;
             continue;
         }
         int vs[3];
@@ -265,11 +275,16 @@ __global__ void __launch_bounds__(kThreads) k_revert_mark(DMesh m, const int* __
             if (d > bd) { bd = d; best = v; }
         }
         if (best < 0) {
-            stuck[agg_inc(&cnt[slot_stuck])] = t;
+            A.stuck[agg_inc(c_stuck)] = t;
             continue;
         }
-        if (atomicExch(&v_rev[best], rev_stamp) != rev_stamp) rev_list[agg_inc(&cnt[slot_rev])] = best;
+        if (atomicExch(&A.v_rev[best], rev_stamp) != rev_stamp) A.rev_list[agg_inc(c_rev)] = best;
     }
+}
+__global__ void __launch_bounds__(kThreads) k_revert_mark(DMesh m, const int* __restrict__ bad, int* __restrict__ cnt,
+                                                          int slot_bad, RevertArgs A, int undo_stamp, int rev_stamp) {
+    revert_mark_body(m, bad, cnt[slot_bad], A, undo_stamp, rev_stamp, &cnt[kCnRev + 1], &cnt[kCnRev], &cnt[kCnMisc],
+                     blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // all triangles around v strictly CCW with v at p (neighbours where they are now)
@@ -301,22 +316,18 @@ __device__ __forceinline__ float2 shifted(const DMesh& m, float2 a, float dx, fl
 // the passes still end with every taken-back vertex at its previous position
 // in the worst case, which is valid.
 template <bool kF32>
-__global__ void __launch_bounds__(kThreads) k_revert_apply(DMesh m, const int* __restrict__ rev_list,
-                                                           const int* __restrict__ cnt, int slot_rev,
-                                                           const float2* __restrict__ acc, int* __restrict__ v_fexp,
-                                                           int* __restrict__ v_soft, int soft_stamp,
-                                                           int* __restrict__ all_rev, int* __restrict__ cnt_w, int slot_all,
-                                                           unsigned long long* __restrict__ stats) {
-    const int n = cnt[slot_rev];
-    unsigned loc[1] = {0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+__device__ __forceinline__ void revert_apply_body(const DMesh& m, const int* __restrict__ rev_list, int n,
+                                                  const float2* __restrict__ acc, int* __restrict__ v_fexp,
+                                                  int* __restrict__ v_soft, int soft_stamp, int* __restrict__ all_rev,
+                                                  int* c_all, unsigned (&loc)[1], int i0, int stride) {
+    for (int i = i0; i < n; i += stride) {
         const int v = rev_list[i];
         const float2 a = acc[v];
         float2 dest = a;
         const bool first = v_soft[v] != soft_stamp;
         if (first) {
             v_soft[v] = soft_stamp;
-            all_rev[agg_inc(&cnt_w[slot_all])] = v;
+            all_rev[agg_inc(c_all)] = v;
             // neighbour displacement and neighbour centroid, relative to a
             float sdx = 0.f, sdy = 0.f, scx = 0.f, scy = 0.f;
             int deg = 0;
@@ -341,6 +352,17 @@ __global__ void __launch_bounds__(kThreads) k_revert_apply(DMesh m, const int* _
         v_fexp[v] = min(v_fexp[v] + 1, 64);
         loc[0]++;
     }
+}
+template <bool kF32>
+__global__ void __launch_bounds__(kThreads) k_revert_apply(DMesh m, const int* __restrict__ rev_list,
+                                                           const int* __restrict__ cnt, int slot_rev,
+                                                           const float2* __restrict__ acc, int* __restrict__ v_fexp,
+                                                           int* __restrict__ v_soft, int soft_stamp,
+                                                           int* __restrict__ all_rev, int* __restrict__ cnt_w, int slot_all,
+                                                           unsigned long long* __restrict__ stats) {
+    unsigned loc[1] = {0};
+    revert_apply_body<kF32>(m, rev_list, cnt[slot_rev], acc, v_fexp, v_soft, soft_stamp, all_rev, &cnt_w[slot_all], loc,
+                            blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
     flush_stats(loc, stats, kStReverts);
 }
 
@@ -348,12 +370,10 @@ __global__ void __launch_bounds__(kThreads) k_revert_apply(DMesh m, const int* _
 // touched by another repair of the same iteration, so each undo restores
 // them exactly) and queue the restored triangles that are still bad
 template <bool kF32>
-__global__ void __launch_bounds__(kThreads) k_undo_apply(DMesh m, const int* __restrict__ undo_list, int* __restrict__ cnt,
-                                                         int slot_undo, int* __restrict__ t_rep, int* __restrict__ bad_out,
-                                                         int slot_out, unsigned long long* __restrict__ stats) {
-    const int n = cnt[slot_undo];
-    unsigned loc[4] = {0, 0, 0, 0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+__device__ __forceinline__ void undo_apply_body(const DMesh& m, const int* __restrict__ undo_list, int n,
+                                                int* __restrict__ t_rep, int* __restrict__ bad_out, int* c_out,
+                                                unsigned (&loc)[4], int i0, int stride) {
+    for (int i = i0; i < n; i += stride) {
         const int e = undo_list[i];
         int h2, h3, h4, h5;
         flip_soa(m, e, h2, h3, h4, h5);
@@ -366,9 +386,17 @@ __global__ void __launch_bounds__(kThreads) k_undo_apply(DMesh m, const int* __r
             tri_verts(m, ts[k], a, b, c);
             int sg = orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st);
             loc[st]++;
-            if (sg <= 0) bad_out[agg_inc(&cnt[slot_out])] = ts[k];
+            if (sg <= 0) bad_out[agg_inc(c_out)] = ts[k];
         }
     }
+}
+template <bool kF32>
+__global__ void __launch_bounds__(kThreads) k_undo_apply(DMesh m, const int* __restrict__ undo_list, int* __restrict__ cnt,
+                                                         int slot_undo, int* __restrict__ t_rep, int* __restrict__ bad_out,
+                                                         int slot_out, unsigned long long* __restrict__ stats) {
+    unsigned loc[4] = {0, 0, 0, 0};
+    undo_apply_body<kF32>(m, undo_list, cnt[slot_undo], t_rep, bad_out, &cnt[slot_out], loc,
+                          blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
     flush_stats(loc, stats, kStOrient0);
 }
 
@@ -433,13 +461,10 @@ __global__ void __launch_bounds__(kThreads) k_repair_flip(DMesh m, int* __restri
 
 // check the triangles around each listed vertex; append non-CCW ones
 template <bool kF32>
-__global__ void __launch_bounds__(kThreads) k_check_ring(DMesh m, const int* __restrict__ vlist, const int* __restrict__ cnt_in,
-                                                         int slot_in, int* __restrict__ t_stamp, int stamp, int* __restrict__ bad,
-                                                         int* __restrict__ cnt_out, int slot_out,
-                                                         unsigned long long* __restrict__ stats) {
-    const int n = cnt_in[slot_in];
-    unsigned loc[4] = {0, 0, 0, 0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+__device__ __forceinline__ void check_ring_body(const DMesh& m, const int* __restrict__ vlist, int n,
+                                                int* __restrict__ t_stamp, int stamp, int* __restrict__ bad, int* c_out,
+                                                unsigned (&loc)[4], int i0, int stride) {
+    for (int i = i0; i < n; i += stride) {
         int v = vlist[i];
         int h0 = m.v_he[v], h = h0, guard = 0;
         do {
@@ -449,12 +474,76 @@ __global__ void __launch_bounds__(kThreads) k_check_ring(DMesh m, const int* __r
                 tri_verts(m, t, a, b, c);
                 int s = orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st);
                 loc[st]++;
-                if (s <= 0) bad[agg_inc(&cnt_out[slot_out])] = t;
+                if (s <= 0) bad[agg_inc(c_out)] = t;
             }
             h = m.he_next[h ^ 1];
         } while (h != h0 && ++guard < 4096);
     }
+}
+template <bool kF32>
+__global__ void __launch_bounds__(kThreads) k_check_ring(DMesh m, const int* __restrict__ vlist, const int* __restrict__ cnt_in,
+                                                         int slot_in, int* __restrict__ t_stamp, int stamp, int* __restrict__ bad,
+                                                         int* __restrict__ cnt_out, int slot_out,
+                                                         unsigned long long* __restrict__ stats) {
+    unsigned loc[4] = {0, 0, 0, 0};
+    check_ring_body<kF32>(m, vlist, cnt_in[slot_in], t_stamp, stamp, bad, &cnt_out[slot_out], loc,
+                          blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
     flush_stats(loc, stats, kStOrient0);
+}
+
+// Take-back passes finished by one workgroup: the same mark / apply / undo /
+// re-check phases as the grid kernels, separated by workgroup barriers, with
+// the list lengths in LDS, until no triangle is bad.  Stamps stamp0 + 2p and
+// stamp0 + 2p + 1 are used by pass p.  out = {passes, final list slot,
+// stuck, undos}.
+template <bool kF32>
+__global__ void __launch_bounds__(1024) k_revert_tail(DMesh m, int* __restrict__ bad0, int* __restrict__ bad1, int cur0,
+                                                      int* __restrict__ cnt, RevertArgs A, int* __restrict__ t_rep,
+                                                      int stamp0, int max_passes, int* __restrict__ v_fexp,
+                                                      int* __restrict__ v_soft, int soft_stamp, int* __restrict__ all_rev,
+                                                      int* __restrict__ t_stamp, unsigned long long* __restrict__ stats,
+                                                      int* __restrict__ out) {
+    __shared__ int s_nb, s_rev, s_undo, s_stuck, s_next;
+    unsigned lrev[1] = {0}, lor[4] = {0, 0, 0, 0};
+    int cur = cur0, pass = 0, undos = 0, stuck = 0;
+    if (threadIdx.x == 0) s_nb = cnt[kCnBad + cur0];
+    __syncthreads();
+    for (; pass < max_passes; pass++) {
+        const int nb = s_nb;
+        if (nb == 0) break;
+        int* bad = cur ? bad1 : bad0;
+        int* bad_next = cur ? bad0 : bad1;
+        __syncthreads();
+        if (threadIdx.x == 0) { s_rev = 0; s_undo = 0; s_stuck = 0; s_next = 0; }
+        __syncthreads();
+        const int st0 = stamp0 + 2 * pass;
+        revert_mark_body(m, bad, nb, A, st0, st0, &s_undo, &s_rev, &s_stuck, threadIdx.x, blockDim.x);
+        __syncthreads();
+        if (s_stuck > 0) { stuck = 1; break; }
+        const int nr = s_rev, nu = s_undo;
+        undos += nu;
+        revert_apply_body<kF32>(m, A.rev_list, nr, A.acc, v_fexp, v_soft, soft_stamp, all_rev, &cnt[kCnRev + 2], lrev,
+                                threadIdx.x, blockDim.x);
+        __syncthreads();
+        undo_apply_body<kF32>(m, A.undo_list, nu, t_rep, bad_next, &s_next, lor, threadIdx.x, blockDim.x);
+        __syncthreads();
+        check_ring_body<kF32>(m, A.rev_list, nr, t_stamp, st0 + 1, bad_next, &s_next, lor, threadIdx.x, blockDim.x);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_nb = s_next;
+            cnt[kCnBad + (1 - cur)] = s_next;
+        }
+        cur = 1 - cur;
+        __syncthreads();
+    }
+    flush_stats(lrev, stats, kStReverts);
+    flush_stats(lor, stats, kStOrient0);
+    if (threadIdx.x == 0) {
+        out[0] = pass;
+        out[1] = cur;
+        out[2] = stuck;
+        out[3] = undos;
+    }
 }
 
 // next attempt for vertices short of their target.  fexp == 0: the whole
@@ -564,7 +653,7 @@ __global__ void __launch_bounds__(kThreads) k_collect_pending(DMesh m, const int
 // moveFlipflop (Host_Triangulation.h:77, 408) and leaves unimplemented.
 // Every flip and split keeps all triangles strictly CCW (checked exactly),
 // so the mesh stays valid; Lawson rounds restore the Delaunay property.
-constexpr int kMaxStar = 48;
+constexpr int kMaxStar = 64;
 
 // Remembering stochastic visibility walk (Devillers, Pion, Teillaud, "Walking
 // in a triangulation", 2002) from triangle t toward p: at each triangle the
@@ -610,12 +699,10 @@ __device__ int locate_point(const DMesh& m, float2 p, int t, int* on, unsigned s
 
 // phase 1 (read-only): find where each listed vertex's target lies
 template <bool kF32>
-__global__ void __launch_bounds__(kThreads) k_reloc_locate(DMesh m, const int* __restrict__ vlist,
-                                                           const int* __restrict__ cnt, int slot,
-                                                           const float2* __restrict__ tgt, int* __restrict__ r_tt,
-                                                           int* __restrict__ r_tt2, int round) {
-    const int n = cnt[slot];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+__device__ __forceinline__ void reloc_locate_body(const DMesh& m, const int* __restrict__ vlist, int n,
+                                                  const float2* __restrict__ tgt, int* __restrict__ r_tt,
+                                                  int* __restrict__ r_tt2, int round, int i0, int stride) {
+    for (int i = i0; i < n; i += stride) {
         const int v = vlist[i];
         const float2 t = tgt[v], q = m.pos[v];
         if (t.x == q.x && t.y == q.y) { r_tt[v] = -3; continue; }
@@ -625,14 +712,20 @@ __global__ void __launch_bounds__(kThreads) k_reloc_locate(DMesh m, const int* _
         r_tt2[v] = (tt >= 0 && on >= 0) ? m.he_t[on ^ 1] : -1;
     }
 }
+template <bool kF32>
+__global__ void __launch_bounds__(kThreads) k_reloc_locate(DMesh m, const int* __restrict__ vlist,
+                                                           const int* __restrict__ cnt, int slot,
+                                                           const float2* __restrict__ tgt, int* __restrict__ r_tt,
+                                                           int* __restrict__ r_tt2, int round) {
+    reloc_locate_body<kF32>(m, vlist, cnt[slot], tgt, r_tt, r_tt2, round, blockIdx.x * blockDim.x + threadIdx.x,
+                            gridDim.x * blockDim.x);
+}
 
 // phase 2: claim the star triangles and the target triangle(s)
-__global__ void __launch_bounds__(kThreads) k_reloc_claim(DMesh m, const int* __restrict__ vlist,
-                                                          const int* __restrict__ cnt, int slot, int round,
-                                                          const int* __restrict__ r_tt, const int* __restrict__ r_tt2,
-                                                          unsigned long long* __restrict__ claim) {
-    const int n = cnt[slot];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+__device__ __forceinline__ void reloc_claim_body(const DMesh& m, const int* __restrict__ vlist, int n, int round,
+                                                 int* __restrict__ r_tt, const int* __restrict__ r_tt2,
+                                                 unsigned long long* __restrict__ claim, int i0, int stride) {
+    for (int i = i0; i < n; i += stride) {
         const int v = vlist[i];
         const int tt = r_tt[v];
         if (tt < 0) continue;
@@ -644,9 +737,19 @@ __global__ void __launch_bounds__(kThreads) k_reloc_claim(DMesh m, const int* __
             if (t >= 0) atomicMax(&claim[t], key);
             h = m.he_next[h ^ 1];
         } while (h != h0 && ++guard < kMaxStar);
+        // the mesh is not modified during this phase, so this is the true
+        // degree: a star too large for the apply phase is given up here
+        if (h != h0) { r_tt[v] = -4; continue; }
         atomicMax(&claim[tt], key);
         if (r_tt2[v] >= 0) atomicMax(&claim[r_tt2[v]], key);
     }
+}
+__global__ void __launch_bounds__(kThreads) k_reloc_claim(DMesh m, const int* __restrict__ vlist,
+                                                          const int* __restrict__ cnt, int slot, int round,
+                                                          int* __restrict__ r_tt, const int* __restrict__ r_tt2,
+                                                          unsigned long long* __restrict__ claim) {
+    reloc_claim_body(m, vlist, cnt[slot], round, r_tt, r_tt2, claim, blockIdx.x * blockDim.x + threadIdx.x,
+                     gridDim.x * blockDim.x);
 }
 
 __device__ __forceinline__ void set_he(const DMesh& m, int h, int nx, int v, int t, int o) {
@@ -673,20 +776,17 @@ __device__ __forceinline__ int tri_contains(const DMesh& m, int X, float2 p, int
 
 // phase 3: winners (all claims held) remove and re-insert their vertex
 template <bool kF32>
-__global__ void __launch_bounds__(kThreads) k_reloc_apply(DMesh m, const int* __restrict__ vlist,
-                                                          const int* __restrict__ cnt, int slot, int round,
-                                                          const int* __restrict__ r_tt, const int* __restrict__ r_tt2,
-                                                          const unsigned long long* __restrict__ claim,
-                                                          const float2* __restrict__ tgt, int* __restrict__ next_list,
-                                                          int* __restrict__ cnt_next, int slot_next,
-                                                          unsigned long long* __restrict__ stats) {
-    const int n = cnt[slot];
-    unsigned loc[2] = {0, 0};  // relocated, given up
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+__device__ __forceinline__ void reloc_apply_body(const DMesh& m, const int* __restrict__ vlist, int n, int round,
+                                                 const int* __restrict__ r_tt, const int* __restrict__ r_tt2,
+                                                 const unsigned long long* __restrict__ claim,
+                                                 const float2* __restrict__ tgt, int* __restrict__ next_list, int* c_next,
+                                                 unsigned (&loc)[6], int i0, int stride) {
+    for (int i = i0; i < n; i += stride) {
         const int v = vlist[i];
         const int tt = r_tt[v], tt2 = r_tt2[v];
         if (tt == -3) continue;              // already at its target
-        if (tt < 0) { loc[1]++; continue; }  // target on a vertex or walk lost: stays put
+        if (tt == -4) { loc[1]++; loc[3]++; continue; }  // star larger than kMaxStar: stays put
+        if (tt < 0) { loc[1]++; loc[2]++; continue; }  // target on a vertex or walk lost: stays put
         const unsigned long long key = claim_key(round, v);
         int star[kMaxStar];
         int deg = 0;
@@ -702,9 +802,9 @@ __global__ void __launch_bounds__(kThreads) k_reloc_apply(DMesh m, const int* __
                 h = m.he_next[h ^ 1];
             } while (h != h0);
         }
-        if (!mine) {  // lost a claim (or star too large): retry next round
-            if (deg < kMaxStar) next_list[agg_inc(&cnt_next[slot_next])] = v;
-            else loc[1]++;
+        if (!mine) {  // lost a claim: retry next round.  (A loser's walk may
+            // cross triangles a winner is rewriting, so its count is not a degree.)
+            next_list[agg_inc(c_next)] = v;
             continue;
         }
         const int deg0 = deg;
@@ -731,6 +831,7 @@ __global__ void __launch_bounds__(kThreads) k_reloc_apply(DMesh m, const int* __
         }
         if (deg > 3) {  // degenerate link (collinear vertices): give up, mesh still valid
             loc[1]++;
+            loc[4]++;
             continue;
         }
         // --- merge the three triangles around v into one
@@ -809,9 +910,61 @@ __global__ void __launch_bounds__(kThreads) k_reloc_apply(DMesh m, const int* __
         }
         m.pos[v] = pins;
         if (pins.x == pt.x && pins.y == pt.y) loc[0]++;
-        else loc[1]++;
+        else { loc[1]++; loc[5]++; }
+    }
+}
+template <bool kF32>
+__global__ void __launch_bounds__(kThreads) k_reloc_apply(DMesh m, const int* __restrict__ vlist,
+                                                          const int* __restrict__ cnt, int slot, int round,
+                                                          const int* __restrict__ r_tt, const int* __restrict__ r_tt2,
+                                                          const unsigned long long* __restrict__ claim,
+                                                          const float2* __restrict__ tgt, int* __restrict__ next_list,
+                                                          int* __restrict__ cnt_next, int slot_next,
+                                                          unsigned long long* __restrict__ stats) {
+    unsigned loc[6] = {0, 0, 0, 0, 0, 0};  // relocated, given up, and the reasons
+    reloc_apply_body<kF32>(m, vlist, cnt[slot], round, r_tt, r_tt2, claim, tgt, next_list, &cnt_next[slot_next], loc,
+                           blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+    flush_stats(loc, stats, kStRelocated);
+}
+
+// relocation rounds finished by one workgroup (locate / claim / apply with
+// workgroup barriers, list lengths in LDS); rounds round0, round0 + 1, ...
+// out = {rounds, final list index (0/1), remaining}
+template <bool kF32>
+__global__ void __launch_bounds__(1024) k_reloc_tail(DMesh m, int* __restrict__ list0, int* __restrict__ list1, int cur0,
+                                                     int n0, int round0, int max_rounds, int* __restrict__ r_tt,
+                                                     int* __restrict__ r_tt2, unsigned long long* __restrict__ claim,
+                                                     const float2* __restrict__ tgt, unsigned long long* __restrict__ stats,
+                                                     int* __restrict__ out) {
+    __shared__ int s_n, s_next;
+    unsigned loc[6] = {0, 0, 0, 0, 0, 0};
+    int cur = cur0, r = 0;
+    if (threadIdx.x == 0) s_n = n0;
+    __syncthreads();
+    for (; r < max_rounds; r++) {
+        const int n = s_n;
+        if (n == 0) break;
+        const int* list = cur ? list1 : list0;
+        int* next = cur ? list0 : list1;
+        const int round = round0 + r;
+        __syncthreads();
+        if (threadIdx.x == 0) s_next = 0;
+        reloc_locate_body<kF32>(m, list, n, tgt, r_tt, r_tt2, round, threadIdx.x, blockDim.x);
+        __syncthreads();
+        reloc_claim_body(m, list, n, round, r_tt, r_tt2, claim, threadIdx.x, blockDim.x);
+        __syncthreads();
+        reloc_apply_body<kF32>(m, list, n, round, r_tt, r_tt2, claim, tgt, next, &s_next, loc, threadIdx.x, blockDim.x);
+        __syncthreads();
+        if (threadIdx.x == 0) s_n = s_next;
+        cur = 1 - cur;
+        __syncthreads();
     }
     flush_stats(loc, stats, kStRelocated);
+    if (threadIdx.x == 0) {
+        out[0] = r;
+        out[1] = cur;
+        out[2] = s_n;
+    }
 }
 
 // ------------------------------------------------------------ flip rounds
@@ -1203,12 +1356,15 @@ struct mcl_tri {
     int* v_soft = nullptr;             // nv: step stamp of a vertex's soft take-back
     int* r_tt = nullptr;               // nv: relocation target triangle
     int* r_tt2 = nullptr;              // nv: second triangle when the target is on an edge
+    int* tail_out = nullptr;           // 8: results of the single-workgroup loops
     // one ring
     int* deg = nullptr;
     int* scan_tmp = nullptr;
     int* h_cnt = nullptr;  // pinned
     KTimer timers[3];      // Lawson full round, check all, move
     int tail_threshold = 8192;  // queue length below which one workgroup finishes the rounds
+    int revert_tail = 1024;     // bad-triangle count below which one workgroup finishes the take-backs
+    int reloc_tail = 1024;      // relocation list length below which one workgroup finishes the rounds
     int round = 1;         // global round counter (claims, edge stamps)
     int stamp = 1;         // global stamp for vertex / triangle marks
     int grid_blocks = 1024;
@@ -1244,7 +1400,7 @@ static int tri_free(mcl_tri* t) {
     void* ptrs[] = {t->m.pos, t->m.he_next, t->m.he_v, t->m.he_t, t->m.he_op, t->m.v_he, t->m.v_pidx, t->m.t_he,
                     t->claim, t->e_stamp, t->cand[0], t->cand[1], t->wants, t->cnt, t->rounds_done, t->stats,
                     t->scalar, t->acc, t->tgt, t->v_mv, t->v_rev, t->v_fexp, t->t_stamp, t->bad[0], t->bad[1],
-                    t->rev, t->allrev, t->pend[0], t->pend[1], t->deg, t->scan_tmp, t->v_touch, t->t_rep, t->t_rep_e, t->e_undo, t->undo, t->stuck, t->touched, t->r_tt, t->r_tt2, t->v_soft};
+                    t->rev, t->allrev, t->pend[0], t->pend[1], t->deg, t->scan_tmp, t->v_touch, t->t_rep, t->t_rep_e, t->e_undo, t->undo, t->stuck, t->touched, t->r_tt, t->r_tt2, t->v_soft, t->tail_out};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (t->h_cnt) (void)hipHostFree(t->h_cnt);
@@ -1294,7 +1450,7 @@ extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv,
     A(t->bad[0], nt); A(t->bad[1], nt); A(t->rev, nv); A(t->allrev, nv); A(t->pend[0], nv); A(t->pend[1], nv);
     A(t->deg, nv + 1); A(t->scan_tmp, scan_scratch_ints(nv + 1) + 16);
     A(t->v_touch, nv); A(t->t_rep, nt); A(t->t_rep_e, nt); A(t->e_undo, m.ne); A(t->undo, m.ne);
-    A(t->stuck, nt); A(t->touched, nv); A(t->r_tt, nv); A(t->r_tt2, nv); A(t->v_soft, nv);
+    A(t->stuck, nt); A(t->touched, nv); A(t->r_tt, nv); A(t->r_tt2, nv); A(t->v_soft, nv); A(t->tail_out, 8);
 #undef A
     if (rc == MCL_OK && hipHostMalloc((void**)&t->h_cnt, sizeof(int) * kCnCount) != hipSuccess) rc = MCL_ERR_HIP;
     if (rc == MCL_OK) rc = upload_aos(t, pos, v2, he4, t1);
@@ -1632,6 +1788,11 @@ static int zero_slots(mcl_tri* t, hipStream_t s, int first, int n) {
 
 // repair rounds on the bad list in bad[*cur] (count slot kCnBad + *cur);
 // returns with the remaining bad triangles in bad[*cur]
+static bool trace_on() {
+    static const bool on = getenv("MCL_TRACE") != nullptr;
+    return on;
+}
+
 static int repair_rounds(mcl_tri* t, hipStream_t s, bool f32, int* cur, int rep_stamp, int* nrep) {
     DMesh& m = t->m;
     *nrep = 0;
@@ -1679,19 +1840,38 @@ static int revert_passes(mcl_tri* t, hipStream_t s, bool f32, int* cur, int mv_s
                          bool* stuck) {
     DMesh& m = t->m;
     *stuck = false;
+    const RevertArgs rargs{t->acc, t->v_mv, mv_stamp, t->t_rep, t->t_rep_e, rep_stamp, t->e_undo, t->undo,
+                           t->v_rev, t->rev, t->stuck};
     for (int pass = 0; pass < 4096; pass++) {
         int nb = t->h_cnt[kCnBad + *cur];
         if (nb == 0) break;
+        if (f32 && nb < t->revert_tail) {
+            // finish in one workgroup: no host round trip per pass
+            const int max_passes = 4096 - pass;
+            const int stamp0 = t->stamp + 1;
+            t->stamp += 2 * max_passes + 2;
+            hipLaunchKernelGGL((k_revert_tail<true>), dim3(1), dim3(1024), 0, s, m, t->bad[0], t->bad[1], *cur, t->cnt,
+                               rargs, t->t_rep, stamp0, max_passes, t->v_fexp, t->v_soft, mv_stamp, t->allrev,
+                               t->t_stamp, t->stats, t->tail_out);
+            MCL_HIP(hipGetLastError());
+            int out[4];
+            MCL_HIP(hipMemcpyAsync(out, t->tail_out, sizeof(out), hipMemcpyDeviceToHost, s));
+            MCL_TRY(read_counts(t, s));
+            if (trace_on()) fprintf(stderr, "  [revert tail] passes %d undos %d stuck %d\n", out[0], out[3], out[2]);
+            *undos += out[3];
+            *cur = out[1];
+            if (out[2]) *stuck = true;
+            return MCL_OK;
+        }
         const int rev_stamp = ++t->stamp, undo_stamp = rev_stamp;
         MCL_TRY(zero_slots(t, s, kCnRev, 2));   // rev list, undo list
         MCL_TRY(zero_slots(t, s, kCnMisc, 1));  // stuck
         hipLaunchKernelGGL(k_revert_mark, dim3(blocks_for(nb)), dim3(kThreads), 0, s, m, t->bad[*cur], t->cnt,
-                           (int)kCnBad + *cur, t->acc, t->v_mv, mv_stamp, t->t_rep, t->t_rep_e, rep_stamp, t->e_undo,
-                           undo_stamp, t->undo, (int)kCnRev + 1, t->v_rev, rev_stamp, t->rev, (int)kCnRev, t->stuck,
-                           (int)kCnMisc);
+                           (int)kCnBad + *cur, rargs, undo_stamp, rev_stamp);
         MCL_TRY(read_counts(t, s));
         if (t->h_cnt[kCnMisc] > 0) { *stuck = true; return MCL_OK; }
         const int nr = t->h_cnt[kCnRev], nu = t->h_cnt[kCnRev + 1];
+        if (trace_on()) fprintf(stderr, "  [revert %d] bad %d rev %d undo %d\n", pass, nb, nr, nu);
         const int nxt = 1 - *cur;
         MCL_TRY(zero_slots(t, s, kCnBad + nxt, 1));
         if (nr > 0) {
@@ -1732,6 +1912,21 @@ static int relocate_rounds(mcl_tri* t, hipStream_t s, bool f32, int n, int* slot
     *rounds = 0;
     int cur = *slot;
     for (int r = 0; r < 256 && n > 0; r++) {
+        if (f32 && n < t->reloc_tail) {
+            const int max_rounds = 256 - r;
+            const int R0 = t->round;
+            hipLaunchKernelGGL((k_reloc_tail<true>), dim3(1), dim3(1024), 0, s, m, t->pend[0], t->pend[1], cur, n, R0,
+                               max_rounds, t->r_tt, t->r_tt2, t->claim, t->tgt, t->stats, t->tail_out);
+            MCL_HIP(hipGetLastError());
+            int out[3];
+            MCL_HIP(hipMemcpyAsync(out, t->tail_out, sizeof(out), hipMemcpyDeviceToHost, s));
+            MCL_HIP(hipStreamSynchronize(s));
+            if (trace_on()) fprintf(stderr, "  [reloc tail] rounds %d left %d\n", out[0], out[2]);
+            t->round = R0 + out[0] + 1;
+            *rounds += out[0];
+            *slot = out[1];
+            return out[2] == 0 ? MCL_OK : MCL_ERR_ROUND_CAP;
+        }
         const int R = t->round++;
         const int nxt = 1 - cur;
         hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->cnt + kCnPend, 1, n);
@@ -1754,6 +1949,7 @@ static int relocate_rounds(mcl_tri* t, hipStream_t s, bool f32, int n, int* slot
                                t->stats);
         MCL_HIP(hipGetLastError());
         MCL_TRY(read_counts(t, s));
+        if (trace_on()) fprintf(stderr, "  [reloc %d] list %d retry %d\n", r, n, t->h_cnt[kCnMove]);
         n = t->h_cnt[kCnMove];
         cur = nxt;
         (*rounds)++;
@@ -1819,6 +2015,12 @@ static int step_impl(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     }
     long long rounds = 0;
     if (rc == MCL_OK) rc = run_rounds<0>(t, true, variant, 0, s, &rounds);
+    if (trace) {
+        unsigned long long h[kStCount];
+        if (hipMemcpy(h, t->stats, sizeof(h), hipMemcpyDeviceToHost) == hipSuccess)
+            fprintf(stderr, "[step] relocation failures: walk %llu star %llu removal %llu put-back %llu\n", h[kStRelWalk],
+                    h[kStRelStar], h[kStRelStuck], h[kStRelPutBack]);
+    }
     if (trace)
         fprintf(stderr, "[step] inverted %lld repairs %d bad_after_repair %d undone %d reverted %d reloc_rounds %d lawson %lld\n",
                 inverted0, nrep, bad_after, nundo, nrev, reloc_rounds, rounds);
