@@ -11,9 +11,10 @@ torch.cuda.synchronize().  Prints one JSON line.
 
 roofline: the dominant kernel (largest GPU time) among the per-kernel HIP
 event timers of the library, credited with algorithmic bytes per launch from
-the per-point model (move 16 B, inversion check 32 B, Delaunay check 56 B,
-grid build 43 B, FRNN count 13 B, FRNN fill 4*k B).  step_roofline: the
-whole step credited with (212 + 4k) B per point.
+the per-unit model (per point: move 16 B, inversion check 32 B, Delaunay
+check 56 B, grid build 43 B, FRNN count 13 B, FRNN fill 4*k B; per relocated
+vertex: 648 B).  step_roofline: the whole step credited with (212 + 4k) B per
+point.
 cpu_baseline: the oracle's restatement of the reference host path (move,
 untangle B/C passes, worklist Lawson) on the same mesh, one thread.
 """
@@ -31,6 +32,7 @@ sys.path.insert(0, HERE)
 
 PEAK_HBM_GBS = 8000.0
 BYTES = {"move": 16, "check": 32, "lawson": 56, "grid_build": 43, "frnn_count": 13}
+RELOC_BYTES = 648  # per relocated vertex: star read 3*6*16 + 7 positions * 8 + rewrite 3*6*16
 
 
 def _args():
@@ -44,7 +46,23 @@ def _args():
     ap.add_argument("--plain-steps", type=int, default=10, help="steps of the plain form timed for comparison")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (reference for HBM byte counters)")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """A plain copy of known size: under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    its counters give the factor between counted and actual bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}))
 
 
 def cpu_baseline(pos0, cfg, periodic, seconds):
@@ -99,6 +117,9 @@ def main():
     if a.gpus != 1:
         print("bench.py runs on one GPU only: every box has one MI355X; use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if a.calibrate_copy:
+        calibrate_copy(a.calibrate_copy)
+        return
     import torch
 
     import mcleap_amd as M
@@ -158,7 +179,7 @@ def main():
 
     # per-kernel times from the live event timers
     kern = {}
-    for name, which in (("lawson", 0), ("check", 1), ("move", 2)):
+    for name, which in (("lawson", 0), ("check", 1), ("move", 2), ("reloc_apply", 3), ("revert_apply", 4)):
         ms, cnt = dt.kernel_timer_read(which)
         kern[name] = (ms, cnt)
     kbar = 0.0
@@ -170,11 +191,15 @@ def main():
     dt.kernel_timer(False)
     bytes_per_pt = dict(BYTES)
     bytes_per_pt["frnn_fill"] = 4.0 * kbar
-    dom = max(kern, key=lambda k_: kern[k_][0])
+    # algorithmic bytes of all timed launches of each kernel class
+    algo_total = {k_: bytes_per_pt[k_] * nv * kern[k_][1] for k_ in bytes_per_pt if k_ in kern}
+    reloc_units = int(sum(st_["relocated_grid"] for st_ in stats))
+    algo_total["reloc_apply"] = RELOC_BYTES * reloc_units
+    dom = max(algo_total, key=lambda k_: kern[k_][0])
     ms_tot, launches = kern[dom]
     avg_ms = ms_tot / max(launches, 1)
-    algo = bytes_per_pt[dom] * nv
-    achieved = algo / (avg_ms * 1e-3) / 1e9
+    algo = algo_total[dom] / max(launches, 1)
+    achieved = algo / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     step_bytes = (212.0 + 4.0 * kbar) * n
 
     # the plain form (host round trip per round, FP64-first predicates) on the same GPU
@@ -196,7 +221,7 @@ def main():
         cpu = cpu_baseline(p0 if periodic else p0[4:], cfg, periodic, a.cpu_seconds)
 
     agg = {key: int(sum(s[key] for s in stats)) for key in ("rounds", "flips", "untangle_flips", "reverts",
-                                                             "relocated", "stalled", "inverted")}
+                                                             "relocated", "relocated_grid", "stalled", "inverted")}
     pin = np.sum([s["pred_incircle"] for s in stats], axis=0)
     por = np.sum([s["pred_orient"] for s in stats], axis=0)
     out = {
@@ -222,6 +247,7 @@ def main():
                           "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
                           "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / PEAK_HBM_GBS, "kbar": kbar},
         "kernels_ms_per_launch": {k_: (v[0] / max(v[1], 1)) for k_, v in kern.items()},
+        "kernels_ms_per_step": {k_: v[0] / a.steps for k_, v in kern.items()},
         "plain_value": plain,
         "fast_vs_plain": (value / plain) if plain else None,
         "per_step_totals": agg,

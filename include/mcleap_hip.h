@@ -52,6 +52,7 @@ typedef struct {
     long long pred_orient[4];  /* orientation decided at FP32 / FP64 / exact-small / exact-big */
     long long pred_incircle[4];/* incircle decided at FP32 / FP64 / exact-small / exact-big */
     double ms;                 /* wall time of the call, host clock */
+    long long relocated_grid;  /* of `relocated`, those done by the full-grid apply launches (kernel timer 3) */
 } mcl_stats_t;
 
 /* =========================================================================
@@ -242,7 +243,8 @@ int mcl_grid_mean_occupancy(mcl_grid_t* g, float* out, void* stream);
 /* GPU time of the main kernels, from HIP event pairs recorded on the launch
  * stream around every launch while enabled (no synchronisation is added).
  * Triangulation kernels: 0 = first Lawson round over all edges,
- * 1 = orientation check of all triangles, 2 = move.
+ * 1 = orientation check of all triangles, 2 = move, 3 = relocation apply
+ * (full-grid launches), 4 = take-back apply (full-grid launches).
  * Grid kernels: 0 = FRNN count pass, 1 = FRNN fill pass, 2 = grid build. */
 int mcl_tri_kernel_timer(mcl_tri_t* t, int enable);
 int mcl_tri_kernel_timer_read(mcl_tri_t* t, int which, double* total_ms, long long* launches);

@@ -48,6 +48,7 @@ class Stats(ctypes.Structure):
         ("pred_orient", ctypes.c_longlong * 4),
         ("pred_incircle", ctypes.c_longlong * 4),
         ("ms", ctypes.c_double),
+        ("relocated_grid", ctypes.c_longlong),
     ]
 
     def as_dict(self):

@@ -57,6 +57,8 @@ enum StatSlot : int {
     kStIncircle0, kStIncircle1, kStIncircle2, kStIncircle3,
     kStRelocated, kStRelocFailed,
     kStRelWalk, kStRelStar, kStRelStuck, kStRelPutBack,  // reasons of the failures
+    kStRelDirect,  // relocations done by moving the vertex inside its own star
+    kStRelocGrid,  // relocations done by the full-grid apply launches (timed by timer 3)
     kStCount
 };
 

@@ -780,7 +780,7 @@ __device__ __forceinline__ void reloc_apply_body(const DMesh& m, const int* __re
                                                  const int* __restrict__ r_tt, const int* __restrict__ r_tt2,
                                                  const unsigned long long* __restrict__ claim,
                                                  const float2* __restrict__ tgt, int* __restrict__ next_list, int* c_next,
-                                                 unsigned (&loc)[6], int i0, int stride) {
+                                                 unsigned (&loc)[7], int i0, int stride) {
     for (int i = i0; i < n; i += stride) {
         const int v = vlist[i];
         const int tt = r_tt[v], tt2 = r_tt2[v];
@@ -921,10 +921,12 @@ __global__ void __launch_bounds__(kThreads) k_reloc_apply(DMesh m, const int* __
                                                           const float2* __restrict__ tgt, int* __restrict__ next_list,
                                                           int* __restrict__ cnt_next, int slot_next,
                                                           unsigned long long* __restrict__ stats) {
-    unsigned loc[6] = {0, 0, 0, 0, 0, 0};  // relocated, given up, and the reasons
+    unsigned loc[7] = {0, 0, 0, 0, 0, 0, 0};  // relocated, given up, the reasons, direct moves
     reloc_apply_body<kF32>(m, vlist, cnt[slot], round, r_tt, r_tt2, claim, tgt, next_list, &cnt_next[slot_next], loc,
                            blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
     flush_stats(loc, stats, kStRelocated);
+    const unsigned grid_loc[1] = {loc[0]};
+    flush_stats(grid_loc, stats, kStRelocGrid);
 }
 
 // relocation rounds finished by one workgroup (locate / claim / apply with
@@ -937,7 +939,7 @@ __global__ void __launch_bounds__(1024) k_reloc_tail(DMesh m, int* __restrict__ 
                                                      const float2* __restrict__ tgt, unsigned long long* __restrict__ stats,
                                                      int* __restrict__ out) {
     __shared__ int s_n, s_next;
-    unsigned loc[6] = {0, 0, 0, 0, 0, 0};
+    unsigned loc[7] = {0, 0, 0, 0, 0, 0, 0};
     int cur = cur0, r = 0;
     if (threadIdx.x == 0) s_n = n0;
     __syncthreads();
@@ -1244,6 +1246,37 @@ __global__ void __launch_bounds__(kThreads) k_closest(DMesh m, const int* __rest
     }
 }
 
+// Structural check of the half-edge mesh with every index range-checked
+// before it is followed (so a broken mesh is reported, never dereferenced):
+// next^3 = id, one face per cycle, dest(h) = org(next(h)), op(h) =
+// org(next(next(h))), t.he and v.he point back.  err[0] += violations.
+__global__ void __launch_bounds__(kThreads) k_validate(DMesh m, int* __restrict__ err) {
+    int bad = 0;
+    const int stride = gridDim.x * blockDim.x;
+    for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < m.nhe; h += stride) {
+        const int n1 = m.he_next[h];
+        if ((unsigned)n1 >= (unsigned)m.nhe) { bad++; continue; }
+        const int n2 = m.he_next[n1];
+        if ((unsigned)n2 >= (unsigned)m.nhe) { bad++; continue; }
+        if (m.he_next[n2] != h) bad++;
+        const int t = m.he_t[h];
+        if (t < -1 || t >= m.nt || m.he_t[n1] != t) bad++;
+        const int v = m.he_v[h];
+        if ((unsigned)v >= (unsigned)m.nv) bad++;
+        if (m.he_v[h ^ 1] != m.he_v[n1]) bad++;
+        if (t >= 0 && m.he_op[h] != m.he_v[n2]) bad++;
+    }
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m.nt; t += stride) {
+        const int e = m.t_he[t];
+        if ((unsigned)e >= (unsigned)m.nhe || m.he_t[e] != t) bad++;
+    }
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += stride) {
+        const int e = m.v_he[v];
+        if ((unsigned)e >= (unsigned)m.nhe || m.he_v[e] != v) bad++;
+    }
+    if (bad) atomicAdd(err, bad);
+}
+
 // ring entry c (0-based) of vertex v in the legacy layouts of k_ring_legacy
 __device__ __forceinline__ int ring_at(const int* ring, int layout, int stride, int nv, int v, int c) {
     return layout == 0 ? ring[(size_t)(c + 1) * nv + v] : ring[(size_t)(c + 1) + (size_t)stride * v];
@@ -1361,7 +1394,7 @@ struct mcl_tri {
     int* deg = nullptr;
     int* scan_tmp = nullptr;
     int* h_cnt = nullptr;  // pinned
-    KTimer timers[3];      // Lawson full round, check all, move
+    KTimer timers[5];      // Lawson full round, check all, move, relocation apply, take-back apply
     int tail_threshold = 8192;  // queue length below which one workgroup finishes the rounds
     int revert_tail = 1024;     // bad-triangle count below which one workgroup finishes the take-backs
     int reloc_tail = 1024;      // relocation list length below which one workgroup finishes the rounds
@@ -1682,6 +1715,7 @@ static int stats_end(mcl_tri* t, hipStream_t s, mcl_stats_t* st, std::chrono::st
     st->reverts = (long long)h[kStReverts];
     st->stalled = (long long)h[kStStalled] + (long long)h[kStRelocFailed];
     st->relocated = (long long)h[kStRelocated];
+    st->relocated_grid = (long long)h[kStRelocGrid];
     st->inverted = (long long)h[kStInverted];
     for (int k = 0; k < 4; k++) {
         st->pred_orient[k] = (long long)h[kStOrient0 + k];
@@ -1793,6 +1827,24 @@ static bool trace_on() {
     return on;
 }
 
+// MCL_VALIDATE set: check the mesh structure after every phase of a step and
+// stop with MCL_ERR_TOPOLOGY (naming the phase) at the first violation
+static bool validate_on() { return getenv("MCL_VALIDATE") != nullptr; }
+static int validate(mcl_tri* t, hipStream_t s, const char* phase) {
+    if (!validate_on()) return MCL_OK;
+    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->tail_out + 7, 1, 0);
+    hipLaunchKernelGGL(k_validate, dim3(t->grid_blocks), dim3(kThreads), 0, s, t->m, t->tail_out + 7);
+    MCL_HIP(hipGetLastError());
+    int e = 0;
+    MCL_HIP(hipMemcpyAsync(&e, t->tail_out + 7, sizeof(int), hipMemcpyDeviceToHost, s));
+    MCL_HIP(hipStreamSynchronize(s));
+    if (e) {
+        fprintf(stderr, "mcleap: mesh structure broken after %s (%d violations)\n", phase, e);
+        return MCL_ERR_TOPOLOGY;
+    }
+    return MCL_OK;
+}
+
 static int repair_rounds(mcl_tri* t, hipStream_t s, bool f32, int* cur, int rep_stamp, int* nrep) {
     DMesh& m = t->m;
     *nrep = 0;
@@ -1875,6 +1927,7 @@ static int revert_passes(mcl_tri* t, hipStream_t s, bool f32, int* cur, int mv_s
         const int nxt = 1 - *cur;
         MCL_TRY(zero_slots(t, s, kCnBad + nxt, 1));
         if (nr > 0) {
+            t->timers[4].mark(s);
             if (f32)
                 hipLaunchKernelGGL((k_revert_apply<true>), dim3(blocks_for(nr)), dim3(kThreads), 0, s, m, t->rev, t->cnt,
                                    (int)kCnRev, t->acc, t->v_fexp, t->v_soft, mv_stamp, t->allrev, t->cnt,
@@ -1883,6 +1936,7 @@ static int revert_passes(mcl_tri* t, hipStream_t s, bool f32, int* cur, int mv_s
                 hipLaunchKernelGGL((k_revert_apply<false>), dim3(blocks_for(nr)), dim3(kThreads), 0, s, m, t->rev, t->cnt,
                                    (int)kCnRev, t->acc, t->v_fexp, t->v_soft, mv_stamp, t->allrev, t->cnt,
                                    (int)kCnRev + 2, t->stats);
+            t->timers[4].mark(s);
         }
         if (nu > 0) {
             *undos += nu;
@@ -1939,6 +1993,7 @@ static int relocate_rounds(mcl_tri* t, hipStream_t s, bool f32, int n, int* slot
                                (int)kCnPend, t->tgt, t->r_tt, t->r_tt2, R);
         hipLaunchKernelGGL(k_reloc_claim, dim3(blocks_for(n)), dim3(kThreads), 0, s, m, t->pend[cur], t->cnt, (int)kCnPend,
                            R, t->r_tt, t->r_tt2, t->claim);
+        t->timers[3].mark(s);
         if (f32)
             hipLaunchKernelGGL((k_reloc_apply<true>), dim3(blocks_for(n)), dim3(kThreads), 0, s, m, t->pend[cur], t->cnt,
                                (int)kCnPend, R, t->r_tt, t->r_tt2, t->claim, t->tgt, t->pend[nxt], t->cnt, (int)kCnMove,
@@ -1947,8 +2002,10 @@ static int relocate_rounds(mcl_tri* t, hipStream_t s, bool f32, int n, int* slot
             hipLaunchKernelGGL((k_reloc_apply<false>), dim3(blocks_for(n)), dim3(kThreads), 0, s, m, t->pend[cur], t->cnt,
                                (int)kCnPend, R, t->r_tt, t->r_tt2, t->claim, t->tgt, t->pend[nxt], t->cnt, (int)kCnMove,
                                t->stats);
+        t->timers[3].mark(s);
         MCL_HIP(hipGetLastError());
         MCL_TRY(read_counts(t, s));
+        MCL_TRY(validate(t, s, "a relocation round"));
         if (trace_on()) fprintf(stderr, "  [reloc %d] list %d retry %d\n", r, n, t->h_cnt[kCnMove]);
         n = t->h_cnt[kCnMove];
         cur = nxt;
@@ -2001,9 +2058,12 @@ static int step_impl(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     const long long inverted0 = t->h_cnt[kCnBad];
     int cur = 0, nrep = 0, nundo = 0, reloc_rounds = 0;
     bool stuck = false;
+    MCL_TRY(validate(t, s, "move"));
     MCL_TRY(repair_rounds(t, s, f32, &cur, mv_stamp, &nrep));
+    MCL_TRY(validate(t, s, "repair rounds"));
     const int bad_after = t->h_cnt[kCnBad + cur];
     MCL_TRY(revert_passes(t, s, f32, &cur, mv_stamp, mv_stamp, &nundo, &stuck));
+    MCL_TRY(validate(t, s, "take-back passes"));
     if (stuck) return MCL_ERR_TOPOLOGY;  // cannot happen: see k_revert_mark
     const int nrev = t->h_cnt[kCnRev + 2];
     int rc = MCL_OK;
@@ -2012,14 +2072,16 @@ static int step_impl(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         MCL_HIP(hipMemcpyAsync(t->pend[0], t->allrev, sizeof(int) * nrev, hipMemcpyDeviceToDevice, s));
         int slot = 0;
         rc = relocate_rounds(t, s, f32, nrev, &slot, &reloc_rounds);
+        if (rc == MCL_OK) rc = validate(t, s, "relocation rounds");
     }
     long long rounds = 0;
     if (rc == MCL_OK) rc = run_rounds<0>(t, true, variant, 0, s, &rounds);
+    if (rc == MCL_OK) rc = validate(t, s, "Lawson rounds");
     if (trace) {
         unsigned long long h[kStCount];
         if (hipMemcpy(h, t->stats, sizeof(h), hipMemcpyDeviceToHost) == hipSuccess)
-            fprintf(stderr, "[step] relocation failures: walk %llu star %llu removal %llu put-back %llu\n", h[kStRelWalk],
-                    h[kStRelStar], h[kStRelStuck], h[kStRelPutBack]);
+            fprintf(stderr, "[step] relocation failures: walk %llu star %llu removal %llu put-back %llu; direct moves %llu\n",
+                    h[kStRelWalk], h[kStRelStar], h[kStRelStuck], h[kStRelPutBack], h[kStRelDirect]);
     }
     if (trace)
         fprintf(stderr, "[step] inverted %lld repairs %d bad_after_repair %d undone %d reverted %d reloc_rounds %d lawson %lld\n",
@@ -2120,7 +2182,7 @@ extern "C" int mcl_tri_kernel_timer(mcl_tri_t* t, int enable) {
     return MCL_OK;
 }
 extern "C" int mcl_tri_kernel_timer_read(mcl_tri_t* t, int which, double* total_ms, long long* launches) {
-    if (!t || which < 0 || which > 2) return MCL_ERR_INPUT;
+    if (!t || which < 0 || which > 4) return MCL_ERR_INPUT;
     double ms = t->timers[which].total_ms(launches);
     if (total_ms) *total_ms = ms;
     return MCL_OK;

@@ -276,7 +276,8 @@ class DeviceTriangulation:
         check(lib().mcl_tri_kernel_timer(self._h, int(enable)), "mcl_tri_kernel_timer")
 
     def kernel_timer_read(self, which):
-        """(total GPU ms, launches) of kernel class `which` (0 Lawson full round, 1 check all, 2 move)"""
+        """(total GPU ms, launches) of kernel class `which` (0 Lawson full round, 1 check all, 2 move,
+        3 relocation apply, 4 take-back apply)"""
         ms, n = ctypes.c_double(), ctypes.c_longlong()
         check(lib().mcl_tri_kernel_timer_read(self._h, int(which), ctypes.byref(ms), ctypes.byref(n)),
               "mcl_tri_kernel_timer_read")
