@@ -85,6 +85,7 @@ __device__ __forceinline__ int agg_inc(int* ctr) {
 }
 
 // ---------------------------------------------------- statistics helpers
+constexpr int kStatShards = 64;
 // per-thread counters reduced in LDS, one global atomic per block and slot
 template <int N>
 __device__ __forceinline__ void flush_stats(const unsigned (&loc)[N], unsigned long long* stats, int slot0) {
@@ -96,7 +97,67 @@ __device__ __forceinline__ void flush_stats(const unsigned (&loc)[N], unsigned l
     for (int k = 0; k < N; k++)
         if (loc[k]) atomicAdd(&acc[k], loc[k]);
     __syncthreads();
-    if (threadIdx.x < N && acc[threadIdx.x]) atomicAdd(&stats[slot0 + threadIdx.x], (unsigned long long)acc[threadIdx.x]);
+    // kStatShards copies of every counter, picked by block: thousands of
+    // blocks flushing the same counter would otherwise serialise on it
+    if (threadIdx.x < N && acc[threadIdx.x])
+        atomicAdd(&stats[(size_t)(slot0 + threadIdx.x) * kStatShards + (blockIdx.x & (kStatShards - 1))],
+                  (unsigned long long)acc[threadIdx.x]);
+}
+
+// Appends, for one tile of the block (one item per thread), the items with
+// pred set to the list counted by *ctr: a wave ballot, one LDS atomic per
+// wave and ONE global atomic per block tile (instead of one per wave), so a
+// heavily appended list does not serialise on its counter.  Every thread of
+// the block must make the same number of calls.  Returns the item's slot, or
+// -1 when pred is false.  s_tile: 2 ints of LDS.
+__device__ __forceinline__ int block_append(int* ctr, bool pred, int* s_tile) {
+    const unsigned long long mask = __ballot(pred);
+    const int lane = __lane_id();
+    const int wcount = __popcll(mask);
+    const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+    int woff = 0;
+    if (lane == 0 && wcount) woff = atomicAdd(&s_tile[0], wcount);
+    woff = __shfl(woff, 0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int tot = s_tile[0];
+        s_tile[1] = tot ? atomicAdd(ctr, tot) : 0;
+        s_tile[0] = 0;
+    }
+    __syncthreads();
+    return pred ? s_tile[1] + woff + rank : -1;
+}
+
+// Like block_append for `count` (0..n) entries per thread: returns the first
+// of this thread's `count` consecutive slots.  Same uniform-call rule.
+__device__ __forceinline__ int block_reserve(int* ctr, int count, int* s_tile) {
+    const int lane = __lane_id();
+    int x = count;  // inclusive prefix sum over the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    const int wtot = __shfl(x, 63);
+    int woff = 0;
+    if (lane == 63 && wtot) woff = atomicAdd(&s_tile[0], wtot);
+    woff = __shfl(woff, 63);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int tot = s_tile[0];
+        s_tile[1] = tot ? atomicAdd(ctr, tot) : 0;
+        s_tile[0] = 0;
+    }
+    __syncthreads();
+    return s_tile[1] + woff + (x - count);
+}
+
+// loc[st]++ with a run-time st would put loc in scratch memory (a memory
+// round trip per predicate); constant indices keep the counters in registers
+template <int N>
+__device__ __forceinline__ void bump(unsigned (&loc)[N], int st) {
+#pragma unroll
+    for (int k = 0; k < N; k++) loc[k] += (st == k) ? 1u : 0u;
 }
 
 __device__ __forceinline__ void tri_verts(const DMesh& m, int t, int& a, int& b, int& c) {
@@ -203,16 +264,22 @@ __global__ void k_integrate(DMesh m, const float* __restrict__ dt, const float2*
 template <bool kF32>
 __global__ void __launch_bounds__(kThreads) k_check_all(DMesh m, int* __restrict__ bad, int* __restrict__ cnt, int slot,
                                                         unsigned long long* __restrict__ stats) {
+    __shared__ int s_tile[2];
+    if (threadIdx.x == 0) s_tile[0] = 0;
+    __syncthreads();
     unsigned loc[4] = {0, 0, 0, 0};
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m.nt; t += gridDim.x * blockDim.x) {
-        int a, b, c, st;
-        tri_verts(m, t, a, b, c);
-        int s = orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st);
-        loc[st]++;
-        if (s <= 0) {
-            int k = agg_inc(&cnt[slot]);
-            if (bad) bad[k] = t;
+    for (int base = blockIdx.x * blockDim.x; base < m.nt; base += gridDim.x * blockDim.x) {
+        const int t = base + threadIdx.x;
+        bool is_bad = false;
+        if (t < m.nt) {
+            int a, b, c, st;
+            tri_verts(m, t, a, b, c);
+            const int s = orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st);
+            bump(loc, st);
+            is_bad = s <= 0;
         }
+        const int k = block_append(&cnt[slot], is_bad, s_tile);
+        if (is_bad && bad) bad[k] = t;
     }
     flush_stats(loc, stats, kStOrient0);
 }
@@ -385,7 +452,7 @@ __device__ __forceinline__ void undo_apply_body(const DMesh& m, const int* __res
             int a, b, c, st;
             tri_verts(m, ts[k], a, b, c);
             int sg = orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st);
-            loc[st]++;
+            bump(loc, st);
             if (sg <= 0) bad_out[agg_inc(c_out)] = ts[k];
         }
     }
@@ -414,8 +481,8 @@ __global__ void __launch_bounds__(kThreads) k_filter_bad(DMesh m, const int* __r
         const int t = bad[i];
         int a, b, c, st;
         tri_verts(m, t, a, b, c);
-        if (orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st) > 0) { loc[st]++; continue; }
-        loc[st]++;
+        if (orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st) > 0) { bump(loc, st); continue; }
+        bump(loc, st);
         bad_out[agg_inc(&cnt[slot_out])] = t;
         const int h0 = m.t_he[t], h1 = m.he_next[h0], h2 = m.he_next[h1];
         const int es[3] = {h0 >> 1, h1 >> 1, h2 >> 1};
@@ -473,7 +540,7 @@ __device__ __forceinline__ void check_ring_body(const DMesh& m, const int* __res
                 int a, b, c, st;
                 tri_verts(m, t, a, b, c);
                 int s = orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st);
-                loc[st]++;
+                bump(loc, st);
                 if (s <= 0) bad[agg_inc(c_out)] = t;
             }
             h = m.he_next[h ^ 1];
@@ -671,12 +738,14 @@ __device__ int locate_point(const DMesh& m, float2 p, int t, int* on, unsigned s
     unsigned r = seed * 0x9E3779B9u + 0x7F4A7C15u;
     for (int step = 0; step < kWalkCap; step++) {
         const int h0 = m.t_he[t];
-        const int hs[3] = {h0, m.he_next[h0], m.he_next[m.he_next[h0]]};
+        const int h1 = m.he_next[h0], h2 = m.he_next[h1];
         r ^= r << 13; r ^= r >> 17; r ^= r << 5;
         const int k0 = (int)(r % 3u);
         int zero = -1, cross = -1;
+#pragma unroll
         for (int k = 0; k < 3; k++) {
-            const int h = hs[(k0 + k) % 3];
+            const int idx = (k0 + k) % 3;  // selects, not an indexed array: stays in registers
+            const int h = idx == 0 ? h0 : (idx == 1 ? h1 : h2);
             const float2 a = m.pos[m.he_v[h]], b = m.pos[m.he_v[h ^ 1]];
             if ((a.x == p.x && a.y == p.y) || (b.x == p.x && b.y == p.y)) return -2;
             if (h == came) continue;  // p is on our side of the edge we crossed
@@ -980,14 +1049,21 @@ __global__ void __launch_bounds__(kThreads) k_round_test(DMesh m, const int* __r
     const int r3 = round % 3;
     const int n = kAll ? n_all : cnt[kCnCand + r3];
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt[kCnCand + (r3 + 2) % 3] = 0;
+    __shared__ int s_tile[2];
+    if (threadIdx.x == 0) s_tile[0] = 0;
+    __syncthreads();
     unsigned loc[4] = {0, 0, 0, 0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int e = kAll ? i : cand[i];
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const int i = base + threadIdx.x;
+        bool want = false;
+        int e = -1;
+        do {  // one item; `break` leaves it without a flip request
+        if (i >= n) break;
+        e = kAll ? i : cand[i];
         const int2 tt = reinterpret_cast<const int2*>(m.he_t)[e];
-        if (tt.x < 0 || tt.y < 0) continue;
+        if (tt.x < 0 || tt.y < 0) break;
         const int2 vv = reinterpret_cast<const int2*>(m.he_v)[e];
         const int2 oo = reinterpret_cast<const int2*>(m.he_op)[e];
-        bool want = false;
         if (kKind == 0) {
             int st;
             // inCircle(org(2e), op(2e+1), org(2e+1), op(2e)) > 0 (kernels.cuh:287-292)
@@ -995,21 +1071,21 @@ __global__ void __launch_bounds__(kThreads) k_round_test(DMesh m, const int* __r
             if (s == kNeedBig) {
                 if (atomicExch(&e_stamp[e], round + 1) != round + 1)
                     cand_next[agg_inc(&cnt[kCnCand + (r3 + 1) % 3])] = e;
-                continue;
+                break;
             }
-            loc[st]++;
+            bump(loc, st);
             want = s > 0;
         } else if (kKind == 2) {
-            if (t_rep[tt.x] == rep_stamp || t_rep[tt.y] == rep_stamp) continue;
+            if (t_rep[tt.x] == rep_stamp || t_rep[tt.y] == rep_stamp) break;
             // repair: one triangle not strictly CCW, the other strictly CCW, and
             // both triangles after the flip strictly CCW (B-step of
             // kernels.cuh:329-331 including its degenerate on-edge case)
             int st;
             float2 p0 = m.pos[vv.x], p1 = m.pos[vv.y], a0 = m.pos[oo.x], a1 = m.pos[oo.y];
-            bool up0 = orient<kF32>(p0, p1, a0, m.periodic, &st) > 0; loc[st]++;
-            bool up1 = orient<kF32>(p1, p0, a1, m.periodic, &st) > 0; loc[st]++;
+            bool up0 = orient<kF32>(p0, p1, a0, m.periodic, &st) > 0; bump(loc, st);
+            bool up1 = orient<kF32>(p1, p0, a1, m.periodic, &st) > 0; bump(loc, st);
             if (up0 != up1) {
-                bool n0 = orient<kF32>(a0, a1, p1, m.periodic, &st) > 0; loc[st]++;
+                bool n0 = orient<kF32>(a0, a1, p1, m.periodic, &st) > 0; bump(loc, st);
                 bool n1 = n0 && orient<kF32>(a1, a0, p0, m.periodic, &st) > 0;
                 want = n0 && n1;
             }
@@ -1018,8 +1094,8 @@ __global__ void __launch_bounds__(kThreads) k_round_test(DMesh m, const int* __r
             // t0 = (org, dst, op0), t1 = (dst, org, op1)
             int st;
             float2 p0 = m.pos[vv.x], p1 = m.pos[vv.y], a0 = m.pos[oo.x], a1 = m.pos[oo.y];
-            int s0 = orient<kF32>(p0, p1, a0, m.periodic, &st); loc[st]++;
-            int s1 = orient<kF32>(p1, p0, a1, m.periodic, &st); loc[st]++;
+            int s0 = orient<kF32>(p0, p1, a0, m.periodic, &st); bump(loc, st);
+            int s1 = orient<kF32>(p1, p0, a1, m.periodic, &st); bump(loc, st);
             bool ccw0 = s0 >= 0, ccw1 = s1 >= 0;
             if (ccw0 != ccw1) {
                 // upright triangle U, inverted triangle I
@@ -1043,8 +1119,10 @@ __global__ void __launch_bounds__(kThreads) k_round_test(DMesh m, const int* __r
             unsigned long long key = claim_key(round, e);
             atomicMax(&claim[tt.x], key);
             atomicMax(&claim[tt.y], key);
-            wants[agg_inc(&cnt[kCnWants + r3])] = e;
         }
+        } while (false);
+        const int k = block_append(&cnt[kCnWants + r3], want, s_tile);
+        if (want) wants[k] = e;
     }
     flush_stats(loc, stats, kKind == 0 ? kStIncircle0 : kStOrient0);
 }
@@ -1062,22 +1140,38 @@ __global__ void __launch_bounds__(kThreads) k_round_flip(DMesh m, int* __restric
     const int n = cnt[kCnWants + r3];
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt[kCnWants + (r3 + 1) % 3] = 0;
     const int slot = kCnCand + (r3 + 1) % 3, stamp = round + 1;
+    __shared__ int s_tile[2];
+    if (threadIdx.x == 0) s_tile[0] = 0;
+    __syncthreads();
     unsigned loc[1] = {0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int e = wants[i];
-        const int2 tt = reinterpret_cast<const int2*>(m.he_t)[e];
-        const unsigned long long key = claim_key(round, e);
-        if (claim[tt.x] == key && claim[tt.y] == key) {
-            int h2, h3, h4, h5;
-            flip_soa(m, e, h2, h3, h4, h5);
-            loc[0]++;
-            enqueue_edge(h2 >> 1, stamp, e_stamp, cand_next, cnt, slot);
-            enqueue_edge(h3 >> 1, stamp, e_stamp, cand_next, cnt, slot);
-            enqueue_edge(h4 >> 1, stamp, e_stamp, cand_next, cnt, slot);
-            enqueue_edge(h5 >> 1, stamp, e_stamp, cand_next, cnt, slot);
-        } else {
-            enqueue_edge(e, stamp, e_stamp, cand_next, cnt, slot);
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const int i = base + threadIdx.x;
+        // up to four edges to queue for the next round (the quad's, after a
+        // flip; the edge itself, after a lost claim), each once per round
+        int q0 = -1, q1 = -1, q2 = -1, q3 = -1;
+        if (i < n) {
+            const int e = wants[i];
+            const int2 tt = reinterpret_cast<const int2*>(m.he_t)[e];
+            const unsigned long long key = claim_key(round, e);
+            if (claim[tt.x] == key && claim[tt.y] == key) {
+                int h2, h3, h4, h5;
+                flip_soa(m, e, h2, h3, h4, h5);
+                loc[0]++;
+                q0 = h2 >> 1; q1 = h3 >> 1; q2 = h4 >> 1; q3 = h5 >> 1;
+            } else {
+                q0 = e;
+            }
+            if (q0 >= 0 && atomicExch(&e_stamp[q0], stamp) == stamp) q0 = -1;
+            if (q1 >= 0 && atomicExch(&e_stamp[q1], stamp) == stamp) q1 = -1;
+            if (q2 >= 0 && atomicExch(&e_stamp[q2], stamp) == stamp) q2 = -1;
+            if (q3 >= 0 && atomicExch(&e_stamp[q3], stamp) == stamp) q3 = -1;
         }
+        const int nq = (q0 >= 0) + (q1 >= 0) + (q2 >= 0) + (q3 >= 0);
+        int p = block_reserve(&cnt[slot], nq, s_tile);
+        if (q0 >= 0) cand_next[p++] = q0;
+        if (q1 >= 0) cand_next[p++] = q1;
+        if (q2 >= 0) cand_next[p++] = q2;
+        if (q3 >= 0) cand_next[p++] = q3;
     }
     flush_stats(loc, stats, stat_slot);
 }
@@ -1114,7 +1208,7 @@ __global__ void __launch_bounds__(1024) k_round_tail(DMesh m, int* __restrict__ 
             int st;
             if (kKind == 0) {
                 int s = incircle<true, true>(m.pos[vv.x], m.pos[oo.y], m.pos[vv.y], m.pos[oo.x], m.periodic, &st);
-                loc[st]++;
+                bump(loc, st);
                 want = s > 0;
             } else {
                 float2 p0 = m.pos[vv.x], p1 = m.pos[vv.y], a0 = m.pos[oo.x], a1 = m.pos[oo.y];
@@ -1478,7 +1572,7 @@ extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv,
     A(m.pos, nv); A(m.he_next, nhe); A(m.he_v, nhe); A(m.he_t, nhe); A(m.he_op, nhe);
     A(m.v_he, nv); A(m.v_pidx, nv); A(m.t_he, nt);
     A(t->claim, nt); A(t->e_stamp, m.ne); A(t->cand[0], m.ne); A(t->cand[1], m.ne); A(t->wants, m.ne);
-    A(t->cnt, kCnCount); A(t->rounds_done, 1); A(t->stats, kStCount); A(t->scalar, 4);
+    A(t->cnt, kCnCount); A(t->rounds_done, 1); A(t->stats, (size_t)kStCount * kStatShards); A(t->scalar, 4);
     A(t->acc, nv); A(t->tgt, nv); A(t->v_mv, nv); A(t->v_rev, nv); A(t->v_fexp, nv); A(t->t_stamp, nt);
     A(t->bad[0], nt); A(t->bad[1], nt); A(t->rev, nv); A(t->allrev, nv); A(t->pend[0], nv); A(t->pend[1], nv);
     A(t->deg, nv + 1); A(t->scan_tmp, scan_scratch_ints(nv + 1) + 16);
@@ -1700,13 +1794,24 @@ static int run_rounds(mcl_tri* t, bool all, int variant, int max_rounds, hipStre
 }
 
 static void stats_begin(mcl_tri* t, hipStream_t s) {
-    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, t->stats, (int)kStCount, 0ull);
+    hipLaunchKernelGGL(k_set_u64, dim3(4), dim3(256), 0, s, t->stats, (int)(kStCount * kStatShards), 0ull);
+}
+// the statistics counters summed over their shards
+static int stats_read(mcl_tri* t, hipStream_t s, unsigned long long (&h)[kStCount]) {
+    std::vector<unsigned long long> raw((size_t)kStCount * kStatShards);
+    MCL_HIP(hipMemcpyAsync(raw.data(), t->stats, sizeof(unsigned long long) * raw.size(), hipMemcpyDeviceToHost, s));
+    MCL_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < kStCount; k++) {
+        unsigned long long a = 0;
+        for (int j = 0; j < kStatShards; j++) a += raw[(size_t)k * kStatShards + j];
+        h[k] = a;
+    }
+    return MCL_OK;
 }
 static int stats_end(mcl_tri* t, hipStream_t s, mcl_stats_t* st, std::chrono::steady_clock::time_point t0) {
     if (!st) return MCL_OK;
     unsigned long long h[kStCount];
-    MCL_HIP(hipMemcpyAsync(h, t->stats, sizeof(h), hipMemcpyDeviceToHost, s));
-    MCL_HIP(hipStreamSynchronize(s));
+    MCL_TRY(stats_read(t, s, h));
     long long rounds = st->rounds;
     memset(st, 0, sizeof(*st));
     st->rounds = rounds;
@@ -2079,7 +2184,7 @@ static int step_impl(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     if (rc == MCL_OK) rc = validate(t, s, "Lawson rounds");
     if (trace) {
         unsigned long long h[kStCount];
-        if (hipMemcpy(h, t->stats, sizeof(h), hipMemcpyDeviceToHost) == hipSuccess)
+        if (stats_read(t, s, h) == MCL_OK)
             fprintf(stderr, "[step] relocation failures: walk %llu star %llu removal %llu put-back %llu; direct moves %llu\n",
                     h[kStRelWalk], h[kStRelStar], h[kStRelStuck], h[kStRelPutBack], h[kStRelDirect]);
     }
