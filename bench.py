@@ -33,6 +33,7 @@ sys.path.insert(0, HERE)
 PEAK_HBM_GBS = 8000.0
 BYTES = {"move": 16, "check": 32, "lawson": 56, "grid_build": 43, "frnn_count": 13}
 RELOC_BYTES = 648  # per relocated vertex: star read 3*6*16 + 7 positions * 8 + rewrite 3*6*16
+STAR_BYTES = 390   # per vertex: own star 28 + 6 neighbour stars 6*28 + ~19 candidate positions 152 + new star 28 + pos 8
 
 
 def _args():
@@ -179,7 +180,8 @@ def main():
 
     # per-kernel times from the live event timers
     kern = {}
-    for name, which in (("lawson", 0), ("check", 1), ("move", 2), ("reloc_apply", 3), ("revert_apply", 4)):
+    for name, which in (("lawson", 0), ("check", 1), ("move", 2), ("reloc_apply", 3), ("revert_apply", 4),
+                        ("star", 5)):
         ms, cnt = dt.kernel_timer_read(which)
         kern[name] = (ms, cnt)
     kbar = 0.0
@@ -195,6 +197,7 @@ def main():
     algo_total = {k_: bytes_per_pt[k_] * nv * kern[k_][1] for k_ in bytes_per_pt if k_ in kern}
     reloc_units = int(sum(st_["relocated_grid"] for st_ in stats))
     algo_total["reloc_apply"] = RELOC_BYTES * reloc_units
+    algo_total["star"] = STAR_BYTES * nv * kern["star"][1]
     dom = max(algo_total, key=lambda k_: kern[k_][0])
     ms_tot, launches = kern[dom]
     avg_ms = ms_tot / max(launches, 1)
@@ -221,7 +224,8 @@ def main():
         cpu = cpu_baseline(p0 if periodic else p0[4:], cfg, periodic, a.cpu_seconds)
 
     agg = {key: int(sum(s[key] for s in stats)) for key in ("rounds", "flips", "untangle_flips", "reverts",
-                                                             "relocated", "relocated_grid", "stalled", "inverted")}
+                                                             "relocated", "relocated_grid", "stalled", "inverted",
+                                                             "star_retries", "fallbacks")}
     pin = np.sum([s["pred_incircle"] for s in stats], axis=0)
     por = np.sum([s["pred_orient"] for s in stats], axis=0)
     out = {

@@ -53,6 +53,8 @@ typedef struct {
     long long pred_incircle[4];/* incircle decided at FP32 / FP64 / exact-small / exact-big */
     double ms;                 /* wall time of the call, host clock */
     long long relocated_grid;  /* of `relocated`, those done by the full-grid apply launches (kernel timer 3) */
+    long long star_retries;    /* torus step: vertices whose star was recomputed from the 3-ring */
+    long long fallbacks;       /* torus step: 1 when the stars did not settle and the flip-based step ran */
 } mcl_stats_t;
 
 /* =========================================================================
@@ -159,6 +161,13 @@ typedef struct {
 } mcl_step_params_t;
 
 /* One kinetic step: displace every non-corner vertex by the synthetic motion,
+ * and restore the exact Delaunay triangulation.  On the torus (variant 0)
+ * every vertex recomputes its Delaunay star from its 2-ring in the previous
+ * mesh (exact incircle on the inverted candidates), inconsistent stars are
+ * recomputed from the 3-ring, and the half-edge arrays are rebuilt from the
+ * stars (half-edge and triangle ids are renumbered; vertex ids are kept);
+ * if the stars do not settle, the flip-based step below runs instead.
+ * The flip-based step (box, and variant 1 everywhere): 
  * keep the mesh valid (repair flips; moves that would still invert a triangle
  * are taken back, and those vertices are removed and re-inserted at their
  * targets), and flip back to the exact Delaunay triangulation.  Replaces the
@@ -244,7 +253,8 @@ int mcl_grid_mean_occupancy(mcl_grid_t* g, float* out, void* stream);
  * stream around every launch while enabled (no synchronisation is added).
  * Triangulation kernels: 0 = first Lawson round over all edges,
  * 1 = orientation check of all triangles, 2 = move, 3 = relocation apply
- * (full-grid launches), 4 = take-back apply (full-grid launches).
+ * (full-grid launches), 4 = take-back apply (full-grid launches),
+ * 5 = star recomputation (torus step).
  * Grid kernels: 0 = FRNN count pass, 1 = FRNN fill pass, 2 = grid build. */
 int mcl_tri_kernel_timer(mcl_tri_t* t, int enable);
 int mcl_tri_kernel_timer_read(mcl_tri_t* t, int which, double* total_ms, long long* launches);

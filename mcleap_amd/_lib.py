@@ -49,6 +49,8 @@ class Stats(ctypes.Structure):
         ("pred_incircle", ctypes.c_longlong * 4),
         ("ms", ctypes.c_double),
         ("relocated_grid", ctypes.c_longlong),
+        ("star_retries", ctypes.c_longlong),
+        ("fallbacks", ctypes.c_longlong),
     ]
 
     def as_dict(self):

@@ -1,0 +1,541 @@
+// mcl_star.hpp -- kinetic step on the torus by star recomputation.
+//
+// After the move, each vertex recomputes its star (its Delaunay neighbours in
+// CCW order) from candidates taken in the previous mesh, then the stars are
+// checked for consistency and the half-edge arrays are rebuilt from them.
+//
+// Star of v from a candidate set C: under the inversion p -> (p - v)/|p - v|^2
+// circles through v become lines, so u is a Delaunay neighbour of v (an empty
+// circle through v and u exists) exactly when u' is a vertex of the convex hull
+// of C' (v inside).  The hull is wrapped counter-clockwise from the nearest
+// candidate (the farthest inverted point, always a hull vertex); the turn test
+// orient(a', b', c') has the sign of incircle(a, b, c, v) (divide the rows of
+// the lifted determinant by |p - v|^2), so every decision is the library's
+// exact incircle with the symbolic tie rule.  One wavefront handles one
+// vertex: each lane finds, for the candidates it owns (two at most), the most
+// clockwise other candidate as seen from it (its hull successor when it is a
+// hull vertex), and the star is then traced through those successors from the
+// nearest candidate.
+//
+// If every triangle (v, s_i, s_i+1) of every star also appears in the stars of
+// its other two vertices, the stars form a triangulation whose every edge is
+// locally Delaunay with respect to its two triangles, i.e. the (unique)
+// Delaunay triangulation.  Vertices whose star is inconsistent are recomputed
+// from the 2-rings of both the previous mesh and the stars found so far (a new
+// neighbour more than two edges away in the previous mesh is two edges away
+// in the new stars of its other neighbours); if that does not settle it, the
+// caller falls back to the flip-based step.
+#pragma once
+#include "mcl_geom.hpp"
+#include "mcl_internal.hpp"
+
+namespace mcl {
+
+constexpr int kStarCap = 24;    // stored neighbours per vertex
+constexpr int kCandMax = 128;   // candidates per vertex: two per lane
+constexpr int kHashSlots = 256; // per-wave LDS hash set for candidate de-duplication
+constexpr int kStarWaves = 4;   // waves per block in k_star
+constexpr int kStarPasses = 5;  // star computations per step before the flip-based fallback
+constexpr int kStarAux = 16;    // star-step counters
+
+// stars of the current mesh: CCW rotation from v.he (h <- next(next(h))^1)
+__global__ void __launch_bounds__(256) k_stars_from_mesh(DMesh m, int* __restrict__ deg, int* __restrict__ nbr,
+                                                         int* __restrict__ fail) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
+        const int h0 = m.v_he[v];
+        int h = h0, d = 0;
+        do {
+            if (d == kStarCap) break;
+            nbr[(size_t)v * kStarCap + d++] = m.he_v[h ^ 1];
+            h = m.he_next[m.he_next[h]] ^ 1;
+        } while (h != h0);
+        if (h != h0) {
+            deg[v] = -1;
+            atomicAdd(fail, 1);
+        } else {
+            deg[v] = d;
+        }
+    }
+}
+
+// inserts c into the LDS hash set and, when it is new, appends it to the
+// candidate list (insertion order, so the list is in ring order)
+__device__ __forceinline__ void cand_insert(int* tab, int* list, int* count, int c) {
+    unsigned h = ((unsigned)c * 2654435761u) >> 24;  // 8 bits
+    for (int probe = 0; probe < kHashSlots; probe++) {
+        const int old = atomicCAS(&tab[h], -1, c);
+        if (old == -1) {
+            const int k = atomicAdd(count, 1);
+            if (k < kCandMax) list[k] = c;
+            return;
+        }
+        if (old == c) return;
+        h = (h + 1) & (kHashSlots - 1);
+    }
+    atomicAdd(count, kCandMax + 1);  // table full: reported as too many candidates
+}
+
+// inserts the star of u (from one star array) into the candidate set of v
+__device__ __forceinline__ void insert_star(int* tab, int* list, int* count, const int* __restrict__ deg,
+                                            const int* __restrict__ nbr, int u, int v, int nv) {
+    const int du = deg[u];
+    for (int k = 0; k < du && k < kStarCap; k++) {
+        const int c = nbr[(size_t)u * kStarCap + k];
+        if (c != v && (unsigned)c < (unsigned)nv) cand_insert(tab, list, count, c);
+    }
+}
+
+constexpr int kStarMaxDepth = 5;  // deepest ring of the previous mesh searched
+
+// The FP32 stage of incircle(a, b, c, v) from precomputed rows (x, y, lift)
+// of a - v, b - v, c - v: the differences rounded to FP32 from their exact
+// values and lift = x * x + y * y in FP32, the same operations in the same
+// order as the filter inside incircle(), so the decision is identical.
+// Returns +-1, or 0 when the filter cannot decide.
+__device__ __forceinline__ int star_incircle_f32(float4 A, float4 B, float4 C) {
+#pragma clang fp contract(off)
+    const float e = 5.9604644775390625e-08f;
+    const float bdxcdy = B.x * C.y, cdxbdy = C.x * B.y;
+    const float cdxady = C.x * A.y, adxcdy = A.x * C.y;
+    const float adxbdy = A.x * B.y, bdxady = B.x * A.y;
+    const float det = A.z * (bdxcdy - cdxbdy) + B.z * (cdxady - adxcdy) + C.z * (adxbdy - bdxady);
+    const float perm = (fabsf(bdxcdy) + fabsf(cdxbdy)) * A.z + (fabsf(cdxady) + fabsf(adxcdy)) * B.z +
+                       (fabsf(adxbdy) + fabsf(bdxady)) * C.z;
+    const float bound = (10.0f + 96.0f * e) * e * perm * 1.0001f;
+    if (perm > 7.9e-31f && perm < 1e37f && (det > bound || -det > bound)) return det > 0 ? 1 : -1;
+    return 0;
+}
+
+// The FP32 stage of orient(v, a, b) from the rows of a - v and b - v (the
+// bound of the orient filter, whose differences carry the same relative
+// error as these rows).  Returns +-1, or 0 when the filter cannot decide.
+__device__ __forceinline__ int star_orient_f32(float4 A, float4 B) {
+#pragma clang fp contract(off)
+    const float e32 = 5.9604644775390625e-08f;
+    const float dl = A.x * B.y, dr = A.y * B.x, det = dl - dr;
+    const float ds = fabsf(dl) + fabsf(dr);
+    const float bound = (3.0f + 16.0f * e32) * e32 * ds * 1.0001f;
+    if (ds > 7.9e-31f && (det > bound || -det > bound)) return det > 0 ? 1 : -1;
+    return 0;
+}
+
+// Recomputes the stars of the listed vertices (all vertices when vlist is
+// null) with the current positions.  Candidates: the 2-ring of v in the star
+// arrays deg_a/nbr_a (the previous mesh), widened to the depth-ring when v's
+// star in deg_b failed (v moved out of its 2-ring), and, when deg_b is given,
+// the 2-ring of v through both the previous stars and deg_b/nbr_b (the stars
+// found so far in this step).  Writes deg_out/nbr_out at row idx of the list
+// (at row v without a list); a vertex whose star cannot be computed gets
+// deg_out = -1 and is counted in fail[0] and, by reason, in fail[4] (too few
+// or too many candidates), fail[5] (undecided test), fail[6] (hull not closed)
+// or fail[7] (v not inside).  The loop over vertices advances kStarWaves
+// vertices per block iteration so that every wave of a block meets the same
+// barriers.  kFast: FP32 filter only, a test it cannot decide fails the
+// vertex (reason 5) for a later pass with the full cascade; this variant
+// carries none of the exact-arithmetic code and keeps a small register
+// footprint.
+template <bool kF32, bool kFast>
+__global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ vlist, int nlist, int depth,
+                                              const int* __restrict__ deg_a, const int* __restrict__ nbr_a,
+                                              const int* __restrict__ deg_b, const int* __restrict__ nbr_b,
+                                              int* __restrict__ deg_out, int* __restrict__ nbr_out,
+                                              int* __restrict__ fail, unsigned long long* __restrict__ stats) {
+    __shared__ int s_tab[kStarWaves][kHashSlots];
+    __shared__ int s_cand[kStarWaves][kCandMax];
+    __shared__ float2 s_pos[kStarWaves][kCandMax];
+    __shared__ float4 s_row[kStarWaves][kCandMax];  // (x, y, lift) of candidate - v
+    __shared__ int s_succ[kStarWaves][kCandMax];
+    __shared__ int s_cnt[kStarWaves];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int n = vlist ? nlist : m.nv;
+    const bool periodic = m.periodic != 0;
+    const int max_depth = depth < 2 ? 2 : (depth > kStarMaxDepth ? kStarMaxDepth : depth);
+    unsigned loc[4] = {0, 0, 0, 0};
+    for (int base = blockIdx.x * kStarWaves; base < n; base += gridDim.x * kStarWaves) {
+        const int idx = base + wave;
+        const bool active = idx < n;
+        const int v = active ? (vlist ? vlist[idx] : idx) : 0;
+        int* tab = s_tab[wave];
+        int* list = s_cand[wave];
+        int* count = &s_cnt[wave];
+        // ---- candidates, de-duplicated in LDS, listed ring by ring
+        for (int k = 0; k < kHashSlots / 64; k++) tab[lane + 64 * k] = -1;
+        if (lane == 0) *count = 0;
+        __syncthreads();
+        const int db = (active && deg_b) ? deg_b[v] : 0;
+        const int my_depth = (active && deg_b && db < 0) ? max_depth : 2;
+        // first ring: lanes 0..31 from the previous mesh, 32..63 from this step's stars
+        if (active) {
+            int ni = -1;
+            if (lane < 32) {
+                if (lane < deg_a[v]) ni = nbr_a[(size_t)v * kStarCap + lane];
+            } else if (lane - 32 < db) {
+                ni = nbr_b[(size_t)v * kStarCap + lane - 32];
+            }
+            if ((unsigned)ni < (unsigned)m.nv) cand_insert(tab, list, count, ni);
+        }
+        __syncthreads();
+        // further rings: expand the entries the previous ring added
+        int lo = 0, hi = *count;
+        for (int dd = 2; dd <= max_depth; dd++) {
+            if (active && dd <= my_depth) {
+                const int top = hi < kCandMax ? hi : kCandMax;
+                for (int i = lo + lane; i < top; i += 64) {
+                    const int u = list[i];
+                    insert_star(tab, list, count, deg_a, nbr_a, u, v, m.nv);
+                    if (deg_b && dd == 2) insert_star(tab, list, count, deg_b, nbr_b, u, v, m.nv);
+                }
+            }
+            __syncthreads();
+            lo = hi;
+            hi = *count;
+        }
+        const int kc = hi;
+        // positions of the candidates and their filter rows relative to v
+        {
+#pragma clang fp contract(off)
+            const float2 pv = m.pos[v];
+            for (int j = lane; active && j < kc && j < kCandMax; j += 64) {
+                const float2 pj = m.pos[list[j]];
+                s_pos[wave][j] = pj;
+                const float x = (float)dexp_value(diff_exact(pj.x, pv.x, 0, periodic));
+                const float y = (float)dexp_value(diff_exact(pj.y, pv.y, 0, periodic));
+                s_row[wave][j] = make_float4(x, y, x * x + y * y, 0.0f);
+            }
+        }
+        __syncthreads();
+        bool ok = active && kc >= 3 && kc <= kCandMax;
+        int why = ok ? 0 : 4;  // failure reason slot in fail[] (0: none)
+        const float2 pv = m.pos[v];
+        // ---- start: the nearest candidate (largest inverted radius); a
+        // candidate on top of v makes the inversion meaningless
+        double dist = 1e300;
+        int best = kCandMax;
+        bool same = false;
+        for (int j = lane; ok && j < kc; j += 64) {
+            const float2 pj = s_pos[wave][j];
+            same |= pj.x == pv.x && pj.y == pv.y;
+            const double dx = dexp_value(diff_exact(pj.x, pv.x, 0, periodic));
+            const double dy = dexp_value(diff_exact(pj.y, pv.y, 0, periodic));
+            const double dd = dx * dx + dy * dy;
+            if (dd < dist) { dist = dd; best = j; }
+        }
+        if (__ballot(same) && ok) { ok = false; why = 4; }
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double od = __shfl_xor(dist, off);
+            const int ob = __shfl_xor(best, off);
+            if (od < dist || (od == dist && ob < best)) { dist = od; best = ob; }
+        }
+        const int start = best;
+        if (start >= kc && ok) { ok = false; why = 4; }
+        // ---- successors: for each owned candidate j, the most clockwise other
+        // candidate as seen from j' (b beats a when orient(j', a', b') < 0,
+        // i.e. incircle(j, a, b, v) < 0); -1 when a test was undecided
+        if (ok) {
+            for (int j = lane; j < kc; j += 64) {
+                const float4 rj = s_row[wave][j];
+                int a = j == 0 ? 1 : 0;
+                float4 ra = s_row[wave][a];
+                bool undecided = false;
+                unsigned fast = 0;
+                for (int b = a + 1; b < kc; b++) {
+                    const float4 rb = s_row[wave][b];
+                    if (b == j) continue;
+                    int sg = kF32 ? star_incircle_f32(rj, ra, rb) : 0;
+                    if (sg != 0) {
+                        fast++;
+                    } else if (kFast) {
+                        undecided = true;
+                    } else {
+                        int st = 0;
+                        sg = incircle<kF32, false>(s_pos[wave][j], s_pos[wave][a], s_pos[wave][b], pv, periodic, &st);
+                        bump(loc, st == kStageExactBig ? 3 : st);
+                        if (sg == kNeedBig || sg == 0) undecided = true;
+                    }
+                    if (sg < 0) { a = b; ra = rb; }
+                }
+                loc[0] += fast;
+                s_succ[wave][j] = undecided ? -1 : a;
+            }
+        }
+        __syncthreads();
+        // ---- trace the hull from the start through the successors
+        int my_slot = -1, d = 0;
+        if (ok) {
+            int cur = start;
+            bool closed = false;
+            for (;;) {
+                if (d == kStarCap) break;
+                if (lane == d) my_slot = cur;
+                d++;
+                const int nx = s_succ[wave][cur];
+                if (nx < 0) { why = 5; break; }
+                if (nx == start) { closed = true; break; }
+                cur = nx;
+            }
+            if (!closed || d < 3) {
+                ok = false;
+                if (!why) why = 6;
+            }
+        }
+        // every triangle (v, s_i, s_i+1) strictly CCW (v inside the hull)
+        if (ok) {
+            const int nl = (lane + 1 < d) ? lane + 1 : 0;
+            const int nslot = __shfl(my_slot, nl);
+            bool bad = false, undecided = false;
+            if (lane < d) {
+                int o = kF32 ? star_orient_f32(s_row[wave][my_slot], s_row[wave][nslot]) : 0;
+                if (o == 0 && !kFast) {
+                    int st;
+                    o = orient<kF32>(pv, s_pos[wave][my_slot], s_pos[wave][nslot], periodic, &st);
+                    bad = o <= 0;
+                } else {
+                    undecided = o == 0;
+                    bad = o < 0;
+                }
+            }
+            if (__ballot(bad)) { ok = false; why = 7; }
+            else if (__ballot(undecided)) { ok = false; why = 5; }
+        }
+        if (active) {
+            const size_t row = vlist ? (size_t)idx : (size_t)v;
+            if (ok) {
+                if (lane < d) nbr_out[row * kStarCap + lane] = s_cand[wave][my_slot];
+                if (lane == 0) deg_out[row] = d;
+            } else if (lane == 0) {
+                deg_out[row] = -1;
+                atomicAdd(fail, 1);
+                atomicAdd(fail + why, 1);
+            }
+        }
+    }
+    flush_stats(loc, stats, kStIncircle0);
+}
+
+// position of x in the star of u (deg du), -1 if absent; the row is read as
+// 16-byte vectors (rows are kStarCap = 24 ints, 96 bytes, 16-byte aligned)
+__device__ __forceinline__ int star_find(const int* __restrict__ nbr, int u, int du, int x) {
+    const int4* r = reinterpret_cast<const int4*>(nbr + (size_t)u * kStarCap);
+    int pos = -1;
+    for (int k = 0; k < du; k += 4) {
+        const int4 q = r[k >> 2];
+        if (q.w == x && k + 3 < du) pos = k + 3;
+        if (q.z == x && k + 2 < du) pos = k + 2;
+        if (q.y == x && k + 1 < du) pos = k + 1;
+        if (q.x == x) pos = k;
+    }
+    return pos;
+}
+
+// Every triangle (v, u, w) of v's star must read (u, w, v) in u's star.
+// Checks the *nlist listed vertices (all when vlist is null).  Vertices that fail
+// (or whose star failed) are appended to `list`, with their star neighbours
+// when mark_nbrs is set.  Without a list the degree sum of the valid stars is
+// added to *sum_deg.
+__global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restrict__ vlist, const int* __restrict__ nlist,
+                                                    const int* __restrict__ deg, const int* __restrict__ nbr,
+                                                    int mark_nbrs, int* __restrict__ mark, int stamp,
+                                                    int* __restrict__ list, int* __restrict__ count,
+                                                    long long* __restrict__ sum_deg) {
+    __shared__ int s_tile[2];
+    if (threadIdx.x == 0) s_tile[0] = 0;
+    __syncthreads();
+    long long my_sum = 0;
+    const int n = vlist ? *nlist : nv;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const int idx = base + threadIdx.x;
+        const bool active = idx < n;
+        const int v = active ? (vlist ? vlist[idx] : idx) : 0;
+        const int d = active ? deg[v] : 3;
+        bool bad = active && d < 3;
+        if (active && !bad) {
+            if (!vlist) my_sum += d;
+            const int* row = nbr + (size_t)v * kStarCap;
+            const int first = row[0];
+            int u = first;
+            for (int i = 0; i < d && !bad; i++) {
+                const int w = (i + 1 == d) ? first : row[i + 1];
+                const int du = deg[u];
+                if (du < 3) {
+                    bad = true;
+                } else {
+                    const int j = star_find(nbr, u, du, w);
+                    if (j < 0 || nbr[(size_t)u * kStarCap + (j + 1 == du ? 0 : j + 1)] != v) bad = true;
+                }
+                u = w;
+            }
+        }
+        // claim v (and its neighbours) once per stamp, then append the claimed ones
+        unsigned claimed = 0;  // bit 0: v, bit 1 + i: neighbour i
+        int nclaim = 0;
+        if (bad) {
+            if (atomicExch(&mark[v], stamp) != stamp) { claimed |= 1u; nclaim++; }
+            for (int i = 0; mark_nbrs && i < d; i++)
+                if (atomicExch(&mark[nbr[(size_t)v * kStarCap + i]], stamp) != stamp) {
+                    claimed |= 2u << i;
+                    nclaim++;
+                }
+        }
+        int at = block_reserve(count, nclaim, s_tile);
+        if (claimed & 1u) list[at++] = v;
+        for (int i = 0; i < d && i < kStarCap; i++)
+            if (claimed & (2u << i)) list[at++] = nbr[(size_t)v * kStarCap + i];
+    }
+    if (my_sum) atomicAdd((unsigned long long*)sum_deg, (unsigned long long)my_sum);
+}
+
+// Copies the recomputed stars (rows by list position) to their vertices,
+// adds the change of the valid-star degree sum to *sum_deg, and lists for
+// re-checking each recomputed vertex with its previous star neighbours (the
+// only vertices whose consistency the new stars can change: a vertex that
+// has a recomputed vertex in its star without being in that vertex's
+// previous star was inconsistent, hence recomputed itself).
+__global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vlist, int nlist,
+                                                      const int* __restrict__ tdeg, const int* __restrict__ tnbr,
+                                                      int* __restrict__ deg, int* __restrict__ nbr,
+                                                      int* __restrict__ mark, int stamp, int* __restrict__ rlist,
+                                                      int* __restrict__ rcount, long long* __restrict__ sum_deg) {
+    __shared__ int s_tile[2];
+    if (threadIdx.x == 0) s_tile[0] = 0;
+    __syncthreads();
+    long long delta = 0;
+    for (int base = blockIdx.x * blockDim.x; base < nlist; base += gridDim.x * blockDim.x) {
+        const int idx = base + threadIdx.x;
+        const bool active = idx < nlist;
+        const int v = active ? vlist[idx] : 0;
+        const int od = active ? deg[v] : 0;
+        unsigned claimed = 0;
+        int nclaim = 0;
+        if (active) {
+            if (atomicExch(&mark[v], stamp) != stamp) { claimed |= 1u; nclaim++; }
+            for (int i = 0; i < od; i++)
+                if (atomicExch(&mark[nbr[(size_t)v * kStarCap + i]], stamp) != stamp) {
+                    claimed |= 2u << i;
+                    nclaim++;
+                }
+        }
+        int at = block_reserve(rcount, nclaim, s_tile);
+        if (claimed & 1u) rlist[at++] = v;
+        for (int i = 0; i < od && i < kStarCap; i++)
+            if (claimed & (2u << i)) rlist[at++] = nbr[(size_t)v * kStarCap + i];
+        if (active) {
+            const int nd = tdeg[idx];
+            delta += (nd >= 3 ? nd : 0) - (od >= 3 ? od : 0);
+            for (int k = 0; k < nd; k++) nbr[(size_t)v * kStarCap + k] = tnbr[(size_t)idx * kStarCap + k];
+            deg[v] = nd;
+        }
+    }
+    if (delta) atomicAdd((unsigned long long*)sum_deg, (unsigned long long)delta);
+}
+
+// per vertex: edges it owns (neighbours with a larger id) and triangles it
+// owns (both other vertices larger)
+__global__ void __launch_bounds__(256) k_star_counts(int nv, const int* __restrict__ deg, const int* __restrict__ nbr,
+                                                     int* __restrict__ ecnt, int* __restrict__ tcnt) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
+        const int d = deg[v];
+        const int* row = nbr + (size_t)v * kStarCap;
+        int ne = 0, nt = 0;
+        const int first = row[0];
+        int u = first;
+        for (int i = 0; i < d; i++) {
+            const int w = (i + 1 == d) ? first : row[i + 1];
+            ne += u > v;
+            nt += (u > v && w > v);
+            u = w;
+        }
+        ecnt[v] = ne;
+        tcnt[v] = nt;
+    }
+}
+
+// Half-edge numbering of the rebuilt mesh: edge {o, x} (o < x) is edge
+// eoff[o] + (rank of x among o's larger neighbours, in star order), its
+// half-edge o -> x even and x -> o odd; triangle (o, x, y) with o its
+// smallest vertex is toff[o] + (rank among o's owned triangles).
+//
+// Pass 1: the id of every outgoing half-edge of v (into hid, one row per
+// vertex like the stars), with its origin and opposite vertex, and v.he.
+__global__ void __launch_bounds__(256) k_star_hid(DMesh m, const int* __restrict__ deg, const int* __restrict__ nbr,
+                                                  const int* __restrict__ eoff, int* __restrict__ hid) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
+        const int d = deg[v];
+        const int* row = nbr + (size_t)v * kStarCap;
+        const int first = row[0];
+        const int ev = eoff[v];
+        int rank = 0, u = first;
+        for (int i = 0; i < d; i++) {
+            const int w = (i + 1 == d) ? first : row[i + 1];
+            int h;
+            if (u > v) {
+                h = 2 * (ev + rank++);
+            } else {  // owned by u: rank of v among u's larger neighbours
+                const int du = deg[u];
+                const int4* r = reinterpret_cast<const int4*>(nbr + (size_t)u * kStarCap);
+                int ru = 0;
+                bool seen = false;
+                for (int k = 0; k < du && !seen; k += 4) {
+                    const int4 q = r[k >> 2];
+                    const int e[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        if (seen || k + c >= du) continue;
+                        if (e[c] == v) seen = true;
+                        else ru += e[c] > u;
+                    }
+                }
+                h = 2 * (eoff[u] + ru) + 1;
+            }
+            hid[(size_t)v * kStarCap + i] = h;
+            if ((unsigned)h < (unsigned)m.nhe) {
+                m.he_v[h] = v;
+                m.he_op[h] = w;
+                if (i == 0) m.v_he[v] = h;
+            }
+            u = w;
+        }
+    }
+}
+
+// Pass 2: next of every incoming half-edge of v (the triangle (v, u_i, u_i+1)
+// has u_i+1 -> v followed by v -> u_i), and the triangles v owns: t.he and
+// the face of their two half-edges at v.
+__global__ void __launch_bounds__(256) k_star_link(DMesh m, const int* __restrict__ deg, const int* __restrict__ nbr,
+                                                   const int* __restrict__ hid, const int* __restrict__ toff) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
+        const int d = deg[v];
+        const int* row = nbr + (size_t)v * kStarCap;
+        const int* hr = hid + (size_t)v * kStarCap;
+        const int first = row[0], h_first = hr[0];
+        int t = toff[v];
+        int u = first, hu = h_first;
+        for (int i = 0; i < d; i++) {
+            const int w = (i + 1 == d) ? first : row[i + 1];
+            const int hw = (i + 1 == d) ? h_first : hr[i + 1];
+            const int hin = hw ^ 1;  // w -> v
+            if ((unsigned)hin < (unsigned)m.nhe) m.he_next[hin] = hu;
+            if (u > v && w > v) {
+                if ((unsigned)t < (unsigned)m.nt && (unsigned)hu < (unsigned)m.nhe && (unsigned)hin < (unsigned)m.nhe) {
+                    m.t_he[t] = hu;
+                    m.he_t[hu] = t;
+                    m.he_t[hin] = t;
+                }
+                t++;
+            }
+            u = w;
+            hu = hw;
+        }
+    }
+}
+
+// Pass 3: the face of the third half-edge of every triangle
+__global__ void __launch_bounds__(256) k_star_face3(DMesh m) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m.nt; t += gridDim.x * blockDim.x) {
+        const int h = m.t_he[t];
+        if ((unsigned)h >= (unsigned)m.nhe) continue;
+        const int h2 = m.he_next[h];
+        if ((unsigned)h2 < (unsigned)m.nhe) m.he_t[h2] = t;
+    }
+}
+
+}  // namespace mcl

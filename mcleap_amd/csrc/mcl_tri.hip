@@ -1342,7 +1342,7 @@ __global__ void __launch_bounds__(kThreads) k_closest(DMesh m, const int* __rest
 
 // Structural check of the half-edge mesh with every index range-checked
 // before it is followed (so a broken mesh is reported, never dereferenced):
-// next^3 = id, one face per cycle, dest(h) = org(next(h)), op(h) =
+// next^3 = id on triangles, one face per cycle, dest(h) = org(next(h)), op(h) =
 // org(next(next(h))), t.he and v.he point back.  err[0] += violations.
 __global__ void __launch_bounds__(kThreads) k_validate(DMesh m, int* __restrict__ err) {
     int bad = 0;
@@ -1352,8 +1352,8 @@ __global__ void __launch_bounds__(kThreads) k_validate(DMesh m, int* __restrict_
         if ((unsigned)n1 >= (unsigned)m.nhe) { bad++; continue; }
         const int n2 = m.he_next[n1];
         if ((unsigned)n2 >= (unsigned)m.nhe) { bad++; continue; }
-        if (m.he_next[n2] != h) bad++;
         const int t = m.he_t[h];
+        if (t >= 0 && m.he_next[n2] != h) bad++;  // triangles are 3-cycles (the outer face of the box is a 4-cycle)
         if (t < -1 || t >= m.nt || m.he_t[n1] != t) bad++;
         const int v = m.he_v[h];
         if ((unsigned)v >= (unsigned)m.nv) bad++;
@@ -1448,6 +1448,8 @@ __global__ void __launch_bounds__(kThreads) k_frnn_graph(DMesh m, float rr, int 
 
 }  // namespace mcl
 
+#include "mcl_star.hpp"
+
 // =========================================================================
 // host side
 // =========================================================================
@@ -1484,11 +1486,24 @@ struct mcl_tri {
     int* r_tt = nullptr;               // nv: relocation target triangle
     int* r_tt2 = nullptr;              // nv: second triangle when the target is on an edge
     int* tail_out = nullptr;           // 8: results of the single-workgroup loops
+    // star recomputation (torus kinetic step)
+    int* star_deg[2] = {nullptr, nullptr};  // nv each
+    int* star_nbr[2] = {nullptr, nullptr};  // nv * kStarCap each
+    int* star_tdeg = nullptr;               // nv: recomputed stars by list position
+    int* star_tnbr = nullptr;               // nv * kStarCap
+    int star_cur = 0;                       // stars of the current mesh
+    bool stars_valid = false;               // star_*[star_cur] describe the current mesh
+    int* s_ecnt = nullptr;                  // nv + 1
+    int* s_tcnt = nullptr;                  // nv + 1
+    int* s_mark = nullptr;                  // nv
+    int* s_list = nullptr;                  // nv
+    int* s_rlist = nullptr;                 // nv: vertices to re-check
+    int* s_aux = nullptr;                   // kStarAux counters (see star_step)
     // one ring
     int* deg = nullptr;
     int* scan_tmp = nullptr;
     int* h_cnt = nullptr;  // pinned
-    KTimer timers[5];      // Lawson full round, check all, move, relocation apply, take-back apply
+    KTimer timers[6];      // Lawson full round, check all, move, relocation apply, take-back apply, star recomputation
     int tail_threshold = 8192;  // queue length below which one workgroup finishes the rounds
     int revert_tail = 1024;     // bad-triangle count below which one workgroup finishes the take-backs
     int reloc_tail = 1024;      // relocation list length below which one workgroup finishes the rounds
@@ -1527,7 +1542,9 @@ static int tri_free(mcl_tri* t) {
     void* ptrs[] = {t->m.pos, t->m.he_next, t->m.he_v, t->m.he_t, t->m.he_op, t->m.v_he, t->m.v_pidx, t->m.t_he,
                     t->claim, t->e_stamp, t->cand[0], t->cand[1], t->wants, t->cnt, t->rounds_done, t->stats,
                     t->scalar, t->acc, t->tgt, t->v_mv, t->v_rev, t->v_fexp, t->t_stamp, t->bad[0], t->bad[1],
-                    t->rev, t->allrev, t->pend[0], t->pend[1], t->deg, t->scan_tmp, t->v_touch, t->t_rep, t->t_rep_e, t->e_undo, t->undo, t->stuck, t->touched, t->r_tt, t->r_tt2, t->v_soft, t->tail_out};
+                    t->rev, t->allrev, t->pend[0], t->pend[1], t->deg, t->scan_tmp, t->v_touch, t->t_rep, t->t_rep_e, t->e_undo, t->undo, t->stuck, t->touched, t->r_tt, t->r_tt2, t->v_soft, t->tail_out,
+                    t->star_deg[0], t->star_deg[1], t->star_nbr[0], t->star_nbr[1], t->s_ecnt, t->s_tcnt,
+                    t->s_mark, t->s_list, t->s_aux, t->star_tdeg, t->star_tnbr, t->s_rlist};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (t->h_cnt) (void)hipHostFree(t->h_cnt);
@@ -1578,6 +1595,13 @@ extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv,
     A(t->deg, nv + 1); A(t->scan_tmp, scan_scratch_ints(nv + 1) + 16);
     A(t->v_touch, nv); A(t->t_rep, nt); A(t->t_rep_e, nt); A(t->e_undo, m.ne); A(t->undo, m.ne);
     A(t->stuck, nt); A(t->touched, nv); A(t->r_tt, nv); A(t->r_tt2, nv); A(t->v_soft, nv); A(t->tail_out, 8);
+    if (periodic) {
+        A(t->star_deg[0], nv); A(t->star_deg[1], nv);
+        A(t->star_nbr[0], (size_t)nv * kStarCap); A(t->star_nbr[1], (size_t)nv * kStarCap);
+        A(t->star_tdeg, nv); A(t->star_tnbr, (size_t)nv * kStarCap);
+        A(t->s_ecnt, nv + 1); A(t->s_tcnt, nv + 1); A(t->s_mark, nv); A(t->s_list, nv); A(t->s_aux, kStarAux);
+        A(t->s_rlist, nv);
+    }
 #undef A
     if (rc == MCL_OK && hipHostMalloc((void**)&t->h_cnt, sizeof(int) * kCnCount) != hipSuccess) rc = MCL_ERR_HIP;
     if (rc == MCL_OK) rc = upload_aos(t, pos, v2, he4, t1);
@@ -1593,7 +1617,8 @@ extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv,
         hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(m.ne)), dim3(kThreads), 0, 0, t->e_undo, m.ne, 0);
         hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(nt)), dim3(kThreads), 0, 0, t->t_stamp, nt, 0);
         hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, 0, t->cnt, (int)kCnCount, 0);
-        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, 0, t->stats, (int)kStCount, 0ull);
+        hipLaunchKernelGGL(k_set_u64, dim3(4), dim3(256), 0, 0, t->stats, (int)(kStCount * kStatShards), 0ull);
+        if (t->s_mark) hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(nv)), dim3(kThreads), 0, 0, t->s_mark, nv, 0);
         if (hipDeviceSynchronize() != hipSuccess) rc = MCL_ERR_HIP;
     }
     if (rc != MCL_OK) { tri_free(t); delete t; return rc; }
@@ -1624,6 +1649,7 @@ extern "C" void* mcl_tri_pos_ptr(mcl_tri_t* t) { return t ? (void*)t->m.pos : nu
 
 extern "C" int mcl_tri_upload(mcl_tri_t* t, const float* pos, const int* v2, const int* he4, const int* t1) {
     if (!t) return MCL_ERR_INPUT;
+    t->stars_valid = false;
     return upload_aos(t, pos, v2, he4, t1);
 }
 
@@ -1832,6 +1858,7 @@ static int stats_end(mcl_tri* t, hipStream_t s, mcl_stats_t* st, std::chrono::st
 
 extern "C" int mcl_tri_delonize(mcl_tri_t* t, int variant, int max_rounds, mcl_stats_t* st, void* stream) {
     if (!t) return MCL_ERR_INPUT;
+    t->stars_valid = false;
     hipStream_t s = (hipStream_t)stream;
     auto t0 = std::chrono::steady_clock::now();
     stats_begin(t, s);
@@ -1870,6 +1897,7 @@ extern "C" int mcl_tri_count_non_delaunay(mcl_tri_t* t, long long* out, void* st
 
 extern "C" int mcl_tri_untangle(mcl_tri_t* t, int max_rounds, mcl_stats_t* st, void* stream) {
     if (!t) return MCL_ERR_INPUT;
+    t->stars_valid = false;
     hipStream_t s = (hipStream_t)stream;
     auto t0 = std::chrono::steady_clock::now();
     stats_begin(t, s);
@@ -2137,6 +2165,7 @@ static int relocate_rounds(mcl_tri* t, hipStream_t s, bool f32, int n, int* slot
 //                re-inserted at their targets
 //   Lawson    -- flip rounds to the exact Delaunay triangulation
 static int step_impl(mcl_tri* t, const mcl_step_params_t* p, const float2* disp, mcl_stats_t* st, void* stream) {
+    t->stars_valid = false;  // flips and relocations change the topology
     hipStream_t s = (hipStream_t)stream;
     DMesh& m = t->m;
     auto t0 = std::chrono::steady_clock::now();
@@ -2200,16 +2229,173 @@ static int step_impl(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     return rc != MCL_OK ? rc : rc2;
 }
 
+// stars of the current mesh into star_*[star_cur] (after any topology change
+// made by the flip-based paths)
+static int ensure_stars(mcl_tri* t, hipStream_t s) {
+    if (t->stars_valid) return MCL_OK;
+    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux, kStarAux, 0);
+    hipLaunchKernelGGL(k_stars_from_mesh, dim3(blocks_for(t->m.nv)), dim3(256), 0, s, t->m, t->star_deg[t->star_cur],
+                       t->star_nbr[t->star_cur], t->s_aux);
+    MCL_HIP(hipGetLastError());
+    int fail = 0;
+    MCL_HIP(hipMemcpyAsync(&fail, t->s_aux, sizeof(int), hipMemcpyDeviceToHost, s));
+    MCL_HIP(hipStreamSynchronize(s));
+    if (fail) return MCL_ERR_CAPACITY;  // a degree above kStarCap
+    t->stars_valid = true;
+    return MCL_OK;
+}
+
+// Kinetic step on the torus by star recomputation (see mcl_star.hpp).
+// Returns MCL_OK with *done = false when the stars could not be settled; the
+// positions are then back at the start of the step and the mesh untouched.
+static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp, mcl_stats_t* st, hipStream_t s,
+                     bool* done) {
+    DMesh& m = t->m;
+    *done = false;
+    if (ensure_stars(t, s) != MCL_OK) return MCL_OK;
+    const int cur = t->star_cur, nxt = 1 - cur;
+    const int mv_stamp = ++t->stamp;
+    t->timers[2].mark(s);
+    hipLaunchKernelGGL(k_move_motion, dim3(blocks_for(m.nv)), dim3(kThreads), 0, s, m, t->acc, t->tgt, t->v_mv, t->v_fexp,
+                       mv_stamp, *p, disp);
+    t->timers[2].mark(s);
+    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux, kStarAux, 0);
+    const int star_blocks = (int)std::min<long long>(((long long)m.nv + kStarWaves - 1) / kStarWaves, 4LL * t->grid_blocks);
+    t->timers[5].mark(s);
+    hipLaunchKernelGGL((k_star<true, true>), dim3(star_blocks), dim3(256), 0, s, m, (const int*)nullptr, 0, 2,
+                       (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur], (const int*)nullptr,
+                       (const int*)nullptr, t->star_deg[nxt], t->star_nbr[nxt], t->s_aux, t->stats);
+    t->timers[5].mark(s);
+    MCL_HIP(hipGetLastError());
+    // s_aux: [0] failed stars, [1] listed vertices, [2..3] degree sum of the
+    // valid stars, [4..7] failures by reason, [8] vertices to re-check
+    long long retried = 0;
+    int passes = 1;
+    bool settled = false;
+    int aux[kStarAux];
+    int nbad = 0;
+    for (int pass = 0; pass < kStarPasses; pass++) {
+        // the first check covers every vertex and lists the inconsistent ones;
+        // later ones cover the re-check list and add the star neighbours of
+        // the inconsistent vertices (a wrong star can agree with its neighbours')
+        hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 1, 1, 0);
+        const int stamp = ++t->stamp;
+        if (pass == 0)
+            hipLaunchKernelGGL(k_star_check, dim3(blocks_for(m.nv)), dim3(256), 0, s, m.nv, (const int*)nullptr,
+                               (const int*)nullptr, (const int*)t->star_deg[nxt], (const int*)t->star_nbr[nxt], 0,
+                               t->s_mark, stamp, t->s_list, t->s_aux + 1, (long long*)(t->s_aux + 2));
+        else
+            hipLaunchKernelGGL(k_star_check, dim3(blocks_for(std::min<long long>(m.nv, (long long)nbad * (kStarCap + 1)))),
+                               dim3(256), 0, s, m.nv, (const int*)t->s_rlist, (const int*)(t->s_aux + 8),
+                               (const int*)t->star_deg[nxt], (const int*)t->star_nbr[nxt], 1, t->s_mark, stamp,
+                               t->s_list, t->s_aux + 1, (long long*)(t->s_aux + 2));
+        MCL_HIP(hipGetLastError());
+        MCL_HIP(hipMemcpyAsync(aux, t->s_aux, sizeof(aux), hipMemcpyDeviceToHost, s));
+        MCL_HIP(hipStreamSynchronize(s));
+        nbad = aux[1];
+        if (trace_on())
+            fprintf(stderr, "  [stars pass %d] inconsistent %d (failed stars so far %d: candidates %d undecided %d open %d outside %d)\n",
+                    pass, nbad, aux[0], aux[4], aux[5], aux[6], aux[7]);
+        if (nbad == 0) {
+            long long sum_deg;
+            memcpy(&sum_deg, aux + 2, sizeof(sum_deg));
+            settled = sum_deg == (long long)m.nhe;  // each half-edge once: the degree sum is 2E
+            break;
+        }
+        if (pass == kStarPasses - 1) break;
+        // recompute the vertices involved from the 2-rings in the previous
+        // mesh and in the stars found so far; a vertex whose star failed
+        // (it left its 2-ring) searches one ring deeper each pass
+        retried += nbad;
+        passes++;
+        // the FP32-filter-only kernel for the first retry, the full cascade
+        // (which also settles the tests the filter left undecided) after it
+        const dim3 rb(std::min(star_blocks, (nbad + kStarWaves - 1) / kStarWaves));
+        if (pass >= 1)
+            hipLaunchKernelGGL((k_star<true, false>), rb, dim3(256), 0, s, m, (const int*)t->s_list, nbad, 2 + pass + 1,
+                               (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur], (const int*)t->star_deg[nxt],
+                               (const int*)t->star_nbr[nxt], t->star_tdeg, t->star_tnbr, t->s_aux, t->stats);
+        else
+            hipLaunchKernelGGL((k_star<true, true>), rb, dim3(256), 0, s, m, (const int*)t->s_list, nbad, 2 + pass + 1,
+                               (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur], (const int*)t->star_deg[nxt],
+                               (const int*)t->star_nbr[nxt], t->star_tdeg, t->star_tnbr, t->s_aux, t->stats);
+        hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 8, 1, 0);
+        const int rstamp = ++t->stamp;
+        hipLaunchKernelGGL(k_star_scatter, dim3(blocks_for(nbad)), dim3(256), 0, s, (const int*)t->s_list, nbad,
+                           (const int*)t->star_tdeg, (const int*)t->star_tnbr, t->star_deg[nxt], t->star_nbr[nxt],
+                           t->s_mark, rstamp, t->s_rlist, t->s_aux + 8, (long long*)(t->s_aux + 2));
+        MCL_HIP(hipGetLastError());
+    }
+    if (settled) {
+        hipLaunchKernelGGL(k_star_counts, dim3(blocks_for(m.nv)), dim3(256), 0, s, m.nv, (const int*)t->star_deg[nxt],
+                           (const int*)t->star_nbr[nxt], t->s_ecnt, t->s_tcnt);
+        MCL_TRY(scan_exclusive(t->s_ecnt, t->s_ecnt, m.nv, t->scan_tmp, s));
+        MCL_TRY(scan_exclusive(t->s_tcnt, t->s_tcnt, m.nv, t->scan_tmp, s));
+        int tot[2];
+        MCL_HIP(hipMemcpyAsync(&tot[0], t->s_ecnt + m.nv, sizeof(int), hipMemcpyDeviceToHost, s));
+        MCL_HIP(hipMemcpyAsync(&tot[1], t->s_tcnt + m.nv, sizeof(int), hipMemcpyDeviceToHost, s));
+        MCL_HIP(hipStreamSynchronize(s));
+        settled = tot[0] == m.ne && tot[1] == m.nt;
+    }
+    if (!settled) {
+        // back to the start of the step; the mesh and its stars are untouched
+        MCL_HIP(hipMemcpyAsync(m.pos, t->acc, sizeof(float2) * m.nv, hipMemcpyDeviceToDevice, s));
+        MCL_HIP(hipStreamSynchronize(s));
+        if (trace_on()) fprintf(stderr, "  [stars] not settled: flip-based step instead\n");
+        return MCL_OK;
+    }
+    // rebuild the half-edge arrays (hid rows in star_tnbr, free after the retries)
+    hipLaunchKernelGGL(k_star_hid, dim3(blocks_for(m.nv)), dim3(256), 0, s, m, (const int*)t->star_deg[nxt],
+                       (const int*)t->star_nbr[nxt], (const int*)t->s_ecnt, t->star_tnbr);
+    hipLaunchKernelGGL(k_star_link, dim3(blocks_for(m.nv)), dim3(256), 0, s, m, (const int*)t->star_deg[nxt],
+                       (const int*)t->star_nbr[nxt], (const int*)t->star_tnbr, (const int*)t->s_tcnt);
+    hipLaunchKernelGGL(k_star_face3, dim3(blocks_for(m.nt)), dim3(256), 0, s, m);
+    MCL_HIP(hipGetLastError());
+    t->star_cur = nxt;
+    t->stars_valid = true;
+    MCL_TRY(validate(t, s, "star rebuild"));
+    if (st) {
+        st->iterations = passes;
+        st->star_retries = retried;
+    }
+    *done = true;
+    return MCL_OK;
+}
+
+static int kinetic_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp, mcl_stats_t* st, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (t->m.periodic && p->variant == 0 && t->star_deg[0]) {
+        auto t0 = std::chrono::steady_clock::now();
+        stats_begin(t, s);
+        bool done = false;
+        mcl_stats_t tmp{};
+        int rc = star_step(t, p, disp, &tmp, s, &done);
+        if (rc != MCL_OK) return rc;
+        if (done) {
+            MCL_TRY(stats_end(t, s, st, t0));
+            if (st) {
+                st->iterations = tmp.iterations;
+                st->star_retries = tmp.star_retries;
+            }
+            return MCL_OK;
+        }
+        rc = step_impl(t, p, disp, st, stream);
+        if (st) st->fallbacks = 1;
+        return rc;
+    }
+    return step_impl(t, p, disp, st, stream);
+}
+
 extern "C" int mcl_tri_step(mcl_tri_t* t, const mcl_step_params_t* p, mcl_stats_t* st, void* stream) {
     if (!t || !p) return MCL_ERR_INPUT;
-    return step_impl(t, p, nullptr, st, stream);
+    return kinetic_step(t, p, nullptr, st, stream);
 }
 
 extern "C" int mcl_tri_move_legalize(mcl_tri_t* t, const void* d_disp, int variant, mcl_stats_t* st, void* stream) {
     if (!t || !d_disp || (variant != 0 && variant != 1)) return MCL_ERR_INPUT;
     mcl_step_params_t p{};
     p.variant = variant;
-    return step_impl(t, &p, (const float2*)d_disp, st, stream);
+    return kinetic_step(t, &p, (const float2*)d_disp, st, stream);
 }
 
 // ---------------------------------------------------------------- one ring
@@ -2287,7 +2473,7 @@ extern "C" int mcl_tri_kernel_timer(mcl_tri_t* t, int enable) {
     return MCL_OK;
 }
 extern "C" int mcl_tri_kernel_timer_read(mcl_tri_t* t, int which, double* total_ms, long long* launches) {
-    if (!t || which < 0 || which > 4) return MCL_ERR_INPUT;
+    if (!t || which < 0 || which > 5) return MCL_ERR_INPUT;
     double ms = t->timers[which].total_ms(launches);
     if (total_ms) *total_ms = ms;
     return MCL_OK;
