@@ -205,6 +205,16 @@ def main():
     achieved = algo / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     step_bytes = (212.0 + 4.0 * kbar) * n
 
+    # the result of the last timed step, read through the half-edge arrays
+    # (rebuilt here from the stars on the torus): every edge exactly
+    # locally Delaunay and no inverted triangle
+    final_nd = int(dt.count_non_delaunay())
+    final_inv = int(dt.count_inverted())
+    if final_nd or final_inv:
+        print("[bench] final mesh: %d non-Delaunay edges, %d inverted triangles" % (final_nd, final_inv),
+              file=sys.stderr, flush=True)
+        sys.exit(1)
+
     # the plain form (host round trip per round, FP64-first predicates) on the same GPU
     plain = None
     if a.plain_steps > 0:
@@ -255,6 +265,7 @@ def main():
         "plain_value": plain,
         "fast_vs_plain": (value / plain) if plain else None,
         "per_step_totals": agg,
+        "final_mesh": {"non_delaunay_edges": final_nd, "inverted_triangles": final_inv},
         "predicates": {"incircle_stage_counts": pin.tolist(), "orient_stage_counts": por.tolist(),
                        "stages": ["fp32", "fp64", "exact_small", "exact_big"]},
         "cpu_baseline": cpu,

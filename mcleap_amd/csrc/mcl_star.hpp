@@ -538,4 +538,23 @@ __global__ void __launch_bounds__(256) k_star_face3(DMesh m) {
     }
 }
 
+// one-ring CSR straight from the stars, in the order of the half-edge walk
+// (clockwise from v.he, which the rebuild sets to v -> first star entry):
+// first entry, then the rest of the star backwards
+__global__ void __launch_bounds__(256) k_star_ring_deg(int nv, const int* __restrict__ deg, int* __restrict__ off) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) off[v] = deg[v];
+}
+__global__ void __launch_bounds__(256) k_star_ring_csr(int nv, const int* __restrict__ deg, const int* __restrict__ nbr,
+                                                       const int* __restrict__ off, int* __restrict__ out, int cap) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
+        const int d = deg[v];
+        const int k0 = off[v];
+        const int* row = nbr + (size_t)v * kStarCap;
+        for (int i = 0; i < d; i++) {
+            const int k = k0 + i;
+            if (k < cap) out[k] = row[i == 0 ? 0 : d - i];
+        }
+    }
+}
+
 }  // namespace mcl

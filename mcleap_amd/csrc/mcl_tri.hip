@@ -1493,6 +1493,7 @@ struct mcl_tri {
     int* star_tnbr = nullptr;               // nv * kStarCap
     int star_cur = 0;                       // stars of the current mesh
     bool stars_valid = false;               // star_*[star_cur] describe the current mesh
+    bool mesh_stale = false;                // half-edge arrays not yet rebuilt from the stars
     int* s_ecnt = nullptr;                  // nv + 1
     int* s_tcnt = nullptr;                  // nv + 1
     int* s_mark = nullptr;                  // nv
@@ -1537,6 +1538,8 @@ static int dalloc(T** p, size_t n) {
         int _r = (x);              \
         if (_r != MCL_OK) return _r; \
     } while (0)
+
+static int ensure_mesh(mcl_tri* t, hipStream_t s);
 
 static int tri_free(mcl_tri* t) {
     void* ptrs[] = {t->m.pos, t->m.he_next, t->m.he_v, t->m.he_t, t->m.he_op, t->m.v_he, t->m.v_pidx, t->m.t_he,
@@ -1650,6 +1653,7 @@ extern "C" void* mcl_tri_pos_ptr(mcl_tri_t* t) { return t ? (void*)t->m.pos : nu
 extern "C" int mcl_tri_upload(mcl_tri_t* t, const float* pos, const int* v2, const int* he4, const int* t1) {
     if (!t) return MCL_ERR_INPUT;
     t->stars_valid = false;
+    t->mesh_stale = false;
     return upload_aos(t, pos, v2, he4, t1);
 }
 
@@ -1657,6 +1661,7 @@ extern "C" int mcl_tri_download(mcl_tri_t* t, float* pos, int* v2, int* he4, int
     if (!t) return MCL_ERR_INPUT;
     DMesh& m = t->m;
     MCL_HIP(hipDeviceSynchronize());
+    MCL_TRY(ensure_mesh(t, 0));
     if (he4) {
         std::vector<int> a(m.nhe), b(m.nhe), c(m.nhe), d(m.nhe);
         MCL_HIP(hipMemcpy(a.data(), m.he_next, sizeof(int) * m.nhe, hipMemcpyDeviceToHost));
@@ -1858,8 +1863,9 @@ static int stats_end(mcl_tri* t, hipStream_t s, mcl_stats_t* st, std::chrono::st
 
 extern "C" int mcl_tri_delonize(mcl_tri_t* t, int variant, int max_rounds, mcl_stats_t* st, void* stream) {
     if (!t) return MCL_ERR_INPUT;
-    t->stars_valid = false;
     hipStream_t s = (hipStream_t)stream;
+    MCL_TRY(ensure_mesh(t, s));
+    t->stars_valid = false;
     auto t0 = std::chrono::steady_clock::now();
     stats_begin(t, s);
     long long rounds = 0;
@@ -1872,6 +1878,7 @@ extern "C" int mcl_tri_delonize(mcl_tri_t* t, int variant, int max_rounds, mcl_s
 extern "C" int mcl_tri_count_inverted(mcl_tri_t* t, long long* out, void* stream) {
     if (!t || !out) return MCL_ERR_INPUT;
     hipStream_t s = (hipStream_t)stream;
+    MCL_TRY(ensure_mesh(t, s));
     hipLaunchKernelGGL(k_prep_round, dim3(1), dim3(64), 0, s, t->cnt, 0, -1);
     hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->cnt + kCnMisc, 1, 0);
     hipLaunchKernelGGL((k_check_all<true>), dim3(t->grid_blocks), dim3(kThreads), 0, s, t->m, (int*)nullptr, t->cnt,
@@ -1885,6 +1892,7 @@ extern "C" int mcl_tri_count_inverted(mcl_tri_t* t, long long* out, void* stream
 extern "C" int mcl_tri_count_non_delaunay(mcl_tri_t* t, long long* out, void* stream) {
     if (!t || !out) return MCL_ERR_INPUT;
     hipStream_t s = (hipStream_t)stream;
+    MCL_TRY(ensure_mesh(t, s));
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, t->scalar, 1, 0ull);
     hipLaunchKernelGGL(k_count_nd, dim3(256), dim3(kThreads), 0, s, t->m, t->scalar);
     MCL_HIP(hipGetLastError());
@@ -1897,8 +1905,9 @@ extern "C" int mcl_tri_count_non_delaunay(mcl_tri_t* t, long long* out, void* st
 
 extern "C" int mcl_tri_untangle(mcl_tri_t* t, int max_rounds, mcl_stats_t* st, void* stream) {
     if (!t) return MCL_ERR_INPUT;
-    t->stars_valid = false;
     hipStream_t s = (hipStream_t)stream;
+    MCL_TRY(ensure_mesh(t, s));
+    t->stars_valid = false;
     auto t0 = std::chrono::steady_clock::now();
     stats_begin(t, s);
     long long rounds = 0;
@@ -2245,6 +2254,38 @@ static int ensure_stars(mcl_tri* t, hipStream_t s) {
     return MCL_OK;
 }
 
+// Rebuilds the half-edge arrays from the stars after star steps.  Consistent
+// stars with the degree sum 2E give E edges and, each triangle being listed
+// by its three vertices, T = 2E/3 triangles; the counts are checked again
+// here before any write.
+static int ensure_mesh(mcl_tri* t, hipStream_t s) {
+    if (!t->mesh_stale) return MCL_OK;
+    DMesh& m = t->m;
+    const int c = t->star_cur;
+    hipLaunchKernelGGL(k_star_counts, dim3(blocks_for(m.nv)), dim3(256), 0, s, m.nv, (const int*)t->star_deg[c],
+                       (const int*)t->star_nbr[c], t->s_ecnt, t->s_tcnt);
+    MCL_TRY(scan_exclusive(t->s_ecnt, t->s_ecnt, m.nv, t->scan_tmp, s));
+    MCL_TRY(scan_exclusive(t->s_tcnt, t->s_tcnt, m.nv, t->scan_tmp, s));
+    int tot[2];
+    MCL_HIP(hipMemcpyAsync(&tot[0], t->s_ecnt + m.nv, sizeof(int), hipMemcpyDeviceToHost, s));
+    MCL_HIP(hipMemcpyAsync(&tot[1], t->s_tcnt + m.nv, sizeof(int), hipMemcpyDeviceToHost, s));
+    MCL_HIP(hipStreamSynchronize(s));
+    if (tot[0] != m.ne || tot[1] != m.nt) {
+        fprintf(stderr, "mcleap: stars give %d edges and %d triangles, the mesh has %d and %d\n", tot[0], tot[1], m.ne,
+                m.nt);
+        return MCL_ERR_TOPOLOGY;
+    }
+    // hid rows in star_tnbr (scratch between star steps)
+    hipLaunchKernelGGL(k_star_hid, dim3(blocks_for(m.nv)), dim3(256), 0, s, m, (const int*)t->star_deg[c],
+                       (const int*)t->star_nbr[c], (const int*)t->s_ecnt, t->star_tnbr);
+    hipLaunchKernelGGL(k_star_link, dim3(blocks_for(m.nv)), dim3(256), 0, s, m, (const int*)t->star_deg[c],
+                       (const int*)t->star_nbr[c], (const int*)t->star_tnbr, (const int*)t->s_tcnt);
+    hipLaunchKernelGGL(k_star_face3, dim3(blocks_for(m.nt)), dim3(256), 0, s, m);
+    MCL_HIP(hipGetLastError());
+    t->mesh_stale = false;
+    return validate(t, s, "star rebuild");
+}
+
 // Kinetic step on the torus by star recomputation (see mcl_star.hpp).
 // Returns MCL_OK with *done = false when the stars could not be settled; the
 // positions are then back at the start of the step and the mesh untouched.
@@ -2326,17 +2367,6 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
                            t->s_mark, rstamp, t->s_rlist, t->s_aux + 8, (long long*)(t->s_aux + 2));
         MCL_HIP(hipGetLastError());
     }
-    if (settled) {
-        hipLaunchKernelGGL(k_star_counts, dim3(blocks_for(m.nv)), dim3(256), 0, s, m.nv, (const int*)t->star_deg[nxt],
-                           (const int*)t->star_nbr[nxt], t->s_ecnt, t->s_tcnt);
-        MCL_TRY(scan_exclusive(t->s_ecnt, t->s_ecnt, m.nv, t->scan_tmp, s));
-        MCL_TRY(scan_exclusive(t->s_tcnt, t->s_tcnt, m.nv, t->scan_tmp, s));
-        int tot[2];
-        MCL_HIP(hipMemcpyAsync(&tot[0], t->s_ecnt + m.nv, sizeof(int), hipMemcpyDeviceToHost, s));
-        MCL_HIP(hipMemcpyAsync(&tot[1], t->s_tcnt + m.nv, sizeof(int), hipMemcpyDeviceToHost, s));
-        MCL_HIP(hipStreamSynchronize(s));
-        settled = tot[0] == m.ne && tot[1] == m.nt;
-    }
     if (!settled) {
         // back to the start of the step; the mesh and its stars are untouched
         MCL_HIP(hipMemcpyAsync(m.pos, t->acc, sizeof(float2) * m.nv, hipMemcpyDeviceToDevice, s));
@@ -2344,16 +2374,12 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         if (trace_on()) fprintf(stderr, "  [stars] not settled: flip-based step instead\n");
         return MCL_OK;
     }
-    // rebuild the half-edge arrays (hid rows in star_tnbr, free after the retries)
-    hipLaunchKernelGGL(k_star_hid, dim3(blocks_for(m.nv)), dim3(256), 0, s, m, (const int*)t->star_deg[nxt],
-                       (const int*)t->star_nbr[nxt], (const int*)t->s_ecnt, t->star_tnbr);
-    hipLaunchKernelGGL(k_star_link, dim3(blocks_for(m.nv)), dim3(256), 0, s, m, (const int*)t->star_deg[nxt],
-                       (const int*)t->star_nbr[nxt], (const int*)t->star_tnbr, (const int*)t->s_tcnt);
-    hipLaunchKernelGGL(k_star_face3, dim3(blocks_for(m.nt)), dim3(256), 0, s, m);
-    MCL_HIP(hipGetLastError());
+    // the stars are the triangulation; the half-edge arrays are rebuilt from
+    // them when an entry point reads them (ensure_mesh)
     t->star_cur = nxt;
     t->stars_valid = true;
-    MCL_TRY(validate(t, s, "star rebuild"));
+    t->mesh_stale = true;
+    if (validate_on()) MCL_TRY(ensure_mesh(t, s));
     if (st) {
         st->iterations = passes;
         st->star_retries = retried;
@@ -2379,10 +2405,12 @@ static int kinetic_step(mcl_tri* t, const mcl_step_params_t* p, const float2* di
             }
             return MCL_OK;
         }
+        MCL_TRY(ensure_mesh(t, s));
         rc = step_impl(t, p, disp, st, stream);
         if (st) st->fallbacks = 1;
         return rc;
     }
+    MCL_TRY(ensure_mesh(t, s));
     return step_impl(t, p, disp, st, stream);
 }
 
@@ -2402,6 +2430,7 @@ extern "C" int mcl_tri_move_legalize(mcl_tri_t* t, const void* d_disp, int varia
 extern "C" int mcl_tri_one_ring(mcl_tri_t* t, int K, int layout, void* d_ring, void* stream) {
     if (!t || !d_ring || K < 2 || (layout != 0 && layout != 1)) return MCL_ERR_INPUT;
     hipStream_t s = (hipStream_t)stream;
+    MCL_TRY(ensure_mesh(t, s));
     hipLaunchKernelGGL(k_ring_legacy, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, K, layout, (int*)d_ring);
     MCL_HIP(hipGetLastError());
     MCL_HIP(hipStreamSynchronize(s));
@@ -2412,11 +2441,25 @@ extern "C" int mcl_tri_one_ring_csr(mcl_tri_t* t, void* d_offsets, void* d_nbrs,
     if (!t || !d_offsets) return MCL_ERR_INPUT;
     hipStream_t s = (hipStream_t)stream;
     int* off = (int*)d_offsets;
-    hipLaunchKernelGGL(k_degree, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, off);
+    // with current stars the ring comes from them (same order as the walk)
+    const bool from_stars = t->stars_valid;
+    const int c = t->star_cur;
+    if (from_stars)
+        hipLaunchKernelGGL(k_star_ring_deg, dim3(blocks_for(t->m.nv)), dim3(256), 0, s, t->m.nv,
+                           (const int*)t->star_deg[c], off);
+    else
+        hipLaunchKernelGGL(k_degree, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, off);
     MCL_TRY(scan_exclusive(off, off, t->m.nv, t->scan_tmp, s));
+    auto fill = [&]() {
+        if (from_stars)
+            hipLaunchKernelGGL(k_star_ring_csr, dim3(blocks_for(t->m.nv)), dim3(256), 0, s, t->m.nv,
+                               (const int*)t->star_deg[c], (const int*)t->star_nbr[c], (const int*)off, (int*)d_nbrs, cap);
+        else
+            hipLaunchKernelGGL(k_ring_csr, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, off, (int*)d_nbrs, cap);
+    };
     if (!total && d_nbrs && cap >= t->m.nhe) {
         // every half-edge leaves exactly one vertex: the total is nhe, no read-back needed
-        hipLaunchKernelGGL(k_ring_csr, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, off, (int*)d_nbrs, cap);
+        fill();
         MCL_HIP(hipGetLastError());
         return MCL_OK;
     }
@@ -2425,7 +2468,7 @@ extern "C" int mcl_tri_one_ring_csr(mcl_tri_t* t, void* d_offsets, void* d_nbrs,
     MCL_HIP(hipStreamSynchronize(s));
     if (total) *total = tot;
     if (d_nbrs && cap > 0) {
-        hipLaunchKernelGGL(k_ring_csr, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, off, (int*)d_nbrs, cap);
+        fill();
         MCL_HIP(hipGetLastError());
         MCL_HIP(hipStreamSynchronize(s));
     }

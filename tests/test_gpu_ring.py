@@ -87,6 +87,34 @@ def test_csr_ring(oracle, periodic):
     assert np.array_equal(off2.cpu().numpy(), off) and np.array_equal(nbr2.cpu().numpy(), nbr)
 
 
+def test_csr_ring_after_star_steps(oracle):
+    """After torus steps (which keep the triangulation as per-vertex stars and
+    rebuild the half-edge arrays only when they are read), the CSR one-ring
+    equals, entry for entry and in order, the clockwise walk from v.he over
+    the downloaded half-edge mesh."""
+    import torch
+    dt = _mesh(20000, 23, True)
+    for k in range(2):
+        st = dt.step(M.MOTION_SHEAR, 2e-4, amp=1e-3, seed=5, step=k)
+        assert st["fallbacks"] == 0
+    nv = dt.m_pos_size
+    off = torch.zeros(nv + 1, dtype=torch.int32, device="cuda")
+    nbr = torch.zeros(dt.m_he_size, dtype=torch.int32, device="cuda")
+    dt.one_ring_csr_async(off, nbr)
+    torch.cuda.synchronize()
+    off, nbr = off.cpu().numpy(), nbr.cpu().numpy()
+    pos, v, he, t = dt.download()
+    for i in range(nv):
+        h0 = h = int(v[i, 1])
+        walk = []
+        while True:
+            walk.append(int(he[h ^ 1, 1]))
+            h = int(he[h ^ 1, 0])
+            if h == h0:
+                break
+        assert nbr[off[i]:off[i + 1]].tolist() == walk, i
+
+
 @pytest.mark.parametrize("n", [10, 100, 1000])
 def test_closest_neighbors_reference_case(oracle, n):
     import torch
