@@ -34,7 +34,6 @@ namespace mcl {
 constexpr int kStarCap = 24;    // stored neighbours per vertex
 constexpr int kCandMax = 128;   // candidates per vertex: two per lane
 constexpr int kHashSlots = 256; // per-wave LDS hash set for candidate de-duplication
-constexpr int kStarWaves = 4;   // waves per block in k_star
 constexpr int kStarPasses = 5;  // star computations per step before the flip-based fallback
 constexpr int kStarAux = 16;    // star-step counters
 
@@ -58,30 +57,35 @@ __global__ void __launch_bounds__(256) k_stars_from_mesh(DMesh m, int* __restric
     }
 }
 
-// inserts c into the LDS hash set and, when it is new, appends it to the
-// candidate list (insertion order, so the list is in ring order)
+// inserts c into the LDS hash set (kH slots, a power of two) and, when it is
+// new, appends it to the candidate list (insertion order, so the list is in
+// ring order; at most kC entries are stored, the count keeps going)
+template <int kH, int kC>
 __device__ __forceinline__ void cand_insert(int* tab, int* list, int* count, int c) {
-    unsigned h = ((unsigned)c * 2654435761u) >> 24;  // 8 bits
-    for (int probe = 0; probe < kHashSlots; probe++) {
+    constexpr int kBits = kH == 128 ? 7 : 8;
+    static_assert((1 << kBits) == kH, "hash table size");
+    unsigned h = ((unsigned)c * 2654435761u) >> (32 - kBits);
+    for (int probe = 0; probe < kH; probe++) {
         const int old = atomicCAS(&tab[h], -1, c);
         if (old == -1) {
             const int k = atomicAdd(count, 1);
-            if (k < kCandMax) list[k] = c;
+            if (k < kC) list[k] = c;
             return;
         }
         if (old == c) return;
-        h = (h + 1) & (kHashSlots - 1);
+        h = (h + 1) & (kH - 1);
     }
-    atomicAdd(count, kCandMax + 1);  // table full: reported as too many candidates
+    atomicAdd(count, kC + 1);  // table full: reported as too many candidates
 }
 
 // inserts the star of u (from one star array) into the candidate set of v
+template <int kH, int kC>
 __device__ __forceinline__ void insert_star(int* tab, int* list, int* count, const int* __restrict__ deg,
                                             const int* __restrict__ nbr, int u, int v, int nv) {
     const int du = deg[u];
     for (int k = 0; k < du && k < kStarCap; k++) {
         const int c = nbr[(size_t)u * kStarCap + k];
-        if (c != v && (unsigned)c < (unsigned)nv) cand_insert(tab, list, count, c);
+        if (c != v && (unsigned)c < (unsigned)nv) cand_insert<kH, kC>(tab, list, count, c);
     }
 }
 
@@ -128,62 +132,74 @@ __device__ __forceinline__ int star_orient_f32(float4 A, float4 B) {
 // (at row v without a list); a vertex whose star cannot be computed gets
 // deg_out = -1 and is counted in fail[0] and, by reason, in fail[4] (too few
 // or too many candidates), fail[5] (undecided test), fail[6] (hull not closed)
-// or fail[7] (v not inside).  The loop over vertices advances kStarWaves
+// or fail[7] (v not inside).
+//
+// kL lanes handle one vertex (64: one per wave, up to 128 candidates; 32: two
+// per wave, up to 64 candidates, which holds nearly every 2-ring and halves
+// the instructions per vertex).  The loop over vertices advances 256 / kL
 // vertices per block iteration so that every wave of a block meets the same
 // barriers.  kFast: FP32 filter only, a test it cannot decide fails the
 // vertex (reason 5) for a later pass with the full cascade; this variant
 // carries none of the exact-arithmetic code and keeps a small register
 // footprint.
-template <bool kF32, bool kFast>
+template <bool kF32, bool kFast, int kL>
 __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ vlist, int nlist, int depth,
                                               const int* __restrict__ deg_a, const int* __restrict__ nbr_a,
                                               const int* __restrict__ deg_b, const int* __restrict__ nbr_b,
                                               int* __restrict__ deg_out, int* __restrict__ nbr_out,
                                               int* __restrict__ fail, unsigned long long* __restrict__ stats) {
-    __shared__ int s_tab[kStarWaves][kHashSlots];
-    __shared__ int s_cand[kStarWaves][kCandMax];
-    __shared__ float2 s_pos[kStarWaves][kCandMax];
-    __shared__ float4 s_row[kStarWaves][kCandMax];  // (x, y, lift) of candidate - v
-    __shared__ int s_succ[kStarWaves][kCandMax];
-    __shared__ int s_cnt[kStarWaves];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    static_assert(kL == 32 || kL == 64, "lanes per vertex");
+    constexpr int kV = 256 / kL;                    // vertices per block iteration
+    constexpr int kC = kL == 64 ? kCandMax : 64;    // candidates per vertex
+    constexpr int kH = 2 * kC;                      // hash slots per vertex
+    __shared__ int s_tab[kV][kH];
+    __shared__ int s_cand[kV][kC];
+    __shared__ float2 s_pos[kV][kC];
+    __shared__ float4 s_row[kV][kC];  // (x, y, lift) of candidate - v
+    __shared__ int s_succ[kV][kC];
+    __shared__ int s_cnt[kV];
+    const int g = threadIdx.x / kL, lane = threadIdx.x % kL;
+    const int gbase = (threadIdx.x & 63) - lane;  // first wave lane of this vertex's group
+    const unsigned long long gmask = kL == 64 ? ~0ull : (0xffffffffull << gbase);
     const int n = vlist ? nlist : m.nv;
     const bool periodic = m.periodic != 0;
     const int max_depth = depth < 2 ? 2 : (depth > kStarMaxDepth ? kStarMaxDepth : depth);
     unsigned loc[4] = {0, 0, 0, 0};
-    for (int base = blockIdx.x * kStarWaves; base < n; base += gridDim.x * kStarWaves) {
-        const int idx = base + wave;
+    for (int base = blockIdx.x * kV; base < n; base += gridDim.x * kV) {
+        const int idx = base + g;
         const bool active = idx < n;
         const int v = active ? (vlist ? vlist[idx] : idx) : 0;
-        int* tab = s_tab[wave];
-        int* list = s_cand[wave];
-        int* count = &s_cnt[wave];
+        int* tab = s_tab[g];
+        int* list = s_cand[g];
+        int* count = &s_cnt[g];
         // ---- candidates, de-duplicated in LDS, listed ring by ring
-        for (int k = 0; k < kHashSlots / 64; k++) tab[lane + 64 * k] = -1;
+        for (int k = 0; k < kH / kL; k++) tab[lane + kL * k] = -1;
         if (lane == 0) *count = 0;
         __syncthreads();
         const int db = (active && deg_b) ? deg_b[v] : 0;
         const int my_depth = (active && deg_b && db < 0) ? max_depth : 2;
-        // first ring: lanes 0..31 from the previous mesh, 32..63 from this step's stars
+        // first ring, from the previous mesh and from this step's stars
         if (active) {
-            int ni = -1;
-            if (lane < 32) {
-                if (lane < deg_a[v]) ni = nbr_a[(size_t)v * kStarCap + lane];
-            } else if (lane - 32 < db) {
-                ni = nbr_b[(size_t)v * kStarCap + lane - 32];
+            const int da = deg_a[v];
+            for (int i = lane; i < da && i < kStarCap; i += kL) {
+                const int ni = nbr_a[(size_t)v * kStarCap + i];
+                if ((unsigned)ni < (unsigned)m.nv) cand_insert<kH, kC>(tab, list, count, ni);
             }
-            if ((unsigned)ni < (unsigned)m.nv) cand_insert(tab, list, count, ni);
+            for (int i = lane; i < db && i < kStarCap; i += kL) {
+                const int ni = nbr_b[(size_t)v * kStarCap + i];
+                if ((unsigned)ni < (unsigned)m.nv) cand_insert<kH, kC>(tab, list, count, ni);
+            }
         }
         __syncthreads();
         // further rings: expand the entries the previous ring added
         int lo = 0, hi = *count;
         for (int dd = 2; dd <= max_depth; dd++) {
             if (active && dd <= my_depth) {
-                const int top = hi < kCandMax ? hi : kCandMax;
-                for (int i = lo + lane; i < top; i += 64) {
+                const int top = hi < kC ? hi : kC;
+                for (int i = lo + lane; i < top; i += kL) {
                     const int u = list[i];
-                    insert_star(tab, list, count, deg_a, nbr_a, u, v, m.nv);
-                    if (deg_b && dd == 2) insert_star(tab, list, count, deg_b, nbr_b, u, v, m.nv);
+                    insert_star<kH, kC>(tab, list, count, deg_a, nbr_a, u, v, m.nv);
+                    if (deg_b && dd == 2) insert_star<kH, kC>(tab, list, count, deg_b, nbr_b, u, v, m.nv);
                 }
             }
             __syncthreads();
@@ -195,16 +211,16 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
         {
 #pragma clang fp contract(off)
             const float2 pv = m.pos[v];
-            for (int j = lane; active && j < kc && j < kCandMax; j += 64) {
+            for (int j = lane; active && j < kc && j < kC; j += kL) {
                 const float2 pj = m.pos[list[j]];
-                s_pos[wave][j] = pj;
+                s_pos[g][j] = pj;
                 const float x = (float)dexp_value(diff_exact(pj.x, pv.x, 0, periodic));
                 const float y = (float)dexp_value(diff_exact(pj.y, pv.y, 0, periodic));
-                s_row[wave][j] = make_float4(x, y, x * x + y * y, 0.0f);
+                s_row[g][j] = make_float4(x, y, x * x + y * y, 0.0f);
             }
         }
         __syncthreads();
-        bool ok = active && kc >= 3 && kc <= kCandMax;
+        bool ok = active && kc >= 3 && kc <= kC;
         int why = ok ? 0 : 4;  // failure reason slot in fail[] (0: none)
         const float2 pv = m.pos[v];
         // ---- start: the nearest candidate (largest inverted radius); a
@@ -212,16 +228,16 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
         double dist = 1e300;
         int best = kCandMax;
         bool same = false;
-        for (int j = lane; ok && j < kc; j += 64) {
-            const float2 pj = s_pos[wave][j];
+        for (int j = lane; ok && j < kc; j += kL) {
+            const float2 pj = s_pos[g][j];
             same |= pj.x == pv.x && pj.y == pv.y;
             const double dx = dexp_value(diff_exact(pj.x, pv.x, 0, periodic));
             const double dy = dexp_value(diff_exact(pj.y, pv.y, 0, periodic));
             const double dd = dx * dx + dy * dy;
             if (dd < dist) { dist = dd; best = j; }
         }
-        if (__ballot(same) && ok) { ok = false; why = 4; }
-        for (int off = 32; off >= 1; off >>= 1) {
+        if ((__ballot(same) & gmask) && ok) { ok = false; why = 4; }
+        for (int off = kL / 2; off >= 1; off >>= 1) {
             const double od = __shfl_xor(dist, off);
             const int ob = __shfl_xor(best, off);
             if (od < dist || (od == dist && ob < best)) { dist = od; best = ob; }
@@ -232,14 +248,14 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
         // candidate as seen from j' (b beats a when orient(j', a', b') < 0,
         // i.e. incircle(j, a, b, v) < 0); -1 when a test was undecided
         if (ok) {
-            for (int j = lane; j < kc; j += 64) {
-                const float4 rj = s_row[wave][j];
+            for (int j = lane; j < kc; j += kL) {
+                const float4 rj = s_row[g][j];
                 int a = j == 0 ? 1 : 0;
-                float4 ra = s_row[wave][a];
+                float4 ra = s_row[g][a];
                 bool undecided = false;
                 unsigned fast = 0;
                 for (int b = a + 1; b < kc; b++) {
-                    const float4 rb = s_row[wave][b];
+                    const float4 rb = s_row[g][b];
                     if (b == j) continue;
                     int sg = kF32 ? star_incircle_f32(rj, ra, rb) : 0;
                     if (sg != 0) {
@@ -248,14 +264,14 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                         undecided = true;
                     } else {
                         int st = 0;
-                        sg = incircle<kF32, false>(s_pos[wave][j], s_pos[wave][a], s_pos[wave][b], pv, periodic, &st);
+                        sg = incircle<kF32, false>(s_pos[g][j], s_pos[g][a], s_pos[g][b], pv, periodic, &st);
                         bump(loc, st == kStageExactBig ? 3 : st);
                         if (sg == kNeedBig || sg == 0) undecided = true;
                     }
                     if (sg < 0) { a = b; ra = rb; }
                 }
                 loc[0] += fast;
-                s_succ[wave][j] = undecided ? -1 : a;
+                s_succ[g][j] = undecided ? -1 : a;
             }
         }
         __syncthreads();
@@ -268,7 +284,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                 if (d == kStarCap) break;
                 if (lane == d) my_slot = cur;
                 d++;
-                const int nx = s_succ[wave][cur];
+                const int nx = s_succ[g][cur];
                 if (nx < 0) { why = 5; break; }
                 if (nx == start) { closed = true; break; }
                 cur = nx;
@@ -281,26 +297,26 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
         // every triangle (v, s_i, s_i+1) strictly CCW (v inside the hull)
         if (ok) {
             const int nl = (lane + 1 < d) ? lane + 1 : 0;
-            const int nslot = __shfl(my_slot, nl);
+            const int nslot = __shfl(my_slot, gbase + nl);
             bool bad = false, undecided = false;
             if (lane < d) {
-                int o = kF32 ? star_orient_f32(s_row[wave][my_slot], s_row[wave][nslot]) : 0;
+                int o = kF32 ? star_orient_f32(s_row[g][my_slot], s_row[g][nslot]) : 0;
                 if (o == 0 && !kFast) {
                     int st;
-                    o = orient<kF32>(pv, s_pos[wave][my_slot], s_pos[wave][nslot], periodic, &st);
+                    o = orient<kF32>(pv, s_pos[g][my_slot], s_pos[g][nslot], periodic, &st);
                     bad = o <= 0;
                 } else {
                     undecided = o == 0;
                     bad = o < 0;
                 }
             }
-            if (__ballot(bad)) { ok = false; why = 7; }
-            else if (__ballot(undecided)) { ok = false; why = 5; }
+            if (__ballot(bad) & gmask) { ok = false; why = 7; }
+            else if (__ballot(undecided) & gmask) { ok = false; why = 5; }
         }
         if (active) {
             const size_t row = vlist ? (size_t)idx : (size_t)v;
             if (ok) {
-                if (lane < d) nbr_out[row * kStarCap + lane] = s_cand[wave][my_slot];
+                if (lane < d) nbr_out[row * kStarCap + lane] = s_cand[g][my_slot];
                 if (lane == 0) deg_out[row] = d;
             } else if (lane == 0) {
                 deg_out[row] = -1;
@@ -328,10 +344,14 @@ __device__ __forceinline__ int star_find(const int* __restrict__ nbr, int u, int
 }
 
 // Every triangle (v, u, w) of v's star must read (u, w, v) in u's star.
-// Checks the *nlist listed vertices (all when vlist is null).  Vertices that fail
+// Checks the *nlist listed vertices (all when vlist is null), kCheckLanes
+// lanes per vertex (lane i checks triangles i, i + 8, ...: independent
+// lookups in flight instead of one chain per vertex).  Vertices that fail
 // (or whose star failed) are appended to `list`, with their star neighbours
 // when mark_nbrs is set.  Without a list the degree sum of the valid stars is
 // added to *sum_deg.
+constexpr int kCheckLanes = 8;
+constexpr int kCheckVerts = 256 / kCheckLanes;  // vertices per block iteration
 __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restrict__ vlist, const int* __restrict__ nlist,
                                                     const int* __restrict__ deg, const int* __restrict__ nbr,
                                                     int mark_nbrs, int* __restrict__ mark, int stamp,
@@ -342,19 +362,20 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
     __syncthreads();
     long long my_sum = 0;
     const int n = vlist ? *nlist : nv;
-    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-        const int idx = base + threadIdx.x;
+    const int sub = threadIdx.x % kCheckLanes, grp = threadIdx.x / kCheckLanes;
+    const int gl = (threadIdx.x & 63) - sub;  // first wave lane of the group
+    for (int base = blockIdx.x * kCheckVerts; base < n; base += gridDim.x * kCheckVerts) {
+        const int idx = base + grp;
         const bool active = idx < n;
         const int v = active ? (vlist ? vlist[idx] : idx) : 0;
         const int d = active ? deg[v] : 3;
+        const int* row = nbr + (size_t)v * kStarCap;
         bool bad = active && d < 3;
         if (active && !bad) {
-            if (!vlist) my_sum += d;
-            const int* row = nbr + (size_t)v * kStarCap;
-            const int first = row[0];
-            int u = first;
-            for (int i = 0; i < d && !bad; i++) {
-                const int w = (i + 1 == d) ? first : row[i + 1];
+            if (!vlist && sub == 0) my_sum += d;
+            for (int i = sub; i < d; i += kCheckLanes) {
+                const int u = row[i];
+                const int w = row[i + 1 == d ? 0 : i + 1];
                 const int du = deg[u];
                 if (du < 3) {
                     bad = true;
@@ -362,24 +383,25 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
                     const int j = star_find(nbr, u, du, w);
                     if (j < 0 || nbr[(size_t)u * kStarCap + (j + 1 == du ? 0 : j + 1)] != v) bad = true;
                 }
-                u = w;
             }
         }
-        // claim v (and its neighbours) once per stamp, then append the claimed ones
-        unsigned claimed = 0;  // bit 0: v, bit 1 + i: neighbour i
+        const bool gbad = ((__ballot(bad) >> gl) & ((1ull << kCheckLanes) - 1ull)) != 0ull;
+        // claim v (lane 0) and its neighbours (lane i: i, i + 8, ...) once per
+        // stamp, then append the claimed ones
+        unsigned claimed = 0;  // bit 0: v, bit 1 + k: neighbour sub + 8k
         int nclaim = 0;
-        if (bad) {
-            if (atomicExch(&mark[v], stamp) != stamp) { claimed |= 1u; nclaim++; }
-            for (int i = 0; mark_nbrs && i < d; i++)
-                if (atomicExch(&mark[nbr[(size_t)v * kStarCap + i]], stamp) != stamp) {
-                    claimed |= 2u << i;
+        if (gbad && active) {
+            if (sub == 0 && atomicExch(&mark[v], stamp) != stamp) { claimed |= 1u; nclaim++; }
+            for (int i = sub, k = 0; mark_nbrs && i < d; i += kCheckLanes, k++)
+                if (atomicExch(&mark[row[i]], stamp) != stamp) {
+                    claimed |= 2u << k;
                     nclaim++;
                 }
         }
         int at = block_reserve(count, nclaim, s_tile);
         if (claimed & 1u) list[at++] = v;
-        for (int i = 0; i < d && i < kStarCap; i++)
-            if (claimed & (2u << i)) list[at++] = nbr[(size_t)v * kStarCap + i];
+        for (int i = sub, k = 0; i < d && i < kStarCap; i += kCheckLanes, k++)
+            if (claimed & (2u << k)) list[at++] = row[i];
     }
     if (my_sum) atomicAdd((unsigned long long*)sum_deg, (unsigned long long)my_sum);
 }

@@ -2301,9 +2301,12 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
                        mv_stamp, *p, disp);
     t->timers[2].mark(s);
     hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux, kStarAux, 0);
-    const int star_blocks = (int)std::min<long long>(((long long)m.nv + kStarWaves - 1) / kStarWaves, 4LL * t->grid_blocks);
+    // blocks for a list of n vertices, kv vertices per block iteration
+    auto star_grid = [&](long long nverts, int kv) {
+        return dim3((unsigned)std::max<long long>(1, std::min<long long>((nverts + kv - 1) / kv, 4LL * t->grid_blocks)));
+    };
     t->timers[5].mark(s);
-    hipLaunchKernelGGL((k_star<true, true>), dim3(star_blocks), dim3(256), 0, s, m, (const int*)nullptr, 0, 2,
+    hipLaunchKernelGGL((k_star<true, true, 32>), star_grid(m.nv, 8), dim3(256), 0, s, m, (const int*)nullptr, 0, 2,
                        (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur], (const int*)nullptr,
                        (const int*)nullptr, t->star_deg[nxt], t->star_nbr[nxt], t->s_aux, t->stats);
     t->timers[5].mark(s);
@@ -2322,11 +2325,11 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 1, 1, 0);
         const int stamp = ++t->stamp;
         if (pass == 0)
-            hipLaunchKernelGGL(k_star_check, dim3(blocks_for(m.nv)), dim3(256), 0, s, m.nv, (const int*)nullptr,
+            hipLaunchKernelGGL(k_star_check, star_grid(m.nv, kCheckVerts), dim3(256), 0, s, m.nv, (const int*)nullptr,
                                (const int*)nullptr, (const int*)t->star_deg[nxt], (const int*)t->star_nbr[nxt], 0,
                                t->s_mark, stamp, t->s_list, t->s_aux + 1, (long long*)(t->s_aux + 2));
         else
-            hipLaunchKernelGGL(k_star_check, dim3(blocks_for(std::min<long long>(m.nv, (long long)nbad * (kStarCap + 1)))),
+            hipLaunchKernelGGL(k_star_check, star_grid(std::min<long long>(m.nv, (long long)nbad * (kStarCap + 1)), kCheckVerts),
                                dim3(256), 0, s, m.nv, (const int*)t->s_rlist, (const int*)(t->s_aux + 8),
                                (const int*)t->star_deg[nxt], (const int*)t->star_nbr[nxt], 1, t->s_mark, stamp,
                                t->s_list, t->s_aux + 1, (long long*)(t->s_aux + 2));
@@ -2351,13 +2354,14 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         passes++;
         // the FP32-filter-only kernel for the first retry, the full cascade
         // (which also settles the tests the filter left undecided) after it
-        const dim3 rb(std::min(star_blocks, (nbad + kStarWaves - 1) / kStarWaves));
         if (pass >= 1)
-            hipLaunchKernelGGL((k_star<true, false>), rb, dim3(256), 0, s, m, (const int*)t->s_list, nbad, 2 + pass + 1,
+            hipLaunchKernelGGL((k_star<true, false, 64>), star_grid(nbad, 4), dim3(256), 0, s, m, (const int*)t->s_list,
+                               nbad, 2 + pass + 1,
                                (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur], (const int*)t->star_deg[nxt],
                                (const int*)t->star_nbr[nxt], t->star_tdeg, t->star_tnbr, t->s_aux, t->stats);
         else
-            hipLaunchKernelGGL((k_star<true, true>), rb, dim3(256), 0, s, m, (const int*)t->s_list, nbad, 2 + pass + 1,
+            hipLaunchKernelGGL((k_star<true, true, 32>), star_grid(nbad, 8), dim3(256), 0, s, m, (const int*)t->s_list,
+                               nbad, 2 + pass + 1,
                                (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur], (const int*)t->star_deg[nxt],
                                (const int*)t->star_nbr[nxt], t->star_tdeg, t->star_tnbr, t->s_aux, t->stats);
         hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 8, 1, 0);
