@@ -128,11 +128,16 @@ __device__ __forceinline__ int star_orient_f32(float4 A, float4 B) {
 // arrays deg_a/nbr_a (the previous mesh), widened to the depth-ring when v's
 // star in deg_b failed (v moved out of its 2-ring), and, when deg_b is given,
 // the 2-ring of v through both the previous stars and deg_b/nbr_b (the stars
-// found so far in this step).  Writes deg_out/nbr_out at row idx of the list
-// (at row v without a list); a vertex whose star cannot be computed gets
-// deg_out = -1 and is counted in fail[0] and, by reason, in fail[4] (too few
-// or too many candidates), fail[5] (undecided test), fail[6] (hull not closed)
-// or fail[7] (v not inside).
+// found so far in this step).  Processes list positions 0..nlist-1, or the
+// *nrows positions in rows when given (vertices vlist[row], or row itself
+// without a list).  Writes deg_out/nbr_out at row idx of the list (at row v
+// without a list); a vertex whose star cannot be computed gets deg_out = -1.
+// With heavy given, a vertex that failed for too many candidates or an
+// undecided test is appended to heavy (heavy[0] the count, the rows after it;
+// counted in fail[10]) for the full kernel; other failures are counted in
+// fail[0] and, by reason, in fail[4] (too few or too many candidates),
+// fail[5] (undecided test), fail[6] (hull not closed) or fail[7] (v not
+// inside).
 //
 // kL lanes handle one vertex (64: one per wave, up to 128 candidates; 32: two
 // per wave, up to 64 candidates, which holds nearly every 2-ring and halves
@@ -143,11 +148,13 @@ __device__ __forceinline__ int star_orient_f32(float4 A, float4 B) {
 // carries none of the exact-arithmetic code and keeps a small register
 // footprint.
 template <bool kF32, bool kFast, int kL>
-__global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ vlist, int nlist, int depth,
+__global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ vlist, int nlist,
+                                              const int* __restrict__ rows, const int* __restrict__ nrows, int depth,
                                               const int* __restrict__ deg_a, const int* __restrict__ nbr_a,
                                               const int* __restrict__ deg_b, const int* __restrict__ nbr_b,
                                               int* __restrict__ deg_out, int* __restrict__ nbr_out,
-                                              int* __restrict__ fail, unsigned long long* __restrict__ stats) {
+                                              int* __restrict__ fail, int* __restrict__ heavy,
+                                              unsigned long long* __restrict__ stats) {
     static_assert(kL == 32 || kL == 64, "lanes per vertex");
     constexpr int kV = 256 / kL;                    // vertices per block iteration
     constexpr int kC = kL == 64 ? kCandMax : 64;    // candidates per vertex
@@ -161,13 +168,13 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
     const int g = threadIdx.x / kL, lane = threadIdx.x % kL;
     const int gbase = (threadIdx.x & 63) - lane;  // first wave lane of this vertex's group
     const unsigned long long gmask = kL == 64 ? ~0ull : (0xffffffffull << gbase);
-    const int n = vlist ? nlist : m.nv;
+    const int n = rows ? *nrows : (vlist ? nlist : m.nv);
     const bool periodic = m.periodic != 0;
     const int max_depth = depth < 2 ? 2 : (depth > kStarMaxDepth ? kStarMaxDepth : depth);
     unsigned loc[4] = {0, 0, 0, 0};
     for (int base = blockIdx.x * kV; base < n; base += gridDim.x * kV) {
-        const int idx = base + g;
-        const bool active = idx < n;
+        const bool active = base + g < n;
+        const int idx = active ? (rows ? rows[base + g] : base + g) : 0;  // output row
         const int v = active ? (vlist ? vlist[idx] : idx) : 0;
         int* tab = s_tab[g];
         int* list = s_cand[g];
@@ -314,14 +321,21 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
             else if (__ballot(undecided) & gmask) { ok = false; why = 5; }
         }
         if (active) {
-            const size_t row = vlist ? (size_t)idx : (size_t)v;
+            const size_t row = (size_t)idx;
             if (ok) {
                 if (lane < d) nbr_out[row * kStarCap + lane] = s_cand[g][my_slot];
                 if (lane == 0) deg_out[row] = d;
             } else if (lane == 0) {
                 deg_out[row] = -1;
-                atomicAdd(fail, 1);
-                atomicAdd(fail + why, 1);
+                if (heavy && (why == 4 || why == 5)) {
+                    // for the full kernel: more candidates than kC, or a test
+                    // the filter could not decide
+                    heavy[1 + atomicAdd(heavy, 1)] = idx;
+                    atomicAdd(fail + 10, 1);
+                } else {
+                    atomicAdd(fail, 1);
+                    atomicAdd(fail + why, 1);
+                }
             }
         }
     }
@@ -345,18 +359,17 @@ __device__ __forceinline__ int star_find(const int* __restrict__ nbr, int u, int
 
 // Every triangle (v, u, w) of v's star must read (u, w, v) in u's star.
 // Checks the *nlist listed vertices (all when vlist is null), kCheckLanes
-// lanes per vertex (lane i checks triangles i, i + 8, ...: independent
-// lookups in flight instead of one chain per vertex).  Vertices that fail
+// lanes per vertex (lane i checks triangles i, i + kCheckLanes, ...).  Vertices that fail
 // (or whose star failed) are appended to `list`, with their star neighbours
 // when mark_nbrs is set.  Without a list the degree sum of the valid stars is
 // added to *sum_deg.
-constexpr int kCheckLanes = 8;
-constexpr int kCheckVerts = 256 / kCheckLanes;  // vertices per block iteration
+template <int kCheckLanes>
 __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restrict__ vlist, const int* __restrict__ nlist,
                                                     const int* __restrict__ deg, const int* __restrict__ nbr,
                                                     int mark_nbrs, int* __restrict__ mark, int stamp,
                                                     int* __restrict__ list, int* __restrict__ count,
                                                     long long* __restrict__ sum_deg) {
+    constexpr int kCheckVerts = 256 / kCheckLanes;  // vertices per block iteration
     __shared__ int s_tile[2];
     if (threadIdx.x == 0) s_tile[0] = 0;
     __syncthreads();
@@ -385,10 +398,10 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
                 }
             }
         }
-        const bool gbad = ((__ballot(bad) >> gl) & ((1ull << kCheckLanes) - 1ull)) != 0ull;
+        const bool gbad = kCheckLanes == 1 ? bad : ((__ballot(bad) >> gl) & ((1ull << kCheckLanes) - 1ull)) != 0ull;
         // claim v (lane 0) and its neighbours (lane i: i, i + 8, ...) once per
         // stamp, then append the claimed ones
-        unsigned claimed = 0;  // bit 0: v, bit 1 + k: neighbour sub + 8k
+        unsigned claimed = 0;  // bit 0: v, bit 1 + k: neighbour sub + kCheckLanes * k
         int nclaim = 0;
         if (gbad && active) {
             if (sub == 0 && atomicExch(&mark[v], stamp) != stamp) { claimed |= 1u; nclaim++; }

@@ -1500,6 +1500,7 @@ struct mcl_tri {
     int* s_list = nullptr;                  // nv
     int* s_rlist = nullptr;                 // nv: vertices to re-check
     int* s_aux = nullptr;                   // kStarAux counters (see star_step)
+    int* s_heavy = nullptr;                 // nv + 1: count, then rows for the full star kernel
     // one ring
     int* deg = nullptr;
     int* scan_tmp = nullptr;
@@ -1547,7 +1548,7 @@ static int tri_free(mcl_tri* t) {
                     t->scalar, t->acc, t->tgt, t->v_mv, t->v_rev, t->v_fexp, t->t_stamp, t->bad[0], t->bad[1],
                     t->rev, t->allrev, t->pend[0], t->pend[1], t->deg, t->scan_tmp, t->v_touch, t->t_rep, t->t_rep_e, t->e_undo, t->undo, t->stuck, t->touched, t->r_tt, t->r_tt2, t->v_soft, t->tail_out,
                     t->star_deg[0], t->star_deg[1], t->star_nbr[0], t->star_nbr[1], t->s_ecnt, t->s_tcnt,
-                    t->s_mark, t->s_list, t->s_aux, t->star_tdeg, t->star_tnbr, t->s_rlist};
+                    t->s_mark, t->s_list, t->s_aux, t->star_tdeg, t->star_tnbr, t->s_rlist, t->s_heavy};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (t->h_cnt) (void)hipHostFree(t->h_cnt);
@@ -1603,7 +1604,7 @@ extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv,
         A(t->star_nbr[0], (size_t)nv * kStarCap); A(t->star_nbr[1], (size_t)nv * kStarCap);
         A(t->star_tdeg, nv); A(t->star_tnbr, (size_t)nv * kStarCap);
         A(t->s_ecnt, nv + 1); A(t->s_tcnt, nv + 1); A(t->s_mark, nv); A(t->s_list, nv); A(t->s_aux, kStarAux);
-        A(t->s_rlist, nv);
+        A(t->s_rlist, nv); A(t->s_heavy, nv + 1);
     }
 #undef A
     if (rc == MCL_OK && hipHostMalloc((void**)&t->h_cnt, sizeof(int) * kCnCount) != hipSuccess) rc = MCL_ERR_HIP;
@@ -2305,10 +2306,21 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     auto star_grid = [&](long long nverts, int kv) {
         return dim3((unsigned)std::max<long long>(1, std::min<long long>((nverts + kv - 1) / kv, 4LL * t->grid_blocks)));
     };
+    // stars of the listed vertices (all without a list) into (odeg, onbr):
+    // the filter-only kernel, two vertices per wave, then the full kernel on
+    // the vertices it could not finish (more than 64 candidates, or a test
+    // the filter could not decide)
+    auto stars = [&](const int* vlist, int nlist, int depth, const int* bdeg, const int* bnbr, int* odeg, int* onbr) {
+        hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_heavy, 1, 0);
+        hipLaunchKernelGGL((k_star<true, true, 32>), star_grid(vlist ? nlist : m.nv, 8), dim3(256), 0, s, m, vlist,
+                           nlist, (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
+                           (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy, t->stats);
+        hipLaunchKernelGGL((k_star<true, false, 64>), dim3(256), dim3(256), 0, s, m, vlist, nlist,
+                           (const int*)(t->s_heavy + 1), (const int*)t->s_heavy, depth, (const int*)t->star_deg[cur],
+                           (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, (int*)nullptr, t->stats);
+    };
     t->timers[5].mark(s);
-    hipLaunchKernelGGL((k_star<true, true, 32>), star_grid(m.nv, 8), dim3(256), 0, s, m, (const int*)nullptr, 0, 2,
-                       (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur], (const int*)nullptr,
-                       (const int*)nullptr, t->star_deg[nxt], t->star_nbr[nxt], t->s_aux, t->stats);
+    stars(nullptr, 0, 2, nullptr, nullptr, t->star_deg[nxt], t->star_nbr[nxt]);
     t->timers[5].mark(s);
     MCL_HIP(hipGetLastError());
     // s_aux: [0] failed stars, [1] listed vertices, [2..3] degree sum of the
@@ -2324,22 +2336,24 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         // the inconsistent vertices (a wrong star can agree with its neighbours')
         hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 1, 1, 0);
         const int stamp = ++t->stamp;
-        if (pass == 0)
-            hipLaunchKernelGGL(k_star_check, star_grid(m.nv, kCheckVerts), dim3(256), 0, s, m.nv, (const int*)nullptr,
-                               (const int*)nullptr, (const int*)t->star_deg[nxt], (const int*)t->star_nbr[nxt], 0,
+        {
+            // one lane per vertex (measured faster at 1M than 4 or 8 lanes per vertex)
+            const long long nchk = pass == 0 ? m.nv : std::min<long long>(m.nv, (long long)nbad * (kStarCap + 1));
+            hipLaunchKernelGGL(k_star_check<1>, star_grid(nchk, 256), dim3(256), 0, s, m.nv,
+                               pass == 0 ? (const int*)nullptr : (const int*)t->s_rlist,
+                               pass == 0 ? (const int*)nullptr : (const int*)(t->s_aux + 8),
+                               (const int*)t->star_deg[nxt], (const int*)t->star_nbr[nxt], pass == 0 ? 0 : 1,
                                t->s_mark, stamp, t->s_list, t->s_aux + 1, (long long*)(t->s_aux + 2));
-        else
-            hipLaunchKernelGGL(k_star_check, star_grid(std::min<long long>(m.nv, (long long)nbad * (kStarCap + 1)), kCheckVerts),
-                               dim3(256), 0, s, m.nv, (const int*)t->s_rlist, (const int*)(t->s_aux + 8),
-                               (const int*)t->star_deg[nxt], (const int*)t->star_nbr[nxt], 1, t->s_mark, stamp,
-                               t->s_list, t->s_aux + 1, (long long*)(t->s_aux + 2));
+        }
         MCL_HIP(hipGetLastError());
         MCL_HIP(hipMemcpyAsync(aux, t->s_aux, sizeof(aux), hipMemcpyDeviceToHost, s));
         MCL_HIP(hipStreamSynchronize(s));
         nbad = aux[1];
         if (trace_on())
-            fprintf(stderr, "  [stars pass %d] inconsistent %d (failed stars so far %d: candidates %d undecided %d open %d outside %d)\n",
-                    pass, nbad, aux[0], aux[4], aux[5], aux[6], aux[7]);
+            fprintf(stderr,
+                    "  [stars pass %d] inconsistent %d (failed stars so far %d: candidates %d undecided %d open %d "
+                    "outside %d; full-kernel vertices %d)\n",
+                    pass, nbad, aux[0], aux[4], aux[5], aux[6], aux[7], aux[10]);
         if (nbad == 0) {
             long long sum_deg;
             memcpy(&sum_deg, aux + 2, sizeof(sum_deg));
@@ -2352,18 +2366,7 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         // (it left its 2-ring) searches one ring deeper each pass
         retried += nbad;
         passes++;
-        // the FP32-filter-only kernel for the first retry, the full cascade
-        // (which also settles the tests the filter left undecided) after it
-        if (pass >= 1)
-            hipLaunchKernelGGL((k_star<true, false, 64>), star_grid(nbad, 4), dim3(256), 0, s, m, (const int*)t->s_list,
-                               nbad, 2 + pass + 1,
-                               (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur], (const int*)t->star_deg[nxt],
-                               (const int*)t->star_nbr[nxt], t->star_tdeg, t->star_tnbr, t->s_aux, t->stats);
-        else
-            hipLaunchKernelGGL((k_star<true, true, 32>), star_grid(nbad, 8), dim3(256), 0, s, m, (const int*)t->s_list,
-                               nbad, 2 + pass + 1,
-                               (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur], (const int*)t->star_deg[nxt],
-                               (const int*)t->star_nbr[nxt], t->star_tdeg, t->star_tnbr, t->s_aux, t->stats);
+        stars(t->s_list, nbad, 2 + pass + 1, t->star_deg[nxt], t->star_nbr[nxt], t->star_tdeg, t->star_tnbr);
         hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 8, 1, 0);
         const int rstamp = ++t->stamp;
         hipLaunchKernelGGL(k_star_scatter, dim3(blocks_for(nbad)), dim3(256), 0, s, (const int*)t->s_list, nbad,
