@@ -106,8 +106,8 @@ __device__ __forceinline__ int star_incircle_f32(float4 A, float4 B, float4 C) {
     const float perm = (fabsf(bdxcdy) + fabsf(cdxbdy)) * A.z + (fabsf(cdxady) + fabsf(adxcdy)) * B.z +
                        (fabsf(adxbdy) + fabsf(bdxady)) * C.z;
     const float bound = (10.0f + 96.0f * e) * e * perm * 1.0001f;
-    if (perm > 7.9e-31f && perm < 1e37f && (det > bound || -det > bound)) return det > 0 ? 1 : -1;
-    return 0;
+    const bool sure = (perm > 7.9e-31f) & (perm < 1e37f) & (fabsf(det) > bound);
+    return sure ? (det > 0.0f ? 1 : -1) : 0;
 }
 
 // The FP32 stage of orient(v, a, b) from the rows of a - v and b - v (the
@@ -254,7 +254,40 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
         // ---- successors: for each owned candidate j, the most clockwise other
         // candidate as seen from j' (b beats a when orient(j', a', b') < 0,
         // i.e. incircle(j, a, b, v) < 0); -1 when a test was undecided
-        if (ok) {
+        if (kFast) {
+            // Branch-free, with wave-uniform loops: both vertices of a wave
+            // run to the larger candidate count, entries past a vertex's own
+            // count (and the lane's own candidate) are masked out, and the
+            // winner is kept by selects.
+            const int kc_g = ok ? kc : 0;
+            int kc_w = kc_g;
+            if (kL == 32) kc_w = max(kc_w, __shfl_xor(kc_w, 32));
+            kc_w = __builtin_amdgcn_readfirstlane(kc_w);
+            for (int jj = 0; jj * kL < kc_w; jj++) {
+                const int j = lane + jj * kL;
+                const bool own = j < kc_g;
+                const float4 rj = s_row[g][own ? j : 0];
+                const int a0 = j == 0 ? 1 : 0;
+                int a = a0;
+                float4 ra = s_row[g][a0];
+                bool undecided = false;
+                unsigned fast = 0;
+                for (int b = 0; b < kc_w; b++) {
+                    const float4 rb = s_row[g][b];
+                    const int sg = star_incircle_f32(rj, ra, rb);
+                    const bool use = (b != j) & (b != a0) & (b < kc_g);
+                    undecided |= (sg == 0) & use;
+                    fast += (sg != 0) & use;
+                    const bool win = (sg < 0) & use;
+                    a = win ? b : a;
+                    ra.x = win ? rb.x : ra.x;
+                    ra.y = win ? rb.y : ra.y;
+                    ra.z = win ? rb.z : ra.z;
+                }
+                loc[0] += fast;
+                if (own) s_succ[g][j] = undecided ? -1 : a;
+            }
+        } else if (ok) {
             for (int j = lane; j < kc; j += kL) {
                 const float4 rj = s_row[g][j];
                 int a = j == 0 ? 1 : 0;
@@ -267,8 +300,6 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                     int sg = kF32 ? star_incircle_f32(rj, ra, rb) : 0;
                     if (sg != 0) {
                         fast++;
-                    } else if (kFast) {
-                        undecided = true;
                     } else {
                         int st = 0;
                         sg = incircle<kF32, false>(s_pos[g][j], s_pos[g][a], s_pos[g][b], pv, periodic, &st);
