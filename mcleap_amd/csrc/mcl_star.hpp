@@ -125,10 +125,10 @@ __device__ __forceinline__ int star_orient_f32(float4 A, float4 B) {
 
 // Recomputes the stars of the listed vertices (all vertices when vlist is
 // null) with the current positions.  Candidates: the 2-ring of v in the star
-// arrays deg_a/nbr_a (the previous mesh), widened to the depth-ring when v's
-// star in deg_b failed (v moved out of its 2-ring), and, when deg_b is given,
-// the 2-ring of v through both the previous stars and deg_b/nbr_b (the stars
-// found so far in this step).  Processes list positions 0..nlist-1, or the
+// arrays deg_a/nbr_a (the previous mesh).  With deg_b/nbr_b (the stars found
+// so far in this step): v's neighbours in both, expanded through deg_b, or,
+// when v's star in deg_b failed (v moved out of its 2-ring), the depth-ring
+// of v in the previous stars plus the 2-ring through deg_b.  Processes list positions 0..nlist-1, or the
 // *nrows positions in rows when given (vertices vlist[row], or row itself
 // without a list).  Writes deg_out/nbr_out at row idx of the list (at row v
 // without a list); a vertex whose star cannot be computed gets deg_out = -1.
@@ -184,7 +184,11 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
         if (lane == 0) *count = 0;
         __syncthreads();
         const int db = (active && deg_b) ? deg_b[v] : 0;
-        const int my_depth = (active && deg_b && db < 0) ? max_depth : 2;
+        // a vertex whose star failed (it left its 2-ring) widens through the
+        // previous stars; a valid but inconsistent one expands its first ring
+        // (old and new neighbours) through the new stars only
+        const bool widen = active && deg_b && db < 0;
+        const int my_depth = widen ? max_depth : 2;
         // first ring, from the previous mesh and from this step's stars
         if (active) {
             const int da = deg_a[v];
@@ -205,7 +209,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                 const int top = hi < kC ? hi : kC;
                 for (int i = lo + lane; i < top; i += kL) {
                     const int u = list[i];
-                    insert_star<kH, kC>(tab, list, count, deg_a, nbr_a, u, v, m.nv);
+                    if (!deg_b || widen) insert_star<kH, kC>(tab, list, count, deg_a, nbr_a, u, v, m.nv);
                     if (deg_b && dd == 2) insert_star<kH, kC>(tab, list, count, deg_b, nbr_b, u, v, m.nv);
                 }
             }
@@ -388,64 +392,103 @@ __device__ __forceinline__ int star_find(const int* __restrict__ nbr, int u, int
     return pos;
 }
 
-// Every triangle (v, u, w) of v's star must read (u, w, v) in u's star.
-// Checks the *nlist listed vertices (all when vlist is null), kCheckLanes
-// lanes per vertex (lane i checks triangles i, i + kCheckLanes, ...).  Vertices that fail
-// (or whose star failed) are appended to `list`, with their star neighbours
-// when mark_nbrs is set.  Without a list the degree sum of the valid stars is
-// added to *sum_deg.
-template <int kCheckLanes>
+// true when some triangle (v, u, w) of v's star (degree d >= 3) does not read
+// (u, w, v) in u's star.  A vertex of degree <= 8 whose neighbours have
+// degree <= 8 (nearly all) is checked with every load issued up front:
+// its row, the neighbours' degrees, the neighbours' first 8 entries.
+__device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, const int* __restrict__ nbr, int v,
+                                                  int d) {
+    const int* row = nbr + (size_t)v * kStarCap;
+    if (d <= 8) {
+        const int4* rv = reinterpret_cast<const int4*>(row);
+        const int4 q0 = rv[0], q1 = rv[1];
+        const int r[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+        int du[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) du[i] = i < d ? deg[r[i]] : 0;
+        int4 lo[8], hi[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            if (i < d && du[i] >= 3 && du[i] <= 8) {
+                const int4* ru = reinterpret_cast<const int4*>(nbr + (size_t)r[i] * kStarCap);
+                lo[i] = ru[0];
+                hi[i] = ru[1];
+            } else {
+                lo[i] = make_int4(-1, -1, -1, -1);
+                hi[i] = lo[i];
+            }
+        }
+        bool bad = false, slow = false;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            if (i >= d) continue;
+            const int w = (i + 1 < d) ? r[(i + 1) & 7] : r[0];
+            const int n = du[i];
+            if (n < 3) { bad = true; continue; }
+            if (n > 8) { slow = true; continue; }
+            const int e[8] = {lo[i].x, lo[i].y, lo[i].z, lo[i].w, hi[i].x, hi[i].y, hi[i].z, hi[i].w};
+            int nx = -1;
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (k < n && e[k] == w) nx = (k + 1 < n) ? e[(k + 1) & 7] : e[0];
+            if (nx != v) bad = true;
+        }
+        if (bad || !slow) return bad;
+    }
+    // general form: one triangle at a time
+    const int first = row[0];
+    int u = first;
+    for (int i = 0; i < d; i++) {
+        const int w = (i + 1 == d) ? first : row[i + 1];
+        const int du = deg[u];
+        if (du < 3) return true;
+        const int j = star_find(nbr, u, du, w);
+        if (j < 0 || nbr[(size_t)u * kStarCap + (j + 1 == du ? 0 : j + 1)] != v) return true;
+        u = w;
+    }
+    return false;
+}
+
+// Checks the *nlist listed vertices (all when vlist is null), one thread per
+// vertex.  Vertices that fail (or whose star failed) are appended to `list`,
+// with their star neighbours when mark_nbrs is set.  Without a list the
+// degree sum of the valid stars is added to *sum_deg.
 __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restrict__ vlist, const int* __restrict__ nlist,
                                                     const int* __restrict__ deg, const int* __restrict__ nbr,
                                                     int mark_nbrs, int* __restrict__ mark, int stamp,
                                                     int* __restrict__ list, int* __restrict__ count,
                                                     long long* __restrict__ sum_deg) {
-    constexpr int kCheckVerts = 256 / kCheckLanes;  // vertices per block iteration
     __shared__ int s_tile[2];
     if (threadIdx.x == 0) s_tile[0] = 0;
     __syncthreads();
     long long my_sum = 0;
     const int n = vlist ? *nlist : nv;
-    const int sub = threadIdx.x % kCheckLanes, grp = threadIdx.x / kCheckLanes;
-    const int gl = (threadIdx.x & 63) - sub;  // first wave lane of the group
-    for (int base = blockIdx.x * kCheckVerts; base < n; base += gridDim.x * kCheckVerts) {
-        const int idx = base + grp;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const int idx = base + threadIdx.x;
         const bool active = idx < n;
         const int v = active ? (vlist ? vlist[idx] : idx) : 0;
         const int d = active ? deg[v] : 3;
         const int* row = nbr + (size_t)v * kStarCap;
         bool bad = active && d < 3;
         if (active && !bad) {
-            if (!vlist && sub == 0) my_sum += d;
-            for (int i = sub; i < d; i += kCheckLanes) {
-                const int u = row[i];
-                const int w = row[i + 1 == d ? 0 : i + 1];
-                const int du = deg[u];
-                if (du < 3) {
-                    bad = true;
-                } else {
-                    const int j = star_find(nbr, u, du, w);
-                    if (j < 0 || nbr[(size_t)u * kStarCap + (j + 1 == du ? 0 : j + 1)] != v) bad = true;
-                }
-            }
+            if (!vlist) my_sum += d;
+            bad = star_inconsistent(deg, nbr, v, d);
         }
-        const bool gbad = kCheckLanes == 1 ? bad : ((__ballot(bad) >> gl) & ((1ull << kCheckLanes) - 1ull)) != 0ull;
-        // claim v (lane 0) and its neighbours (lane i: i, i + 8, ...) once per
-        // stamp, then append the claimed ones
-        unsigned claimed = 0;  // bit 0: v, bit 1 + k: neighbour sub + kCheckLanes * k
+        // claim v and its neighbours once per stamp, then append the claimed ones
+        unsigned claimed = 0;  // bit 0: v, bit 1 + i: neighbour i
         int nclaim = 0;
-        if (gbad && active) {
-            if (sub == 0 && atomicExch(&mark[v], stamp) != stamp) { claimed |= 1u; nclaim++; }
-            for (int i = sub, k = 0; mark_nbrs && i < d; i += kCheckLanes, k++)
+        if (bad) {
+            if (atomicExch(&mark[v], stamp) != stamp) { claimed |= 1u; nclaim++; }
+            for (int i = 0; mark_nbrs && i < d; i++)
                 if (atomicExch(&mark[row[i]], stamp) != stamp) {
-                    claimed |= 2u << k;
+                    claimed |= 2u << i;
                     nclaim++;
                 }
         }
         int at = block_reserve(count, nclaim, s_tile);
         if (claimed & 1u) list[at++] = v;
-        for (int i = sub, k = 0; i < d && i < kStarCap; i += kCheckLanes, k++)
-            if (claimed & (2u << k)) list[at++] = row[i];
+        for (int i = 0; i < d && i < kStarCap; i++)
+            if (claimed & (2u << i)) list[at++] = row[i];
     }
     if (my_sum) atomicAdd((unsigned long long*)sum_deg, (unsigned long long)my_sum);
 }

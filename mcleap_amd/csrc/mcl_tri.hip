@@ -2337,9 +2337,8 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 1, 1, 0);
         const int stamp = ++t->stamp;
         {
-            // one lane per vertex (measured faster at 1M than 4 or 8 lanes per vertex)
             const long long nchk = pass == 0 ? m.nv : std::min<long long>(m.nv, (long long)nbad * (kStarCap + 1));
-            hipLaunchKernelGGL(k_star_check<1>, star_grid(nchk, 256), dim3(256), 0, s, m.nv,
+            hipLaunchKernelGGL(k_star_check, star_grid(nchk, 256), dim3(256), 0, s, m.nv,
                                pass == 0 ? (const int*)nullptr : (const int*)t->s_rlist,
                                pass == 0 ? (const int*)nullptr : (const int*)(t->s_aux + 8),
                                (const int*)t->star_deg[nxt], (const int*)t->star_nbr[nxt], pass == 0 ? 0 : 1,
