@@ -89,6 +89,21 @@ __device__ __forceinline__ void insert_star(int* tab, int* list, int* count, con
     }
 }
 
+// position of x in the star of u (deg du), -1 if absent; the row is read as
+// 16-byte vectors (rows are kStarCap = 24 ints, 96 bytes, 16-byte aligned)
+__device__ __forceinline__ int star_find(const int* __restrict__ nbr, int u, int du, int x) {
+    const int4* r = reinterpret_cast<const int4*>(nbr + (size_t)u * kStarCap);
+    int pos = -1;
+    for (int k = 0; k < du; k += 4) {
+        const int4 q = r[k >> 2];
+        if (q.w == x && k + 3 < du) pos = k + 3;
+        if (q.z == x && k + 2 < du) pos = k + 2;
+        if (q.y == x && k + 1 < du) pos = k + 1;
+        if (q.x == x) pos = k;
+    }
+    return pos;
+}
+
 constexpr int kStarMaxDepth = 5;  // deepest ring of the previous mesh searched
 
 // The FP32 stage of incircle(a, b, c, v) from precomputed rows (x, y, lift)
@@ -147,7 +162,7 @@ __device__ __forceinline__ int star_orient_f32(float4 A, float4 B) {
 // vertex (reason 5) for a later pass with the full cascade; this variant
 // carries none of the exact-arithmetic code and keeps a small register
 // footprint.
-template <bool kF32, bool kFast, int kL>
+template <bool kF32, bool kFast, int kL, bool kRing2>
 __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ vlist, int nlist,
                                               const int* __restrict__ rows, const int* __restrict__ nrows, int depth,
                                               const int* __restrict__ deg_a, const int* __restrict__ nbr_a,
@@ -162,7 +177,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
     __shared__ int s_tab[kV][kH];
     __shared__ int s_cand[kV][kC];
     __shared__ float2 s_pos[kV][kC];
-    __shared__ float4 s_row[kV][kC];  // (x, y, lift) of candidate - v
+    __shared__ float4 s_row[kV][kC];  // (x, y, lift) of candidate - v, candidate id (bits)
     __shared__ int s_succ[kV][kC];
     __shared__ int s_cnt[kV];
     const int g = threadIdx.x / kL, lane = threadIdx.x % kL;
@@ -179,45 +194,82 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
         int* tab = s_tab[g];
         int* list = s_cand[g];
         int* count = &s_cnt[g];
-        // ---- candidates, de-duplicated in LDS, listed ring by ring
-        for (int k = 0; k < kH / kL; k++) tab[lane + kL * k] = -1;
-        if (lane == 0) *count = 0;
-        __syncthreads();
-        const int db = (active && deg_b) ? deg_b[v] : 0;
-        // a vertex whose star failed (it left its 2-ring) widens through the
-        // previous stars; a valid but inconsistent one expands its first ring
-        // (old and new neighbours) through the new stars only
-        const bool widen = active && deg_b && db < 0;
-        const int my_depth = widen ? max_depth : 2;
-        // first ring, from the previous mesh and from this step's stars
-        if (active) {
-            const int da = deg_a[v];
-            for (int i = lane; i < da && i < kStarCap; i += kL) {
-                const int ni = nbr_a[(size_t)v * kStarCap + i];
-                if ((unsigned)ni < (unsigned)m.nv) cand_insert<kH, kC>(tab, list, count, ni);
+        int kc;
+        if (kRing2) {
+            // ---- candidates straight from the previous (consistent) stars:
+            // v's star, then for each neighbour u_i the part of u_i's star
+            // behind it.  In u_i's star v is followed by u_i-1 and preceded by
+            // u_i+1, whose predecessor there is the vertex opposite edge
+            // (u_i, u_i+1), listed by u_i+1 as the first entry behind it; so
+            // u_i contributes the entries from two after v to three before v.
+            // A vertex can still come twice (two non-consecutive neighbours);
+            // the hull step compares candidates by id, so that is harmless.
+            const int d = active ? deg_a[v] : 0;
+            int u = -1, du = 0, p = -1, cnt = 0;
+            if (lane < d) {
+                u = nbr_a[(size_t)v * kStarCap + lane];
+                du = deg_a[u];
+                p = star_find(nbr_a, u, du, v);
+                cnt = p >= 0 ? (du > 4 ? du - 4 : 0) : du;
             }
-            for (int i = lane; i < db && i < kStarCap; i += kL) {
-                const int ni = nbr_b[(size_t)v * kStarCap + i];
-                if ((unsigned)ni < (unsigned)m.nv) cand_insert<kH, kC>(tab, list, count, ni);
+            int x = cnt;  // inclusive prefix over the group's lanes
+            for (int off = 1; off < kL; off <<= 1) {
+                const int y = __shfl_up(x, off);
+                if (lane >= off) x += y;
             }
-        }
-        __syncthreads();
-        // further rings: expand the entries the previous ring added
-        int lo = 0, hi = *count;
-        for (int dd = 2; dd <= max_depth; dd++) {
-            if (active && dd <= my_depth) {
-                const int top = hi < kC ? hi : kC;
-                for (int i = lo + lane; i < top; i += kL) {
-                    const int u = list[i];
-                    if (!deg_b || widen) insert_star<kH, kC>(tab, list, count, deg_a, nbr_a, u, v, m.nv);
-                    if (deg_b && dd == 2) insert_star<kH, kC>(tab, list, count, deg_b, nbr_b, u, v, m.nv);
+            kc = d + __shfl(x, gbase + kL - 1);
+            if (lane < d && lane < kC) list[lane] = u;
+            int at = d + x - cnt;
+            for (int k = 0; k < cnt; k++, at++) {
+                int q = k;
+                if (p >= 0) {
+                    q = p + 2 + k;
+                    if (q >= du) q -= du;
+                }
+                if (at < kC) list[at] = nbr_a[(size_t)u * kStarCap + q];
+            }
+            __syncthreads();
+        } else {
+            // ---- candidates, de-duplicated in LDS, listed ring by ring
+            for (int k = 0; k < kH / kL; k++) tab[lane + kL * k] = -1;
+            if (lane == 0) *count = 0;
+            __syncthreads();
+            const int db = (active && deg_b) ? deg_b[v] : 0;
+            // a vertex whose star failed (it left its 2-ring) widens through the
+            // previous stars; a valid but inconsistent one expands its first ring
+            // (old and new neighbours) through the new stars only
+            const bool widen = active && deg_b && db < 0;
+            const int my_depth = widen ? max_depth : 2;
+            // first ring, from the previous mesh and from this step's stars
+            if (active) {
+                const int da = deg_a[v];
+                for (int i = lane; i < da && i < kStarCap; i += kL) {
+                    const int ni = nbr_a[(size_t)v * kStarCap + i];
+                    if ((unsigned)ni < (unsigned)m.nv) cand_insert<kH, kC>(tab, list, count, ni);
+                }
+                for (int i = lane; i < db && i < kStarCap; i += kL) {
+                    const int ni = nbr_b[(size_t)v * kStarCap + i];
+                    if ((unsigned)ni < (unsigned)m.nv) cand_insert<kH, kC>(tab, list, count, ni);
                 }
             }
             __syncthreads();
-            lo = hi;
-            hi = *count;
+            // further rings: expand the entries the previous ring added
+            int lo = 0, hi = *count;
+            for (int dd = 2; dd <= max_depth; dd++) {
+                if (active && dd <= my_depth) {
+                    const int top = hi < kC ? hi : kC;
+                    for (int i = lo + lane; i < top; i += kL) {
+                        const int u = list[i];
+                        if (!deg_b || widen) insert_star<kH, kC>(tab, list, count, deg_a, nbr_a, u, v, m.nv);
+                        if (deg_b && dd == 2) insert_star<kH, kC>(tab, list, count, deg_b, nbr_b, u, v, m.nv);
+                    }
+                }
+                __syncthreads();
+                lo = hi;
+                hi = *count;
+            }
+            kc = hi;
         }
-        const int kc = hi;
         // positions of the candidates and their filter rows relative to v
         {
 #pragma clang fp contract(off)
@@ -227,7 +279,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                 s_pos[g][j] = pj;
                 const float x = (float)dexp_value(diff_exact(pj.x, pv.x, 0, periodic));
                 const float y = (float)dexp_value(diff_exact(pj.y, pv.y, 0, periodic));
-                s_row[g][j] = make_float4(x, y, x * x + y * y, 0.0f);
+                s_row[g][j] = make_float4(x, y, x * x + y * y, __int_as_float(list[j]));
             }
         }
         __syncthreads();
@@ -271,36 +323,48 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                 const int j = lane + jj * kL;
                 const bool own = j < kc_g;
                 const float4 rj = s_row[g][own ? j : 0];
-                const int a0 = j == 0 ? 1 : 0;
-                int a = a0;
-                float4 ra = s_row[g][a0];
+                const int idj = __float_as_int(rj.w);
+                int a = -1;  // none yet: the first other candidate is taken as is
+                float4 ra = rj;
                 bool undecided = false;
                 unsigned fast = 0;
                 for (int b = 0; b < kc_w; b++) {
                     const float4 rb = s_row[g][b];
+                    const int idb = __float_as_int(rb.w);
                     const int sg = star_incircle_f32(rj, ra, rb);
-                    const bool use = (b != j) & (b != a0) & (b < kc_g);
-                    undecided |= (sg == 0) & use;
-                    fast += (sg != 0) & use;
-                    const bool win = (sg < 0) & use;
+                    const bool first = a < 0;
+                    const bool use = (b < kc_g) & (idb != idj) & (first | (idb != __float_as_int(ra.w)));
+                    const bool test = use & !first;
+                    undecided |= (sg == 0) & test;
+                    fast += (sg != 0) & test;
+                    const bool win = use & (first | (sg < 0));
                     a = win ? b : a;
                     ra.x = win ? rb.x : ra.x;
                     ra.y = win ? rb.y : ra.y;
                     ra.z = win ? rb.z : ra.z;
+                    ra.w = win ? rb.w : ra.w;
                 }
                 loc[0] += fast;
-                if (own) s_succ[g][j] = undecided ? -1 : a;
+                if (own) s_succ[g][j] = (undecided || a < 0) ? -1 : a;
             }
         } else if (ok) {
             for (int j = lane; j < kc; j += kL) {
                 const float4 rj = s_row[g][j];
-                int a = j == 0 ? 1 : 0;
-                float4 ra = s_row[g][a];
+                const int idj = __float_as_int(rj.w);
+                int a = -1;
+                float4 ra = rj;
                 bool undecided = false;
                 unsigned fast = 0;
-                for (int b = a + 1; b < kc; b++) {
+                for (int b = 0; b < kc; b++) {
                     const float4 rb = s_row[g][b];
-                    if (b == j) continue;
+                    const int idb = __float_as_int(rb.w);
+                    if (idb == idj) continue;
+                    if (a < 0) {
+                        a = b;
+                        ra = rb;
+                        continue;
+                    }
+                    if (idb == __float_as_int(ra.w)) continue;
                     int sg = kF32 ? star_incircle_f32(rj, ra, rb) : 0;
                     if (sg != 0) {
                         fast++;
@@ -313,7 +377,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                     if (sg < 0) { a = b; ra = rb; }
                 }
                 loc[0] += fast;
-                s_succ[g][j] = undecided ? -1 : a;
+                s_succ[g][j] = (undecided || a < 0) ? -1 : a;
             }
         }
         __syncthreads();
@@ -328,7 +392,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                 d++;
                 const int nx = s_succ[g][cur];
                 if (nx < 0) { why = 5; break; }
-                if (nx == start) { closed = true; break; }
+                if (s_cand[g][nx] == s_cand[g][start]) { closed = true; break; }
                 cur = nx;
             }
             if (!closed || d < 3) {
@@ -377,20 +441,6 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
     flush_stats(loc, stats, kStIncircle0);
 }
 
-// position of x in the star of u (deg du), -1 if absent; the row is read as
-// 16-byte vectors (rows are kStarCap = 24 ints, 96 bytes, 16-byte aligned)
-__device__ __forceinline__ int star_find(const int* __restrict__ nbr, int u, int du, int x) {
-    const int4* r = reinterpret_cast<const int4*>(nbr + (size_t)u * kStarCap);
-    int pos = -1;
-    for (int k = 0; k < du; k += 4) {
-        const int4 q = r[k >> 2];
-        if (q.w == x && k + 3 < du) pos = k + 3;
-        if (q.z == x && k + 2 < du) pos = k + 2;
-        if (q.y == x && k + 1 < du) pos = k + 1;
-        if (q.x == x) pos = k;
-    }
-    return pos;
-}
 
 // true when some triangle (v, u, w) of v's star (degree d >= 3) does not read
 // (u, w, v) in u's star.  A vertex of degree <= 8 whose neighbours have

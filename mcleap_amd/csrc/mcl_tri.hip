@@ -2312,10 +2312,16 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     // the filter could not decide)
     auto stars = [&](const int* vlist, int nlist, int depth, const int* bdeg, const int* bnbr, int* odeg, int* onbr) {
         hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_heavy, 1, 0);
-        hipLaunchKernelGGL((k_star<true, true, 32>), star_grid(vlist ? nlist : m.nv, 8), dim3(256), 0, s, m, vlist,
-                           nlist, (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
-                           (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy, t->stats);
-        hipLaunchKernelGGL((k_star<true, false, 64>), dim3(256), dim3(256), 0, s, m, vlist, nlist,
+        if (!vlist && !bdeg)  // first pass: 2-rings read off the previous stars directly
+            hipLaunchKernelGGL((k_star<true, true, 32, true>), star_grid(m.nv, 8), dim3(256), 0, s, m, vlist, nlist,
+                               (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
+                               (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy, t->stats);
+        else
+            hipLaunchKernelGGL((k_star<true, true, 32, false>), star_grid(vlist ? nlist : m.nv, 8), dim3(256), 0, s, m,
+                               vlist, nlist, (const int*)nullptr, (const int*)nullptr, depth,
+                               (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr,
+                               t->s_aux, t->s_heavy, t->stats);
+        hipLaunchKernelGGL((k_star<true, false, 64, false>), dim3(256), dim3(256), 0, s, m, vlist, nlist,
                            (const int*)(t->s_heavy + 1), (const int*)t->s_heavy, depth, (const int*)t->star_deg[cur],
                            (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, (int*)nullptr, t->stats);
     };
