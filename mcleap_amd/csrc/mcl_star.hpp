@@ -36,6 +36,7 @@ constexpr int kCandMax = 128;   // candidates per vertex: two per lane
 constexpr int kHashSlots = 256; // per-wave LDS hash set for candidate de-duplication
 constexpr int kStarPasses = 5;  // star computations per step before the flip-based fallback
 constexpr int kStarAux = 16;    // star-step counters
+constexpr int kStarSyncFrom = 3;  // retry passes queued before the host reads the counts
 
 // stars of the current mesh: CCW rotation from v.he (h <- next(next(h))^1)
 __global__ void __launch_bounds__(256) k_stars_from_mesh(DMesh m, int* __restrict__ deg, int* __restrict__ nbr,
@@ -143,7 +144,8 @@ __device__ __forceinline__ int star_orient_f32(float4 A, float4 B) {
 // arrays deg_a/nbr_a (the previous mesh).  With deg_b/nbr_b (the stars found
 // so far in this step): v's neighbours in both, expanded through deg_b, or,
 // when v's star in deg_b failed (v moved out of its 2-ring), the depth-ring
-// of v in the previous stars plus the 2-ring through deg_b.  Processes list positions 0..nlist-1, or the
+// of v in the previous stars plus the 2-ring through deg_b.  Processes list
+// positions 0..nlist-1 (the count read from *nlist_dev when given), or the
 // *nrows positions in rows when given (vertices vlist[row], or row itself
 // without a list).  Writes deg_out/nbr_out at row idx of the list (at row v
 // without a list); a vertex whose star cannot be computed gets deg_out = -1.
@@ -164,7 +166,8 @@ __device__ __forceinline__ int star_orient_f32(float4 A, float4 B) {
 // footprint.
 template <bool kF32, bool kFast, int kL, bool kRing2>
 __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ vlist, int nlist,
-                                              const int* __restrict__ rows, const int* __restrict__ nrows, int depth,
+                                              const int* __restrict__ nlist_dev, const int* __restrict__ rows,
+                                              const int* __restrict__ nrows, int depth,
                                               const int* __restrict__ deg_a, const int* __restrict__ nbr_a,
                                               const int* __restrict__ deg_b, const int* __restrict__ nbr_b,
                                               int* __restrict__ deg_out, int* __restrict__ nbr_out,
@@ -183,7 +186,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
     const int g = threadIdx.x / kL, lane = threadIdx.x % kL;
     const int gbase = (threadIdx.x & 63) - lane;  // first wave lane of this vertex's group
     const unsigned long long gmask = kL == 64 ? ~0ull : (0xffffffffull << gbase);
-    const int n = rows ? *nrows : (vlist ? nlist : m.nv);
+    const int n = rows ? *nrows : (vlist ? (nlist_dev ? *nlist_dev : nlist) : m.nv);
     const bool periodic = m.periodic != 0;
     const int max_depth = depth < 2 ? 2 : (depth > kStarMaxDepth ? kStarMaxDepth : depth);
     unsigned loc[4] = {0, 0, 0, 0};
@@ -543,20 +546,27 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
     if (my_sum) atomicAdd((unsigned long long*)sum_deg, (unsigned long long)my_sum);
 }
 
-// Copies the recomputed stars (rows by list position) to their vertices,
+// Copies the recomputed stars (rows by list position, *nlist_dev of them) to
+// their vertices, counts them into *retried (and the pass into the int after it),
 // adds the change of the valid-star degree sum to *sum_deg, and lists for
 // re-checking each recomputed vertex with its previous star neighbours (the
 // only vertices whose consistency the new stars can change: a vertex that
 // has a recomputed vertex in its star without being in that vertex's
 // previous star was inconsistent, hence recomputed itself).
-__global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vlist, int nlist,
+__global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vlist, const int* __restrict__ nlist_dev,
                                                       const int* __restrict__ tdeg, const int* __restrict__ tnbr,
                                                       int* __restrict__ deg, int* __restrict__ nbr,
                                                       int* __restrict__ mark, int stamp, int* __restrict__ rlist,
-                                                      int* __restrict__ rcount, long long* __restrict__ sum_deg) {
+                                                      int* __restrict__ rcount, long long* __restrict__ sum_deg,
+                                                      long long* __restrict__ retried) {
     __shared__ int s_tile[2];
     if (threadIdx.x == 0) s_tile[0] = 0;
     __syncthreads();
+    const int nlist = *nlist_dev;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && nlist) {
+        atomicAdd((unsigned long long*)retried, (unsigned long long)nlist);
+        atomicAdd((int*)(retried + 1), 1);  // passes with a non-empty list
+    }
     long long delta = 0;
     for (int base = blockIdx.x * blockDim.x; base < nlist; base += gridDim.x * blockDim.x) {
         const int idx = base + threadIdx.x;

@@ -2306,78 +2306,92 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     auto star_grid = [&](long long nverts, int kv) {
         return dim3((unsigned)std::max<long long>(1, std::min<long long>((nverts + kv - 1) / kv, 4LL * t->grid_blocks)));
     };
-    // stars of the listed vertices (all without a list) into (odeg, onbr):
-    // the filter-only kernel, two vertices per wave, then the full kernel on
-    // the vertices it could not finish (more than 64 candidates, or a test
-    // the filter could not decide)
-    auto stars = [&](const int* vlist, int nlist, int depth, const int* bdeg, const int* bnbr, int* odeg, int* onbr) {
+    // list lengths stay on the device: retries, scatters and re-checks read
+    // them there, with grids sized for a long list (a short one leaves most
+    // blocks idle at once)
+    const dim3 list_grid((unsigned)std::min<long long>(4LL * t->grid_blocks, 2048));
+    // stars of the listed vertices (all without a list; count *nlist_dev)
+    // into (odeg, onbr): the filter-only kernel, two vertices per wave, then
+    // the full kernel on the vertices it could not finish (more than 64
+    // candidates, or a test the filter could not decide)
+    auto stars = [&](const int* vlist, const int* nlist_dev, int depth, const int* bdeg, const int* bnbr, int* odeg,
+                     int* onbr) {
         hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_heavy, 1, 0);
-        if (!vlist && !bdeg)  // first pass: 2-rings read off the previous stars directly
-            hipLaunchKernelGGL((k_star<true, true, 32, true>), star_grid(m.nv, 8), dim3(256), 0, s, m, vlist, nlist,
-                               (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
-                               (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy, t->stats);
-        else
-            hipLaunchKernelGGL((k_star<true, true, 32, false>), star_grid(vlist ? nlist : m.nv, 8), dim3(256), 0, s, m,
-                               vlist, nlist, (const int*)nullptr, (const int*)nullptr, depth,
+        if (!vlist)  // first pass: 2-rings read off the previous stars directly
+            hipLaunchKernelGGL((k_star<true, true, 32, true>), star_grid(m.nv, 8), dim3(256), 0, s, m, vlist, 0,
+                               nlist_dev, (const int*)nullptr, (const int*)nullptr, depth,
                                (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr,
                                t->s_aux, t->s_heavy, t->stats);
-        hipLaunchKernelGGL((k_star<true, false, 64, false>), dim3(256), dim3(256), 0, s, m, vlist, nlist,
+        else
+            hipLaunchKernelGGL((k_star<true, true, 32, false>), list_grid, dim3(256), 0, s, m, vlist, 0, nlist_dev,
+                               (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
+                               (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy, t->stats);
+        hipLaunchKernelGGL((k_star<true, false, 64, false>), dim3(256), dim3(256), 0, s, m, vlist, 0, nlist_dev,
                            (const int*)(t->s_heavy + 1), (const int*)t->s_heavy, depth, (const int*)t->star_deg[cur],
                            (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, (int*)nullptr, t->stats);
     };
     t->timers[5].mark(s);
-    stars(nullptr, 0, 2, nullptr, nullptr, t->star_deg[nxt], t->star_nbr[nxt]);
+    stars(nullptr, nullptr, 2, nullptr, nullptr, t->star_deg[nxt], t->star_nbr[nxt]);
     t->timers[5].mark(s);
     MCL_HIP(hipGetLastError());
     // s_aux: [0] failed stars, [1] listed vertices, [2..3] degree sum of the
-    // valid stars, [4..7] failures by reason, [8] vertices to re-check
-    long long retried = 0;
-    int passes = 1;
-    bool settled = false;
-    int aux[kStarAux];
-    int nbad = 0;
-    for (int pass = 0; pass < kStarPasses; pass++) {
-        // the first check covers every vertex and lists the inconsistent ones;
-        // later ones cover the re-check list and add the star neighbours of
-        // the inconsistent vertices (a wrong star can agree with its neighbours')
+    // valid stars, [4..7] failures by reason, [8] vertices to re-check,
+    // [10] vertices handed to the full kernel, [12..13] vertices retried,
+    // [14] retry passes with a non-empty list
+    //
+    // The first check covers every vertex and lists the inconsistent ones;
+    // later ones cover the re-check list and add the star neighbours of the
+    // inconsistent vertices (a wrong star can agree with its neighbours').
+    auto check = [&](bool first) {
         hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 1, 1, 0);
         const int stamp = ++t->stamp;
-        {
-            const long long nchk = pass == 0 ? m.nv : std::min<long long>(m.nv, (long long)nbad * (kStarCap + 1));
-            hipLaunchKernelGGL(k_star_check, star_grid(nchk, 256), dim3(256), 0, s, m.nv,
-                               pass == 0 ? (const int*)nullptr : (const int*)t->s_rlist,
-                               pass == 0 ? (const int*)nullptr : (const int*)(t->s_aux + 8),
-                               (const int*)t->star_deg[nxt], (const int*)t->star_nbr[nxt], pass == 0 ? 0 : 1,
-                               t->s_mark, stamp, t->s_list, t->s_aux + 1, (long long*)(t->s_aux + 2));
-        }
-        MCL_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_star_check, first ? star_grid(m.nv, 256) : list_grid, dim3(256), 0, s, m.nv,
+                           first ? (const int*)nullptr : (const int*)t->s_rlist,
+                           first ? (const int*)nullptr : (const int*)(t->s_aux + 8), (const int*)t->star_deg[nxt],
+                           (const int*)t->star_nbr[nxt], first ? 0 : 1, t->s_mark, stamp, t->s_list, t->s_aux + 1,
+                           (long long*)(t->s_aux + 2));
+    };
+    int aux[kStarAux];
+    const bool trace = trace_on();
+    auto read_aux = [&](int pass) -> int {
         MCL_HIP(hipMemcpyAsync(aux, t->s_aux, sizeof(aux), hipMemcpyDeviceToHost, s));
         MCL_HIP(hipStreamSynchronize(s));
-        nbad = aux[1];
-        if (trace_on())
+        if (trace)
             fprintf(stderr,
                     "  [stars pass %d] inconsistent %d (failed stars so far %d: candidates %d undecided %d open %d "
                     "outside %d; full-kernel vertices %d)\n",
-                    pass, nbad, aux[0], aux[4], aux[5], aux[6], aux[7], aux[10]);
-        if (nbad == 0) {
-            long long sum_deg;
-            memcpy(&sum_deg, aux + 2, sizeof(sum_deg));
-            settled = sum_deg == (long long)m.nhe;  // each half-edge once: the degree sum is 2E
-            break;
-        }
-        if (pass == kStarPasses - 1) break;
-        // recompute the vertices involved from the 2-rings in the previous
-        // mesh and in the stars found so far; a vertex whose star failed
-        // (it left its 2-ring) searches one ring deeper each pass
-        retried += nbad;
-        passes++;
-        stars(t->s_list, nbad, 2 + pass + 1, t->star_deg[nxt], t->star_nbr[nxt], t->star_tdeg, t->star_tnbr);
+                    pass, aux[1], aux[0], aux[4], aux[5], aux[6], aux[7], aux[10]);
+        return MCL_OK;
+    };
+    check(true);
+    MCL_HIP(hipGetLastError());
+    if (trace) MCL_TRY(read_aux(0));
+    // Retries run without reading the counts back until kStarSyncFrom passes
+    // have been queued (at 1M points under noise of 0.2 spacings three are
+    // needed); after that the host reads the count after every pass.
+    bool settled = false;
+    for (int r = 1; r < kStarPasses; r++) {
+        // recompute the listed vertices from the 2-rings in the previous mesh
+        // and in the stars found so far; a vertex whose star failed (it left
+        // its 2-ring) searches one ring deeper each pass
+        stars(t->s_list, t->s_aux + 1, 2 + r, t->star_deg[nxt], t->star_nbr[nxt], t->star_tdeg, t->star_tnbr);
         hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 8, 1, 0);
         const int rstamp = ++t->stamp;
-        hipLaunchKernelGGL(k_star_scatter, dim3(blocks_for(nbad)), dim3(256), 0, s, (const int*)t->s_list, nbad,
-                           (const int*)t->star_tdeg, (const int*)t->star_tnbr, t->star_deg[nxt], t->star_nbr[nxt],
-                           t->s_mark, rstamp, t->s_rlist, t->s_aux + 8, (long long*)(t->s_aux + 2));
+        hipLaunchKernelGGL(k_star_scatter, list_grid, dim3(256), 0, s, (const int*)t->s_list,
+                           (const int*)(t->s_aux + 1), (const int*)t->star_tdeg, (const int*)t->star_tnbr,
+                           t->star_deg[nxt], t->star_nbr[nxt], t->s_mark, rstamp, t->s_rlist, t->s_aux + 8,
+                           (long long*)(t->s_aux + 2), (long long*)(t->s_aux + 12));
+        check(false);
         MCL_HIP(hipGetLastError());
+        if (trace || r >= kStarSyncFrom || r == kStarPasses - 1) {
+            MCL_TRY(read_aux(r));
+            if (aux[1] == 0) {
+                long long sum_deg;
+                memcpy(&sum_deg, aux + 2, sizeof(sum_deg));
+                settled = sum_deg == (long long)m.nhe;  // each half-edge once: the degree sum is 2E
+                break;
+            }
+        }
     }
     if (!settled) {
         // back to the start of the step; the mesh and its stars are untouched
@@ -2393,7 +2407,9 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     t->mesh_stale = true;
     if (validate_on()) MCL_TRY(ensure_mesh(t, s));
     if (st) {
-        st->iterations = passes;
+        long long retried;
+        memcpy(&retried, aux + 12, sizeof(retried));
+        st->iterations = 1 + aux[14];
         st->star_retries = retried;
     }
     *done = true;
