@@ -154,9 +154,13 @@ MCL_HD Dexp diff_exact(double b, double a, int k, bool periodic) {
     d.n = exp_grow(d.c, n, (double)k);
     return d;
 }
+// the components summed in order; constant indices keep a Dexp in registers
+// (a loop over d.n would index the array at run time and put it in scratch)
 MCL_HD double dexp_value(const Dexp& d) {
     double s = 0.0;
-    for (int i = 0; i < d.n; i++) s += d.c[i];
+    if (d.n > 0) s += d.c[0];
+    if (d.n > 1) s += d.c[1];
+    if (d.n > 2) s += d.c[2];
     return s;
 }
 MCL_HD float min_image_f(float d) {
