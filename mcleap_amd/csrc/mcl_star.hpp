@@ -105,6 +105,30 @@ __device__ __forceinline__ int star_find(const int* __restrict__ nbr, int u, int
     return pos;
 }
 
+// XCD-aware split of n items over the grid: the blocks of a launch are dealt
+// round-robin to the 8 XCDs, each with its own L2, so block b runs on XCD
+// b % 8.  Each XCD gets one contiguous share of the (Hilbert-ordered) items,
+// so that neighbouring items, which gather the same rows, meet in one L2.
+// Yields this block's first item, the end of its XCD's share and the stride
+// (all block-uniform); falls back to the plain grid-stride split when the
+// grid is not a multiple of 8.
+constexpr int kXcds = 8;
+__device__ __forceinline__ void xcd_split(int n, int per_block, int& first, int& end, int& stride) {
+    const int nb = gridDim.x;
+    if (nb % kXcds != 0) {
+        first = blockIdx.x * per_block;
+        end = n;
+        stride = nb * per_block;
+        return;
+    }
+    const int x = blockIdx.x % kXcds, k = blockIdx.x / kXcds, per_xcd = nb / kXcds;
+    const int share = (((n + kXcds - 1) / kXcds) + per_block - 1) / per_block * per_block;
+    const int lo = min(n, x * share);
+    first = lo + k * per_block;
+    end = min(n, lo + share);
+    stride = per_xcd * per_block;
+}
+
 constexpr int kStarMaxDepth = 5;  // deepest ring of the previous mesh searched
 
 // The FP32 stage of incircle(a, b, c, v) from precomputed rows (x, y, lift)
@@ -190,8 +214,10 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
     const bool periodic = m.periodic != 0;
     const int max_depth = depth < 2 ? 2 : (depth > kStarMaxDepth ? kStarMaxDepth : depth);
     unsigned loc[4] = {0, 0, 0, 0};
-    for (int base = blockIdx.x * kV; base < n; base += gridDim.x * kV) {
-        const bool active = base + g < n;
+    int first, end, stride;
+    xcd_split(n, kV, first, end, stride);
+    for (int base = first; base < end; base += stride) {
+        const bool active = base + g < end;
         const int idx = active ? (rows ? rows[base + g] : base + g) : 0;  // output row
         const int v = active ? (vlist ? vlist[idx] : idx) : 0;
         int* tab = s_tab[g];
@@ -516,9 +542,11 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
     __syncthreads();
     long long my_sum = 0;
     const int n = vlist ? *nlist : nv;
-    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    int first, end, stride;
+    xcd_split(n, blockDim.x, first, end, stride);
+    for (int base = first; base < end; base += stride) {
         const int idx = base + threadIdx.x;
-        const bool active = idx < n;
+        const bool active = idx < end;
         const int v = active ? (vlist ? vlist[idx] : idx) : 0;
         const int d = active ? deg[v] : 3;
         const int* row = nbr + (size_t)v * kStarCap;

@@ -2303,13 +2303,16 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     t->timers[2].mark(s);
     hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux, kStarAux, 0);
     // blocks for a list of n vertices, kv vertices per block iteration
+    // (a multiple of 8 when large enough, for the XCD-aware split)
     auto star_grid = [&](long long nverts, int kv) {
-        return dim3((unsigned)std::max<long long>(1, std::min<long long>((nverts + kv - 1) / kv, 4LL * t->grid_blocks)));
+        long long b = std::max<long long>(1, std::min<long long>((nverts + kv - 1) / kv, 4LL * t->grid_blocks));
+        if (b >= 64) b &= ~7LL;
+        return dim3((unsigned)b);
     };
     // list lengths stay on the device: retries, scatters and re-checks read
     // them there, with grids sized for a long list (a short one leaves most
     // blocks idle at once)
-    const dim3 list_grid((unsigned)std::min<long long>(4LL * t->grid_blocks, 2048));
+    const dim3 list_grid((unsigned)(std::min<long long>(4LL * t->grid_blocks, 2048) & ~7LL));
     // stars of the listed vertices (all without a list; count *nlist_dev)
     // into (odeg, onbr): the filter-only kernel, two vertices per wave, then
     // the full kernel on the vertices it could not finish (more than 64
