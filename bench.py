@@ -3,20 +3,27 @@
     python bench.py [--steps K] [--warmup W] [--workload c2] [--gpus 1]
 
 A step is one pass of the hot path over the whole point set: move every
-point, keep the triangulation valid (repair flips / take-backs /
-relocation), Lawson flips back to the exact Delaunay triangulation, one-ring
-CSR, and the uniform-grid FRNN rebuild + query (CSR).  Inputs live in HBM
-before the timed region; the K timed steps are bracketed by
-torch.cuda.synchronize().  Prints one JSON line.
+point, restore the exact Delaunay triangulation (on the torus: per-vertex
+star recomputation with consistency checks; in the box: repair flips /
+take-backs / relocation / Lawson flips), one-ring CSR, and the uniform-grid
+FRNN rebuild + query (CSR).  Inputs live in HBM before the timed region; the
+K timed steps are bracketed by torch.cuda.synchronize().  After the timed
+steps the final mesh is read through the half-edge arrays and checked (every
+edge exactly locally Delaunay, no inverted triangle).  Prints one JSON line.
 
-roofline: the dominant kernel (largest GPU time) among the per-kernel HIP
-event timers of the library, credited with algorithmic bytes per launch from
+roofline: the dominant kernel class (largest GPU time) among the library's
+per-kernel HIP event timers, credited with algorithmic bytes per launch from
 the per-unit model (per point: move 16 B, inversion check 32 B, Delaunay
-check 56 B, grid build 43 B, FRNN count 13 B, FRNN fill 4*k B; per relocated
-vertex: 648 B).  step_roofline: the whole step credited with (212 + 4k) B per
-point.
-cpu_baseline: the oracle's restatement of the reference host path (move,
-untangle B/C passes, worklist Lawson) on the same mesh, one thread.
+check 56 B, star 390 B, grid build 43 B, FRNN count 13 B, FRNN fill 4*k B;
+per relocated vertex: 648 B).  traffic: the HBM bytes per launch of that
+kernel measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated on a copy
+kernel) for this workload, read from profiles/pmc_traffic.json when it holds
+them.  valu_roofline: the same launch's VALU instruction count (SQ_INSTS_VALU,
+same file) over the measured launch time against the issue peak (256 CUs x 4
+SIMDs x one wave64 instruction per 4 cycles at 2.4 GHz).  step_roofline: the
+whole step credited with (212 + 4k) B per point.
+cpu_baseline: the oracle's exact host triangulation rebuilt after each move,
+one thread.
 """
 import argparse
 import json
@@ -31,6 +38,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 PEAK_HBM_GBS = 8000.0
+PEAK_VALU_WAVE_INST = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions per second
 BYTES = {"move": 16, "check": 32, "lawson": 56, "grid_build": 43, "frnn_count": 13}
 RELOC_BYTES = 648  # per relocated vertex: star read 3*6*16 + 7 positions * 8 + rewrite 3*6*16
 STAR_BYTES = 390   # per vertex: own star 28 + 6 neighbour stars 6*28 + ~19 candidate positions 152 + new star 28 + pos 8
@@ -204,6 +212,19 @@ def main():
     algo = algo_total[dom] / max(launches, 1)
     achieved = algo / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     step_bytes = (212.0 + 4.0 * kbar) * n
+    # measured HBM traffic and VALU instructions of the dominant kernel (profiles/)
+    traffic, valu_roof = None, None
+    try:
+        with open(os.path.join(HERE, "profiles", "pmc_traffic.json")) as f:
+            pmc = json.load(f).get(a.workload, {}).get(dom)
+    except (OSError, ValueError):
+        pmc = None
+    if pmc and (a.n == 0):
+        traffic = pmc["read_bytes"] + pmc["write_bytes"]
+        if pmc.get("valu_inst") and avg_ms > 0:
+            rate = pmc["valu_inst"] / (avg_ms * 1e-3)
+            valu_roof = {"valu_inst_per_launch": pmc["valu_inst"], "achieved": rate, "peak": PEAK_VALU_WAVE_INST,
+                         "unit": "wave64 VALU inst/s", "frac": rate / PEAK_VALU_WAVE_INST, "source": pmc["source"]}
 
     # the result of the last timed step, read through the half-edge arrays
     # (rebuilt here from the stars on the torus): every edge exactly
@@ -255,8 +276,9 @@ def main():
                    "amp": cfg["amp"], "frnn_radius": radius, "variant": a.variant,
                    "host_build_s": round(build_s, 3)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "algo_bytes_per_launch": algo, "avg_launch_ms": avg_ms, "launches": launches},
+        "valu_roofline": valu_roof,
         "step_roofline": {"algo_bytes_per_step": step_bytes,
                           "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
                           "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / PEAK_HBM_GBS, "kbar": kbar},

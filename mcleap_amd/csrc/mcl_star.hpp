@@ -315,27 +315,47 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
         bool ok = active && kc >= 3 && kc <= kC;
         int why = ok ? 0 : 4;  // failure reason slot in fail[] (0: none)
         const float2 pv = m.pos[v];
-        // ---- start: the nearest candidate (largest inverted radius); a
-        // candidate on top of v makes the inversion meaningless
-        double dist = 1e300;
+        // ---- start: the nearest candidate (largest inverted radius), always
+        // a hull vertex.  The full kernel takes exact squared distances; the
+        // filter kernel the FP32 lift of the filter rows, which can pick a
+        // near-tie that is not on the hull: the trace then does not close and
+        // the vertex goes to the full kernel.  A candidate on top of v makes
+        // the inversion meaningless.
         int best = kCandMax;
         bool same = false;
-        for (int j = lane; ok && j < kc; j += kL) {
-            const float2 pj = s_pos[g][j];
-            same |= pj.x == pv.x && pj.y == pv.y;
-            const double dx = dexp_value(diff_exact(pj.x, pv.x, 0, periodic));
-            const double dy = dexp_value(diff_exact(pj.y, pv.y, 0, periodic));
-            const double dd = dx * dx + dy * dy;
-            if (dd < dist) { dist = dd; best = j; }
+        if (kFast) {
+            float dist = 3.4e38f;
+            for (int j = lane; ok && j < kc; j += kL) {
+                const float4 rj = s_row[g][j];
+                const float2 pj = s_pos[g][j];
+                same |= pj.x == pv.x && pj.y == pv.y;
+                if (rj.z < dist) { dist = rj.z; best = j; }
+            }
+            for (int off = kL / 2; off >= 1; off >>= 1) {
+                const float od = __shfl_xor(dist, off);
+                const int ob = __shfl_xor(best, off);
+                if (od < dist || (od == dist && ob < best)) { dist = od; best = ob; }
+            }
+        } else {
+            double dist = 1e300;
+            for (int j = lane; ok && j < kc; j += kL) {
+                const float2 pj = s_pos[g][j];
+                same |= pj.x == pv.x && pj.y == pv.y;
+                const double dx = dexp_value(diff_exact(pj.x, pv.x, 0, periodic));
+                const double dy = dexp_value(diff_exact(pj.y, pv.y, 0, periodic));
+                const double dd = dx * dx + dy * dy;
+                if (dd < dist) { dist = dd; best = j; }
+            }
+            for (int off = kL / 2; off >= 1; off >>= 1) {
+                const double od = __shfl_xor(dist, off);
+                const int ob = __shfl_xor(best, off);
+                if (od < dist || (od == dist && ob < best)) { dist = od; best = ob; }
+            }
         }
         if ((__ballot(same) & gmask) && ok) { ok = false; why = 4; }
-        for (int off = kL / 2; off >= 1; off >>= 1) {
-            const double od = __shfl_xor(dist, off);
-            const int ob = __shfl_xor(best, off);
-            if (od < dist || (od == dist && ob < best)) { dist = od; best = ob; }
-        }
         const int start = best;
         if (start >= kc && ok) { ok = false; why = 4; }
+        const int id_start = ok ? __float_as_int(s_row[g][start].w) : -1;
         // ---- successors: for each owned candidate j, the most clockwise other
         // candidate as seen from j' (b beats a when orient(j', a', b') < 0,
         // i.e. incircle(j, a, b, v) < 0); -1 when a test was undecided
@@ -374,7 +394,11 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                     ra.w = win ? rb.w : ra.w;
                 }
                 loc[0] += fast;
-                if (own) s_succ[g][j] = (undecided || a < 0) ? -1 : a;
+                // successor, offset by kC when it is the start (same id)
+                if (own) {
+                    const bool to_start = a >= 0 && __float_as_int(ra.w) == id_start;
+                    s_succ[g][j] = (undecided || a < 0) ? -1 : (to_start ? a + kC : a);
+                }
             }
         } else if (ok) {
             for (int j = lane; j < kc; j += kL) {
@@ -406,7 +430,8 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                     if (sg < 0) { a = b; ra = rb; }
                 }
                 loc[0] += fast;
-                s_succ[g][j] = (undecided || a < 0) ? -1 : a;
+                const bool to_start = a >= 0 && __float_as_int(ra.w) == id_start;
+                s_succ[g][j] = (undecided || a < 0) ? -1 : (to_start ? a + kC : a);
             }
         }
         __syncthreads();
@@ -421,7 +446,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                 d++;
                 const int nx = s_succ[g][cur];
                 if (nx < 0) { why = 5; break; }
-                if (s_cand[g][nx] == s_cand[g][start]) { closed = true; break; }
+                if (nx >= kC) { closed = true; break; }  // back at the start
                 cur = nx;
             }
             if (!closed || d < 3) {
