@@ -129,6 +129,8 @@ __device__ __forceinline__ void xcd_split(int n, int per_block, int& first, int&
     stride = per_xcd * per_block;
 }
 
+constexpr float kStarIccBound = 16.0f * 5.9604644775390625e-08f;  // the scan's incircle filter, x permanent
+
 constexpr int kStarMaxDepth = 5;  // deepest ring of the previous mesh searched
 
 // The FP32 stage of incircle(a, b, c, v) from precomputed rows (x, y, lift)
@@ -371,33 +373,47 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
             for (int jj = 0; jj * kL < kc_w; jj++) {
                 const int j = lane + jj * kL;
                 const bool own = j < kc_g;
-                const float4 rj = s_row[g][own ? j : 0];
-                const int idj = __float_as_int(rj.w);
-                int a = -1;  // none yet: the first other candidate is taken as is
-                float4 ra = rj;
-                bool undecided = false;
-                unsigned fast = 0;
+                const float4 A = s_row[g][own ? j : 0];
+                const int idj = __float_as_int(A.w);
+                // the first best: slot 0, or slot 1 when slot 0 is j itself
+                // (two leading copies of j are left to the full kernel)
+                const float4 r0 = s_row[g][0], r1 = s_row[g][1];
+                const bool take1 = __float_as_int(r0.w) == idj;
+                int a = take1 ? 1 : 0;
+                float4 B = take1 ? r1 : r0;
+                bool undecided = __float_as_int(B.w) == idj;
+                // the minor and permanent part that depend on A and B only
+                float m3 = A.x * B.y - B.x * A.y;
+                float s3 = fabsf(A.x * B.y) + fabsf(B.x * A.y);
                 for (int b = 0; b < kc_w; b++) {
-                    const float4 rb = s_row[g][b];
-                    const int idb = __float_as_int(rb.w);
-                    const int sg = star_incircle_f32(rj, ra, rb);
-                    const bool first = a < 0;
-                    const bool use = (b < kc_g) & (idb != idj) & (first | (idb != __float_as_int(ra.w)));
-                    const bool test = use & !first;
-                    undecided |= (sg == 0) & test;
-                    fast += (sg != 0) & test;
-                    const bool win = use & (first | (sg < 0));
+                    const float4 C = s_row[g][b];
+                    const int idb = __float_as_int(C.w);
+                    // incircle(j, a, b, v) from the rows, FMA form; the bound
+                    // 16 eps * permanent covers this evaluation order (about
+                    // 11 eps from input, product, difference and FMA roundings)
+                    const float bxcy = B.x * C.y, cxby = C.x * B.y;
+                    const float cxay = C.x * A.y, axcy = A.x * C.y;
+                    const float m1 = bxcy - cxby, m2 = cxay - axcy;
+                    const float s1 = fabsf(bxcy) + fabsf(cxby), s2 = fabsf(cxay) + fabsf(axcy);
+                    const float det = fmaf(A.z, m1, fmaf(B.z, m2, C.z * m3));
+                    const float perm = fmaf(A.z, s1, fmaf(B.z, s2, C.z * s3));
+                    const bool sure = (perm > 7.9e-31f) & (fabsf(det) > kStarIccBound * perm);
+                    const bool use = (b < kc_g) & (idb != idj) & (idb != __float_as_int(B.w));
+                    undecided |= use & !sure;
+                    const bool win = use & sure & (det < 0.0f);
                     a = win ? b : a;
-                    ra.x = win ? rb.x : ra.x;
-                    ra.y = win ? rb.y : ra.y;
-                    ra.z = win ? rb.z : ra.z;
-                    ra.w = win ? rb.w : ra.w;
+                    B.x = win ? C.x : B.x;
+                    B.y = win ? C.y : B.y;
+                    B.z = win ? C.z : B.z;
+                    B.w = win ? C.w : B.w;
+                    m3 = win ? -m2 : m3;
+                    s3 = win ? s2 : s3;
                 }
-                loc[0] += fast;
+                if (own && kc_g > 2) loc[0] += (unsigned)(kc_g - 2);  // filter tests (duplicates aside)
                 // successor, offset by kC when it is the start (same id)
                 if (own) {
-                    const bool to_start = a >= 0 && __float_as_int(ra.w) == id_start;
-                    s_succ[g][j] = (undecided || a < 0) ? -1 : (to_start ? a + kC : a);
+                    const bool to_start = __float_as_int(B.w) == id_start;
+                    s_succ[g][j] = undecided ? -1 : (to_start ? a + kC : a);
                 }
             }
         } else if (ok) {
