@@ -569,6 +569,21 @@ __device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, c
     return false;
 }
 
+// Appends the s_fill items of an LDS buffer to a global list (one atomic,
+// coalesced copy) and empties the buffer.  Called by every thread of the
+// block, after a barrier.
+constexpr int kCheckBuf = 2048;
+__device__ __forceinline__ void star_flush(const int* s_buf, int* s_fill, int* s_base, int* count,
+                                           int* __restrict__ list) {
+    if (threadIdx.x == 0) *s_base = atomicAdd(count, *s_fill);
+    __syncthreads();
+    const int nf = *s_fill, base = *s_base;
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) list[base + i] = s_buf[i];
+    __syncthreads();
+    if (threadIdx.x == 0) *s_fill = 0;
+    __syncthreads();
+}
+
 // Checks the *nlist listed vertices (all when vlist is null), one thread per
 // vertex.  Vertices that fail (or whose star failed) are appended to `list`,
 // with their star neighbours when mark_nbrs is set.  Without a list the
@@ -579,7 +594,12 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
                                                     int* __restrict__ list, int* __restrict__ count,
                                                     long long* __restrict__ sum_deg) {
     __shared__ int s_tile[2];
-    if (threadIdx.x == 0) s_tile[0] = 0;
+    __shared__ int s_buf[kCheckBuf];  // appends of the first check, flushed once per few tiles
+    __shared__ int s_fill, s_base;
+    if (threadIdx.x == 0) {
+        s_tile[0] = 0;
+        s_fill = 0;
+    }
     __syncthreads();
     long long my_sum = 0;
     const int n = vlist ? *nlist : nv;
@@ -607,10 +627,29 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
                     nclaim++;
                 }
         }
-        int at = block_reserve(count, nclaim, s_tile);
-        if (claimed & 1u) list[at++] = v;
-        for (int i = 0; i < d && i < kStarCap; i++)
-            if (claimed & (2u << i)) list[at++] = row[i];
+        if (!mark_nbrs) {
+            // at most one item per thread: collect in LDS, one global
+            // reservation and a coalesced copy per flush (a single counter
+            // serialises its atomics, one per tile was a large part of the
+            // first check at 1M vertices)
+            const unsigned long long bal = __ballot(claimed != 0u);
+            const int lane = threadIdx.x & 63;
+            int woff = 0;
+            if (lane == 0 && bal) woff = atomicAdd(&s_fill, __popcll(bal));
+            woff = __shfl(woff, 0);
+            if (claimed) s_buf[woff + __popcll(bal & ((1ull << lane) - 1ull))] = v;
+            __syncthreads();
+            if (s_fill > kCheckBuf - (int)blockDim.x) star_flush(s_buf, &s_fill, &s_base, count, list);
+        } else {
+            int at = block_reserve(count, nclaim, s_tile);
+            if (claimed & 1u) list[at++] = v;
+            for (int i = 0; i < d && i < kStarCap; i++)
+                if (claimed & (2u << i)) list[at++] = row[i];
+        }
+    }
+    if (!mark_nbrs) {
+        __syncthreads();
+        if (s_fill > 0) star_flush(s_buf, &s_fill, &s_base, count, list);
     }
     if (my_sum) atomicAdd((unsigned long long*)sum_deg, (unsigned long long)my_sum);
 }

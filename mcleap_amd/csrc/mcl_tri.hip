@@ -2348,7 +2348,9 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     auto check = [&](bool first) {
         hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 1, 1, 0);
         const int stamp = ++t->stamp;
-        hipLaunchKernelGGL(k_star_check, first ? star_grid(m.nv, 256) : list_grid, dim3(256), 0, s, m.nv,
+        // the first check: 4 tiles of 256 vertices per block, its appends
+        // flushed once per block
+        hipLaunchKernelGGL(k_star_check, first ? star_grid(m.nv, 1024) : list_grid, dim3(256), 0, s, m.nv,
                            first ? (const int*)nullptr : (const int*)t->s_rlist,
                            first ? (const int*)nullptr : (const int*)(t->s_aux + 8), (const int*)t->star_deg[nxt],
                            (const int*)t->star_nbr[nxt], first ? 0 : 1, t->s_mark, stamp, t->s_list, t->s_aux + 1,
