@@ -569,19 +569,53 @@ __device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, c
     return false;
 }
 
-// Appends the s_fill items of an LDS buffer to a global list (one atomic,
-// coalesced copy) and empties the buffer.  Called by every thread of the
-// block, after a barrier.
-constexpr int kCheckBuf = 2048;
-__device__ __forceinline__ void star_flush(const int* s_buf, int* s_fill, int* s_base, int* count,
-                                           int* __restrict__ list) {
-    if (threadIdx.x == 0) *s_base = atomicAdd(count, *s_fill);
+// Block-buffered appends to a global list: items collect in an LDS buffer
+// and go out with one atomic reservation and a coalesced copy per flush (a
+// single counter serialises its atomics; one reservation per 256-thread tile
+// was a large part of the check at 1M vertices).
+constexpr int kAppendBuf = 8192;  // ints of LDS per block
+struct AppendBuf {
+    int buf[kAppendBuf];
+    int fill, base;
+};
+__device__ __forceinline__ void append_init(AppendBuf& ab) {
+    if (threadIdx.x == 0) ab.fill = 0;
     __syncthreads();
-    const int nf = *s_fill, base = *s_base;
-    for (int i = threadIdx.x; i < nf; i += blockDim.x) list[base + i] = s_buf[i];
+}
+// this thread's offset for n items (a wave prefix sum, one LDS atomic per wave)
+__device__ __forceinline__ int append_reserve(AppendBuf& ab, int n) {
+    const int lane = threadIdx.x & 63;
+    int x = n;
+#pragma unroll
+    for (int dd = 1; dd < 64; dd <<= 1) {
+        const int y = __shfl_up(x, dd);
+        if (lane >= dd) x += y;
+    }
+    const int wtot = __shfl(x, 63);
+    int woff = 0;
+    if (lane == 63 && wtot) woff = atomicAdd(&ab.fill, wtot);
+    woff = __shfl(woff, 63);
+    return woff + x - n;
+}
+// writes the buffer out and empties it; every thread of the block, after a barrier
+__device__ __forceinline__ void append_flush(AppendBuf& ab, int* count, int* __restrict__ list) {
+    if (threadIdx.x == 0 && ab.fill) ab.base = atomicAdd(count, ab.fill);
     __syncthreads();
-    if (threadIdx.x == 0) *s_fill = 0;
+    const int nf = ab.fill, base = ab.base;
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) list[base + i] = ab.buf[i];
     __syncthreads();
+    if (threadIdx.x == 0) ab.fill = 0;
+    __syncthreads();
+}
+// after a tile's items are written (and a barrier): flush when the next tile,
+// up to max_tile items, might not fit
+__device__ __forceinline__ void append_tile_end(AppendBuf& ab, int max_tile, int* count, int* __restrict__ list) {
+    __syncthreads();
+    if (ab.fill > kAppendBuf - max_tile) append_flush(ab, count, list);
+}
+__device__ __forceinline__ void append_finish(AppendBuf& ab, int* count, int* __restrict__ list) {
+    __syncthreads();
+    if (ab.fill > 0) append_flush(ab, count, list);
 }
 
 // Checks the *nlist listed vertices (all when vlist is null), one thread per
@@ -593,14 +627,9 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
                                                     int mark_nbrs, int* __restrict__ mark, int stamp,
                                                     int* __restrict__ list, int* __restrict__ count,
                                                     long long* __restrict__ sum_deg) {
-    __shared__ int s_tile[2];
-    __shared__ int s_buf[kCheckBuf];  // appends of the first check, flushed once per few tiles
-    __shared__ int s_fill, s_base;
-    if (threadIdx.x == 0) {
-        s_tile[0] = 0;
-        s_fill = 0;
-    }
-    __syncthreads();
+    __shared__ AppendBuf ab;
+    append_init(ab);
+    const int max_tile = (int)blockDim.x * (mark_nbrs ? kStarCap + 1 : 1);
     long long my_sum = 0;
     const int n = vlist ? *nlist : nv;
     int first, end, stride;
@@ -627,30 +656,13 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
                     nclaim++;
                 }
         }
-        if (!mark_nbrs) {
-            // at most one item per thread: collect in LDS, one global
-            // reservation and a coalesced copy per flush (a single counter
-            // serialises its atomics, one per tile was a large part of the
-            // first check at 1M vertices)
-            const unsigned long long bal = __ballot(claimed != 0u);
-            const int lane = threadIdx.x & 63;
-            int woff = 0;
-            if (lane == 0 && bal) woff = atomicAdd(&s_fill, __popcll(bal));
-            woff = __shfl(woff, 0);
-            if (claimed) s_buf[woff + __popcll(bal & ((1ull << lane) - 1ull))] = v;
-            __syncthreads();
-            if (s_fill > kCheckBuf - (int)blockDim.x) star_flush(s_buf, &s_fill, &s_base, count, list);
-        } else {
-            int at = block_reserve(count, nclaim, s_tile);
-            if (claimed & 1u) list[at++] = v;
-            for (int i = 0; i < d && i < kStarCap; i++)
-                if (claimed & (2u << i)) list[at++] = row[i];
-        }
+        int at = append_reserve(ab, nclaim);
+        if (claimed & 1u) ab.buf[at++] = v;
+        for (int i = 0; i < d && i < kStarCap; i++)
+            if (claimed & (2u << i)) ab.buf[at++] = row[i];
+        append_tile_end(ab, max_tile, count, list);
     }
-    if (!mark_nbrs) {
-        __syncthreads();
-        if (s_fill > 0) star_flush(s_buf, &s_fill, &s_base, count, list);
-    }
+    append_finish(ab, count, list);
     if (my_sum) atomicAdd((unsigned long long*)sum_deg, (unsigned long long)my_sum);
 }
 
@@ -667,9 +679,8 @@ __global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vl
                                                       int* __restrict__ mark, int stamp, int* __restrict__ rlist,
                                                       int* __restrict__ rcount, long long* __restrict__ sum_deg,
                                                       long long* __restrict__ retried) {
-    __shared__ int s_tile[2];
-    if (threadIdx.x == 0) s_tile[0] = 0;
-    __syncthreads();
+    __shared__ AppendBuf ab;
+    append_init(ab);
     const int nlist = *nlist_dev;
     if (blockIdx.x == 0 && threadIdx.x == 0 && nlist) {
         atomicAdd((unsigned long long*)retried, (unsigned long long)nlist);
@@ -691,17 +702,19 @@ __global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vl
                     nclaim++;
                 }
         }
-        int at = block_reserve(rcount, nclaim, s_tile);
-        if (claimed & 1u) rlist[at++] = v;
+        int at = append_reserve(ab, nclaim);
+        if (claimed & 1u) ab.buf[at++] = v;
         for (int i = 0; i < od && i < kStarCap; i++)
-            if (claimed & (2u << i)) rlist[at++] = nbr[(size_t)v * kStarCap + i];
+            if (claimed & (2u << i)) ab.buf[at++] = nbr[(size_t)v * kStarCap + i];
         if (active) {
             const int nd = tdeg[idx];
             delta += (nd >= 3 ? nd : 0) - (od >= 3 ? od : 0);
             for (int k = 0; k < nd; k++) nbr[(size_t)v * kStarCap + k] = tnbr[(size_t)idx * kStarCap + k];
             deg[v] = nd;
         }
+        append_tile_end(ab, (int)blockDim.x * (kStarCap + 1), rcount, rlist);
     }
+    append_finish(ab, rcount, rlist);
     if (delta) atomicAdd((unsigned long long*)sum_deg, (unsigned long long)delta);
 }
 
