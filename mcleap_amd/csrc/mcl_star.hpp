@@ -512,12 +512,40 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
 }
 
 
-// true when some triangle (v, u, w) of v's star (degree d >= 3) does not read
-// (u, w, v) in u's star.  A vertex of degree <= 8 whose neighbours have
-// degree <= 8 (nearly all) is checked with every load issued up front:
-// its row, the neighbours' degrees, the neighbours' first 8 entries.
+// Which of two triangles on the same side of edge (v, u) -- v's (v, u, w)
+// and u's (v, u, w') -- is not Delaunay: 1 when w' lies inside
+// circle(v, u, w) (v's triangle loses), 0 when it does not (u's loses), -1
+// when the FP32 filter cannot tell.  Rows relative to w' as in the star
+// kernel (exact minimum-image differences rounded to FP32), the FP32 stage of
+// incircle(v, u, w, w').
+__device__ __forceinline__ int star_conflict(const float2* __restrict__ pos, int v, int u, int w, int wp,
+                                             bool periodic) {
+#pragma clang fp contract(off)
+    const float2 q = pos[wp];
+    const int ids[3] = {v, u, w};
+    float4 R[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float2 p = pos[ids[k]];
+        const float x = (float)dexp_value(diff_exact(p.x, q.x, 0, periodic));
+        const float y = (float)dexp_value(diff_exact(p.y, q.y, 0, periodic));
+        R[k] = make_float4(x, y, x * x + y * y, 0.0f);
+    }
+    const int sg = star_incircle_f32(R[0], R[1], R[2]);
+    return sg > 0 ? 1 : (sg < 0 ? 0 : -1);
+}
+
+// true when v's star (degree d >= 3) must be recomputed.  A triangle (v, u, w)
+// of v's star is consistent when w directly precedes v in u's star.  Without
+// `decide` any inconsistent triangle marks v; with it, v is marked only when
+// v is missing from u's star or v's triangle loses against u's (v, u, w')
+// (star_conflict), the other vertex being marked by its own check.  A vertex
+// of degree <= 8 whose neighbours have degree <= 8 (nearly all) is checked
+// with every load issued up front: its row, the neighbours' degrees, the
+// neighbours' first 8 entries.
 __device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, const int* __restrict__ nbr, int v,
-                                                  int d) {
+                                                  int d, bool decide, const float2* __restrict__ pos,
+                                                  bool periodic) {
     const int* row = nbr + (size_t)v * kStarCap;
     if (d <= 8) {
         const int4* rv = reinterpret_cast<const int4*>(row);
@@ -539,19 +567,43 @@ __device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, c
             }
         }
         bool bad = false, slow = false;
+        unsigned conf = 0;  // triangles to decide by star_conflict
+        int pv[8];          // per triangle: the entry before v in u's star
 #pragma unroll
         for (int i = 0; i < 8; i++) {
+            pv[i] = -1;
             if (i >= d) continue;
             const int w = (i + 1 < d) ? r[(i + 1) & 7] : r[0];
             const int n = du[i];
             if (n < 3) { bad = true; continue; }
             if (n > 8) { slow = true; continue; }
             const int e[8] = {lo[i].x, lo[i].y, lo[i].z, lo[i].w, hi[i].x, hi[i].y, hi[i].z, hi[i].w};
-            int nx = -1;
+            int last = e[0];
+#pragma unroll
+            for (int k = 1; k < 8; k++) last = (k == n - 1) ? e[k] : last;
+            int pred = -1;
 #pragma unroll
             for (int k = 0; k < 8; k++)
-                if (k < n && e[k] == w) nx = (k + 1 < n) ? e[(k + 1) & 7] : e[0];
-            if (nx != v) bad = true;
+                if (k < n && e[k] == v) pred = k > 0 ? e[k - 1] : last;
+            pv[i] = pred;
+            if (pred == w) continue;
+            if (!decide || pred < 0) bad = true;
+            else conf |= 1u << i;
+        }
+        // the conflicts, one at a time (the test appears once in the code)
+#pragma unroll 1
+        while (conf && !bad) {
+            const int i = __ffs(conf) - 1;
+            conf &= conf - 1;
+            int u = 0, w = 0, p = -1;
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (k == i) {
+                    u = r[k];
+                    w = (k + 1 < d) ? r[(k + 1) & 7] : r[0];
+                    p = pv[k];
+                }
+            if (star_conflict(pos, v, u, w, p, periodic) != 0) bad = true;
         }
         if (bad || !slow) return bad;
     }
@@ -562,8 +614,9 @@ __device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, c
         const int w = (i + 1 == d) ? first : row[i + 1];
         const int du = deg[u];
         if (du < 3) return true;
-        const int j = star_find(nbr, u, du, w);
-        if (j < 0 || nbr[(size_t)u * kStarCap + (j + 1 == du ? 0 : j + 1)] != v) return true;
+        const int j = star_find(nbr, u, du, v);
+        const int pred = j < 0 ? -1 : nbr[(size_t)u * kStarCap + (j == 0 ? du - 1 : j - 1)];
+        if (pred != w && (!decide || pred < 0 || star_conflict(pos, v, u, w, pred, periodic) != 0)) return true;
         u = w;
     }
     return false;
@@ -619,11 +672,14 @@ __device__ __forceinline__ void append_finish(AppendBuf& ab, int* count, int* __
 }
 
 // Checks the *nlist listed vertices (all when vlist is null), one thread per
-// vertex.  Vertices that fail (or whose star failed) are appended to `list`,
-// with their star neighbours when mark_nbrs is set.  Without a list the
+// vertex.  Vertices that fail (or whose star failed) are appended to `list`;
+// with mark_nbrs set, with their star neighbours and any inconsistency marks
+// v, otherwise only the side of a conflict that loses the incircle test
+// (star_inconsistent with `decide`).  Without a list the
 // degree sum of the valid stars is added to *sum_deg.
 __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restrict__ vlist, const int* __restrict__ nlist,
                                                     const int* __restrict__ deg, const int* __restrict__ nbr,
+                                                    const float2* __restrict__ pos, int periodic,
                                                     int mark_nbrs, int* __restrict__ mark, int stamp,
                                                     int* __restrict__ list, int* __restrict__ count,
                                                     long long* __restrict__ sum_deg) {
@@ -643,7 +699,7 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
         bool bad = active && d < 3;
         if (active && !bad) {
             if (!vlist) my_sum += d;
-            bad = star_inconsistent(deg, nbr, v, d);
+            bad = star_inconsistent(deg, nbr, v, d, !mark_nbrs, pos, periodic != 0);
         }
         // claim v and its neighbours once per stamp, then append the claimed ones
         unsigned claimed = 0;  // bit 0: v, bit 1 + i: neighbour i

@@ -2353,7 +2353,8 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         hipLaunchKernelGGL(k_star_check, first ? star_grid(m.nv, 1024) : list_grid, dim3(256), 0, s, m.nv,
                            first ? (const int*)nullptr : (const int*)t->s_rlist,
                            first ? (const int*)nullptr : (const int*)(t->s_aux + 8), (const int*)t->star_deg[nxt],
-                           (const int*)t->star_nbr[nxt], first ? 0 : 1, t->s_mark, stamp, t->s_list, t->s_aux + 1,
+                           (const int*)t->star_nbr[nxt], (const float2*)m.pos, m.periodic, first ? 0 : 1, t->s_mark,
+                           stamp, t->s_list, t->s_aux + 1,
                            (long long*)(t->s_aux + 2));
     };
     int aux[kStarAux];
