@@ -36,6 +36,7 @@ constexpr int kCandMax = 128;   // candidates per vertex: two per lane
 constexpr int kHashSlots = 256; // per-wave LDS hash set for candidate de-duplication
 constexpr int kStarPasses = 5;  // star computations per step before the flip-based fallback
 constexpr int kStarAux = 16;    // star-step counters
+constexpr int kStarDecidePasses = 2;  // checks that list only the losing side of each conflict
 constexpr int kStarSyncFrom = 3;  // retry passes queued before the host reads the counts
 
 // stars of the current mesh: CCW rotation from v.he (h <- next(next(h))^1)

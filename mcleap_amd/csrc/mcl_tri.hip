@@ -2345,7 +2345,9 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     // The first check covers every vertex and lists the inconsistent ones;
     // later ones cover the re-check list and add the star neighbours of the
     // inconsistent vertices (a wrong star can agree with its neighbours').
-    auto check = [&](bool first) {
+    // mark_nbrs: list the star neighbours of the inconsistent vertices too
+    // (otherwise each conflict lists only the side that loses its incircle test)
+    auto check = [&](bool first, bool mark_nbrs) {
         hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 1, 1, 0);
         const int stamp = ++t->stamp;
         // the first check: 4 tiles of 256 vertices per block, its appends
@@ -2353,7 +2355,7 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         hipLaunchKernelGGL(k_star_check, first ? star_grid(m.nv, 1024) : list_grid, dim3(256), 0, s, m.nv,
                            first ? (const int*)nullptr : (const int*)t->s_rlist,
                            first ? (const int*)nullptr : (const int*)(t->s_aux + 8), (const int*)t->star_deg[nxt],
-                           (const int*)t->star_nbr[nxt], (const float2*)m.pos, m.periodic, first ? 0 : 1, t->s_mark,
+                           (const int*)t->star_nbr[nxt], (const float2*)m.pos, m.periodic, mark_nbrs ? 1 : 0, t->s_mark,
                            stamp, t->s_list, t->s_aux + 1,
                            (long long*)(t->s_aux + 2));
     };
@@ -2369,7 +2371,7 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
                     pass, aux[1], aux[0], aux[4], aux[5], aux[6], aux[7], aux[10]);
         return MCL_OK;
     };
-    check(true);
+    check(true, false);
     MCL_HIP(hipGetLastError());
     if (trace) MCL_TRY(read_aux(0));
     // Retries run without reading the counts back until kStarSyncFrom passes
@@ -2387,7 +2389,7 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
                            (const int*)(t->s_aux + 1), (const int*)t->star_tdeg, (const int*)t->star_tnbr,
                            t->star_deg[nxt], t->star_nbr[nxt], t->s_mark, rstamp, t->s_rlist, t->s_aux + 8,
                            (long long*)(t->s_aux + 2), (long long*)(t->s_aux + 12));
-        check(false);
+        check(false, r >= kStarDecidePasses);
         MCL_HIP(hipGetLastError());
         if (trace || r >= kStarSyncFrom || r == kStarPasses - 1) {
             MCL_TRY(read_aux(r));
