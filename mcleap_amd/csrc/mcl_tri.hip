@@ -1505,6 +1505,8 @@ struct mcl_tri {
     int* deg = nullptr;
     int* scan_tmp = nullptr;
     int* h_cnt = nullptr;  // pinned
+    unsigned long long* h_stats = nullptr;  // pinned copy of the statistics shards
+    bool h_stats_fresh = false;             // h_stats holds the counters of the current step
     KTimer timers[6];      // Lawson full round, check all, move, relocation apply, take-back apply, star recomputation
     int tail_threshold = 8192;  // queue length below which one workgroup finishes the rounds
     int revert_tail = 1024;     // bad-triangle count below which one workgroup finishes the take-backs
@@ -1552,6 +1554,7 @@ static int tri_free(mcl_tri* t) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (t->h_cnt) (void)hipHostFree(t->h_cnt);
+    if (t->h_stats) (void)hipHostFree(t->h_stats);
     return MCL_OK;
 }
 
@@ -1608,6 +1611,9 @@ extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv,
     }
 #undef A
     if (rc == MCL_OK && hipHostMalloc((void**)&t->h_cnt, sizeof(int) * kCnCount) != hipSuccess) rc = MCL_ERR_HIP;
+    if (rc == MCL_OK &&
+        hipHostMalloc((void**)&t->h_stats, sizeof(unsigned long long) * kStCount * kStatShards) != hipSuccess)
+        rc = MCL_ERR_HIP;
     if (rc == MCL_OK) rc = upload_aos(t, pos, v2, he4, t1);
     if (rc == MCL_OK) {
         hipLaunchKernelGGL(k_set_u64, dim3(blocks_for(nt)), dim3(kThreads), 0, 0, t->claim, nt, 0ull);
@@ -1826,13 +1832,18 @@ static int run_rounds(mcl_tri* t, bool all, int variant, int max_rounds, hipStre
 }
 
 static void stats_begin(mcl_tri* t, hipStream_t s) {
+    t->h_stats_fresh = false;
     hipLaunchKernelGGL(k_set_u64, dim3(4), dim3(256), 0, s, t->stats, (int)(kStCount * kStatShards), 0ull);
 }
 // the statistics counters summed over their shards
 static int stats_read(mcl_tri* t, hipStream_t s, unsigned long long (&h)[kStCount]) {
-    std::vector<unsigned long long> raw((size_t)kStCount * kStatShards);
-    MCL_HIP(hipMemcpyAsync(raw.data(), t->stats, sizeof(unsigned long long) * raw.size(), hipMemcpyDeviceToHost, s));
-    MCL_HIP(hipStreamSynchronize(s));
+    const size_t nraw = (size_t)kStCount * kStatShards;
+    const unsigned long long* raw = t->h_stats;
+    if (!t->h_stats_fresh) {
+        MCL_HIP(hipMemcpyAsync(t->h_stats, t->stats, sizeof(unsigned long long) * nraw, hipMemcpyDeviceToHost, s));
+        MCL_HIP(hipStreamSynchronize(s));
+    }
+    t->h_stats_fresh = false;
     for (int k = 0; k < kStCount; k++) {
         unsigned long long a = 0;
         for (int j = 0; j < kStatShards; j++) a += raw[(size_t)k * kStatShards + j];
@@ -2361,9 +2372,13 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     };
     int aux[kStarAux];
     const bool trace = trace_on();
+    // the counters, and the statistics with them (one synchronisation)
     auto read_aux = [&](int pass) -> int {
         MCL_HIP(hipMemcpyAsync(aux, t->s_aux, sizeof(aux), hipMemcpyDeviceToHost, s));
+        MCL_HIP(hipMemcpyAsync(t->h_stats, t->stats, sizeof(unsigned long long) * kStCount * kStatShards,
+                               hipMemcpyDeviceToHost, s));
         MCL_HIP(hipStreamSynchronize(s));
+        t->h_stats_fresh = true;
         if (trace)
             fprintf(stderr,
                     "  [stars pass %d] inconsistent %d (failed stars so far %d: candidates %d undecided %d open %d "
