@@ -40,6 +40,7 @@ std::vector<int> hilbert_order(const float* xy, int n);
 struct DMesh {
     int nv = 0, nhe = 0, nt = 0, ne = 0;
     int periodic = 0;
+    int ncorner = 0;  // corner vertices come first (box: ids 0..3, outer face through them)
     float2* pos = nullptr;
     int* he_next = nullptr;
     int* he_v = nullptr;

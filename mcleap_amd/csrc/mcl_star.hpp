@@ -34,15 +34,43 @@ namespace mcl {
 constexpr int kStarCap = 24;    // stored neighbours per vertex
 constexpr int kCandMax = 128;   // candidates per vertex: two per lane
 constexpr int kHashSlots = 256; // per-wave LDS hash set for candidate de-duplication
-constexpr int kStarPasses = 5;  // star computations per step before the flip-based fallback
-constexpr int kStarAux = 16;    // star-step counters
-constexpr int kStarDecidePasses = 2;  // checks that list only the losing side of each conflict
+constexpr int kStarPasses = 12;  // star computations per step before the flip-based fallback
+constexpr int kStarAux = 24;    // star-step counters
+constexpr int kStarDecidePasses = 8;  // checks that list only the losing side of each conflict
 constexpr int kStarSyncFrom = 3;  // retry passes queued before the host reads the counts
 
-// stars of the current mesh: CCW rotation from v.he (h <- next(next(h))^1)
+// stars of the current mesh: CCW rotation from v.he (h <- next(next(h))^1).
+// A corner's star is an open fan [A, x1, ..., B] whose ends are the two
+// corners next to it on the outer square (the outer face lies between B and
+// A): walked clockwise, h <- next(h^1), which passes the outer face through
+// its 4-cycle, and reversed so that it starts after the outer half-edge.
 __global__ void __launch_bounds__(256) k_stars_from_mesh(DMesh m, int* __restrict__ deg, int* __restrict__ nbr,
                                                          int* __restrict__ fail) {
-    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.ncorner; v += gridDim.x * blockDim.x) {
+        // clockwise from the outgoing half-edge that borders the outer face (v -> B)
+        int h = m.v_he[v], guard = 0;
+        while (m.he_t[h] != -1 && guard++ < 4 * kStarCap) h = m.he_next[h ^ 1];
+        int d = 0;
+        const int h0 = h;
+        bool okf = m.he_t[h] == -1;
+        do {
+            if (d == kStarCap) { okf = false; break; }
+            nbr[(size_t)v * kStarCap + d++] = m.he_v[h ^ 1];  // B, xk, ..., x1, A
+            h = m.he_next[h ^ 1];
+        } while (okf && h != h0);
+        if (okf && h == h0) {
+            for (int i = 0; i < d / 2; i++) {  // reverse: A, x1, ..., xk, B
+                const int a = nbr[(size_t)v * kStarCap + i], b = nbr[(size_t)v * kStarCap + d - 1 - i];
+                nbr[(size_t)v * kStarCap + i] = b;
+                nbr[(size_t)v * kStarCap + d - 1 - i] = a;
+            }
+            deg[v] = d;
+        } else {
+            deg[v] = -1;
+            atomicAdd(fail, 1);
+        }
+    }
+    for (int v = m.ncorner + blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
         const int h0 = m.v_he[v];
         int h = h0, d = 0;
         do {
@@ -220,9 +248,12 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
     int first, end, stride;
     xcd_split(n, kV, first, end, stride);
     for (int base = first; base < end; base += stride) {
-        const bool active = base + g < end;
-        const int idx = active ? (rows ? rows[base + g] : base + g) : 0;  // output row
-        const int v = active ? (vlist ? vlist[idx] : idx) : 0;
+        const int idx0 = base + g < end ? (rows ? rows[base + g] : base + g) : 0;  // output row
+        const int v0 = base + g < end ? (vlist ? vlist[idx0] : idx0) : 0;
+        // corners get no star here (theirs are derived from their neighbours')
+        const bool active = base + g < end && v0 >= m.ncorner;
+        const int idx = active ? idx0 : 0;
+        const int v = active ? v0 : 0;
         int* tab = s_tab[g];
         int* list = s_cand[g];
         int* count = &s_cnt[g];
@@ -292,8 +323,11 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                     const int top = hi < kC ? hi : kC;
                     for (int i = lo + lane; i < top; i += kL) {
                         const int u = list[i];
-                        if (!deg_b || widen) insert_star<kH, kC>(tab, list, count, deg_a, nbr_a, u, v, m.nv);
-                        if (deg_b && dd == 2) insert_star<kH, kC>(tab, list, count, deg_b, nbr_b, u, v, m.nv);
+                        // (this step's rows of the corners are not built yet: their previous ones)
+                        if (!deg_b || widen || u < m.ncorner)
+                            insert_star<kH, kC>(tab, list, count, deg_a, nbr_a, u, v, m.nv);
+                        if (deg_b && dd == 2 && u >= m.ncorner)
+                            insert_star<kH, kC>(tab, list, count, deg_b, nbr_b, u, v, m.nv);
                     }
                 }
                 __syncthreads();
@@ -546,7 +580,7 @@ __device__ __forceinline__ int star_conflict(const float2* __restrict__ pos, int
 // neighbours' first 8 entries.
 __device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, const int* __restrict__ nbr, int v,
                                                   int d, bool decide, const float2* __restrict__ pos,
-                                                  bool periodic) {
+                                                  bool periodic, int ncorner) {
     const int* row = nbr + (size_t)v * kStarCap;
     if (d <= 8) {
         const int4* rv = reinterpret_cast<const int4*>(row);
@@ -554,7 +588,7 @@ __device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, c
         const int r[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
         int du[8];
 #pragma unroll
-        for (int i = 0; i < 8; i++) du[i] = i < d ? deg[r[i]] : 0;
+        for (int i = 0; i < 8; i++) du[i] = (i < d && r[i] >= ncorner) ? deg[r[i]] : 0;
         int4 lo[8], hi[8];
 #pragma unroll
         for (int i = 0; i < 8; i++) {
@@ -573,7 +607,7 @@ __device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, c
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             pv[i] = -1;
-            if (i >= d) continue;
+            if (i >= d || r[i] < ncorner) continue;  // corner rows are derived later, not looked up
             const int w = (i + 1 < d) ? r[(i + 1) & 7] : r[0];
             const int n = du[i];
             if (n < 3) { bad = true; continue; }
@@ -613,6 +647,10 @@ __device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, c
     int u = first;
     for (int i = 0; i < d; i++) {
         const int w = (i + 1 == d) ? first : row[i + 1];
+        if (u < ncorner) {
+            u = w;
+            continue;
+        }
         const int du = deg[u];
         if (du < 3) return true;
         const int j = star_find(nbr, u, du, v);
@@ -680,7 +718,7 @@ __device__ __forceinline__ void append_finish(AppendBuf& ab, int* count, int* __
 // degree sum of the valid stars is added to *sum_deg.
 __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restrict__ vlist, const int* __restrict__ nlist,
                                                     const int* __restrict__ deg, const int* __restrict__ nbr,
-                                                    const float2* __restrict__ pos, int periodic,
+                                                    const float2* __restrict__ pos, int periodic, int ncorner,
                                                     int mark_nbrs, int* __restrict__ mark, int stamp,
                                                     int* __restrict__ list, int* __restrict__ count,
                                                     long long* __restrict__ sum_deg) {
@@ -693,14 +731,15 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
     xcd_split(n, blockDim.x, first, end, stride);
     for (int base = first; base < end; base += stride) {
         const int idx = base + threadIdx.x;
-        const bool active = idx < end;
-        const int v = active ? (vlist ? vlist[idx] : idx) : 0;
+        const int v0 = idx < end ? (vlist ? vlist[idx] : idx) : 0;
+        const bool active = idx < end && v0 >= ncorner;  // corner stars are derived, not checked
+        const int v = active ? v0 : 0;
         const int d = active ? deg[v] : 3;
         const int* row = nbr + (size_t)v * kStarCap;
         bool bad = active && d < 3;
         if (active && !bad) {
             if (!vlist) my_sum += d;
-            bad = star_inconsistent(deg, nbr, v, d, !mark_nbrs, pos, periodic != 0);
+            bad = star_inconsistent(deg, nbr, v, d, !mark_nbrs, pos, periodic != 0, ncorner);
         }
         // claim v and its neighbours once per stamp, then append the claimed ones
         unsigned claimed = 0;  // bit 0: v, bit 1 + i: neighbour i
@@ -708,7 +747,7 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
         if (bad) {
             if (atomicExch(&mark[v], stamp) != stamp) { claimed |= 1u; nclaim++; }
             for (int i = 0; mark_nbrs && i < d; i++)
-                if (atomicExch(&mark[row[i]], stamp) != stamp) {
+                if (row[i] >= ncorner && atomicExch(&mark[row[i]], stamp) != stamp) {
                     claimed |= 2u << i;
                     nclaim++;
                 }
@@ -775,10 +814,98 @@ __global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vl
     if (delta) atomicAdd((unsigned long long*)sum_deg, (unsigned long long)delta);
 }
 
+// Corner fans from their neighbours' stars.  For a non-corner x with corner
+// c at position i of its star, the triangles (x, c, s_i+1) and (x, s_i-1, c)
+// put, in c's CCW fan, s_i+1 right before x and s_i-1 right after it.  Each
+// x adds (x, before, after) to c's bucket (kCornerBucket entries at most).
+constexpr int kCornerBucket = 64;
+__global__ void __launch_bounds__(256) k_corner_collect(int nv, int ncorner, const int* __restrict__ deg,
+                                                        const int* __restrict__ nbr, int* __restrict__ bucket,
+                                                        int* __restrict__ bcount) {
+    for (int x = ncorner + blockIdx.x * blockDim.x + threadIdx.x; x < nv; x += gridDim.x * blockDim.x) {
+        const int d = deg[x];
+        const int* row = nbr + (size_t)x * kStarCap;
+        for (int i = 0; i < d; i++) {
+            const int c = row[i];
+            if (c >= ncorner) continue;
+            const int k = atomicAdd(&bcount[c], 1);
+            if (k < kCornerBucket) {
+                int* e = bucket + ((size_t)c * kCornerBucket + k) * 3;
+                e[0] = x;
+                e[1] = row[i + 1 == d ? 0 : i + 1];  // before x in c's fan
+                e[2] = row[i == 0 ? d - 1 : i - 1];  // after x
+            }
+        }
+    }
+}
+
+// One thread per corner: chain the bucket into the fan [A, x1, ..., xk, B]
+// (A, B corners; another corner may sit inside the fan when two corners are
+// joined by a diagonal) and write it as the corner's star.  Every link must
+// agree from both sides (x's successor y has x as predecessor); a corner
+// whose entries do not chain into one fan of at most kStarCap is counted in
+// *fail.  Adds the corner degrees to *sum_deg.
+__global__ void k_corner_fans(int ncorner, const int* __restrict__ bucket, const int* __restrict__ bcount,
+                              int* __restrict__ deg, int* __restrict__ nbr, int* __restrict__ fail,
+                              long long* __restrict__ sum_deg) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncorner) return;
+    const int k = bcount[c];
+    const int* b = bucket + (size_t)c * kCornerBucket * 3;  // (x, before, after) per entry
+    bool ok = k >= 1 && k <= kCornerBucket;
+    // the first entry: its predecessor is a corner that follows no entry
+    int first = -1;
+    for (int i = 0; ok && i < k; i++) {
+        const int before = b[i * 3 + 1];
+        if (before >= ncorner) continue;
+        bool follows = false;
+        for (int j = 0; j < k; j++) follows |= b[j * 3 + 2] == before;
+        if (follows) continue;
+        if (first >= 0) ok = false;  // two fan starts
+        first = i;
+    }
+    ok = ok && first >= 0;
+    int d = 0;
+    unsigned long long used = 0ull;  // entries placed (k <= 64)
+    int* row = nbr + (size_t)c * kStarCap;
+    if (ok) {
+        row[d++] = b[first * 3 + 1];  // A
+        int cur = first;
+        while (ok) {
+            if (d == kStarCap || ((used >> cur) & 1ull)) { ok = false; break; }
+            used |= 1ull << cur;
+            const int x = b[cur * 3], after = b[cur * 3 + 2];
+            row[d++] = x;
+            int nx = -1;
+            if (after >= ncorner) {
+                for (int j = 0; j < k; j++)
+                    if (b[j * 3] == after) nx = j;
+                if (nx < 0 || b[nx * 3 + 1] != x) { ok = false; break; }  // links must agree
+            } else {
+                if (d == kStarCap) { ok = false; break; }
+                row[d++] = after;  // a corner: the end B, or the fan goes on past it
+                for (int j = 0; j < k; j++)
+                    if (!((used >> j) & 1ull) && b[j * 3 + 1] == after) nx = j;
+                if (nx < 0) break;
+            }
+            cur = nx;
+        }
+        ok = ok && __popcll(used) == k;
+    }
+    if (ok) {
+        deg[c] = d;
+        atomicAdd((unsigned long long*)sum_deg, (unsigned long long)d);
+    } else {
+        deg[c] = -1;
+        atomicAdd(fail, 1);
+    }
+}
+
 // per vertex: edges it owns (neighbours with a larger id) and triangles it
 // owns (both other vertices larger)
-__global__ void __launch_bounds__(256) k_star_counts(int nv, const int* __restrict__ deg, const int* __restrict__ nbr,
-                                                     int* __restrict__ ecnt, int* __restrict__ tcnt) {
+__global__ void __launch_bounds__(256) k_star_counts(int nv, int ncorner, const int* __restrict__ deg,
+                                                     const int* __restrict__ nbr, int* __restrict__ ecnt,
+                                                     int* __restrict__ tcnt) {
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
         const int d = deg[v];
         const int* row = nbr + (size_t)v * kStarCap;
@@ -788,7 +915,8 @@ __global__ void __launch_bounds__(256) k_star_counts(int nv, const int* __restri
         for (int i = 0; i < d; i++) {
             const int w = (i + 1 == d) ? first : row[i + 1];
             ne += u > v;
-            nt += (u > v && w > v);
+            // a corner's fan does not close: (B, A) borders the outer face
+            nt += (u > v && w > v && !(v < ncorner && i + 1 == d));
             u = w;
         }
         ecnt[v] = ne;
@@ -835,8 +963,10 @@ __global__ void __launch_bounds__(256) k_star_hid(DMesh m, const int* __restrict
             }
             hid[(size_t)v * kStarCap + i] = h;
             if ((unsigned)h < (unsigned)m.nhe) {
+                const bool outer = v < m.ncorner && i + 1 == d;  // corner -> B: the outer face
                 m.he_v[h] = v;
-                m.he_op[h] = w;
+                m.he_op[h] = outer ? -1 : w;
+                if (outer) m.he_t[h] = -1;
                 if (i == 0) m.v_he[v] = h;
             }
             u = w;
@@ -859,9 +989,9 @@ __global__ void __launch_bounds__(256) k_star_link(DMesh m, const int* __restric
         for (int i = 0; i < d; i++) {
             const int w = (i + 1 == d) ? first : row[i + 1];
             const int hw = (i + 1 == d) ? h_first : hr[i + 1];
-            const int hin = hw ^ 1;  // w -> v
+            const int hin = hw ^ 1;  // w -> v (at a corner's fan end: A -> corner, outer face)
             if ((unsigned)hin < (unsigned)m.nhe) m.he_next[hin] = hu;
-            if (u > v && w > v) {
+            if (u > v && w > v && !(v < m.ncorner && i + 1 == d)) {
                 if ((unsigned)t < (unsigned)m.nt && (unsigned)hu < (unsigned)m.nhe && (unsigned)hin < (unsigned)m.nhe) {
                     m.t_he[t] = hu;
                     m.he_t[hu] = t;

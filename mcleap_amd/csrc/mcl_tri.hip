@@ -1501,6 +1501,8 @@ struct mcl_tri {
     int* s_rlist = nullptr;                 // nv: vertices to re-check
     int* s_aux = nullptr;                   // kStarAux counters (see star_step)
     int* s_heavy = nullptr;                 // nv + 1: count, then rows for the full star kernel
+    int* c_bucket = nullptr;                // ncorner * kCornerBucket * 3: corner fan entries
+    int* c_count = nullptr;                 // ncorner
     // one ring
     int* deg = nullptr;
     int* scan_tmp = nullptr;
@@ -1550,7 +1552,8 @@ static int tri_free(mcl_tri* t) {
                     t->scalar, t->acc, t->tgt, t->v_mv, t->v_rev, t->v_fexp, t->t_stamp, t->bad[0], t->bad[1],
                     t->rev, t->allrev, t->pend[0], t->pend[1], t->deg, t->scan_tmp, t->v_touch, t->t_rep, t->t_rep_e, t->e_undo, t->undo, t->stuck, t->touched, t->r_tt, t->r_tt2, t->v_soft, t->tail_out,
                     t->star_deg[0], t->star_deg[1], t->star_nbr[0], t->star_nbr[1], t->s_ecnt, t->s_tcnt,
-                    t->s_mark, t->s_list, t->s_aux, t->star_tdeg, t->star_tnbr, t->s_rlist, t->s_heavy};
+                    t->s_mark, t->s_list, t->s_aux, t->star_tdeg, t->star_tnbr, t->s_rlist, t->s_heavy,
+                    t->c_bucket, t->c_count};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (t->h_cnt) (void)hipHostFree(t->h_cnt);
@@ -1591,6 +1594,20 @@ extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv,
     mcl_tri* t = new mcl_tri();
     DMesh& m = t->m;
     m.nv = nv; m.nhe = nhe; m.nt = nt; m.ne = nhe / 2; m.periodic = periodic ? 1 : 0;
+    // box meshes keep the 4 corners at vertices 0..3 (their star path needs it)
+    m.ncorner = 0;
+    if (!periodic && nv > 4) {
+        bool corners = true;
+        for (int v = 0; v < 4; v++) {
+            const int pi = v2[2 * v];
+            if (pi < 0 || pi >= npos || !(fabsf(pos[2 * pi]) >= 1.0e6f || fabsf(pos[2 * pi + 1]) >= 1.0e6f)) corners = false;
+        }
+        for (int v = 4; corners && v < nv; v++) {
+            const int pi = v2[2 * v];
+            if (pi >= 0 && pi < npos && (fabsf(pos[2 * pi]) >= 1.0e6f || fabsf(pos[2 * pi + 1]) >= 1.0e6f)) corners = false;
+        }
+        if (corners) m.ncorner = 4;
+    }
     int rc = MCL_OK;
 #define A(p, n) if (rc == MCL_OK) rc = dalloc(&(p), (size_t)(n))
     A(m.pos, nv); A(m.he_next, nhe); A(m.he_v, nhe); A(m.he_t, nhe); A(m.he_op, nhe);
@@ -1602,12 +1619,13 @@ extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv,
     A(t->deg, nv + 1); A(t->scan_tmp, scan_scratch_ints(nv + 1) + 16);
     A(t->v_touch, nv); A(t->t_rep, nt); A(t->t_rep_e, nt); A(t->e_undo, m.ne); A(t->undo, m.ne);
     A(t->stuck, nt); A(t->touched, nv); A(t->r_tt, nv); A(t->r_tt2, nv); A(t->v_soft, nv); A(t->tail_out, 8);
-    if (periodic) {
+    if (periodic || m.ncorner > 0) {
         A(t->star_deg[0], nv); A(t->star_deg[1], nv);
         A(t->star_nbr[0], (size_t)nv * kStarCap); A(t->star_nbr[1], (size_t)nv * kStarCap);
         A(t->star_tdeg, nv); A(t->star_tnbr, (size_t)nv * kStarCap);
         A(t->s_ecnt, nv + 1); A(t->s_tcnt, nv + 1); A(t->s_mark, nv); A(t->s_list, nv); A(t->s_aux, kStarAux);
         A(t->s_rlist, nv); A(t->s_heavy, nv + 1);
+        if (m.ncorner > 0) { A(t->c_bucket, (size_t)m.ncorner * kCornerBucket * 3); A(t->c_count, m.ncorner); }
     }
 #undef A
     if (rc == MCL_OK && hipHostMalloc((void**)&t->h_cnt, sizeof(int) * kCnCount) != hipSuccess) rc = MCL_ERR_HIP;
@@ -2274,7 +2292,7 @@ static int ensure_mesh(mcl_tri* t, hipStream_t s) {
     if (!t->mesh_stale) return MCL_OK;
     DMesh& m = t->m;
     const int c = t->star_cur;
-    hipLaunchKernelGGL(k_star_counts, dim3(blocks_for(m.nv)), dim3(256), 0, s, m.nv, (const int*)t->star_deg[c],
+    hipLaunchKernelGGL(k_star_counts, dim3(blocks_for(m.nv)), dim3(256), 0, s, m.nv, m.ncorner, (const int*)t->star_deg[c],
                        (const int*)t->star_nbr[c], t->s_ecnt, t->s_tcnt);
     MCL_TRY(scan_exclusive(t->s_ecnt, t->s_ecnt, m.nv, t->scan_tmp, s));
     MCL_TRY(scan_exclusive(t->s_tcnt, t->s_tcnt, m.nv, t->scan_tmp, s));
@@ -2366,14 +2384,27 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         hipLaunchKernelGGL(k_star_check, first ? star_grid(m.nv, 1024) : list_grid, dim3(256), 0, s, m.nv,
                            first ? (const int*)nullptr : (const int*)t->s_rlist,
                            first ? (const int*)nullptr : (const int*)(t->s_aux + 8), (const int*)t->star_deg[nxt],
-                           (const int*)t->star_nbr[nxt], (const float2*)m.pos, m.periodic, mark_nbrs ? 1 : 0, t->s_mark,
+                           (const int*)t->star_nbr[nxt], (const float2*)m.pos, m.periodic, m.ncorner,
+                           mark_nbrs ? 1 : 0, t->s_mark,
                            stamp, t->s_list, t->s_aux + 1,
                            (long long*)(t->s_aux + 2));
     };
     int aux[kStarAux];
     const bool trace = trace_on();
+    // corner fans from this step's stars (box meshes), into aux[16..18]
+    auto corner_fans = [&]() {
+        if (m.ncorner == 0) return;
+        hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 16, 3, 0);
+        hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->c_count, m.ncorner, 0);
+        hipLaunchKernelGGL(k_corner_collect, dim3(blocks_for(m.nv)), dim3(256), 0, s, m.nv, m.ncorner,
+                           (const int*)t->star_deg[nxt], (const int*)t->star_nbr[nxt], t->c_bucket, t->c_count);
+        hipLaunchKernelGGL(k_corner_fans, dim3(1), dim3(64), 0, s, m.ncorner, (const int*)t->c_bucket,
+                           (const int*)t->c_count, t->star_deg[nxt], t->star_nbr[nxt], t->s_aux + 18,
+                           (long long*)(t->s_aux + 16));
+    };
     // the counters, and the statistics with them (one synchronisation)
     auto read_aux = [&](int pass) -> int {
+        corner_fans();
         MCL_HIP(hipMemcpyAsync(aux, t->s_aux, sizeof(aux), hipMemcpyDeviceToHost, s));
         MCL_HIP(hipMemcpyAsync(t->h_stats, t->stats, sizeof(unsigned long long) * kStCount * kStatShards,
                                hipMemcpyDeviceToHost, s));
@@ -2409,9 +2440,14 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         if (trace || r >= kStarSyncFrom || r == kStarPasses - 1) {
             MCL_TRY(read_aux(r));
             if (aux[1] == 0) {
-                long long sum_deg;
+                long long sum_deg, sum_corner = 0;
                 memcpy(&sum_deg, aux + 2, sizeof(sum_deg));
-                settled = sum_deg == (long long)m.nhe;  // each half-edge once: the degree sum is 2E
+                if (m.ncorner) memcpy(&sum_corner, aux + 16, sizeof(sum_corner));
+                // each half-edge once: the degree sum is 2E; every corner fan chained
+                settled = sum_deg + sum_corner == (long long)m.nhe && aux[18] == 0;
+                if (trace && !settled)
+                    fprintf(stderr, "  [stars] degree sum %lld + corners %lld vs %d half-edges, corner failures %d\n",
+                            sum_deg, sum_corner, m.nhe, aux[18]);
                 break;
             }
         }
@@ -2441,7 +2477,7 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
 
 static int kinetic_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp, mcl_stats_t* st, void* stream) {
     hipStream_t s = (hipStream_t)stream;
-    if (t->m.periodic && p->variant == 0 && t->star_deg[0]) {
+    if ((t->m.periodic || t->m.ncorner == 4) && p->variant == 0 && t->star_deg[0]) {
         auto t0 = std::chrono::steady_clock::now();
         stats_begin(t, s);
         bool done = false;
