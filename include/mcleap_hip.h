@@ -248,13 +248,25 @@ void* mcl_grid_sorted_ptr(mcl_grid_t* g);
 /* mean_num_particle_in_cell (gridCount2d.h:304-334) */
 int mcl_grid_mean_occupancy(mcl_grid_t* g, float* out, void* stream);
 
+/* --------------------------------------------------------------- options */
+/* No counterpart in the reference (its step has no variants): choices of the
+ * fast kinetic step (variant 0).
+ * MCL_OPT_STAR_MIN_VERTICES: meshes with at least this many vertices (default
+ *   65536) restore the triangulation by per-vertex star recomputation (torus,
+ *   and box meshes with their 4 corners at vertices 0..3); smaller ones, where
+ *   the number of launches dominates, by the flip-based step.  0 = always.
+ * MCL_OPT_STAR_ENABLE: 0 = never use the star recomputation (default 1). */
+#define MCL_OPT_STAR_MIN_VERTICES 1
+#define MCL_OPT_STAR_ENABLE 2
+int mcl_tri_set_option(mcl_tri_t* t, int option, long long value);
+
 /* ------------------------------------------------------- kernel timers */
 /* GPU time of the main kernels, from HIP event pairs recorded on the launch
  * stream around every launch while enabled (no synchronisation is added).
  * Triangulation kernels: 0 = first Lawson round over all edges,
  * 1 = orientation check of all triangles, 2 = move, 3 = relocation apply
  * (full-grid launches), 4 = take-back apply (full-grid launches),
- * 5 = star recomputation (torus step).
+ * 5 = star recomputation (first pass of the star step).
  * Grid kernels: 0 = FRNN count pass, 1 = FRNN fill pass, 2 = grid build. */
 int mcl_tri_kernel_timer(mcl_tri_t* t, int enable);
 int mcl_tri_kernel_timer_read(mcl_tri_t* t, int which, double* total_ms, long long* launches);

@@ -26,6 +26,8 @@ MOTION_SHEAR = 1
 GRID_ROWMAJOR = 0
 GRID_MORTON = 1
 GRID_PERIODIC = 2
+OPT_STAR_MIN_VERTICES = 1
+OPT_STAR_ENABLE = 2
 
 
 class MclError(RuntimeError):
@@ -108,6 +110,7 @@ SIGNATURES = {
     "mcl_grid_mean_occupancy": [_p, _p, _p],
     "mcl_grid_build": [_p, _p, _p],
     "mcl_grid_sorted_ptr": [_p],
+    "mcl_tri_set_option": [_p, _i, ctypes.c_longlong],
     "mcl_tri_kernel_timer": [_p, _i],
     "mcl_tri_kernel_timer_read": [_p, _i, _p, _p],
     "mcl_grid_kernel_timer": [_p, _i],

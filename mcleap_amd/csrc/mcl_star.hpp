@@ -36,6 +36,7 @@ constexpr int kCandMax = 128;   // candidates per vertex: two per lane
 constexpr int kHashSlots = 256; // per-wave LDS hash set for candidate de-duplication
 constexpr int kStarPasses = 12;  // star computations per step before the flip-based fallback
 constexpr int kStarAux = 24;    // star-step counters
+constexpr int kStarMinVertices = 1 << 16;  // smaller meshes take the flip-based step
 constexpr int kStarDecidePasses = 8;  // checks that list only the losing side of each conflict
 constexpr int kStarSyncFrom = 3;  // retry passes queued before the host reads the counts
 
@@ -209,7 +210,8 @@ __device__ __forceinline__ int star_orient_f32(float4 A, float4 B) {
 // counted in fail[10]) for the full kernel; other failures are counted in
 // fail[0] and, by reason, in fail[4] (too few or too many candidates),
 // fail[5] (undecided test), fail[6] (hull not closed) or fail[7] (v not
-// inside).
+// inside).  acc: the positions at the start of the step (a vertex found on
+// top of another goes back there; null: it just fails).
 //
 // kL lanes handle one vertex (64: one per wave, up to 128 candidates; 32: two
 // per wave, up to 64 candidates, which holds nearly every 2-ring and halves
@@ -227,7 +229,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                                               const int* __restrict__ deg_b, const int* __restrict__ nbr_b,
                                               int* __restrict__ deg_out, int* __restrict__ nbr_out,
                                               int* __restrict__ fail, int* __restrict__ heavy,
-                                              unsigned long long* __restrict__ stats) {
+                                              const float2* __restrict__ acc, unsigned long long* __restrict__ stats) {
     static_assert(kL == 32 || kL == 64, "lanes per vertex");
     constexpr int kV = 256 / kL;                    // vertices per block iteration
     constexpr int kC = kL == 64 ? kCandMax : 64;    // candidates per vertex
@@ -245,6 +247,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
     const bool periodic = m.periodic != 0;
     const int max_depth = depth < 2 ? 2 : (depth > kStarMaxDepth ? kStarMaxDepth : depth);
     unsigned loc[4] = {0, 0, 0, 0};
+    unsigned stalled = 0;
     int first, end, stride;
     xcd_split(n, kV, first, end, stride);
     for (int base = first; base < end; base += stride) {
@@ -389,7 +392,17 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                 if (od < dist || (od == dist && ob < best)) { dist = od; best = ob; }
             }
         }
-        if ((__ballot(same) & gmask) && ok) { ok = false; why = 4; }
+        if ((__ballot(same) & gmask) && ok) {
+            // v landed exactly on another vertex (points clamped into the same
+            // corner of the box): v keeps its previous position this step and
+            // its star is computed again in the next pass
+            ok = false;
+            why = 4;
+            if (!kFast && acc && lane == 0) {  // (the filter kernel hands it to the full kernel)
+                m.pos[v] = acc[v];
+                stalled++;
+            }
+        }
         const int start = best;
         if (start >= kc && ok) { ok = false; why = 4; }
         const int id_start = ok ? __float_as_int(s_row[g][start].w) : -1;
@@ -476,7 +489,10 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                         int st = 0;
                         sg = incircle<kF32, false>(s_pos[g][j], s_pos[g][a], s_pos[g][b], pv, periodic, &st);
                         bump(loc, st == kStageExactBig ? 3 : st);
-                        if (sg == kNeedBig || sg == 0) undecided = true;
+                        // 0 after the tie rule: the four points are collinear
+                        // (e.g. points clamped onto one side of the box); such a
+                        // candidate is not more clockwise than the one held
+                        if (sg == kNeedBig) undecided = true;
                     }
                     if (sg < 0) { a = b; ra = rb; }
                 }
@@ -544,6 +560,8 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
         }
     }
     flush_stats(loc, stats, kStIncircle0);
+    const unsigned st1[1] = {stalled};
+    flush_stats(st1, stats, kStStalled);
 }
 
 
@@ -812,6 +830,16 @@ __global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vl
     }
     append_finish(ab, rcount, rlist);
     if (delta) atomicAdd((unsigned long long*)sum_deg, (unsigned long long)delta);
+}
+
+// After the first check: when more than limit vertices are inconsistent
+// (motion the star step cannot follow in a few passes), empty the list and
+// flag the step as given up (aux[19]); later passes then do nothing.
+__global__ void k_star_give_up(int* __restrict__ aux, int limit) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && aux[1] > limit) {
+        aux[1] = 0;
+        aux[19] = 1;
+    }
 }
 
 // Corner fans from their neighbours' stars.  For a non-corner x with corner

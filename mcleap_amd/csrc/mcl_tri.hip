@@ -1494,6 +1494,10 @@ struct mcl_tri {
     int star_cur = 0;                       // stars of the current mesh
     bool stars_valid = false;               // star_*[star_cur] describe the current mesh
     bool mesh_stale = false;                // half-edge arrays not yet rebuilt from the stars
+    int star_min_vertices = kStarMinVertices;  // MCL_OPT_STAR_MIN_VERTICES
+    bool star_enable = true;                   // MCL_OPT_STAR_ENABLE
+    int star_skip = 0;                      // steps left before the star step is tried again
+    int star_backoff = 0;                   // length of the last such pause
     int* s_ecnt = nullptr;                  // nv + 1
     int* s_tcnt = nullptr;                  // nv + 1
     int* s_mark = nullptr;                  // nv
@@ -2353,14 +2357,14 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
             hipLaunchKernelGGL((k_star<true, true, 32, true>), star_grid(m.nv, 8), dim3(256), 0, s, m, vlist, 0,
                                nlist_dev, (const int*)nullptr, (const int*)nullptr, depth,
                                (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr,
-                               t->s_aux, t->s_heavy, t->stats);
+                               t->s_aux, t->s_heavy, (const float2*)t->acc, t->stats);
         else
             hipLaunchKernelGGL((k_star<true, true, 32, false>), list_grid, dim3(256), 0, s, m, vlist, 0, nlist_dev,
                                (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
-                               (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy, t->stats);
+                               (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy, (const float2*)t->acc, t->stats);
         hipLaunchKernelGGL((k_star<true, false, 64, false>), dim3(256), dim3(256), 0, s, m, vlist, 0, nlist_dev,
                            (const int*)(t->s_heavy + 1), (const int*)t->s_heavy, depth, (const int*)t->star_deg[cur],
-                           (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, (int*)nullptr, t->stats);
+                           (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, (int*)nullptr, (const float2*)t->acc, t->stats);
     };
     t->timers[5].mark(s);
     stars(nullptr, nullptr, 2, nullptr, nullptr, t->star_deg[nxt], t->star_nbr[nxt]);
@@ -2418,6 +2422,7 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         return MCL_OK;
     };
     check(true, false);
+    hipLaunchKernelGGL(k_star_give_up, dim3(1), dim3(64), 0, s, t->s_aux, (int)(0.4 * m.nv));
     MCL_HIP(hipGetLastError());
     if (trace) MCL_TRY(read_aux(0));
     // Retries run without reading the counts back until kStarSyncFrom passes
@@ -2444,7 +2449,7 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
                 memcpy(&sum_deg, aux + 2, sizeof(sum_deg));
                 if (m.ncorner) memcpy(&sum_corner, aux + 16, sizeof(sum_corner));
                 // each half-edge once: the degree sum is 2E; every corner fan chained
-                settled = sum_deg + sum_corner == (long long)m.nhe && aux[18] == 0;
+                settled = sum_deg + sum_corner == (long long)m.nhe && aux[18] == 0 && aux[19] == 0;
                 if (trace && !settled)
                     fprintf(stderr, "  [stars] degree sum %lld + corners %lld vs %d half-edges, corner failures %d\n",
                             sum_deg, sum_corner, m.nhe, aux[18]);
@@ -2477,7 +2482,18 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
 
 static int kinetic_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp, mcl_stats_t* st, void* stream) {
     hipStream_t s = (hipStream_t)stream;
-    if ((t->m.periodic || t->m.ncorner == 4) && p->variant == 0 && t->star_deg[0]) {
+    // the star step, unless it fell back recently: after a fallback it is not
+    // tried for 1, 2, 4, ... 64 steps (workloads whose motion it cannot follow
+    // then cost the flip-based step only)
+    // (small meshes: the flip-based step needs fewer launches)
+    const bool try_star = (t->m.periodic || t->m.ncorner == 4) && p->variant == 0 && t->star_deg[0] &&
+                          t->star_enable && t->m.nv >= t->star_min_vertices;
+    if (try_star && t->star_skip > 0) {
+        t->star_skip--;
+        MCL_TRY(ensure_mesh(t, s));
+        return step_impl(t, p, disp, st, stream);
+    }
+    if (try_star) {
         auto t0 = std::chrono::steady_clock::now();
         stats_begin(t, s);
         bool done = false;
@@ -2485,6 +2501,7 @@ static int kinetic_step(mcl_tri* t, const mcl_step_params_t* p, const float2* di
         int rc = star_step(t, p, disp, &tmp, s, &done);
         if (rc != MCL_OK) return rc;
         if (done) {
+            t->star_backoff = 0;
             MCL_TRY(stats_end(t, s, st, t0));
             if (st) {
                 st->iterations = tmp.iterations;
@@ -2492,6 +2509,8 @@ static int kinetic_step(mcl_tri* t, const mcl_step_params_t* p, const float2* di
             }
             return MCL_OK;
         }
+        t->star_backoff = t->star_backoff ? std::min(2 * t->star_backoff, 64) : 1;
+        t->star_skip = t->star_backoff;
         MCL_TRY(ensure_mesh(t, s));
         rc = step_impl(t, p, disp, st, stream);
         if (st) st->fallbacks = 1;
@@ -2595,6 +2614,21 @@ extern "C" int mcl_tri_frnn_graph(mcl_tri_t* t, float r, int K, int layout, cons
     MCL_HIP(hipStreamSynchronize(s));
     if (overflow) *overflow = ov;
     return ov ? MCL_ERR_CAPACITY : MCL_OK;
+}
+
+extern "C" int mcl_tri_set_option(mcl_tri_t* t, int option, long long value) {
+    if (!t) return MCL_ERR_INPUT;
+    switch (option) {
+        case MCL_OPT_STAR_MIN_VERTICES:
+            if (value < 0) return MCL_ERR_INPUT;
+            t->star_min_vertices = (int)std::min<long long>(value, INT32_MAX);
+            return MCL_OK;
+        case MCL_OPT_STAR_ENABLE:
+            t->star_enable = value != 0;
+            return MCL_OK;
+        default:
+            return MCL_ERR_INPUT;
+    }
 }
 
 extern "C" int mcl_tri_kernel_timer(mcl_tri_t* t, int enable) {

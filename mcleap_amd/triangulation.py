@@ -272,6 +272,10 @@ class DeviceTriangulation:
         check(lib().mcl_tri_one_ring_csr(self._h, _dev_ptr(offsets), _dev_ptr(nbrs), int(nbrs.numel()), None, None),
               "mcl_tri_one_ring_csr")
 
+    def set_option(self, option, value):
+        """mcl_tri_set_option: OPT_STAR_MIN_VERTICES / OPT_STAR_ENABLE (fast step choices)"""
+        check(lib().mcl_tri_set_option(self._h, int(option), int(value)), "mcl_tri_set_option")
+
     def kernel_timer(self, enable=True):
         check(lib().mcl_tri_kernel_timer(self._h, int(enable)), "mcl_tri_kernel_timer")
 

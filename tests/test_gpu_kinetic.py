@@ -50,16 +50,21 @@ def test_upload_download_roundtrip(oracle):
     assert dt.count_non_delaunay() == 0
 
 
-@pytest.mark.parametrize("variant", [0, 1])
-def test_box_steps_config1(oracle, variant):
-    """Config 1 shape (N=10,000, sigma=1e-3, clamped), checked after every step."""
+@pytest.mark.parametrize("variant,star", [(0, False), (0, True), (1, False)])
+def test_box_steps_config1(oracle, variant, star):
+    """Config 1 shape (N=10,000, sigma=1e-3, clamped), checked after every
+    step; star=True forces the star recomputation (corner fans, points
+    clamped onto the sides of the box), star=False the flip-based step."""
     O = oracle
     xy = uniform_points(10000, 1)
     H = M.HostTriangulation()
     H.addDelaunayPoints(xy)
     dt = M.DeviceTriangulation(H)
+    dt.set_option(M.OPT_STAR_MIN_VERTICES, 0)
+    dt.set_option(M.OPT_STAR_ENABLE, 1 if star else 0)
     prev = dt.download_positions()
     nsteps = 12 if variant == 0 else 6
+    star_steps = 0
     for k in range(nsteps):
         st = dt.step(M.MOTION_GAUSS, 1e-3, seed=1, step=k, variant=variant)
         print("step", k, st)
@@ -70,7 +75,10 @@ def test_box_steps_config1(oracle, variant):
         # a vertex whose relocation meets a degenerate (collinear) link keeps
         # its position for the step; the mesh is still exact for the positions
         assert st["stalled"] <= 3
+        star_steps += star and st["fallbacks"] == 0 and st["rounds"] == 0
         prev = pos
+    if star:
+        assert star_steps >= nsteps // 2, star_steps  # most steps settle by stars
 
 
 def test_box_large_moves(oracle):
@@ -88,16 +96,24 @@ def test_box_large_moves(oracle):
         _check_box(O, dt, "large step %d" % k)
 
 
-def test_torus_steps(oracle):
-    """Config 2 shape on the torus at N=20,000 (shear flow + noise, wrapped)."""
+@pytest.mark.parametrize("star", [True, False])
+def test_torus_steps(oracle, star):
+    """Config 2 shape on the torus at N=20,000 (shear flow + noise, wrapped),
+    by star recomputation or by the flip-based step."""
     O = oracle
     xy = uniform_points(20000, 2)
     H = M.HostTriangulation(periodic=True)
     H.addDelaunayPoints(xy)
     dt = M.DeviceTriangulation(H)
+    dt.set_option(M.OPT_STAR_MIN_VERTICES, 0)
+    dt.set_option(M.OPT_STAR_ENABLE, 1 if star else 0)
     for k in range(6):
         st = dt.step(M.MOTION_SHEAR, 2e-4, amp=1e-3, seed=2, step=k)
         print(st)
+        if star:
+            assert st["fallbacks"] == 0 and st["rounds"] == 0  # settled by stars
+        else:
+            assert st["rounds"] > 0
         _check_torus(O, dt, "torus step %d" % k)
 
 
