@@ -51,6 +51,7 @@ def _args():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--n", type=int, default=0, help="override point count (testing only)")
+    ap.add_argument("--ninst", type=int, default=0, help="override the ensemble's instance count (testing only)")
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--plain-steps", type=int, default=10, help="steps of the plain form timed for comparison")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -121,6 +122,66 @@ def cpu_baseline(pos0, cfg, periodic, seconds):
                          platform.processor() or platform.machine(), bool(ok))}
 
 
+def ensemble_sample(counts, k=32):
+    """k instances spread over the size range (every len/k-th by size)"""
+    order = np.argsort(counts, kind="stable")
+    return [int(order[j]) for j in np.linspace(0, len(order) - 1, min(k, len(order))).round().astype(int)]
+
+
+def plain_ensemble(M, torch, inst_pts, cfg, steps):
+    """Per-instance launches on the same GPU for a sample of the ensemble's
+    instances: each instance its own triangulation, stepped one after the
+    other.  Returns point-steps/s of the plain form (flip-based step with
+    FP64-first predicates, variant 1) and of the same fast step as the
+    batched run (variant 0) without batching."""
+    sample = ensemble_sample(np.array([len(p) for p in inst_pts]))
+    dts = []
+    for i in sample:
+        h = M.HostTriangulation()
+        q = inst_pts[i].copy()
+        h.addDelaunayPoints(q)
+        dts.append((M.DeviceTriangulation(h), len(q)))
+    npts = sum(n for _, n in dts)
+    out = []
+    for variant in (1, 0):
+        for d, _ in dts:
+            d.step(cfg["motion"], cfg["sigma"], cfg["amp"], seed=cfg["seed"], step=0, variant=variant)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            for d, _ in dts:
+                d.step(cfg["motion"], cfg["sigma"], cfg["amp"], seed=cfg["seed"], step=1 + k, variant=variant)
+        torch.cuda.synchronize()
+        out.append(npts * steps / (time.perf_counter() - t0))
+    return out[0], out[1]
+
+
+def cpu_baseline_ensemble(inst_pos, cfg, seconds):
+    """The oracle rebuilding each sampled instance's exact DT after its move,
+    instances in turn, one thread."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(12345)
+    pos = [np.ascontiguousarray(p, np.float32) for p in inst_pos]
+    t0 = time.perf_counter()
+    pts = 0
+    rounds = 0
+    while time.perf_counter() - t0 < seconds:
+        for j, p in enumerate(pos):
+            d = cfg["sigma"] * rng.standard_normal(p.shape)
+            p = np.ascontiguousarray(np.clip(p + d.astype(np.float32), 0.0, 1.0 - 2.0 ** -24), np.float32)
+            T = O.HostTri(O.HostTri.EXACT)
+            T.add_delaunay_points(p, 0)
+            pos[j] = p
+            pts += len(p)
+        rounds += 1
+    el = time.perf_counter() - t0
+    return {"value": pts / el, "unit": "point-steps/s", "cores": 1, "kind": "port",
+            "sample": "oracle (restated Host_Triangulation incremental build, exact predicates) rebuilding the "
+                      "exact box DT after each move for %d instances spread over the size range (%d points), "
+                      "%d rounds in %.1f s, 1 thread, host %s"
+                      % (len(pos), sum(len(p) for p in pos), rounds, el, platform.processor() or platform.machine())}
+
+
 def main():
     a = _args()
     if a.gpus != 1:
@@ -132,17 +193,28 @@ def main():
     import torch
 
     import mcleap_amd as M
-    from mcleap_amd.workloads import CONFIGS, grid_shape, initial_points
+    from mcleap_amd.workloads import CONFIGS, ensemble_points, grid_shape, initial_points
 
     cfg = dict(CONFIGS[a.workload])
-    xy = initial_points(a.workload, a.n or None)
-    n = len(xy)
     periodic = cfg["domain"] == "torus"
-    t0 = time.time()
-    H = M.HostTriangulation(periodic=periodic)
-    H.addDelaunayPoints(xy)
-    build_s = time.time() - t0
-    print("[bench] %s: %d points, host build %.2f s" % (a.workload, n, build_s), file=sys.stderr, flush=True)
+    batch = a.workload == "c5"
+    if batch:
+        # the ensemble as one mesh: every launch advances all instances
+        inst_pts = ensemble_points(a.ninst or None)
+        n = sum(len(p) for p in inst_pts)
+        t0 = time.time()
+        H = M.HostTriangulationBatch(inst_pts)
+        build_s = time.time() - t0
+        print("[bench] c5: %d instances, %d points, host build %.2f s" % (H.ninst, n, build_s), file=sys.stderr,
+              flush=True)
+    else:
+        xy = initial_points(a.workload, a.n or None)
+        n = len(xy)
+        t0 = time.time()
+        H = M.HostTriangulation(periodic=periodic)
+        H.addDelaunayPoints(xy)
+        build_s = time.time() - t0
+        print("[bench] %s: %d points, host build %.2f s" % (a.workload, n, build_s), file=sys.stderr, flush=True)
     dt = M.DeviceTriangulation(H)
     nv = dt.m_pos_size
     dev = "cuda"
@@ -238,7 +310,12 @@ def main():
 
     # the plain form (host round trip per round, FP64-first predicates) on the same GPU
     plain = None
-    if a.plain_steps > 0:
+    unbatched = None
+    if a.plain_steps > 0 and batch:
+        plain, unbatched = plain_ensemble(M, torch, inst_pts, cfg, a.plain_steps)
+        print("[bench] plain per-instance form: %.3g point-steps/s, unbatched fast form %.3g" % (plain, unbatched),
+              file=sys.stderr, flush=True)
+    elif a.plain_steps > 0:
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         for _ in range(a.plain_steps):
@@ -252,7 +329,11 @@ def main():
     if not a.no_cpu:
         print("[bench] cpu baseline (%.0f s budget)" % a.cpu_seconds, file=sys.stderr, flush=True)
         p0 = dt.download_positions()
-        cpu = cpu_baseline(p0 if periodic else p0[4:], cfg, periodic, a.cpu_seconds)
+        if batch:
+            cpu = cpu_baseline_ensemble([p0[H.vstart[i]:H.vstart[i + 1]] for i in ensemble_sample(H.counts)], cfg,
+                                        a.cpu_seconds)
+        else:
+            cpu = cpu_baseline(p0 if periodic else p0[4:], cfg, periodic, a.cpu_seconds)
 
     agg = {key: int(sum(s[key] for s in stats)) for key in ("rounds", "flips", "untangle_flips", "reverts",
                                                              "relocated", "relocated_grid", "stalled", "inverted",
@@ -273,6 +354,7 @@ def main():
         "dtype": "fp32 coordinates, exact predicates (fp32 -> fp64 -> expansion)",
         "data": "synthetic (seeded uniform points, Philox motion on the GPU)",
         "config": {"workload": a.workload, "n_points": n, "domain": cfg["domain"], "sigma": cfg["sigma"],
+                   "instances": int(H.ninst) if batch else 1,
                    "amp": cfg["amp"], "frnn_radius": radius, "variant": a.variant,
                    "host_build_s": round(build_s, 3)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
@@ -285,6 +367,7 @@ def main():
         "kernels_ms_per_launch": {k_: (v[0] / max(v[1], 1)) for k_, v in kern.items()},
         "kernels_ms_per_step": {k_: v[0] / a.steps for k_, v in kern.items()},
         "plain_value": plain,
+        "unbatched_value": unbatched,
         "fast_vs_plain": (value / plain) if plain else None,
         "per_step_totals": agg,
         "final_mesh": {"non_delaunay_edges": final_nd, "inverted_triangles": final_inv},

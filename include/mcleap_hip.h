@@ -73,6 +73,20 @@ int mcl_hmesh_sizes(const mcl_hmesh_t* m, int* npos, int* nv, int* nhe, int* nt)
 /* copy out the AoS arrays (any pointer may be NULL); perm has n entries */
 int mcl_hmesh_export(const mcl_hmesh_t* m, float* pos, int* v2, int* he4, int* t1, int* perm);
 int mcl_hmesh_free(mcl_hmesh_t* m);
+/* An ensemble of ninst independent box triangulations as ONE mesh, so that
+ * every device call advances all of them together (no counterpart in the
+ * reference, which builds one HostTriangulation per instance,
+ * Host_Triangulation.h:107-137, 296-300).  xy holds the instances' points back
+ * to back, counts[i] of them for instance i.  Vertex layout: the 4 corners of
+ * every instance first (instance i: 4i..4i+3, at +-1e7 like a single box
+ * mesh), then instance 0's points, instance 1's, ...; perm maps the points'
+ * vertex order to indices into xy.  Instances are built in nthreads host
+ * threads (0: all cores). */
+int mcl_hmesh_build_batch(const float* xy, const int* counts, int ninst, int nthreads, mcl_hmesh_t** out);
+/* ninst + 1 offsets each: instance i's points are vertices
+ * vstart[i]..vstart[i+1]-1, its half-edges hestart[i]..hestart[i+1]-1, its
+ * triangles tstart[i]..tstart[i+1]-1.  MCL_ERR_INPUT for a single mesh. */
+int mcl_hmesh_batch_offsets(const mcl_hmesh_t* m, int* vstart, int* hestart, int* tstart);
 
 /* Host-side operations on the caller's AoS arrays (pos: float pairs indexed
  * by Vertex.pos_index; he4/t1/v2 as above), exact predicates, in place. */

@@ -86,6 +86,8 @@ SIGNATURES = {
     "mcl_hmesh_sizes": [_p, _p, _p, _p, _p],
     "mcl_hmesh_export": [_p, _p, _p, _p, _p, _p],
     "mcl_hmesh_free": [_p],
+    "mcl_hmesh_build_batch": [_p, _p, _i, _i, _pp],
+    "mcl_hmesh_batch_offsets": [_p, _p, _p, _p],
     "mcl_tri_create": [_p, _i, _p, _i, _p, _i, _p, _i, _i, _pp],
     "mcl_tri_destroy": [_p],
     "mcl_tri_sizes": [_p, _p, _p, _p, _p],

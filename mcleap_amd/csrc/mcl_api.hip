@@ -1,6 +1,10 @@
 // mcl_api.hip -- host-mesh construction entry points and library info.
+#include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "mcl_internal.hpp"
 
@@ -15,6 +19,112 @@ extern "C" int mcl_hmesh_build(const float* xy, int n, int periodic, mcl_hmesh_t
     int rc = periodic ? mcl::build_torus(xy, n, h->m) : mcl::build_box(xy, n, h->m);
     if (rc != MCL_OK) { delete h; return rc; }
     *out = h;
+    return MCL_OK;
+}
+
+// Ensemble of independent box triangulations as one mesh: every instance is
+// built on its own (build_box, one instance per host thread, largest first),
+// then copied into the union with all corners first (instance i: 4i..4i+3)
+// and each instance's points, half-edges and triangles after the previous
+// instance's.  Half-edge offsets stay even, so twins remain h ^ 1.
+extern "C" int mcl_hmesh_build_batch(const float* xy, const int* counts, int ninst, int nthreads,
+                                     mcl_hmesh_t** out) {
+    if (!out || !counts || ninst <= 0) return MCL_ERR_INPUT;
+    std::vector<long long> pstart(ninst + 1, 0);
+    for (int i = 0; i < ninst; i++) {
+        if (counts[i] < 0) return MCL_ERR_INPUT;
+        pstart[i + 1] = pstart[i] + counts[i];
+    }
+    const long long npts = pstart[ninst];
+    if (npts > 0 && !xy) return MCL_ERR_INPUT;
+    if (npts + 4LL * ninst >= (1LL << 31) / 6) return MCL_ERR_CAPACITY;
+    std::vector<mcl::HostMesh> part(ninst);
+    std::vector<int> rc(ninst, MCL_OK);
+    std::vector<int> order(ninst);
+    for (int i = 0; i < ninst; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return counts[a] > counts[b]; });
+    int nt = nthreads > 0 ? nthreads : (int)std::thread::hardware_concurrency();
+    nt = std::max(1, std::min(nt, ninst));
+    std::atomic<int> next{0};
+    auto build = [&]() {
+        for (int k; (k = next.fetch_add(1)) < ninst;) {
+            const int i = order[k];
+            rc[i] = mcl::build_box(xy + 2 * pstart[i], counts[i], part[i]);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nt; w++) pool.emplace_back(build);
+    build();
+    for (auto& th : pool) th.join();
+    for (int i = 0; i < ninst; i++)
+        if (rc[i] != MCL_OK) return rc[i];
+    mcl_hmesh* h = new mcl_hmesh();
+    mcl::HostMesh& m = h->m;
+    m.periodic = false;
+    m.vstart.resize(ninst + 1);
+    m.hestart.resize(ninst + 1);
+    m.tstart.resize(ninst + 1);
+    long long nv = 4LL * ninst, nhe = 0, ntri = 0;
+    for (int i = 0; i < ninst; i++) {
+        m.vstart[i] = (int)nv;
+        m.hestart[i] = (int)nhe;
+        m.tstart[i] = (int)ntri;
+        nv += part[i].nv - 4;
+        nhe += (long long)part[i].he.size() / 4;
+        ntri += (long long)part[i].t.size();
+        m.flips += part[i].flips;
+    }
+    if (nhe >= (1LL << 31)) { delete h; return MCL_ERR_CAPACITY; }
+    m.vstart[ninst] = (int)nv;
+    m.hestart[ninst] = (int)nhe;
+    m.tstart[ninst] = (int)ntri;
+    m.nv = (int)nv;
+    m.pos.resize(2 * (size_t)nv);
+    m.vrec.resize(2 * (size_t)nv);
+    m.he.resize(4 * (size_t)nhe);
+    m.t.resize((size_t)ntri);
+    m.perm.resize((size_t)npts);
+    next = 0;
+    auto copy = [&]() {
+        for (int i; (i = next.fetch_add(1)) < ninst;) {
+            mcl::HostMesh& q = part[i];
+            const int vs = m.vstart[i], hs = m.hestart[i], ts = m.tstart[i];
+            auto vmap = [&](int v) { return v < 0 ? v : (v < 4 ? 4 * i + v : vs + v - 4); };
+            for (int v = 0; v < q.nv; v++) {
+                const int g = vmap(v);
+                m.pos[2 * (size_t)g] = q.pos[2 * (size_t)v];
+                m.pos[2 * (size_t)g + 1] = q.pos[2 * (size_t)v + 1];
+                m.vrec[2 * (size_t)g] = g;
+                m.vrec[2 * (size_t)g + 1] = q.vrec[2 * (size_t)v + 1] + hs;
+            }
+            const size_t nh = q.he.size() / 4;
+            for (size_t e = 0; e < nh; e++) {
+                int* o = &m.he[4 * ((size_t)hs + e)];
+                o[0] = q.he[4 * e] + hs;
+                o[1] = vmap(q.he[4 * e + 1]);
+                o[2] = q.he[4 * e + 2] < 0 ? q.he[4 * e + 2] : q.he[4 * e + 2] + ts;
+                o[3] = vmap(q.he[4 * e + 3]);
+            }
+            for (size_t k = 0; k < q.t.size(); k++) m.t[(size_t)ts + k] = q.t[k] + hs;
+            for (size_t k = 0; k < q.perm.size(); k++)
+                m.perm[(size_t)(vs - 4LL * ninst) + k] = (int)pstart[i] + q.perm[k];
+            q = mcl::HostMesh();  // release the instance's storage
+        }
+    };
+    pool.clear();
+    for (int w = 1; w < nt; w++) pool.emplace_back(copy);
+    copy();
+    for (auto& th : pool) th.join();
+    *out = h;
+    return MCL_OK;
+}
+
+extern "C" int mcl_hmesh_batch_offsets(const mcl_hmesh_t* h, int* vstart, int* hestart, int* tstart) {
+    if (!h || h->m.vstart.empty()) return MCL_ERR_INPUT;
+    const size_t n = h->m.vstart.size();
+    if (vstart) memcpy(vstart, h->m.vstart.data(), sizeof(int) * n);
+    if (hestart) memcpy(hestart, h->m.hestart.data(), sizeof(int) * n);
+    if (tstart) memcpy(tstart, h->m.tstart.data(), sizeof(int) * n);
     return MCL_OK;
 }
 

@@ -28,6 +28,10 @@ struct HostMesh {
     std::vector<int> t;       // nt
     std::vector<int> perm;    // n input indices in vertex order
     long long flips = 0;
+    // ensemble of box instances (empty for a single mesh): instance i owns
+    // corners 4i..4i+3, points vstart[i]..vstart[i+1]-1, half-edges
+    // hestart[i].. and triangles tstart[i]..
+    std::vector<int> vstart, hestart, tstart;
 };
 int build_box(const float* xy, int n, HostMesh& m);
 int build_torus(const float* xy, int n, HostMesh& m);

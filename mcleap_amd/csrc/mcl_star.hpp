@@ -872,15 +872,31 @@ __global__ void __launch_bounds__(256) k_corner_collect(int nv, int ncorner, con
 // joined by a diagonal) and write it as the corner's star.  Every link must
 // agree from both sides (x's successor y has x as predecessor); a corner
 // whose entries do not chain into one fan of at most kStarCap is counted in
-// *fail.  Adds the corner degrees to *sum_deg.
+// *fail.  Adds the corner degrees to *sum_deg.  A corner that no vertex
+// lists keeps its previous fan (pdeg, pnbr) if that fan holds corners only:
+// the corners of an instance with no points.
 __global__ void k_corner_fans(int ncorner, const int* __restrict__ bucket, const int* __restrict__ bcount,
-                              int* __restrict__ deg, int* __restrict__ nbr, int* __restrict__ fail,
-                              long long* __restrict__ sum_deg) {
+                              const int* __restrict__ pdeg, const int* __restrict__ pnbr, int* __restrict__ deg,
+                              int* __restrict__ nbr, int* __restrict__ fail, long long* __restrict__ sum_deg) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncorner) return;
     const int k = bcount[c];
     const int* b = bucket + (size_t)c * kCornerBucket * 3;  // (x, before, after) per entry
-    bool ok = k >= 1 && k <= kCornerBucket;
+    if (k == 0) {
+        const int d = pdeg[c];
+        bool ok = d >= 2 && d <= kStarCap;
+        for (int i = 0; ok && i < d; i++) ok = pnbr[(size_t)c * kStarCap + i] < ncorner;
+        if (ok) {
+            for (int i = 0; i < d; i++) nbr[(size_t)c * kStarCap + i] = pnbr[(size_t)c * kStarCap + i];
+            deg[c] = d;
+            atomicAdd((unsigned long long*)sum_deg, (unsigned long long)d);
+        } else {
+            deg[c] = -1;
+            atomicAdd(fail, 1);
+        }
+        return;
+    }
+    bool ok = k <= kCornerBucket;
     // the first entry: its predecessor is a corner that follows no entry
     int first = -1;
     for (int i = 0; ok && i < k; i++) {

@@ -1598,19 +1598,19 @@ extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv,
     mcl_tri* t = new mcl_tri();
     DMesh& m = t->m;
     m.nv = nv; m.nhe = nhe; m.nt = nt; m.ne = nhe / 2; m.periodic = periodic ? 1 : 0;
-    // box meshes keep the 4 corners at vertices 0..3 (their star path needs it)
+    // box meshes keep their corners first: vertices 0..3, or 4i..4i+3 for
+    // instance i of an ensemble (the star path needs it)
     m.ncorner = 0;
     if (!periodic && nv > 4) {
-        bool corners = true;
-        for (int v = 0; v < 4; v++) {
+        auto corner_at = [&](int v) {
             const int pi = v2[2 * v];
-            if (pi < 0 || pi >= npos || !(fabsf(pos[2 * pi]) >= 1.0e6f || fabsf(pos[2 * pi + 1]) >= 1.0e6f)) corners = false;
-        }
-        for (int v = 4; corners && v < nv; v++) {
-            const int pi = v2[2 * v];
-            if (pi >= 0 && pi < npos && (fabsf(pos[2 * pi]) >= 1.0e6f || fabsf(pos[2 * pi + 1]) >= 1.0e6f)) corners = false;
-        }
-        if (corners) m.ncorner = 4;
+            return pi >= 0 && pi < npos && (fabsf(pos[2 * pi]) >= 1.0e6f || fabsf(pos[2 * pi + 1]) >= 1.0e6f);
+        };
+        int k = 0;
+        while (k < nv && corner_at(k)) k++;
+        bool ok = k > 0 && k % 4 == 0;
+        for (int v = k; ok && v < nv; v++) ok = !corner_at(v);
+        if (ok) m.ncorner = k;
     }
     int rc = MCL_OK;
 #define A(p, n) if (rc == MCL_OK) rc = dalloc(&(p), (size_t)(n))
@@ -2399,11 +2399,12 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     auto corner_fans = [&]() {
         if (m.ncorner == 0) return;
         hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 16, 3, 0);
-        hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->c_count, m.ncorner, 0);
+        hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(m.ncorner)), dim3(256), 0, s, t->c_count, m.ncorner, 0);
         hipLaunchKernelGGL(k_corner_collect, dim3(blocks_for(m.nv)), dim3(256), 0, s, m.nv, m.ncorner,
                            (const int*)t->star_deg[nxt], (const int*)t->star_nbr[nxt], t->c_bucket, t->c_count);
-        hipLaunchKernelGGL(k_corner_fans, dim3(1), dim3(64), 0, s, m.ncorner, (const int*)t->c_bucket,
-                           (const int*)t->c_count, t->star_deg[nxt], t->star_nbr[nxt], t->s_aux + 18,
+        hipLaunchKernelGGL(k_corner_fans, dim3((m.ncorner + 63) / 64), dim3(64), 0, s, m.ncorner, (const int*)t->c_bucket,
+                           (const int*)t->c_count, (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur],
+                           t->star_deg[nxt], t->star_nbr[nxt], t->s_aux + 18,
                            (long long*)(t->s_aux + 16));
     };
     // the counters, and the statistics with them (one synchronisation)
@@ -2486,7 +2487,7 @@ static int kinetic_step(mcl_tri* t, const mcl_step_params_t* p, const float2* di
     // tried for 1, 2, 4, ... 64 steps (workloads whose motion it cannot follow
     // then cost the flip-based step only)
     // (small meshes: the flip-based step needs fewer launches)
-    const bool try_star = (t->m.periodic || t->m.ncorner == 4) && p->variant == 0 && t->star_deg[0] &&
+    const bool try_star = (t->m.periodic || t->m.ncorner > 0) && p->variant == 0 && t->star_deg[0] &&
                           t->star_enable && t->m.nv >= t->star_min_vertices;
     if (try_star && t->star_skip > 0) {
         t->star_skip--;

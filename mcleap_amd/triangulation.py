@@ -95,6 +95,59 @@ class HostTriangulation:
         return len(self._points)
 
 
+class HostTriangulationBatch:
+    """An ensemble of independent box triangulations held as one mesh.
+
+    The reference builds one HostTriangulation per instance
+    (Host_Triangulation.h:107-137, 296-300); here every instance is built on
+    the host (one per thread) and the results are concatenated, so that one
+    DeviceTriangulation advances the whole ensemble with each launch.
+    Vertex layout: the 4 corners of every instance first (instance i:
+    4i..4i+3), then each instance's points in its own Hilbert order.
+    `points` is a list of (n_i, 2) arrays.  Attributes: m_pos, m_v, m_he,
+    m_t (the AoS arrays of the union), vstart / hestart / tstart (ninst + 1
+    offsets), perm (vertex order -> index into the concatenated input).
+    """
+
+    periodic = False
+
+    def __init__(self, points, nthreads=0):
+        pts = [np.ascontiguousarray(np.asarray(p, np.float32).reshape(-1, 2)) for p in points]
+        self.ninst = len(pts)
+        counts = np.array([len(p) for p in pts], np.int32)
+        allpts = np.ascontiguousarray(np.concatenate(pts) if pts else np.zeros((0, 2), np.float32))
+        L = lib()
+        h = _vp()
+        check(L.mcl_hmesh_build_batch(_np_ptr(allpts), _np_ptr(counts), self.ninst, int(nthreads),
+                                      ctypes.byref(h)), "mcl_hmesh_build_batch")
+        try:
+            sz = [ctypes.c_int() for _ in range(4)]
+            check(L.mcl_hmesh_sizes(h, *[ctypes.byref(s) for s in sz]), "mcl_hmesh_sizes")
+            npos, nv, nhe, nt = (s.value for s in sz)
+            self.m_pos = np.zeros((npos, 2), np.float32)
+            self.m_v = np.zeros((nv, 2), np.int32)
+            self.m_he = np.zeros((nhe, 4), np.int32)
+            self.m_t = np.zeros(nt, np.int32)
+            self.perm = np.zeros(len(allpts), np.int32)
+            check(L.mcl_hmesh_export(h, _np_ptr(self.m_pos), _np_ptr(self.m_v), _np_ptr(self.m_he),
+                                     _np_ptr(self.m_t), _np_ptr(self.perm)), "mcl_hmesh_export")
+            self.vstart = np.zeros(self.ninst + 1, np.int32)
+            self.hestart = np.zeros(self.ninst + 1, np.int32)
+            self.tstart = np.zeros(self.ninst + 1, np.int32)
+            check(L.mcl_hmesh_batch_offsets(h, _np_ptr(self.vstart), _np_ptr(self.hestart), _np_ptr(self.tstart)),
+                  "mcl_hmesh_batch_offsets")
+        finally:
+            L.mcl_hmesh_free(h)
+        self.counts = counts
+
+    def num_points(self):
+        return int(self.counts.sum())
+
+    def instance_vertices(self, i):
+        """vertex ids of instance i: its 4 corners, then its points"""
+        return np.concatenate([np.arange(4 * i, 4 * i + 4), np.arange(self.vstart[i], self.vstart[i + 1])])
+
+
 class DeviceTriangulation:
     """Device triangulation (Device_Triangulation.h:128-223), SoA on the GPU."""
 

@@ -26,11 +26,18 @@ CONFIGS = {
                radius=3.0 / 2048.0),
     "c4": dict(n=1_000_000, seed=4, domain="box", motion=_lib.MOTION_GAUSS, sigma=1e-7, amp=0.0, steps=50,
                radius=None),
+    "c5": dict(ninst=4096, seed=5, domain="box", motion=_lib.MOTION_GAUSS, sigma=5e-4, amp=0.0, steps=100,
+               radius=None, nmin=256, nmax=65536),
 }
 
 
 def unique_points(xy):
-    _, idx = np.unique(xy, axis=0, return_index=True)
+    """xy without repeated points (first occurrence kept, order preserved)"""
+    b = np.ascontiguousarray(xy + np.float32(0.0), np.float32).view(np.uint32).astype(np.uint64)  # -0 -> +0
+    key = (b[:, 0] << np.uint64(32)) | b[:, 1]
+    _, idx = np.unique(key, return_index=True)
+    if len(idx) == len(xy):
+        return xy
     return xy[np.sort(idx)]
 
 
@@ -60,3 +67,28 @@ def grid_shape(radius, domain):
     """cells of side >= radius covering [0, 1)"""
     nc = max(3, int(np.floor(1.0 / radius)))
     return nc, 1.0 / nc
+
+
+def ensemble_sizes(ninst=None, seed=None, nmin=None, nmax=None):
+    """instance sizes of the batched ensemble: floor(exp U[ln nmin, ln nmax]),
+    by default nmin = 256, nmax = 65536"""
+    cfg = CONFIGS["c5"]
+    rng = np.random.default_rng(cfg["seed"] if seed is None else seed)
+    u = rng.uniform(np.log(nmin or cfg["nmin"]), np.log(nmax or cfg["nmax"]), ninst or cfg["ninst"])
+    return np.floor(np.exp(u)).astype(np.int64)
+
+
+def ensemble_points(ninst=None, seed=None, nmin=None, nmax=None, sizes=None):
+    """one (n_i, 2) float32 array per instance, uniform in [0, 1)^2, from
+    per-instance PCG64 streams (seed, i), duplicates dropped"""
+    cfg = CONFIGS["c5"]
+    seed = cfg["seed"] if seed is None else seed
+    if sizes is None:
+        sizes = ensemble_sizes(ninst, seed, nmin, nmax)
+    out = []
+    for i, n in enumerate(sizes):
+        rng = np.random.default_rng([seed, i])
+        xy = rng.random((int(n), 2), dtype=np.float64).astype(np.float32)
+        xy[xy >= 1.0] = 0.0
+        out.append(unique_points(xy))
+    return out

@@ -1,0 +1,52 @@
+"""Host build of the batched ensemble (mcl_hmesh_build_batch): the union
+mesh holds every instance's own exact triangulation with its ids offset, the
+corners of all instances first."""
+import numpy as np
+
+import mcleap_amd as M
+from mcleap_amd.workloads import ensemble_points, ensemble_sizes
+
+
+def batch_sizes():
+    """an empty instance, tiny ones, and log-uniform sizes (24 instances)"""
+    return np.concatenate([[0, 1, 2, 3, 17], ensemble_sizes(19, seed=11, nmin=64, nmax=3000)])
+
+
+def test_batch_build_matches_single(oracle):
+    pts = ensemble_points(seed=11, sizes=batch_sizes())
+    B = M.HostTriangulationBatch(pts, nthreads=3)
+    assert B.ninst == len(pts) and B.vstart[0] == 4 * B.ninst
+    assert B.vstart[-1] == len(B.m_pos) and B.hestart[-1] == len(B.m_he) and B.tstart[-1] == len(B.m_t)
+    off = 0
+    for i, p in enumerate(pts):
+        H = M.HostTriangulation()
+        q = p.copy()
+        H.addDelaunayPoints(q)
+        vs = B.instance_vertices(i)
+        hs, ts = B.hestart[i], B.tstart[i]
+        assert np.array_equal(B.m_pos[vs], H.m_pos)
+        g = B.m_he[hs:B.hestart[i + 1]]
+        loc = H.m_he
+        assert np.array_equal(g[:, 0] - hs, loc[:, 0])
+        assert np.array_equal(g[:, 1], vs[loc[:, 1]])
+        assert np.array_equal(g[:, 2], np.where(loc[:, 2] >= 0, loc[:, 2] + ts, loc[:, 2]))
+        assert np.array_equal(g[:, 3], np.where(loc[:, 3] >= 0, vs[np.maximum(loc[:, 3], 0)], loc[:, 3]))
+        assert np.array_equal(B.m_t[ts:B.tstart[i + 1]] - hs, H.m_t)
+        assert np.array_equal(B.m_v[vs, 1] - hs, H.m_v[:, 1]) and np.array_equal(B.m_v[:, 0], np.arange(len(B.m_v)))
+        seg = B.perm[B.vstart[i] - 4 * B.ninst:B.vstart[i + 1] - 4 * B.ninst]
+        assert np.array_equal(seg - off, H.perm)
+        # and the instance is the oracle's exact DT of its points
+        ref = oracle.delaunay(H.m_pos[4:])
+        p2, v2, h2, t2 = ref.export()
+        assert np.array_equal(oracle.canonical_triangles(H.m_he, H.m_t), oracle.canonical_triangles(h2, t2))
+        off += len(p)
+
+
+def test_batch_build_rejects_bad_input():
+    bad = [np.array([[0.1, 0.1], [0.1, 0.1]], np.float32)]  # a duplicate point
+    try:
+        M.HostTriangulationBatch([np.zeros((3, 2), np.float32) + [[0.2, 0.3]], *bad])
+    except M.MclError as e:
+        assert "3" in str(e) or "DUPLICATE" in str(e).upper()
+    else:
+        raise AssertionError("duplicate points accepted")
