@@ -222,7 +222,15 @@ def main():
     ring_nbr = torch.zeros(dt.m_he_size, dtype=torch.int32, device=dev)
     radius = cfg["radius"]
     grid = None
-    if radius:
+    bgrid = None
+    if batch:
+        # one grid per instance, r_i = 2/sqrt(n_i), sorted and queried together
+        radius = "2/sqrt(n_i)"
+        rad_i = np.array([2.0 / np.sqrt(max(int(c), 1)) for c in H.counts], np.float32)
+        bgrid = M.GridCount2dBatch(H.vstart, np.minimum(rad_i, np.float32(1.0 / 3.0)))
+        f_off = torch.zeros(bgrid.numP + 1, dtype=torch.int32, device=dev)
+        f_nbr = torch.zeros(bgrid.numP * 32, dtype=torch.int32, device=dev)
+    elif radius:
         nc, cs = grid_shape(radius, cfg["domain"])
         flags = M.GRID_MORTON | (M.GRID_PERIODIC if periodic else 0)
         grid = M.GridCount2d(nv, (0.0, 0.0), (cs, cs), (nc, nc), flags)
@@ -236,6 +244,9 @@ def main():
         if grid is not None:
             grid.build(pos_t)
             grid.frnn_csr_async(grid.sorted_ptr(), radius, f_off, f_nbr)
+        if bgrid is not None:
+            bgrid.build(pos_t)
+            bgrid.frnn_csr_async(f_off, f_nbr)
         return st
 
     k = 0
@@ -244,8 +255,9 @@ def main():
         k += 1
     torch.cuda.synchronize()
     dt.kernel_timer(True)
-    if grid is not None:
-        grid.kernel_timer(True)
+    for g_ in (grid, bgrid):
+        if g_ is not None:
+            g_.kernel_timer(True)
     stats = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -265,11 +277,13 @@ def main():
         ms, cnt = dt.kernel_timer_read(which)
         kern[name] = (ms, cnt)
     kbar = 0.0
-    if grid is not None:
+    for g_ in (grid, bgrid):
+        if g_ is None:
+            continue
         for name, which in (("frnn_count", 0), ("frnn_fill", 1), ("grid_build", 2)):
-            ms, cnt = grid.kernel_timer_read(which)
+            ms, cnt = g_.kernel_timer_read(which)
             kern[name] = (ms, cnt)
-        kbar = float(f_off[-1].item()) / nv
+        kbar = float(f_off[-1].item()) / g_.numP
     dt.kernel_timer(False)
     bytes_per_pt = dict(BYTES)
     bytes_per_pt["frnn_fill"] = 4.0 * kbar

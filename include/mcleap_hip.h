@@ -266,10 +266,13 @@ int mcl_grid_mean_occupancy(mcl_grid_t* g, float* out, void* stream);
 /* No counterpart in the reference (its step has no variants): choices of the
  * fast kinetic step (variant 0).
  * MCL_OPT_STAR_MIN_VERTICES: meshes with at least this many vertices (default
- *   65536) restore the triangulation by per-vertex star recomputation (torus,
- *   and box meshes with their 4 corners at vertices 0..3); smaller ones, where
- *   the number of launches dominates, by the flip-based step.  0 = always.
- * MCL_OPT_STAR_ENABLE: 0 = never use the star recomputation (default 1). */
+ *   65536) may restore the triangulation by per-vertex star recomputation
+ *   (torus, and box meshes with their corners first); smaller ones, where
+ *   the number of launches dominates, use the flip-based step.  0 = always.
+ * MCL_OPT_STAR_ENABLE: 0 = never use the star recomputation; 1 = every step
+ *   (after a step where it did not settle, skipped for 1, 2, 4 ... 64 steps);
+ *   2 (default) = the path whose last step was faster, the other one timed
+ *   again every 32 steps. */
 #define MCL_OPT_STAR_MIN_VERTICES 1
 #define MCL_OPT_STAR_ENABLE 2
 int mcl_tri_set_option(mcl_tri_t* t, int option, long long value);
@@ -286,6 +289,40 @@ int mcl_tri_kernel_timer(mcl_tri_t* t, int enable);
 int mcl_tri_kernel_timer_read(mcl_tri_t* t, int which, double* total_ms, long long* launches);
 int mcl_grid_kernel_timer(mcl_grid_t* g, int enable);
 int mcl_grid_kernel_timer_read(mcl_grid_t* g, int which, double* total_ms, long long* launches);
+
+/* =========================================================================
+ * Batched grids for an ensemble (no counterpart in the reference, where each
+ * instance would hold its own GridCount2d, gridCount2d.h:23-48)
+ * ========================================================================= */
+typedef struct mcl_bgrid mcl_bgrid_t;
+/* Instance i's points are entries vstart[i]..vstart[i+1]-1 of the position
+ * array (mcl_hmesh_batch_offsets), all in [0,1)^2.  Its grid has
+ * nc_i = max(3, floor(1/radius[i])) cells per side (clamped at the border)
+ * and its query radius is radius[i]; MCL_ERR_INPUT when a cell is smaller
+ * than the radius. */
+int mcl_bgrid_create(int ninst, const int* vstart, const float* radius, mcl_bgrid_t** out);
+int mcl_bgrid_destroy(mcl_bgrid_t* g);
+/* points sorted (npoints = vstart[ninst] - vstart[0], first = vstart[0]),
+ * total cells */
+int mcl_bgrid_sizes(const mcl_bgrid_t* g, int* npoints, int* first, int* ncells);
+/* counting sort of every instance's points by (instance, cell) in one pass
+ * (GridCount2d::update, gridCount2d.h:194-258, for all grids at once); the
+ * sorted positions stay in the grid.  Asynchronous on `stream`. */
+int mcl_bgrid_build(mcl_bgrid_t* g, const void* d_pos, void* stream);
+void* mcl_bgrid_sorted_ptr(mcl_bgrid_t* g);
+/* d_ind[v - first] = sorted slot of point v, d_perm[s] = v - first of slot s,
+ * d_cell_start = ncells + 1 slot offsets (instance i's cells after instance
+ * i-1's, row-major) */
+int mcl_bgrid_index_ptrs(mcl_bgrid_t* g, void** d_ind, void** d_perm, void** d_cell_start);
+/* FRNN CSR over the sorted points (apply_f_frnn + SaveNeighborsFunctor,
+ * gridCount2d.h:262-302, with each instance's own radius): d_offsets has
+ * npoints + 1 entries, neighbours are sorted slots of the same instance.
+ * total = NULL: asynchronous, writes past cap dropped; else as
+ * mcl_grid_frnn_csr. */
+int mcl_bgrid_frnn_csr(mcl_bgrid_t* g, void* d_offsets, void* d_nbrs, int cap, long long* total, void* stream);
+/* HIP-event timers: 0 FRNN count, 1 FRNN fill, 2 build */
+int mcl_bgrid_kernel_timer(mcl_bgrid_t* g, int enable);
+int mcl_bgrid_kernel_timer_read(mcl_bgrid_t* g, int which, double* total_ms, long long* launches);
 
 /* ----------------------------------------------------------------- misc */
 /* library build string and the device it runs on */

@@ -7,10 +7,12 @@ DeviceTriangulation, GridCount2d) used by tests and benchmarks.
 """
 from ._lib import (GRID_MORTON, GRID_PERIODIC, GRID_ROWMAJOR, MOTION_GAUSS, MOTION_SHEAR, OPT_STAR_ENABLE,
                    OPT_STAR_MIN_VERTICES, MclError, lib)
-from .triangulation import DeviceTriangulation, GridCount2d, HostTriangulation, HostTriangulationBatch
+from .triangulation import (DeviceTriangulation, GridCount2d, GridCount2dBatch, HostTriangulation,
+                            HostTriangulationBatch)
 
 __all__ = [
-    "HostTriangulation", "HostTriangulationBatch", "DeviceTriangulation", "GridCount2d", "MclError", "lib",
+    "HostTriangulation", "HostTriangulationBatch", "DeviceTriangulation", "GridCount2d", "GridCount2dBatch",
+    "MclError", "lib",
     "MOTION_GAUSS", "MOTION_SHEAR", "GRID_ROWMAJOR", "GRID_MORTON", "GRID_PERIODIC", "OPT_STAR_MIN_VERTICES",
     "OPT_STAR_ENABLE",
 ]

@@ -121,10 +121,20 @@ SIGNATURES = {
     "mcl_tri_closest_ring": [_p, _i, _i, _p, _p, _p],
     "mcl_tri_move_legalize": [_p, _p, _i, _p, _p],
     "mcl_tri_frnn_graph": [_p, _f, _i, _i, _p, _i, _p, ctypes.POINTER(_i), _p],
+    "mcl_bgrid_create": [_i, _p, _p, _pp],
+    "mcl_bgrid_destroy": [_p],
+    "mcl_bgrid_sizes": [_p, _p, _p, _p],
+    "mcl_bgrid_build": [_p, _p, _p],
+    "mcl_bgrid_sorted_ptr": [_p],
+    "mcl_bgrid_index_ptrs": [_p, _pp, _pp, _pp],
+    "mcl_bgrid_frnn_csr": [_p, _p, _p, _i, _p, _p],
+    "mcl_bgrid_kernel_timer": [_p, _i],
+    "mcl_bgrid_kernel_timer_read": [_p, _i, _p, _p],
     "mcl_version": [],
     "mcl_device_info": [_p, _i, _p],
 }
-RESTYPES = {"mcl_tri_pos_ptr": _p, "mcl_grid_sorted_ptr": _p, "mcl_version": ctypes.c_char_p}
+RESTYPES = {"mcl_tri_pos_ptr": _p, "mcl_grid_sorted_ptr": _p, "mcl_bgrid_sorted_ptr": _p,
+            "mcl_version": ctypes.c_char_p}
 
 
 def lib():

@@ -16,7 +16,10 @@
 // `hh.x > nx` bound (gridCount2d.h:278-281); MCL_GRID_MORTON orders cells
 // along a Morton curve (cells of a tile are contiguous);
 // MCL_GRID_PERIODIC wraps cells and distances on the unit torus.
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <vector>
 
 #include "mcl_geom.hpp"
 #include "mcl_internal.hpp"
@@ -338,6 +341,257 @@ extern "C" int mcl_grid_kernel_timer(mcl_grid_t* G, int enable) {
     return MCL_OK;
 }
 extern "C" int mcl_grid_kernel_timer_read(mcl_grid_t* G, int which, double* total_ms, long long* launches) {
+    if (!G || which < 0 || which > 2) return MCL_ERR_INPUT;
+    double ms = G->timers[which].total_ms(launches);
+    if (total_ms) *total_ms = ms;
+    return MCL_OK;
+}
+
+// ------------------------------------------------------------------------
+// Batched grids: one uniform grid per instance of an ensemble held as one
+// point array (instance i: points vstart[i]..vstart[i+1]-1, all in [0,1)^2).
+// Instance i has nc_i = max(3, floor(1/r_i)) cells per side of side
+// cs_i = 1/nc_i, cells clamped at the border (as MCL_GRID_MORTON), and its
+// cells are keys cbase[i] .. cbase[i] + nc_i^2 - 1 in row-major order, so
+// one counting sort orders every instance's points by (instance, cell).
+// The query is the reference's per-grid test (gridCount2d.h:284-290) with
+// the instance's own radius: j != i in the 3x3 cells of i's instance grid,
+// r2 = dx*dx + dy*dy <= r_i*r_i in FP32 without contraction.
+namespace {
+
+struct BGridParams {
+    const int* vstart;  // ninst + 1
+    const int* nc;      // ninst
+    const float* cs;    // ninst
+    const float* rad2;  // ninst
+    const int* cbase;   // ninst + 1
+    int ninst, v0, n;   // points v0 .. v0 + n - 1
+};
+
+// the instance holding point v: the largest i with vstart[i] <= v
+__device__ __forceinline__ int bgrid_inst(const BGridParams& b, int v) {
+    int lo = 0, hi = b.ninst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b.vstart[mid] <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int2 bgrid_cell(float cs, int nc, float2 p) {
+    int x = (int)__fdiv_rn(p.x, cs), y = (int)__fdiv_rn(p.y, cs);
+    x = min(max(x, 0), nc - 1);
+    y = min(max(y, 0), nc - 1);
+    return make_int2(x, y);
+}
+
+__global__ void __launch_bounds__(kT) k_bkey_count(BGridParams b, const float2* __restrict__ pos,
+                                                   int* __restrict__ key, int* __restrict__ count) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
+        const int v = b.v0 + i;
+        const int t = bgrid_inst(b, v);
+        const int nc = b.nc[t];
+        const int2 c = bgrid_cell(b.cs[t], nc, pos[v]);
+        const int k = b.cbase[t] + c.y * nc + c.x;
+        key[i] = k;
+        atomicAdd(&count[k], 1);
+    }
+}
+
+// pass 0 counts, pass 1 writes (sorted slots; perm[s] = point index - v0)
+template <int kPass>
+__global__ void __launch_bounds__(kT) k_bfrnn(BGridParams b, const float2* __restrict__ spos,
+                                              const int* __restrict__ perm, const int* __restrict__ start,
+                                              int* __restrict__ cnt, const int* __restrict__ off, int* __restrict__ nbr,
+                                              int cap) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
+        const float2 pi = spos[i];
+        const int t = bgrid_inst(b, b.v0 + perm[i]);
+        const int nc = b.nc[t], cb = b.cbase[t];
+        const float rad2 = b.rad2[t];
+        const int2 h = bgrid_cell(b.cs[t], nc, pi);
+        int c = 0, w = kPass ? off[i] : 0;
+        for (int y = max(h.y - 1, 0); y <= min(h.y + 1, nc - 1); y++) {
+            for (int x = max(h.x - 1, 0); x <= min(h.x + 1, nc - 1); x++) {
+                const int k = cb + y * nc + x;
+                const int e = start[k + 1];
+                for (int j = start[k]; j < e; j++) {
+                    if (j == i) continue;
+                    const float2 pj = spos[j];
+                    const float dx = __fsub_rn(pj.x, pi.x), dy = __fsub_rn(pj.y, pi.y);
+                    const float r2 = __fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy));
+                    if (r2 <= rad2) {
+                        if (kPass && w < cap) nbr[w] = j;
+                        w++;
+                        c++;
+                    }
+                }
+            }
+        }
+        if (!kPass) cnt[i] = c;
+    }
+}
+
+}  // namespace
+
+struct mcl_bgrid {
+    int ninst = 0, v0 = 0, n = 0, nkeys = 0;
+    int *vstart = nullptr, *nc = nullptr, *cbase = nullptr;
+    float *cs = nullptr, *rad2 = nullptr;
+    int *key = nullptr, *start = nullptr, *fill = nullptr, *perm = nullptr, *ind = nullptr, *scan_tmp = nullptr;
+    float2* spos = nullptr;
+    mcl::KTimer timers[3];  // FRNN count, FRNN fill, build
+    BGridParams params() const { return {vstart, nc, cs, rad2, cbase, ninst, v0, n}; }
+};
+
+static void bgrid_free(mcl_bgrid* G) {
+    void* p[] = {G->vstart, G->nc, G->cbase, G->cs, G->rad2, G->key, G->start, G->fill, G->perm, G->ind,
+                 G->scan_tmp, G->spos};
+    for (void* q : p)
+        if (q) (void)hipFree(q);
+}
+
+extern "C" int mcl_bgrid_create(int ninst, const int* vstart, const float* radius, mcl_bgrid_t** out) {
+    if (!out || ninst <= 0 || !vstart || !radius) return MCL_ERR_INPUT;
+    std::vector<int> nc(ninst), cbase(ninst + 1, 0);
+    std::vector<float> cs(ninst), rad2(ninst);
+    long long keys = 0;
+    for (int i = 0; i < ninst; i++) {
+        if (vstart[i + 1] < vstart[i] || vstart[i] < 0 || !(radius[i] > 0.0f) || !(radius[i] < 1e30f))
+            return MCL_ERR_INPUT;
+        const double k = std::floor(1.0 / (double)radius[i]);
+        nc[i] = (int)std::max(3.0, std::min(k, 4096.0));
+        cs[i] = 1.0f / (float)nc[i];
+        if (!(cs[i] >= radius[i])) return MCL_ERR_INPUT;  // a 3x3 query must cover the radius
+        rad2[i] = radius[i] * radius[i];
+        cbase[i] = (int)keys;
+        keys += (long long)nc[i] * nc[i];
+        if (keys >= (1LL << 30)) return MCL_ERR_CAPACITY;
+    }
+    cbase[ninst] = (int)keys;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MCL_ERR_NO_DEVICE;
+    mcl_bgrid* G = new mcl_bgrid();
+    G->ninst = ninst;
+    G->v0 = vstart[0];
+    G->n = vstart[ninst] - vstart[0];
+    G->nkeys = (int)keys;
+    const int n = G->n, nk = G->nkeys;
+    bool ok = hipMalloc(&G->vstart, sizeof(int) * (ninst + 1)) == hipSuccess &&
+              hipMalloc(&G->nc, sizeof(int) * ninst) == hipSuccess &&
+              hipMalloc(&G->cbase, sizeof(int) * (ninst + 1)) == hipSuccess &&
+              hipMalloc(&G->cs, sizeof(float) * ninst) == hipSuccess &&
+              hipMalloc(&G->rad2, sizeof(float) * ninst) == hipSuccess &&
+              hipMalloc(&G->key, sizeof(int) * (n + 1)) == hipSuccess &&
+              hipMalloc(&G->start, sizeof(int) * (nk + 1)) == hipSuccess &&
+              hipMalloc(&G->fill, sizeof(int) * (nk + 1)) == hipSuccess &&
+              hipMalloc(&G->perm, sizeof(int) * (n + 1)) == hipSuccess &&
+              hipMalloc(&G->ind, sizeof(int) * (n + 1)) == hipSuccess &&
+              hipMalloc(&G->scan_tmp, sizeof(int) * (scan_scratch_ints((nk > n ? nk : n) + 1) + 16)) == hipSuccess &&
+              hipMalloc(&G->spos, sizeof(float2) * (n + 1)) == hipSuccess;
+    ok = ok && hipMemcpy(G->vstart, vstart, sizeof(int) * (ninst + 1), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(G->nc, nc.data(), sizeof(int) * ninst, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(G->cbase, cbase.data(), sizeof(int) * (ninst + 1), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(G->cs, cs.data(), sizeof(float) * ninst, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(G->rad2, rad2.data(), sizeof(float) * ninst, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, 0, G->start, nk + 1);
+        ok = hipDeviceSynchronize() == hipSuccess;
+    }
+    if (!ok) { bgrid_free(G); delete G; return MCL_ERR_HIP; }
+    *out = G;
+    return MCL_OK;
+}
+
+extern "C" int mcl_bgrid_destroy(mcl_bgrid_t* G) {
+    if (!G) return MCL_OK;
+    (void)hipDeviceSynchronize();
+    bgrid_free(G);
+    delete G;
+    return MCL_OK;
+}
+
+extern "C" int mcl_bgrid_sizes(const mcl_bgrid_t* G, int* npoints, int* first, int* ncells) {
+    if (!G) return MCL_ERR_INPUT;
+    if (npoints) *npoints = G->n;
+    if (first) *first = G->v0;
+    if (ncells) *ncells = G->nkeys;
+    return MCL_OK;
+}
+
+extern "C" int mcl_bgrid_build(mcl_bgrid_t* G, const void* d_pos, void* stream) {
+    if (!G || !d_pos) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    const int n = G->n, nk = G->nkeys;
+    const BGridParams b = G->params();
+    G->timers[2].mark(s);
+    hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, G->start, nk + 1);
+    hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, G->fill, nk + 1);
+    hipLaunchKernelGGL(k_bkey_count, dim3(blocks(n)), dim3(kT), 0, s, b, (const float2*)d_pos, G->key, G->start);
+    int rc = scan_exclusive(G->start, G->start, nk, G->scan_tmp, s);
+    if (rc != MCL_OK) return rc;
+    hipLaunchKernelGGL(k_slot, dim3(blocks(n)), dim3(kT), 0, s, G->key, n, G->start, G->fill, G->perm);
+    hipLaunchKernelGGL(k_cell_sort, dim3(blocks(nk)), dim3(kT), 0, s, G->start, nk, G->perm);
+    hipLaunchKernelGGL(k_gather, dim3(blocks(n)), dim3(kT), 0, s, G->perm, n, (const float2*)d_pos + G->v0,
+                       (const float2*)nullptr, G->spos, (float2*)nullptr, G->ind);
+    G->timers[2].mark(s);
+    MCL_HIP(hipGetLastError());
+    return MCL_OK;
+}
+
+extern "C" void* mcl_bgrid_sorted_ptr(mcl_bgrid_t* G) { return G ? (void*)G->spos : nullptr; }
+
+extern "C" int mcl_bgrid_index_ptrs(mcl_bgrid_t* G, void** d_ind, void** d_perm, void** d_cell_start) {
+    if (!G) return MCL_ERR_INPUT;
+    if (d_ind) *d_ind = G->ind;
+    if (d_perm) *d_perm = G->perm;
+    if (d_cell_start) *d_cell_start = G->start;
+    return MCL_OK;
+}
+
+extern "C" int mcl_bgrid_frnn_csr(mcl_bgrid_t* G, void* d_offsets, void* d_nbrs, int cap, long long* total,
+                                  void* stream) {
+    if (!G || !d_offsets) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    const int n = G->n;
+    const BGridParams b = G->params();
+    int* off = (int*)d_offsets;
+    G->timers[0].mark(s);
+    hipLaunchKernelGGL(k_bfrnn<0>, dim3(blocks(n)), dim3(kT), 0, s, b, (const float2*)G->spos, (const int*)G->perm,
+                       (const int*)G->start, off, (const int*)nullptr, (int*)nullptr, 0);
+    G->timers[0].mark(s);
+    int rc = scan_exclusive(off, off, n, G->scan_tmp, s);
+    if (rc != MCL_OK) return rc;
+    auto fill = [&]() {
+        G->timers[1].mark(s);
+        if (d_nbrs && cap > 0)
+            hipLaunchKernelGGL(k_bfrnn<1>, dim3(blocks(n)), dim3(kT), 0, s, b, (const float2*)G->spos,
+                               (const int*)G->perm, (const int*)G->start, (int*)nullptr, (const int*)off,
+                               (int*)d_nbrs, cap);
+        G->timers[1].mark(s);
+    };
+    if (!total) {  // asynchronous form: the fill pass keeps writes below cap
+        fill();
+        MCL_HIP(hipGetLastError());
+        return MCL_OK;
+    }
+    int tot = 0;
+    MCL_HIP(hipMemcpyAsync(&tot, off + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    MCL_HIP(hipStreamSynchronize(s));
+    *total = tot;
+    fill();
+    MCL_HIP(hipGetLastError());
+    MCL_HIP(hipStreamSynchronize(s));
+    return tot > cap ? MCL_ERR_CAPACITY : MCL_OK;
+}
+
+extern "C" int mcl_bgrid_kernel_timer(mcl_bgrid_t* G, int enable) {
+    if (!G) return MCL_ERR_INPUT;
+    for (auto& k : G->timers) k.reset(enable != 0);
+    return MCL_OK;
+}
+extern "C" int mcl_bgrid_kernel_timer_read(mcl_bgrid_t* G, int which, double* total_ms, long long* launches) {
     if (!G || which < 0 || which > 2) return MCL_ERR_INPUT;
     double ms = G->timers[which].total_ms(launches);
     if (total_ms) *total_ms = ms;

@@ -32,6 +32,17 @@
 namespace mcl {
 
 constexpr int kStarCap = 24;    // stored neighbours per vertex
+constexpr int kCornerCap = 128; // stored fan entries per corner (points clamped onto a side all reach a corner)
+constexpr int kCornerWindow = 3;  // fan entries on each side of a vertex that a corner adds to its candidates
+
+// Star rows: vertex v >= ncorner at v * kStarCap; the corners' (longer) fans
+// after all vertex rows, at nv * kStarCap + c * kCornerCap.
+__host__ __device__ __forceinline__ size_t star_row(int v, int nv, int ncorner) {
+    return v < ncorner ? (size_t)nv * kStarCap + (size_t)v * kCornerCap : (size_t)v * kStarCap;
+}
+__host__ __device__ __forceinline__ size_t star_rows_size(int nv, int ncorner) {
+    return (size_t)nv * kStarCap + (size_t)ncorner * kCornerCap;
+}
 constexpr int kCandMax = 128;   // candidates per vertex: two per lane
 constexpr int kHashSlots = 256; // per-wave LDS hash set for candidate de-duplication
 constexpr int kStarPasses = 12;  // star computations per step before the flip-based fallback
@@ -50,20 +61,21 @@ __global__ void __launch_bounds__(256) k_stars_from_mesh(DMesh m, int* __restric
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.ncorner; v += gridDim.x * blockDim.x) {
         // clockwise from the outgoing half-edge that borders the outer face (v -> B)
         int h = m.v_he[v], guard = 0;
-        while (m.he_t[h] != -1 && guard++ < 4 * kStarCap) h = m.he_next[h ^ 1];
+        while (m.he_t[h] != -1 && guard++ < 2 * kCornerCap) h = m.he_next[h ^ 1];
         int d = 0;
         const int h0 = h;
         bool okf = m.he_t[h] == -1;
+        int* row = nbr + star_row(v, m.nv, m.ncorner);
         do {
-            if (d == kStarCap) { okf = false; break; }
-            nbr[(size_t)v * kStarCap + d++] = m.he_v[h ^ 1];  // B, xk, ..., x1, A
+            if (d == kCornerCap) { okf = false; break; }
+            row[d++] = m.he_v[h ^ 1];  // B, xk, ..., x1, A
             h = m.he_next[h ^ 1];
         } while (okf && h != h0);
         if (okf && h == h0) {
             for (int i = 0; i < d / 2; i++) {  // reverse: A, x1, ..., xk, B
-                const int a = nbr[(size_t)v * kStarCap + i], b = nbr[(size_t)v * kStarCap + d - 1 - i];
-                nbr[(size_t)v * kStarCap + i] = b;
-                nbr[(size_t)v * kStarCap + d - 1 - i] = a;
+                const int a = row[i], b = row[d - 1 - i];
+                row[i] = b;
+                row[d - 1 - i] = a;
             }
             deg[v] = d;
         } else {
@@ -109,21 +121,36 @@ __device__ __forceinline__ void cand_insert(int* tab, int* list, int* count, int
     atomicAdd(count, kC + 1);  // table full: reported as too many candidates
 }
 
-// inserts the star of u (from one star array) into the candidate set of v
+// inserts the star of u (from one star array) into the candidate set of v;
+// of a corner's fan only the entries within kCornerWindow + 1 of v
 template <int kH, int kC>
 __device__ __forceinline__ void insert_star(int* tab, int* list, int* count, const int* __restrict__ deg,
-                                            const int* __restrict__ nbr, int u, int v, int nv) {
+                                            const int* __restrict__ nbr, int u, int v, int nv, int ncorner) {
     const int du = deg[u];
+    const int* row = nbr + star_row(u, nv, ncorner);
+    if (u < ncorner) {
+        if (du < 1 || du > kCornerCap) return;
+        int p = -1;
+        for (int k = 0; k < du; k++)
+            if (row[k] == v) p = k;
+        if (p < 0) return;
+        for (int k = -kCornerWindow - 1; k <= kCornerWindow + 1; k++) {
+            const int c = row[((p + k) % du + du) % du];
+            if (c != v && (unsigned)c < (unsigned)nv) cand_insert<kH, kC>(tab, list, count, c);
+        }
+        return;
+    }
     for (int k = 0; k < du && k < kStarCap; k++) {
-        const int c = nbr[(size_t)u * kStarCap + k];
+        const int c = row[k];
         if (c != v && (unsigned)c < (unsigned)nv) cand_insert<kH, kC>(tab, list, count, c);
     }
 }
 
-// position of x in the star of u (deg du), -1 if absent; the row is read as
-// 16-byte vectors (rows are kStarCap = 24 ints, 96 bytes, 16-byte aligned)
-__device__ __forceinline__ int star_find(const int* __restrict__ nbr, int u, int du, int x) {
-    const int4* r = reinterpret_cast<const int4*>(nbr + (size_t)u * kStarCap);
+// position of x in a star row of du entries, -1 if absent; the row is read as
+// 16-byte vectors (rows are kStarCap = 24 or kCornerCap = 128 ints, 16-byte
+// aligned)
+__device__ __forceinline__ int row_find(const int* __restrict__ row, int du, int x) {
+    const int4* r = reinterpret_cast<const int4*>(row);
     int pos = -1;
     for (int k = 0; k < du; k += 4) {
         const int4 q = r[k >> 2];
@@ -133,6 +160,10 @@ __device__ __forceinline__ int star_find(const int* __restrict__ nbr, int u, int
         if (q.x == x) pos = k;
     }
     return pos;
+}
+// position of x in the star of vertex u >= ncorner
+__device__ __forceinline__ int star_find(const int* __restrict__ nbr, int u, int du, int x) {
+    return row_find(nbr + (size_t)u * kStarCap, du, x);
 }
 
 // XCD-aware split of n items over the grid: the blocks of a launch are dealt
@@ -270,13 +301,22 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
             // u_i contributes the entries from two after v to three before v.
             // A vertex can still come twice (two non-consecutive neighbours);
             // the hull step compares candidates by id, so that is harmless.
+            // A corner's fan adds only the kCornerWindow entries on either side
+            // of that range (its far entries are hull points far from v).
             const int d = active ? deg_a[v] : 0;
             int u = -1, du = 0, p = -1, cnt = 0;
+            const int* urow = nbr_a;
             if (lane < d) {
                 u = nbr_a[(size_t)v * kStarCap + lane];
                 du = deg_a[u];
-                p = star_find(nbr_a, u, du, v);
-                cnt = p >= 0 ? (du > 4 ? du - 4 : 0) : du;
+                urow = nbr_a + star_row(u, m.nv, m.ncorner);
+                if (u < m.ncorner) {
+                    p = (du >= 1 && du <= kCornerCap) ? row_find(urow, du, v) : -1;
+                    cnt = p >= 0 && du > 4 ? min(du - 4, 2 * kCornerWindow) : 0;
+                } else {
+                    p = row_find(urow, du, v);
+                    cnt = p >= 0 ? (du > 4 ? du - 4 : 0) : du;
+                }
             }
             int x = cnt;  // inclusive prefix over the group's lanes
             for (int off = 1; off < kL; off <<= 1) {
@@ -286,13 +326,17 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
             kc = d + __shfl(x, gbase + kL - 1);
             if (lane < d && lane < kC) list[lane] = u;
             int at = d + x - cnt;
+            const bool fan = u >= 0 && u < m.ncorner;
             for (int k = 0; k < cnt; k++, at++) {
                 int q = k;
-                if (p >= 0) {
+                if (fan) {
+                    q = k < kCornerWindow ? p + 2 + k : p - 3 - (k - kCornerWindow);
+                    q = (q % du + du) % du;
+                } else if (p >= 0) {
                     q = p + 2 + k;
                     if (q >= du) q -= du;
                 }
-                if (at < kC) list[at] = nbr_a[(size_t)u * kStarCap + q];
+                if (at < kC) list[at] = urow[q];
             }
             __syncthreads();
         } else {
@@ -328,9 +372,9 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                         const int u = list[i];
                         // (this step's rows of the corners are not built yet: their previous ones)
                         if (!deg_b || widen || u < m.ncorner)
-                            insert_star<kH, kC>(tab, list, count, deg_a, nbr_a, u, v, m.nv);
+                            insert_star<kH, kC>(tab, list, count, deg_a, nbr_a, u, v, m.nv, m.ncorner);
                         if (deg_b && dd == 2 && u >= m.ncorner)
-                            insert_star<kH, kC>(tab, list, count, deg_b, nbr_b, u, v, m.nv);
+                            insert_star<kH, kC>(tab, list, count, deg_b, nbr_b, u, v, m.nv, m.ncorner);
                     }
                 }
                 __syncthreads();
@@ -845,8 +889,7 @@ __global__ void k_star_give_up(int* __restrict__ aux, int limit) {
 // Corner fans from their neighbours' stars.  For a non-corner x with corner
 // c at position i of its star, the triangles (x, c, s_i+1) and (x, s_i-1, c)
 // put, in c's CCW fan, s_i+1 right before x and s_i-1 right after it.  Each
-// x adds (x, before, after) to c's bucket (kCornerBucket entries at most).
-constexpr int kCornerBucket = 64;
+// x adds (x, before, after) to c's bucket (kCornerCap entries at most).
 __global__ void __launch_bounds__(256) k_corner_collect(int nv, int ncorner, const int* __restrict__ deg,
                                                         const int* __restrict__ nbr, int* __restrict__ bucket,
                                                         int* __restrict__ bcount) {
@@ -857,8 +900,8 @@ __global__ void __launch_bounds__(256) k_corner_collect(int nv, int ncorner, con
             const int c = row[i];
             if (c >= ncorner) continue;
             const int k = atomicAdd(&bcount[c], 1);
-            if (k < kCornerBucket) {
-                int* e = bucket + ((size_t)c * kCornerBucket + k) * 3;
+            if (k < kCornerCap) {
+                int* e = bucket + ((size_t)c * kCornerCap + k) * 3;
                 e[0] = x;
                 e[1] = row[i + 1 == d ? 0 : i + 1];  // before x in c's fan
                 e[2] = row[i == 0 ? d - 1 : i - 1];  // after x
@@ -871,23 +914,25 @@ __global__ void __launch_bounds__(256) k_corner_collect(int nv, int ncorner, con
 // (A, B corners; another corner may sit inside the fan when two corners are
 // joined by a diagonal) and write it as the corner's star.  Every link must
 // agree from both sides (x's successor y has x as predecessor); a corner
-// whose entries do not chain into one fan of at most kStarCap is counted in
+// whose entries do not chain into one fan of at most kCornerCap is counted in
 // *fail.  Adds the corner degrees to *sum_deg.  A corner that no vertex
 // lists keeps its previous fan (pdeg, pnbr) if that fan holds corners only:
 // the corners of an instance with no points.
-__global__ void k_corner_fans(int ncorner, const int* __restrict__ bucket, const int* __restrict__ bcount,
+__global__ void k_corner_fans(int nv, int ncorner, const int* __restrict__ bucket, const int* __restrict__ bcount,
                               const int* __restrict__ pdeg, const int* __restrict__ pnbr, int* __restrict__ deg,
                               int* __restrict__ nbr, int* __restrict__ fail, long long* __restrict__ sum_deg) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncorner) return;
     const int k = bcount[c];
-    const int* b = bucket + (size_t)c * kCornerBucket * 3;  // (x, before, after) per entry
+    const int* b = bucket + (size_t)c * kCornerCap * 3;  // (x, before, after) per entry
+    int* row = nbr + star_row(c, nv, ncorner);
     if (k == 0) {
         const int d = pdeg[c];
-        bool ok = d >= 2 && d <= kStarCap;
-        for (int i = 0; ok && i < d; i++) ok = pnbr[(size_t)c * kStarCap + i] < ncorner;
+        const int* prow = pnbr + star_row(c, nv, ncorner);
+        bool ok = d >= 2 && d <= kCornerCap;
+        for (int i = 0; ok && i < d; i++) ok = prow[i] < ncorner;
         if (ok) {
-            for (int i = 0; i < d; i++) nbr[(size_t)c * kStarCap + i] = pnbr[(size_t)c * kStarCap + i];
+            for (int i = 0; i < d; i++) row[i] = prow[i];
             deg[c] = d;
             atomicAdd((unsigned long long*)sum_deg, (unsigned long long)d);
         } else {
@@ -896,7 +941,7 @@ __global__ void k_corner_fans(int ncorner, const int* __restrict__ bucket, const
         }
         return;
     }
-    bool ok = k <= kCornerBucket;
+    bool ok = k <= kCornerCap;
     // the first entry: its predecessor is a corner that follows no entry
     int first = -1;
     for (int i = 0; ok && i < k; i++) {
@@ -910,14 +955,15 @@ __global__ void k_corner_fans(int ncorner, const int* __restrict__ bucket, const
     }
     ok = ok && first >= 0;
     int d = 0;
-    unsigned long long used = 0ull;  // entries placed (k <= 64)
-    int* row = nbr + (size_t)c * kStarCap;
+    unsigned long long used_lo = 0ull, used_hi = 0ull;  // entries placed (k <= 128)
+    auto used = [&](int j) { return ((j < 64 ? used_lo >> j : used_hi >> (j - 64)) & 1ull) != 0; };
     if (ok) {
         row[d++] = b[first * 3 + 1];  // A
         int cur = first;
         while (ok) {
-            if (d == kStarCap || ((used >> cur) & 1ull)) { ok = false; break; }
-            used |= 1ull << cur;
+            if (d == kCornerCap || used(cur)) { ok = false; break; }
+            if (cur < 64) used_lo |= 1ull << cur;
+            else used_hi |= 1ull << (cur - 64);
             const int x = b[cur * 3], after = b[cur * 3 + 2];
             row[d++] = x;
             int nx = -1;
@@ -926,15 +972,15 @@ __global__ void k_corner_fans(int ncorner, const int* __restrict__ bucket, const
                     if (b[j * 3] == after) nx = j;
                 if (nx < 0 || b[nx * 3 + 1] != x) { ok = false; break; }  // links must agree
             } else {
-                if (d == kStarCap) { ok = false; break; }
+                if (d == kCornerCap) { ok = false; break; }
                 row[d++] = after;  // a corner: the end B, or the fan goes on past it
                 for (int j = 0; j < k; j++)
-                    if (!((used >> j) & 1ull) && b[j * 3 + 1] == after) nx = j;
+                    if (!used(j) && b[j * 3 + 1] == after) nx = j;
                 if (nx < 0) break;
             }
             cur = nx;
         }
-        ok = ok && __popcll(used) == k;
+        ok = ok && __popcll(used_lo) + __popcll(used_hi) == k;
     }
     if (ok) {
         deg[c] = d;
@@ -952,7 +998,7 @@ __global__ void __launch_bounds__(256) k_star_counts(int nv, int ncorner, const 
                                                      int* __restrict__ tcnt) {
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
         const int d = deg[v];
-        const int* row = nbr + (size_t)v * kStarCap;
+        const int* row = nbr + star_row(v, nv, ncorner);
         int ne = 0, nt = 0;
         const int first = row[0];
         int u = first;
@@ -979,7 +1025,8 @@ __global__ void __launch_bounds__(256) k_star_hid(DMesh m, const int* __restrict
                                                   const int* __restrict__ eoff, int* __restrict__ hid) {
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
         const int d = deg[v];
-        const int* row = nbr + (size_t)v * kStarCap;
+        const size_t rv = star_row(v, m.nv, m.ncorner);
+        const int* row = nbr + rv;
         const int first = row[0];
         const int ev = eoff[v];
         int rank = 0, u = first;
@@ -990,7 +1037,7 @@ __global__ void __launch_bounds__(256) k_star_hid(DMesh m, const int* __restrict
                 h = 2 * (ev + rank++);
             } else {  // owned by u: rank of v among u's larger neighbours
                 const int du = deg[u];
-                const int4* r = reinterpret_cast<const int4*>(nbr + (size_t)u * kStarCap);
+                const int4* r = reinterpret_cast<const int4*>(nbr + star_row(u, m.nv, m.ncorner));
                 int ru = 0;
                 bool seen = false;
                 for (int k = 0; k < du && !seen; k += 4) {
@@ -1005,7 +1052,7 @@ __global__ void __launch_bounds__(256) k_star_hid(DMesh m, const int* __restrict
                 }
                 h = 2 * (eoff[u] + ru) + 1;
             }
-            hid[(size_t)v * kStarCap + i] = h;
+            hid[rv + i] = h;
             if ((unsigned)h < (unsigned)m.nhe) {
                 const bool outer = v < m.ncorner && i + 1 == d;  // corner -> B: the outer face
                 m.he_v[h] = v;
@@ -1025,8 +1072,9 @@ __global__ void __launch_bounds__(256) k_star_link(DMesh m, const int* __restric
                                                    const int* __restrict__ hid, const int* __restrict__ toff) {
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
         const int d = deg[v];
-        const int* row = nbr + (size_t)v * kStarCap;
-        const int* hr = hid + (size_t)v * kStarCap;
+        const size_t rv = star_row(v, m.nv, m.ncorner);
+        const int* row = nbr + rv;
+        const int* hr = hid + rv;
         const int first = row[0], h_first = hr[0];
         int t = toff[v];
         int u = first, hu = h_first;
@@ -1065,12 +1113,13 @@ __global__ void __launch_bounds__(256) k_star_face3(DMesh m) {
 __global__ void __launch_bounds__(256) k_star_ring_deg(int nv, const int* __restrict__ deg, int* __restrict__ off) {
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) off[v] = deg[v];
 }
-__global__ void __launch_bounds__(256) k_star_ring_csr(int nv, const int* __restrict__ deg, const int* __restrict__ nbr,
-                                                       const int* __restrict__ off, int* __restrict__ out, int cap) {
+__global__ void __launch_bounds__(256) k_star_ring_csr(int nv, int ncorner, const int* __restrict__ deg,
+                                                       const int* __restrict__ nbr, const int* __restrict__ off,
+                                                       int* __restrict__ out, int cap) {
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
         const int d = deg[v];
         const int k0 = off[v];
-        const int* row = nbr + (size_t)v * kStarCap;
+        const int* row = nbr + star_row(v, nv, ncorner);
         for (int i = 0; i < d; i++) {
             const int k = k0 + i;
             if (k < cap) out[k] = row[i == 0 ? 0 : d - i];

@@ -1276,6 +1276,22 @@ __global__ void __launch_bounds__(kThreads) k_count_nd(DMesh m, unsigned long lo
     if (loc) atomicAdd(out, (unsigned long long)loc);
 }
 
+// the first non-Delaunay edges (diagnostic): up to 8 edge ids into out[1..],
+// their count into out[0]
+__global__ void __launch_bounds__(kThreads) k_list_nd(DMesh m, int* out) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < m.ne; e += gridDim.x * blockDim.x) {
+        const int2 tt = reinterpret_cast<const int2*>(m.he_t)[e];
+        if (tt.x < 0 || tt.y < 0) continue;
+        const int2 vv = reinterpret_cast<const int2*>(m.he_v)[e];
+        const int2 oo = reinterpret_cast<const int2*>(m.he_op)[e];
+        int st;
+        if (incircle<true, true>(m.pos[vv.x], m.pos[oo.y], m.pos[vv.y], m.pos[oo.x], m.periodic, &st) > 0) {
+            const int k = atomicAdd(out, 1);
+            if (k < 8) out[1 + k] = e;
+        }
+    }
+}
+
 __global__ void k_set_ints(int* p, int n, int v) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
@@ -1371,6 +1387,19 @@ __global__ void __launch_bounds__(kThreads) k_validate(DMesh m, int* __restrict_
     if (bad) atomicAdd(err, bad);
 }
 
+// triangles that are not strictly CCW (exact; diagnostic)
+__global__ void __launch_bounds__(kThreads) k_count_inv(DMesh m, int* __restrict__ err) {
+    int bad = 0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m.nt; t += gridDim.x * blockDim.x) {
+        const int h0 = m.t_he[t];
+        if ((unsigned)h0 >= (unsigned)m.nhe) continue;
+        const int h1 = m.he_next[h0], h2 = m.he_next[h1];
+        int st;
+        if (orient<true>(m.pos[m.he_v[h0]], m.pos[m.he_v[h1]], m.pos[m.he_v[h2]], m.periodic, &st) <= 0) bad++;
+    }
+    if (bad) atomicAdd(err, bad);
+}
+
 // ring entry c (0-based) of vertex v in the legacy layouts of k_ring_legacy
 __device__ __forceinline__ int ring_at(const int* ring, int layout, int stride, int nv, int v, int c) {
     return layout == 0 ? ring[(size_t)(c + 1) * nv + v] : ring[(size_t)(c + 1) + (size_t)stride * v];
@@ -1453,6 +1482,8 @@ __global__ void __launch_bounds__(kThreads) k_frnn_graph(DMesh m, float rr, int 
 // =========================================================================
 // host side
 // =========================================================================
+constexpr int kPathProbe = 64;  // first interval between measurements of the slower step path
+
 struct mcl_tri {
     DMesh m;
     // flip rounds
@@ -1488,14 +1519,17 @@ struct mcl_tri {
     int* tail_out = nullptr;           // 8: results of the single-workgroup loops
     // star recomputation (torus kinetic step)
     int* star_deg[2] = {nullptr, nullptr};  // nv each
-    int* star_nbr[2] = {nullptr, nullptr};  // nv * kStarCap each
+    int* star_nbr[2] = {nullptr, nullptr};  // star_rows_size(nv, ncorner) each
     int* star_tdeg = nullptr;               // nv: recomputed stars by list position
-    int* star_tnbr = nullptr;               // nv * kStarCap
+    int* star_tnbr = nullptr;               // star_rows_size(nv, ncorner)
     int star_cur = 0;                       // stars of the current mesh
     bool stars_valid = false;               // star_*[star_cur] describe the current mesh
     bool mesh_stale = false;                // half-edge arrays not yet rebuilt from the stars
     int star_min_vertices = kStarMinVertices;  // MCL_OPT_STAR_MIN_VERTICES
-    bool star_enable = true;                   // MCL_OPT_STAR_ENABLE
+    int star_enable = 2;                       // MCL_OPT_STAR_ENABLE: 0 off, 1 every step, 2 the faster path
+    double path_ms[2] = {-1.0, -1.0};          // last step time of the flip-based (0) and star (1) paths
+    int path_since = 0;                        // steps since the slower path was last measured
+    int path_probe = kPathProbe;               // steps between such measurements
     int star_skip = 0;                      // steps left before the star step is tried again
     int star_backoff = 0;                   // length of the last such pause
     int* s_ecnt = nullptr;                  // nv + 1
@@ -1505,7 +1539,7 @@ struct mcl_tri {
     int* s_rlist = nullptr;                 // nv: vertices to re-check
     int* s_aux = nullptr;                   // kStarAux counters (see star_step)
     int* s_heavy = nullptr;                 // nv + 1: count, then rows for the full star kernel
-    int* c_bucket = nullptr;                // ncorner * kCornerBucket * 3: corner fan entries
+    int* c_bucket = nullptr;                // ncorner * kCornerCap * 3: corner fan entries
     int* c_count = nullptr;                 // ncorner
     // one ring
     int* deg = nullptr;
@@ -1625,11 +1659,13 @@ extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv,
     A(t->stuck, nt); A(t->touched, nv); A(t->r_tt, nv); A(t->r_tt2, nv); A(t->v_soft, nv); A(t->tail_out, 8);
     if (periodic || m.ncorner > 0) {
         A(t->star_deg[0], nv); A(t->star_deg[1], nv);
-        A(t->star_nbr[0], (size_t)nv * kStarCap); A(t->star_nbr[1], (size_t)nv * kStarCap);
-        A(t->star_tdeg, nv); A(t->star_tnbr, (size_t)nv * kStarCap);
+        // vertex rows, then the corners' fans (star_row); star_tnbr also holds
+        // the rebuild's half-edge ids in the same layout
+        A(t->star_nbr[0], star_rows_size(nv, m.ncorner)); A(t->star_nbr[1], star_rows_size(nv, m.ncorner));
+        A(t->star_tdeg, nv); A(t->star_tnbr, star_rows_size(nv, m.ncorner));
         A(t->s_ecnt, nv + 1); A(t->s_tcnt, nv + 1); A(t->s_mark, nv); A(t->s_list, nv); A(t->s_aux, kStarAux);
         A(t->s_rlist, nv); A(t->s_heavy, nv + 1);
-        if (m.ncorner > 0) { A(t->c_bucket, (size_t)m.ncorner * kCornerBucket * 3); A(t->c_count, m.ncorner); }
+        if (m.ncorner > 0) { A(t->c_bucket, (size_t)m.ncorner * kCornerCap * 3); A(t->c_count, m.ncorner); }
     }
 #undef A
     if (rc == MCL_OK && hipHostMalloc((void**)&t->h_cnt, sizeof(int) * kCnCount) != hipSuccess) rc = MCL_ERR_HIP;
@@ -1754,6 +1790,34 @@ __global__ void k_prep_round(int* cnt, int r3, int ncand) {
 // large, then one workgroup finishes the rest (no host round trips until
 // the end).  variant 1: one test + one flip launch per round with a host
 // read of the queue length in between (FP64-first predicates).
+static bool trace_on();
+static bool validate_on();
+
+// prints the first non-Delaunay edges with their quads (diagnostic)
+static void trace_nd_edges(mcl_tri* t, hipStream_t s) {
+    const DMesh& m = t->m;
+    int* d = nullptr;
+    if (hipMalloc(&d, sizeof(int) * 9) != hipSuccess) return;
+    int h[9] = {0};
+    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, d, 9, 0);
+    hipLaunchKernelGGL(k_list_nd, dim3(256), dim3(kThreads), 0, s, m, d);
+    if (hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s) == hipSuccess && hipStreamSynchronize(s) == hipSuccess) {
+        fprintf(stderr, "  [lawson cap] %d non-Delaunay edges\n", h[0]);
+        for (int i = 0; i < std::min(h[0], 8); i++) {
+            const int e = h[1 + i];
+            int v[2], o[2];
+            float2 p[4];
+            (void)hipMemcpy(v, m.he_v + 2 * e, sizeof(v), hipMemcpyDeviceToHost);
+            (void)hipMemcpy(o, m.he_op + 2 * e, sizeof(o), hipMemcpyDeviceToHost);
+            const int ids[4] = {v[0], o[1], v[1], o[0]};
+            for (int k = 0; k < 4; k++) (void)hipMemcpy(&p[k], m.pos + ids[k], sizeof(float2), hipMemcpyDeviceToHost);
+            fprintf(stderr, "    edge %d: %d (%.9g, %.9g) %d (%.9g, %.9g) %d (%.9g, %.9g) %d (%.9g, %.9g)\n", e, ids[0],
+                    p[0].x, p[0].y, ids[1], p[1].x, p[1].y, ids[2], p[2].x, p[2].y, ids[3], p[3].x, p[3].y);
+        }
+    }
+    (void)hipFree(d);
+}
+
 template <int kKind>
 static int run_rounds(mcl_tri* t, bool all, int variant, int max_rounds, hipStream_t s, long long* rounds_out) {
     DMesh& m = t->m;
@@ -1812,10 +1876,31 @@ static int run_rounds(mcl_tri* t, bool all, int variant, int max_rounds, hipStre
     // variant 0: full-grid rounds while the queue is large, then the
     // single-workgroup tail finishes the small remainder
     const int gb = t->grid_blocks;
+    // MCL_TRACE and MCL_VALIDATE set: every round on the full grid, each
+    // followed by an exact count of the triangles that are not strictly CCW
+    const bool watch = trace_on() && validate_on();
+    int* d_inv = nullptr;
+    if (watch && hipMalloc(&d_inv, sizeof(int)) != hipSuccess) d_inv = nullptr;
     for (int k = 0; r < cap; k++, r++) {
         if (k > 0) {
             MCL_TRY(read_counts(t, s));
-            if (t->h_cnt[kCnCand + t->round % 3] < t->tail_threshold) break;
+            const int left = t->h_cnt[kCnCand + t->round % 3];
+            // no flip wanted last round: what is queued needs the big exact
+            // stage, which only the single-workgroup tail evaluates
+            if (t->h_cnt[kCnWants + (t->round - 1) % 3] == 0 && left > 0) break;
+            if (d_inv) {
+                int inv = 0;
+                hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, d_inv, 1, 0);
+                hipLaunchKernelGGL(k_count_inv, dim3(gb), dim3(kThreads), 0, s, m, d_inv);
+                MCL_HIP(hipMemcpyAsync(&inv, d_inv, sizeof(int), hipMemcpyDeviceToHost, s));
+                MCL_HIP(hipStreamSynchronize(s));
+                if (inv || k < 4 || k % 256 == 0)
+                    fprintf(stderr, "  [lawson round %d] queue %d, triangles not strictly CCW %d\n", k, left, inv);
+                if (inv) trace_nd_edges(t, s);
+                if (inv || left == 0) break;
+            } else if (left < t->tail_threshold) {
+                break;
+            }
         }
         const int R = t->round;
         int* cand = t->cand[R & 1];
@@ -1836,6 +1921,7 @@ static int run_rounds(mcl_tri* t, bool all, int variant, int max_rounds, hipStre
         done++;
         first = false;
     }
+    if (d_inv) (void)hipFree(d_inv);
     const int R = t->round;
     const int tail_cap = cap - r;
     if (tail_cap > 0) {
@@ -1849,7 +1935,10 @@ static int run_rounds(mcl_tri* t, bool all, int variant, int max_rounds, hipStre
     t->round = R + tail_rounds + 1;  // +1 keeps claim keys of the next call above the tail's
     done += tail_rounds;
     if (rounds_out) *rounds_out = done;
-    if (t->h_cnt[kCnCand + (R + tail_rounds) % 3] != 0) return MCL_ERR_ROUND_CAP;
+    if (t->h_cnt[kCnCand + (R + tail_rounds) % 3] != 0) {
+        if (trace_on()) trace_nd_edges(t, s);
+        return MCL_ERR_ROUND_CAP;
+    }
     return MCL_OK;
 }
 
@@ -2017,6 +2106,14 @@ static int validate(mcl_tri* t, hipStream_t s, const char* phase) {
     if (e) {
         fprintf(stderr, "mcleap: mesh structure broken after %s (%d violations)\n", phase, e);
         return MCL_ERR_TOPOLOGY;
+    }
+    if (trace_on()) {
+        hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->tail_out + 7, 1, 0);
+        hipLaunchKernelGGL(k_count_inv, dim3(t->grid_blocks), dim3(kThreads), 0, s, t->m, t->tail_out + 7);
+        MCL_HIP(hipGetLastError());
+        MCL_HIP(hipMemcpyAsync(&e, t->tail_out + 7, sizeof(int), hipMemcpyDeviceToHost, s));
+        MCL_HIP(hipStreamSynchronize(s));
+        fprintf(stderr, "  [validate] after %s: %d triangles not strictly CCW\n", phase, e);
     }
     return MCL_OK;
 }
@@ -2283,7 +2380,7 @@ static int ensure_stars(mcl_tri* t, hipStream_t s) {
     int fail = 0;
     MCL_HIP(hipMemcpyAsync(&fail, t->s_aux, sizeof(int), hipMemcpyDeviceToHost, s));
     MCL_HIP(hipStreamSynchronize(s));
-    if (fail) return MCL_ERR_CAPACITY;  // a degree above kStarCap
+    if (fail) return MCL_ERR_CAPACITY;  // a degree above kStarCap (kCornerCap at a corner)
     t->stars_valid = true;
     return MCL_OK;
 }
@@ -2402,7 +2499,7 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(m.ncorner)), dim3(256), 0, s, t->c_count, m.ncorner, 0);
         hipLaunchKernelGGL(k_corner_collect, dim3(blocks_for(m.nv)), dim3(256), 0, s, m.nv, m.ncorner,
                            (const int*)t->star_deg[nxt], (const int*)t->star_nbr[nxt], t->c_bucket, t->c_count);
-        hipLaunchKernelGGL(k_corner_fans, dim3((m.ncorner + 63) / 64), dim3(64), 0, s, m.ncorner, (const int*)t->c_bucket,
+        hipLaunchKernelGGL(k_corner_fans, dim3((m.ncorner + 63) / 64), dim3(64), 0, s, m.nv, m.ncorner, (const int*)t->c_bucket,
                            (const int*)t->c_count, (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur],
                            t->star_deg[nxt], t->star_nbr[nxt], t->s_aux + 18,
                            (long long*)(t->s_aux + 16));
@@ -2458,6 +2555,22 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
             }
         }
     }
+    if (!settled && trace && aux[1] > 0) {
+        // the first vertices still listed: id, position, star degree (-1: failed)
+        const int k = std::min(aux[1], 12);
+        int ids[12];
+        MCL_HIP(hipMemcpyAsync(ids, t->s_list, sizeof(int) * k, hipMemcpyDeviceToHost, s));
+        MCL_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < k; i++) {
+            float2 q;
+            int d = 0;
+            if (ids[i] < 0 || ids[i] >= m.nv) continue;
+            MCL_HIP(hipMemcpyAsync(&q, m.pos + ids[i], sizeof(q), hipMemcpyDeviceToHost, s));
+            MCL_HIP(hipMemcpyAsync(&d, t->star_deg[nxt] + ids[i], sizeof(d), hipMemcpyDeviceToHost, s));
+            MCL_HIP(hipStreamSynchronize(s));
+            fprintf(stderr, "  [stars] listed %d at (%.9g, %.9g) degree %d\n", ids[i], q.x, q.y, d);
+        }
+    }
     if (!settled) {
         // back to the start of the step; the mesh and its stars are untouched
         MCL_HIP(hipMemcpyAsync(m.pos, t->acc, sizeof(float2) * m.nv, hipMemcpyDeviceToDevice, s));
@@ -2481,44 +2594,102 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     return MCL_OK;
 }
 
+// The star step; false in *done when it did not settle (the flip-based step
+// then ran instead, st->fallbacks = 1).
+static int star_or_fallback(mcl_tri* t, const mcl_step_params_t* p, const float2* disp, mcl_stats_t* st, void* stream,
+                            bool* done) {
+    hipStream_t s = (hipStream_t)stream;
+    auto t0 = std::chrono::steady_clock::now();
+    stats_begin(t, s);
+    mcl_stats_t tmp{};
+    *done = false;
+    int rc = star_step(t, p, disp, &tmp, s, done);
+    if (rc != MCL_OK) return rc;
+    if (*done) {
+        MCL_TRY(stats_end(t, s, st, t0));
+        if (st) {
+            st->iterations = tmp.iterations;
+            st->star_retries = tmp.star_retries;
+        }
+        return MCL_OK;
+    }
+    MCL_TRY(ensure_mesh(t, s));
+    rc = step_impl(t, p, disp, st, stream);
+    if (st) st->fallbacks = 1;
+    return rc;
+}
+
+// Path choice.  MCL_OPT_STAR_ENABLE = 2 (default): each step takes the path
+// whose last step was faster (host clock over the whole step, a star step
+// that fell back counting its attempt and the flip-based step after it); the
+// other path is measured again after kPathProbe x (slow / fast) steps, so
+// that probing costs at most 1 / kPathProbe of the run, and twice as late
+// each time the measurement keeps the choice (up to 4096 steps); a change in
+// the motion still moves the choice.  = 1: the star step every step, except that after
+// a fallback it is not tried for 1, 2, 4, ... 64 steps.  Meshes below
+// MCL_OPT_STAR_MIN_VERTICES (where the number of launches dominates) take the
+// flip-based step.
 static int kinetic_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp, mcl_stats_t* st, void* stream) {
     hipStream_t s = (hipStream_t)stream;
-    // the star step, unless it fell back recently: after a fallback it is not
-    // tried for 1, 2, 4, ... 64 steps (workloads whose motion it cannot follow
-    // then cost the flip-based step only)
-    // (small meshes: the flip-based step needs fewer launches)
-    const bool try_star = (t->m.periodic || t->m.ncorner > 0) && p->variant == 0 && t->star_deg[0] &&
-                          t->star_enable && t->m.nv >= t->star_min_vertices;
-    if (try_star && t->star_skip > 0) {
+    const bool star_ok = (t->m.periodic || t->m.ncorner > 0) && p->variant == 0 && t->star_deg[0] &&
+                         t->star_enable != 0 && t->m.nv >= t->star_min_vertices;
+    if (!star_ok) {
+        MCL_TRY(ensure_mesh(t, s));
+        return step_impl(t, p, disp, st, stream);
+    }
+    if (t->star_enable == 2) {
+        int path;
+        bool probe = false;
+        if (t->path_ms[1] < 0.0) path = 1;
+        else if (t->path_ms[0] < 0.0) path = 0;
+        else {
+            path = t->path_ms[1] <= t->path_ms[0] ? 1 : 0;
+            if (++t->path_since >= t->path_probe) {
+                path = 1 - path;
+                probe = true;
+                t->path_since = 0;
+            }
+        }
+        MCL_HIP(hipStreamSynchronize(s));
+        auto t0 = std::chrono::steady_clock::now();
+        int rc;
+        if (path == 1) {
+            bool done = false;
+            rc = star_or_fallback(t, p, disp, st, stream, &done);
+        } else {
+            MCL_TRY(ensure_mesh(t, s));
+            rc = step_impl(t, p, disp, st, stream);
+        }
+        if (rc != MCL_OK) return rc;
+        MCL_HIP(hipStreamSynchronize(s));
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (probe) {  // the probed path stays slower: probe half as often
+            const bool keeps = ms > t->path_ms[1 - path];
+            t->path_probe = keeps ? std::min(2 * t->path_probe, 4096) : kPathProbe;
+        }
+        t->path_ms[path] = ms;
+        if (t->path_ms[0] > 0.0 && t->path_ms[1] > 0.0) {
+            // a probe costs (slow - fast); after kPathProbe * slow / fast steps
+            // of the fast path that is at most 1 / kPathProbe of the time
+            const double r = std::max(t->path_ms[0], t->path_ms[1]) / std::min(t->path_ms[0], t->path_ms[1]);
+            t->path_probe = std::max(t->path_probe, (int)std::min(4096.0, kPathProbe * r));
+        }
+        return MCL_OK;
+    }
+    if (t->star_skip > 0) {
         t->star_skip--;
         MCL_TRY(ensure_mesh(t, s));
         return step_impl(t, p, disp, st, stream);
     }
-    if (try_star) {
-        auto t0 = std::chrono::steady_clock::now();
-        stats_begin(t, s);
-        bool done = false;
-        mcl_stats_t tmp{};
-        int rc = star_step(t, p, disp, &tmp, s, &done);
-        if (rc != MCL_OK) return rc;
-        if (done) {
-            t->star_backoff = 0;
-            MCL_TRY(stats_end(t, s, st, t0));
-            if (st) {
-                st->iterations = tmp.iterations;
-                st->star_retries = tmp.star_retries;
-            }
-            return MCL_OK;
-        }
+    bool done = false;
+    MCL_TRY(star_or_fallback(t, p, disp, st, stream, &done));
+    if (done) {
+        t->star_backoff = 0;
+    } else {
         t->star_backoff = t->star_backoff ? std::min(2 * t->star_backoff, 64) : 1;
         t->star_skip = t->star_backoff;
-        MCL_TRY(ensure_mesh(t, s));
-        rc = step_impl(t, p, disp, st, stream);
-        if (st) st->fallbacks = 1;
-        return rc;
     }
-    MCL_TRY(ensure_mesh(t, s));
-    return step_impl(t, p, disp, st, stream);
+    return MCL_OK;
 }
 
 extern "C" int mcl_tri_step(mcl_tri_t* t, const mcl_step_params_t* p, mcl_stats_t* st, void* stream) {
@@ -2559,7 +2730,7 @@ extern "C" int mcl_tri_one_ring_csr(mcl_tri_t* t, void* d_offsets, void* d_nbrs,
     MCL_TRY(scan_exclusive(off, off, t->m.nv, t->scan_tmp, s));
     auto fill = [&]() {
         if (from_stars)
-            hipLaunchKernelGGL(k_star_ring_csr, dim3(blocks_for(t->m.nv)), dim3(256), 0, s, t->m.nv,
+            hipLaunchKernelGGL(k_star_ring_csr, dim3(blocks_for(t->m.nv)), dim3(256), 0, s, t->m.nv, t->m.ncorner,
                                (const int*)t->star_deg[c], (const int*)t->star_nbr[c], (const int*)off, (int*)d_nbrs, cap);
         else
             hipLaunchKernelGGL(k_ring_csr, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, off, (int*)d_nbrs, cap);
@@ -2625,7 +2796,12 @@ extern "C" int mcl_tri_set_option(mcl_tri_t* t, int option, long long value) {
             t->star_min_vertices = (int)std::min<long long>(value, INT32_MAX);
             return MCL_OK;
         case MCL_OPT_STAR_ENABLE:
-            t->star_enable = value != 0;
+            if (value < 0 || value > 2) return MCL_ERR_INPUT;
+            t->star_enable = (int)value;
+            t->path_ms[0] = t->path_ms[1] = -1.0;
+            t->path_since = 0;
+            t->path_probe = kPathProbe;
+            t->star_skip = t->star_backoff = 0;
             return MCL_OK;
         default:
             return MCL_ERR_INPUT;

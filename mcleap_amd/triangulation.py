@@ -450,3 +450,79 @@ class GridCount2d:
         out = ctypes.c_float()
         check(lib().mcl_grid_mean_occupancy(self._g, ctypes.byref(out), None), "mcl_grid_mean_occupancy")
         return out.value
+
+
+class GridCount2dBatch:
+    """One uniform grid per instance of an ensemble, sorted and queried in
+    single launches (each instance of the reference would hold its own
+    GridCount2d, gridCount2d.h:23-48).  vstart: the ensemble's ninst + 1
+    point offsets (HostTriangulationBatch.vstart); radius: one query radius
+    per instance (cells of max(3, floor(1/r_i)) per side)."""
+
+    def __init__(self, vstart, radius):
+        vstart = np.ascontiguousarray(vstart, np.int32)
+        radius = np.ascontiguousarray(radius, np.float32)
+        self.ninst = len(vstart) - 1
+        if len(radius) != self.ninst:
+            raise ValueError("one radius per instance")
+        self._g = _vp()
+        check(lib().mcl_bgrid_create(self.ninst, _np_ptr(vstart), _np_ptr(radius), ctypes.byref(self._g)),
+              "mcl_bgrid_create")
+        n, first, cells = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().mcl_bgrid_sizes(self._g, ctypes.byref(n), ctypes.byref(first), ctypes.byref(cells)),
+              "mcl_bgrid_sizes")
+        self.numP, self.first, self.ncells = n.value, first.value, cells.value
+
+    def __del__(self):
+        if getattr(self, "_g", None) and self._g.value:
+            try:
+                lib().mcl_bgrid_destroy(self._g)
+            except Exception:
+                pass
+            self._g = _vp()
+
+    def build(self, pos):
+        """counting sort of every instance's points (pos: the ensemble's device positions; async)"""
+        check(lib().mcl_bgrid_build(self._g, _dev_ptr(pos), None), "mcl_bgrid_build")
+
+    def sorted_ptr(self):
+        return lib().mcl_bgrid_sorted_ptr(self._g)
+
+    def sorted_pos(self):
+        import torch
+        return _tensor_from_ptr(self.sorted_ptr(), (self.numP, 2), torch.float32)
+
+    def index_ptrs(self):
+        ind, perm, start = _vp(), _vp(), _vp()
+        check(lib().mcl_bgrid_index_ptrs(self._g, ctypes.byref(ind), ctypes.byref(perm), ctypes.byref(start)),
+              "mcl_bgrid_index_ptrs")
+        return ind.value, perm.value, start.value
+
+    def perm(self):
+        """point (vertex id - first) of each sorted slot, a torch int32 copy"""
+        import torch
+        _, perm, _ = self.index_ptrs()
+        return _tensor_from_ptr(perm, (self.numP,), torch.int32).clone()
+
+    def frnn_csr(self, offsets, nbrs=None):
+        """pairs within each instance's radius, CSR over sorted slots; returns the pair count"""
+        tot = ctypes.c_longlong()
+        cap = 0 if nbrs is None else int(nbrs.numel())
+        rc = lib().mcl_bgrid_frnn_csr(self._g, _dev_ptr(offsets), _dev_ptr(nbrs), cap, ctypes.byref(tot), None)
+        if nbrs is not None:
+            check(rc, "mcl_bgrid_frnn_csr")
+        return tot.value
+
+    def frnn_csr_async(self, offsets, nbrs):
+        check(lib().mcl_bgrid_frnn_csr(self._g, _dev_ptr(offsets), _dev_ptr(nbrs), int(nbrs.numel()), None, None),
+              "mcl_bgrid_frnn_csr")
+
+    def kernel_timer(self, enable=True):
+        check(lib().mcl_bgrid_kernel_timer(self._g, int(enable)), "mcl_bgrid_kernel_timer")
+
+    def kernel_timer_read(self, which):
+        """(total GPU ms, launches) of kernel class `which` (0 FRNN count, 1 FRNN fill, 2 build)"""
+        ms, n = ctypes.c_double(), ctypes.c_longlong()
+        check(lib().mcl_bgrid_kernel_timer_read(self._g, int(which), ctypes.byref(ms), ctypes.byref(n)),
+              "mcl_bgrid_kernel_timer_read")
+        return ms.value, n.value

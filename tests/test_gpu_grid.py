@@ -145,3 +145,57 @@ def test_apply_f_frnn_includes_self_pairs(oracle):
     g.apply_f_frnn(f, pos, 0.05)
     assert np.sum(seen["i"] == seen["j"]) == len(xy)
     assert np.all(seen["dist"] <= 0.05 + 1e-7)
+
+
+def _brute_sets(xy, rad):
+    """pairs with dx*dx + dy*dy <= rad*rad in FP32, separately rounded"""
+    xy = np.asarray(xy, np.float32)
+    r2 = np.float32(rad) * np.float32(rad)
+    out = {}
+    for i in range(len(xy)):
+        d = xy - xy[i]
+        q = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]
+        js = np.nonzero(q <= r2)[0]
+        out[i] = sorted(int(j) for j in js if j != i)
+    return out
+
+
+def test_batch_grid_frnn(oracle):
+    """GridCount2dBatch: every instance's neighbour sets (its own radius
+    2/sqrt(n_i), its own cells) equal brute force on that instance alone;
+    no pair crosses instances (all instances share [0,1)^2)."""
+    import torch
+
+    from mcleap_amd.workloads import ensemble_points
+    sizes = [1, 2, 40, 300, 1000, 2500, 700, 5000, 64, 3]
+    pts = ensemble_points(seed=21, sizes=sizes)
+    B = M.HostTriangulationBatch(pts)
+    rad = np.array([2.0 / np.sqrt(max(len(p), 1)) for p in pts], np.float32)
+    rad = np.minimum(rad, np.float32(1.0 / 3.0))
+    g = M.GridCount2dBatch(B.vstart, rad)
+    assert g.numP == len(B.m_pos) - 4 * B.ninst and g.first == 4 * B.ninst
+    pos = torch.from_numpy(B.m_pos).cuda()
+    g.build(pos)
+    n = g.numP
+    off = torch.zeros(n + 1, dtype=torch.int32, device="cuda")
+    tot = g.frnn_csr(off)
+    nbr = torch.zeros(max(tot, 1), dtype=torch.int32, device="cuda")
+    assert g.frnn_csr(off, nbr) == tot
+    off, nbr = off.cpu().numpy(), nbr.cpu().numpy()[:tot]
+    perm = g.perm().cpu().numpy()
+    spos = g.sorted_pos().cpu().numpy()
+    assert np.array_equal(spos, B.m_pos[g.first + perm])
+    got = {}
+    for s in range(n):
+        got[int(perm[s]) + g.first] = sorted(int(perm[j]) + g.first for j in nbr[off[s]:off[s + 1]])
+    for i in range(B.ninst):
+        vs = np.arange(B.vstart[i], B.vstart[i + 1])
+        want = _brute_sets(B.m_pos[vs], rad[i])
+        for k, v in enumerate(vs):
+            assert got[int(v)] == [int(vs[j]) for j in want[k]], (i, k)
+    # the async form gives the same CSR
+    off2 = torch.zeros(n + 1, dtype=torch.int32, device="cuda")
+    nbr2 = torch.zeros(max(tot, 1), dtype=torch.int32, device="cuda")
+    g.frnn_csr_async(off2, nbr2)
+    torch.cuda.synchronize()
+    assert np.array_equal(off2.cpu().numpy(), off) and np.array_equal(nbr2.cpu().numpy()[:tot], nbr)

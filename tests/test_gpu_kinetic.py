@@ -205,3 +205,25 @@ def test_untangle_api(oracle):
     pos, v, he, t = dt.download()
     rc, msg = O.check_mesh(pos, v, he, t, periodic=False, require_ccw=False)
     assert rc == 0, msg
+
+
+def test_path_choice_auto(oracle):
+    """MCL_OPT_STAR_ENABLE = 2 (the default): the first step takes the star
+    path, the second the flip-based path (both timed), later steps the one
+    that was faster; every step ends on the exact Delaunay triangulation."""
+    O = oracle
+    xy = uniform_points(20000, 31)
+    H = M.HostTriangulation(periodic=True)
+    H.addDelaunayPoints(xy)
+    dt = M.DeviceTriangulation(H)
+    dt.set_option(M.OPT_STAR_MIN_VERTICES, 0)
+    paths = []
+    for k in range(5):
+        st = dt.step(M.MOTION_SHEAR, 2e-4, amp=1e-3, seed=9, step=k)
+        paths.append("star" if st["rounds"] == 0 and st["fallbacks"] == 0 else "flip")
+        _check_torus(O, dt, "auto step %d" % k, full=(k == 4))
+    print(paths)
+    assert paths[0] == "star" and paths[1] == "flip"
+    assert len(set(paths[2:])) == 1  # the faster path, kept
+    with pytest.raises(M.MclError):
+        dt.set_option(M.OPT_STAR_ENABLE, 3)

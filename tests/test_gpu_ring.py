@@ -95,6 +95,7 @@ def test_csr_ring_after_star_steps(oracle):
     import torch
     dt = _mesh(20000, 23, True)
     dt.set_option(M.OPT_STAR_MIN_VERTICES, 0)
+    dt.set_option(M.OPT_STAR_ENABLE, 1)
     for k in range(2):
         st = dt.step(M.MOTION_SHEAR, 2e-4, amp=1e-3, seed=5, step=k)
         assert st["fallbacks"] == 0
