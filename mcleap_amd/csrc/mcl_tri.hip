@@ -1483,6 +1483,8 @@ __global__ void __launch_bounds__(kThreads) k_frnn_graph(DMesh m, float rr, int 
 // host side
 // =========================================================================
 constexpr int kPathProbe = 64;  // first interval between measurements of the slower step path
+constexpr int kBigBlocks = 256;   // workgroups of the rounds that run the big exact stage
+constexpr int kBigGridMin = 2048;  // queue length from which those rounds run on kBigBlocks workgroups
 
 struct mcl_tri {
     DMesh m;
@@ -1886,7 +1888,7 @@ static int run_rounds(mcl_tri* t, bool all, int variant, int max_rounds, hipStre
             MCL_TRY(read_counts(t, s));
             const int left = t->h_cnt[kCnCand + t->round % 3];
             // no flip wanted last round: what is queued needs the big exact
-            // stage, which only the single-workgroup tail evaluates
+            // stage (below)
             if (t->h_cnt[kCnWants + (t->round - 1) % 3] == 0 && left > 0) break;
             if (d_inv) {
                 int inv = 0;
@@ -1922,6 +1924,24 @@ static int run_rounds(mcl_tri* t, bool all, int variant, int max_rounds, hipStre
         first = false;
     }
     if (d_inv) (void)hipFree(d_inv);
+    // a long queue of cases for the big exact stage (box meshes with many
+    // points on the sides, next to the corners): rounds on kBigBlocks
+    // workgroups, whose per-lane private arrays bound how many can run; the
+    // single-workgroup tail takes the rest
+    for (; r < cap; r++) {
+        MCL_TRY(read_counts(t, s));
+        if (t->h_cnt[kCnCand + t->round % 3] < kBigGridMin) break;
+        const int R = t->round;
+        const int stat = kKind == 0 ? kStFlips : kStUntangleFlips;
+        hipLaunchKernelGGL((k_round_test<kKind, false, true, true>), dim3(kBigBlocks), dim3(kThreads), 0, s, m,
+                           t->cand[R & 1], t->cnt, R, 0, t->wants, t->claim, t->e_stamp, t->cand[(R + 1) & 1],
+                           t->stats);
+        hipLaunchKernelGGL(k_round_flip, dim3(kBigBlocks), dim3(kThreads), 0, s, m, t->cnt, R, t->wants, t->claim,
+                           t->e_stamp, t->cand[(R + 1) & 1], t->stats, stat);
+        MCL_HIP(hipGetLastError());
+        t->round++;
+        done++;
+    }
     const int R = t->round;
     const int tail_cap = cap - r;
     if (tail_cap > 0) {
@@ -1930,7 +1950,7 @@ static int run_rounds(mcl_tri* t, bool all, int variant, int max_rounds, hipStre
     }
     MCL_HIP(hipGetLastError());
     int tail_rounds = 0;
-    MCL_HIP(hipMemcpyAsync(&tail_rounds, t->rounds_done, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (tail_cap > 0) MCL_HIP(hipMemcpyAsync(&tail_rounds, t->rounds_done, sizeof(int), hipMemcpyDeviceToHost, s));
     MCL_TRY(read_counts(t, s));
     t->round = R + tail_rounds + 1;  // +1 keeps claim keys of the next call above the tail's
     done += tail_rounds;
