@@ -129,6 +129,26 @@ int mcl_tri_destroy(mcl_tri_t* t);
 int mcl_tri_sizes(const mcl_tri_t* t, int* npos, int* nv, int* nhe, int* nt);
 /* transferToHost (Device_Triangulation.h:245-253): SoA -> AoS; any may be NULL */
 int mcl_tri_download(mcl_tri_t* t, float* pos, int* v2, int* he4, int* t1);
+/* 1 for a torus triangulation, 0 for a box (or ensemble of boxes) */
+int mcl_tri_is_periodic(const mcl_tri_t* t, int* periodic);
+
+/* ------------------------------------------------------------ state files
+ * Save / resume (no counterpart in the reference, whose state lives only in
+ * the host object's vectors, Host_Triangulation.h:27-30).  The file holds
+ * exactly those AoS arrays (pos, v2, he4, t1) after a 40-byte header
+ * ("MCLTRI01", version 1, periodic, npos, nv, nhe, nt, 2 reserved int32) and
+ * ends with an FNV-1a 64 checksum of the arrays; little-endian.  Loading a
+ * file that is truncated, corrupted or not such a file gives MCL_ERR_INPUT.
+ * Steps resumed from a saved state reproduce the uninterrupted run: the
+ * motion is a function of (seed, step, vertex). */
+int mcl_state_save(const char* path, int periodic, const float* pos, int npos, const int* v2, int nv,
+                   const int* he4, int nhe, const int* t1, int nt);
+int mcl_state_info(const char* path, int* periodic, int* npos, int* nv, int* nhe, int* nt);
+/* any array may be NULL (skipped, still checksummed) */
+int mcl_state_load(const char* path, float* pos, int* v2, int* he4, int* t1);
+/* transferToHost + mcl_state_save; mcl_state_load + mcl_tri_create */
+int mcl_tri_save(mcl_tri_t* t, const char* path);
+int mcl_tri_load(const char* path, mcl_tri_t** out);
 /* transferToDevice (Device_Triangulation.h:255-263) */
 int mcl_tri_upload(mcl_tri_t* t, const float* pos, const int* v2, const int* he4, const int* t1);
 /* public m_pos (device glm::vec2*, Device_Triangulation.h:139): float2 device pointer */

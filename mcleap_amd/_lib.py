@@ -130,6 +130,12 @@ SIGNATURES = {
     "mcl_bgrid_frnn_csr": [_p, _p, _p, _i, _p, _p],
     "mcl_bgrid_kernel_timer": [_p, _i],
     "mcl_bgrid_kernel_timer_read": [_p, _i, _p, _p],
+    "mcl_tri_is_periodic": [_p, _p],
+    "mcl_state_save": [ctypes.c_char_p, _i, _p, _i, _p, _i, _p, _i, _p, _i],
+    "mcl_state_info": [ctypes.c_char_p, _p, _p, _p, _p, _p],
+    "mcl_state_load": [ctypes.c_char_p, _p, _p, _p, _p],
+    "mcl_tri_save": [_p, ctypes.c_char_p],
+    "mcl_tri_load": [ctypes.c_char_p, _pp],
     "mcl_version": [],
     "mcl_device_info": [_p, _i, _p],
 }

@@ -1,6 +1,7 @@
 // mcl_api.hip -- host-mesh construction entry points and library info.
 #include <algorithm>
 #include <atomic>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -167,4 +168,131 @@ extern "C" int mcl_device_info(char* name, int len, int* cu_count) {
     }
     if (cu_count) *cu_count = p.multiProcessorCount;
     return MCL_OK;
+}
+
+// ------------------------------------------------------------ state files
+// A triangulation on disk: the AoS arrays of the reference's host object
+// (Host_Triangulation.h:27-30, as transferToHost fills them) after a 48-byte
+// header, then an FNV-1a 64 checksum of the arrays.
+//   char magic[8] "MCLTRI01"; int32 version 1, periodic, npos, nv, nhe, nt,
+//   two reserved zeros; float pos[2 npos]; int32 v2[2 nv]; int32 he4[4 nhe];
+//   int32 t1[nt]; uint64 checksum.  Little-endian.
+namespace {
+constexpr char kMagic[8] = {'M', 'C', 'L', 'T', 'R', 'I', '0', '1'};
+struct StateHeader {
+    char magic[8];
+    int32_t version, periodic, npos, nv, nhe, nt, reserved[2];
+};
+static_assert(sizeof(StateHeader) == 40, "header layout");
+void fnv(uint64_t& h, const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; i++) {
+        h ^= b[i];
+        h *= 1099511628211ull;
+    }
+}
+bool read_header(FILE* f, StateHeader& hd) {
+    if (fread(&hd, sizeof(hd), 1, f) != 1) return false;
+    if (memcmp(hd.magic, kMagic, 8) != 0 || hd.version != 1) return false;
+    if (hd.npos < 0 || hd.nv < 0 || hd.nhe < 0 || hd.nt < 0 || (hd.nhe & 1)) return false;
+    return true;
+}
+}  // namespace
+
+extern "C" int mcl_state_save(const char* path, int periodic, const float* pos, int npos, const int* v2, int nv,
+                              const int* he4, int nhe, const int* t1, int nt) {
+    if (!path || npos < 0 || nv < 0 || nhe < 0 || nt < 0 || (npos && !pos) || (nv && !v2) || (nhe && !he4) ||
+        (nt && !t1))
+        return MCL_ERR_INPUT;
+    FILE* f = fopen(path, "wb");
+    if (!f) return MCL_ERR_INPUT;
+    StateHeader hd;
+    memcpy(hd.magic, kMagic, 8);
+    hd.version = 1;
+    hd.periodic = periodic ? 1 : 0;
+    hd.npos = npos; hd.nv = nv; hd.nhe = nhe; hd.nt = nt;
+    hd.reserved[0] = hd.reserved[1] = 0;
+    uint64_t h = 14695981039346656037ull;
+    const struct { const void* p; size_t n; } parts[4] = {
+        {pos, sizeof(float) * 2 * (size_t)npos}, {v2, sizeof(int) * 2 * (size_t)nv},
+        {he4, sizeof(int) * 4 * (size_t)nhe}, {t1, sizeof(int) * (size_t)nt}};
+    bool ok = fwrite(&hd, sizeof(hd), 1, f) == 1;
+    for (const auto& q : parts) {
+        if (!ok || q.n == 0) continue;
+        ok = fwrite(q.p, 1, q.n, f) == q.n;
+        fnv(h, q.p, q.n);
+    }
+    ok = ok && fwrite(&h, sizeof(h), 1, f) == 1;
+    ok = (fclose(f) == 0) && ok;
+    return ok ? MCL_OK : MCL_ERR_INPUT;
+}
+
+extern "C" int mcl_state_info(const char* path, int* periodic, int* npos, int* nv, int* nhe, int* nt) {
+    if (!path) return MCL_ERR_INPUT;
+    FILE* f = fopen(path, "rb");
+    if (!f) return MCL_ERR_INPUT;
+    StateHeader hd;
+    const bool ok = read_header(f, hd);
+    fclose(f);
+    if (!ok) return MCL_ERR_INPUT;
+    if (periodic) *periodic = hd.periodic;
+    if (npos) *npos = hd.npos;
+    if (nv) *nv = hd.nv;
+    if (nhe) *nhe = hd.nhe;
+    if (nt) *nt = hd.nt;
+    return MCL_OK;
+}
+
+extern "C" int mcl_state_load(const char* path, float* pos, int* v2, int* he4, int* t1) {
+    if (!path) return MCL_ERR_INPUT;
+    FILE* f = fopen(path, "rb");
+    if (!f) return MCL_ERR_INPUT;
+    StateHeader hd;
+    bool ok = read_header(f, hd);
+    uint64_t h = 14695981039346656037ull, stored = 0;
+    if (ok) {
+        const struct { void* p; size_t n; } parts[4] = {
+            {pos, sizeof(float) * 2 * (size_t)hd.npos}, {v2, sizeof(int) * 2 * (size_t)hd.nv},
+            {he4, sizeof(int) * 4 * (size_t)hd.nhe}, {t1, sizeof(int) * (size_t)hd.nt}};
+        std::vector<unsigned char> tmp;
+        for (const auto& q : parts) {
+            if (!ok || q.n == 0) continue;
+            void* dst = q.p;
+            if (!dst) {  // not wanted: read into scratch for the checksum
+                tmp.resize(q.n);
+                dst = tmp.data();
+            }
+            ok = fread(dst, 1, q.n, f) == q.n;
+            if (ok) fnv(h, dst, q.n);
+        }
+        ok = ok && fread(&stored, sizeof(stored), 1, f) == 1 && stored == h && fgetc(f) == EOF;
+    }
+    fclose(f);
+    return ok ? MCL_OK : MCL_ERR_INPUT;
+}
+
+extern "C" int mcl_tri_save(mcl_tri_t* t, const char* path) {
+    if (!t || !path) return MCL_ERR_INPUT;
+    int npos = 0, nv = 0, nhe = 0, nt = 0, periodic = 0;
+    int rc = mcl_tri_sizes(t, &npos, &nv, &nhe, &nt);
+    if (rc != MCL_OK) return rc;
+    rc = mcl_tri_is_periodic(t, &periodic);
+    if (rc != MCL_OK) return rc;
+    std::vector<float> pos(2 * (size_t)npos);
+    std::vector<int> v2(2 * (size_t)nv), he4(4 * (size_t)nhe), t1((size_t)nt);
+    rc = mcl_tri_download(t, pos.data(), v2.data(), he4.data(), t1.data());
+    if (rc != MCL_OK) return rc;
+    return mcl_state_save(path, periodic, pos.data(), npos, v2.data(), nv, he4.data(), nhe, t1.data(), nt);
+}
+
+extern "C" int mcl_tri_load(const char* path, mcl_tri_t** out) {
+    if (!path || !out) return MCL_ERR_INPUT;
+    int periodic = 0, npos = 0, nv = 0, nhe = 0, nt = 0;
+    int rc = mcl_state_info(path, &periodic, &npos, &nv, &nhe, &nt);
+    if (rc != MCL_OK) return rc;
+    std::vector<float> pos(2 * (size_t)npos);
+    std::vector<int> v2(2 * (size_t)nv), he4(4 * (size_t)nhe), t1((size_t)nt);
+    rc = mcl_state_load(path, pos.data(), v2.data(), he4.data(), t1.data());
+    if (rc != MCL_OK) return rc;
+    return mcl_tri_create(pos.data(), npos, v2.data(), nv, he4.data(), nhe, t1.data(), nt, periodic, out);
 }

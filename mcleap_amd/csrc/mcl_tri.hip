@@ -1724,6 +1724,12 @@ extern "C" int mcl_tri_upload(mcl_tri_t* t, const float* pos, const int* v2, con
     return upload_aos(t, pos, v2, he4, t1);
 }
 
+extern "C" int mcl_tri_is_periodic(const mcl_tri_t* t, int* periodic) {
+    if (!t || !periodic) return MCL_ERR_INPUT;
+    *periodic = t->m.periodic;
+    return MCL_OK;
+}
+
 extern "C" int mcl_tri_download(mcl_tri_t* t, float* pos, int* v2, int* he4, int* t1) {
     if (!t) return MCL_ERR_INPUT;
     DMesh& m = t->m;

@@ -94,6 +94,35 @@ class HostTriangulation:
     def num_points(self):
         return len(self._points)
 
+    def save_state(self, path):
+        """writes m_pos, m_v, m_he, m_t to a state file (mcl_state_save)"""
+        check(lib().mcl_state_save(str(path).encode(), int(self.periodic),
+                                   _np_ptr(np.ascontiguousarray(self.m_pos, np.float32)), len(self.m_pos),
+                                   _np_ptr(np.ascontiguousarray(self.m_v, np.int32)), len(self.m_v),
+                                   _np_ptr(np.ascontiguousarray(self.m_he, np.int32)), len(self.m_he),
+                                   _np_ptr(np.ascontiguousarray(self.m_t, np.int32)), len(self.m_t)),
+              "mcl_state_save")
+
+    @classmethod
+    def load_state(cls, path):
+        """a host triangulation holding a state file's arrays (mcl_state_load);
+        DeviceTriangulation(it) resumes on the GPU"""
+        L = lib()
+        q = [ctypes.c_int() for _ in range(5)]
+        check(L.mcl_state_info(str(path).encode(), *[ctypes.byref(x) for x in q]), "mcl_state_info")
+        periodic, npos, nv, nhe, nt = (x.value for x in q)
+        h = cls.__new__(cls)
+        h.periodic = bool(periodic)
+        h.m_pos = np.zeros((npos, 2), np.float32)
+        h.m_v = np.zeros((nv, 2), np.int32)
+        h.m_he = np.zeros((nhe, 4), np.int32)
+        h.m_t = np.zeros(nt, np.int32)
+        check(L.mcl_state_load(str(path).encode(), _np_ptr(h.m_pos), _np_ptr(h.m_v), _np_ptr(h.m_he), _np_ptr(h.m_t)),
+              "mcl_state_load")
+        h.perm = np.zeros(0, np.int32)
+        h._points = h.m_pos[4:] if not h.periodic else h.m_pos
+        return h
+
 
 class HostTriangulationBatch:
     """An ensemble of independent box triangulations held as one mesh.
@@ -188,6 +217,11 @@ class DeviceTriangulation:
         return _tensor_from_ptr(self.m_pos, (self.m_pos_size, 2), torch.float32)
 
     # ------------------------------------------------------------ transfers
+    def save_state(self, path):
+        """the current triangulation to a state file (mcl_tri_save);
+        HostTriangulation.load_state(path) reads it back"""
+        check(lib().mcl_tri_save(self._h, str(path).encode()), "mcl_tri_save")
+
     def transferToHost(self):
         h = self.h_triangulation
         check(lib().mcl_tri_download(self._h, _np_ptr(h.m_pos), _np_ptr(h.m_v), _np_ptr(h.m_he),

@@ -227,3 +227,39 @@ def test_path_choice_auto(oracle):
     assert len(set(paths[2:])) == 1  # the faster path, kept
     with pytest.raises(M.MclError):
         dt.set_option(M.OPT_STAR_ENABLE, 3)
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_save_and_resume(oracle, tmp_path, periodic):
+    """a triangulation saved after 3 steps and loaded into a new device object
+    continues exactly as the original: same positions and the same canonical
+    triangles after 2 more steps of the same motion"""
+    O = oracle
+    xy = uniform_points(20000, 41)
+    H = M.HostTriangulation(periodic=periodic)
+    H.addDelaunayPoints(xy)
+    dt = M.DeviceTriangulation(H)
+    motion = (M.MOTION_SHEAR, 2e-4, 1e-3) if periodic else (M.MOTION_GAUSS, 1e-3, 0.0)
+    for d in (dt,):
+        d.set_option(M.OPT_STAR_MIN_VERTICES, 0)
+        d.set_option(M.OPT_STAR_ENABLE, 1 if periodic else 0)
+    for k in range(3):
+        dt.step(motion[0], motion[1], amp=motion[2], seed=4, step=k)
+    path = tmp_path / "state.mcl"
+    dt.save_state(path)
+    G = M.HostTriangulation.load_state(path)
+    assert G.periodic == periodic
+    dt2 = M.DeviceTriangulation(G)
+    dt2.set_option(M.OPT_STAR_MIN_VERTICES, 0)
+    dt2.set_option(M.OPT_STAR_ENABLE, 1 if periodic else 0)
+    p1, v1, h1, t1 = dt.download()
+    p2, v2, h2, t2 = dt2.download()
+    assert np.array_equal(p1, p2) and np.array_equal(h1, h2) and np.array_equal(t1, t2)
+    for k in range(3, 5):
+        dt.step(motion[0], motion[1], amp=motion[2], seed=4, step=k)
+        dt2.step(motion[0], motion[1], amp=motion[2], seed=4, step=k)
+    p1, v1, h1, t1 = dt.download()
+    p2, v2, h2, t2 = dt2.download()
+    assert np.array_equal(p1, p2)
+    assert np.array_equal(O.canonical_triangles(h1, t1), O.canonical_triangles(h2, t2))
+    assert O.count_non_delaunay(p2, h2, periodic=periodic) == 0

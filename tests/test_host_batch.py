@@ -50,3 +50,38 @@ def test_batch_build_rejects_bad_input():
         assert "3" in str(e) or "DUPLICATE" in str(e).upper()
     else:
         raise AssertionError("duplicate points accepted")
+
+
+def test_state_file_round_trip(tmp_path, oracle):
+    """mcl_state_save / info / load: the arrays come back bit for bit; a
+    truncated or corrupted file, or another file, is rejected"""
+    xy = np.random.default_rng(3).random((500, 2)).astype(np.float32)
+    H = M.HostTriangulation()
+    H.addDelaunayPoints(xy)
+    path = tmp_path / "tri.mcl"
+    H.save_state(path)
+    G = M.HostTriangulation.load_state(path)
+    assert not G.periodic
+    for a, b in ((H.m_pos, G.m_pos), (H.m_v, G.m_v), (H.m_he, G.m_he), (H.m_t, G.m_t)):
+        assert a.dtype == b.dtype and np.array_equal(a, b)
+    raw = path.read_bytes()
+    assert raw[:8] == b"MCLTRI01" and len(raw) == 40 + 8 * len(H.m_pos) + 8 * len(H.m_v) + 16 * len(H.m_he) + \
+        4 * len(H.m_t) + 8
+    bad = bytearray(raw)
+    bad[40 + 123] ^= 0x10
+    (tmp_path / "bad.mcl").write_bytes(bytes(bad))
+    (tmp_path / "short.mcl").write_bytes(raw[:-9])
+    (tmp_path / "long.mcl").write_bytes(raw + b"\0")
+    (tmp_path / "other.mcl").write_bytes(b"not a state file at all, no magic here")
+    for name in ("bad.mcl", "short.mcl", "long.mcl", "other.mcl", "missing.mcl"):
+        try:
+            M.HostTriangulation.load_state(tmp_path / name)
+        except M.MclError:
+            pass
+        else:
+            raise AssertionError(name + " accepted")
+    # a torus state keeps its flag
+    T = M.HostTriangulation(periodic=True)
+    T.addDelaunayPoints(xy)
+    T.save_state(tmp_path / "torus.mcl")
+    assert M.HostTriangulation.load_state(tmp_path / "torus.mcl").periodic
