@@ -30,19 +30,6 @@
 
 namespace mcl {
 
-// ------------------------------------------------------------ Hilbert key
-static uint64_t hilbert_d(uint32_t n, uint32_t x, uint32_t y) {
-    uint64_t d = 0;
-    for (uint32_t s = n / 2; s > 0; s /= 2) {
-        uint32_t rx = (x & s) > 0, ry = (y & s) > 0;
-        d += (uint64_t)s * s * ((3 * rx) ^ ry);
-        if (ry == 0) {
-            if (rx == 1) { x = s - 1 - x; y = s - 1 - y; }
-            std::swap(x, y);
-        }
-    }
-    return d;
-}
 
 std::vector<int> hilbert_order(const float* xy, int n) {
     float lo[2] = {1e30f, 1e30f}, hi[2] = {-1e30f, -1e30f};

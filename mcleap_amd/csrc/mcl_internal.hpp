@@ -34,6 +34,26 @@ struct HostMesh {
     std::vector<int> vstart, hestart, tstart;
 };
 int build_box(const float* xy, int n, HostMesh& m);
+
+// Hilbert index of cell (x, y) on an n x n grid (n a power of two); the
+// vertex order of every box build (host and device)
+__host__ __device__ inline uint64_t hilbert_d(uint32_t n, uint32_t x, uint32_t y) {
+    uint64_t d = 0;
+    for (uint32_t s = n / 2; s > 0; s /= 2) {
+        const uint32_t rx = (x & s) > 0, ry = (y & s) > 0;
+        d += (uint64_t)s * s * ((3 * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = s - 1 - x;
+                y = s - 1 - y;
+            }
+            const uint32_t tmp = x;
+            x = y;
+            y = tmp;
+        }
+    }
+    return d;
+}
 int build_torus(const float* xy, int n, HostMesh& m);
 std::vector<int> hilbert_order(const float* xy, int n);
 
@@ -54,6 +74,11 @@ struct DMesh {
     int* v_pidx = nullptr;
     int* t_he = nullptr;
 };
+
+// a valid triangulation of n points at their row positions, built on the
+// device into m (nv = n + 4, nhe = 6n + 10, nt = 2n + 2); target gets every
+// vertex's own position, perm (or null) the input index of vertex 4 + k
+int gbuild_box(DMesh& m, const float2* xy, int n, float2* target, int* perm, hipStream_t s);
 
 // statistics slots kept on the device (unsigned long long)
 enum StatSlot : int {

@@ -209,6 +209,7 @@ __device__ __forceinline__ void flip_soa(const DMesh& m, int e, int& he2, int& h
 }
 
 // -------------------------------------------------------------- movement
+constexpr int kMotionTarget = 1000;  // internal: the displacement array holds absolute targets
 // synthetic displacement of every non-corner vertex; stores the accepted
 // position (acc), the target (tgt) and moves the vertex to its target
 __global__ void __launch_bounds__(kThreads) k_move_motion(DMesh m, float2* __restrict__ acc, float2* __restrict__ tgt,
@@ -219,7 +220,9 @@ __global__ void __launch_bounds__(kThreads) k_move_motion(DMesh m, float2* __res
         acc[v] = q;
         if (is_corner(q)) { tgt[v] = q; continue; }
         float2 t;
-        if (disp) {  // caller displacement (movePoints semantics); the torus wraps
+        if (disp && p.motion == kMotionTarget) {  // absolute targets (the device build)
+            t = disp[v];
+        } else if (disp) {  // caller displacement (movePoints semantics); the torus wraps
             const float2 d = disp[v];
             t = make_float2(q.x + d.x, q.y + d.y);
             if (m.periodic) t = make_float2(wrap01(t.x), wrap01(t.y));
@@ -1625,29 +1628,18 @@ static int upload_aos(mcl_tri* t, const float* pos, const int* v2, const int* he
     return MCL_OK;
 }
 
-extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv, const int* he4, int nhe,
-                              const int* t1, int nt, int periodic, mcl_tri_t** out) {
-    if (!out || !pos || !v2 || !he4 || !t1 || npos != nv || nhe <= 0 || (nhe & 1) || nt <= 0) return MCL_ERR_INPUT;
+// Allocates a triangulation of nv vertices, nhe half-edges and nt triangles
+// (the mesh arrays are left for the caller to fill) and clears its work
+// arrays.  ncorner: leading corner vertices of a box mesh (0 on the torus).
+int tri_make(int nv, int nhe, int nt, int periodic, int ncorner, mcl_tri** out) {
+    if (!out || nv <= 0 || nhe <= 0 || (nhe & 1) || nt <= 0) return MCL_ERR_INPUT;
     if (nhe / 2 >= (1 << 28)) return MCL_ERR_CAPACITY;  // claim keys hold 28-bit edge ids
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MCL_ERR_NO_DEVICE;
     mcl_tri* t = new mcl_tri();
     DMesh& m = t->m;
     m.nv = nv; m.nhe = nhe; m.nt = nt; m.ne = nhe / 2; m.periodic = periodic ? 1 : 0;
-    // box meshes keep their corners first: vertices 0..3, or 4i..4i+3 for
-    // instance i of an ensemble (the star path needs it)
-    m.ncorner = 0;
-    if (!periodic && nv > 4) {
-        auto corner_at = [&](int v) {
-            const int pi = v2[2 * v];
-            return pi >= 0 && pi < npos && (fabsf(pos[2 * pi]) >= 1.0e6f || fabsf(pos[2 * pi + 1]) >= 1.0e6f);
-        };
-        int k = 0;
-        while (k < nv && corner_at(k)) k++;
-        bool ok = k > 0 && k % 4 == 0;
-        for (int v = k; ok && v < nv; v++) ok = !corner_at(v);
-        if (ok) m.ncorner = k;
-    }
+    m.ncorner = periodic ? 0 : ncorner;
     int rc = MCL_OK;
 #define A(p, n) if (rc == MCL_OK) rc = dalloc(&(p), (size_t)(n))
     A(m.pos, nv); A(m.he_next, nhe); A(m.he_v, nhe); A(m.he_t, nhe); A(m.he_op, nhe);
@@ -1674,7 +1666,6 @@ extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv,
     if (rc == MCL_OK &&
         hipHostMalloc((void**)&t->h_stats, sizeof(unsigned long long) * kStCount * kStatShards) != hipSuccess)
         rc = MCL_ERR_HIP;
-    if (rc == MCL_OK) rc = upload_aos(t, pos, v2, he4, t1);
     if (rc == MCL_OK) {
         hipLaunchKernelGGL(k_set_u64, dim3(blocks_for(nt)), dim3(kThreads), 0, 0, t->claim, nt, 0ull);
         hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(m.ne)), dim3(kThreads), 0, 0, t->e_stamp, m.ne, 0);
@@ -1694,6 +1685,33 @@ extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv,
     if (rc != MCL_OK) { tri_free(t); delete t; return rc; }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, 0) == hipSuccess) t->grid_blocks = prop.multiProcessorCount * 8;
+    *out = t;
+    return MCL_OK;
+}
+
+
+extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv, const int* he4, int nhe,
+                              const int* t1, int nt, int periodic, mcl_tri_t** out) {
+    if (!out || !pos || !v2 || !he4 || !t1 || npos != nv || nhe <= 0 || (nhe & 1) || nt <= 0) return MCL_ERR_INPUT;
+    // box meshes keep their corners first: vertices 0..3, or 4i..4i+3 for
+    // instance i of an ensemble (the star path needs it)
+    int ncorner = 0;
+    if (!periodic && nv > 4) {
+        auto corner_at = [&](int v) {
+            const int pi = v2[2 * v];
+            return pi >= 0 && pi < npos && (fabsf(pos[2 * pi]) >= 1.0e6f || fabsf(pos[2 * pi + 1]) >= 1.0e6f);
+        };
+        int k = 0;
+        while (k < nv && corner_at(k)) k++;
+        bool ok = k > 0 && k % 4 == 0;
+        for (int v = k; ok && v < nv; v++) ok = !corner_at(v);
+        if (ok) ncorner = k;
+    }
+    mcl_tri* t = nullptr;
+    int rc = tri_make(nv, nhe, nt, periodic, ncorner, &t);
+    if (rc != MCL_OK) return rc;
+    rc = upload_aos(t, pos, v2, he4, t1);
+    if (rc != MCL_OK) { tri_free(t); delete t; return rc; }
     *out = t;
     return MCL_OK;
 }
@@ -2715,6 +2733,83 @@ static int kinetic_step(mcl_tri* t, const mcl_step_params_t* p, const float2* di
         t->star_backoff = t->star_backoff ? std::min(2 * t->star_backoff, 64) : 1;
         t->star_skip = t->star_backoff;
     }
+    return MCL_OK;
+}
+
+// vertices not at their target (exact compare); the first 4 into n[2..5]
+// (n[1]: listed so far)
+__global__ void __launch_bounds__(kThreads) k_off_target(DMesh m, const float2* __restrict__ target, int* __restrict__ n) {
+    int c = 0;
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
+        const float2 p = m.pos[v], q = target[v];
+        if (!(p.x == q.x && p.y == q.y)) {
+            c++;
+            const int k = atomicAdd(&n[1], 1);
+            if (k < 4) n[2 + k] = v;
+        }
+    }
+    if (c) atomicAdd(n, c);
+}
+
+// The exact Delaunay triangulation of n device points, built on the GPU: a
+// valid triangulation of the points at row positions (mcl_gbuild.hip), then
+// flip-based kinetic steps to the points' own positions (repeated while a
+// vertex stalled).  Vertex numbering as mcl_hmesh_build's.
+extern "C" int mcl_tri_build(const void* d_xy, int n, void* d_perm, mcl_tri_t** out, void* stream) {
+    if (!out || !d_xy || n <= 0) return MCL_ERR_INPUT;
+    if ((long long)n * 6 + 10 >= (1LL << 31)) return MCL_ERR_CAPACITY;
+    hipStream_t s = (hipStream_t)stream;
+    mcl_tri* t = nullptr;
+    MCL_TRY(tri_make(n + 4, 6 * n + 10, 2 * n + 2, 0, 4, &t));
+    float2* target = nullptr;
+    int* offc = nullptr;
+    int rc = dalloc(&target, (size_t)n + 4);
+    if (rc == MCL_OK) rc = dalloc(&offc, 8);
+    if (rc == MCL_OK) rc = gbuild_box(t->m, (const float2*)d_xy, n, target, (int*)d_perm, s);
+    if (rc == MCL_OK) rc = validate(t, s, "device build");
+    mcl_step_params_t p{};
+    p.motion = kMotionTarget;
+    p.variant = 0;
+    for (int it = 0; rc == MCL_OK; it++) {
+        rc = step_impl(t, &p, target, nullptr, stream);
+        if (rc != MCL_OK) break;
+        hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, offc, 6, 0);
+        hipLaunchKernelGGL(k_off_target, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, (const float2*)target,
+                           offc);
+        int oc[6] = {0, 0, 0, 0, 0, 0};
+        if (hipMemcpyAsync(oc, offc, sizeof(oc), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            rc = MCL_ERR_HIP;
+            break;
+        }
+        const int off = oc[0];
+        if (trace_on()) {
+            fprintf(stderr, "  [device build] pass %d: %d vertices off target\n", it, off);
+            for (int k = 0; k < std::min(off, 4); k++) {
+                float2 a, b;
+                int deg = 0;
+                (void)hipMemcpy(&a, t->m.pos + oc[2 + k], sizeof(a), hipMemcpyDeviceToHost);
+                (void)hipMemcpy(&b, target + oc[2 + k], sizeof(b), hipMemcpyDeviceToHost);
+                int h0 = 0, h = 0;
+                (void)hipMemcpy(&h0, t->m.v_he + oc[2 + k], sizeof(int), hipMemcpyDeviceToHost);
+                h = h0;
+                do {  // degree, by the half-edge walk
+                    int tw = 0;
+                    (void)hipMemcpy(&tw, t->m.he_next + (h ^ 1), sizeof(int), hipMemcpyDeviceToHost);
+                    h = tw;
+                    deg++;
+                } while (h != h0 && deg < 4096);
+                fprintf(stderr, "    vertex %d at (%.9g, %.9g) target (%.9g, %.9g) degree %d\n", oc[2 + k], a.x, a.y,
+                        b.x, b.y, deg);
+            }
+        }
+        if (off == 0) break;
+        if (it == 15) rc = MCL_ERR_UNRESOLVED;
+    }
+    if (target) (void)hipFree(target);
+    if (offc) (void)hipFree(offc);
+    if (rc != MCL_OK) { tri_free(t); delete t; return rc; }
+    *out = t;
     return MCL_OK;
 }
 
