@@ -29,6 +29,20 @@ struct DeviceTriangulation {
         mcl_tri_sizes(t_, &m_pos_size, &m_v_size, &m_he_size, &m_t_size);
         m_pos = static_cast<vec2*>(mcl_tri_pos_ptr(t_));
     }
+    // The same object built on the GPU from n device points (mcl_tri_build:
+    // HostTriangulation::addDelaunayPoints, Host_Triangulation.h:296-300, and
+    // this class's constructor in one device pass).  h receives the arrays,
+    // with the vertex numbering the host build gives.
+    DeviceTriangulation(HostTriangulation* h, const vec2* d_points, int n) : h_triangulation(h) {
+        mcleap::must(mcl_tri_build(d_points, n, nullptr, &t_, nullptr), "DeviceTriangulation(points)");
+        adopt();
+    }
+    // Resumed from a state file written by saveState (mcl_tri_load).
+    DeviceTriangulation(HostTriangulation* h, const char* state_path) : h_triangulation(h) {
+        mcleap::must(mcl_tri_load(state_path, &t_), "DeviceTriangulation(state)");
+        adopt();
+    }
+    bool saveState(const char* path) { return mcleap::ok(mcl_tri_save(t_, path), "saveState"); }
     ~DeviceTriangulation() { mcl_tri_destroy(t_); }
     DeviceTriangulation(const DeviceTriangulation&) = delete;
     DeviceTriangulation& operator=(const DeviceTriangulation&) = delete;
@@ -135,4 +149,18 @@ struct DeviceTriangulation {
 
   private:
     mcl_tri_t* t_ = nullptr;
+    // sizes, m_pos and the host object's arrays for a triangulation made by the library
+    void adopt() {
+        mcl_tri_sizes(t_, &m_pos_size, &m_v_size, &m_he_size, &m_t_size);
+        m_pos = static_cast<vec2*>(mcl_tri_pos_ptr(t_));
+        int per = 0;
+        mcl_tri_is_periodic(t_, &per);
+        HostTriangulation* h = h_triangulation;
+        h->periodic = per != 0;
+        h->m_pos.resize(m_pos_size);
+        h->m_v.resize(m_v_size);
+        h->m_he.resize(m_he_size);
+        h->m_t.resize(m_t_size);
+        transferToHost();
+    }
 };

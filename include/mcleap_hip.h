@@ -144,7 +144,8 @@ int mcl_tri_is_periodic(const mcl_tri_t* t, int* periodic);
  * Save / resume (no counterpart in the reference, whose state lives only in
  * the host object's vectors, Host_Triangulation.h:27-30).  The file holds
  * exactly those AoS arrays (pos, v2, he4, t1) after a 40-byte header
- * ("MCLTRI01", version 1, periodic, npos, nv, nhe, nt, 2 reserved int32) and
+ * ("MCLTRI01", version 1, periodic, npos, nv, nhe, nt, and the library's
+ * flip-round and stamp counters, 0 from mcl_state_save) and
  * ends with an FNV-1a 64 checksum of the arrays; little-endian.  Loading a
  * file that is truncated, corrupted or not such a file gives MCL_ERR_INPUT.
  * Steps resumed from a saved state reproduce the uninterrupted run: the

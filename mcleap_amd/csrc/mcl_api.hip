@@ -175,8 +175,9 @@ extern "C" int mcl_device_info(char* name, int len, int* cu_count) {
 // (Host_Triangulation.h:27-30, as transferToHost fills them) after a 48-byte
 // header, then an FNV-1a 64 checksum of the arrays.
 //   char magic[8] "MCLTRI01"; int32 version 1, periodic, npos, nv, nhe, nt,
-//   two reserved zeros; float pos[2 npos]; int32 v2[2 nv]; int32 he4[4 nhe];
-//   int32 t1[nt]; uint64 checksum.  Little-endian.
+//   the library's round and stamp counters (0, 0 from mcl_state_save);
+//   float pos[2 npos]; int32 v2[2 nv]; int32 he4[4 nhe]; int32 t1[nt];
+//   uint64 checksum.  Little-endian.
 namespace {
 constexpr char kMagic[8] = {'M', 'C', 'L', 'T', 'R', 'I', '0', '1'};
 struct StateHeader {
@@ -199,8 +200,8 @@ bool read_header(FILE* f, StateHeader& hd) {
 }
 }  // namespace
 
-extern "C" int mcl_state_save(const char* path, int periodic, const float* pos, int npos, const int* v2, int nv,
-                              const int* he4, int nhe, const int* t1, int nt) {
+static int state_write(const char* path, int periodic, const float* pos, int npos, const int* v2, int nv,
+                       const int* he4, int nhe, const int* t1, int nt, int r0, int r1) {
     if (!path || npos < 0 || nv < 0 || nhe < 0 || nt < 0 || (npos && !pos) || (nv && !v2) || (nhe && !he4) ||
         (nt && !t1))
         return MCL_ERR_INPUT;
@@ -211,7 +212,8 @@ extern "C" int mcl_state_save(const char* path, int periodic, const float* pos, 
     hd.version = 1;
     hd.periodic = periodic ? 1 : 0;
     hd.npos = npos; hd.nv = nv; hd.nhe = nhe; hd.nt = nt;
-    hd.reserved[0] = hd.reserved[1] = 0;
+    hd.reserved[0] = r0;
+    hd.reserved[1] = r1;
     uint64_t h = 14695981039346656037ull;
     const struct { const void* p; size_t n; } parts[4] = {
         {pos, sizeof(float) * 2 * (size_t)npos}, {v2, sizeof(int) * 2 * (size_t)nv},
@@ -225,6 +227,11 @@ extern "C" int mcl_state_save(const char* path, int periodic, const float* pos, 
     ok = ok && fwrite(&h, sizeof(h), 1, f) == 1;
     ok = (fclose(f) == 0) && ok;
     return ok ? MCL_OK : MCL_ERR_INPUT;
+}
+
+extern "C" int mcl_state_save(const char* path, int periodic, const float* pos, int npos, const int* v2, int nv,
+                              const int* he4, int nhe, const int* t1, int nt) {
+    return state_write(path, periodic, pos, npos, v2, nv, he4, nhe, t1, nt, 0, 0);
 }
 
 extern "C" int mcl_state_info(const char* path, int* periodic, int* npos, int* nv, int* nhe, int* nt) {
@@ -282,7 +289,9 @@ extern "C" int mcl_tri_save(mcl_tri_t* t, const char* path) {
     std::vector<int> v2(2 * (size_t)nv), he4(4 * (size_t)nhe), t1((size_t)nt);
     rc = mcl_tri_download(t, pos.data(), v2.data(), he4.data(), t1.data());
     if (rc != MCL_OK) return rc;
-    return mcl_state_save(path, periodic, pos.data(), npos, v2.data(), nv, he4.data(), nhe, t1.data(), nt);
+    int round = 0, stamp = 0;
+    mcl::tri_counters(t, &round, &stamp, false);
+    return state_write(path, periodic, pos.data(), npos, v2.data(), nv, he4.data(), nhe, t1.data(), nt, round, stamp);
 }
 
 extern "C" int mcl_tri_load(const char* path, mcl_tri_t** out) {
@@ -294,5 +303,14 @@ extern "C" int mcl_tri_load(const char* path, mcl_tri_t** out) {
     std::vector<int> v2(2 * (size_t)nv), he4(4 * (size_t)nhe), t1((size_t)nt);
     rc = mcl_state_load(path, pos.data(), v2.data(), he4.data(), t1.data());
     if (rc != MCL_OK) return rc;
-    return mcl_tri_create(pos.data(), npos, v2.data(), nv, he4.data(), nhe, t1.data(), nt, periodic, out);
+    StateHeader hd;
+    FILE* f = fopen(path, "rb");
+    const bool ok = f && read_header(f, hd);
+    if (f) fclose(f);
+    if (!ok) return MCL_ERR_INPUT;
+    rc = mcl_tri_create(pos.data(), npos, v2.data(), nv, he4.data(), nhe, t1.data(), nt, periodic, out);
+    if (rc != MCL_OK) return rc;
+    // the flip rounds' keys and walk seeds continue where the saved run was
+    if (hd.reserved[0] > 0 && hd.reserved[1] > 0) mcl::tri_counters(*out, &hd.reserved[0], &hd.reserved[1], true);
+    return MCL_OK;
 }

@@ -75,6 +75,10 @@ struct DMesh {
     int* t_he = nullptr;
 };
 
+// the flip rounds' round and stamp counters of a triangulation (read, or
+// set when set is true); saved in state files so that a resumed run repeats
+int tri_counters(mcl_tri_t* t, int* round, int* stamp, bool set);
+
 // a valid triangulation of n points at their row positions, built on the
 // device into m (nv = n + 4, nhe = 6n + 10, nt = 2n + 2); target gets every
 // vertex's own position, perm (or null) the input index of vertex 4 + k

@@ -1742,6 +1742,18 @@ extern "C" int mcl_tri_upload(mcl_tri_t* t, const float* pos, const int* v2, con
     return upload_aos(t, pos, v2, he4, t1);
 }
 
+int mcl::tri_counters(mcl_tri_t* t, int* round, int* stamp, bool set) {
+    if (!t || !round || !stamp) return MCL_ERR_INPUT;
+    if (set) {
+        t->round = *round;
+        t->stamp = *stamp;
+    } else {
+        *round = t->round;
+        *stamp = t->stamp;
+    }
+    return MCL_OK;
+}
+
 extern "C" int mcl_tri_is_periodic(const mcl_tri_t* t, int* periodic) {
     if (!t || !periodic) return MCL_ERR_INPUT;
     *periodic = t->m.periodic;

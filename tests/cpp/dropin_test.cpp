@@ -190,6 +190,32 @@ int main() {
         }
         CHECK(gc.mean_num_particle_in_cell() > 0.f, "mean occupancy");
     }
+    // built on the GPU from device points, saved and resumed
+    {
+        vec2* d_pts = nullptr;
+        (void)hipMalloc(&d_pts, sizeof(vec2) * numP);
+        (void)hipMemcpy(d_pts, h_pos.data(), sizeof(vec2) * numP, hipMemcpyHostToDevice);
+        HostTriangulation hb(false), hh(false), hr(false);
+        std::vector<vec2> pts = h_pos;
+        CHECK(hh.addDelaunayPoints(pts), "host build for the device build");
+        DeviceTriangulation db(&hb, d_pts, numP);
+        CHECK(hb.m_pos.size() == hh.m_pos.size(), "device build size");
+        bool same = hb.m_pos.size() == hh.m_pos.size();
+        for (size_t v = 0; same && v < hb.m_pos.size(); v++)
+            same = hb.m_pos[v].x == hh.m_pos[v].x && hb.m_pos[v].y == hh.m_pos[v].y;
+        CHECK(same, "device build: the host build's vertex numbering");
+        check_mesh(hb, "device build");
+        const char* path = "dropin_state.mcl";
+        CHECK(db.saveState(path), "saveState");
+        DeviceTriangulation dr(&hr, path);
+        bool eq = hr.m_he.size() == hb.m_he.size() && hr.m_pos.size() == hb.m_pos.size();
+        for (size_t h = 0; eq && h < hr.m_he.size(); h++)
+            eq = hr.m_he[h].next == hb.m_he[h].next && hr.m_he[h].v == hb.m_he[h].v && hr.m_he[h].t == hb.m_he[h].t &&
+                 hr.m_he[h].op == hb.m_he[h].op;
+        CHECK(eq, "resumed state equals the saved one");
+        std::remove(path);
+        (void)hipFree(d_pts);
+    }
     (void)hipFree(d_move);
     (void)hipFree(d_pos);
     (void)hipFree(d_ring);
