@@ -53,6 +53,8 @@ def _args():
     ap.add_argument("--n", type=int, default=0, help="override point count (testing only)")
     ap.add_argument("--ninst", type=int, default=0, help="override the ensemble's instance count (testing only)")
     ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--build", choices=("device", "host"), default="device",
+                    help="initial triangulation on the GPU (mcl_tri_build) or on the host (single instances)")
     ap.add_argument("--plain-steps", type=int, default=10, help="steps of the plain form timed for comparison")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -198,6 +200,7 @@ def main():
     cfg = dict(CONFIGS[a.workload])
     periodic = cfg["domain"] == "torus"
     batch = a.workload == "c5"
+    dt = None
     if batch:
         # the ensemble as one mesh: every launch advances all instances
         inst_pts = ensemble_points(a.ninst or None)
@@ -211,11 +214,18 @@ def main():
         xy = initial_points(a.workload, a.n or None)
         n = len(xy)
         t0 = time.time()
-        H = M.HostTriangulation(periodic=periodic)
-        H.addDelaunayPoints(xy)
+        dt = None
+        if a.build == "device":
+            # the exact triangulation built on the GPU (same result and vertex numbering as the host build)
+            dt = M.DeviceTriangulation.from_points(xy, periodic=periodic)
+            torch.cuda.synchronize()
+        else:
+            H = M.HostTriangulation(periodic=periodic)
+            H.addDelaunayPoints(xy)
         build_s = time.time() - t0
-        print("[bench] %s: %d points, host build %.2f s" % (a.workload, n, build_s), file=sys.stderr, flush=True)
-    dt = M.DeviceTriangulation(H)
+        print("[bench] %s: %d points, %s build %.2f s" % (a.workload, n, a.build, build_s), file=sys.stderr, flush=True)
+    if dt is None:
+        dt = M.DeviceTriangulation(H)
     nv = dt.m_pos_size
     dev = "cuda"
     ring_off = torch.zeros(nv + 1, dtype=torch.int32, device=dev)
@@ -370,7 +380,7 @@ def main():
         "config": {"workload": a.workload, "n_points": n, "domain": cfg["domain"], "sigma": cfg["sigma"],
                    "instances": int(H.ninst) if batch else 1,
                    "amp": cfg["amp"], "frnn_radius": radius, "variant": a.variant,
-                   "host_build_s": round(build_s, 3)},
+                   "build": "host threads" if batch else a.build, "build_s": round(build_s, 3)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "algo_bytes_per_launch": algo, "avg_launch_ms": avg_ms, "launches": launches},

@@ -31,10 +31,11 @@ struct DeviceTriangulation {
     }
     // The same object built on the GPU from n device points (mcl_tri_build:
     // HostTriangulation::addDelaunayPoints, Host_Triangulation.h:296-300, and
-    // this class's constructor in one device pass).  h receives the arrays,
-    // with the vertex numbering the host build gives.
+    // this class's constructor in one device pass; a torus when h->periodic).
+    // h receives the arrays, with the vertex numbering the host build gives.
     DeviceTriangulation(HostTriangulation* h, const vec2* d_points, int n) : h_triangulation(h) {
-        mcleap::must(mcl_tri_build(d_points, n, nullptr, &t_, nullptr), "DeviceTriangulation(points)");
+        mcleap::must(mcl_tri_build(d_points, n, h->periodic ? 1 : 0, nullptr, &t_, nullptr),
+                     "DeviceTriangulation(points)");
         adopt();
     }
     // Resumed from a state file written by saveState (mcl_tri_load).

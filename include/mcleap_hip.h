@@ -131,12 +131,16 @@ int mcl_tri_sizes(const mcl_tri_t* t, int* npos, int* nv, int* nhe, int* nt);
 int mcl_tri_download(mcl_tri_t* t, float* pos, int* v2, int* he4, int* t1);
 /* HostTriangulation() + addDelaunayPoints(points) + DeviceTriangulation(host)
  * (Host_Triangulation.h:107-137, 296-300; Device_Triangulation.h:232-306) in
- * one call, built on the GPU from n device points (float2, box layout, inside
- * the corners): a valid triangulation at row positions, then kinetic steps
- * to the points.  The result is the exact Delaunay triangulation with the
- * vertex numbering of mcl_hmesh_build (vertex 4 + k = point d_perm[k];
- * d_perm may be NULL).  MCL_ERR_DUPLICATE for repeated points. */
-int mcl_tri_build(const void* d_xy, int n, void* d_perm, mcl_tri_t** out, void* stream);
+ * one call, built on the GPU from n device points (float2): a valid
+ * triangulation at row positions, then kinetic steps to the points.
+ * periodic = 0: box layout, points inside the corners; periodic = 1: the
+ * unit torus, points in [0,1)^2, n >= 16, rows of points less than half the
+ * torus apart (else MCL_ERR_INPUT).  The result is the exact Delaunay
+ * triangulation with the vertex numbering of mcl_hmesh_build (vertex base + k
+ * = point d_perm[k], base 4 in a box, 0 on the torus; d_perm may be NULL).
+ * MCL_ERR_DUPLICATE for repeated points; MCL_ERR_UNRESOLVED when vertices
+ * cannot reach their points (all points on one line). */
+int mcl_tri_build(const void* d_xy, int n, int periodic, void* d_perm, mcl_tri_t** out, void* stream);
 /* 1 for a torus triangulation, 0 for a box (or ensemble of boxes) */
 int mcl_tri_is_periodic(const mcl_tri_t* t, int* periodic);
 

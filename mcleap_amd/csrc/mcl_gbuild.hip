@@ -78,15 +78,16 @@ __global__ void __launch_bounds__(kB) k_hilbert_keys(const float2* __restrict__ 
     }
 }
 
-// vertex 4 + k = point order[k]: target positions and (y, x) keys
+// vertex base + k = point order[k] (base 4 in a box, 0 on the torus):
+// target positions and (y, x) keys
 __global__ void __launch_bounds__(kB) k_place(const float2* __restrict__ xy, const int* __restrict__ order, int n,
-                                              float2* __restrict__ target, unsigned long long* __restrict__ yx,
-                                              int* __restrict__ vid) {
+                                              int base, float2* __restrict__ target,
+                                              unsigned long long* __restrict__ yx, int* __restrict__ vid) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const float2 p = xy[order[k]];
-        target[4 + k] = p;
+        target[base + k] = p;
         yx[k] = ((unsigned long long)ford(p.y) << 32) | ford(p.x);
-        vid[k] = 4 + k;
+        vid[k] = base + k;
     }
 }
 
@@ -207,13 +208,68 @@ __global__ void __launch_bounds__(kB) k_row_pos(const int* __restrict__ lat, Row
     }
 }
 
+// torus rows: every point at its own x raised to one ulp above its left
+// neighbour (a row is a closed loop, so no point is pinned); a row where
+// that reaches 1 is spread evenly instead
+__global__ void __launch_bounds__(kB) k_row_pos_torus(const int* __restrict__ lat, Rows rows,
+                                                      const float* __restrict__ ry, const float2* __restrict__ target,
+                                                      float2* __restrict__ pos) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rows.R; r += gridDim.x * blockDim.x) {
+        const int s = rows.start(r), m = rows.count(r);
+        const float y = ry[r];
+        float prev = -1.0f;
+        bool ok = true;
+        for (int c = 0; c < m; c++) {
+            float x = target[lat[s + c]].x;
+            if (!(x > prev)) x = nextafterf(prev, 3.0e38f);
+            if (!(x < 1.0f)) {
+                ok = false;
+                break;
+            }
+            pos[lat[s + c]] = make_float2(x, y);
+            prev = x;
+        }
+        if (!ok)
+            for (int c = 0; c < m; c++) pos[lat[s + c]] = make_float2(((float)c + 0.5f) / (float)m, y);
+    }
+}
+
+// torus strips: row r and row r+1 (mod R) bound an annulus, triangulated by
+// a zipper that starts at the rung between the first points of both rows
+// and runs once around (m_r + m_r+1 triangles, from triangle
+// rows.start(r) + rows.start(r+1) - m0); x positions are unrolled past 1
+__global__ void __launch_bounds__(kB) k_strips_torus(const int* __restrict__ lat, Rows rows,
+                                                     const float2* __restrict__ pos, int3* __restrict__ tri) {
+    const int R = rows.R, m0 = rows.count(0);
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < R; r += gridDim.x * blockDim.x) {
+        const int rt = r + 1 == R ? 0 : r + 1;
+        const int sb = rows.start(r), mb = rows.count(r), st = rows.start(rt), mt = rows.count(rt);
+        int o = rows.start(r) + rows.start(r + 1) - m0;
+        auto bx = [&](int i) { return i < mb ? pos[lat[sb + i]].x : pos[lat[sb + i - mb]].x + 1.0f; };
+        auto tx = [&](int j) { return j < mt ? pos[lat[st + j]].x : pos[lat[st + j - mt]].x + 1.0f; };
+        auto bv = [&](int i) { return lat[sb + (i < mb ? i : i - mb)]; };
+        auto tv = [&](int j) { return lat[st + (j < mt ? j : j - mt)]; };
+        int i = 0, j = 0;
+        while (i < mb || j < mt) {
+            const bool bottom = j == mt || (i < mb && bx(i + 1) <= tx(j + 1));
+            if (bottom) {
+                tri[o++] = make_int3(bv(i), bv(i + 1), tv(j));
+                i++;
+            } else {
+                tri[o++] = make_int3(bv(i), tv(j + 1), tv(j));
+                j++;
+            }
+        }
+    }
+}
+
 // triangles that are not strictly CCW (exact)
 __global__ void __launch_bounds__(kB) k_tri_check(const int3* __restrict__ tri, int nt, const float2* __restrict__ pos,
-                                                  int* __restrict__ bad) {
+                                                  int periodic, int* __restrict__ bad) {
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
         const int3 q = tri[t];
         int st;
-        if (orient<true>(pos[q.x], pos[q.y], pos[q.z], false, &st) <= 0) atomicAdd(bad, 1);
+        if (orient<true>(pos[q.x], pos[q.y], pos[q.z], periodic != 0, &st) <= 0) atomicAdd(bad, 1);
     }
 }
 
@@ -278,6 +334,7 @@ __global__ void __launch_bounds__(kB) k_fans(const int* __restrict__ lat, Rows r
 // half-edge slots: 3 per triangle (corner k: org tri[k], dst tri[k+1]), then
 // the outer face BR->BL, BL->TL, TL->TR, TR->BR
 __device__ __forceinline__ void slot_ends(const int3* tri, int nt, int s, int& a, int& b) {
+    // (s >= 3 nt only in a box mesh: its outer face)
     if (s < 3 * nt) {
         const int3 t = tri[s / 3];
         const int k = s % 3;
@@ -291,9 +348,8 @@ __device__ __forceinline__ void slot_ends(const int3* tri, int nt, int s, int& a
     }
 }
 
-__global__ void __launch_bounds__(kB) k_edge_keys(const int3* __restrict__ tri, int nt,
+__global__ void __launch_bounds__(kB) k_edge_keys(const int3* __restrict__ tri, int nt, int ns,
                                                   unsigned long long* __restrict__ key, int* __restrict__ slot) {
-    const int ns = 3 * nt + 4;
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x) {
         int a, b;
         slot_ends(tri, nt, s, a, b);
@@ -304,9 +360,8 @@ __global__ void __launch_bounds__(kB) k_edge_keys(const int3* __restrict__ tri, 
 
 // sorted position p -> edge p / 2; each key exactly twice, once each way
 __global__ void __launch_bounds__(kB) k_pair(const unsigned long long* __restrict__ key, const int* __restrict__ slot,
-                                             const int3* __restrict__ tri, int nt, int* __restrict__ hslot,
+                                             const int3* __restrict__ tri, int nt, int ns, int* __restrict__ hslot,
                                              int* __restrict__ bad) {
-    const int ns = 3 * nt + 4;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < ns; p += gridDim.x * blockDim.x) {
         const int q = p ^ 1;
         bool ok = q < ns && key[q] == key[p];
@@ -323,9 +378,8 @@ __global__ void __launch_bounds__(kB) k_pair(const unsigned long long* __restric
     }
 }
 
-__global__ void __launch_bounds__(kB) k_fill_mesh(DMesh m, const int3* __restrict__ tri, int nt,
+__global__ void __launch_bounds__(kB) k_fill_mesh(DMesh m, const int3* __restrict__ tri, int nt, int ns,
                                                   const int* __restrict__ hslot, int* __restrict__ bad) {
-    const int ns = 3 * nt + 4;
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x) {
         int a, b;
         slot_ends(tri, nt, s, a, b);
@@ -356,7 +410,7 @@ __global__ void __launch_bounds__(kB) k_vinit(DMesh m, float2* __restrict__ targ
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
         m.v_he[v] = 0x7fffffff;
         m.v_pidx[v] = v;
-        if (v < 4) {
+        if (v < 4 && !m.periodic) {
             const float lo = -10000000.f, hi = 10000000.f;  // Common_Triangulation.h:12-13
             const float2 c = make_float2(v == 1 || v == 2 ? hi : lo, v >= 2 ? hi : lo);
             m.pos[v] = c;
@@ -390,12 +444,17 @@ struct DevBuf {
 
 }  // namespace
 
-// Fills m (nv = n + 4, nhe = 6n + 10, nt = 2n + 2, allocated) with a valid
-// triangulation of the points at their row positions; target receives every
-// vertex's own position (corners included) and perm (may be null) the input
-// index of vertex 4 + k.
-int gbuild_box(DMesh& m, const float2* xy, int n, float2* target, int* perm, hipStream_t s) {
-    if (n <= 0 || m.nv != n + 4 || m.nhe != 6 * n + 10 || m.nt != 2 * n + 2) return MCL_ERR_INPUT;
+// Fills m with a valid triangulation of the points at their row positions.
+// Box (m.periodic = 0): nv = n + 4, nhe = 6n + 10, nt = 2n + 2, corners
+// first.  Torus: nv = n, nhe = 6n, nt = 2n, points in [0,1)^2, n >= 16.
+// target receives every vertex's own position (corners included) and perm
+// (may be null) the input index of vertex base + k.
+int gbuild(DMesh& m, const float2* xy, int n, float2* target, int* perm, hipStream_t s) {
+    const bool torus = m.periodic != 0;
+    const int base = torus ? 0 : 4;
+    if (n <= 0 || m.nv != n + base || m.nhe != 6 * n + (torus ? 0 : 10) || m.nt != 2 * n + (torus ? 0 : 2))
+        return MCL_ERR_INPUT;
+    if (torus && n < 16) return MCL_ERR_INPUT;
     int rc = MCL_OK;
     DevBuf mm, hk, hk2, hi, hi2, yx, yx2, vid, vid2, dup, ry;
     if ((rc = mm.alloc(sizeof(uint32_t) * 8))) return rc;
@@ -406,8 +465,10 @@ int gbuild_box(DMesh& m, const float2* xy, int n, float2* target, int* perm, hip
     MCL_HIP(hipMemcpyAsync(h, mm.p, sizeof(h), hipMemcpyDeviceToHost, s));
     MCL_HIP(hipStreamSynchronize(s));
     const float lo[2] = {funord(h[0]), funord(h[1])}, hi_[2] = {funord(h[2]), funord(h[3])};
-    for (int k = 0; k < 2; k++)
+    for (int k = 0; k < 2; k++) {
         if (!(lo[k] > -1e7f && hi_[k] < 1e7f)) return MCL_ERR_INPUT;  // outside the corners (or NaN)
+        if (torus && !(lo[k] >= 0.0f && hi_[k] < 1.0f)) return MCL_ERR_INPUT;
+    }
     // 1. vertex order (the host build's keys and ties)
     const uint32_t G = 1u << 16;
     const double sx = (hi_[0] > lo[0]) ? (G - 1) / ((double)hi_[0] - lo[0]) : 0.0;
@@ -425,8 +486,8 @@ int gbuild_box(DMesh& m, const float2* xy, int n, float2* target, int* perm, hip
     if ((rc = yx.alloc(8 * (size_t)n)) || (rc = yx2.alloc(8 * (size_t)n)) || (rc = vid.alloc(4 * (size_t)n)) ||
         (rc = vid2.alloc(4 * (size_t)n)) || (rc = dup.alloc(8)))
         return rc;
-    hipLaunchKernelGGL(k_place, dim3(nblocks(n)), dim3(kB), 0, s, xy, order, n, target, yx.get<unsigned long long>(),
-                       vid.get<int>());
+    hipLaunchKernelGGL(k_place, dim3(nblocks(n)), dim3(kB), 0, s, xy, order, n, base, target,
+                       yx.get<unsigned long long>(), vid.get<int>());
     GB_TRY(radix_pairs(yx.get<unsigned long long>(), yx2.get<unsigned long long>(), vid.get<int>(), vid2.get<int>(),
                         n, 64, s));
     Rows rows;
@@ -434,6 +495,7 @@ int gbuild_box(DMesh& m, const float2* xy, int n, float2* target, int* perm, hip
     while ((long long)(rows.R + 1) * (rows.R + 1) <= n) rows.R++;
     rows.q = n / rows.R;
     rows.extra = n % rows.R;
+    if (torus && (rows.R < 3 || rows.q < 3)) return MCL_ERR_INPUT;
     MCL_HIP(hipMemsetAsync(dup.p, 0, 8, s));
     hipLaunchKernelGGL(k_row_keys, dim3(nblocks(n)), dim3(kB), 0, s, yx2.get<unsigned long long>(), vid2.get<int>(), n,
                        rows, yx.get<unsigned long long>(), vid.get<int>(), dup.get<int>());
@@ -455,42 +517,52 @@ int gbuild_box(DMesh& m, const float2* xy, int n, float2* target, int* perm, hip
     DevBuf tri, ek, ek2, es, es2, hs, bad;
     if ((rc = tri.alloc(sizeof(int3) * (size_t)nt)) || (rc = bad.alloc(8))) return rc;
     for (int bulge = 1; bulge >= 0; bulge--) {
-        MCL_HIP(hipMemsetAsync(dup.p, 0, 8, s));
-        hipLaunchKernelGGL(k_row_pos, dim3(nblocks(rows.R)), dim3(kB), 0, s, lat, rows, (const float*)ry.get<float>(),
-                           X0, X1, (const float2*)target, m.pos, dup.get<int>(), bulge);
-        MCL_HIP(hipMemcpyAsync(&ndup, dup.p, sizeof(int), hipMemcpyDeviceToHost, s));
-        MCL_HIP(hipStreamSynchronize(s));
-        if (ndup) return MCL_ERR_INPUT;
-        hipLaunchKernelGGL(k_strips, dim3(nblocks(rows.R)), dim3(kB), 0, s, lat, rows, (const float2*)m.pos,
-                           tri.get<int3>());
-        hipLaunchKernelGGL(k_fans, dim3(nblocks(2 * (n + rows.R) + 8)), dim3(kB), 0, s, lat, rows, n, tri.get<int3>());
+        if (torus) {
+            hipLaunchKernelGGL(k_row_pos_torus, dim3(nblocks(rows.R)), dim3(kB), 0, s, lat, rows,
+                               (const float*)ry.get<float>(), (const float2*)target, m.pos);
+            hipLaunchKernelGGL(k_strips_torus, dim3(nblocks(rows.R)), dim3(kB), 0, s, lat, rows, (const float2*)m.pos,
+                               tri.get<int3>());
+        } else {
+            MCL_HIP(hipMemsetAsync(dup.p, 0, 8, s));
+            hipLaunchKernelGGL(k_row_pos, dim3(nblocks(rows.R)), dim3(kB), 0, s, lat, rows,
+                               (const float*)ry.get<float>(), X0, X1, (const float2*)target, m.pos, dup.get<int>(),
+                               bulge);
+            MCL_HIP(hipMemcpyAsync(&ndup, dup.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            MCL_HIP(hipStreamSynchronize(s));
+            if (ndup) return MCL_ERR_INPUT;
+            hipLaunchKernelGGL(k_strips, dim3(nblocks(rows.R)), dim3(kB), 0, s, lat, rows, (const float2*)m.pos,
+                               tri.get<int3>());
+            hipLaunchKernelGGL(k_fans, dim3(nblocks(2 * (n + rows.R) + 8)), dim3(kB), 0, s, lat, rows, n,
+                               tri.get<int3>());
+        }
         MCL_HIP(hipMemsetAsync(bad.p, 0, 8, s));
         hipLaunchKernelGGL(k_tri_check, dim3(nblocks(nt)), dim3(kB), 0, s, (const int3*)tri.get<int3>(), nt,
-                           (const float2*)m.pos, bad.get<int>());
+                           (const float2*)m.pos, m.periodic, bad.get<int>());
         MCL_HIP(hipGetLastError());
         int nbad = 0;
         MCL_HIP(hipMemcpyAsync(&nbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
         MCL_HIP(hipStreamSynchronize(s));
         if (nbad == 0) break;
+        if (torus) return MCL_ERR_INPUT;  // rows too sparse for the torus rows (a row gap or x gap >= 1/2)
         if (bulge == 0) return MCL_ERR_TOPOLOGY;
     }
     // 3. half-edges: pair the twins by one sort of the undirected edge keys
-    const int ns = 3 * nt + 4;  // = nhe
+    const int ns = 3 * nt + (torus ? 0 : 4);  // = nhe
     if ((rc = ek.alloc(8 * (size_t)ns)) || (rc = ek2.alloc(8 * (size_t)ns)) || (rc = es.alloc(4 * (size_t)ns)) ||
         (rc = es2.alloc(4 * (size_t)ns)) || (rc = hs.alloc(4 * (size_t)ns)))
         return rc;
-    hipLaunchKernelGGL(k_edge_keys, dim3(nblocks(ns)), dim3(kB), 0, s, (const int3*)tri.get<int3>(), nt,
+    hipLaunchKernelGGL(k_edge_keys, dim3(nblocks(ns)), dim3(kB), 0, s, (const int3*)tri.get<int3>(), nt, ns,
                        ek.get<unsigned long long>(), es.get<int>());
     GB_TRY(radix_pairs(ek.get<unsigned long long>(), ek2.get<unsigned long long>(), es.get<int>(), es2.get<int>(), ns,
                         64, s));
     MCL_HIP(hipMemsetAsync(bad.p, 0, 8, s));
     hipLaunchKernelGGL(k_pair, dim3(nblocks(ns)), dim3(kB), 0, s, (const unsigned long long*)ek2.get<unsigned long long>(),
-                       (const int*)es2.get<int>(), (const int3*)tri.get<int3>(), nt, hs.get<int>(), bad.get<int>());
+                       (const int*)es2.get<int>(), (const int3*)tri.get<int3>(), nt, ns, hs.get<int>(), bad.get<int>());
     int nbad = 0;
     MCL_HIP(hipMemcpyAsync(&nbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
     MCL_HIP(hipStreamSynchronize(s));
     if (nbad) return MCL_ERR_TOPOLOGY;
-    hipLaunchKernelGGL(k_fill_mesh, dim3(nblocks(ns)), dim3(kB), 0, s, m, (const int3*)tri.get<int3>(), nt,
+    hipLaunchKernelGGL(k_fill_mesh, dim3(nblocks(ns)), dim3(kB), 0, s, m, (const int3*)tri.get<int3>(), nt, ns,
                        (const int*)hs.get<int>(), bad.get<int>());
     MCL_HIP(hipGetLastError());
     MCL_HIP(hipMemcpyAsync(&nbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));

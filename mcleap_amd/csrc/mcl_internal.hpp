@@ -80,9 +80,10 @@ struct DMesh {
 int tri_counters(mcl_tri_t* t, int* round, int* stamp, bool set);
 
 // a valid triangulation of n points at their row positions, built on the
-// device into m (nv = n + 4, nhe = 6n + 10, nt = 2n + 2); target gets every
-// vertex's own position, perm (or null) the input index of vertex 4 + k
-int gbuild_box(DMesh& m, const float2* xy, int n, float2* target, int* perm, hipStream_t s);
+// device into m (box: nv = n + 4, nhe = 6n + 10, nt = 2n + 2; torus
+// (m.periodic): nv = n, nhe = 6n, nt = 2n); target gets every vertex's own
+// position, perm (or null) the input index of each point vertex in order
+int gbuild(DMesh& m, const float2* xy, int n, float2* target, int* perm, hipStream_t s);
 
 // statistics slots kept on the device (unsigned long long)
 enum StatSlot : int {

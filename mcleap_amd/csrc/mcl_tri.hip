@@ -2651,9 +2651,9 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
 }
 
 // The star step; false in *done when it did not settle (the flip-based step
-// then ran instead, st->fallbacks = 1).
+// then ran instead, st->fallbacks = 1, its host time in *flip_ms).
 static int star_or_fallback(mcl_tri* t, const mcl_step_params_t* p, const float2* disp, mcl_stats_t* st, void* stream,
-                            bool* done) {
+                            bool* done, double* flip_ms = nullptr) {
     hipStream_t s = (hipStream_t)stream;
     auto t0 = std::chrono::steady_clock::now();
     stats_begin(t, s);
@@ -2669,9 +2669,14 @@ static int star_or_fallback(mcl_tri* t, const mcl_step_params_t* p, const float2
         }
         return MCL_OK;
     }
+    auto t1 = std::chrono::steady_clock::now();
     MCL_TRY(ensure_mesh(t, s));
     rc = step_impl(t, p, disp, st, stream);
     if (st) st->fallbacks = 1;
+    if (flip_ms && rc == MCL_OK) {
+        MCL_HIP(hipStreamSynchronize(s));
+        *flip_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+    }
     return rc;
 }
 
@@ -2709,9 +2714,10 @@ static int kinetic_step(mcl_tri* t, const mcl_step_params_t* p, const float2* di
         MCL_HIP(hipStreamSynchronize(s));
         auto t0 = std::chrono::steady_clock::now();
         int rc;
+        double flip_ms = -1.0;
         if (path == 1) {
             bool done = false;
-            rc = star_or_fallback(t, p, disp, st, stream, &done);
+            rc = star_or_fallback(t, p, disp, st, stream, &done, &flip_ms);
         } else {
             MCL_TRY(ensure_mesh(t, s));
             rc = step_impl(t, p, disp, st, stream);
@@ -2724,6 +2730,7 @@ static int kinetic_step(mcl_tri* t, const mcl_step_params_t* p, const float2* di
             t->path_probe = keeps ? std::min(2 * t->path_probe, 4096) : kPathProbe;
         }
         t->path_ms[path] = ms;
+        if (flip_ms >= 0.0) t->path_ms[0] = flip_ms;  // a fallback is also a flip-step measurement
         if (t->path_ms[0] > 0.0 && t->path_ms[1] > 0.0) {
             // a probe costs (slow - fast); after kPathProbe * slow / fast steps
             // of the fast path that is at most 1 / kPathProbe of the time
@@ -2766,18 +2773,19 @@ __global__ void __launch_bounds__(kThreads) k_off_target(DMesh m, const float2* 
 // The exact Delaunay triangulation of n device points, built on the GPU: a
 // valid triangulation of the points at row positions (mcl_gbuild.hip), then
 // flip-based kinetic steps to the points' own positions (repeated while a
-// vertex stalled).  Vertex numbering as mcl_hmesh_build's.
-extern "C" int mcl_tri_build(const void* d_xy, int n, void* d_perm, mcl_tri_t** out, void* stream) {
+// vertex stalled).  Box or torus; vertex numbering as mcl_hmesh_build's.
+extern "C" int mcl_tri_build(const void* d_xy, int n, int periodic, void* d_perm, mcl_tri_t** out, void* stream) {
     if (!out || !d_xy || n <= 0) return MCL_ERR_INPUT;
     if ((long long)n * 6 + 10 >= (1LL << 31)) return MCL_ERR_CAPACITY;
     hipStream_t s = (hipStream_t)stream;
     mcl_tri* t = nullptr;
-    MCL_TRY(tri_make(n + 4, 6 * n + 10, 2 * n + 2, 0, 4, &t));
+    const int base = periodic ? 0 : 4;
+    MCL_TRY(tri_make(n + base, 6 * n + (periodic ? 0 : 10), 2 * n + (periodic ? 0 : 2), periodic, base, &t));
     float2* target = nullptr;
     int* offc = nullptr;
-    int rc = dalloc(&target, (size_t)n + 4);
+    int rc = dalloc(&target, (size_t)n + base);
     if (rc == MCL_OK) rc = dalloc(&offc, 8);
-    if (rc == MCL_OK) rc = gbuild_box(t->m, (const float2*)d_xy, n, target, (int*)d_perm, s);
+    if (rc == MCL_OK) rc = gbuild(t->m, (const float2*)d_xy, n, target, (int*)d_perm, s);
     if (rc == MCL_OK) rc = validate(t, s, "device build");
     mcl_step_params_t p{};
     p.motion = kMotionTarget;

@@ -199,12 +199,13 @@ class DeviceTriangulation:
         self.last_stats = None
 
     @classmethod
-    def from_points(cls, points):
-        """HostTriangulation().addDelaunayPoints(points) + DeviceTriangulation(host)
-        built on the GPU (mcl_tri_build): box layout, the host build's vertex
-        numbering.  points: (n, 2) float32 numpy array or CUDA tensor.  The
-        returned object's h_triangulation holds the arrays of the result
-        (filled by transferToHost) and perm (vertex 4 + k = points[perm[k]])."""
+    def from_points(cls, points, periodic=False):
+        """HostTriangulation(periodic).addDelaunayPoints(points) +
+        DeviceTriangulation(host) built on the GPU (mcl_tri_build), with the
+        host build's vertex numbering.  points: (n, 2) float32 numpy array or
+        CUDA tensor.  The returned object's h_triangulation holds the arrays
+        of the result (filled by transferToHost) and perm (vertex base + k =
+        points[perm[k]], base 4 in a box, 0 on the torus)."""
         import torch
         if isinstance(points, torch.Tensor):
             d = points.detach().to(device="cuda", dtype=torch.float32).contiguous().reshape(-1, 2)
@@ -215,23 +216,25 @@ class DeviceTriangulation:
         self = cls.__new__(cls)
         self._h = _vp()
         torch.cuda.synchronize()
-        check(lib().mcl_tri_build(_dev_ptr(d), n, _dev_ptr(perm), ctypes.byref(self._h), None), "mcl_tri_build")
+        check(lib().mcl_tri_build(_dev_ptr(d), n, int(bool(periodic)), _dev_ptr(perm), ctypes.byref(self._h), None),
+              "mcl_tri_build")
+        base, extra_he, extra_t = (0, 0, 0) if periodic else (4, 10, 2)
         host = HostTriangulation.__new__(HostTriangulation)
-        host.periodic = False
-        host.m_pos = np.zeros((n + 4, 2), np.float32)
-        host.m_v = np.zeros((n + 4, 2), np.int32)
-        host.m_he = np.zeros((6 * n + 10, 4), np.int32)
-        host.m_t = np.zeros(2 * n + 2, np.int32)
+        host.periodic = bool(periodic)
+        host.m_pos = np.zeros((n + base, 2), np.float32)
+        host.m_v = np.zeros((n + base, 2), np.int32)
+        host.m_he = np.zeros((6 * n + extra_he, 4), np.int32)
+        host.m_t = np.zeros(2 * n + extra_t, np.int32)
         host.perm = perm[:n].cpu().numpy()
         host._points = np.zeros((0, 2), np.float32)
         self.h_triangulation = host
-        self.m_pos_size = self.m_v_size = n + 4
-        self.m_he_size = 6 * n + 10
-        self.m_t_size = 2 * n + 2
-        self.periodic = False
+        self.m_pos_size = self.m_v_size = n + base
+        self.m_he_size = 6 * n + extra_he
+        self.m_t_size = 2 * n + extra_t
+        self.periodic = bool(periodic)
         self.last_stats = None
         self.transferToHost()
-        host._points = host.m_pos[4:].copy()
+        host._points = host.m_pos[base:].copy()
         return self
 
     def __del__(self):

@@ -65,3 +65,31 @@ def test_device_build_rejects_duplicates():
     xy[57] = xy[12]
     with pytest.raises(M.MclError):
         M.DeviceTriangulation.from_points(xy)
+
+
+@pytest.mark.parametrize("n", [200, 20000])
+def test_device_build_torus_equals_host_build(oracle, n):
+    """the torus: rows closed into loops, strips into annuli; the same
+    vertex numbering, positions and canonical triangles as the host's torus
+    build, a valid periodic mesh with every edge exactly locally Delaunay"""
+    O = oracle
+    xy = unique_points(np.random.default_rng(n).random((n, 2)).astype(np.float32))
+    xy[xy >= 1.0] = 0.0
+    H = M.HostTriangulation(periodic=True)
+    q = xy.copy()
+    H.addDelaunayPoints(q)
+    D = M.DeviceTriangulation.from_points(xy, periodic=True)
+    h = D.h_triangulation
+    assert h.periodic and np.array_equal(h.perm, H.perm) and np.array_equal(h.m_pos, H.m_pos)
+    rc, msg = O.check_mesh(h.m_pos, h.m_v, h.m_he, h.m_t, periodic=True, require_ccw=True)
+    assert rc == 0, msg
+    assert O.count_non_delaunay(h.m_pos, h.m_he, periodic=True) == 0
+    assert np.array_equal(O.canonical_triangles(h.m_he, h.m_t), O.canonical_triangles(H.m_he, H.m_t))
+    st = D.step(M.MOTION_SHEAR, 2e-4, amp=1e-3, seed=3, step=0)
+    assert D.count_inverted() == 0 and D.count_non_delaunay() == 0, st
+
+
+def test_device_build_torus_too_few_points():
+    xy = np.random.default_rng(1).random((10, 2)).astype(np.float32)
+    with pytest.raises(M.MclError):
+        M.DeviceTriangulation.from_points(xy, periodic=True)
