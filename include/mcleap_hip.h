@@ -141,6 +141,17 @@ int mcl_tri_download(mcl_tri_t* t, float* pos, int* v2, int* he4, int* t1);
  * MCL_ERR_DUPLICATE for repeated points; MCL_ERR_UNRESOLVED when vertices
  * cannot reach their points (all points on one line). */
 int mcl_tri_build(const void* d_xy, int n, int periodic, void* d_perm, mcl_tri_t** out, void* stream);
+/* mcl_tri_build for an ensemble of ninst independent box triangulations held
+ * as one mesh (no counterpart in the reference, which builds one
+ * HostTriangulation per instance, Host_Triangulation.h:107-137, 296-300).
+ * d_xy holds the instances' points back to back on the device, counts (a
+ * host array) gives each instance's number of points (0 allowed, total > 0).
+ * Result: the exact Delaunay triangulation of every instance, with the
+ * vertex layout and numbering of mcl_hmesh_build_batch (corners of instance
+ * i at 4i..4i+3, then every instance's points in its own Hilbert order;
+ * vertex 4 ninst + k = point d_perm[k] of d_xy).  Half-edges and triangles
+ * of one instance are not contiguous. */
+int mcl_batch_build(const void* d_xy, const int* counts, int ninst, void* d_perm, mcl_tri_t** out, void* stream);
 /* 1 for a torus triangulation, 0 for a box (or ensemble of boxes) */
 int mcl_tri_is_periodic(const mcl_tri_t* t, int* periodic);
 

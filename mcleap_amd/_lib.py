@@ -132,6 +132,7 @@ SIGNATURES = {
     "mcl_bgrid_kernel_timer_read": [_p, _i, _p, _p],
     "mcl_tri_is_periodic": [_p, _p],
     "mcl_tri_build": [_p, _i, _i, _p, _pp, _p],
+    "mcl_batch_build": [_p, _p, _i, _p, _pp, _p],
     "mcl_state_save": [ctypes.c_char_p, _i, _p, _i, _p, _i, _p, _i, _p, _i],
     "mcl_state_info": [ctypes.c_char_p, _p, _p, _p, _p, _p],
     "mcl_state_load": [ctypes.c_char_p, _p, _p, _p, _p],

@@ -79,11 +79,13 @@ struct DMesh {
 // set when set is true); saved in state files so that a resumed run repeats
 int tri_counters(mcl_tri_t* t, int* round, int* stamp, bool set);
 
-// a valid triangulation of n points at their row positions, built on the
-// device into m (box: nv = n + 4, nhe = 6n + 10, nt = 2n + 2; torus
-// (m.periodic): nv = n, nhe = 6n, nt = 2n); target gets every vertex's own
-// position, perm (or null) the input index of each point vertex in order
-int gbuild(DMesh& m, const float2* xy, int n, float2* target, int* perm, hipStream_t s);
+// a valid triangulation of points at their row positions, built on the
+// device into m: ninst box instances of counts[i] points (nv = N + 4 ninst,
+// nhe = 6N + 10 ninst, nt = 2N + 2 ninst, every instance's corners first) or
+// one torus (m.periodic, ninst = 1: nv = N, nhe = 6N, nt = 2N); target gets
+// every vertex's own position, perm (or null) the input index of each point
+// vertex in order
+int gbuild(DMesh& m, const float2* xy, const int* counts, int ninst, float2* target, int* perm, hipStream_t s);
 
 // statistics slots kept on the device (unsigned long long)
 enum StatSlot : int {

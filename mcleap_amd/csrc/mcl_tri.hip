@@ -380,9 +380,10 @@ __device__ __forceinline__ float2 shifted(const DMesh& m, float2 a, float dx, fl
 // phase 2a: move the marked vertices back.  The first time in a step a vertex
 // is sent back it tries positions consistent with its neighbours' motion --
 // its previous position shifted by their mean displacement, then the
-// centroid of its neighbours -- and keeps the first at which its whole star
-// is strictly CCW; otherwise (and on any later take-back) it returns to its
-// previous position.  Soft moves happen at most once per vertex and step, so
+// centroid of its neighbours other than box corners (a corner would pull it
+// far out of the point cloud, where it would count as a corner itself) --
+// and keeps the first at which its whole star is strictly CCW; otherwise
+// (and on any later take-back) it returns to its previous position.  Soft moves happen at most once per vertex and step, so
 // the passes still end with every taken-back vertex at its previous position
 // in the worst case, which is valid.
 template <bool kF32>
@@ -400,7 +401,7 @@ __device__ __forceinline__ void revert_apply_body(const DMesh& m, const int* __r
             all_rev[agg_inc(c_all)] = v;
             // neighbour displacement and neighbour centroid, relative to a
             float sdx = 0.f, sdy = 0.f, scx = 0.f, scy = 0.f;
-            int deg = 0;
+            int deg = 0, ncen = 0;
             const int h0 = m.v_he[v];
             int h = h0;
             do {
@@ -408,15 +409,17 @@ __device__ __forceinline__ void revert_apply_body(const DMesh& m, const int* __r
                 const float2 pu = m.pos[u], au = acc[u];
                 float dx = pu.x - au.x, dy = pu.y - au.y, cx = pu.x - a.x, cy = pu.y - a.y;
                 if (m.periodic) { dx = min_image_f(dx); dy = min_image_f(dy); cx = min_image_f(cx); cy = min_image_f(cy); }
-                sdx += dx; sdy += dy; scx += cx; scy += cy;
+                sdx += dx; sdy += dy;
+                if (!is_corner(pu)) { scx += cx; scy += cy; ncen++; }
                 deg++;
                 h = m.he_next[h ^ 1];
             } while (h != h0 && deg < 64);
             const float inv = 1.0f / (float)deg;
             const float2 c1 = shifted(m, a, sdx * inv, sdy * inv);
-            const float2 c2 = shifted(m, a, scx * inv, scy * inv);
-            if (star_valid<kF32>(m, v, c1)) dest = c1;
-            else if (star_valid<kF32>(m, v, c2)) dest = c2;
+            const float icen = ncen ? 1.0f / (float)ncen : 0.0f;
+            const float2 c2 = shifted(m, a, scx * icen, scy * icen);
+            if (!is_corner(c1) && star_valid<kF32>(m, v, c1)) dest = c1;
+            else if (ncen && !is_corner(c2) && star_valid<kF32>(m, v, c2)) dest = c2;
         }
         m.pos[v] = dest;
         v_fexp[v] = min(v_fexp[v] + 1, 64);
@@ -2770,22 +2773,32 @@ __global__ void __launch_bounds__(kThreads) k_off_target(DMesh m, const float2* 
     if (c) atomicAdd(n, c);
 }
 
-// The exact Delaunay triangulation of n device points, built on the GPU: a
+// The exact Delaunay triangulation of device points, built on the GPU: a
 // valid triangulation of the points at row positions (mcl_gbuild.hip), then
 // flip-based kinetic steps to the points' own positions (repeated while a
-// vertex stalled).  Box or torus; vertex numbering as mcl_hmesh_build's.
-extern "C" int mcl_tri_build(const void* d_xy, int n, int periodic, void* d_perm, mcl_tri_t** out, void* stream) {
-    if (!out || !d_xy || n <= 0) return MCL_ERR_INPUT;
-    if ((long long)n * 6 + 10 >= (1LL << 31)) return MCL_ERR_CAPACITY;
+// vertex stalled).  One box or torus triangulation (ninst = 1), or an
+// ensemble of ninst box triangulations of counts[i] points each; vertex
+// numbering as mcl_hmesh_build's and mcl_hmesh_build_batch's.
+static int build_impl(const void* d_xy, const int* counts, int ninst, int periodic, void* d_perm, mcl_tri_t** out,
+                      void* stream) {
+    if (!out || !d_xy || !counts || ninst <= 0 || (periodic && ninst != 1)) return MCL_ERR_INPUT;
+    long long n = 0;
+    for (int i = 0; i < ninst; i++) {
+        if (counts[i] < 0) return MCL_ERR_INPUT;
+        n += counts[i];
+    }
+    if (n <= 0) return MCL_ERR_INPUT;
+    const long long base = periodic ? 0 : 4LL * ninst;
+    const long long nhe = 6 * n + (periodic ? 0 : 10LL * ninst);
+    if (nhe >= (1LL << 31)) return MCL_ERR_CAPACITY;
     hipStream_t s = (hipStream_t)stream;
     mcl_tri* t = nullptr;
-    const int base = periodic ? 0 : 4;
-    MCL_TRY(tri_make(n + base, 6 * n + (periodic ? 0 : 10), 2 * n + (periodic ? 0 : 2), periodic, base, &t));
+    MCL_TRY(tri_make((int)(n + base), (int)nhe, (int)(2 * n + (periodic ? 0 : 2LL * ninst)), periodic, (int)base, &t));
     float2* target = nullptr;
     int* offc = nullptr;
-    int rc = dalloc(&target, (size_t)n + base);
+    int rc = dalloc(&target, (size_t)(n + base));
     if (rc == MCL_OK) rc = dalloc(&offc, 8);
-    if (rc == MCL_OK) rc = gbuild(t->m, (const float2*)d_xy, n, target, (int*)d_perm, s);
+    if (rc == MCL_OK) rc = gbuild(t->m, (const float2*)d_xy, counts, ninst, target, (int*)d_perm, s);
     if (rc == MCL_OK) rc = validate(t, s, "device build");
     mcl_step_params_t p{};
     p.motion = kMotionTarget;
@@ -2831,6 +2844,15 @@ extern "C" int mcl_tri_build(const void* d_xy, int n, int periodic, void* d_perm
     if (rc != MCL_OK) { tri_free(t); delete t; return rc; }
     *out = t;
     return MCL_OK;
+}
+
+extern "C" int mcl_tri_build(const void* d_xy, int n, int periodic, void* d_perm, mcl_tri_t** out, void* stream) {
+    return build_impl(d_xy, &n, 1, periodic, d_perm, out, stream);
+}
+
+extern "C" int mcl_batch_build(const void* d_xy, const int* counts, int ninst, void* d_perm, mcl_tri_t** out,
+                               void* stream) {
+    return build_impl(d_xy, counts, ninst, 0, d_perm, out, stream);
 }
 
 extern "C" int mcl_tri_step(mcl_tri_t* t, const mcl_step_params_t* p, mcl_stats_t* st, void* stream) {

@@ -106,7 +106,8 @@ class HostTriangulation:
     @classmethod
     def load_state(cls, path):
         """a host triangulation holding a state file's arrays (mcl_state_load);
-        DeviceTriangulation(it) resumes on the GPU"""
+        DeviceTriangulation(it) continues from them with fresh round
+        counters (DeviceTriangulation.load_state restores those too)"""
         L = lib()
         q = [ctypes.c_int() for _ in range(5)]
         check(L.mcl_state_info(str(path).encode(), *[ctypes.byref(x) for x in q]), "mcl_state_info")
@@ -237,6 +238,52 @@ class DeviceTriangulation:
         host._points = host.m_pos[base:].copy()
         return self
 
+    @classmethod
+    def from_ensemble(cls, points):
+        """HostTriangulationBatch(points) + DeviceTriangulation(batch) built on
+        the GPU (mcl_batch_build).  points: a list of (n_i, 2) float32 arrays
+        (or CUDA tensors).  The result's h_triangulation is a
+        HostTriangulationBatch holding the arrays of the result (filled by
+        transferToHost), counts, vstart and perm as the host build's;
+        hestart and tstart are None (on the GPU an instance's half-edges and
+        triangles are not contiguous)."""
+        import torch
+        parts = []
+        for p in points:
+            if isinstance(p, torch.Tensor):
+                parts.append(p.detach().to(device="cuda", dtype=torch.float32).reshape(-1, 2))
+            else:
+                parts.append(torch.from_numpy(np.ascontiguousarray(np.asarray(p, np.float32).reshape(-1, 2))).cuda())
+        ninst = len(parts)
+        counts = np.array([int(p.shape[0]) for p in parts], np.int32)
+        d = torch.cat(parts).contiguous() if parts else torch.zeros((0, 2), dtype=torch.float32, device="cuda")
+        n = int(counts.sum())
+        perm = torch.zeros(max(n, 1), dtype=torch.int32, device="cuda")
+        self = cls.__new__(cls)
+        self._h = _vp()
+        torch.cuda.synchronize()
+        check(lib().mcl_batch_build(_dev_ptr(d), _np_ptr(counts), ninst, _dev_ptr(perm), ctypes.byref(self._h), None),
+              "mcl_batch_build")
+        nv = n + 4 * ninst
+        host = HostTriangulationBatch.__new__(HostTriangulationBatch)
+        host.ninst = ninst
+        host.counts = counts
+        host.vstart = (4 * ninst + np.concatenate([[0], np.cumsum(counts)])).astype(np.int32)
+        host.hestart = host.tstart = None
+        host.perm = perm[:n].cpu().numpy()
+        host.m_pos = np.zeros((nv, 2), np.float32)
+        host.m_v = np.zeros((nv, 2), np.int32)
+        host.m_he = np.zeros((6 * n + 10 * ninst, 4), np.int32)
+        host.m_t = np.zeros(2 * n + 2 * ninst, np.int32)
+        self.h_triangulation = host
+        self.m_pos_size = self.m_v_size = nv
+        self.m_he_size = 6 * n + 10 * ninst
+        self.m_t_size = 2 * n + 2 * ninst
+        self.periodic = False
+        self.last_stats = None
+        self.transferToHost()
+        return self
+
     def __del__(self):
         if getattr(self, "_h", None) and self._h.value:
             try:
@@ -258,8 +305,27 @@ class DeviceTriangulation:
     # ------------------------------------------------------------ transfers
     def save_state(self, path):
         """the current triangulation to a state file (mcl_tri_save);
-        HostTriangulation.load_state(path) reads it back"""
+        DeviceTriangulation.load_state(path) resumes it, and
+        HostTriangulation.load_state(path) reads its arrays back"""
         check(lib().mcl_tri_save(self._h, str(path).encode()), "mcl_tri_save")
+
+    @classmethod
+    def load_state(cls, path):
+        """a device triangulation resumed from a state file (mcl_tri_load):
+        the arrays and the flip rounds' counters of the saved object, so that
+        the same steps give the same positions as they would have given it.
+        h_triangulation holds the file's arrays."""
+        host = HostTriangulation.load_state(path)
+        self = cls.__new__(cls)
+        self._h = _vp()
+        check(lib().mcl_tri_load(str(path).encode(), ctypes.byref(self._h)), "mcl_tri_load")
+        self.h_triangulation = host
+        self.m_pos_size = self.m_v_size = len(host.m_pos)
+        self.m_he_size = len(host.m_he)
+        self.m_t_size = len(host.m_t)
+        self.periodic = host.periodic
+        self.last_stats = None
+        return self
 
     def transferToHost(self):
         h = self.h_triangulation

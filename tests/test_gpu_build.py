@@ -93,3 +93,63 @@ def test_device_build_torus_too_few_points():
     xy = np.random.default_rng(1).random((10, 2)).astype(np.float32)
     with pytest.raises(M.MclError):
         M.DeviceTriangulation.from_points(xy, periodic=True)
+
+
+def _batch_check_equal(O, xs):
+    """mcl_batch_build against mcl_hmesh_build_batch: the same vertex
+    numbering and positions, per instance the same canonical triangles; a
+    valid mesh per instance with every edge exactly locally Delaunay"""
+    from tests.test_gpu_batch import _check_batch, _submesh
+    B = M.HostTriangulationBatch(xs)
+    D = M.DeviceTriangulation.from_ensemble(xs)
+    h = D.h_triangulation
+    assert np.array_equal(h.vstart, B.vstart)
+    assert np.array_equal(h.perm, B.perm)
+    assert np.array_equal(h.m_pos, B.m_pos)
+    _check_batch(O, h, D, "device batch build")
+    pos, v, he, t = D.download()
+    for i in range(B.ninst):
+        if B.counts[i] == 0:
+            continue
+        _, _, h_d, t_d = _submesh(h, pos, v, he, t, i)
+        _, _, h_h, t_h = _submesh(B, B.m_pos, B.m_v, B.m_he, B.m_t, i)
+        assert np.array_equal(O.canonical_triangles(h_d, t_d), O.canonical_triangles(h_h, t_h)), i
+    return D, h
+
+
+def test_device_batch_build_equals_host_batch(oracle):
+    """24 instances, n = 0 .. 3000 (the empty one is the square's two
+    triangles), then two kinetic steps of the device-built ensemble"""
+    from mcleap_amd.workloads import ensemble_points
+    from tests.test_gpu_batch import _check_batch
+    from tests.test_host_batch import batch_sizes
+    xs = ensemble_points(seed=11, sizes=batch_sizes())
+    print("instances", len(xs), "points", sum(len(x) for x in xs), flush=True)
+    D, h = _batch_check_equal(oracle, xs)
+    D.set_option(M.OPT_STAR_MIN_VERTICES, 0)
+    for k in range(2):
+        st = D.step(M.MOTION_GAUSS, 2e-3, seed=5, step=k)
+        print("step", k, st, flush=True)
+        _check_batch(oracle, h, D, "device-built batch step %d" % k)
+
+
+def test_device_batch_build_c5_sizes(oracle):
+    """64 instances with the C5 size law (n = 256 .. 65536)"""
+    from mcleap_amd.workloads import ensemble_points
+    xs = ensemble_points(64, seed=5)
+    print("instances", len(xs), "points", sum(len(x) for x in xs), flush=True)
+    _batch_check_equal(oracle, xs)
+
+
+def test_device_batch_build_rejects_duplicates():
+    """a point repeated inside one instance is rejected; the same point in
+    two instances is not a duplicate"""
+    rng = np.random.default_rng(4)
+    a, b = rng.random((50, 2)).astype(np.float32), rng.random((80, 2)).astype(np.float32)
+    b[3] = a[7]
+    D = M.DeviceTriangulation.from_ensemble([a, b])
+    assert D.count_inverted() == 0 and D.count_non_delaunay() == 0
+    b[9] = b[3]
+    with pytest.raises(M.MclError) as e:
+        M.DeviceTriangulation.from_ensemble([a, b])
+    assert e.value.code == 3  # MCL_ERR_DUPLICATE

@@ -231,9 +231,10 @@ def test_path_choice_auto(oracle):
 
 @pytest.mark.parametrize("periodic", [True, False])
 def test_save_and_resume(oracle, tmp_path, periodic):
-    """a triangulation saved after 3 steps and loaded into a new device object
-    continues exactly as the original: same positions and the same canonical
-    triangles after 2 more steps of the same motion"""
+    """a triangulation saved after 3 steps and resumed by a new device object
+    (mcl_tri_load: arrays and round counters) continues exactly as the
+    original: same positions and the same canonical triangles after 2 more
+    steps of the same motion"""
     O = oracle
     xy = uniform_points(20000, 41)
     H = M.HostTriangulation(periodic=periodic)
@@ -249,7 +250,8 @@ def test_save_and_resume(oracle, tmp_path, periodic):
     dt.save_state(path)
     G = M.HostTriangulation.load_state(path)
     assert G.periodic == periodic
-    dt2 = M.DeviceTriangulation(G)
+    assert np.array_equal(M.DeviceTriangulation(G).download()[0], G.m_pos)
+    dt2 = M.DeviceTriangulation.load_state(path)
     dt2.set_option(M.OPT_STAR_MIN_VERTICES, 0)
     dt2.set_option(M.OPT_STAR_ENABLE, 1 if periodic else 0)
     p1, v1, h1, t1 = dt.download()
