@@ -412,20 +412,22 @@ __global__ void __launch_bounds__(kT) k_bfrnn(BGridParams b, const float2* __res
         const float rad2 = b.rad2[t];
         const int2 h = bgrid_cell(b.cs[t], nc, pi);
         int c = 0, w = kPass ? off[i] : 0;
+        // keys are row-major within an instance, so the up to 3 cells of one
+        // row are one contiguous range of the sorted points (same order as
+        // cell by cell)
+        const int x0 = max(h.x - 1, 0), x1 = min(h.x + 1, nc - 1);
         for (int y = max(h.y - 1, 0); y <= min(h.y + 1, nc - 1); y++) {
-            for (int x = max(h.x - 1, 0); x <= min(h.x + 1, nc - 1); x++) {
-                const int k = cb + y * nc + x;
-                const int e = start[k + 1];
-                for (int j = start[k]; j < e; j++) {
-                    if (j == i) continue;
-                    const float2 pj = spos[j];
-                    const float dx = __fsub_rn(pj.x, pi.x), dy = __fsub_rn(pj.y, pi.y);
-                    const float r2 = __fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy));
-                    if (r2 <= rad2) {
-                        if (kPass && w < cap) nbr[w] = j;
-                        w++;
-                        c++;
-                    }
+            const int k = cb + y * nc;
+            const int e = start[k + x1 + 1];
+            for (int j = start[k + x0]; j < e; j++) {
+                if (j == i) continue;
+                const float2 pj = spos[j];
+                const float dx = __fsub_rn(pj.x, pi.x), dy = __fsub_rn(pj.y, pi.y);
+                const float r2 = __fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy));
+                if (r2 <= rad2) {
+                    if (kPass && w < cap) nbr[w] = j;
+                    w++;
+                    c++;
                 }
             }
         }
