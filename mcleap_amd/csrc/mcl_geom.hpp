@@ -277,25 +277,34 @@ MCL_HD int sos_resolve(const InSigns& s, float2 a, float2 b, float2 c, float2 d)
     return 0;
 }
 
+// The FP32 stage of incircle(a, b, c, d) on the plane (no minimum image):
+// +-1, or 0 when its error bound cannot decide.
+MCL_HD int incircle_f32_plane(float2 a, float2 b, float2 c, float2 d) {
+#pragma clang fp contract(off)
+    const float e = 5.9604644775390625e-08f;
+    float adx = a.x - d.x, ady = a.y - d.y, bdx = b.x - d.x, bdy = b.y - d.y;
+    float cdx = c.x - d.x, cdy = c.y - d.y;
+    float bdxcdy = bdx * cdy, cdxbdy = cdx * bdy, alift = adx * adx + ady * ady;
+    float cdxady = cdx * ady, adxcdy = adx * cdy, blift = bdx * bdx + bdy * bdy;
+    float adxbdy = adx * bdy, bdxady = bdx * ady, clift = cdx * cdx + cdy * cdy;
+    float det = alift * (bdxcdy - cdxbdy) + blift * (cdxady - adxcdy) + clift * (adxbdy - bdxady);
+    float perm = (fabsf(bdxcdy) + fabsf(cdxbdy)) * alift + (fabsf(cdxady) + fabsf(adxcdy)) * blift +
+                 (fabsf(adxbdy) + fabsf(bdxady)) * clift;
+    float bound = (10.0f + 96.0f * e) * e * perm;
+    if (perm > 7.9e-31f && perm < 1e37f && (det > bound || -det > bound)) return det > 0 ? 1 : -1;
+    return 0;
+}
+
 // incircle(a, b, c, d) with the symbolic-perturbation tie rule.  Returns
 // +-1, or kNeedBig when kBig is false and only the big exact stage decides.
 template <bool kF32, bool kBig>
 MCL_HD int incircle(float2 a, float2 b, float2 c, float2 d, bool periodic, int* stage) {
 #pragma clang fp contract(off)
     if (kF32 && !periodic) {
-        const float e = 5.9604644775390625e-08f;
-        float adx = a.x - d.x, ady = a.y - d.y, bdx = b.x - d.x, bdy = b.y - d.y;
-        float cdx = c.x - d.x, cdy = c.y - d.y;
-        float bdxcdy = bdx * cdy, cdxbdy = cdx * bdy, alift = adx * adx + ady * ady;
-        float cdxady = cdx * ady, adxcdy = adx * cdy, blift = bdx * bdx + bdy * bdy;
-        float adxbdy = adx * bdy, bdxady = bdx * ady, clift = cdx * cdx + cdy * cdy;
-        float det = alift * (bdxcdy - cdxbdy) + blift * (cdxady - adxcdy) + clift * (adxbdy - bdxady);
-        float perm = (fabsf(bdxcdy) + fabsf(cdxbdy)) * alift + (fabsf(cdxady) + fabsf(adxcdy)) * blift +
-                     (fabsf(adxbdy) + fabsf(bdxady)) * clift;
-        float bound = (10.0f + 96.0f * e) * e * perm;
-        if (perm > 7.9e-31f && perm < 1e37f && (det > bound || -det > bound)) {
+        const int sg = incircle_f32_plane(a, b, c, d);
+        if (sg != 0) {
             *stage = kStageF32;
-            return det > 0 ? 1 : -1;
+            return sg;
         }
     }
     Dexp adx = diff_exact(a.x, d.x, 0, periodic), ady = diff_exact(a.y, d.y, 0, periodic);

@@ -152,4 +152,27 @@ struct KTimer {
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     }
 };
+// XCD-aware split of n items over the grid: the blocks of a launch are dealt
+// round-robin to the 8 XCDs, each with its own L2, so block b runs on XCD
+// b % 8.  Each XCD gets one contiguous share of the (Hilbert-ordered) items,
+// so that neighbouring items, which gather the same rows, meet in one L2.
+// Yields this block's first item, the end of its XCD's share and the stride
+// (all block-uniform); falls back to the plain grid-stride split when the
+// grid is not a multiple of 8.
+constexpr int kXcds = 8;
+__device__ __forceinline__ void xcd_split(int n, int per_block, int& first, int& end, int& stride) {
+    const int nb = gridDim.x;
+    if (nb % kXcds != 0) {
+        first = blockIdx.x * per_block;
+        end = n;
+        stride = nb * per_block;
+        return;
+    }
+    const int x = blockIdx.x % kXcds, k = blockIdx.x / kXcds, per_xcd = nb / kXcds;
+    const int share = (((n + kXcds - 1) / kXcds) + per_block - 1) / per_block * per_block;
+    const int lo = min(n, x * share);
+    first = lo + k * per_block;
+    end = min(n, lo + share);
+    stride = per_xcd * per_block;
+}
 }  // namespace mcl

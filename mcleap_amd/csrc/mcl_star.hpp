@@ -166,30 +166,6 @@ __device__ __forceinline__ int star_find(const int* __restrict__ nbr, int u, int
     return row_find(nbr + (size_t)u * kStarCap, du, x);
 }
 
-// XCD-aware split of n items over the grid: the blocks of a launch are dealt
-// round-robin to the 8 XCDs, each with its own L2, so block b runs on XCD
-// b % 8.  Each XCD gets one contiguous share of the (Hilbert-ordered) items,
-// so that neighbouring items, which gather the same rows, meet in one L2.
-// Yields this block's first item, the end of its XCD's share and the stride
-// (all block-uniform); falls back to the plain grid-stride split when the
-// grid is not a multiple of 8.
-constexpr int kXcds = 8;
-__device__ __forceinline__ void xcd_split(int n, int per_block, int& first, int& end, int& stride) {
-    const int nb = gridDim.x;
-    if (nb % kXcds != 0) {
-        first = blockIdx.x * per_block;
-        end = n;
-        stride = nb * per_block;
-        return;
-    }
-    const int x = blockIdx.x % kXcds, k = blockIdx.x / kXcds, per_xcd = nb / kXcds;
-    const int share = (((n + kXcds - 1) / kXcds) + per_block - 1) / per_block * per_block;
-    const int lo = min(n, x * share);
-    first = lo + k * per_block;
-    end = min(n, lo + share);
-    stride = per_xcd * per_block;
-}
-
 constexpr float kStarIccBound = 16.0f * 5.9604644775390625e-08f;  // the scan's incircle filter, x permanent
 
 constexpr int kStarMaxDepth = 5;  // deepest ring of the previous mesh searched

@@ -271,10 +271,12 @@ __global__ void __launch_bounds__(kThreads) k_check_all(DMesh m, int* __restrict
     if (threadIdx.x == 0) s_tile[0] = 0;
     __syncthreads();
     unsigned loc[4] = {0, 0, 0, 0};
-    for (int base = blockIdx.x * blockDim.x; base < m.nt; base += gridDim.x * blockDim.x) {
+    int first, end, stride;
+    xcd_split(m.nt, blockDim.x, first, end, stride);
+    for (int base = first; base < end; base += stride) {
         const int t = base + threadIdx.x;
         bool is_bad = false;
-        if (t < m.nt) {
+        if (t < end) {
             int a, b, c, st;
             tri_verts(m, t, a, b, c);
             const int s = orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st);
@@ -1045,8 +1047,12 @@ __global__ void __launch_bounds__(1024) k_reloc_tail(DMesh m, int* __restrict__ 
 }
 
 // ------------------------------------------------------------ flip rounds
-// kind 0 = Lawson (exact incircle), kind 1 = B/C untangle (reference steps)
-template <int kKind, bool kAll, bool kF32, bool kBig>
+// kind 0 = Lawson (exact incircle), kind 1 = B/C untangle (reference steps).
+// kFilter (Lawson on a box mesh): the FP32 filter alone, which keeps the
+// kernel small (more waves in flight for the gathers of a round over every
+// edge); an edge it cannot decide is queued for the next round, whose kernel
+// has the exact stages.
+template <int kKind, bool kAll, bool kF32, bool kBig, bool kFilter = false>
 __global__ void __launch_bounds__(kThreads) k_round_test(DMesh m, const int* __restrict__ cand, int* __restrict__ cnt,
                                                          int round, int n_all, int* __restrict__ wants,
                                                          unsigned long long* __restrict__ claim, int* __restrict__ e_stamp,
@@ -1059,21 +1065,29 @@ __global__ void __launch_bounds__(kThreads) k_round_test(DMesh m, const int* __r
     if (threadIdx.x == 0) s_tile[0] = 0;
     __syncthreads();
     unsigned loc[4] = {0, 0, 0, 0};
-    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    int first, end, stride;
+    xcd_split(n, blockDim.x, first, end, stride);
+    for (int base = first; base < end; base += stride) {
         const int i = base + threadIdx.x;
         bool want = false;
         int e = -1;
         do {  // one item; `break` leaves it without a flip request
-        if (i >= n) break;
+        if (i >= end) break;
         e = kAll ? i : cand[i];
         const int2 tt = reinterpret_cast<const int2*>(m.he_t)[e];
         if (tt.x < 0 || tt.y < 0) break;
         const int2 vv = reinterpret_cast<const int2*>(m.he_v)[e];
         const int2 oo = reinterpret_cast<const int2*>(m.he_op)[e];
         if (kKind == 0) {
-            int st;
+            int st = kStageF32;
             // inCircle(org(2e), op(2e+1), org(2e+1), op(2e)) > 0 (kernels.cuh:287-292)
-            int s = incircle<kF32, kBig>(m.pos[vv.x], m.pos[oo.y], m.pos[vv.y], m.pos[oo.x], m.periodic, &st);
+            int s;
+            if (kFilter) {
+                s = incircle_f32_plane(m.pos[vv.x], m.pos[oo.y], m.pos[vv.y], m.pos[oo.x]);
+                if (s == 0) s = kNeedBig;
+            } else {
+                s = incircle<kF32, kBig>(m.pos[vv.x], m.pos[oo.y], m.pos[vv.y], m.pos[oo.x], m.periodic, &st);
+            }
             if (s == kNeedBig) {
                 if (atomicExch(&e_stamp[e], round + 1) != round + 1)
                     cand_next[agg_inc(&cnt[kCnCand + (r3 + 1) % 3])] = e;
@@ -1949,8 +1963,12 @@ static int run_rounds(mcl_tri* t, bool all, int variant, int max_rounds, hipStre
         const int stat = kKind == 0 ? kStFlips : kStUntangleFlips;
         if (first) {
             t->timers[0].mark(s);
-            hipLaunchKernelGGL((k_round_test<kKind, true, true, false>), dim3(gb), dim3(kThreads), 0, s, m, cand, t->cnt, R,
-                               m.ne, t->wants, t->claim, t->e_stamp, cand_next, t->stats);
+            if (kKind == 0 && !m.periodic)
+                hipLaunchKernelGGL((k_round_test<kKind, true, true, false, true>), dim3(gb), dim3(kThreads), 0, s, m, cand,
+                                   t->cnt, R, m.ne, t->wants, t->claim, t->e_stamp, cand_next, t->stats);
+            else
+                hipLaunchKernelGGL((k_round_test<kKind, true, true, false>), dim3(gb), dim3(kThreads), 0, s, m, cand,
+                                   t->cnt, R, m.ne, t->wants, t->claim, t->e_stamp, cand_next, t->stats);
             t->timers[0].mark(s);
         }
         else
