@@ -1,0 +1,67 @@
+"""Parity at the workloads' full sizes (1M-point torus under shear + noise,
+1M-point near-degenerate lattice, 4.2M clustered points under Brownian
+motion of half a spacing), through properties that do not depend on size:
+after the steps, the downloaded mesh is a valid triangulation (half-edge
+invariants, Euler's formula, every triangle strictly CCW) of the downloaded
+positions in which every edge passes the oracle's exact incircle test with the
+symbolic tie rule -- which makes it the unique Delaunay triangulation of those
+positions.  On the torus it is also compared, as a canonical triangle set,
+with the oracle's own from-scratch periodic triangulation of the positions.
+The initial triangulations are built on the GPU (mcl_tri_build)."""
+import numpy as np
+import pytest
+
+import mcleap_amd as M
+from mcleap_amd.workloads import CONFIGS, initial_points
+
+pytestmark = pytest.mark.gpu
+
+
+def _steps(dt, name, k):
+    cfg = CONFIGS[name]
+    motion = M.MOTION_SHEAR if cfg["motion"] == 1 else M.MOTION_GAUSS
+    paths = []
+    for s in range(k):
+        st = dt.step(motion, cfg["sigma"], amp=cfg["amp"], seed=3, step=s)
+        print(name, "step", s, {x: st[x] for x in ("rounds", "flips", "relocated", "stalled", "fallbacks", "ms")},
+              flush=True)
+        assert dt.count_inverted() == 0 and dt.count_non_delaunay() == 0
+        paths.append("star" if st["rounds"] == 0 and st["fallbacks"] == 0 else "flip")
+    return paths
+
+
+def _check(O, dt, periodic):
+    pos, v, he, t = dt.download()
+    rc, msg = O.check_mesh(pos, v, he, t, periodic=periodic, require_ccw=True)
+    assert rc == 0, msg
+    assert O.count_non_delaunay(pos, he, periodic=periodic) == 0
+    return pos, he, t
+
+
+def test_c2_torus_1m(oracle):
+    O = oracle
+    xy = initial_points("c2")
+    assert len(xy) == 1 << 20
+    dt = M.DeviceTriangulation.from_points(xy, periodic=True)
+    paths = _steps(dt, "c2", 4)
+    assert "star" in paths, paths
+    pos, he, t = _check(O, dt, True)
+    moved = np.any(pos != dt.h_triangulation._points, axis=1).mean()
+    assert moved > 0.99, moved
+    ref = O.torus_delaunay(pos)
+    assert np.array_equal(O.canonical_triangles(he, t), O.canonical_from_list(ref))
+
+
+def test_c4_lattice_1m(oracle):
+    xy = initial_points("c4")
+    dt = M.DeviceTriangulation.from_points(xy)
+    _steps(dt, "c4", 3)
+    _check(oracle, dt, False)
+
+
+def test_c3_clustered_4m(oracle):
+    xy = initial_points("c3")
+    assert len(xy) > 4_000_000
+    dt = M.DeviceTriangulation.from_points(xy)
+    _steps(dt, "c3", 2)
+    _check(oracle, dt, False)
