@@ -96,6 +96,33 @@ def test_torus_grid(oracle):
     assert np.array_equal(counts, bc)
 
 
+@pytest.mark.parametrize("flags", [M.GRID_MORTON | M.GRID_PERIODIC, M.GRID_MORTON])
+def test_async_csr_equals_counted(oracle, flags):
+    """The asynchronous CSR form (no host read-back) gives the same offsets
+    and neighbour lists as the counted form, also with a dense cluster of
+    points that have hundreds of neighbours each."""
+    import torch
+    rng = np.random.default_rng(8)
+    n0 = 20000
+    r = 2.0 / np.sqrt(n0)
+    xy = np.concatenate([rng.random((n0, 2)), 0.5 + 0.3 * r * (rng.random((300, 2)) - 0.5)]).astype(np.float32)
+    xy[xy >= 1.0] = 0.0
+    nc = int(np.floor(1.0 / r))
+    cs = 1.0 / nc
+    g, pos = _run_grid(xy, (0, 0), (cs, cs), (nc, nc), r, flags)
+    n = g.numP
+    off = torch.zeros(n + 1, dtype=torch.int32, device="cuda")
+    tot = g.frnn_csr(pos, r, off)
+    nbr = torch.zeros(tot, dtype=torch.int32, device="cuda")
+    assert g.frnn_csr(pos, r, off, nbr) == tot
+    assert (off[1:] - off[:-1]).max().item() > 200
+    off2 = torch.zeros(n + 1, dtype=torch.int32, device="cuda")
+    nbr2 = torch.full((tot,), -1, dtype=torch.int32, device="cuda")
+    g.frnn_csr_async(pos, r, off2, nbr2)
+    torch.cuda.synchronize()
+    assert torch.equal(off, off2) and torch.equal(nbr, nbr2)
+
+
 def test_update_permutes_in_place(oracle):
     import torch
     rng = np.random.default_rng(3)
