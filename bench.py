@@ -381,12 +381,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 coordinates, exact predicates (fp32 -> fp64 -> expansion)",
+        "dtype": "fp32",
         "data": "synthetic (seeded uniform points, Philox motion on the GPU)",
         "config": {"workload": a.workload, "n_points": n, "domain": cfg["domain"], "sigma": cfg["sigma"],
                    "instances": int(H.ninst) if batch else 1,
                    "amp": cfg["amp"], "frnn_radius": radius, "variant": a.variant,
-                   "build": a.build, "build_s": round(build_s, 3)},
+                   "build": a.build, "build_s": round(build_s, 3),
+                   "predicates": "exact: fp32 filter -> fp64 filter -> expansion arithmetic"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "algo_bytes_per_launch": algo, "avg_launch_ms": avg_ms, "launches": launches},
