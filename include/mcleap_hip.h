@@ -315,10 +315,15 @@ int mcl_grid_mean_occupancy(mcl_grid_t* g, float* out, void* stream);
  *   the number of launches dominates, use the flip-based step.  0 = always.
  * MCL_OPT_STAR_ENABLE: 0 = never use the star recomputation; 1 = every step
  *   (after a step where it did not settle, skipped for 1, 2, 4 ... 64 steps);
- *   2 (default) = the path whose last step was faster, the other one timed
- *   again every 32 steps. */
+ *   2 (default) = the path whose last step was faster, the slower one timed
+ *   again after 64 x (slow / fast) steps, twice as late each time it stays
+ *   slower (up to 4096 steps).
+ * MCL_OPT_STAR_GIVE_UP: the star step falls back to the flip-based step at
+ *   once when its first consistency check lists more than this percentage of
+ *   the vertices (1..100, default 40; 100 = always run the retry passes). */
 #define MCL_OPT_STAR_MIN_VERTICES 1
 #define MCL_OPT_STAR_ENABLE 2
+#define MCL_OPT_STAR_GIVE_UP 3
 int mcl_tri_set_option(mcl_tri_t* t, int option, long long value);
 
 /* ------------------------------------------------------- kernel timers */

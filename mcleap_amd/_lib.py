@@ -28,6 +28,7 @@ GRID_MORTON = 1
 GRID_PERIODIC = 2
 OPT_STAR_MIN_VERTICES = 1
 OPT_STAR_ENABLE = 2
+OPT_STAR_GIVE_UP = 3
 
 
 class MclError(RuntimeError):

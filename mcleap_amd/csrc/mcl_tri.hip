@@ -1549,6 +1549,7 @@ struct mcl_tri {
     bool mesh_stale = false;                // half-edge arrays not yet rebuilt from the stars
     int star_min_vertices = kStarMinVertices;  // MCL_OPT_STAR_MIN_VERTICES
     int star_enable = 2;                       // MCL_OPT_STAR_ENABLE: 0 off, 1 every step, 2 the faster path
+    int star_give_up = 40;                     // MCL_OPT_STAR_GIVE_UP: percent of vertices the first check may list
     double path_ms[2] = {-1.0, -1.0};          // last step time of the flip-based (0) and star (1) paths
     int path_since = 0;                        // steps since the slower path was last measured
     int path_probe = kPathProbe;               // steps between such measurements
@@ -2597,7 +2598,8 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         return MCL_OK;
     };
     check(true, false);
-    hipLaunchKernelGGL(k_star_give_up, dim3(1), dim3(64), 0, s, t->s_aux, (int)(0.4 * m.nv));
+    hipLaunchKernelGGL(k_star_give_up, dim3(1), dim3(64), 0, s, t->s_aux,
+                       (int)std::min<long long>((long long)t->star_give_up * m.nv / 100, m.nv));
     MCL_HIP(hipGetLastError());
     if (trace) MCL_TRY(read_aux(0));
     // Retries run without reading the counts back until kStarSyncFrom passes
@@ -2983,6 +2985,10 @@ extern "C" int mcl_tri_set_option(mcl_tri_t* t, int option, long long value) {
             t->path_since = 0;
             t->path_probe = kPathProbe;
             t->star_skip = t->star_backoff = 0;
+            return MCL_OK;
+        case MCL_OPT_STAR_GIVE_UP:
+            if (value < 1 || value > 100) return MCL_ERR_INPUT;
+            t->star_give_up = (int)value;
             return MCL_OK;
         default:
             return MCL_ERR_INPUT;

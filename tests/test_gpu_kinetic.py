@@ -227,6 +227,10 @@ def test_path_choice_auto(oracle):
     assert len(set(paths[2:])) == 1  # the faster path, kept
     with pytest.raises(M.MclError):
         dt.set_option(M.OPT_STAR_ENABLE, 3)
+    dt.set_option(M.OPT_STAR_GIVE_UP, 100)
+    for bad in (0, 101):
+        with pytest.raises(M.MclError):
+            dt.set_option(M.OPT_STAR_GIVE_UP, bad)
 
 
 @pytest.mark.parametrize("periodic", [True, False])
