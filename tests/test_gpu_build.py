@@ -153,3 +153,20 @@ def test_device_batch_build_rejects_duplicates():
     with pytest.raises(M.MclError) as e:
         M.DeviceTriangulation.from_ensemble([a, b])
     assert e.value.code == 3  # MCL_ERR_DUPLICATE
+
+
+@pytest.mark.parametrize("case", ["one_point_instances", "empty_first_and_last", "tiny_xrange", "negative_coords"])
+def test_device_batch_build_edge_cases(oracle, case):
+    """instances of one point each; empty first and last instances; an
+    instance whose x range is 1e-6 wide (rows spread over it); coordinates
+    outside [0, 1) -- each equal to the host batch build"""
+    rng = np.random.default_rng(3)
+    xs = {
+        "one_point_instances": lambda: [rng.random((1, 2)) for _ in range(7)],
+        "empty_first_and_last": lambda: [np.zeros((0, 2)), rng.random((50, 2)), np.zeros((0, 2))],
+        "tiny_xrange": lambda: [np.stack([0.5 + 1e-6 * rng.random(40), rng.random(40)], 1), rng.random((30, 2))],
+        "negative_coords": lambda: [(rng.random((200, 2)) * 2 - 1) * 1000, rng.random((100, 2))],
+    }[case]()
+    xs = [unique_points(np.asarray(x, np.float32)) for x in xs]
+    print(case, [len(x) for x in xs], flush=True)
+    _batch_check_equal(oracle, xs)
