@@ -321,7 +321,7 @@ def main():
             pmc = json.load(f).get(a.workload, {}).get(dom)
     except (OSError, ValueError):
         pmc = None
-    if pmc and (a.n == 0):
+    if pmc and a.n == 0 and a.ninst == 0:
         traffic = pmc["read_bytes"] + pmc["write_bytes"]
         if pmc.get("valu_inst") and avg_ms > 0:
             rate = pmc["valu_inst"] / (avg_ms * 1e-3)
