@@ -209,7 +209,8 @@ __device__ __forceinline__ void flip_soa(const DMesh& m, int e, int& he2, int& h
 }
 
 // -------------------------------------------------------------- movement
-constexpr int kMotionTarget = 1000;  // internal: the displacement array holds absolute targets
+constexpr int kMotionTarget = 1000;      // internal: the displacement array holds absolute targets
+constexpr int kMotionTargetHalf = 1001;  // internal: half way from the current position to those targets
 // synthetic displacement of every non-corner vertex; stores the accepted
 // position (acc), the target (tgt) and moves the vertex to its target
 __global__ void __launch_bounds__(kThreads) k_move_motion(DMesh m, float2* __restrict__ acc, float2* __restrict__ tgt,
@@ -222,6 +223,10 @@ __global__ void __launch_bounds__(kThreads) k_move_motion(DMesh m, float2* __res
         float2 t;
         if (disp && p.motion == kMotionTarget) {  // absolute targets (the device build)
             t = disp[v];
+        } else if (disp && p.motion == kMotionTargetHalf) {
+#pragma clang fp contract(off)
+            const float2 d = disp[v];
+            t = (d.x == q.x && d.y == q.y) ? q : make_float2(q.x + 0.5f * (d.x - q.x), q.y + 0.5f * (d.y - q.y));
         } else if (disp) {  // caller displacement (movePoints semantics); the torus wraps
             const float2 d = disp[v];
             t = make_float2(q.x + d.x, q.y + d.y);
@@ -1321,8 +1326,14 @@ __global__ void k_set_u64(unsigned long long* p, int n, unsigned long long v) {
 
 // ---------------------------------------------------------------- one ring
 // out-degree by CW rotation he <- next(he^1) from v.he (kernels.cuh:67-76)
+// (both walks split over the XCDs: each L2 serves one contiguous range of
+// vertices and the half-edges around them)
 __global__ void __launch_bounds__(kThreads) k_degree(DMesh m, int* __restrict__ deg) {
-    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
+    int first, last, stride;
+    xcd_split(m.nv, blockDim.x, first, last, stride);
+    for (int base = first; base < last; base += stride) {
+        const int v = base + threadIdx.x;
+        if (v >= last) break;
         int h0 = m.v_he[v], h = h0, c = 0;
         do {
             c++;
@@ -1332,7 +1343,11 @@ __global__ void __launch_bounds__(kThreads) k_degree(DMesh m, int* __restrict__ 
     }
 }
 __global__ void __launch_bounds__(kThreads) k_ring_csr(DMesh m, const int* __restrict__ off, int* __restrict__ nbr, int cap) {
-    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
+    int first, last, stride;
+    xcd_split(m.nv, blockDim.x, first, last, stride);
+    for (int base = first; base < last; base += stride) {
+        const int v = base + threadIdx.x;
+        if (v >= last) break;
         int h0 = m.v_he[v], h = h0, k = off[v], end = off[v + 1];
         do {
             if (k < cap && k < end) nbr[k] = m.he_v[h ^ 1];
@@ -2821,9 +2836,12 @@ static int build_impl(const void* d_xy, const int* counts, int ninst, int period
     if (rc == MCL_OK) rc = gbuild(t->m, (const float2*)d_xy, counts, ninst, target, (int*)d_perm, s);
     if (rc == MCL_OK) rc = validate(t, s, "device build");
     mcl_step_params_t p{};
-    p.motion = kMotionTarget;
     p.variant = 0;
     for (int it = 0; rc == MCL_OK; it++) {
+        // straight to the targets; after 8 passes, every other pass goes half
+        // way (a vertex that keeps stalling on the straight move meets other
+        // links on the way)
+        p.motion = (it >= 8 && (it & 1) == 0) ? kMotionTargetHalf : kMotionTarget;
         rc = step_impl(t, &p, target, nullptr, stream);
         if (rc != MCL_OK) break;
         hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, offc, 6, 0);
@@ -2857,7 +2875,7 @@ static int build_impl(const void* d_xy, const int* counts, int ninst, int period
             }
         }
         if (off == 0) break;
-        if (it == 15) rc = MCL_ERR_UNRESOLVED;
+        if (it == 31) rc = MCL_ERR_UNRESOLVED;
     }
     if (target) (void)hipFree(target);
     if (offc) (void)hipFree(offc);
