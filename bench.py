@@ -208,9 +208,15 @@ def main():
         t0 = time.time()
         if a.build == "device":
             # every instance's exact triangulation built on the GPU at once (same result and numbering)
-            dt = M.DeviceTriangulation.from_ensemble(inst_pts)
-            torch.cuda.synchronize()
-            H = dt.h_triangulation
+            try:
+                dt = M.DeviceTriangulation.from_ensemble(inst_pts)
+                torch.cuda.synchronize()
+                H = dt.h_triangulation
+            except M.MclError as e:  # the steps are what is measured: build on the host instead
+                print("[bench] device build failed (%s): building on the host" % e, file=sys.stderr, flush=True)
+                a.build = "host (device build failed)"
+                dt = None
+                H = M.HostTriangulationBatch(inst_pts)
         else:
             H = M.HostTriangulationBatch(inst_pts)
         build_s = time.time() - t0
@@ -223,9 +229,14 @@ def main():
         dt = None
         if a.build == "device":
             # the exact triangulation built on the GPU (same result and vertex numbering as the host build)
-            dt = M.DeviceTriangulation.from_points(xy, periodic=periodic)
-            torch.cuda.synchronize()
-        else:
+            try:
+                dt = M.DeviceTriangulation.from_points(xy, periodic=periodic)
+                torch.cuda.synchronize()
+            except M.MclError as e:  # the steps are what is measured: build on the host instead
+                print("[bench] device build failed (%s): building on the host" % e, file=sys.stderr, flush=True)
+                a.build = "host (device build failed)"
+                dt = None
+        if dt is None:
             H = M.HostTriangulation(periodic=periodic)
             H.addDelaunayPoints(xy)
         build_s = time.time() - t0
