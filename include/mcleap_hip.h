@@ -55,6 +55,8 @@ typedef struct {
     long long relocated_grid;  /* of `relocated`, those done by the full-grid apply launches (kernel timer 3) */
     long long star_retries;    /* torus step: vertices whose star was recomputed from the 3-ring */
     long long fallbacks;       /* torus step: 1 when the stars did not settle and the flip-based step ran */
+    long long duplicates;      /* vertices left at their previous position because another vertex
+                                  holds exactly the same target (not counted in `stalled`) */
 } mcl_stats_t;
 
 /* =========================================================================
@@ -230,8 +232,9 @@ typedef struct {
  * are taken back, and those vertices are removed and re-inserted at their
  * targets), and flip back to the exact Delaunay triangulation.  Replaces the
  * move + untangle2 + delonize2 sequence of moving_device_mesh.cu:119-155.
- * st->stalled counts vertices left short of their target (a target on an
- * existing vertex, or a degenerate star). */
+ * st->stalled counts vertices left short of their target for another reason
+ * than a duplicate target (st->duplicates): a relocation whose target stays
+ * occupied by a vertex that does not leave, or a star above 64 triangles. */
 int mcl_tri_step(mcl_tri_t* t, const mcl_step_params_t* p, mcl_stats_t* st, void* stream);
 /* movePoints(d) followed by untangle + delonize (moving_device_mesh.cu:119-155)
  * as one call that always ends on a valid mesh that is exactly Delaunay:
@@ -239,6 +242,16 @@ int mcl_tri_step(mcl_tri_t* t, const mcl_step_params_t* p, mcl_stats_t* st, void
  * array d (float2 per vertex, corners ignored; on the torus the targets
  * wrap into [0,1)).  Box targets must stay inside the corner square. */
 int mcl_tri_move_legalize(mcl_tri_t* t, const void* d_disp, int variant, mcl_stats_t* st, void* stream);
+/* Both step calls return MCL_ERR_UNRESOLVED (after finishing the step) when a
+ * vertex could not reach its target for any reason other than a duplicate
+ * target (st->stalled > 0); the mesh is then still valid and exactly
+ * Delaunay for the positions it holds. */
+/* the targets of the last step (device float2 per vertex; corners hold their
+ * own positions): after a step every vertex is at its target except the
+ * st->duplicates vertices that share a target with another vertex (no
+ * counterpart in the reference, whose moves have no targets,
+ * kernels.cuh:38-50) */
+void* mcl_tri_target_ptr(mcl_tri_t* t);
 
 /* ------------------------------------------------------------- queries */
 /* oneRing<K>(int*) (layout 0, SoA: ring[(c+1)*nv + i], count in ring[i]) and

@@ -54,6 +54,7 @@ class Stats(ctypes.Structure):
         ("relocated_grid", ctypes.c_longlong),
         ("star_retries", ctypes.c_longlong),
         ("fallbacks", ctypes.c_longlong),
+        ("duplicates", ctypes.c_longlong),
     ]
 
     def as_dict(self):
@@ -95,6 +96,7 @@ SIGNATURES = {
     "mcl_tri_download": [_p, _p, _p, _p, _p],
     "mcl_tri_upload": [_p, _p, _p, _p, _p],
     "mcl_tri_pos_ptr": [_p],
+    "mcl_tri_target_ptr": [_p],
     "mcl_tri_move": [_p, _p, _p],
     "mcl_tri_integrate": [_p, _p, _p, _p],
     "mcl_tri_delonize": [_p, _i, _i, _p, _p],
@@ -142,7 +144,7 @@ SIGNATURES = {
     "mcl_version": [],
     "mcl_device_info": [_p, _i, _p],
 }
-RESTYPES = {"mcl_tri_pos_ptr": _p, "mcl_grid_sorted_ptr": _p, "mcl_bgrid_sorted_ptr": _p,
+RESTYPES = {"mcl_tri_pos_ptr": _p, "mcl_tri_target_ptr": _p, "mcl_grid_sorted_ptr": _p, "mcl_bgrid_sorted_ptr": _p,
             "mcl_version": ctypes.c_char_p}
 
 

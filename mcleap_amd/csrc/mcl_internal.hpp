@@ -94,7 +94,7 @@ enum StatSlot : int {
     kStIncircle0, kStIncircle1, kStIncircle2, kStIncircle3,
     kStRelocated, kStRelocFailed,
     kStRelWalk, kStRelStar, kStRelStuck, kStRelPutBack,  // reasons of the failures
-    kStRelDirect,  // relocations done by moving the vertex inside its own star
+    kStRelDup,     // vertices left where they were because another vertex holds the same target
     kStRelocGrid,  // relocations done by the full-grid apply launches (timed by timer 3)
     kStCount
 };

@@ -217,8 +217,10 @@ __device__ __forceinline__ int star_orient_f32(float4 A, float4 B) {
 // counted in fail[10]) for the full kernel; other failures are counted in
 // fail[0] and, by reason, in fail[4] (too few or too many candidates),
 // fail[5] (undecided test), fail[6] (hull not closed) or fail[7] (v not
-// inside).  acc: the positions at the start of the step (a vertex found on
-// top of another goes back there; null: it just fails).
+// inside).  acc: the positions at the start of the step (of vertices found
+// on one point, all but the smallest id go back there, counted as
+// duplicates when that vertex is at its own target tgt, else as stalled;
+// null: they just fail).
 //
 // kL lanes handle one vertex (64: one per wave, up to 128 candidates; 32: two
 // per wave, up to 64 candidates, which holds nearly every 2-ring and halves
@@ -236,7 +238,8 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                                               const int* __restrict__ deg_b, const int* __restrict__ nbr_b,
                                               int* __restrict__ deg_out, int* __restrict__ nbr_out,
                                               int* __restrict__ fail, int* __restrict__ heavy,
-                                              const float2* __restrict__ acc, unsigned long long* __restrict__ stats) {
+                                              const float2* __restrict__ acc, const float2* __restrict__ tgt,
+                                              unsigned long long* __restrict__ stats) {
     static_assert(kL == 32 || kL == 64, "lanes per vertex");
     constexpr int kV = 256 / kL;                    // vertices per block iteration
     constexpr int kC = kL == 64 ? kCandMax : 64;    // candidates per vertex
@@ -254,7 +257,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
     const bool periodic = m.periodic != 0;
     const int max_depth = depth < 2 ? 2 : (depth > kStarMaxDepth ? kStarMaxDepth : depth);
     unsigned loc[4] = {0, 0, 0, 0};
-    unsigned stalled = 0;
+    unsigned stalled[2] = {0, 0};  // stalled, duplicates
     int first, end, stride;
     xcd_split(n, kV, first, end, stride);
     for (int base = first; base < end; base += stride) {
@@ -383,6 +386,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
         // the inversion meaningless.
         int best = kCandMax;
         bool same = false;
+        int same_lo = 0x7fffffff;  // smallest id of a candidate on top of v (full kernel)
         if (kFast) {
             float dist = 3.4e38f;
             for (int j = lane; ok && j < kc; j += kL) {
@@ -400,7 +404,10 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
             double dist = 1e300;
             for (int j = lane; ok && j < kc; j += kL) {
                 const float2 pj = s_pos[g][j];
-                same |= pj.x == pv.x && pj.y == pv.y;
+                if (pj.x == pv.x && pj.y == pv.y) {
+                    same = true;
+                    same_lo = min(same_lo, s_cand[g][j]);
+                }
                 const double dx = dexp_value(diff_exact(pj.x, pv.x, 0, periodic));
                 const double dy = dexp_value(diff_exact(pj.y, pv.y, 0, periodic));
                 const double dd = dx * dx + dy * dy;
@@ -412,15 +419,19 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                 if (od < dist || (od == dist && ob < best)) { dist = od; best = ob; }
             }
         }
+        if (!kFast)
+            for (int off = kL / 2; off >= 1; off >>= 1) same_lo = min(same_lo, __shfl_xor(same_lo, off));
         if ((__ballot(same) & gmask) && ok) {
             // v landed exactly on another vertex (points clamped into the same
-            // corner of the box): v keeps its previous position this step and
-            // its star is computed again in the next pass
+            // corner of the box): of the vertices on that point the smallest
+            // id keeps it; the others take their previous positions for this
+            // step, and every one of them is computed again in the next pass
             ok = false;
             why = 4;
-            if (!kFast && acc && lane == 0) {  // (the filter kernel hands it to the full kernel)
+            if (!kFast && acc && lane == 0 && same_lo < v) {  // (the filter kernel hands it to the full kernel)
+                const float2 pu = m.pos[same_lo], tu = tgt[same_lo];
                 m.pos[v] = acc[v];
-                stalled++;
+                stalled[(pu.x == tu.x && pu.y == tu.y) ? 1 : 0]++;
             }
         }
         const int start = best;
@@ -580,8 +591,9 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
         }
     }
     flush_stats(loc, stats, kStIncircle0);
-    const unsigned st1[1] = {stalled};
+    const unsigned st1[1] = {stalled[0]}, st2[1] = {stalled[1]};
     flush_stats(st1, stats, kStStalled);
+    flush_stats(st2, stats, kStRelDup);
 }
 
 

@@ -742,8 +742,10 @@ constexpr int kMaxStar = 64;
 // deterministic visibility walk can cycle forever in a triangulation that is
 // not Delaunay; the random start makes termination certain with probability
 // one.  Returns the containing triangle (and in *on the half-edge of it that
-// p lies on, or -1), -1 if the walk leaves the mesh or exceeds kWalkCap
-// triangles, -2 if p coincides with a vertex.
+// p lies on, or -1); -1 if the walk leaves the mesh (p outside the
+// triangulated domain); -5 if it exceeds kWalkCap triangles (a retry starts
+// from another random sequence); -2 if p coincides with a vertex, whose id
+// is then in *on.
 constexpr int kWalkCap = 2048;
 template <bool kF32>
 __device__ int locate_point(const DMesh& m, float2 p, int t, int* on, unsigned seed) {
@@ -759,8 +761,10 @@ __device__ int locate_point(const DMesh& m, float2 p, int t, int* on, unsigned s
         for (int k = 0; k < 3; k++) {
             const int idx = (k0 + k) % 3;  // selects, not an indexed array: stays in registers
             const int h = idx == 0 ? h0 : (idx == 1 ? h1 : h2);
-            const float2 a = m.pos[m.he_v[h]], b = m.pos[m.he_v[h ^ 1]];
-            if ((a.x == p.x && a.y == p.y) || (b.x == p.x && b.y == p.y)) return -2;
+            const int ia = m.he_v[h], ib = m.he_v[h ^ 1];
+            const float2 a = m.pos[ia], b = m.pos[ib];
+            if (a.x == p.x && a.y == p.y) { *on = ia; return -2; }
+            if (b.x == p.x && b.y == p.y) { *on = ib; return -2; }
             if (h == came) continue;  // p is on our side of the edge we crossed
             int st;
             const int o = orient<kF32>(a, b, p, m.periodic, &st);
@@ -776,10 +780,18 @@ __device__ int locate_point(const DMesh& m, float2 p, int t, int* on, unsigned s
         came = cross ^ 1;
         t = nt;
     }
-    return -1;
+    return -5;
 }
 
-// phase 1 (read-only): find where each listed vertex's target lies
+// phase 1 (read-only): find where each listed vertex's target lies.
+// r_tt[v]: the target triangle, or
+//   -3  v is at its target already
+//   -2  the target is the position of a vertex w whose own target it is
+//       (two vertices with one target: v stays where it is, a duplicate)
+//   -6  the target is the position of a vertex w that has not reached its
+//       own target yet: v waits for w to leave (retried next round)
+//   -5  the walk exceeded its length cap (retried next round)
+//   -1  the walk left the mesh (target outside the triangulated domain)
 template <bool kF32>
 __device__ __forceinline__ void reloc_locate_body(const DMesh& m, const int* __restrict__ vlist, int n,
                                                   const float2* __restrict__ tgt, int* __restrict__ r_tt,
@@ -789,7 +801,12 @@ __device__ __forceinline__ void reloc_locate_body(const DMesh& m, const int* __r
         const float2 t = tgt[v], q = m.pos[v];
         if (t.x == q.x && t.y == q.y) { r_tt[v] = -3; continue; }
         int on = -1;
-        const int tt = locate_point<kF32>(m, t, m.he_t[m.v_he[v]], &on, (unsigned)v ^ ((unsigned)round << 20));
+        int tt = locate_point<kF32>(m, t, m.he_t[m.v_he[v]], &on, (unsigned)v ^ ((unsigned)round << 20));
+        if (tt == -2) {
+            const float2 tw = tgt[on];
+            tt = (tw.x == t.x && tw.y == t.y) ? -2 : -6;
+            on = -1;
+        }
         r_tt[v] = tt;
         r_tt2[v] = (tt >= 0 && on >= 0) ? m.he_t[on ^ 1] : -1;
     }
@@ -856,19 +873,54 @@ __device__ __forceinline__ int tri_contains(const DMesh& m, int X, float2 p, int
     return 1;
 }
 
-// phase 3: winners (all claims held) remove and re-insert their vertex
+// Ear test of the removal: the ear (c, a, b) of v's link polygon (CCW order
+// c, a, b; strictly convex at a) is a proper ear when no other link vertex
+// lies in the closed triangle, i.e. when the diagonal (c, b) runs inside the
+// polygon and touches its boundary only at c and b.  h: the spoke v -> a.
+template <bool kF32>
+__device__ __forceinline__ bool ear_empty(const DMesh& m, int h, int a, int b, int c, float2 pa, float2 pb,
+                                          float2 pc) {
+    int st;
+    int g = m.he_next[h ^ 1], guard = 0;
+    while (g != h && ++guard < kMaxStar) {
+        const int x = m.he_v[g ^ 1];
+        if (x != a && x != b && x != c) {
+            const float2 px = m.pos[x];
+            if (orient<kF32>(pc, pa, px, m.periodic, &st) >= 0 && orient<kF32>(pa, pb, px, m.periodic, &st) >= 0 &&
+                orient<kF32>(pb, pc, px, m.periodic, &st) >= 0)
+                return false;
+        }
+        g = m.he_next[g ^ 1];
+    }
+    return true;
+}
+
+// phase 3: winners (all claims held) remove and re-insert their vertex.
+// loc: {relocated, stalled, walk left the mesh or retries spent, star above
+// kMaxStar, removal stuck, put back, duplicate targets}
+constexpr int kRelocRetries = 64;  // waits and walk retries per vertex and step
 template <bool kF32>
 __device__ __forceinline__ void reloc_apply_body(const DMesh& m, const int* __restrict__ vlist, int n, int round,
                                                  const int* __restrict__ r_tt, const int* __restrict__ r_tt2,
-                                                 const unsigned long long* __restrict__ claim,
+                                                 int* __restrict__ r_wait, const unsigned long long* __restrict__ claim,
                                                  const float2* __restrict__ tgt, int* __restrict__ next_list, int* c_next,
                                                  unsigned (&loc)[7], int i0, int stride) {
     for (int i = i0; i < n; i += stride) {
         const int v = vlist[i];
         const int tt = r_tt[v], tt2 = r_tt2[v];
         if (tt == -3) continue;              // already at its target
+        if (tt == -2) { loc[6]++; continue; }  // another vertex holds the same target: stays put
+        if (tt == -6 || tt == -5) {          // waits for a vertex to leave its target, or walks again
+            if (++r_wait[v] <= kRelocRetries) {
+                next_list[agg_inc(c_next)] = v;
+            } else {
+                loc[1]++;
+                loc[2]++;
+            }
+            continue;
+        }
         if (tt == -4) { loc[1]++; loc[3]++; continue; }  // star larger than kMaxStar: stays put
-        if (tt < 0) { loc[1]++; loc[2]++; continue; }  // target on a vertex or walk lost: stays put
+        if (tt < 0) { loc[1]++; loc[2]++; continue; }  // target outside the mesh: stays put
         const unsigned long long key = claim_key(round, v);
         int star[kMaxStar];
         int deg = 0;
@@ -892,7 +944,17 @@ __device__ __forceinline__ void reloc_apply_body(const DMesh& m, const int* __re
         const int deg0 = deg;
         const float2 pv = m.pos[v], pt = tgt[v];
         int st;
-        // --- remove: flip an edge (v, a) whose two new triangles are strictly CCW
+        // --- remove: flip spokes (v, a) down to degree 3.  Each flip clips
+        // the ear (c, a, b) off v's link polygon (CCW order c, a, b) and
+        // adds the ear triangle, which must be strictly CCW.  First only ears
+        // that keep v strictly inside the rest (both new triangles strictly
+        // CCW).  A link with collinear vertices can have none: v on the line
+        // (c, b) of every convex ear, as at the centre of a square.  Then any
+        // proper ear is clipped; v's own triangles may turn degenerate, but
+        // they all disappear in the merge, and the ears of a simple polygon
+        // with four or more vertices always include a proper one (Meisters'
+        // two-ears theorem), whose last remaining triangle is strictly CCW.
+        bool strict = true;
         while (deg > 3) {
             bool flipped = false;
             const int h0 = m.v_he[v];
@@ -900,7 +962,9 @@ __device__ __forceinline__ void reloc_apply_body(const DMesh& m, const int* __re
             do {
                 const int a = m.he_v[h ^ 1], b = m.he_op[h], c = m.he_op[h ^ 1];
                 const float2 pa = m.pos[a], pb = m.pos[b], pc = m.pos[c];
-                if (orient<kF32>(pb, pc, pa, m.periodic, &st) > 0 && orient<kF32>(pc, pb, pv, m.periodic, &st) > 0) {
+                if (orient<kF32>(pb, pc, pa, m.periodic, &st) > 0 &&
+                    (strict ? orient<kF32>(pc, pb, pv, m.periodic, &st) > 0
+                            : ear_empty<kF32>(m, h, a, b, c, pa, pb, pc))) {
                     int h2, h3, h4, h5;
                     flip_soa(m, h >> 1, h2, h3, h4, h5);  // keeps v.he an outgoing half-edge of v
                     flipped = true;
@@ -908,10 +972,14 @@ __device__ __forceinline__ void reloc_apply_body(const DMesh& m, const int* __re
                 }
                 h = m.he_next[h ^ 1];
             } while (h != h0 && ++guard < kMaxStar);
-            if (!flipped) break;
+            if (!flipped) {
+                if (!strict) break;
+                strict = false;
+                continue;
+            }
             deg--;
         }
-        if (deg > 3) {  // degenerate link (collinear vertices): give up, mesh still valid
+        if (deg > 3) {  // no proper ear (a link that is not a simple polygon): mesh still valid
             loc[1]++;
             loc[4]++;
             continue;
@@ -999,13 +1067,14 @@ template <bool kF32>
 __global__ void __launch_bounds__(kThreads) k_reloc_apply(DMesh m, const int* __restrict__ vlist,
                                                           const int* __restrict__ cnt, int slot, int round,
                                                           const int* __restrict__ r_tt, const int* __restrict__ r_tt2,
+                                                          int* __restrict__ r_wait,
                                                           const unsigned long long* __restrict__ claim,
                                                           const float2* __restrict__ tgt, int* __restrict__ next_list,
                                                           int* __restrict__ cnt_next, int slot_next,
                                                           unsigned long long* __restrict__ stats) {
-    unsigned loc[7] = {0, 0, 0, 0, 0, 0, 0};  // relocated, given up, the reasons, direct moves
-    reloc_apply_body<kF32>(m, vlist, cnt[slot], round, r_tt, r_tt2, claim, tgt, next_list, &cnt_next[slot_next], loc,
-                           blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+    unsigned loc[7] = {0, 0, 0, 0, 0, 0, 0};  // relocated, stalled, the reasons, duplicates
+    reloc_apply_body<kF32>(m, vlist, cnt[slot], round, r_tt, r_tt2, r_wait, claim, tgt, next_list,
+                           &cnt_next[slot_next], loc, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
     flush_stats(loc, stats, kStRelocated);
     const unsigned grid_loc[1] = {loc[0]};
     flush_stats(grid_loc, stats, kStRelocGrid);
@@ -1017,7 +1086,8 @@ __global__ void __launch_bounds__(kThreads) k_reloc_apply(DMesh m, const int* __
 template <bool kF32>
 __global__ void __launch_bounds__(1024) k_reloc_tail(DMesh m, int* __restrict__ list0, int* __restrict__ list1, int cur0,
                                                      int n0, int round0, int max_rounds, int* __restrict__ r_tt,
-                                                     int* __restrict__ r_tt2, unsigned long long* __restrict__ claim,
+                                                     int* __restrict__ r_tt2, int* __restrict__ r_wait,
+                                                     unsigned long long* __restrict__ claim,
                                                      const float2* __restrict__ tgt, unsigned long long* __restrict__ stats,
                                                      int* __restrict__ out) {
     __shared__ int s_n, s_next;
@@ -1037,7 +1107,8 @@ __global__ void __launch_bounds__(1024) k_reloc_tail(DMesh m, int* __restrict__ 
         __syncthreads();
         reloc_claim_body(m, list, n, round, r_tt, r_tt2, claim, threadIdx.x, blockDim.x);
         __syncthreads();
-        reloc_apply_body<kF32>(m, list, n, round, r_tt, r_tt2, claim, tgt, next, &s_next, loc, threadIdx.x, blockDim.x);
+        reloc_apply_body<kF32>(m, list, n, round, r_tt, r_tt2, r_wait, claim, tgt, next, &s_next, loc, threadIdx.x,
+                               blockDim.x);
         __syncthreads();
         if (threadIdx.x == 0) s_n = s_next;
         cur = 1 - cur;
@@ -1048,6 +1119,17 @@ __global__ void __launch_bounds__(1024) k_reloc_tail(DMesh m, int* __restrict__ 
         out[0] = r;
         out[1] = cur;
         out[2] = s_n;
+    }
+}
+
+// the relocation list of a step (the vertices sent back) with their retry
+// counts cleared
+__global__ void __launch_bounds__(kThreads) k_reloc_init(const int* __restrict__ src, int n, int* __restrict__ dst,
+                                                         int* __restrict__ r_wait) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int v = src[i];
+        dst[i] = v;
+        r_wait[v] = 0;
     }
 }
 
@@ -1553,6 +1635,7 @@ struct mcl_tri {
     int* v_soft = nullptr;             // nv: step stamp of a vertex's soft take-back
     int* r_tt = nullptr;               // nv: relocation target triangle
     int* r_tt2 = nullptr;              // nv: second triangle when the target is on an edge
+    int* r_wait = nullptr;             // nv: relocation retries (waits, walks) in this step
     int* tail_out = nullptr;           // 8: results of the single-workgroup loops
     // star recomputation (torus kinetic step)
     int* star_deg[2] = {nullptr, nullptr};  // nv each
@@ -1626,7 +1709,7 @@ static int tri_free(mcl_tri* t) {
     void* ptrs[] = {t->m.pos, t->m.he_next, t->m.he_v, t->m.he_t, t->m.he_op, t->m.v_he, t->m.v_pidx, t->m.t_he,
                     t->claim, t->e_stamp, t->cand[0], t->cand[1], t->wants, t->cnt, t->rounds_done, t->stats,
                     t->scalar, t->acc, t->tgt, t->v_mv, t->v_rev, t->v_fexp, t->t_stamp, t->bad[0], t->bad[1],
-                    t->rev, t->allrev, t->pend[0], t->pend[1], t->deg, t->scan_tmp, t->v_touch, t->t_rep, t->t_rep_e, t->e_undo, t->undo, t->stuck, t->touched, t->r_tt, t->r_tt2, t->v_soft, t->tail_out,
+                    t->rev, t->allrev, t->pend[0], t->pend[1], t->deg, t->scan_tmp, t->v_touch, t->t_rep, t->t_rep_e, t->e_undo, t->undo, t->stuck, t->touched, t->r_tt, t->r_tt2, t->r_wait, t->v_soft, t->tail_out,
                     t->star_deg[0], t->star_deg[1], t->star_nbr[0], t->star_nbr[1], t->s_ecnt, t->s_tcnt,
                     t->s_mark, t->s_list, t->s_aux, t->star_tdeg, t->star_tnbr, t->s_rlist, t->s_heavy,
                     t->c_bucket, t->c_count};
@@ -1683,7 +1766,7 @@ int tri_make(int nv, int nhe, int nt, int periodic, int ncorner, mcl_tri** out) 
     A(t->bad[0], nt); A(t->bad[1], nt); A(t->rev, nv); A(t->allrev, nv); A(t->pend[0], nv); A(t->pend[1], nv);
     A(t->deg, nv + 1); A(t->scan_tmp, scan_scratch_ints(nv + 1) + 16);
     A(t->v_touch, nv); A(t->t_rep, nt); A(t->t_rep_e, nt); A(t->e_undo, m.ne); A(t->undo, m.ne);
-    A(t->stuck, nt); A(t->touched, nv); A(t->r_tt, nv); A(t->r_tt2, nv); A(t->v_soft, nv); A(t->tail_out, 8);
+    A(t->stuck, nt); A(t->touched, nv); A(t->r_tt, nv); A(t->r_tt2, nv); A(t->r_wait, nv); A(t->v_soft, nv); A(t->tail_out, 8);
     if (periodic || m.ncorner > 0) {
         A(t->star_deg[0], nv); A(t->star_deg[1], nv);
         // vertex rows, then the corners' fans (star_row); star_tnbr also holds
@@ -2066,6 +2149,7 @@ static int stats_end(mcl_tri* t, hipStream_t s, mcl_stats_t* st, std::chrono::st
     st->untangle_flips = (long long)h[kStUntangleFlips];
     st->reverts = (long long)h[kStReverts];
     st->stalled = (long long)h[kStStalled] + (long long)h[kStRelocFailed];
+    st->duplicates = (long long)h[kStRelDup];
     st->relocated = (long long)h[kStRelocated];
     st->relocated_grid = (long long)h[kStRelocGrid];
     st->inverted = (long long)h[kStInverted];
@@ -2336,7 +2420,7 @@ static int relocate_rounds(mcl_tri* t, hipStream_t s, bool f32, int n, int* slot
             const int max_rounds = 256 - r;
             const int R0 = t->round;
             hipLaunchKernelGGL((k_reloc_tail<true>), dim3(1), dim3(1024), 0, s, m, t->pend[0], t->pend[1], cur, n, R0,
-                               max_rounds, t->r_tt, t->r_tt2, t->claim, t->tgt, t->stats, t->tail_out);
+                               max_rounds, t->r_tt, t->r_tt2, t->r_wait, t->claim, t->tgt, t->stats, t->tail_out);
             MCL_HIP(hipGetLastError());
             int out[3];
             MCL_HIP(hipMemcpyAsync(out, t->tail_out, sizeof(out), hipMemcpyDeviceToHost, s));
@@ -2362,12 +2446,12 @@ static int relocate_rounds(mcl_tri* t, hipStream_t s, bool f32, int n, int* slot
         t->timers[3].mark(s);
         if (f32)
             hipLaunchKernelGGL((k_reloc_apply<true>), dim3(blocks_for(n)), dim3(kThreads), 0, s, m, t->pend[cur], t->cnt,
-                               (int)kCnPend, R, t->r_tt, t->r_tt2, t->claim, t->tgt, t->pend[nxt], t->cnt, (int)kCnMove,
-                               t->stats);
+                               (int)kCnPend, R, t->r_tt, t->r_tt2, t->r_wait, t->claim, t->tgt, t->pend[nxt], t->cnt,
+                               (int)kCnMove, t->stats);
         else
             hipLaunchKernelGGL((k_reloc_apply<false>), dim3(blocks_for(n)), dim3(kThreads), 0, s, m, t->pend[cur], t->cnt,
-                               (int)kCnPend, R, t->r_tt, t->r_tt2, t->claim, t->tgt, t->pend[nxt], t->cnt, (int)kCnMove,
-                               t->stats);
+                               (int)kCnPend, R, t->r_tt, t->r_tt2, t->r_wait, t->claim, t->tgt, t->pend[nxt], t->cnt,
+                               (int)kCnMove, t->stats);
         t->timers[3].mark(s);
         MCL_HIP(hipGetLastError());
         MCL_TRY(read_counts(t, s));
@@ -2436,7 +2520,9 @@ static int step_impl(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     int rc = MCL_OK;
     if (nrev > 0) {
         // relocate the vertices that were sent back (list allrev -> pend[0])
-        MCL_HIP(hipMemcpyAsync(t->pend[0], t->allrev, sizeof(int) * nrev, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_reloc_init, dim3(blocks_for(nrev)), dim3(kThreads), 0, s, (const int*)t->allrev, nrev,
+                           t->pend[0], t->r_wait);
+        MCL_HIP(hipGetLastError());
         int slot = 0;
         rc = relocate_rounds(t, s, f32, nrev, &slot, &reloc_rounds);
         if (rc == MCL_OK) rc = validate(t, s, "relocation rounds");
@@ -2447,8 +2533,8 @@ static int step_impl(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     if (trace) {
         unsigned long long h[kStCount];
         if (stats_read(t, s, h) == MCL_OK)
-            fprintf(stderr, "[step] relocation failures: walk %llu star %llu removal %llu put-back %llu; direct moves %llu\n",
-                    h[kStRelWalk], h[kStRelStar], h[kStRelStuck], h[kStRelPutBack], h[kStRelDirect]);
+            fprintf(stderr, "[step] relocation failures: walk %llu star %llu removal %llu put-back %llu; duplicates %llu\n",
+                    h[kStRelWalk], h[kStRelStar], h[kStRelStuck], h[kStRelPutBack], h[kStRelDup]);
     }
     if (trace)
         fprintf(stderr, "[step] inverted %lld repairs %d bad_after_repair %d undone %d reverted %d reloc_rounds %d lawson %lld\n",
@@ -2547,14 +2633,17 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
             hipLaunchKernelGGL((k_star<true, true, 32, true>), star_grid(m.nv, 8), dim3(256), 0, s, m, vlist, 0,
                                nlist_dev, (const int*)nullptr, (const int*)nullptr, depth,
                                (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr,
-                               t->s_aux, t->s_heavy, (const float2*)t->acc, t->stats);
+                               t->s_aux, t->s_heavy, (const float2*)t->acc,
+                               (const float2*)t->tgt, t->stats);
         else
             hipLaunchKernelGGL((k_star<true, true, 32, false>), list_grid, dim3(256), 0, s, m, vlist, 0, nlist_dev,
                                (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
-                               (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy, (const float2*)t->acc, t->stats);
+                               (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy, (const float2*)t->acc,
+                               (const float2*)t->tgt, t->stats);
         hipLaunchKernelGGL((k_star<true, false, 64, false>), dim3(256), dim3(256), 0, s, m, vlist, 0, nlist_dev,
                            (const int*)(t->s_heavy + 1), (const int*)t->s_heavy, depth, (const int*)t->star_deg[cur],
-                           (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, (int*)nullptr, (const float2*)t->acc, t->stats);
+                           (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, (int*)nullptr, (const float2*)t->acc,
+                               (const float2*)t->tgt, t->stats);
     };
     t->timers[5].mark(s);
     stars(nullptr, nullptr, 2, nullptr, nullptr, t->star_deg[nxt], t->star_nbr[nxt]);
@@ -2893,17 +2982,30 @@ extern "C" int mcl_batch_build(const void* d_xy, const int* counts, int ninst, v
     return build_impl(d_xy, counts, ninst, 0, d_perm, out, stream);
 }
 
+// A step that leaves a vertex short of its target (other than a duplicate
+// target) reports MCL_ERR_UNRESOLVED; the mesh is still valid and exactly
+// Delaunay for the positions it holds.
 extern "C" int mcl_tri_step(mcl_tri_t* t, const mcl_step_params_t* p, mcl_stats_t* st, void* stream) {
     if (!t || !p) return MCL_ERR_INPUT;
-    return kinetic_step(t, p, nullptr, st, stream);
+    mcl_stats_t local{};
+    mcl_stats_t* sp = st ? st : &local;
+    int rc = kinetic_step(t, p, nullptr, sp, stream);
+    if (rc == MCL_OK && sp->stalled > 0) rc = MCL_ERR_UNRESOLVED;
+    return rc;
 }
 
 extern "C" int mcl_tri_move_legalize(mcl_tri_t* t, const void* d_disp, int variant, mcl_stats_t* st, void* stream) {
     if (!t || !d_disp || (variant != 0 && variant != 1)) return MCL_ERR_INPUT;
     mcl_step_params_t p{};
     p.variant = variant;
-    return kinetic_step(t, &p, (const float2*)d_disp, st, stream);
+    mcl_stats_t local{};
+    mcl_stats_t* sp = st ? st : &local;
+    int rc = kinetic_step(t, &p, (const float2*)d_disp, sp, stream);
+    if (rc == MCL_OK && sp->stalled > 0) rc = MCL_ERR_UNRESOLVED;
+    return rc;
 }
+
+extern "C" void* mcl_tri_target_ptr(mcl_tri_t* t) { return t ? (void*)t->tgt : nullptr; }
 
 // ---------------------------------------------------------------- one ring
 extern "C" int mcl_tri_one_ring(mcl_tri_t* t, int K, int layout, void* d_ring, void* stream) {

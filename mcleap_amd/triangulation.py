@@ -297,6 +297,15 @@ class DeviceTriangulation:
         """device address of the float2 positions (Device_Triangulation.h:139)"""
         return lib().mcl_tri_pos_ptr(self._h)
 
+    def download_targets(self):
+        """the targets of the last step (mcl_tri_target_ptr) as a host
+        (V, 2) float32 array: every vertex is at its target after a step,
+        except the last step's `duplicates` (vertices sharing one target)"""
+        import torch
+        t = _tensor_from_ptr(lib().mcl_tri_target_ptr(self._h), (self.m_pos_size, 2), torch.float32)
+        torch.cuda.synchronize()
+        return t.cpu().numpy().copy()
+
     def pos_tensor(self):
         """the device positions as a torch view (no copy)"""
         import torch
@@ -397,17 +406,18 @@ class DeviceTriangulation:
         p = _lib.StepParams(int(motion), float(sigma), float(amp), int(seed), int(step), int(variant),
                             int(max_iterations))
         st = _lib.Stats()
-        check(lib().mcl_tri_step(self._h, ctypes.byref(p), ctypes.byref(st), None), "mcl_tri_step")
+        rc = lib().mcl_tri_step(self._h, ctypes.byref(p), ctypes.byref(st), None)
         self.last_stats = st.as_dict()
+        check(rc, "mcl_tri_step")
         return self.last_stats
 
     def move_legalize(self, d, variant=0):
         """movePoints(d) + untangle + delonize as one call that always ends
         valid and exactly Delaunay (d: device (V,2) float32 tensor)"""
         st = _lib.Stats()
-        check(lib().mcl_tri_move_legalize(self._h, _dev_ptr(d), int(variant), ctypes.byref(st), None),
-              "mcl_tri_move_legalize")
+        rc = lib().mcl_tri_move_legalize(self._h, _dev_ptr(d), int(variant), ctypes.byref(st), None)
         self.last_stats = st.as_dict()
+        check(rc, "mcl_tri_move_legalize")
         return self.last_stats
 
     def count_inverted(self):

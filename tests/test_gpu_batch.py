@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import mcleap_amd as M
+from tests.conftest import assert_at_targets
 from mcleap_amd.workloads import ensemble_points
 from tests.test_host_batch import batch_sizes
 
@@ -87,6 +88,7 @@ def test_batch_steps(oracle, star):
         assert np.array_equal(pos[:nc], prev[:nc])  # corners never move
         assert np.all(pos[nc:] >= 0) and np.all(pos[nc:] < 1)
         assert np.any(pos[nc:] != prev[nc:], axis=1).mean() > 0.99
+        assert_at_targets(dt, st, "star %s step %d" % (star, k))
         star_steps += star and st["fallbacks"] == 0 and st["rounds"] == 0
         prev = pos
     if star:
@@ -104,4 +106,5 @@ def test_batch_c5_sizes(oracle):
         st = dt.step(M.MOTION_GAUSS, 5e-4, seed=5, step=k)
         print("step", k, st, flush=True)
         assert dt.count_inverted() == 0 and dt.count_non_delaunay() == 0
+        assert_at_targets(dt, st, "step %d" % k)
     _check_batch(O=oracle, B=B, dt=dt, tag="after 6 steps", instances=[0, 1, 17, 63])

@@ -13,6 +13,7 @@ import pytest
 
 import mcleap_amd as M
 from mcleap_amd.workloads import CONFIGS, initial_points
+from tests.conftest import assert_at_targets
 
 pytestmark = pytest.mark.gpu
 
@@ -23,9 +24,10 @@ def _steps(dt, name, k):
     paths = []
     for s in range(k):
         st = dt.step(motion, cfg["sigma"], amp=cfg["amp"], seed=3, step=s)
-        print(name, "step", s, {x: st[x] for x in ("rounds", "flips", "relocated", "stalled", "fallbacks", "ms")},
-              flush=True)
+        print(name, "step", s, {x: st[x] for x in ("rounds", "flips", "relocated", "stalled", "duplicates",
+                                                  "fallbacks", "ms")}, flush=True)
         assert dt.count_inverted() == 0 and dt.count_non_delaunay() == 0
+        assert_at_targets(dt, st, "%s step %d" % (name, s))
         paths.append("star" if st["rounds"] == 0 and st["fallbacks"] == 0 else "flip")
     return paths
 

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import mcleap_amd as M
-from tests.conftest import uniform_points
+from tests.conftest import assert_at_targets, uniform_points
 
 pytestmark = pytest.mark.gpu
 
@@ -63,7 +63,7 @@ def test_box_steps_config1(oracle, variant, star):
     dt.set_option(M.OPT_STAR_MIN_VERTICES, 0)
     dt.set_option(M.OPT_STAR_ENABLE, 1 if star else 0)
     prev = dt.download_positions()
-    nsteps = 12 if variant == 0 else 6
+    nsteps = 50 if variant == 0 else 12
     star_steps = 0
     for k in range(nsteps):
         st = dt.step(M.MOTION_GAUSS, 1e-3, seed=1, step=k, variant=variant)
@@ -72,9 +72,9 @@ def test_box_steps_config1(oracle, variant, star):
         moved = np.any(pos != prev, axis=1)
         assert moved[4:].mean() > 0.99  # nearly every point moved
         assert np.all(pos[4:] >= 0) and np.all(pos[4:] < 1)
-        # a vertex whose relocation meets a degenerate (collinear) link keeps
-        # its position for the step; the mesh is still exact for the positions
-        assert st["stalled"] <= 3
+        # every vertex at its target (points clamped onto the sides of the
+        # box give collinear links; only exact duplicate targets stay back)
+        assert_at_targets(dt, st, "variant %d step %d" % (variant, k))
         star_steps += star and st["fallbacks"] == 0 and st["rounds"] == 0
         prev = pos
     if star:
@@ -93,7 +93,9 @@ def test_box_large_moves(oracle):
         st = dt.step(M.MOTION_GAUSS, 0.5 / np.sqrt(4000), seed=3, step=k)
         print(st)
         assert st["inverted"] > 0 or k > 0
+        assert st["relocated"] > 0
         _check_box(O, dt, "large step %d" % k)
+        assert_at_targets(dt, st, "large step %d" % k)
 
 
 @pytest.mark.parametrize("star", [True, False])
@@ -115,6 +117,7 @@ def test_torus_steps(oracle, star):
         else:
             assert st["rounds"] > 0
         _check_torus(O, dt, "torus step %d" % k)
+        assert_at_targets(dt, st, "torus step %d" % k)
 
 
 def test_delonize_from_perturbed(oracle):
@@ -174,11 +177,11 @@ def test_move_legalize_caller_displacement(oracle, periodic):
             want = np.mod(want, np.float32(1.0))
             want[want >= 1.0] = 0.0
         sel = slice(None) if periodic else slice(4, None)
-        # a vertex whose relocation meets a degenerate or oversized star stays
-        # at its previous position (counted in stalled); every other vertex
-        # is exactly at pos + d
-        reached = np.all(pos[sel] == want.astype(np.float32)[sel], axis=1)
-        assert (~reached).sum() == st["stalled"] <= 3
+        # the targets are exactly pos + d, and every vertex is there
+        tgt = dt.download_targets()
+        assert np.array_equal(tgt[sel], want.astype(np.float32)[sel])
+        assert_at_targets(dt, st, "move_legalize %d" % k)
+        assert st["duplicates"] == 0
 
 
 def test_untangle_api(oracle):
