@@ -2,8 +2,9 @@
 // gridCount2d.h:23-302) as a counting sort plus a CSR query.
 //
 // update():   key -> histogram (global atomics) -> exclusive scan ->
-//             slot assignment -> per-cell sort of the slots by previous
-//             index (deterministic; the reference's atomic order is not) ->
+//             slot assignment (atomic order) -> each point's rank by index
+//             within its cell (deterministic and stable; the reference's
+//             atomic order is not) ->
 //             gather into the sorted arrays -> copy back over the caller's
 //             buffers (the reference sorts the caller's array in place,
 //             gridCount2d.h:207-208, 256).
@@ -18,7 +19,9 @@
 // MCL_GRID_PERIODIC wraps cells and distances on the unit torus.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <vector>
 
 #include "mcl_geom.hpp"
@@ -82,6 +85,7 @@ __global__ void __launch_bounds__(kT) k_key_count(GridParams g, const float2* __
     }
 }
 
+// each point into a slot of its cell, in atomic (arbitrary) order
 __global__ void __launch_bounds__(kT) k_slot(const int* __restrict__ key, int n, const int* __restrict__ start,
                                              int* __restrict__ fill, int* __restrict__ perm) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -90,15 +94,19 @@ __global__ void __launch_bounds__(kT) k_slot(const int* __restrict__ key, int n,
     }
 }
 
-// order each cell's slots by previous index (insertion sort, cells are small)
-__global__ void __launch_bounds__(kT) k_cell_sort(const int* __restrict__ start, int nkeys, int* __restrict__ perm) {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nkeys; c += gridDim.x * blockDim.x) {
-        int b = start[c], e = start[c + 1];
-        for (int i = b + 1; i < e; i++) {
-            int v = perm[i], j = i - 1;
-            while (j >= b && perm[j] > v) { perm[j + 1] = perm[j]; j--; }
-            perm[j + 1] = v;
-        }
+// the stable order within each cell: the point in slot s of the arbitrary
+// order goes to its cell's start plus the number of the cell's points with a
+// smaller index (one thread per point; a cell's slots are contiguous and
+// read by all of its threads, so they stay in cache)
+__global__ void __launch_bounds__(kT) k_cell_rank(const int* __restrict__ key, int n, const int* __restrict__ start,
+                                                  const int* __restrict__ perm_any, int* __restrict__ perm) {
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        const int i = perm_any[s];
+        const int k = key[i];
+        const int b = start[k], e = start[k + 1];
+        int r = 0;
+        for (int j = b; j < e; j++) r += perm_any[j] < i;
+        perm[b + r] = i;
     }
 }
 
@@ -166,6 +174,81 @@ int blocks(long long n) {
     return (int)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
 }
 
+// The buffers of one counting sort by cell.  key, perm, ind: n + 1 ints;
+// start, fill: nkeys + 1 ints; scan_tmp: scan_scratch_ints(max(nkeys, n) + 1)
+// ints; spos (and svel when vel is given): n float2.
+struct SortBufs {
+    int *key, *start, *fill, *perm, *ind, *scan_tmp;
+    float2 *spos, *svel;
+};
+
+// Counting sort of n points by cell: key + histogram, exclusive scan of the
+// histogram, slots in atomic order (into ind), the stable order within each
+// cell (into perm), then the gather of the sorted positions and ind[i] = the
+// slot of point i.  Asynchronous on s; uses only the given buffers.
+int grid_sort(const GridParams& g, const float2* pos, const float2* vel, int n, const SortBufs& b, hipStream_t s) {
+    const int nk = g.nkeys;
+    hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, b.start, nk + 1);
+    hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, b.fill, nk + 1);
+    hipLaunchKernelGGL(k_key_count, dim3(blocks(n)), dim3(kT), 0, s, g, pos, n, b.key, b.start);
+    MCL_TRY(scan_exclusive(b.start, b.start, nk, b.scan_tmp, s));
+    hipLaunchKernelGGL(k_slot, dim3(blocks(n)), dim3(kT), 0, s, b.key, n, b.start, b.fill, b.ind);
+    hipLaunchKernelGGL(k_cell_rank, dim3(blocks(n)), dim3(kT), 0, s, b.key, n, b.start, (const int*)b.ind, b.perm);
+    hipLaunchKernelGGL(k_gather, dim3(blocks(n)), dim3(kT), 0, s, b.perm, n, pos, vel, b.spos, b.svel, b.ind);
+    MCL_HIP(hipGetLastError());
+    return MCL_OK;
+}
+
+// the count pass, its scan and (with d_nbrs) the fill pass of the CSR query
+// over sorted positions spos; total == NULL: no host read, writes past cap
+// dropped; else *total = the pair count after a synchronisation
+int frnn_csr(const GridParams& g, const float2* spos, int n, const int* start, float rad, int* off, int* nbrs,
+             int cap, long long* total, int* scan_tmp, hipStream_t s, KTimer* tc, KTimer* tf) {
+    const float rad2 = rad * rad;
+    if (tc) tc->mark(s);
+    hipLaunchKernelGGL(k_frnn<0>, dim3(blocks(n)), dim3(kT), 0, s, g, spos, n, start, rad2, off, (const int*)nullptr,
+                       (int*)nullptr, 0);
+    if (tc) tc->mark(s);
+    MCL_TRY(scan_exclusive(off, off, n, scan_tmp, s));
+    if (!total) {
+        if (tf) tf->mark(s);
+        if (nbrs && cap > 0)
+            hipLaunchKernelGGL(k_frnn<1>, dim3(blocks(n)), dim3(kT), 0, s, g, spos, n, start, rad2, (int*)nullptr,
+                               (const int*)off, nbrs, cap);
+        if (tf) tf->mark(s);
+        MCL_HIP(hipGetLastError());
+        return MCL_OK;
+    }
+    int tot = 0;
+    MCL_HIP(hipMemcpyAsync(&tot, off + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    MCL_HIP(hipStreamSynchronize(s));
+    *total = tot;
+    if (nbrs && cap > 0) {
+        hipLaunchKernelGGL(k_frnn<1>, dim3(blocks(n)), dim3(kT), 0, s, g, spos, n, start, rad2, (int*)nullptr,
+                           (const int*)off, nbrs, cap);
+        MCL_HIP(hipGetLastError());
+        MCL_HIP(hipStreamSynchronize(s));
+    }
+    return tot > cap ? MCL_ERR_CAPACITY : MCL_OK;
+}
+
+// grid geometry from the mcl_grid_create arguments (MCL_ERR_INPUT if invalid)
+int grid_params(float minx, float miny, float csx, float csy, int ncx, int ncy, int flags, GridParams* g) {
+    if (ncx <= 0 || ncy <= 0 || !(csx > 0) || !(csy > 0)) return MCL_ERR_INPUT;
+    const int morton = (flags & 1) ? 1 : 0, periodic = (flags & MCL_GRID_PERIODIC) ? 1 : 0;
+    if (morton && (ncx > 65536 || ncy > 65536)) return MCL_ERR_INPUT;
+    if (periodic && (ncx < 3 || ncy < 3)) return MCL_ERR_INPUT;
+    if ((long long)ncx * ncy >= (1LL << 31)) return MCL_ERR_INPUT;
+    *g = {minx, miny, csx, csy, ncx, ncy, ncx * ncy, ncx * ncy, morton, periodic};
+    if (morton) {
+        long long side = 1;
+        while (side < ncx || side < ncy) side <<= 1;
+        if (side * side >= (1LL << 31)) return MCL_ERR_INPUT;
+        g->nkeys = (int)(side * side);
+    }
+    return MCL_OK;
+}
+
 }  // namespace
 
 struct mcl_grid {
@@ -182,6 +265,7 @@ struct mcl_grid {
     float2* svel = nullptr;
     unsigned long long* scalar = nullptr;
     mcl::KTimer timers[3];  // FRNN count, FRNN fill, build
+    SortBufs bufs() const { return {key, start, fill, perm, ind, scan_tmp, spos, svel}; }
 };
 
 static void grid_free(mcl_grid* G) {
@@ -192,20 +276,14 @@ static void grid_free(mcl_grid* G) {
 
 extern "C" int mcl_grid_create(int nump, float minx, float miny, float csx, float csy, int ncx, int ncy, int flags,
                                mcl_grid_t** out) {
-    if (!out || nump < 0 || ncx <= 0 || ncy <= 0 || !(csx > 0) || !(csy > 0)) return MCL_ERR_INPUT;
-    const int morton = (flags & 1) ? 1 : 0, periodic = (flags & MCL_GRID_PERIODIC) ? 1 : 0;
-    if (morton && (ncx > 65536 || ncy > 65536)) return MCL_ERR_INPUT;
-    if (periodic && (ncx < 3 || ncy < 3)) return MCL_ERR_INPUT;
+    if (!out || nump < 0) return MCL_ERR_INPUT;
+    GridParams gp;
+    MCL_TRY(grid_params(minx, miny, csx, csy, ncx, ncy, flags, &gp));
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MCL_ERR_NO_DEVICE;
     mcl_grid* G = new mcl_grid();
     G->nump = nump;
-    G->g = {minx, miny, csx, csy, ncx, ncy, ncx * ncy, ncx * ncy, morton, periodic};
-    if (morton) {
-        int side = 1;
-        while (side < ncx || side < ncy) side <<= 1;
-        G->g.nkeys = side * side;
-    }
+    G->g = gp;
     const int nk = G->g.nkeys;
     bool ok = hipMalloc(&G->key, sizeof(int) * (nump + 1)) == hipSuccess &&
               hipMalloc(&G->start, sizeof(int) * (nk + 1)) == hipSuccess &&
@@ -235,18 +313,10 @@ extern "C" int mcl_grid_destroy(mcl_grid_t* G) {
 extern "C" int mcl_grid_update(mcl_grid_t* G, void* d_pos, void* d_vel, void* stream) {
     if (!G || !d_pos) return MCL_ERR_INPUT;
     hipStream_t s = (hipStream_t)stream;
-    const int n = G->nump, nk = G->g.nkeys;
+    const int n = G->nump;
     float2* pos = (float2*)d_pos;
     float2* vel = (float2*)d_vel;
-    hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, G->start, nk + 1);
-    hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, G->fill, nk + 1);
-    hipLaunchKernelGGL(k_key_count, dim3(blocks(n)), dim3(kT), 0, s, G->g, pos, n, G->key, G->start);
-    int rc = scan_exclusive(G->start, G->start, nk, G->scan_tmp, s);
-    if (rc != MCL_OK) return rc;
-    hipLaunchKernelGGL(k_slot, dim3(blocks(n)), dim3(kT), 0, s, G->key, n, G->start, G->fill, G->perm);
-    hipLaunchKernelGGL(k_cell_sort, dim3(blocks(nk)), dim3(kT), 0, s, G->start, nk, G->perm);
-    hipLaunchKernelGGL(k_gather, dim3(blocks(n)), dim3(kT), 0, s, G->perm, n, pos, vel, G->spos, G->svel, G->ind);
-    MCL_HIP(hipGetLastError());
+    MCL_TRY(grid_sort(G->g, pos, vel, n, G->bufs(), s));
     MCL_HIP(hipMemcpyAsync(pos, G->spos, sizeof(float2) * n, hipMemcpyDeviceToDevice, s));
     if (vel) MCL_HIP(hipMemcpyAsync(vel, G->svel, sizeof(float2) * n, hipMemcpyDeviceToDevice, s));
     MCL_HIP(hipStreamSynchronize(s));
@@ -258,20 +328,10 @@ extern "C" int mcl_grid_update(mcl_grid_t* G, void* d_pos, void* d_vel, void* st
 extern "C" int mcl_grid_build(mcl_grid_t* G, const void* d_src, void* stream) {
     if (!G || !d_src) return MCL_ERR_INPUT;
     hipStream_t s = (hipStream_t)stream;
-    const int n = G->nump, nk = G->g.nkeys;
     const float2* pos = (const float2*)d_src;
     G->timers[2].mark(s);
-    hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, G->start, nk + 1);
-    hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, G->fill, nk + 1);
-    hipLaunchKernelGGL(k_key_count, dim3(blocks(n)), dim3(kT), 0, s, G->g, pos, n, G->key, G->start);
-    int rc = scan_exclusive(G->start, G->start, nk, G->scan_tmp, s);
-    if (rc != MCL_OK) return rc;
-    hipLaunchKernelGGL(k_slot, dim3(blocks(n)), dim3(kT), 0, s, G->key, n, G->start, G->fill, G->perm);
-    hipLaunchKernelGGL(k_cell_sort, dim3(blocks(nk)), dim3(kT), 0, s, G->start, nk, G->perm);
-    hipLaunchKernelGGL(k_gather, dim3(blocks(n)), dim3(kT), 0, s, G->perm, n, pos, (const float2*)nullptr, G->spos,
-                       G->svel, G->ind);
+    MCL_TRY(grid_sort(G->g, pos, nullptr, G->nump, G->bufs(), s));
     G->timers[2].mark(s);
-    MCL_HIP(hipGetLastError());
     return MCL_OK;
 }
 
@@ -290,36 +350,8 @@ extern "C" int mcl_grid_index_ptrs(mcl_grid_t* G, void** d_ind, void** d_perm, v
 extern "C" int mcl_grid_frnn_csr(mcl_grid_t* G, const void* d_pos, float rad, void* d_offsets, void* d_nbrs, int cap,
                                  long long* total, void* stream) {
     if (!G || !d_pos || !d_offsets) return MCL_ERR_INPUT;
-    hipStream_t s = (hipStream_t)stream;
-    const int n = G->nump;
-    const float rad2 = rad * rad;
-    int* off = (int*)d_offsets;
-    G->timers[0].mark(s);
-    hipLaunchKernelGGL(k_frnn<0>, dim3(blocks(n)), dim3(kT), 0, s, G->g, (const float2*)d_pos, n, G->start, rad2, off,
-                       (const int*)nullptr, (int*)nullptr, 0);
-    G->timers[0].mark(s);
-    int rc = scan_exclusive(off, off, n, G->scan_tmp, s);
-    if (rc != MCL_OK) return rc;
-    if (!total) {  // asynchronous form: the fill pass keeps writes below cap
-        G->timers[1].mark(s);
-        if (d_nbrs && cap > 0)
-            hipLaunchKernelGGL(k_frnn<1>, dim3(blocks(n)), dim3(kT), 0, s, G->g, (const float2*)d_pos, n, G->start, rad2,
-                               (int*)nullptr, (const int*)off, (int*)d_nbrs, cap);
-        G->timers[1].mark(s);
-        MCL_HIP(hipGetLastError());
-        return MCL_OK;
-    }
-    int tot = 0;
-    MCL_HIP(hipMemcpyAsync(&tot, off + n, sizeof(int), hipMemcpyDeviceToHost, s));
-    MCL_HIP(hipStreamSynchronize(s));
-    *total = tot;
-    if (d_nbrs && cap > 0) {
-        hipLaunchKernelGGL(k_frnn<1>, dim3(blocks(n)), dim3(kT), 0, s, G->g, (const float2*)d_pos, n, G->start, rad2,
-                           (int*)nullptr, (const int*)off, (int*)d_nbrs, cap);
-        MCL_HIP(hipGetLastError());
-        MCL_HIP(hipStreamSynchronize(s));
-    }
-    return tot > cap ? MCL_ERR_CAPACITY : MCL_OK;
+    return frnn_csr(G->g, (const float2*)d_pos, G->nump, G->start, rad, (int*)d_offsets, (int*)d_nbrs, cap, total,
+                    G->scan_tmp, (hipStream_t)stream, &G->timers[0], &G->timers[1]);
 }
 
 extern "C" int mcl_grid_mean_occupancy(mcl_grid_t* G, float* out, void* stream) {
@@ -533,8 +565,9 @@ extern "C" int mcl_bgrid_build(mcl_bgrid_t* G, const void* d_pos, void* stream) 
     hipLaunchKernelGGL(k_bkey_count, dim3(blocks(n)), dim3(kT), 0, s, b, (const float2*)d_pos, G->key, G->start);
     int rc = scan_exclusive(G->start, G->start, nk, G->scan_tmp, s);
     if (rc != MCL_OK) return rc;
-    hipLaunchKernelGGL(k_slot, dim3(blocks(n)), dim3(kT), 0, s, G->key, n, G->start, G->fill, G->perm);
-    hipLaunchKernelGGL(k_cell_sort, dim3(blocks(nk)), dim3(kT), 0, s, G->start, nk, G->perm);
+    hipLaunchKernelGGL(k_slot, dim3(blocks(n)), dim3(kT), 0, s, G->key, n, G->start, G->fill, G->ind);
+    hipLaunchKernelGGL(k_cell_rank, dim3(blocks(n)), dim3(kT), 0, s, G->key, n, G->start, (const int*)G->ind,
+                       G->perm);
     hipLaunchKernelGGL(k_gather, dim3(blocks(n)), dim3(kT), 0, s, G->perm, n, (const float2*)d_pos + G->v0,
                        (const float2*)nullptr, G->spos, (float2*)nullptr, G->ind);
     G->timers[2].mark(s);

@@ -14,6 +14,13 @@
         if (_e != hipSuccess) return mcl::hip_fail(_e, #call, __FILE__, __LINE__); \
     } while (0)
 
+// return a non-OK status from the enclosing function
+#define MCL_TRY(x)                   \
+    do {                             \
+        int _r = (x);                \
+        if (_r != MCL_OK) return _r; \
+    } while (0)
+
 namespace mcl {
 
 int hip_fail(hipError_t e, const char* what, const char* file, int line);

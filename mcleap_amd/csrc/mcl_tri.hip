@@ -1697,12 +1697,6 @@ static int dalloc(T** p, size_t n) {
     return MCL_OK;
 }
 
-#define MCL_TRY(x)                 \
-    do {                           \
-        int _r = (x);              \
-        if (_r != MCL_OK) return _r; \
-    } while (0)
-
 static int ensure_mesh(mcl_tri* t, hipStream_t s);
 
 static int tri_free(mcl_tri* t) {
