@@ -316,6 +316,24 @@ int mcl_grid_frnn_csr(mcl_grid_t* g, const void* d_pos, float rad, void* d_offse
  * positions stay in the grid (mcl_grid_sorted_ptr); asynchronous */
 int mcl_grid_build(mcl_grid_t* g, const void* d_src_pos, void* stream);
 void* mcl_grid_sorted_ptr(mcl_grid_t* g);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): mcl_grid_create + mcl_grid_build + mcl_grid_frnn_csr in one
+ * call that allocates nothing.  The grid (geometry and flags as
+ * mcl_grid_create) sorts the nump device points d_pos by cell into a
+ * library-owned static device array; the result equals the handle path's:
+ * d_offsets (nump + 1 ints) and d_nbrs (cap ints) as mcl_grid_frnn_csr over
+ * the sorted points, d_perm (nump ints, may be NULL) the point index of each
+ * sorted slot (mcl_grid_index_ptrs' perm).  At most 1,048,576 points and
+ * 1,048,576 cell keys (cells, or the power-of-two Morton square with
+ * MCL_GRID_MORTON): MCL_ERR_CAPACITY beyond.  The static array (about 30 MB)
+ * is held by every GPU the library runs on until the process ends and is
+ * shared by all calls on that GPU: calls that overlap on one GPU (other
+ * streams, other host threads) race.  total == NULL: asynchronous; else as
+ * mcl_grid_frnn_csr. */
+int mcl_grid_frnn_csr_noalloc(const void* d_pos, int nump, float minx, float miny, float csx, float csy,
+                              int ncx, int ncy, int flags, float rad, void* d_perm, void* d_offsets,
+                              void* d_nbrs, int cap, long long* total, void* stream);
 /* mean_num_particle_in_cell (gridCount2d.h:304-334) */
 int mcl_grid_mean_occupancy(mcl_grid_t* g, float* out, void* stream);
 

@@ -115,6 +115,7 @@ SIGNATURES = {
     "mcl_grid_mean_occupancy": [_p, _p, _p],
     "mcl_grid_build": [_p, _p, _p],
     "mcl_grid_sorted_ptr": [_p],
+    "mcl_grid_frnn_csr_noalloc": [_p, _i, _f, _f, _f, _f, _i, _i, _i, _f, _p, _p, _p, _i, _p, _p],
     "mcl_tri_set_option": [_p, _i, ctypes.c_longlong],
     "mcl_tri_kernel_timer": [_p, _i],
     "mcl_tri_kernel_timer_read": [_p, _i, _p, _p],

@@ -354,6 +354,91 @@ extern "C" int mcl_grid_frnn_csr(mcl_grid_t* G, const void* d_pos, float rad, vo
                     G->scan_tmp, (hipStream_t)stream, &G->timers[0], &G->timers[1]);
 }
 
+// ------------------------------------------------------------------------
+// The grid FRNN without any allocation (mcl_grid_frnn_csr_noalloc): the
+// counting sort's buffers live in `scratch`, a file-scope device array of
+// fixed size that every GPU holds from module load until the process ends.
+// Its device address is obtained once per device with hipGetSymbolAddress
+// (the array's name in host code is not a device address).  All calls on one
+// GPU share the array: calls that overlap on one GPU (concurrent streams or
+// host threads) race; each call clears what must start at zero.
+namespace {
+constexpr int kNaMaxPoints = 1 << 20;  // points per call
+constexpr int kNaMaxKeys = 1 << 20;    // cell keys (cells, or the Morton square)
+constexpr int kNaScanInts = (((kNaMaxPoints > kNaMaxKeys ? kNaMaxPoints : kNaMaxKeys) + 1 + 1023) / 1024) + 1 + 16;
+constexpr size_t kNaInts = 3 * (size_t)(kNaMaxPoints + 1)       // key, perm, ind
+                           + 2 * (size_t)(kNaMaxKeys + 1)       // start, fill
+                           + (size_t)kNaScanInts                // scan partials
+                           + 2 * (size_t)(kNaMaxPoints + 64);   // sorted positions (float2), 256-byte aligned
+constexpr int kNaMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) int scratch[kNaInts];
+
+static std::mutex g_na_mu;
+static int* g_na_ptr[kNaMaxDevices];
+
+// scratch's device address on the current device (once per device)
+static int na_scratch(int** out) {
+    int dev = -1;
+    MCL_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNaMaxDevices) return MCL_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fprintf(stderr, "mcleap: hipGetSymbolAddress(scratch) failed on device %d: %s\n", dev,
+                    hipGetErrorString(e));
+            return MCL_ERR_HIP;
+        }
+        g_na_ptr[dev] = (int*)p;
+    }
+    *out = g_na_ptr[dev];
+    return MCL_OK;
+}
+
+extern "C" int mcl_grid_frnn_csr_noalloc(const void* d_pos, int nump, float minx, float miny, float csx, float csy,
+                                         int ncx, int ncy, int flags, float rad, void* d_perm, void* d_offsets,
+                                         void* d_nbrs, int cap, long long* total, void* stream) {
+    if (!d_pos || !d_offsets || nump < 0) return MCL_ERR_INPUT;
+    GridParams g;
+    MCL_TRY(grid_params(minx, miny, csx, csy, ncx, ncy, flags, &g));
+    if (nump > kNaMaxPoints || g.nkeys > kNaMaxKeys) {
+        fprintf(stderr,
+                "mcleap: mcl_grid_frnn_csr_noalloc takes at most %d points and %d cell keys (got %d points, %d keys)\n",
+                kNaMaxPoints, kNaMaxKeys, nump, g.nkeys);
+        return MCL_ERR_CAPACITY;
+    }
+    int* base = nullptr;
+    MCL_TRY(na_scratch(&base));
+    hipStream_t s = (hipStream_t)stream;
+    SortBufs b;
+    int* q = base;
+    b.key = q; q += kNaMaxPoints + 1;
+    b.perm = q; q += kNaMaxPoints + 1;
+    b.ind = q; q += kNaMaxPoints + 1;
+    b.start = q; q += kNaMaxKeys + 1;
+    b.fill = q; q += kNaMaxKeys + 1;
+    b.scan_tmp = q; q += kNaScanInts;
+    q = (int*)(((uintptr_t)q + 255) & ~(uintptr_t)255);
+    b.spos = (float2*)q;
+    b.svel = nullptr;
+    if (nump == 0) {
+        hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, (int*)d_offsets, 1);
+        MCL_HIP(hipGetLastError());
+        if (total) {
+            MCL_HIP(hipStreamSynchronize(s));
+            *total = 0;
+        }
+        return MCL_OK;
+    }
+    MCL_TRY(grid_sort(g, (const float2*)d_pos, nullptr, nump, b, s));
+    if (d_perm) MCL_HIP(hipMemcpyAsync(d_perm, b.perm, sizeof(int) * nump, hipMemcpyDeviceToDevice, s));
+    return frnn_csr(g, b.spos, nump, b.start, rad, (int*)d_offsets, (int*)d_nbrs, cap, total, b.scan_tmp, s, nullptr,
+                    nullptr);
+}
+
 extern "C" int mcl_grid_mean_occupancy(mcl_grid_t* G, float* out, void* stream) {
     if (!G || !out) return MCL_ERR_INPUT;
     hipStream_t s = (hipStream_t)stream;
