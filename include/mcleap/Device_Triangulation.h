@@ -79,11 +79,20 @@ struct DeviceTriangulation {
     bool integrateMovePoints(float* dt, vec2* v) {
         return mcleap::ok(mcl_tri_integrate(t_, dt, v, nullptr), "integrateMovePoints");
     }
-    // declared and left unimplemented in the reference
-    bool moveFlipflop(int, vec2) { return false; }
-    bool moveFlipflopDelaunay(int, vec2) { return false; }
-    bool moveUntangling(int, vec2) { return false; }
-    bool moveUntanglingDelaunay(int, vec2) { return false; }
+    // declared in the reference and left as stubs returning false
+    // (Device_Triangulation.h:183-186, 411-417): remove and re-insert
+    // p_index at its position + d (then Lawson rounds), or move it and
+    // untangle (then Lawson rounds); moveVertices does the same for many
+    // vertices at once (host arrays ids, d)
+    bool moveFlipflop(int p_index, vec2 d) { return moveVertices(&p_index, &d, 1, MCL_MOVE_FLIPFLOP); }
+    bool moveFlipflopDelaunay(int p_index, vec2 d) { return moveVertices(&p_index, &d, 1, MCL_MOVE_FLIPFLOP_DELAUNAY); }
+    bool moveUntangling(int p_index, vec2 d) { return moveVertices(&p_index, &d, 1, MCL_MOVE_UNTANGLING); }
+    bool moveUntanglingDelaunay(int p_index, vec2 d) {
+        return moveVertices(&p_index, &d, 1, MCL_MOVE_UNTANGLING_DELAUNAY);
+    }
+    bool moveVertices(const int* ids, const vec2* d, int n, int mode) {
+        return mcl_tri_move_vertices(t_, ids, reinterpret_cast<const float*>(d), n, mode, &last_stats, nullptr) == MCL_OK;
+    }
 
     // -------------------------------------------
     // untangling (Device_Triangulation.h:422-495): false when inverted triangles remain

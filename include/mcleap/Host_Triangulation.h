@@ -130,11 +130,21 @@ struct HostTriangulation {
         m_pos[m_v[p_index].pos_index] += d;
         return true;
     }
-    // the reference declares these and leaves them unimplemented (returns false)
-    bool moveFlipflop(int, vec2) { return false; }
-    bool moveFlipflopDelaunay(int, vec2) { return false; }
-    bool moveUntangling(int p_index, vec2 d) { return movePoint(p_index, d) && untangle(); }
-    bool moveUntanglingDelaunay(int p_index, vec2 d) { return moveUntangling(p_index, d) && delonize(); }
+    // declared and left unimplemented in the reference (Host_Triangulation.h:77-80,
+    // 408-413): remove and re-insert p_index at its position + d (then Lawson
+    // flips), or move it and untangle (then Lawson flips); false when the
+    // vertex could not get there (a target on another vertex or outside the
+    // mesh) or, untangling, a triangle stays inverted
+    bool moveFlipflop(int p_index, vec2 d) { return moveVertex(p_index, d, MCL_MOVE_FLIPFLOP); }
+    bool moveFlipflopDelaunay(int p_index, vec2 d) { return moveVertex(p_index, d, MCL_MOVE_FLIPFLOP_DELAUNAY); }
+    bool moveUntangling(int p_index, vec2 d) { return moveVertex(p_index, d, MCL_MOVE_UNTANGLING); }
+    bool moveUntanglingDelaunay(int p_index, vec2 d) { return moveVertex(p_index, d, MCL_MOVE_UNTANGLING_DELAUNAY); }
+    bool moveVertex(int p_index, vec2 d, int mode) {
+        long long f = 0;
+        const float dd[2] = {d.x, d.y};
+        return mcl_hmesh_move_vertex(fp(), ip(m_he), (int)m_he.size(), ip(m_t), (int)m_t.size(), ip(m_v),
+                                     (int)m_v.size(), periodic, p_index, dd, mode, &f) == MCL_OK;
+    }
 
     // -------------------------------------------
     // untangling (Host_Triangulation.h:417-471): one pass of B/C steps

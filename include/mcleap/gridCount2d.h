@@ -5,8 +5,9 @@
 // exposes d_ind (new slot of each old index), as the reference does.
 // apply_f_frnn<F>(f, pos, rad) calls the caller's device functor
 //   __device__ void operator()(const int& i, const int& j, const vec2 d, const double dist)
-// for every pair of sorted slots within rad (d = pos[j] - pos[i], FP32
-// squared distance <= rad*rad), i == j included, like the reference.  The
+// for every pair of sorted slots within rad (d = pos[j] - pos[i], the
+// minimum image on a periodic grid; FP32 squared distance <= rad*rad),
+// i == j included, like the reference.  The
 // pairs come from the library's CSR query; a small adapter kernel defined
 // here runs the functor, so a translation unit that uses apply_f_frnn is
 // compiled with hipcc.
@@ -19,9 +20,11 @@
 #include "Common_Triangulation.h"
 
 namespace mcleap {
+// periodic != 0: d is the minimum image on the unit torus (each component
+// wrapped into [-0.5, 0.5]), as the library's query measured it
 template <class Functor, class V>
 __global__ void apply_f_frnn_csr_kernel(Functor f, int n, const V* __restrict__ pos, const int* __restrict__ off,
-                                        const int* __restrict__ nbr) {
+                                        const int* __restrict__ nbr, int periodic) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const V pi = pos[i];
@@ -34,6 +37,10 @@ __global__ void apply_f_frnn_csr_kernel(Functor f, int n, const V* __restrict__ 
         V d;
         d.x = __fsub_rn(pos[j].x, pi.x);
         d.y = __fsub_rn(pos[j].y, pi.y);
+        if (periodic) {
+            d.x = d.x > 0.5f ? d.x - 1.0f : (d.x < -0.5f ? d.x + 1.0f : d.x);
+            d.y = d.y > 0.5f ? d.y - 1.0f : (d.y < -0.5f ? d.y + 1.0f : d.y);
+        }
         const float r2 = __fadd_rn(__fmul_rn(d.x, d.x), __fmul_rn(d.y, d.y));
         f(i, j, d, (double)sqrtf(r2));
     }
@@ -49,7 +56,8 @@ struct GridCount2d {
 
     // flags: MCL_GRID_ROWMAJOR (the reference's cell key), MCL_GRID_MORTON,
     // optionally | MCL_GRID_PERIODIC (wrapped cells, minimum-image distances)
-    GridCount2d(int numP_, vec2 min, vec2 cell_size, ivec2 num_cells, int flags = MCL_GRID_ROWMAJOR) : numP(numP_) {
+    GridCount2d(int numP_, vec2 min, vec2 cell_size, ivec2 num_cells, int flags = MCL_GRID_ROWMAJOR)
+        : numP(numP_), periodic_((flags & MCL_GRID_PERIODIC) ? 1 : 0) {
         mcleap::must(mcl_grid_create(numP, min.x, min.y, cell_size.x, cell_size.y, num_cells.x, num_cells.y, flags, &g_),
                      "GridCount2d");
         void *ind = nullptr, *perm = nullptr, *start = nullptr, *cnt = nullptr;
@@ -80,7 +88,7 @@ struct GridCount2d {
         }
         if (!mcleap::ok(mcl_grid_frnn_csr(g_, pos, rad, off_, nbr_, (int)cap_, &total, nullptr), "apply_f_frnn")) return;
         hipLaunchKernelGGL((mcleap::apply_f_frnn_csr_kernel<Functor, vec2>), dim3((numP + blocksize - 1) / blocksize),
-                           dim3(blocksize), 0, 0, f, numP, pos, off_, nbr_);
+                           dim3(blocksize), 0, 0, f, numP, pos, off_, nbr_, periodic_);
         (void)hipDeviceSynchronize();
     }
 
@@ -92,6 +100,7 @@ struct GridCount2d {
     mcl_grid_t* handle() { return g_; }
 
   private:
+    int periodic_ = 0;
     mcl_grid_t* g_ = nullptr;
     int* off_ = nullptr;
     int* nbr_ = nullptr;

@@ -109,6 +109,24 @@ int mcl_hmesh_flip(float* pos, int* he4, int nhe, int* t1, int nt, int* v2, int 
  * unflipped and reported as MCL_ERR_UNRESOLVED. */
 int mcl_hmesh_delonize(float* pos, int* he4, int nhe, int* t1, int nt, int* v2, int nv, int periodic,
                        const int* seeds, int nseeds, long long* flips);
+/* moveFlipflop / moveFlipflopDelaunay / moveUntangling / moveUntanglingDelaunay
+ * (Host_Triangulation.h:77-80, declared and left unimplemented there,
+ * :408-413) on the host arrays: vertex v moves by d (wrapped on the torus).
+ * MCL_MOVE_FLIPFLOP: v is removed from its star (flips down to degree 3 and a
+ * 3 -> 1 merge) and re-inserted at the target (1 -> 3, or 2 -> 4 on an edge);
+ * every triangle stays strictly CCW, the mesh need not be Delaunay.
+ * MCL_MOVE_FLIPFLOP_DELAUNAY: then Lawson flips from the rewritten triangles.
+ * MCL_MOVE_UNTANGLING: v simply moves, then B/C untangle passes run until
+ * none flips; MCL_ERR_UNRESOLVED if a triangle stays inverted.
+ * MCL_MOVE_UNTANGLING_DELAUNAY: then Lawson flips over every edge.
+ * A target on another vertex gives MCL_ERR_DUPLICATE, one outside the mesh
+ * (or a vertex on the outer face) MCL_ERR_INPUT; *flips = flips made. */
+#define MCL_MOVE_FLIPFLOP 0
+#define MCL_MOVE_FLIPFLOP_DELAUNAY 1
+#define MCL_MOVE_UNTANGLING 2
+#define MCL_MOVE_UNTANGLING_DELAUNAY 3
+int mcl_hmesh_move_vertex(float* pos, int* he4, int nhe, int* t1, int nt, int* v2, int nv, int periodic,
+                          int v, const float* d, int mode, long long* flips);
 /* untangle (Host_Triangulation.h:417-471): one pass of the B/C steps over all
  * edges with exact orientation; *flips = flips made */
 int mcl_hmesh_untangle(float* pos, int* he4, int nhe, int* t1, int nt, int* v2, int nv, int periodic,
@@ -187,6 +205,18 @@ int mcl_tri_move(mcl_tri_t* t, const void* d_disp, void* stream);
  * (Device_Triangulation.h:403-410, kernels.cuh:45-50) */
 int mcl_tri_integrate(mcl_tri_t* t, const void* d_dt, const void* d_vel, void* stream);
 
+/* moveFlipflop / moveFlipflopDelaunay / moveUntangling / moveUntanglingDelaunay
+ * (Device_Triangulation.h:183-186, stubs returning false at :411-417) for n
+ * vertices at once: ids[i] (host array, distinct, not box corners) moves by
+ * disp[2i], disp[2i+1] (host array).  Modes as mcl_hmesh_move_vertex; the
+ * flip-flop modes run the kinetic step's relocation rounds on the listed
+ * vertices (claims, parallel removal and re-insertion), then for
+ * MCL_MOVE_FLIPFLOP_DELAUNAY the Lawson rounds.  MCL_ERR_UNRESOLVED (mesh
+ * still valid) when a vertex did not reach its target (st->stalled,
+ * st->duplicates) or, in the untangling modes, triangles stay inverted.
+ * The targets are readable through mcl_tri_target_ptr afterwards. */
+int mcl_tri_move_vertices(mcl_tri_t* t, const int* ids, const float* disp, int n, int mode, mcl_stats_t* st,
+                          void* stream);
 /* delonize / delonize2 (Device_Triangulation.h:324-390): parallel Lawson flip
  * rounds until no edge is illegal.  Requires a mesh with no inverted
  * triangle.  Exact incircle; MCL_ERR_ROUND_CAP if max_rounds is hit

@@ -26,6 +26,10 @@ MOTION_SHEAR = 1
 GRID_ROWMAJOR = 0
 GRID_MORTON = 1
 GRID_PERIODIC = 2
+MOVE_FLIPFLOP = 0
+MOVE_FLIPFLOP_DELAUNAY = 1
+MOVE_UNTANGLING = 2
+MOVE_UNTANGLING_DELAUNAY = 3
 OPT_STAR_MIN_VERTICES = 1
 OPT_STAR_ENABLE = 2
 OPT_STAR_GIVE_UP = 3
@@ -98,6 +102,10 @@ SIGNATURES = {
     "mcl_tri_pos_ptr": [_p],
     "mcl_tri_target_ptr": [_p],
     "mcl_tri_move": [_p, _p, _p],
+    "mcl_tri_move_vertices": [_p, _p, _p, _i, _i, _p, _p],
+    "mcl_hmesh_move_vertex": [_p, _p, _i, _p, _i, _p, _i, _i, _i, _p, _i, _p],
+    "mcl_hmesh_delonize": [_p, _p, _i, _p, _i, _p, _i, _i, _p, _i, _p],
+    "mcl_hmesh_untangle": [_p, _p, _i, _p, _i, _p, _i, _i, _p],
     "mcl_tri_integrate": [_p, _p, _p, _p],
     "mcl_tri_delonize": [_p, _i, _i, _p, _p],
     "mcl_tri_untangle": [_p, _i, _p, _p],

@@ -16,7 +16,9 @@
 // device memory in round-indexed slots (round % 3), each slot cleared by the
 // kernel two steps before its next use, so no memset is needed between
 // launches (safe under graph capture).
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -177,7 +179,7 @@ __device__ __forceinline__ unsigned long long claim_key(int round, int e) {
            (unsigned long long)(uint32_t)e;
 }
 
-__device__ __forceinline__ void flip_soa(const DMesh& m, int e, int& he2, int& he3, int& he4, int& he5) {
+MCL_HD void flip_soa(const DMesh& m, int e, int& he2, int& he3, int& he4, int& he5) {
     const int he0 = 2 * e, he1 = he0 + 1;
     const int2 nx = reinterpret_cast<const int2*>(m.he_next)[e];
     he2 = nx.x;
@@ -748,7 +750,7 @@ constexpr int kMaxStar = 64;
 // is then in *on.
 constexpr int kWalkCap = 2048;
 template <bool kF32>
-__device__ int locate_point(const DMesh& m, float2 p, int t, int* on, unsigned seed) {
+MCL_HD int locate_point(const DMesh& m, float2 p, int t, int* on, unsigned seed) {
     int came = -1;  // half-edge of t we entered through
     unsigned r = seed * 0x9E3779B9u + 0x7F4A7C15u;
     for (int step = 0; step < kWalkCap; step++) {
@@ -851,7 +853,7 @@ __global__ void __launch_bounds__(kThreads) k_reloc_claim(DMesh m, const int* __
                      gridDim.x * blockDim.x);
 }
 
-__device__ __forceinline__ void set_he(const DMesh& m, int h, int nx, int v, int t, int o) {
+MCL_HD void set_he(const DMesh& m, int h, int nx, int v, int t, int o) {
     m.he_next[h] = nx;
     m.he_v[h] = v;
     m.he_t[h] = t;
@@ -860,7 +862,7 @@ __device__ __forceinline__ void set_he(const DMesh& m, int h, int nx, int v, int
 
 // containment of p in triangle X: 1 inside (on_edge = half-edge or -1), 0 outside
 template <bool kF32>
-__device__ __forceinline__ int tri_contains(const DMesh& m, int X, float2 p, int* on_edge) {
+MCL_HD int tri_contains(const DMesh& m, int X, float2 p, int* on_edge) {
     const int h0 = m.t_he[X];
     const int hs[3] = {h0, m.he_next[h0], m.he_next[m.he_next[h0]]};
     int zero = -1, st;
@@ -878,7 +880,7 @@ __device__ __forceinline__ int tri_contains(const DMesh& m, int X, float2 p, int
 // lies in the closed triangle, i.e. when the diagonal (c, b) runs inside the
 // polygon and touches its boundary only at c and b.  h: the spoke v -> a.
 template <bool kF32>
-__device__ __forceinline__ bool ear_empty(const DMesh& m, int h, int a, int b, int c, float2 pa, float2 pb,
+MCL_HD bool ear_empty(const DMesh& m, int h, int a, int b, int c, float2 pa, float2 pb,
                                           float2 pc) {
     int st;
     int g = m.he_next[h ^ 1], guard = 0;
@@ -893,6 +895,131 @@ __device__ __forceinline__ bool ear_empty(const DMesh& m, int h, int a, int b, i
         g = m.he_next[g ^ 1];
     }
     return true;
+}
+
+// Removal and re-insertion of vertex v (the mesh part of a relocation; the
+// caller holds every triangle involved): star[0..deg0) are the triangles
+// around v, tt (and tt2 when the target lies on an edge) the triangle(s)
+// holding the target pt.  Returns 0 when v is at pt, 1 when the removal found
+// no proper ear (mesh unchanged but for the flips made), 2 when v had to be
+// put back at its old position.  Host and device.
+template <bool kF32>
+MCL_HD int reloc_core(const DMesh& m, int v, const int* star, int deg0, int tt, int tt2, float2 pt) {
+    int deg = deg0;
+    const float2 pv = m.pos[v];
+    int st;
+    // --- remove: flip spokes (v, a) down to degree 3.  Each flip clips
+    // the ear (c, a, b) off v's link polygon (CCW order c, a, b) and
+    // adds the ear triangle, which must be strictly CCW.  First only ears
+    // that keep v strictly inside the rest (both new triangles strictly
+    // CCW).  A link with collinear vertices can have none: v on the line
+    // (c, b) of every convex ear, as at the centre of a square.  Then any
+    // proper ear is clipped; v's own triangles may turn degenerate, but
+    // they all disappear in the merge, and the ears of a simple polygon
+    // with four or more vertices always include a proper one (Meisters'
+    // two-ears theorem), whose last remaining triangle is strictly CCW.
+    bool strict = true;
+    while (deg > 3) {
+        bool flipped = false;
+        const int h0 = m.v_he[v];
+        int h = h0, guard = 0;
+        do {
+            const int a = m.he_v[h ^ 1], b = m.he_op[h], c = m.he_op[h ^ 1];
+            const float2 pa = m.pos[a], pb = m.pos[b], pc = m.pos[c];
+            if (orient<kF32>(pb, pc, pa, m.periodic, &st) > 0 &&
+                (strict ? orient<kF32>(pc, pb, pv, m.periodic, &st) > 0
+                        : ear_empty<kF32>(m, h, a, b, c, pa, pb, pc))) {
+                int h2, h3, h4, h5;
+                flip_soa(m, h >> 1, h2, h3, h4, h5);  // keeps v.he an outgoing half-edge of v
+                flipped = true;
+                break;
+            }
+            h = m.he_next[h ^ 1];
+        } while (h != h0 && ++guard < kMaxStar);
+        if (!flipped) {
+            if (!strict) break;
+            strict = false;
+            continue;
+        }
+        deg--;
+    }
+    if (deg > 3) return 1;  // no proper ear (a link that is not a simple polygon): mesh still valid
+    // --- merge the three triangles around v into one
+    int hv[3], av[3], tv[3], gv[3], ov[3];
+    {
+        int h = m.v_he[v];
+        for (int k = 0; k < 3; k++) {
+            hv[k] = h;
+            av[k] = m.he_v[h ^ 1];
+            tv[k] = m.he_t[h];
+            gv[k] = m.he_next[h];  // link half-edge a_k -> op(h_k)
+            ov[k] = m.he_op[h];
+            h = m.he_next[h ^ 1];
+        }
+    }
+    const int T0 = tv[0], F1 = tv[1], F2 = tv[2];
+    const int E0 = hv[0] >> 1, E1 = hv[1] >> 1, E2 = hv[2] >> 1;
+    const int vsum = av[0] + av[1] + av[2];
+    for (int k = 0; k < 3; k++) {
+        int j = 0;
+        for (int q = 0; q < 3; q++)
+            if (av[q] == ov[k]) j = q;
+        m.he_next[gv[k]] = gv[j];
+        m.he_t[gv[k]] = T0;
+        m.he_op[gv[k]] = vsum - av[k] - ov[k];
+        m.v_he[av[k]] = gv[k];
+    }
+    m.t_he[T0] = gv[0];
+    // --- the triangle that receives v: search the claimed live triangles
+    int cand[kMaxStar + 2], nc = 0;
+    for (int k = 0; k < deg0; k++)
+        if (star[k] != F1 && star[k] != F2) cand[nc++] = star[k];
+    bool tt_known = false, tt2_known = tt2 < 0;
+    for (int k = 0; k < nc; k++) { tt_known |= cand[k] == tt; tt2_known |= cand[k] == tt2; }
+    if (!tt_known && tt != F1 && tt != F2) cand[nc++] = tt;
+    if (!tt2_known && tt2 != F1 && tt2 != F2) cand[nc++] = tt2;
+    float2 pins = pt;
+    int X = -1, on = -1;
+    for (int pass = 0; pass < 2 && X < 0; pass++) {
+        for (int k = 0; k < nc && X < 0; k++) {
+            int oe = -1;
+            if (tri_contains<kF32>(m, cand[k], pins, &oe)) {
+                if (oe >= 0) {  // on an edge: the other side must be ours and live
+                    const int tw = m.he_t[oe ^ 1];
+                    bool ours = false;
+                    for (int q = 0; q < nc; q++) ours |= cand[q] == tw;
+                    if (!ours) continue;
+                }
+                X = cand[k];
+                on = oe;
+            }
+        }
+        if (X < 0) pins = pv;  // cannot place the target: put v back where it was
+    }
+    // the old position lies strictly inside the merged star polygon, so X >= 0 here
+    const int n0 = 2 * E0, n1 = 2 * E1, n2 = 2 * E2;
+    if (on < 0) {
+        const int x0 = m.t_he[X], x1 = m.he_next[x0], x2 = m.he_next[x1];
+        const int a = m.he_v[x0], b = m.he_v[x1], c = m.he_v[x2];
+        set_he(m, x0, n0, a, X, v); set_he(m, n0, n2 ^ 1, b, X, a); set_he(m, n2 ^ 1, x0, v, X, b);
+        set_he(m, x1, n1, b, F1, v); set_he(m, n1, n0 ^ 1, c, F1, b); set_he(m, n0 ^ 1, x1, v, F1, c);
+        set_he(m, x2, n2, c, F2, v); set_he(m, n2, n1 ^ 1, a, F2, c); set_he(m, n1 ^ 1, x2, v, F2, a);
+        m.t_he[X] = x0; m.t_he[F1] = x1; m.t_he[F2] = x2;
+        m.v_he[v] = n2 ^ 1;
+    } else {
+        const int he0 = on, he1 = on ^ 1;
+        const int he2 = m.he_next[he0], he3 = m.he_next[he2], he4 = m.he_next[he1], he5 = m.he_next[he4];
+        const int t0 = X, t1 = m.he_t[he1];
+        const int a = m.he_v[he0], b = m.he_v[he1], c = m.he_v[he3], d = m.he_v[he5];
+        set_he(m, he0, n1, a, t0, c); set_he(m, n1, he3, v, t0, a); set_he(m, he3, he0, c, t0, v);
+        set_he(m, n0, he2, v, F1, c); set_he(m, he2, n1 ^ 1, b, F1, v); set_he(m, n1 ^ 1, n0, c, F1, b);
+        set_he(m, he1, he4, v, t1, d); set_he(m, he4, n2 ^ 1, a, t1, v); set_he(m, n2 ^ 1, he1, d, t1, a);
+        set_he(m, n0 ^ 1, n2, b, F2, d); set_he(m, n2, he5, v, F2, b); set_he(m, he5, n0 ^ 1, d, F2, v);
+        m.t_he[t0] = he0; m.t_he[F1] = n0; m.t_he[t1] = he1; m.t_he[F2] = n0 ^ 1;
+        m.v_he[v] = n0; m.v_he[a] = he0; m.v_he[b] = he2;
+    }
+    m.pos[v] = pins;
+    return (pins.x == pt.x && pins.y == pt.y) ? 0 : 2;
 }
 
 // phase 3: winners (all claims held) remove and re-insert their vertex.
@@ -941,125 +1068,9 @@ __device__ __forceinline__ void reloc_apply_body(const DMesh& m, const int* __re
             next_list[agg_inc(c_next)] = v;
             continue;
         }
-        const int deg0 = deg;
-        const float2 pv = m.pos[v], pt = tgt[v];
-        int st;
-        // --- remove: flip spokes (v, a) down to degree 3.  Each flip clips
-        // the ear (c, a, b) off v's link polygon (CCW order c, a, b) and
-        // adds the ear triangle, which must be strictly CCW.  First only ears
-        // that keep v strictly inside the rest (both new triangles strictly
-        // CCW).  A link with collinear vertices can have none: v on the line
-        // (c, b) of every convex ear, as at the centre of a square.  Then any
-        // proper ear is clipped; v's own triangles may turn degenerate, but
-        // they all disappear in the merge, and the ears of a simple polygon
-        // with four or more vertices always include a proper one (Meisters'
-        // two-ears theorem), whose last remaining triangle is strictly CCW.
-        bool strict = true;
-        while (deg > 3) {
-            bool flipped = false;
-            const int h0 = m.v_he[v];
-            int h = h0, guard = 0;
-            do {
-                const int a = m.he_v[h ^ 1], b = m.he_op[h], c = m.he_op[h ^ 1];
-                const float2 pa = m.pos[a], pb = m.pos[b], pc = m.pos[c];
-                if (orient<kF32>(pb, pc, pa, m.periodic, &st) > 0 &&
-                    (strict ? orient<kF32>(pc, pb, pv, m.periodic, &st) > 0
-                            : ear_empty<kF32>(m, h, a, b, c, pa, pb, pc))) {
-                    int h2, h3, h4, h5;
-                    flip_soa(m, h >> 1, h2, h3, h4, h5);  // keeps v.he an outgoing half-edge of v
-                    flipped = true;
-                    break;
-                }
-                h = m.he_next[h ^ 1];
-            } while (h != h0 && ++guard < kMaxStar);
-            if (!flipped) {
-                if (!strict) break;
-                strict = false;
-                continue;
-            }
-            deg--;
-        }
-        if (deg > 3) {  // no proper ear (a link that is not a simple polygon): mesh still valid
-            loc[1]++;
-            loc[4]++;
-            continue;
-        }
-        // --- merge the three triangles around v into one
-        int hv[3], av[3], tv[3], gv[3], ov[3];
-        {
-            int h = m.v_he[v];
-            for (int k = 0; k < 3; k++) {
-                hv[k] = h;
-                av[k] = m.he_v[h ^ 1];
-                tv[k] = m.he_t[h];
-                gv[k] = m.he_next[h];  // link half-edge a_k -> op(h_k)
-                ov[k] = m.he_op[h];
-                h = m.he_next[h ^ 1];
-            }
-        }
-        const int T0 = tv[0], F1 = tv[1], F2 = tv[2];
-        const int E0 = hv[0] >> 1, E1 = hv[1] >> 1, E2 = hv[2] >> 1;
-        const int vsum = av[0] + av[1] + av[2];
-        for (int k = 0; k < 3; k++) {
-            int j = 0;
-            for (int q = 0; q < 3; q++)
-                if (av[q] == ov[k]) j = q;
-            m.he_next[gv[k]] = gv[j];
-            m.he_t[gv[k]] = T0;
-            m.he_op[gv[k]] = vsum - av[k] - ov[k];
-            m.v_he[av[k]] = gv[k];
-        }
-        m.t_he[T0] = gv[0];
-        // --- the triangle that receives v: search the claimed live triangles
-        int cand[kMaxStar + 2], nc = 0;
-        for (int k = 0; k < deg0; k++)
-            if (star[k] != F1 && star[k] != F2) cand[nc++] = star[k];
-        bool tt_known = false, tt2_known = tt2 < 0;
-        for (int k = 0; k < nc; k++) { tt_known |= cand[k] == tt; tt2_known |= cand[k] == tt2; }
-        if (!tt_known && tt != F1 && tt != F2) cand[nc++] = tt;
-        if (!tt2_known && tt2 != F1 && tt2 != F2) cand[nc++] = tt2;
-        float2 pins = pt;
-        int X = -1, on = -1;
-        for (int pass = 0; pass < 2 && X < 0; pass++) {
-            for (int k = 0; k < nc && X < 0; k++) {
-                int oe = -1;
-                if (tri_contains<kF32>(m, cand[k], pins, &oe)) {
-                    if (oe >= 0) {  // on an edge: the other side must be ours and live
-                        const int tw = m.he_t[oe ^ 1];
-                        bool ours = false;
-                        for (int q = 0; q < nc; q++) ours |= cand[q] == tw;
-                        if (!ours) continue;
-                    }
-                    X = cand[k];
-                    on = oe;
-                }
-            }
-            if (X < 0) pins = pv;  // cannot place the target: put v back where it was
-        }
-        // the old position lies strictly inside the merged star polygon, so X >= 0 here
-        const int n0 = 2 * E0, n1 = 2 * E1, n2 = 2 * E2;
-        if (on < 0) {
-            const int x0 = m.t_he[X], x1 = m.he_next[x0], x2 = m.he_next[x1];
-            const int a = m.he_v[x0], b = m.he_v[x1], c = m.he_v[x2];
-            set_he(m, x0, n0, a, X, v); set_he(m, n0, n2 ^ 1, b, X, a); set_he(m, n2 ^ 1, x0, v, X, b);
-            set_he(m, x1, n1, b, F1, v); set_he(m, n1, n0 ^ 1, c, F1, b); set_he(m, n0 ^ 1, x1, v, F1, c);
-            set_he(m, x2, n2, c, F2, v); set_he(m, n2, n1 ^ 1, a, F2, c); set_he(m, n1 ^ 1, x2, v, F2, a);
-            m.t_he[X] = x0; m.t_he[F1] = x1; m.t_he[F2] = x2;
-            m.v_he[v] = n2 ^ 1;
-        } else {
-            const int he0 = on, he1 = on ^ 1;
-            const int he2 = m.he_next[he0], he3 = m.he_next[he2], he4 = m.he_next[he1], he5 = m.he_next[he4];
-            const int t0 = X, t1 = m.he_t[he1];
-            const int a = m.he_v[he0], b = m.he_v[he1], c = m.he_v[he3], d = m.he_v[he5];
-            set_he(m, he0, n1, a, t0, c); set_he(m, n1, he3, v, t0, a); set_he(m, he3, he0, c, t0, v);
-            set_he(m, n0, he2, v, F1, c); set_he(m, he2, n1 ^ 1, b, F1, v); set_he(m, n1 ^ 1, n0, c, F1, b);
-            set_he(m, he1, he4, v, t1, d); set_he(m, he4, n2 ^ 1, a, t1, v); set_he(m, n2 ^ 1, he1, d, t1, a);
-            set_he(m, n0 ^ 1, n2, b, F2, d); set_he(m, n2, he5, v, F2, b); set_he(m, he5, n0 ^ 1, d, F2, v);
-            m.t_he[t0] = he0; m.t_he[F1] = n0; m.t_he[t1] = he1; m.t_he[F2] = n0 ^ 1;
-            m.v_he[v] = n0; m.v_he[a] = he0; m.v_he[b] = he2;
-        }
-        m.pos[v] = pins;
-        if (pins.x == pt.x && pins.y == pt.y) loc[0]++;
+        const int rc = reloc_core<kF32>(m, v, star, deg, tt, tt2, tgt[v]);
+        if (rc == 0) loc[0]++;
+        else if (rc == 1) { loc[1]++; loc[4]++; }
         else { loc[1]++; loc[5]++; }
     }
 }
@@ -1714,6 +1725,51 @@ static int tri_free(mcl_tri* t) {
     return MCL_OK;
 }
 
+// Host-side structure check of AoS arrays before they reach the device
+// (kernels follow these indices unchecked): every index in range, finite
+// positions, next^3 = id around each triangle with one face per cycle,
+// dest(h) = org(next(h)), op(h) = org(next(next(h))), the outer face's
+// half-edges (t = -1) chained among themselves, v.he leaving its vertex and
+// t.he lying on its triangle.  MCL_ERR_INPUT on the first violation.
+static int check_aos(int nv, int nhe, int nt, const float* pos, const int* v2, const int* he4, const int* t1) {
+    auto bad = [](const char* what, long long i) {
+        fprintf(stderr, "mcleap: mesh arrays rejected: %s (element %lld)\n", what, i);
+        return MCL_ERR_INPUT;
+    };
+    if (pos)
+        for (long long i = 0; i < 2LL * nv; i++)
+            if (!std::isfinite(pos[i])) return bad("position not finite", i / 2);
+    if (!he4 && !v2 && !t1) return MCL_OK;
+    if (!he4 || !v2 || !t1) return bad("half-edges, vertices and triangles must come together", 0);
+    for (int h = 0; h < nhe; h++) {
+        const int nx = he4[4 * h], v = he4[4 * h + 1], t = he4[4 * h + 2], op = he4[4 * h + 3];
+        if (nx < 0 || nx >= nhe) return bad("next out of range", h);
+        if (v < 0 || v >= nv) return bad("origin out of range", h);
+        if (t < -1 || t >= nt) return bad("triangle out of range", h);
+        if (op < -1 || op >= nv) return bad("opposite vertex out of range", h);
+    }
+    for (int h = 0; h < nhe; h++) {
+        const int nx = he4[4 * h], t = he4[4 * h + 2];
+        if (he4[4 * nx + 2] != t) return bad("next leaves the face", h);
+        if (he4[4 * nx + 1] != he4[4 * (h ^ 1) + 1]) return bad("org(next(h)) != dest(h)", h);
+        if (t >= 0) {
+            const int n2 = he4[4 * nx];
+            if (he4[4 * n2] != h) return bad("next^3 != id on a triangle", h);
+            if (he4[4 * h + 3] != he4[4 * n2 + 1]) return bad("op(h) != org(next(next(h)))", h);
+        }
+    }
+    for (int v = 0; v < nv; v++) {
+        const int hv = v2[2 * v + 1];
+        if (v2[2 * v] < 0 || v2[2 * v] >= nv) return bad("vertex position index out of range", v);
+        if (hv < 0 || hv >= nhe || he4[4 * hv + 1] != v) return bad("v.he does not leave its vertex", v);
+    }
+    for (int t = 0; t < nt; t++) {
+        const int ht = t1[t];
+        if (ht < 0 || ht >= nhe || he4[4 * ht + 2] != t) return bad("t.he not on its triangle", t);
+    }
+    return MCL_OK;
+}
+
 static int upload_aos(mcl_tri* t, const float* pos, const int* v2, const int* he4, const int* t1) {
     DMesh& m = t->m;
     std::vector<int> a(m.nhe), b(m.nhe), c(m.nhe), d(m.nhe);
@@ -1817,6 +1873,7 @@ extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv,
         for (int v = k; ok && v < nv; v++) ok = !corner_at(v);
         if (ok) ncorner = k;
     }
+    MCL_TRY(check_aos(nv, nhe, nt, pos, v2, he4, t1));
     mcl_tri* t = nullptr;
     int rc = tri_make(nv, nhe, nt, periodic, ncorner, &t);
     if (rc != MCL_OK) return rc;
@@ -1847,6 +1904,7 @@ extern "C" void* mcl_tri_pos_ptr(mcl_tri_t* t) { return t ? (void*)t->m.pos : nu
 
 extern "C" int mcl_tri_upload(mcl_tri_t* t, const float* pos, const int* v2, const int* he4, const int* t1) {
     if (!t) return MCL_ERR_INPUT;
+    MCL_TRY(check_aos(t->m.nv, t->m.nhe, t->m.nt, pos, v2, he4, t1));
     t->stars_valid = false;
     t->mesh_stale = false;
     return upload_aos(t, pos, v2, he4, t1);
@@ -1913,6 +1971,182 @@ extern "C" int mcl_tri_integrate(mcl_tri_t* t, const void* d_dt, const void* d_v
     MCL_HIP(hipGetLastError());
     MCL_HIP(hipStreamSynchronize(s));
     return MCL_OK;
+}
+
+// moveFlipflop on the host arrays: vertex v removed and re-inserted at
+// pos + d by the relocation's removal / insertion (reloc_core), then, for
+// MCL_MOVE_FLIPFLOP_DELAUNAY, Lawson flips from the edges of the triangles
+// it rewrote; MCL_MOVE_UNTANGLING(_DELAUNAY): pos += d, B/C untangle passes
+// until none flips, then (if no triangle stays inverted) Lawson flips.
+extern "C" int mcl_hmesh_move_vertex(float* pos, int* he4, int nhe, int* t1, int nt, int* v2, int nv, int periodic,
+                                     int v, const float* d, int mode, long long* flips) {
+    if (!pos || !he4 || !t1 || !v2 || !d || v < 0 || v >= nv || mode < 0 || mode > 3) return MCL_ERR_INPUT;
+    if (!std::isfinite(d[0]) || !std::isfinite(d[1])) return MCL_ERR_INPUT;
+    MCL_TRY(check_aos(nv, nhe, nt, nullptr, v2, he4, t1));
+    long long nf = 0;
+    if (flips) *flips = 0;
+    const int pv = v2[2 * v];
+    float2 pt = make_float2(pos[2 * pv] + d[0], pos[2 * pv + 1] + d[1]);
+    if (periodic) {
+        pt.x -= floorf(pt.x);
+        pt.y -= floorf(pt.y);
+        if (pt.x >= 1.0f) pt.x = 0.0f;
+        if (pt.y >= 1.0f) pt.y = 0.0f;
+    }
+    if (mode >= 2) {
+        pos[2 * pv] = pt.x;
+        pos[2 * pv + 1] = pt.y;
+        for (int pass = 0; pass < 1024; pass++) {
+            long long f = 0;
+            MCL_TRY(mcl_hmesh_untangle(pos, he4, nhe, t1, nt, v2, nv, periodic, &f));
+            nf += f;
+            if (f == 0) break;
+        }
+        if (flips) *flips = nf;
+        for (int t = 0; t < nt; t++) {  // any triangle still not strictly CCW?
+            const int h0 = t1[t], h1 = he4[4 * h0], h2 = he4[4 * h1];
+            const int a = v2[2 * he4[4 * h0 + 1]], b = v2[2 * he4[4 * h1 + 1]], c = v2[2 * he4[4 * h2 + 1]];
+            if (mcl_orient2d(pos + 2 * a, pos + 2 * b, pos + 2 * c, periodic) <= 0) return MCL_ERR_UNRESOLVED;
+        }
+        if (mode == 3) {
+            long long f = 0;
+            const int rc = mcl_hmesh_delonize(pos, he4, nhe, t1, nt, v2, nv, periodic, nullptr, 0, &f);
+            if (flips) *flips = nf + f;
+            return rc;
+        }
+        return MCL_OK;
+    }
+    // the mesh as SoA host arrays (positions per vertex)
+    std::vector<float2> P(nv);
+    std::vector<int> nx(nhe), hv(nhe), ht(nhe), ho(nhe), vh(nv), th(t1, t1 + nt);
+    for (int i = 0; i < nv; i++) {
+        P[i] = make_float2(pos[2 * v2[2 * i]], pos[2 * v2[2 * i] + 1]);
+        vh[i] = v2[2 * i + 1];
+    }
+    for (int h = 0; h < nhe; h++) {
+        nx[h] = he4[4 * h]; hv[h] = he4[4 * h + 1]; ht[h] = he4[4 * h + 2]; ho[h] = he4[4 * h + 3];
+    }
+    DMesh m;
+    m.nv = nv; m.nhe = nhe; m.nt = nt; m.ne = nhe / 2; m.periodic = periodic != 0;
+    m.pos = P.data(); m.he_next = nx.data(); m.he_v = hv.data(); m.he_t = ht.data(); m.he_op = ho.data();
+    m.v_he = vh.data(); m.t_he = th.data();
+    if (pt.x == P[v].x && pt.y == P[v].y) return MCL_OK;
+    // v's star: every triangle around it (a vertex on the outer face cannot be moved this way)
+    int star[kMaxStar], deg = 0;
+    {
+        const int h0 = vh[v];
+        int h = h0;
+        do {
+            if (deg >= kMaxStar || ht[h] < 0) return MCL_ERR_INPUT;
+            star[deg++] = ht[h];
+            h = nx[h ^ 1];
+        } while (h != h0);
+    }
+    int on = -1, tt = -5;
+    for (unsigned seed = 1; tt == -5 && seed <= 16; seed++) tt = locate_point<true>(m, pt, ht[vh[v]], &on, seed * 7919u + v);
+    if (tt == -2) return MCL_ERR_DUPLICATE;  // the target is another vertex's position
+    if (tt < 0) return MCL_ERR_INPUT;        // outside the triangulated domain
+    const int tt2 = on >= 0 ? ht[on ^ 1] : -1;
+    const int rc = reloc_core<true>(m, v, star, deg, tt, tt2, pt);
+    // back to the caller's AoS arrays
+    for (int i = 0; i < nv; i++) {
+        v2[2 * i + 1] = vh[i];
+        pos[2 * v2[2 * i]] = P[i].x;
+        pos[2 * v2[2 * i] + 1] = P[i].y;
+    }
+    for (int h = 0; h < nhe; h++) {
+        he4[4 * h] = nx[h]; he4[4 * h + 1] = hv[h]; he4[4 * h + 2] = ht[h]; he4[4 * h + 3] = ho[h];
+    }
+    for (int t = 0; t < nt; t++) t1[t] = th[t];
+    if (rc != 0) return MCL_ERR_UNRESOLVED;
+    if (mode == 1) {
+        // Lawson flips from the half-edges of every triangle the relocation rewrote
+        std::vector<int> seeds;
+        auto add = [&](int T) {
+            if (T < 0) return;
+            int h = th[T];
+            for (int k = 0; k < 3; k++, h = nx[h]) seeds.push_back(h);
+        };
+        for (int k = 0; k < deg; k++) add(star[k]);
+        add(tt);
+        add(tt2);
+        return mcl_hmesh_delonize(pos, he4, nhe, t1, nt, v2, nv, periodic, seeds.data(), (int)seeds.size(), flips);
+    }
+    return MCL_OK;
+}
+
+// targets of an explicit move of listed vertices: every vertex's own
+// position, then pos + d (wrapped on the torus) for the listed ones, whose
+// retry counts are cleared
+__global__ void __launch_bounds__(kThreads) k_targets_here(DMesh m, float2* __restrict__ tgt) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) tgt[v] = m.pos[v];
+}
+__global__ void __launch_bounds__(kThreads) k_targets_listed(DMesh m, const int* __restrict__ ids,
+                                                             const float2* __restrict__ d, int n, int apply,
+                                                             float2* __restrict__ tgt, int* __restrict__ r_wait) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int v = ids[i];
+        const float2 q = m.pos[v];
+        float2 x = make_float2(q.x + d[i].x, q.y + d[i].y);
+        if (m.periodic) x = make_float2(wrap01(x.x), wrap01(x.y));
+        tgt[v] = x;
+        r_wait[v] = 0;
+        if (apply) m.pos[v] = x;
+    }
+}
+
+static int read_counts(mcl_tri* t, hipStream_t s);
+static int relocate_rounds(mcl_tri* t, hipStream_t s, bool f32, int n, int* slot, int* rounds);
+template <int kKind>
+static int run_rounds(mcl_tri* t, bool all, int variant, int max_rounds, hipStream_t s, long long* rounds_out);
+static void stats_begin(mcl_tri* t, hipStream_t s);
+static int stats_end(mcl_tri* t, hipStream_t s, mcl_stats_t* st, std::chrono::steady_clock::time_point t0);
+
+extern "C" int mcl_tri_move_vertices(mcl_tri_t* t, const int* ids, const float* disp, int n, int mode,
+                                     mcl_stats_t* st, void* stream) {
+    if (!t || n < 0 || (n > 0 && (!ids || !disp)) || mode < 0 || mode > 3 || n > t->m.nv) return MCL_ERR_INPUT;
+    {
+        std::vector<int> chk(ids, ids + n);
+        std::sort(chk.begin(), chk.end());
+        for (int i = 0; i < n; i++)
+            if (chk[i] < t->m.ncorner || chk[i] >= t->m.nv || (i && chk[i] == chk[i - 1])) return MCL_ERR_INPUT;
+        for (int i = 0; i < 2 * n; i++)
+            if (!std::isfinite(disp[i])) return MCL_ERR_INPUT;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    MCL_TRY(ensure_mesh(t, s));
+    t->stars_valid = false;
+    auto t0 = std::chrono::steady_clock::now();
+    stats_begin(t, s);
+    if (n == 0) return stats_end(t, s, st, t0);
+    DMesh& m = t->m;
+    // the ids into the relocation list, the displacements into acc (free between steps)
+    MCL_HIP(hipMemcpyAsync(t->pend[0], ids, sizeof(int) * n, hipMemcpyHostToDevice, s));
+    MCL_HIP(hipMemcpyAsync(t->acc, disp, sizeof(float2) * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_targets_here, dim3(blocks_for(m.nv)), dim3(kThreads), 0, s, m, t->tgt);
+    hipLaunchKernelGGL(k_targets_listed, dim3(blocks_for(n)), dim3(kThreads), 0, s, m, (const int*)t->pend[0],
+                       (const float2*)t->acc, n, mode >= 2 ? 1 : 0, t->tgt, t->r_wait);
+    MCL_HIP(hipGetLastError());
+    int rc = MCL_OK;
+    long long rounds = 0;
+    if (mode < 2) {
+        int slot = 0, rr = 0;
+        rc = relocate_rounds(t, s, true, n, &slot, &rr);
+        if (rc == MCL_OK && mode == 1) rc = run_rounds<0>(t, true, 0, 0, s, &rounds);
+    } else {
+        rc = run_rounds<1>(t, true, 1, 0, s, &rounds);
+        if (rc == MCL_ERR_ROUND_CAP) rc = MCL_OK;
+        long long inv = 0;
+        if (rc == MCL_OK) rc = mcl_tri_count_inverted(t, &inv, stream);
+        if (rc == MCL_OK && inv) rc = MCL_ERR_UNRESOLVED;
+        if (rc == MCL_OK && mode == 3) rc = run_rounds<0>(t, true, 0, 0, s, &rounds);
+    }
+    mcl_stats_t local{};
+    mcl_stats_t* sp = st ? st : &local;
+    sp->rounds = rounds;
+    const int rc2 = stats_end(t, s, sp, t0);
+    if (rc == MCL_OK && (sp->stalled > 0 || sp->duplicates > 0)) rc = MCL_ERR_UNRESOLVED;
+    return rc != MCL_OK ? rc : rc2;
 }
 
 // ------------------------------------------------------------ round driver

@@ -94,6 +94,58 @@ class HostTriangulation:
     def num_points(self):
         return len(self._points)
 
+    def _arrays(self):
+        for a in ("m_pos", "m_v", "m_he", "m_t"):
+            setattr(self, a, np.ascontiguousarray(getattr(self, a)))
+        return (_np_ptr(self.m_pos), _np_ptr(self.m_he), len(self.m_he), _np_ptr(self.m_t), len(self.m_t),
+                _np_ptr(self.m_v), len(self.m_v), int(self.periodic))
+
+    def delonize(self):
+        """Host_Triangulation.h:305-319: Lawson flips (exact incircle) until
+        every edge is locally Delaunay; False when a reflex quad (a tangled
+        mesh) was left unflipped"""
+        f = ctypes.c_longlong()
+        rc = lib().mcl_hmesh_delonize(*self._arrays(), None, 0, ctypes.byref(f))
+        self.last_flips = f.value
+        return rc == 0
+
+    def untangle(self):
+        """Host_Triangulation.h:417-423: one pass of the B/C steps; True when
+        something was flipped (the reference's return value)"""
+        f = ctypes.c_longlong()
+        check(lib().mcl_hmesh_untangle(*self._arrays(), ctypes.byref(f)), "mcl_hmesh_untangle")
+        self.last_flips = f.value
+        return f.value > 0
+
+    def _move_vertex(self, p_index, d, mode):
+        d = np.asarray(d, np.float32).reshape(2)
+        f = ctypes.c_longlong()
+        for a in ("m_pos", "m_v", "m_he", "m_t"):
+            setattr(self, a, np.ascontiguousarray(getattr(self, a)))
+        rc = lib().mcl_hmesh_move_vertex(_np_ptr(self.m_pos), _np_ptr(self.m_he), len(self.m_he), _np_ptr(self.m_t),
+                                         len(self.m_t), _np_ptr(self.m_v), len(self.m_v), int(self.periodic),
+                                         int(p_index), _np_ptr(d), int(mode), ctypes.byref(f))
+        self.last_flips = f.value
+        return rc == 0
+
+    def moveFlipflop(self, p_index, d):
+        """Host_Triangulation.h:77: removes vertex p_index and re-inserts it at
+        its position + d (the mesh stays valid, not necessarily Delaunay)"""
+        return self._move_vertex(p_index, d, _lib.MOVE_FLIPFLOP)
+
+    def moveFlipflopDelaunay(self, p_index, d):
+        """Host_Triangulation.h:78: moveFlipflop, then Lawson flips"""
+        return self._move_vertex(p_index, d, _lib.MOVE_FLIPFLOP_DELAUNAY)
+
+    def moveUntangling(self, p_index, d):
+        """Host_Triangulation.h:79: moves, then untangles (False when
+        inverted triangles remain)"""
+        return self._move_vertex(p_index, d, _lib.MOVE_UNTANGLING)
+
+    def moveUntanglingDelaunay(self, p_index, d):
+        """Host_Triangulation.h:80: moveUntangling, then Lawson flips"""
+        return self._move_vertex(p_index, d, _lib.MOVE_UNTANGLING_DELAUNAY)
+
     def save_state(self, path):
         """writes m_pos, m_v, m_he, m_t to a state file (mcl_state_save)"""
         check(lib().mcl_state_save(str(path).encode(), int(self.periodic),
@@ -390,6 +442,36 @@ class DeviceTriangulation:
         return rc == 0
 
     untangle2 = untangle  # Device_Triangulation.h:461-495
+
+    def move_vertices(self, ids, d, mode):
+        """mcl_tri_move_vertices: ids (host ints) move by d (host (n, 2)
+        float32) with mode MOVE_*; returns the status (0 when every listed
+        vertex is at its target and, in the untangling modes, nothing stays
+        inverted)"""
+        ids = np.ascontiguousarray(np.asarray(ids, np.int32).reshape(-1))
+        d = np.ascontiguousarray(np.asarray(d, np.float32).reshape(-1, 2))
+        st = _lib.Stats()
+        rc = lib().mcl_tri_move_vertices(self._h, _np_ptr(ids), _np_ptr(d), len(ids), int(mode), ctypes.byref(st),
+                                         None)
+        self.last_stats = st.as_dict()
+        return rc
+
+    def moveFlipflop(self, p_index, d):
+        """Device_Triangulation.h:183 (a stub in the reference): remove and
+        re-insert one vertex at its position + d"""
+        return self.move_vertices([p_index], [d], _lib.MOVE_FLIPFLOP) == 0
+
+    def moveFlipflopDelaunay(self, p_index, d):
+        """Device_Triangulation.h:184: moveFlipflop, then Lawson rounds"""
+        return self.move_vertices([p_index], [d], _lib.MOVE_FLIPFLOP_DELAUNAY) == 0
+
+    def moveUntangling(self, p_index, d):
+        """Device_Triangulation.h:185: move, then untangle rounds"""
+        return self.move_vertices([p_index], [d], _lib.MOVE_UNTANGLING) == 0
+
+    def moveUntanglingDelaunay(self, p_index, d):
+        """Device_Triangulation.h:186: moveUntangling, then Lawson rounds"""
+        return self.move_vertices([p_index], [d], _lib.MOVE_UNTANGLING_DELAUNAY) == 0
 
     def movePoints(self, d):
         """Device_Triangulation.h:395-402: d is a device (V,2) float32 tensor"""

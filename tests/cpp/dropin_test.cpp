@@ -6,7 +6,9 @@
 // Prints "DROPIN OK" and exits 0 when every check holds.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <random>
 #include <vector>
 
@@ -41,6 +43,21 @@ struct SaveNeighborsFunctor {
             const int k = atomicAdd(&num_neighbors[i], 1);
             if (k < max_neighbors) neighbors[i * max_neighbors + k] = j;
         }
+    }
+};
+
+// every functor call on a periodic grid: its largest |d| and dist (float bits
+// of non-negative values order like ints), and the pair count per slot
+struct TorusProbeFunctor {
+    float rad;
+    int* max_bits;  // [0] max dist, [1] max |d.x|, [2] max |d.y|
+    int* count;
+    __device__ void operator()(const int& i, const int& j, const vec2 d, const double dist) {
+        if (i == j) return;
+        atomicMax(&max_bits[0], __float_as_int((float)dist));
+        atomicMax(&max_bits[1], __float_as_int(fabsf(d.x)));
+        atomicMax(&max_bits[2], __float_as_int(fabsf(d.y)));
+        if (dist <= rad) atomicAdd(&count[i], 1);
     }
 };
 
@@ -215,6 +232,62 @@ int main() {
         CHECK(eq, "resumed state equals the saved one");
         std::remove(path);
         (void)hipFree(d_pts);
+    }
+    // periodic grid: pairs across x = 0 and y = 0 reach the functor with the
+    // minimum-image d and dist <= rad (gridCount2d.h:284-290 on the torus)
+    {
+        const int n = 4000;
+        const float r = 0.03f;
+        std::mt19937 g(21);
+        std::uniform_real_distribution<float> u(0.f, 1.f);
+        std::vector<vec2> p(n);
+        for (int i = 0; i < n; i++) {
+            p[i] = vec2(u(g), u(g));
+            if (i % 4 == 0) p[i].x = (i % 8 == 0) ? 0.01f * u(g) : 1.f - 0.01f * u(g);  // near the seam x = 0
+            if (i % 5 == 0) p[i].y = (i % 10 == 0) ? 0.01f * u(g) : 1.f - 0.01f * u(g);  // near y = 0
+            if (p[i].x >= 1.f) p[i].x = 0.f;
+            if (p[i].y >= 1.f) p[i].y = 0.f;
+        }
+        const int nc = (int)(1.0f / r);
+        GridCount2d gt(n, vec2(0.f, 0.f), vec2(1.0f / nc, 1.0f / nc), ivec2(nc, nc), MCL_GRID_MORTON | MCL_GRID_PERIODIC);
+        vec2* dp = nullptr;
+        TorusProbeFunctor tp{r, nullptr, nullptr};
+        (void)hipMalloc(&dp, sizeof(vec2) * n);
+        (void)hipMalloc(&tp.max_bits, sizeof(int) * 3);
+        (void)hipMalloc(&tp.count, sizeof(int) * n);
+        (void)hipMemcpy(dp, p.data(), sizeof(vec2) * n, hipMemcpyHostToDevice);
+        (void)hipMemset(tp.max_bits, 0, sizeof(int) * 3);
+        (void)hipMemset(tp.count, 0, sizeof(int) * n);
+        gt.update(dp);
+        gt.apply_f_frnn<TorusProbeFunctor>(tp, dp, r);
+        int mb[3];
+        std::vector<int> cnt(n);
+        std::vector<vec2> sp(n);
+        (void)hipMemcpy(mb, tp.max_bits, sizeof(mb), hipMemcpyDeviceToHost);
+        (void)hipMemcpy(cnt.data(), tp.count, sizeof(int) * n, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(sp.data(), dp, sizeof(vec2) * n, hipMemcpyDeviceToHost);
+        float mx[3];
+        std::memcpy(mx, mb, sizeof(mx));
+        CHECK(mx[0] <= r && mx[1] <= r && mx[2] <= r, "periodic functor: max dist %g, |dx| %g, |dy| %g > rad %g",
+              mx[0], mx[1], mx[2], r);
+        auto wrap = [](float d) { return d > 0.5f ? d - 1.f : (d < -0.5f ? d + 1.f : d); };
+        int seam_pairs = 0;
+        for (int i = 0; i < n; i++) {
+            int brute = 0;
+            for (int j = 0; j < n; j++) {
+                if (j == i) continue;
+                const float dx = wrap(sp[j].x - sp[i].x), dy = wrap(sp[j].y - sp[i].y);
+                if (dx * dx + dy * dy <= r * r) {
+                    brute++;
+                    seam_pairs += std::fabs(sp[j].x - sp[i].x) > 0.5f || std::fabs(sp[j].y - sp[i].y) > 0.5f;
+                }
+            }
+            CHECK(cnt[i] == brute, "periodic grid FRNN count of slot %d: %d vs %d", i, cnt[i], brute);
+        }
+        CHECK(seam_pairs > 100, "pairs across the seams: %d", seam_pairs);
+        (void)hipFree(dp);
+        (void)hipFree(tp.max_bits);
+        (void)hipFree(tp.count);
     }
     (void)hipFree(d_move);
     (void)hipFree(d_pos);

@@ -35,6 +35,15 @@ int main(int argc, char** argv) {
     for (int i = 4; i < (int)h.m_pos.size(); i += 3) h.movePoint(i, vec2(g(rng), g(rng)));
     if (!h.untangle()) return 8;
     if (!h.delonize()) return 9;
+    // remove-and-reinsert moves (moveFlipflop*, stubs in the reference), ending Delaunay
+    for (int i = 10; i < (int)h.m_pos.size(); i += 37) {
+        const vec2 p = h.m_pos[i], d(0.03f * g(rng) / 2e-4f, 0.03f * g(rng) / 2e-4f);
+        if (p.x + d.x <= 0.f || p.x + d.x >= 1.f || p.y + d.y <= 0.f || p.y + d.y >= 1.f) continue;
+        if (!h.moveFlipflop(i, d)) return 11;
+        if (h.m_pos[i].x != p.x + d.x || h.m_pos[i].y != p.y + d.y) return 12;
+    }
+    if (!h.delonize()) return 13;
+    if (!h.moveFlipflopDelaunay(9, vec2(0.001f, -0.002f))) return 14;
     // queries
     if (h.oneRing(20).size() < 5) return 10;
     std::printf("%zu %zu %zu\n", h.m_pos.size(), h.m_he.size(), h.m_t.size());

@@ -1,7 +1,8 @@
 """The C++ drop-in HostTriangulation (include/mcleap/Host_Triangulation.h)
 compiled with g++ against libmcleap_hip.so and run on the CPU: two batches
 of insertions + one single insertion (vertex ids stay stable), point
-location, movePoint + untangle + delonize on the host.  The final mesh must
+location, movePoint + untangle + delonize, moveFlipflop / moveFlipflopDelaunay on
+the host.  The final mesh must
 be valid, exactly Delaunay and equal to the oracle's triangulation of the
 final positions."""
 import os

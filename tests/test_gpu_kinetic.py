@@ -272,3 +272,59 @@ def test_save_and_resume(oracle, tmp_path, periodic):
     assert np.array_equal(p1, p2)
     assert np.array_equal(O.canonical_triangles(h1, t1), O.canonical_triangles(h2, t2))
     assert O.count_non_delaunay(p2, h2, periodic=periodic) == 0
+
+
+def test_move_vertices_flipflop(oracle):
+    """moveFlipflop / moveFlipflopDelaunay / moveUntanglingDelaunay
+    (Device_Triangulation.h:183-186, stubs in the reference) through
+    mcl_tri_move_vertices: 500 vertices removed and re-inserted two spacings
+    away in parallel rounds keep every triangle strictly CCW and land exactly
+    at pos + d; with Lawson rounds the mesh equals the oracle's exact DT"""
+    O = oracle
+    n = 20000
+    H = M.HostTriangulation()
+    H.addDelaunayPoints(uniform_points(n, 12))
+    dt = M.DeviceTriangulation(H)
+    rng = np.random.default_rng(6)
+    sp = 1.0 / np.sqrt(n)
+    for mode in (M.MOVE_FLIPFLOP, M.MOVE_FLIPFLOP_DELAUNAY):
+        pos0 = dt.download_positions()
+        ids = rng.choice(np.arange(4, n + 4), 500, replace=False).astype(np.int32)
+        d = (2.0 * sp * rng.standard_normal((500, 2))).astype(np.float32)
+        want = pos0[ids] + d
+        d[np.any((want <= 0) | (want >= 1), axis=1)] = 0
+        rc = dt.move_vertices(ids, d, mode)
+        print(mode, rc, dt.last_stats)
+        assert rc == 0 and dt.last_stats["relocated"] > 400
+        pos, v, he, t = dt.download()
+        assert np.array_equal(pos[ids], pos0[ids] + d)
+        others = np.setdiff1d(np.arange(len(pos)), ids)
+        assert np.array_equal(pos[others], pos0[others])
+        rc_, msg = O.check_mesh(pos, v, he, t, periodic=False, require_ccw=True)
+        assert rc_ == 0, msg
+        if mode == M.MOVE_FLIPFLOP:
+            dt.delonize2()
+        _check_box(O, dt, "mode %d" % mode)
+    v = 77
+    p0 = dt.download_positions()[v]
+    assert dt.moveFlipflop(v, np.float32([1.5 * sp, -1.0 * sp]))
+    assert dt.moveUntanglingDelaunay(v + 1, np.float32([1e-5, 1e-5]))
+    assert np.array_equal(dt.download_positions()[v], p0 + np.float32([1.5 * sp, -1.0 * sp]))
+    dt.delonize2()
+    _check_box(O, dt, "single moves")
+
+
+def test_move_flipflop_degenerate_link_device(oracle):
+    """the centre of a square of four neighbours (on the diagonal of every ear
+    of its link) is removed and re-inserted outside its old star on the GPU"""
+    O = oracle
+    ring = [(0.5 + 0.3 * np.cos(a), 0.5 + 0.3 * np.sin(a)) for a in np.linspace(0, 2 * np.pi, 24, endpoint=False)]
+    pts = np.array([(0.5, 0.5), (0.4, 0.5), (0.6, 0.5), (0.5, 0.4), (0.5, 0.6)] + ring, np.float32)
+    H = M.HostTriangulation()
+    H.addDelaunayPoints(pts.copy())
+    dt = M.DeviceTriangulation(H)
+    v = {tuple(p): i for i, p in enumerate(H.m_pos.tolist())}[(0.5, 0.5)]
+    assert np.sum(H.m_he[:, 1] == v) == 4
+    assert dt.moveFlipflopDelaunay(v, np.float32([0.17, 0.06]))
+    assert np.array_equal(dt.download_positions()[v], np.float32([0.5, 0.5]) + np.float32([0.17, 0.06]))
+    _check_box(O, dt, "degenerate link")

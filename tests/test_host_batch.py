@@ -85,3 +85,55 @@ def test_state_file_round_trip(tmp_path, oracle):
     T.addDelaunayPoints(xy)
     T.save_state(tmp_path / "torus.mcl")
     assert M.HostTriangulation.load_state(tmp_path / "torus.mcl").periodic
+
+
+def test_state_file_with_bad_indices_rejected_before_the_device(tmp_path):
+    """a state file whose checksum is valid but whose arrays break the mesh
+    structure (indices out of range, next^3 != id, v.he / t.he pointing
+    elsewhere) is refused by mcl_tri_load and mcl_tri_create with
+    MCL_ERR_INPUT on the host, before anything reaches the device"""
+    import ctypes
+    xy = np.random.default_rng(5).random((300, 2)).astype(np.float32)
+    H = M.HostTriangulation()
+    H.addDelaunayPoints(xy)
+    nhe, nv, nt = len(H.m_he), len(H.m_v), len(H.m_t)
+    inner = int(np.flatnonzero(H.m_he[:, 2] >= 0)[7])
+
+    def variant(fn):
+        pos, v, he, t = H.m_pos.copy(), H.m_v.copy(), H.m_he.copy(), H.m_t.copy()
+        fn(pos, v, he, t)
+        return pos, v, he, t
+
+    cases = {
+        "next out of range": lambda p, v, he, t: he.__setitem__((3, 0), nhe + 5),
+        "origin out of range": lambda p, v, he, t: he.__setitem__((3, 1), -2),
+        "triangle out of range": lambda p, v, he, t: he.__setitem__((inner, 2), nt),
+        "opposite out of range": lambda p, v, he, t: he.__setitem__((inner, 3), nv + 1),
+        "next^3": lambda p, v, he, t: he.__setitem__((inner, 0), inner),
+        "v.he": lambda p, v, he, t: v.__setitem__((9, 1), int(v[10, 1])),
+        "t.he": lambda p, v, he, t: t.__setitem__(4, int(t[5])),
+        "nan position": lambda p, v, he, t: p.__setitem__((6, 0), np.nan),
+    }
+    L = M.lib()
+    for name, fn in cases.items():
+        pos, v, he, t = variant(fn)
+        path = tmp_path / "bad.mcl"
+        rc = L.mcl_state_save(str(path).encode(), 0, pos.ctypes.data_as(ctypes.c_void_p), len(pos),
+                              v.ctypes.data_as(ctypes.c_void_p), nv,
+                              he.ctypes.data_as(ctypes.c_void_p), nhe, t.ctypes.data_as(ctypes.c_void_p), nt)
+        assert rc == 0, name
+        h = ctypes.c_void_p()
+        assert L.mcl_tri_load(str(path).encode(), ctypes.byref(h)) == 2, name
+        rc = L.mcl_tri_create(pos.ctypes.data_as(ctypes.c_void_p), len(pos), v.ctypes.data_as(ctypes.c_void_p), nv,
+                              he.ctypes.data_as(ctypes.c_void_p), nhe, t.ctypes.data_as(ctypes.c_void_p), nt, 0,
+                              ctypes.byref(h))
+        assert rc == 2, name
+    # the unmodified arrays pass the check (here: no device, so the
+    # allocation that follows reports that instead)
+    h = ctypes.c_void_p()
+    rc = L.mcl_tri_create(H.m_pos.ctypes.data_as(ctypes.c_void_p), nv, H.m_v.ctypes.data_as(ctypes.c_void_p), nv,
+                          H.m_he.ctypes.data_as(ctypes.c_void_p), nhe, H.m_t.ctypes.data_as(ctypes.c_void_p), nt, 0,
+                          ctypes.byref(h))
+    assert rc != 2
+    if rc == 0:
+        L.mcl_tri_destroy(h)

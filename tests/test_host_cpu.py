@@ -114,3 +114,83 @@ def test_cocircular_lattice_is_unique_under_the_tie_rule(oracle):
     remap = np.concatenate([np.arange(4), 4 + perm])  # oracle vertex 4+k holds point perm[k]
     c3 = O.canonical_from_list(remap[c3])
     assert np.array_equal(O.canonical_triangles(H.m_he, H.m_t), c3)
+
+
+def _dt_equal(O, H, tag):
+    """H (box) is a valid mesh equal to the oracle's exact DT of its positions"""
+    rc, msg = O.check_mesh(H.m_pos, H.m_v, H.m_he, H.m_t, periodic=False, require_ccw=True)
+    assert rc == 0, "%s: %s" % (tag, msg)
+    assert O.count_non_delaunay(H.m_pos, H.m_he) == 0, tag
+    ref = O.delaunay(H.m_pos[4:])
+    _, _, h2, t2 = ref.export()
+    assert np.array_equal(O.canonical_triangles(H.m_he, H.m_t), O.canonical_triangles(h2, t2)), tag
+
+
+def test_move_flipflop_host(oracle):
+    """moveFlipflop (Host_Triangulation.h:77, a stub in the reference): a
+    vertex removed and re-inserted two spacings away keeps every triangle
+    strictly CCW and lands exactly at pos + d; moveFlipflopDelaunay then
+    equals the oracle's exact Delaunay triangulation of the positions"""
+    O = oracle
+    n = 2000
+    H = M.HostTriangulation()
+    H.addDelaunayPoints(uniform_points(n, 8))
+    rng = np.random.default_rng(4)
+    sp = 1.0 / np.sqrt(n)
+    moved = 0
+    for k in range(60):
+        v = int(rng.integers(4, n + 4))
+        d = (2.0 * sp * rng.standard_normal(2)).astype(np.float32)
+        want = H.m_pos[v] + d
+        if np.any(want <= 0) or np.any(want >= 1):
+            continue
+        delaunay = k >= 30  # first moveFlipflop alone, then (from a Delaunay mesh) with Lawson flips
+        if k == 30:
+            assert H.delonize()
+            _dt_equal(O, H, "moveFlipflop x30 + delonize")
+        ok = (H.moveFlipflopDelaunay if delaunay else H.moveFlipflop)(v, d)
+        assert ok, k
+        moved += 1
+        assert np.array_equal(H.m_pos[v], want.astype(np.float32)), k
+        rc, msg = O.check_mesh(H.m_pos, H.m_v, H.m_he, H.m_t, periodic=False, require_ccw=True)
+        assert rc == 0, "step %d: %s" % (k, msg)
+        if delaunay:
+            _dt_equal(O, H, "moveFlipflopDelaunay %d" % k)
+    assert moved > 40
+    assert H.moveUntanglingDelaunay(5, np.float32([1e-4, -1e-4]))
+    _dt_equal(O, H, "moveUntanglingDelaunay")
+
+
+def test_move_flipflop_degenerate_link(oracle):
+    """a vertex at the centre of a square of four neighbours (v on the
+    diagonal of every ear of its link) is removed by the proper-ear rule and
+    re-inserted outside its old star; also a target on another vertex
+    (refused) and on an edge (2 -> 4 insertion)"""
+    O = oracle
+    c = np.float32(0.5)
+    ring = [(0.5 + 0.3 * np.cos(a), 0.5 + 0.3 * np.sin(a)) for a in np.linspace(0, 2 * np.pi, 24, endpoint=False)]
+    pts = np.array([(c, c), (0.4, 0.5), (0.6, 0.5), (0.5, 0.4), (0.5, 0.6)] + ring, np.float32)
+    H = M.HostTriangulation()
+    H.addDelaunayPoints(pts.copy())
+    vid = {tuple(p): i for i, p in enumerate(H.m_pos.tolist())}
+    v = vid[(0.5, 0.5)]
+    deg = np.sum(H.m_he[:, 1] == v)
+    assert deg == 4
+    assert H.moveFlipflop(v, np.float32([0.17, 0.06]))
+    assert np.array_equal(H.m_pos[v], np.float32([0.5, 0.5]) + np.float32([0.17, 0.06]))
+    rc, msg = O.check_mesh(H.m_pos, H.m_v, H.m_he, H.m_t, periodic=False, require_ccw=True)
+    assert rc == 0, msg
+    assert H.delonize()
+    _dt_equal(O, H, "after the degenerate removal")
+    # onto another vertex: refused, mesh unchanged
+    before = [a.copy() for a in (H.m_pos, H.m_v, H.m_he, H.m_t)]
+    u = vid[(np.float32(0.4), np.float32(0.5))]
+    assert not H.moveFlipflop(v, H.m_pos[u] - H.m_pos[v])
+    for a, b in zip(before, (H.m_pos, H.m_v, H.m_he, H.m_t)):
+        assert np.array_equal(a, b)
+    # onto the midpoint of the edge between two lattice neighbours (exact in FP32)
+    w = vid[(np.float32(0.5), np.float32(0.4))]
+    mid = np.float32([0.45, 0.45])
+    assert H.moveFlipflopDelaunay(w, mid - H.m_pos[w])
+    assert np.array_equal(H.m_pos[w], mid)
+    _dt_equal(O, H, "after the on-edge insertion")
