@@ -19,6 +19,7 @@ struct DeviceTriangulation {
     vec2* m_pos = nullptr;  // device positions (float2), owned by the library
     int blocksize = 256;
     mcl_stats_t last_stats{};  // statistics of the last delonize/untangle/step
+    int ring_overflow = 0;  // vertices with more than maxRingSize - 1 neighbours in the last oneRing / oneRing2
 
     explicit DeviceTriangulation(HostTriangulation* h) : h_triangulation(h) {
         mcleap::must(mcl_tri_create(reinterpret_cast<const float*>(h->m_pos.data()), (int)h->m_pos.size(),
@@ -113,11 +114,11 @@ struct DeviceTriangulation {
     // are caller-allocated device memory in the reference layouts
     template <int maxRingSize>
     bool oneRing(int* ring_neighbors) {
-        return mcleap::ok(mcl_tri_one_ring(t_, maxRingSize, 0, ring_neighbors, nullptr), "oneRing");
+        return mcleap::ok(mcl_tri_one_ring(t_, maxRingSize, 0, ring_neighbors, &ring_overflow, nullptr), "oneRing");
     }
     template <int maxRingSize>
     bool oneRing2(int* ring_neighbors) {
-        return mcleap::ok(mcl_tri_one_ring(t_, maxRingSize, 1, ring_neighbors, nullptr), "oneRing2");
+        return mcleap::ok(mcl_tri_one_ring(t_, maxRingSize, 1, ring_neighbors, &ring_overflow, nullptr), "oneRing2");
     }
     template <int maxRingSize>
     bool closestNeighbors(int* ring_neighbors, int* closest_neighbors) {

@@ -287,8 +287,13 @@ void* mcl_tri_target_ptr(mcl_tri_t* t);
 /* oneRing<K>(int*) (layout 0, SoA: ring[(c+1)*nv + i], count in ring[i]) and
  * oneRing2<K>(int*) (layout 1, AoS: ring[(c+1) + K*i], count in ring[K*i]),
  * kernels.cuh:54-121: CW rotation from v.he, at most K-1 neighbours kept.
- * d_ring is caller-allocated device memory of nv*K ints. */
-int mcl_tri_one_ring(mcl_tri_t* t, int K, int layout, void* d_ring, void* stream);
+ * d_ring is caller-allocated device memory of nv*K ints.  Where the reference
+ * truncates silently (kernels.cuh:68, "TODO: do something about this case"),
+ * *overflow (may be NULL) receives the number of vertices other than box
+ * corners with more than K-1 neighbours, and the call returns
+ * MCL_ERR_CAPACITY when it is not 0 (the truncated rings are still written;
+ * a corner's fan of hull points is truncated without being counted). */
+int mcl_tri_one_ring(mcl_tri_t* t, int K, int layout, void* d_ring, int* overflow, void* stream);
 /* one-ring as CSR (no cap): d_offsets nv+1 ints, d_nbrs cap ints; same CW
  * order as above.  *total = number of entries (MCL_ERR_CAPACITY if > cap).
  * With total == NULL and cap >= nhe the call is asynchronous (the entry

@@ -10,6 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmcleap_hip.so")
 
 MCL_OK = 0
+MCL_ERR_CAPACITY = 5
 ERRORS = {
     1: "HIP runtime error",
     2: "invalid input",
@@ -112,7 +113,7 @@ SIGNATURES = {
     "mcl_tri_count_inverted": [_p, _p, _p],
     "mcl_tri_count_non_delaunay": [_p, _p, _p],
     "mcl_tri_step": [_p, _p, _p, _p],
-    "mcl_tri_one_ring": [_p, _i, _i, _p, _p],
+    "mcl_tri_one_ring": [_p, _i, _i, _p, ctypes.POINTER(_i), _p],
     "mcl_tri_one_ring_csr": [_p, _p, _p, _i, _p, _p],
     "mcl_tri_closest": [_p, _p, _p, _p, _p],
     "mcl_grid_create": [_i, _f, _f, _f, _f, _i, _i, _i, _pp],

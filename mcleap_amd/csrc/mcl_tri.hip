@@ -1450,11 +1450,18 @@ __global__ void __launch_bounds__(kThreads) k_ring_csr(DMesh m, const int* __res
     }
 }
 // oneRing<K> (layout 0) / oneRing2<K> (layout 1): kernels.cuh:54-121
-__global__ void __launch_bounds__(kThreads) k_ring_legacy(DMesh m, int K, int layout, int* __restrict__ ring) {
+// (a vertex with more than K - 1 neighbours keeps the first K - 1, as the
+// reference does, and is counted in *overflow unless it is a box corner,
+// whose fan of hull points is not a point's neighbourhood)
+__global__ void __launch_bounds__(kThreads) k_ring_legacy(DMesh m, int K, int layout, int* __restrict__ ring,
+                                                          int* __restrict__ overflow) {
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
         int h0 = m.v_he[v], h = h0, c = 0;
         do {
-            if (c + 2 > K) break;
+            if (c + 2 > K) {
+                if (v >= m.ncorner) atomicAdd(overflow, 1);
+                break;
+            }
             int w = m.he_v[h ^ 1];
             if (layout == 0) ring[(size_t)(c + 1) * m.nv + v] = w;
             else ring[(size_t)(c + 1) + (size_t)K * v] = w;
@@ -3236,14 +3243,19 @@ extern "C" int mcl_tri_move_legalize(mcl_tri_t* t, const void* d_disp, int varia
 extern "C" void* mcl_tri_target_ptr(mcl_tri_t* t) { return t ? (void*)t->tgt : nullptr; }
 
 // ---------------------------------------------------------------- one ring
-extern "C" int mcl_tri_one_ring(mcl_tri_t* t, int K, int layout, void* d_ring, void* stream) {
+extern "C" int mcl_tri_one_ring(mcl_tri_t* t, int K, int layout, void* d_ring, int* overflow, void* stream) {
     if (!t || !d_ring || K < 2 || (layout != 0 && layout != 1)) return MCL_ERR_INPUT;
     hipStream_t s = (hipStream_t)stream;
     MCL_TRY(ensure_mesh(t, s));
-    hipLaunchKernelGGL(k_ring_legacy, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, K, layout, (int*)d_ring);
+    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->cnt + kCnMisc, 1, 0);
+    hipLaunchKernelGGL(k_ring_legacy, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, K, layout, (int*)d_ring,
+                       t->cnt + kCnMisc);
     MCL_HIP(hipGetLastError());
+    int ov = 0;
+    MCL_HIP(hipMemcpyAsync(&ov, t->cnt + kCnMisc, sizeof(int), hipMemcpyDeviceToHost, s));
     MCL_HIP(hipStreamSynchronize(s));
-    return MCL_OK;
+    if (overflow) *overflow = ov;
+    return ov ? MCL_ERR_CAPACITY : MCL_OK;
 }
 
 extern "C" int mcl_tri_one_ring_csr(mcl_tri_t* t, void* d_offsets, void* d_nbrs, int cap, int* total, void* stream) {

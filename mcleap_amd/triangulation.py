@@ -513,15 +513,24 @@ class DeviceTriangulation:
         return out.value
 
     # ------------------------------------------------------------- queries
+    def _one_ring(self, K, layout, ring):
+        ov = ctypes.c_int()
+        rc = lib().mcl_tri_one_ring(self._h, int(K), layout, _dev_ptr(ring), ctypes.byref(ov), None)
+        self.ring_overflow = ov.value
+        if rc != _lib.MCL_ERR_CAPACITY:
+            check(rc, "mcl_tri_one_ring")
+        return rc == 0
+
     def oneRing(self, K, ring):
-        """oneRing<K>(int*): SoA ring[(c+1)*V + i], count in ring[i] (kernels.cuh:54-84)"""
-        check(lib().mcl_tri_one_ring(self._h, int(K), 0, _dev_ptr(ring), None), "mcl_tri_one_ring")
-        return True
+        """oneRing<K>(int*): SoA ring[(c+1)*V + i], count in ring[i]
+        (kernels.cuh:54-84).  False (ring_overflow = the number of vertices)
+        when a vertex has more than K - 1 neighbours; those keep the first K - 1"""
+        return self._one_ring(K, 0, ring)
 
     def oneRing2(self, K, ring):
-        """oneRing2<K>(int*): AoS ring[(c+1) + K*i], count in ring[K*i] (kernels.cuh:91-121)"""
-        check(lib().mcl_tri_one_ring(self._h, int(K), 1, _dev_ptr(ring), None), "mcl_tri_one_ring")
-        return True
+        """oneRing2<K>(int*): AoS ring[(c+1) + K*i], count in ring[K*i]
+        (kernels.cuh:91-121); overflow as oneRing"""
+        return self._one_ring(K, 1, ring)
 
     def one_ring_csr(self, offsets, nbrs=None):
         """CSR one-ring (no cap); returns the entry count"""

@@ -226,3 +226,116 @@ def test_batch_grid_frnn(oracle):
     g.frnn_csr_async(off2, nbr2)
     torch.cuda.synchronize()
     assert np.array_equal(off2.cpu().numpy(), off) and np.array_equal(nbr2.cpu().numpy()[:tot], nbr)
+
+
+def _pairs_gpu(g, pos_dev, rad):
+    """all (i, j) pairs of the CSR query as original ids, sorted, int64 keys"""
+    import torch
+    n = g.numP
+    off = torch.zeros(n + 1, dtype=torch.int32, device="cuda")
+    tot = g.frnn_csr(pos_dev, rad, off)
+    nbr = torch.zeros(max(tot, 1), dtype=torch.int32, device="cuda")
+    assert g.frnn_csr(pos_dev, rad, off, nbr) == tot
+    off, nbr = off.cpu().numpy(), nbr.cpu().numpy()[:tot].astype(np.int64)
+    perm = g.perm().cpu().numpy().astype(np.int64)
+    src = np.repeat(perm, np.diff(off))
+    return np.sort(src * n + perm[nbr])
+
+
+def _pairs_oracle(counts, nbr):
+    n = len(counts)
+    src = np.repeat(np.arange(n, dtype=np.int64), counts)
+    mask = np.arange(nbr.shape[1])[None, :] < counts[:, None]
+    return np.sort(src * n + nbr[mask].astype(np.int64))
+
+
+def test_c1_frnn_after_every_step(oracle):
+    """C1 (10,000 points, box, sigma = 1e-3 clamped, r = 0.02, the reference's
+    row-major cell keys): after each of the 50 kinetic steps the grid FRNN
+    pairs of the moved points equal the oracle's grid search; every 10 steps
+    they are also checked against brute force"""
+    import torch
+    from tests.conftest import uniform_points
+    O = oracle
+    H = M.HostTriangulation()
+    H.addDelaunayPoints(uniform_points(10000, 1))
+    dt = M.DeviceTriangulation(H)
+    n = dt.m_pos_size - 4
+    g = M.GridCount2d(n, (0.0, 0.0), (0.02, 0.02), (50, 50), M.GRID_ROWMAJOR)
+    for k in range(50):
+        dt.step(M.MOTION_GAUSS, 1e-3, seed=1, step=k)
+        pts = dt.download_positions()[4:].copy()
+        pos = torch.from_numpy(pts.copy()).cuda()  # update() sorts a copy (moving_device_mesh.cu:125-130)
+        g.update(pos)
+        got = _pairs_gpu(g, pos, 0.02)
+        counts, nbr = O.grid_frnn(pts, (0, 0), (0.02, 0.02), (50, 50), 0.02, cap=128)
+        assert counts.max() < 128
+        assert np.array_equal(got, _pairs_oracle(counts, nbr)), "step %d" % k
+        if k % 10 == 9:
+            # brute force: the reference's cell formula (gridCount2d.h:94-102,
+            # FP32 divide, truncate, % nx) misses two kinds of pairs, both
+            # reproduced here: a point clamped to 1 - 2^-24 divides to exactly
+            # 50.0 in FP32 and wraps into cell 0, and with cells exactly r
+            # wide a pair at distance ~r can fall two cells apart
+            bc, bn = O.brute_frnn(pts, 0.02, cap=128)
+            brute = _pairs_oracle(bc, bn)
+            assert np.all(np.isin(got, brute))
+            miss = np.setdiff1d(brute, got)
+            i, j = miss // n, miss % n
+            d = pts[j] - pts[i]
+            r2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]
+            wraps = np.any(pts / np.float32(0.02) >= np.float32(50.0), axis=1)
+            assert np.all(wraps[i] | wraps[j] | (r2 > np.float32(0.0199) ** 2)), "step %d" % k
+            assert np.all(np.isin(miss, brute))
+
+
+def test_c2_torus_frnn_1m(oracle):
+    """C2 at full size: 2^20 points on the torus, r = 1/512 on the 512 x 512
+    periodic grid (Morton keys): the ~13M pairs equal the oracle's periodic
+    grid search"""
+    import torch
+    from mcleap_amd.workloads import grid_shape, initial_points
+    O = oracle
+    pts = initial_points("c2")
+    n = len(pts)
+    r = np.float32(2.0 / np.sqrt(n))
+    nc, cs = grid_shape(float(r), "torus")
+    g = M.GridCount2d(n, (0.0, 0.0), (cs, cs), (nc, nc), M.GRID_MORTON | M.GRID_PERIODIC)
+    pos = torch.from_numpy(pts.copy()).cuda()
+    g.update(pos)
+    got = _pairs_gpu(g, pos, float(r))
+    counts, nbr = O.grid_frnn(pts, (0, 0), (cs, cs), (nc, nc), float(r), periodic=True, cap=64)
+    assert counts.max() < 64
+    want = _pairs_oracle(counts, nbr)
+    print("pairs", len(got), "mean", len(got) / n)
+    assert len(got) > 12 * n and np.array_equal(got, want)
+
+
+def test_c3_density_frnn(oracle):
+    """C3's clustered density (64 Gaussian blobs, 4.2M points, r = 3/sqrt(N)):
+    the points of a window around the densest blob centre (over 80 neighbours
+    per point on average, up to ~550), queried on the C3 grid geometry, give the same pairs as the
+    oracle's grid search and brute force"""
+    import torch
+    from mcleap_amd.workloads import CONFIGS, grid_shape, initial_points
+    O = oracle
+    allp = initial_points("c3")
+    r = np.float32(3.0 / np.sqrt(CONFIGS["c3"]["n"]))
+    nc, cs = grid_shape(float(r), "box")
+    cell = np.floor(allp / np.float32(0.01)).astype(np.int64)
+    key, cnt = np.unique(cell[:, 0] * 1000 + cell[:, 1], return_counts=True)
+    cx, cy = divmod(int(key[np.argmax(cnt)]), 1000)
+    c = np.array([(cx + 0.5) * 0.01, (cy + 0.5) * 0.01], np.float32)
+    pts = allp[np.all(np.abs(allp - c) < 0.012, axis=1)]
+    n = len(pts)
+    g = M.GridCount2d(n, (0.0, 0.0), (cs, cs), (nc, nc), M.GRID_MORTON)
+    pos = torch.from_numpy(pts.copy()).cuda()
+    g.update(pos)
+    got = _pairs_gpu(g, pos, float(r))
+    counts, nbr = O.grid_frnn(pts, (0, 0), (cs, cs), (nc, nc), float(r), cap=1536)
+    assert counts.max() < 1536
+    print("window points", n, "mean neighbours", len(got) / n)
+    assert len(got) / n > 80
+    assert np.array_equal(got, _pairs_oracle(counts, nbr))
+    bc, bn = O.brute_frnn(pts, float(r), cap=1536)
+    assert np.array_equal(got, _pairs_oracle(bc, bn))

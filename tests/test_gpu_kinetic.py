@@ -328,3 +328,57 @@ def test_move_flipflop_degenerate_link_device(oracle):
     assert dt.moveFlipflopDelaunay(v, np.float32([0.17, 0.06]))
     assert np.array_equal(dt.download_positions()[v], np.float32([0.5, 0.5]) + np.float32([0.17, 0.06]))
     _check_box(O, dt, "degenerate link")
+
+
+def test_integrate_move_points(oracle):
+    """integrateMovePoints(dt, v) (kernels.cuh:45-50, Device_Triangulation.h:
+    403-410): pos += v * dt in FP32 for every point, corners fixed; bit-equal
+    to the same FP32 arithmetic on the host; the mesh is then legalised"""
+    import torch
+    O = oracle
+    n = 5000
+    H = M.HostTriangulation()
+    H.addDelaunayPoints(uniform_points(n, 14))
+    dt = M.DeviceTriangulation(H)
+    rng = np.random.default_rng(2)
+    nv = dt.m_pos_size
+    vel = (1e-3 * rng.standard_normal((nv, 2))).astype(np.float32)
+    step = rng.uniform(0.05, 0.2, nv).astype(np.float32)
+    p0 = dt.download_positions()
+    dt.integrateMovePoints(torch.from_numpy(step).cuda(), torch.from_numpy(vel).cuda())
+    p1 = dt.download_positions()
+    want = p0 + vel * step[:, None]
+    assert np.array_equal(p1[:4], p0[:4])
+    assert np.array_equal(p1[4:], want[4:])
+    # the moved points' triangulation, restored by the kinetic path's relocation-free form
+    if dt.count_inverted() == 0:
+        dt.delonize2()
+        _check_box(O, dt, "integrate + delonize2")
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_round_cap_reported(oracle, variant):
+    """delonize with max_rounds = 1 on a mesh that needs more rounds stops
+    with MCL_ERR_ROUND_CAP (the reference's loop has no cap,
+    Device_Triangulation.h:335-348) and a valid mesh; an uncapped call then
+    finishes the exact Delaunay triangulation"""
+    import torch
+    O = oracle
+    H = M.HostTriangulation()
+    H.addDelaunayPoints(uniform_points(5000, 4))
+    dt = M.DeviceTriangulation(H)
+    d = torch.zeros((dt.m_pos_size, 2), dtype=torch.float32, device="cuda")
+    g = torch.Generator(device="cuda").manual_seed(5)
+    d[4:] = 3e-5 * torch.randn((dt.m_pos_size - 4, 2), generator=g, device="cuda")
+    dt.movePoints(d)
+    torch.cuda.synchronize()
+    assert dt.count_inverted() == 0 and dt.count_non_delaunay() > 0
+    fn = dt.delonize2 if variant == 0 else dt.delonize
+    with pytest.raises(M.MclError) as e:
+        fn(max_rounds=1)
+    assert e.value.code == 7
+    pos, v, he, t = dt.download()
+    rc, msg = O.check_mesh(pos, v, he, t, periodic=False, require_ccw=True)
+    assert rc == 0, msg
+    fn()
+    _check_box(O, dt, "after the cap")

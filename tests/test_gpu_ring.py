@@ -49,9 +49,16 @@ def test_legacy_ring_layouts(oracle, layout, K):
     pos, v, he, t = dt.download()
     nv = len(pos)
     ring = torch.full((nv * K,), -7, dtype=torch.int32, device="cuda")
-    (dt.oneRing if layout == 0 else dt.oneRing2)(K, ring)
+    ok = (dt.oneRing if layout == 0 else dt.oneRing2)(K, ring)
     r = ring.cpu().numpy()
     adj = _adjacency(he, t, nv)
+    # points with more than K - 1 neighbours are counted (the reference
+    # truncates them silently) and the call reports it; the corners' fans
+    # are truncated without being counted
+    over = sum(len(a) > K - 1 for a in adj[4:])
+    assert dt.ring_overflow == over and ok == (over == 0)
+    if K == 5:
+        assert over > 0
     for i in range(4, nv):
         cnt = r[i] if layout == 0 else r[K * i]
         ids = [r[(c + 1) * nv + i] if layout == 0 else r[(c + 1) + K * i] for c in range(cnt)]
