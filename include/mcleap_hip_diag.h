@@ -21,6 +21,8 @@ extern "C" {
 #define MCL_PRED_INCIRCLE_F64 3    /* incircle, FP64 filter -> exact */
 #define MCL_PRED_ORIENT_PLAIN 4    /* sign of the FP64 determinant, no filter, no exact stage */
 #define MCL_PRED_INCIRCLE_PLAIN 5  /* sign of the FP64 lifted determinant, no filter */
+#define MCL_PRED_INCIRCLE_EXACT 6  /* incircle by exact expansion arithmetic alone, tie rule (no filter) */
+#define MCL_PRED_ORIENT_EXACT 7    /* orient2d by exact expansion arithmetic alone */
 
 /* Evaluates n predicate cases on the device.  d_pts holds float2 points,
  * 3 per case for the orientation kinds (a, b, c: sign of (a-c) x (b-c)) and
@@ -28,7 +30,8 @@ extern "C" {
  * the circle through CCW a, b, c).  periodic != 0 takes differences as the
  * minimum image on the unit torus.  d_sign receives -1/0/+1 per case;
  * d_stage (may be NULL) receives the deciding stage 0 FP32, 1 FP64,
- * 2 exact-small, 3 exact-big (-1 for the plain kinds).  Synchronous. */
+ * 2 exact-small, 3 exact-big (-1 for the plain and exact-only kinds).
+ * Synchronous. */
 int mcl_eval_predicates(int kind, int n, const void* d_pts, int periodic, void* d_sign, void* d_stage,
                         void* stream);
 

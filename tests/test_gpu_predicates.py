@@ -71,3 +71,16 @@ def test_plain_fp64_forms_disagree_on_adversarial_cases():
         wrong_i += int((got != want).sum())
     print("plain FP64: %d orient and %d incircle signs differ from exact" % (wrong_o, wrong_i))
     assert wrong_i > 0
+
+
+def test_exact_only_kinds_equal_golden():
+    """the exact-only kinds (no filter: every case through the expansion
+    arithmetic, the reference for the sign-disagreement counts bench.py
+    reports) give the golden exact signs, with the tie rule for incircle"""
+    for x, want, per in _split("orient"):
+        got, _ = _eval(7, x, per)
+        assert np.array_equal(got, want), (per, np.nonzero(got != want)[0][:10])
+    p = GOLD["incircle_periodic"]
+    for per in (False, True):
+        got, _ = _eval(6, GOLD["incircle_in"][p == per], per)
+        assert np.array_equal(got, GOLD["incircle_sos"][p == per]), per
