@@ -6,8 +6,11 @@ A step is one pass of the hot path over the whole point set: move every
 point, restore the exact Delaunay triangulation (on the torus: per-vertex
 star recomputation with consistency checks; in the box: repair flips /
 take-backs / relocation / Lawson flips), one-ring CSR, and the uniform-grid
-FRNN rebuild + query (CSR).  Inputs live in HBM before the timed region; the
-K timed steps are bracketed by torch.cuda.synchronize().  After the timed
+FRNN rebuild + query (CSR), issued as one mcl_tri_timestep call (with
+--graph 1, the default, the star step's head and the tail are hipGraph
+replays).  Inputs live in HBM before the timed region; the K timed steps are
+bracketed by torch.cuda.synchronize().  Then --timer-steps further steps run
+with eager launches and per-kernel HIP event timers, for the roofline.  After the timed
 steps the final mesh is read through the half-edge arrays and checked (every
 edge exactly locally Delaunay, no inverted triangle, every vertex at its
 target except exact-duplicate targets); a failed check exits non-zero.
@@ -88,6 +91,10 @@ def _args():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the c5 sub-record of the default c2 run")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--no-pred-check", action="store_true")
+    ap.add_argument("--graph", type=int, default=1, choices=(0, 1),
+                    help="replay each timestep's star head and tail from hipGraphs (MCL_OPT_GRAPH)")
+    ap.add_argument("--timer-steps", type=int, default=10,
+                    help="eager steps timed after the run with per-kernel event timers (the roofline figures)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (reference for HBM byte counters)")
     return ap.parse_args()
@@ -341,26 +348,28 @@ def run_workload(a, M, torch, workload, steps, warmup, plain_steps, cpu_seconds,
         f_nbr = torch.zeros(f_cap, dtype=torch.int32, device=dev)
 
     def step(k, variant=a.variant):
+        if bgrid is None:
+            # the whole timestep in one call: kinetic step, CSR one-ring, grid sort + FRNN
+            return dt.timestep(cfg["motion"], cfg["sigma"], cfg["amp"], seed=cfg["seed"], step=k, variant=variant,
+                               ring=(ring_off, ring_nbr), grid=grid, rad=radius or 0.0,
+                               frnn=(f_off, f_nbr) if grid is not None else None)
         st = dt.step(cfg["motion"], cfg["sigma"], cfg["amp"], seed=cfg["seed"], step=k, variant=variant)
         dt.one_ring_csr_async(ring_off, ring_nbr)
-        if grid is not None:
-            grid.build(pos_t)
-            grid.frnn_csr_async(grid.sorted_ptr(), radius, f_off, f_nbr)
-        if bgrid is not None:
-            bgrid.build(pos_t)
-            bgrid.frnn_csr_async(f_off, f_nbr)
+        bgrid.build(pos_t)
+        bgrid.frnn_csr_async(f_off, f_nbr)
         return st
 
-    max_pairs = 0
+    def timers(on):
+        dt.kernel_timer(on)
+        for g_ in (grid, bgrid):
+            if g_ is not None:
+                g_.kernel_timer(on)
+
+    dt.set_option(M.OPT_GRAPH, a.graph)
     k = 0
     for _ in range(warmup):
         step(k)
         k += 1
-    torch.cuda.synchronize()
-    dt.kernel_timer(True)
-    for g_ in (grid, bgrid):
-        if g_ is not None:
-            g_.kernel_timer(True)
     stats = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -370,14 +379,29 @@ def run_workload(a, M, torch, workload, steps, warmup, plain_steps, cpu_seconds,
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     ms_per_step = 1e3 * el / steps
-    print("[bench] %s timed %d steps: %.3f ms/step" % (workload, steps, ms_per_step), file=sys.stderr, flush=True)
     value = n * steps / el
+    print("[bench] %s timed %d steps: %.3f ms/step%s" % (workload, steps, ms_per_step, " (graphs)" if a.graph else ""),
+          file=sys.stderr, flush=True)
+    # per-kernel GPU times: further steps launched eagerly with event timers
+    dt.set_option(M.OPT_GRAPH, 0)
+    timers(True)
+    tsteps = max(1, a.timer_steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(tsteps):
+        stats.append(step(k))
+        k += 1
+    torch.cuda.synchronize()
+    eager_ms = 1e3 * (time.perf_counter() - t1) / tsteps
+    print("[bench] %s %d eager steps with kernel timers: %.3f ms/step" % (workload, tsteps, eager_ms),
+          file=sys.stderr, flush=True)
 
     # per-kernel times from the live event timers
     kern = {}
     for name, which in (("lawson", 0), ("check", 1), ("move", 2), ("reloc_apply", 3), ("revert_apply", 4),
                         ("star", 5)):
         kern[name] = dt.kernel_timer_read(which)
+    kern = {k_: v_ for k_, v_ in kern.items() if v_[1] > 0}
     kbar = 0.0
     for g_ in (grid, bgrid):
         if g_ is None:
@@ -388,7 +412,7 @@ def run_workload(a, M, torch, workload, steps, warmup, plain_steps, cpu_seconds,
         kbar = last_pairs / g_.numP
         if last_pairs > f_cap:
             fail("FRNN output needs %d entries, the buffer holds %d" % (last_pairs, f_cap))
-    dt.kernel_timer(False)
+    timers(False)
     bytes_per_pt = dict(PHASE_BYTES)
     bytes_per_pt["frnn_fill"] = 4.0 * kbar
     # algorithmic bytes of all timed launches of each kernel class (none for
@@ -491,7 +515,9 @@ def run_workload(a, M, torch, workload, steps, warmup, plain_steps, cpu_seconds,
                           "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
                           "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / PEAK_HBM_GBS, "kbar": kbar},
         "kernels_ms_per_launch": {k_: (v_[0] / max(v_[1], 1)) for k_, v_ in kern.items()},
-        "kernels_ms_per_step": {k_: v_[0] / steps for k_, v_ in kern.items()},
+        "kernels_ms_per_step": {k_: v_[0] / tsteps for k_, v_ in kern.items()},
+        "graph": bool(a.graph),
+        "eager_ms_per_step": eager_ms,
         "plain_value": plain,
         "plain_sample": plain_sample,
         "unbatched_value": unbatched,
@@ -554,7 +580,8 @@ def main():
         "dtype": "fp32",
         "data": "synthetic (seeded uniform points, Philox motion on the GPU)",
     }
-    for key in ("config", "roofline", "valu_roofline", "step_roofline", "kernels_ms_per_launch", "kernels_ms_per_step",
+    for key in ("config", "graph", "eager_ms_per_step", "roofline", "valu_roofline", "step_roofline",
+                "kernels_ms_per_launch", "kernels_ms_per_step",
                 "plain_value", "plain_sample", "fast_vs_plain", "unbatched_value", "per_step_totals", "final_mesh",
                 "frnn", "predicates", "predicate_check", "cpu_baseline"):
         out[key] = rec[key]

@@ -136,6 +136,7 @@ int mcl_hmesh_untangle(float* pos, int* he4, int nhe, int* t1, int nt, int* v2, 
  * Device triangulation  (DeviceTriangulation, Device_Triangulation.h:128-223)
  * ========================================================================= */
 typedef struct mcl_tri mcl_tri_t;
+typedef struct mcl_grid mcl_grid_t; /* the uniform grid of GridCount2d, declared below */
 
 /* DeviceTriangulation(HostTriangulation*): allocateMemory + transferToDevice
  * (Device_Triangulation.h:232-236, 255-285).  The AoS host arrays are stored
@@ -283,6 +284,19 @@ int mcl_tri_move_legalize(mcl_tri_t* t, const void* d_disp, int variant, mcl_sta
  * kernels.cuh:38-50) */
 void* mcl_tri_target_ptr(mcl_tri_t* t);
 
+/* One whole timestep of the reference's driver loop (moving_device_mesh.cu:
+ * 109-243: move, untangle + legalize, one-ring, grid update + FRNN query):
+ * mcl_tri_step, then, asynchronously, the CSR one-ring into d_ring_off /
+ * d_ring_nbrs (as mcl_tri_one_ring_csr; ring_cap >= nhe) when d_ring_off is
+ * given, and the grid g's counting sort of the new positions with its CSR
+ * FRNN query into d_frnn_off / d_frnn_nbrs (as mcl_grid_build +
+ * mcl_grid_frnn_csr with total = NULL: writes past frnn_cap dropped,
+ * d_frnn_off[nump] the full count) when g is given (g sized for every vertex).
+ * With MCL_OPT_GRAPH the tail is one graph launch.  Status as mcl_tri_step. */
+int mcl_tri_timestep(mcl_tri_t* t, const mcl_step_params_t* p, void* d_ring_off, void* d_ring_nbrs, int ring_cap,
+                     mcl_grid_t* g, float rad, void* d_frnn_off, void* d_frnn_nbrs, int frnn_cap, mcl_stats_t* st,
+                     void* stream);
+
 /* ------------------------------------------------------------- queries */
 /* oneRing<K>(int*) (layout 0, SoA: ring[(c+1)*nv + i], count in ring[i]) and
  * oneRing2<K>(int*) (layout 1, AoS: ring[(c+1) + K*i], count in ring[K*i]),
@@ -322,7 +336,6 @@ int mcl_tri_frnn_graph(mcl_tri_t* t, float r, int K, int layout, const void* d_r
 /* =========================================================================
  * Uniform-grid FRNN  (GridCount2d, gridCount2d.h:23-48)
  * ========================================================================= */
-typedef struct mcl_grid mcl_grid_t;
 #define MCL_GRID_ROWMAJOR 0    /* reference cell key (y*nx + x) % (nx*ny) */
 #define MCL_GRID_MORTON 1      /* Morton-ordered cell key */
 #define MCL_GRID_PERIODIC 2    /* flag: wrap cells, minimum-image distances */
@@ -386,10 +399,15 @@ int mcl_grid_mean_occupancy(mcl_grid_t* g, float* out, void* stream);
  *   slower (up to 4096 steps).
  * MCL_OPT_STAR_GIVE_UP: the star step falls back to the flip-based step at
  *   once when its first consistency check lists more than this percentage of
- *   the vertices (1..100, default 40; 100 = always run the retry passes). */
+ *   the vertices (1..100, default 40; 100 = always run the retry passes).
+ * MCL_OPT_GRAPH: 1 = the star step's head (move, first pass, checks and the
+ *   retries before the first host read) and mcl_tri_timestep's tail are
+ *   captured once as hipGraphs and replayed every step (the results are those
+ *   of the eager launches); 0 (default) = eager launches. */
 #define MCL_OPT_STAR_MIN_VERTICES 1
 #define MCL_OPT_STAR_ENABLE 2
 #define MCL_OPT_STAR_GIVE_UP 3
+#define MCL_OPT_GRAPH 4
 int mcl_tri_set_option(mcl_tri_t* t, int option, long long value);
 
 /* ------------------------------------------------------- kernel timers */

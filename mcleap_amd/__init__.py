@@ -6,7 +6,7 @@ Python mirror of the reference's host interface (HostTriangulation,
 DeviceTriangulation, GridCount2d) used by tests and benchmarks.
 """
 from ._lib import (GRID_MORTON, GRID_PERIODIC, GRID_ROWMAJOR, MOTION_GAUSS, MOTION_SHEAR, MOVE_FLIPFLOP,
-                   MOVE_FLIPFLOP_DELAUNAY, MOVE_UNTANGLING, MOVE_UNTANGLING_DELAUNAY, OPT_STAR_ENABLE, OPT_STAR_GIVE_UP,
+                   MOVE_FLIPFLOP_DELAUNAY, MOVE_UNTANGLING, MOVE_UNTANGLING_DELAUNAY, OPT_GRAPH, OPT_STAR_ENABLE, OPT_STAR_GIVE_UP,
                    OPT_STAR_MIN_VERTICES, MclError, lib)
 from .triangulation import (DeviceTriangulation, GridCount2d, GridCount2dBatch, HostTriangulation,
                             HostTriangulationBatch)
@@ -16,5 +16,5 @@ __all__ = [
     "MclError", "lib",
     "MOTION_GAUSS", "MOTION_SHEAR", "GRID_ROWMAJOR", "GRID_MORTON", "GRID_PERIODIC", "OPT_STAR_MIN_VERTICES",
     "OPT_STAR_ENABLE",
-    "OPT_STAR_GIVE_UP", "MOVE_FLIPFLOP", "MOVE_FLIPFLOP_DELAUNAY", "MOVE_UNTANGLING", "MOVE_UNTANGLING_DELAUNAY",
+    "OPT_STAR_GIVE_UP", "OPT_GRAPH", "MOVE_FLIPFLOP", "MOVE_FLIPFLOP_DELAUNAY", "MOVE_UNTANGLING", "MOVE_UNTANGLING_DELAUNAY",
 ]

@@ -34,6 +34,7 @@ MOVE_UNTANGLING_DELAUNAY = 3
 OPT_STAR_MIN_VERTICES = 1
 OPT_STAR_ENABLE = 2
 OPT_STAR_GIVE_UP = 3
+OPT_GRAPH = 4
 
 
 class MclError(RuntimeError):
@@ -113,6 +114,7 @@ SIGNATURES = {
     "mcl_tri_count_inverted": [_p, _p, _p],
     "mcl_tri_count_non_delaunay": [_p, _p, _p],
     "mcl_tri_step": [_p, _p, _p, _p],
+    "mcl_tri_timestep": [_p, _p, _p, _p, _i, _p, _f, _p, _p, _i, _p, _p],
     "mcl_tri_one_ring": [_p, _i, _i, _p, ctypes.POINTER(_i), _p],
     "mcl_tri_one_ring_csr": [_p, _p, _p, _i, _p, _p],
     "mcl_tri_closest": [_p, _p, _p, _p, _p],

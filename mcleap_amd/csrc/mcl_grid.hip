@@ -354,6 +354,19 @@ extern "C" int mcl_grid_frnn_csr(mcl_grid_t* G, const void* d_pos, float rad, vo
                     G->scan_tmp, (hipStream_t)stream, &G->timers[0], &G->timers[1]);
 }
 
+// the counting sort of pos into the grid and the CSR query over the sorted
+// copy, asynchronous and without host reads (writes past cap dropped), as a
+// timestep's tail enqueues (or captures) it
+int mcl::grid_enqueue_frnn(mcl_grid_t* G, const float2* pos, float rad, int* off, int* nbrs, int cap,
+                           hipStream_t s) {
+    if (!G || !pos || !off) return MCL_ERR_INPUT;
+    G->timers[2].mark(s);
+    MCL_TRY(grid_sort(G->g, pos, nullptr, G->nump, G->bufs(), s));
+    G->timers[2].mark(s);
+    return frnn_csr(G->g, G->spos, G->nump, G->start, rad, off, nbrs, cap, nullptr, G->scan_tmp, s, &G->timers[0],
+                    &G->timers[1]);
+}
+
 // ------------------------------------------------------------------------
 // The grid FRNN without any allocation (mcl_grid_frnn_csr_noalloc): the
 // counting sort's buffers live in `scratch`, a file-scope device array of

@@ -85,6 +85,8 @@ struct DMesh {
 // the flip rounds' round and stamp counters of a triangulation (read, or
 // set when set is true); saved in state files so that a resumed run repeats
 int tri_counters(mcl_tri_t* t, int* round, int* stamp, bool set);
+// grid sort of pos + the asynchronous CSR FRNN query, no host reads
+int grid_enqueue_frnn(mcl_grid_t* g, const float2* pos, float rad, int* off, int* nbrs, int cap, hipStream_t s);
 
 // a valid triangulation of points at their row positions, built on the
 // device into m: ninst box instances of counts[i] points (nv = N + 4 ninst,
@@ -135,6 +137,8 @@ struct KTimer {
     }
     void mark(hipStream_t s) {
         if (!on) return;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return;  // not in graphs
         if (used == ev.size()) {
             hipEvent_t e;
             if (hipEventCreate(&e) != hipSuccess) { on = false; return; }

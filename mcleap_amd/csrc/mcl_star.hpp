@@ -769,10 +769,11 @@ __device__ __forceinline__ void append_finish(AppendBuf& ab, int* count, int* __
 __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restrict__ vlist, const int* __restrict__ nlist,
                                                     const int* __restrict__ deg, const int* __restrict__ nbr,
                                                     const float2* __restrict__ pos, int periodic, int ncorner,
-                                                    int mark_nbrs, int* __restrict__ mark, int stamp,
-                                                    int* __restrict__ list, int* __restrict__ count,
+                                                    int mark_nbrs, int* __restrict__ mark, const int* __restrict__ stamp_base,
+                                                    int stamp_off, int* __restrict__ list, int* __restrict__ count,
                                                     long long* __restrict__ sum_deg) {
     __shared__ AppendBuf ab;
+    const int stamp = *stamp_base + stamp_off;
     append_init(ab);
     const int max_tile = (int)blockDim.x * (mark_nbrs ? kStarCap + 1 : 1);
     long long my_sum = 0;
@@ -822,10 +823,12 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
 __global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vlist, const int* __restrict__ nlist_dev,
                                                       const int* __restrict__ tdeg, const int* __restrict__ tnbr,
                                                       int* __restrict__ deg, int* __restrict__ nbr,
-                                                      int* __restrict__ mark, int stamp, int* __restrict__ rlist,
+                                                      int* __restrict__ mark, const int* __restrict__ stamp_base,
+                                                      int stamp_off, int* __restrict__ rlist,
                                                       int* __restrict__ rcount, long long* __restrict__ sum_deg,
                                                       long long* __restrict__ retried) {
     __shared__ AppendBuf ab;
+    const int stamp = *stamp_base + stamp_off;
     append_init(ab);
     const int nlist = *nlist_dev;
     if (blockIdx.x == 0 && threadIdx.x == 0 && nlist) {
@@ -867,6 +870,12 @@ __global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vl
 // After the first check: when more than limit vertices are inconsistent
 // (motion the star step cannot follow in a few passes), empty the list and
 // flag the step as given up (aux[19]); later passes then do nothing.
+// the mark stamps of one star step: *base + 1 .. *base + kStampSpan - 1
+constexpr int kStampSpan = 64;
+__global__ void k_stamp_advance(int* __restrict__ base) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *base += kStampSpan;
+}
+
 __global__ void k_star_give_up(int* __restrict__ aux, int limit) {
     if (blockIdx.x == 0 && threadIdx.x == 0 && aux[1] > limit) {
         aux[1] = 0;

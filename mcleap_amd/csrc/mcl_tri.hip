@@ -215,9 +215,13 @@ constexpr int kMotionTarget = 1000;      // internal: the displacement array hol
 constexpr int kMotionTargetHalf = 1001;  // internal: half way from the current position to those targets
 // synthetic displacement of every non-corner vertex; stores the accepted
 // position (acc), the target (tgt) and moves the vertex to its target
+// (pdev: the parameters in device memory, read at run time, as a captured
+// graph needs; null: the by-value pv)
 __global__ void __launch_bounds__(kThreads) k_move_motion(DMesh m, float2* __restrict__ acc, float2* __restrict__ tgt,
                                                           int* __restrict__ v_mv, int* __restrict__ v_fexp, int stamp,
-                                                          mcl_step_params_t p, const float2* __restrict__ disp) {
+                                                          mcl_step_params_t pv, const mcl_step_params_t* __restrict__ pdev,
+                                                          const float2* __restrict__ disp) {
+    const mcl_step_params_t p = pdev ? *pdev : pv;
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
         float2 q = m.pos[v];
         acc[v] = q;
@@ -1680,6 +1684,15 @@ struct mcl_tri {
     int* s_heavy = nullptr;                 // nv + 1: count, then rows for the full star kernel
     int* c_bucket = nullptr;                // ncorner * kCornerCap * 3: corner fan entries
     int* c_count = nullptr;                 // ncorner
+    int* d_stamp = nullptr;                 // 1: base of the star checks' mark stamps (advanced on the device)
+    mcl_step_params_t* d_params = nullptr;  // the step's parameters, read by the captured move
+    mcl_step_params_t* h_params = nullptr;  // pinned staging copy of them
+    int graph = 0;                          // MCL_OPT_GRAPH: star steps (and timesteps) replayed from hipGraphs
+    hipGraphExec_t g_head[2] = {nullptr, nullptr};  // star-step head per star_cur
+    hipGraphExec_t g_post[2] = {nullptr, nullptr};  // timestep tail (ring CSR + grid FRNN) per star_cur
+    const void* post_key[8] = {};           // the buffers g_post was captured with
+    bool capturing = false;                 // kernel timers are not recorded under capture
+    hipStream_t cap_stream = nullptr;       // stream the graphs are captured on (the caller's may be the null stream)
     // one ring
     int* deg = nullptr;
     int* scan_tmp = nullptr;
@@ -1724,9 +1737,15 @@ static int tri_free(mcl_tri* t) {
                     t->rev, t->allrev, t->pend[0], t->pend[1], t->deg, t->scan_tmp, t->v_touch, t->t_rep, t->t_rep_e, t->e_undo, t->undo, t->stuck, t->touched, t->r_tt, t->r_tt2, t->r_wait, t->v_soft, t->tail_out,
                     t->star_deg[0], t->star_deg[1], t->star_nbr[0], t->star_nbr[1], t->s_ecnt, t->s_tcnt,
                     t->s_mark, t->s_list, t->s_aux, t->star_tdeg, t->star_tnbr, t->s_rlist, t->s_heavy,
-                    t->c_bucket, t->c_count};
+                    t->c_bucket, t->c_count, t->d_stamp, t->d_params};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    for (int k = 0; k < 2; k++) {
+        if (t->g_head[k]) (void)hipGraphExecDestroy(t->g_head[k]);
+        if (t->g_post[k]) (void)hipGraphExecDestroy(t->g_post[k]);
+    }
+    if (t->h_params) (void)hipHostFree(t->h_params);
+    if (t->cap_stream) (void)hipStreamDestroy(t->cap_stream);
     if (t->h_cnt) (void)hipHostFree(t->h_cnt);
     if (t->h_stats) (void)hipHostFree(t->h_stats);
     return MCL_OK;
@@ -1831,11 +1850,12 @@ int tri_make(int nv, int nhe, int nt, int periodic, int ncorner, mcl_tri** out) 
         A(t->star_nbr[0], star_rows_size(nv, m.ncorner)); A(t->star_nbr[1], star_rows_size(nv, m.ncorner));
         A(t->star_tdeg, nv); A(t->star_tnbr, star_rows_size(nv, m.ncorner));
         A(t->s_ecnt, nv + 1); A(t->s_tcnt, nv + 1); A(t->s_mark, nv); A(t->s_list, nv); A(t->s_aux, kStarAux);
-        A(t->s_rlist, nv); A(t->s_heavy, nv + 1);
+        A(t->s_rlist, nv); A(t->s_heavy, nv + 1); A(t->d_stamp, 1); A(t->d_params, 1);
         if (m.ncorner > 0) { A(t->c_bucket, (size_t)m.ncorner * kCornerCap * 3); A(t->c_count, m.ncorner); }
     }
 #undef A
     if (rc == MCL_OK && hipHostMalloc((void**)&t->h_cnt, sizeof(int) * kCnCount) != hipSuccess) rc = MCL_ERR_HIP;
+    if (rc == MCL_OK && hipHostMalloc((void**)&t->h_params, sizeof(mcl_step_params_t)) != hipSuccess) rc = MCL_ERR_HIP;
     if (rc == MCL_OK &&
         hipHostMalloc((void**)&t->h_stats, sizeof(unsigned long long) * kStCount * kStatShards) != hipSuccess)
         rc = MCL_ERR_HIP;
@@ -1853,6 +1873,7 @@ int tri_make(int nv, int nhe, int nt, int periodic, int ncorner, mcl_tri** out) 
         hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, 0, t->cnt, (int)kCnCount, 0);
         hipLaunchKernelGGL(k_set_u64, dim3(4), dim3(256), 0, 0, t->stats, (int)(kStCount * kStatShards), 0ull);
         if (t->s_mark) hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(nv)), dim3(kThreads), 0, 0, t->s_mark, nv, 0);
+        if (t->d_stamp) hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, 0, t->d_stamp, 1, 0);
         if (hipDeviceSynchronize() != hipSuccess) rc = MCL_ERR_HIP;
     }
     if (rc != MCL_OK) { tri_free(t); delete t; return rc; }
@@ -2730,7 +2751,7 @@ static int step_impl(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     const int mv_stamp = ++t->stamp;
     t->timers[2].mark(s);
     hipLaunchKernelGGL(k_move_motion, dim3(blocks_for(m.nv)), dim3(kThreads), 0, s, m, t->acc, t->tgt, t->v_mv,
-                       t->v_fexp, mv_stamp, *p, disp);
+                       t->v_fexp, mv_stamp, *p, (const mcl_step_params_t*)nullptr, disp);
     t->timers[2].mark(s);
     MCL_TRY(zero_slots(t, s, kCnBad, 11));  // bad, rev, move, pend, err, misc, touched
     t->timers[1].mark(s);
@@ -2834,96 +2855,190 @@ static int ensure_mesh(mcl_tri* t, hipStream_t s) {
 // Kinetic step on the torus by star recomputation (see mcl_star.hpp).
 // Returns MCL_OK with *done = false when the stars could not be settled; the
 // positions are then back at the start of the step and the mesh untouched.
+//
+// The step's first kStarHeadPasses star passes (the move, the first pass over
+// every vertex with its consistency check and give-up test, and retries of the
+// listed vertices) are enqueued without any host read: every list length
+// stays on the device, kernels over an empty list return at once, and the
+// check stamps come from a device counter.  With MCL_OPT_GRAPH this head is
+// captured once per star buffer parity and replayed as one hipGraph launch
+// (its parameters copied to the device first).  The host then reads the
+// counters once; the rare step that has not settled continues eagerly, one
+// read per pass.
+constexpr int kStarHeadPasses = kStarSyncFrom + 1;
+
+namespace {
+// launch geometry of the star kernels
+struct StarGrid {
+    mcl_tri* t;
+    // blocks for n vertices, kv vertices per block iteration (a multiple of 8
+    // when large enough, for the XCD-aware split)
+    dim3 full(long long nverts, int kv) const {
+        long long b = std::max<long long>(1, std::min<long long>((nverts + kv - 1) / kv, 4LL * t->grid_blocks));
+        if (b >= 64) b &= ~7LL;
+        return dim3((unsigned)b);
+    }
+    // list lengths stay on the device: grids sized for a long list
+    dim3 list() const { return dim3((unsigned)(std::min<long long>(4LL * t->grid_blocks, 2048) & ~7LL)); }
+};
+
+// stars of the listed vertices (all without a list; count *nlist_dev) into
+// (odeg, onbr): the filter-only kernel, two vertices per wave, then the full
+// kernel on the vertices it could not finish (more than 64 candidates, or a
+// test the filter could not decide)
+void star_pass_stars(mcl_tri* t, hipStream_t s, int cur, const int* vlist, const int* nlist_dev, int depth,
+                     const int* bdeg, const int* bnbr, int* odeg, int* onbr) {
+    DMesh& m = t->m;
+    const StarGrid G{t};
+    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_heavy, 1, 0);
+    if (!vlist)  // first pass: 2-rings read off the previous stars directly
+        hipLaunchKernelGGL((k_star<true, true, 32, true>), G.full(m.nv, 8), dim3(256), 0, s, m, vlist, 0, nlist_dev,
+                           (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
+                           (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy,
+                           (const float2*)t->acc, (const float2*)t->tgt, t->stats);
+    else
+        hipLaunchKernelGGL((k_star<true, true, 32, false>), G.list(), dim3(256), 0, s, m, vlist, 0, nlist_dev,
+                           (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
+                           (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy,
+                           (const float2*)t->acc, (const float2*)t->tgt, t->stats);
+    hipLaunchKernelGGL((k_star<true, false, 64, false>), dim3(256), dim3(256), 0, s, m, vlist, 0, nlist_dev,
+                       (const int*)(t->s_heavy + 1), (const int*)t->s_heavy, depth, (const int*)t->star_deg[cur],
+                       (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, (int*)nullptr,
+                       (const float2*)t->acc, (const float2*)t->tgt, t->stats);
+}
+
+// s_aux: [0] failed stars, [1] listed vertices, [2..3] degree sum of the
+// valid stars, [4..7] failures by reason, [8] vertices to re-check,
+// [10] vertices handed to the full kernel, [12..13] vertices retried,
+// [14] retry passes with a non-empty list, [16..18] corner fans, [19] gave up
+//
+// The first check covers every vertex and lists the inconsistent ones; later
+// ones cover the re-check list.  mark_nbrs: list the star neighbours of the
+// inconsistent vertices too (otherwise each conflict lists only the side that
+// loses its incircle test).  stamp_off: this check's offset from the step's
+// stamp base.
+void star_check(mcl_tri* t, hipStream_t s, int nxt, bool first, bool mark_nbrs, int stamp_off) {
+    DMesh& m = t->m;
+    const StarGrid G{t};
+    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 1, 1, 0);
+    // the first check: 4 tiles of 256 vertices per block, its appends
+    // flushed once per block
+    hipLaunchKernelGGL(k_star_check, first ? G.full(m.nv, 1024) : G.list(), dim3(256), 0, s, m.nv,
+                       first ? (const int*)nullptr : (const int*)t->s_rlist,
+                       first ? (const int*)nullptr : (const int*)(t->s_aux + 8), (const int*)t->star_deg[nxt],
+                       (const int*)t->star_nbr[nxt], (const float2*)m.pos, m.periodic, m.ncorner, mark_nbrs ? 1 : 0,
+                       t->s_mark, (const int*)t->d_stamp, stamp_off, t->s_list, t->s_aux + 1,
+                       (long long*)(t->s_aux + 2));
+}
+
+// retry pass r (1 <= r < kStarPasses): recompute the listed vertices from the
+// 2-rings in the previous mesh and in the stars found so far (a vertex whose
+// star failed, having left its 2-ring, searches one ring deeper each pass),
+// copy them in and re-check
+void star_retry(mcl_tri* t, hipStream_t s, int cur, int nxt, int r) {
+    const StarGrid G{t};
+    star_pass_stars(t, s, cur, t->s_list, t->s_aux + 1, 2 + r, t->star_deg[nxt], t->star_nbr[nxt], t->star_tdeg,
+                    t->star_tnbr);
+    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 8, 1, 0);
+    hipLaunchKernelGGL(k_star_scatter, G.list(), dim3(256), 0, s, (const int*)t->s_list, (const int*)(t->s_aux + 1),
+                       (const int*)t->star_tdeg, (const int*)t->star_tnbr, t->star_deg[nxt], t->star_nbr[nxt],
+                       t->s_mark, (const int*)t->d_stamp, 2 * r, t->s_rlist, t->s_aux + 8,
+                       (long long*)(t->s_aux + 2), (long long*)(t->s_aux + 12));
+    star_check(t, s, nxt, false, r >= kStarDecidePasses, 2 * r + 1);
+}
+
+// corner fans from this step's stars (box meshes), into aux[16..18]
+void star_corner_fans(mcl_tri* t, hipStream_t s, int cur, int nxt) {
+    DMesh& m = t->m;
+    if (m.ncorner == 0) return;
+    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 16, 3, 0);
+    hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(m.ncorner)), dim3(256), 0, s, t->c_count, m.ncorner, 0);
+    hipLaunchKernelGGL(k_corner_collect, dim3(blocks_for(m.nv)), dim3(256), 0, s, m.nv, m.ncorner,
+                       (const int*)t->star_deg[nxt], (const int*)t->star_nbr[nxt], t->c_bucket, t->c_count);
+    hipLaunchKernelGGL(k_corner_fans, dim3((m.ncorner + 63) / 64), dim3(64), 0, s, m.nv, m.ncorner,
+                       (const int*)t->c_bucket, (const int*)t->c_count, (const int*)t->star_deg[cur],
+                       (const int*)t->star_nbr[cur], t->star_deg[nxt], t->star_nbr[nxt], t->s_aux + 18,
+                       (long long*)(t->s_aux + 16));
+}
+
+// the head of a star step (see above); pdev: the step parameters in device
+// memory (graph), else p by value.  Ends with the corner fans, so that the
+// counters read after it describe a complete set of stars.
+void star_head(mcl_tri* t, hipStream_t s, int cur, int nxt, const mcl_step_params_t* p,
+               const mcl_step_params_t* pdev, const float2* disp, int mv_stamp) {
+    DMesh& m = t->m;
+    hipLaunchKernelGGL(k_stamp_advance, dim3(1), dim3(64), 0, s, t->d_stamp);
+    t->timers[2].mark(s);
+    hipLaunchKernelGGL(k_move_motion, dim3(blocks_for(m.nv)), dim3(kThreads), 0, s, m, t->acc, t->tgt, t->v_mv,
+                       t->v_fexp, mv_stamp, *p, pdev, disp);
+    t->timers[2].mark(s);
+    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux, kStarAux, 0);
+    t->timers[5].mark(s);
+    star_pass_stars(t, s, cur, nullptr, nullptr, 2, nullptr, nullptr, t->star_deg[nxt], t->star_nbr[nxt]);
+    t->timers[5].mark(s);
+    star_check(t, s, nxt, true, false, 1);
+    hipLaunchKernelGGL(k_star_give_up, dim3(1), dim3(64), 0, s, t->s_aux,
+                       (int)std::min<long long>((long long)t->star_give_up * m.nv / 100, m.nv));
+    for (int r = 1; r < kStarHeadPasses; r++) star_retry(t, s, cur, nxt, r);
+    star_corner_fans(t, s, cur, nxt);
+}
+}  // namespace
+
+static void drop_graphs(mcl_tri* t) {
+    for (int k = 0; k < 2; k++) {
+        if (t->g_head[k]) (void)hipGraphExecDestroy(t->g_head[k]);
+        if (t->g_post[k]) (void)hipGraphExecDestroy(t->g_post[k]);
+        t->g_head[k] = t->g_post[k] = nullptr;
+    }
+}
+
+// captures the work `enqueue(cs)` puts on the capture stream cs as a graph
+// and instantiates it (the graph is then launched on the caller's stream)
+template <class F>
+static int capture_graph(mcl_tri* t, hipGraphExec_t* out, F&& enqueue) {
+    if (!t->cap_stream) MCL_HIP(hipStreamCreateWithFlags(&t->cap_stream, hipStreamNonBlocking));
+    hipStream_t s = t->cap_stream;
+    hipGraph_t g = nullptr;
+    MCL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    t->capturing = true;
+    enqueue(s);
+    t->capturing = false;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    }
+    const hipError_t e2 = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) return hip_fail(e2, "hipGraphInstantiate", __FILE__, __LINE__);
+    return MCL_OK;
+}
+
 static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp, mcl_stats_t* st, hipStream_t s,
                      bool* done) {
     DMesh& m = t->m;
     *done = false;
     if (ensure_stars(t, s) != MCL_OK) return MCL_OK;
     const int cur = t->star_cur, nxt = 1 - cur;
-    const int mv_stamp = ++t->stamp;
-    t->timers[2].mark(s);
-    hipLaunchKernelGGL(k_move_motion, dim3(blocks_for(m.nv)), dim3(kThreads), 0, s, m, t->acc, t->tgt, t->v_mv, t->v_fexp,
-                       mv_stamp, *p, disp);
-    t->timers[2].mark(s);
-    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux, kStarAux, 0);
-    // blocks for a list of n vertices, kv vertices per block iteration
-    // (a multiple of 8 when large enough, for the XCD-aware split)
-    auto star_grid = [&](long long nverts, int kv) {
-        long long b = std::max<long long>(1, std::min<long long>((nverts + kv - 1) / kv, 4LL * t->grid_blocks));
-        if (b >= 64) b &= ~7LL;
-        return dim3((unsigned)b);
-    };
-    // list lengths stay on the device: retries, scatters and re-checks read
-    // them there, with grids sized for a long list (a short one leaves most
-    // blocks idle at once)
-    const dim3 list_grid((unsigned)(std::min<long long>(4LL * t->grid_blocks, 2048) & ~7LL));
-    // stars of the listed vertices (all without a list; count *nlist_dev)
-    // into (odeg, onbr): the filter-only kernel, two vertices per wave, then
-    // the full kernel on the vertices it could not finish (more than 64
-    // candidates, or a test the filter could not decide)
-    auto stars = [&](const int* vlist, const int* nlist_dev, int depth, const int* bdeg, const int* bnbr, int* odeg,
-                     int* onbr) {
-        hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_heavy, 1, 0);
-        if (!vlist)  // first pass: 2-rings read off the previous stars directly
-            hipLaunchKernelGGL((k_star<true, true, 32, true>), star_grid(m.nv, 8), dim3(256), 0, s, m, vlist, 0,
-                               nlist_dev, (const int*)nullptr, (const int*)nullptr, depth,
-                               (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr,
-                               t->s_aux, t->s_heavy, (const float2*)t->acc,
-                               (const float2*)t->tgt, t->stats);
-        else
-            hipLaunchKernelGGL((k_star<true, true, 32, false>), list_grid, dim3(256), 0, s, m, vlist, 0, nlist_dev,
-                               (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
-                               (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy, (const float2*)t->acc,
-                               (const float2*)t->tgt, t->stats);
-        hipLaunchKernelGGL((k_star<true, false, 64, false>), dim3(256), dim3(256), 0, s, m, vlist, 0, nlist_dev,
-                           (const int*)(t->s_heavy + 1), (const int*)t->s_heavy, depth, (const int*)t->star_deg[cur],
-                           (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, (int*)nullptr, (const float2*)t->acc,
-                               (const float2*)t->tgt, t->stats);
-    };
-    t->timers[5].mark(s);
-    stars(nullptr, nullptr, 2, nullptr, nullptr, t->star_deg[nxt], t->star_nbr[nxt]);
-    t->timers[5].mark(s);
-    MCL_HIP(hipGetLastError());
-    // s_aux: [0] failed stars, [1] listed vertices, [2..3] degree sum of the
-    // valid stars, [4..7] failures by reason, [8] vertices to re-check,
-    // [10] vertices handed to the full kernel, [12..13] vertices retried,
-    // [14] retry passes with a non-empty list
-    //
-    // The first check covers every vertex and lists the inconsistent ones;
-    // later ones cover the re-check list and add the star neighbours of the
-    // inconsistent vertices (a wrong star can agree with its neighbours').
-    // mark_nbrs: list the star neighbours of the inconsistent vertices too
-    // (otherwise each conflict lists only the side that loses its incircle test)
-    auto check = [&](bool first, bool mark_nbrs) {
-        hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 1, 1, 0);
-        const int stamp = ++t->stamp;
-        // the first check: 4 tiles of 256 vertices per block, its appends
-        // flushed once per block
-        hipLaunchKernelGGL(k_star_check, first ? star_grid(m.nv, 1024) : list_grid, dim3(256), 0, s, m.nv,
-                           first ? (const int*)nullptr : (const int*)t->s_rlist,
-                           first ? (const int*)nullptr : (const int*)(t->s_aux + 8), (const int*)t->star_deg[nxt],
-                           (const int*)t->star_nbr[nxt], (const float2*)m.pos, m.periodic, m.ncorner,
-                           mark_nbrs ? 1 : 0, t->s_mark,
-                           stamp, t->s_list, t->s_aux + 1,
-                           (long long*)(t->s_aux + 2));
-    };
-    int aux[kStarAux];
     const bool trace = trace_on();
-    // corner fans from this step's stars (box meshes), into aux[16..18]
-    auto corner_fans = [&]() {
-        if (m.ncorner == 0) return;
-        hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 16, 3, 0);
-        hipLaunchKernelGGL(k_set_ints, dim3(blocks_for(m.ncorner)), dim3(256), 0, s, t->c_count, m.ncorner, 0);
-        hipLaunchKernelGGL(k_corner_collect, dim3(blocks_for(m.nv)), dim3(256), 0, s, m.nv, m.ncorner,
-                           (const int*)t->star_deg[nxt], (const int*)t->star_nbr[nxt], t->c_bucket, t->c_count);
-        hipLaunchKernelGGL(k_corner_fans, dim3((m.ncorner + 63) / 64), dim3(64), 0, s, m.nv, m.ncorner, (const int*)t->c_bucket,
-                           (const int*)t->c_count, (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur],
-                           t->star_deg[nxt], t->star_nbr[nxt], t->s_aux + 18,
-                           (long long*)(t->s_aux + 16));
-    };
+    if (t->graph && !disp && !trace) {
+        if (!t->g_head[cur]) {
+            // v_mv marks matter only to the flip-based step, which writes its own
+            const int mv_stamp = ++t->stamp;
+            MCL_TRY(capture_graph(t, &t->g_head[cur], [&](hipStream_t cs) {
+                star_head(t, cs, cur, nxt, p, t->d_params, nullptr, mv_stamp);
+            }));
+        }
+        *t->h_params = *p;
+        MCL_HIP(hipMemcpyAsync(t->d_params, t->h_params, sizeof(mcl_step_params_t), hipMemcpyHostToDevice, s));
+        MCL_HIP(hipGraphLaunch(t->g_head[cur], s));
+    } else {
+        star_head(t, s, cur, nxt, p, nullptr, disp, ++t->stamp);
+    }
+    MCL_HIP(hipGetLastError());
+    int aux[kStarAux];
     // the counters, and the statistics with them (one synchronisation)
     auto read_aux = [&](int pass) -> int {
-        corner_fans();
         MCL_HIP(hipMemcpyAsync(aux, t->s_aux, sizeof(aux), hipMemcpyDeviceToHost, s));
         MCL_HIP(hipMemcpyAsync(t->h_stats, t->stats, sizeof(unsigned long long) * kStCount * kStatShards,
                                hipMemcpyDeviceToHost, s));
@@ -2936,41 +3051,24 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
                     pass, aux[1], aux[0], aux[4], aux[5], aux[6], aux[7], aux[10]);
         return MCL_OK;
     };
-    check(true, false);
-    hipLaunchKernelGGL(k_star_give_up, dim3(1), dim3(64), 0, s, t->s_aux,
-                       (int)std::min<long long>((long long)t->star_give_up * m.nv / 100, m.nv));
-    MCL_HIP(hipGetLastError());
-    if (trace) MCL_TRY(read_aux(0));
-    // Retries run without reading the counts back until kStarSyncFrom passes
-    // have been queued (at 1M points under noise of 0.2 spacings three are
-    // needed); after that the host reads the count after every pass.
     bool settled = false;
-    for (int r = 1; r < kStarPasses; r++) {
-        // recompute the listed vertices from the 2-rings in the previous mesh
-        // and in the stars found so far; a vertex whose star failed (it left
-        // its 2-ring) searches one ring deeper each pass
-        stars(t->s_list, t->s_aux + 1, 2 + r, t->star_deg[nxt], t->star_nbr[nxt], t->star_tdeg, t->star_tnbr);
-        hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 8, 1, 0);
-        const int rstamp = ++t->stamp;
-        hipLaunchKernelGGL(k_star_scatter, list_grid, dim3(256), 0, s, (const int*)t->s_list,
-                           (const int*)(t->s_aux + 1), (const int*)t->star_tdeg, (const int*)t->star_tnbr,
-                           t->star_deg[nxt], t->star_nbr[nxt], t->s_mark, rstamp, t->s_rlist, t->s_aux + 8,
-                           (long long*)(t->s_aux + 2), (long long*)(t->s_aux + 12));
-        check(false, r >= kStarDecidePasses);
-        MCL_HIP(hipGetLastError());
-        if (trace || r >= kStarSyncFrom || r == kStarPasses - 1) {
-            MCL_TRY(read_aux(r));
-            if (aux[1] == 0) {
-                long long sum_deg, sum_corner = 0;
-                memcpy(&sum_deg, aux + 2, sizeof(sum_deg));
-                if (m.ncorner) memcpy(&sum_corner, aux + 16, sizeof(sum_corner));
-                // each half-edge once: the degree sum is 2E; every corner fan chained
-                settled = sum_deg + sum_corner == (long long)m.nhe && aux[18] == 0 && aux[19] == 0;
-                if (trace && !settled)
-                    fprintf(stderr, "  [stars] degree sum %lld + corners %lld vs %d half-edges, corner failures %d\n",
-                            sum_deg, sum_corner, m.nhe, aux[18]);
-                break;
-            }
+    for (int r = kStarHeadPasses - 1; r < kStarPasses; r++) {
+        if (r >= kStarHeadPasses) {
+            star_retry(t, s, cur, nxt, r);
+            star_corner_fans(t, s, cur, nxt);
+            MCL_HIP(hipGetLastError());
+        }
+        MCL_TRY(read_aux(r));
+        if (aux[1] == 0) {
+            long long sum_deg, sum_corner = 0;
+            memcpy(&sum_deg, aux + 2, sizeof(sum_deg));
+            if (m.ncorner) memcpy(&sum_corner, aux + 16, sizeof(sum_corner));
+            // each half-edge once: the degree sum is 2E; every corner fan chained
+            settled = sum_deg + sum_corner == (long long)m.nhe && aux[18] == 0 && aux[19] == 0;
+            if (trace && !settled)
+                fprintf(stderr, "  [stars] degree sum %lld + corners %lld vs %d half-edges, corner failures %d\n",
+                        sum_deg, sum_corner, m.nhe, aux[18]);
+            break;
         }
     }
     if (!settled && trace && aux[1] > 0) {
@@ -3258,6 +3356,65 @@ extern "C" int mcl_tri_one_ring(mcl_tri_t* t, int K, int layout, void* d_ring, i
     return ov ? MCL_ERR_CAPACITY : MCL_OK;
 }
 
+// the CSR one-ring, asynchronous (cap >= nhe: every half-edge leaves one
+// vertex), from the stars when they are current, else from the half-edges
+static int enqueue_ring_csr(mcl_tri* t, hipStream_t s, int* off, int* nbrs, int cap) {
+    const int c = t->star_cur;
+    if (t->stars_valid)
+        hipLaunchKernelGGL(k_star_ring_deg, dim3(blocks_for(t->m.nv)), dim3(256), 0, s, t->m.nv,
+                           (const int*)t->star_deg[c], off);
+    else
+        hipLaunchKernelGGL(k_degree, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, off);
+    MCL_TRY(scan_exclusive(off, off, t->m.nv, t->scan_tmp, s));
+    if (t->stars_valid)
+        hipLaunchKernelGGL(k_star_ring_csr, dim3(blocks_for(t->m.nv)), dim3(256), 0, s, t->m.nv, t->m.ncorner,
+                           (const int*)t->star_deg[c], (const int*)t->star_nbr[c], (const int*)off, nbrs, cap);
+    else
+        hipLaunchKernelGGL(k_ring_csr, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, off, nbrs, cap);
+    MCL_HIP(hipGetLastError());
+    return MCL_OK;
+}
+
+extern "C" int mcl_tri_timestep(mcl_tri_t* t, const mcl_step_params_t* p, void* d_ring_off, void* d_ring_nbrs,
+                                int ring_cap, mcl_grid_t* g, float rad, void* d_frnn_off, void* d_frnn_nbrs,
+                                int frnn_cap, mcl_stats_t* st, void* stream) {
+    if (!t || !p || (d_ring_off && (!d_ring_nbrs || ring_cap < t->m.nhe)) || (g && !d_frnn_off)) return MCL_ERR_INPUT;
+    hipStream_t s = (hipStream_t)stream;
+    int rc = mcl_tri_step(t, p, st, stream);
+    if (rc != MCL_OK && rc != MCL_ERR_UNRESOLVED) return rc;
+    // the tail: mesh one-ring and grid FRNN of the new positions
+    if (!d_ring_off && !g) return rc;
+    if (!t->stars_valid) MCL_TRY(ensure_mesh(t, s));
+    auto tail = [&](hipStream_t cs) -> int {
+        if (d_ring_off) MCL_TRY(enqueue_ring_csr(t, cs, (int*)d_ring_off, (int*)d_ring_nbrs, ring_cap));
+        if (g) MCL_TRY(grid_enqueue_frnn(g, t->m.pos, rad, (int*)d_frnn_off, (int*)d_frnn_nbrs, frnn_cap, cs));
+        return MCL_OK;
+    };
+    if (t->graph && t->stars_valid) {
+        float radv = rad;
+        const void* key[8] = {d_ring_off, d_ring_nbrs, (const void*)(intptr_t)ring_cap, g,
+                              (const void*)(intptr_t)*(const int*)&radv, d_frnn_off, d_frnn_nbrs,
+                              (const void*)(intptr_t)frnn_cap};
+        if (memcmp(key, t->post_key, sizeof(key)) != 0) {
+            for (int k = 0; k < 2; k++) {
+                if (t->g_post[k]) (void)hipGraphExecDestroy(t->g_post[k]);
+                t->g_post[k] = nullptr;
+            }
+            memcpy(t->post_key, key, sizeof(key));
+        }
+        const int c = t->star_cur;
+        if (!t->g_post[c]) {
+            int trc = MCL_OK;
+            MCL_TRY(capture_graph(t, &t->g_post[c], [&](hipStream_t cs) { trc = tail(cs); }));
+            MCL_TRY(trc);
+        }
+        MCL_HIP(hipGraphLaunch(t->g_post[c], s));
+        return rc;
+    }
+    MCL_TRY(tail(s));
+    return rc;
+}
+
 extern "C" int mcl_tri_one_ring_csr(mcl_tri_t* t, void* d_offsets, void* d_nbrs, int cap, int* total, void* stream) {
     if (!t || !d_offsets) return MCL_ERR_INPUT;
     hipStream_t s = (hipStream_t)stream;
@@ -3349,6 +3506,11 @@ extern "C" int mcl_tri_set_option(mcl_tri_t* t, int option, long long value) {
         case MCL_OPT_STAR_GIVE_UP:
             if (value < 1 || value > 100) return MCL_ERR_INPUT;
             t->star_give_up = (int)value;
+            drop_graphs(t);
+            return MCL_OK;
+        case MCL_OPT_GRAPH:
+            if (value < 0 || value > 1) return MCL_ERR_INPUT;
+            t->graph = (int)value;
             return MCL_OK;
         default:
             return MCL_ERR_INPUT;

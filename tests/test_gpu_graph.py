@@ -1,0 +1,117 @@
+"""Whole timesteps replayed from hipGraphs (MCL_OPT_GRAPH) against the same
+timesteps launched eagerly: two device triangulations of the same points,
+stepped with the same motion, one with MCL_OPT_GRAPH = 1; after every step
+the positions, the triangulation, the CSR one-ring and the grid FRNN CSR must
+be identical, and the triangulation the exact Delaunay one (oracle).  Graph
+replays alternate with eager work on the other object, and both star buffer
+parities are replayed several times; a step that gives up on the stars (the
+flip-based fallback) must also agree."""
+import numpy as np
+import pytest
+
+import mcleap_amd as M
+from mcleap_amd.workloads import grid_shape
+from tests.conftest import assert_at_targets, uniform_points
+
+pytestmark = pytest.mark.gpu
+
+
+class _Run:
+    def __init__(self, H, graph, periodic, rad, give_up=None):
+        import torch
+        self.dt = M.DeviceTriangulation(H)
+        self.dt.set_option(M.OPT_STAR_MIN_VERTICES, 0)
+        self.dt.set_option(M.OPT_STAR_ENABLE, 1)
+        if give_up:
+            self.dt.set_option(M.OPT_STAR_GIVE_UP, give_up)
+        self.dt.set_option(M.OPT_GRAPH, int(graph))
+        nv = self.dt.m_pos_size
+        nc, cs = grid_shape(rad, "torus" if periodic else "box")
+        flags = M.GRID_MORTON | (M.GRID_PERIODIC if periodic else 0)
+        self.grid = M.GridCount2d(nv, (0.0, 0.0), (cs, cs), (nc, nc), flags)
+        self.ring = (torch.zeros(nv + 1, dtype=torch.int32, device="cuda"),
+                     torch.zeros(self.dt.m_he_size, dtype=torch.int32, device="cuda"))
+        self.frnn = (torch.zeros(nv + 1, dtype=torch.int32, device="cuda"),
+                     torch.zeros(nv * 64, dtype=torch.int32, device="cuda"))
+        self.rad = rad
+
+    def step(self, motion, sigma, amp, k):
+        return self.dt.timestep(motion, sigma, amp=amp, seed=7, step=k, ring=self.ring, grid=self.grid,
+                                rad=self.rad, frnn=self.frnn)
+
+    def state(self):
+        import torch
+        torch.cuda.synchronize()
+        pos, v, he, t = self.dt.download()
+        return {"pos": pos, "he": he, "t": t, "v": v,
+                "ring_off": self.ring[0].cpu().numpy(), "ring_nbr": self.ring[1].cpu().numpy(),
+                "frnn_off": self.frnn[0].cpu().numpy(),
+                "frnn_nbr": self.frnn[1].cpu().numpy()[: int(self.frnn[0][-1].item())],
+                "perm": self.grid.perm().cpu().numpy()}
+
+
+def _compare(O, a, b, tag, exact_mesh):
+    """positions and everything derived from them only (the grid FRNN) are
+    always equal; the triangulation is equal as a canonical triangle set, and
+    also array by array (half-edge numbering, ring order) while every step so
+    far took the star path: the flip-based step's parallel take-backs need not
+    number half-edges the same way twice"""
+    for key in ("pos", "frnn_off", "frnn_nbr", "perm"):
+        assert np.array_equal(a[key], b[key]), "%s: %s differs" % (tag, key)
+    assert np.array_equal(O.canonical_triangles(a["he"], a["t"]), O.canonical_triangles(b["he"], b["t"])), tag
+    if exact_mesh:
+        for key in ("he", "t", "v", "ring_off", "ring_nbr"):
+            assert np.array_equal(a[key], b[key]), "%s: %s differs" % (tag, key)
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_graph_timesteps_equal_eager(oracle, periodic):
+    O = oracle
+    n = 20000 if periodic else 10000
+    H = M.HostTriangulation(periodic=periodic)
+    H.addDelaunayPoints(uniform_points(n, 44))
+    rad = float(np.float32(2.0 / np.sqrt(n)))
+    eager, graph = _Run(H, False, periodic, rad), _Run(H, True, periodic, rad)
+    motion = (M.MOTION_SHEAR, 2e-4 * np.sqrt(2 ** 20 / n), 1e-3) if periodic else (M.MOTION_GAUSS, 1e-3, 0.0)
+    star_steps = 0
+    all_star = True
+    for k in range(8):
+        se = eager.step(motion[0], motion[1], motion[2], k)
+        sg = graph.step(motion[0], motion[1], motion[2], k)
+        assert se["fallbacks"] == sg["fallbacks"], k
+        star = sg["fallbacks"] == 0 and sg["rounds"] == 0
+        star_steps += star
+        all_star &= star
+        a, b = eager.state(), graph.state()
+        _compare(O, a, b, "step %d" % k, all_star)
+        assert_at_targets(graph.dt, sg, "graph step %d" % k)
+        if k % 3 == 2:
+            rc, msg = O.check_mesh(b["pos"], b["v"], b["he"], b["t"], periodic=periodic, require_ccw=True)
+            assert rc == 0, msg
+            assert O.count_non_delaunay(b["pos"], b["he"], periodic=periodic) == 0
+    assert star_steps >= (6 if periodic else 4), star_steps
+
+
+def test_graph_fallback_equals_eager(oracle):
+    """a star step that gives up (the first check lists more than 1% of the
+    vertices) falls back to the flip-based step in both modes, with equal
+    results"""
+    n = 20000
+    H = M.HostTriangulation(periodic=True)
+    H.addDelaunayPoints(uniform_points(n, 45))
+    rad = float(np.float32(2.0 / np.sqrt(n)))
+    eager, graph = _Run(H, False, True, rad, give_up=1), _Run(H, True, True, rad, give_up=1)
+    fallbacks = 0
+    for k in range(3):
+        se = eager.step(M.MOTION_SHEAR, 1e-3, 1e-3, k)
+        sg = graph.step(M.MOTION_SHEAR, 1e-3, 1e-3, k)
+        # (after a fallback the star step pauses for a step, in both modes)
+        assert se["fallbacks"] == sg["fallbacks"]
+        fallbacks += sg["fallbacks"]
+        a, b = eager.state(), graph.state()
+        # the flip-based step's parallel take-backs may differ in their
+        # intermediate meshes; the result is the unique Delaunay triangulation
+        assert np.array_equal(a["pos"], b["pos"])
+        assert np.array_equal(oracle.canonical_triangles(a["he"], a["t"]), oracle.canonical_triangles(b["he"], b["t"]))
+        assert_at_targets(graph.dt, sg, "fallback step %d" % k)
+    assert fallbacks >= 1
