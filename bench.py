@@ -281,6 +281,7 @@ def fail(msg):
 
 def run_workload(a, M, torch, workload, steps, warmup, plain_steps, cpu_seconds, pred_check):
     """one workload: build, warm up, time `steps` steps, check, compare"""
+    from mcleap_amd import tuning
     from mcleap_amd.workloads import CONFIGS, ensemble_points, grid_shape, initial_points
 
     cfg = dict(CONFIGS[workload])
@@ -325,6 +326,7 @@ def run_workload(a, M, torch, workload, steps, warmup, plain_steps, cpu_seconds,
     ring_nbr = torch.zeros(dt.m_he_size, dtype=torch.int32, device=dev)
     radius = cfg["radius"]
     grid = bgrid = None
+    cells = 1
     pos_t = dt.pos_tensor()
     if batch:
         # one grid per instance, r_i = 2/sqrt(n_i), sorted and queried together
@@ -335,7 +337,9 @@ def run_workload(a, M, torch, workload, steps, warmup, plain_steps, cpu_seconds,
         bgrid.build(pos_t)
         total = bgrid.frnn_csr(f_off)
     elif radius:
-        nc, cs = grid_shape(radius, cfg["domain"])
+        # cells of the radius, or of a fraction of it when the tuning table says so
+        cells = tuning.cells_per_radius("torus" if periodic else "box", nv)
+        nc, cs = grid_shape(radius / cells, cfg["domain"])
         flags = M.GRID_MORTON | (M.GRID_PERIODIC if periodic else 0)
         grid = M.GridCount2d(nv, (0.0, 0.0), (cs, cs), (nc, nc), flags)
         f_off = torch.zeros(nv + 1, dtype=torch.int32, device=dev)
@@ -503,6 +507,10 @@ def run_workload(a, M, torch, workload, steps, warmup, plain_steps, cpu_seconds,
                    "amp": cfg["amp"], "frnn_radius": radius, "variant": a.variant,
                    "build": a.build, "build_s": round(build_s, 3),
                    "predicates": "exact: fp32 filter -> fp64 filter -> expansion arithmetic",
+                   "tuning": {"table": os.path.relpath(tuning.table_path(), HERE) if tuning.table_path() else None,
+                              "options": dt.options(),
+                              "grid_block": grid.block() if grid is not None else None,
+                              "frnn_cells_per_radius": cells if grid is not None else None},
                    "half_edges": "star path: rebuilt from the stars only when read (not inside the timed steps)"
                    if periodic else "maintained by every step"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",

@@ -403,12 +403,43 @@ int mcl_grid_mean_occupancy(mcl_grid_t* g, float* out, void* stream);
  * MCL_OPT_GRAPH: 1 = the star step's head (move, first pass, checks and the
  *   retries before the first host read) and mcl_tri_timestep's tail are
  *   captured once as hipGraphs and replayed every step (the results are those
- *   of the eager launches); 0 (default) = eager launches. */
+ *   of the eager launches); 0 (default) = eager launches.
+ * Round-loop parameters of the flip-based step (the reference hard-codes its
+ * launch parameters instead, e.g. blocksize = 128, Device_Triangulation.h:151):
+ * MCL_OPT_TAIL_THRESHOLD: Lawson queue length below which one workgroup
+ *   finishes the remaining rounds in one launch (default 8192).
+ * MCL_OPT_REVERT_TAIL / MCL_OPT_RELOC_TAIL: the same for take-back passes and
+ *   relocation rounds (default 1024 each).
+ * MCL_OPT_BIG_GRID_MIN: queue length of big-exact-only Lawson cases from
+ *   which they run on 256 workgroups before the tail (default 2048).
+ * MCL_OPT_STAR_HEAD_PASSES: star passes enqueued (and, with MCL_OPT_GRAPH,
+ *   captured) before the first host read (1..12, default 4).
+ * Defaults are overridden at creation by the autotuner's table
+ * (mcl_tuning_path). */
 #define MCL_OPT_STAR_MIN_VERTICES 1
 #define MCL_OPT_STAR_ENABLE 2
 #define MCL_OPT_STAR_GIVE_UP 3
 #define MCL_OPT_GRAPH 4
+#define MCL_OPT_TAIL_THRESHOLD 5
+#define MCL_OPT_REVERT_TAIL 6
+#define MCL_OPT_RELOC_TAIL 7
+#define MCL_OPT_BIG_GRID_MIN 8
+#define MCL_OPT_STAR_HEAD_PASSES 9
 int mcl_tri_set_option(mcl_tri_t* t, int option, long long value);
+int mcl_tri_get_option(const mcl_tri_t* t, int option, long long* value);
+/* GridCount2d's launch parameter (the reference's public blocksize = 64,
+ * gridCount2d.h:35): MCL_GRID_OPT_BLOCK, threads per block of the grid's
+ * sort and query kernels, 64, 128 or 256 (default 256, or the tuning table's). */
+#define MCL_GRID_OPT_BLOCK 1
+int mcl_grid_set_option(mcl_grid_t* g, int option, long long value);
+int mcl_grid_get_option(const mcl_grid_t* g, int option, long long* value);
+/* The autotuner's table in use: the file MCL_TUNING names (MCL_TUNING=off:
+ * none), else tuning.json next to libmcleap_hip.so, written by
+ * mcleap_amd/autotune.py; "" when no table was loaded.  Each entry (domain
+ * torus / box / ensemble, log2 of the vertex count) sets the options above
+ * and grid_block for meshes and grids of its class and size (within a factor
+ * of 4). */
+const char* mcl_tuning_path(void);
 
 /* ------------------------------------------------------- kernel timers */
 /* GPU time of the main kernels, from HIP event pairs recorded on the launch

@@ -35,6 +35,15 @@ OPT_STAR_MIN_VERTICES = 1
 OPT_STAR_ENABLE = 2
 OPT_STAR_GIVE_UP = 3
 OPT_GRAPH = 4
+OPT_TAIL_THRESHOLD = 5
+OPT_REVERT_TAIL = 6
+OPT_RELOC_TAIL = 7
+OPT_BIG_GRID_MIN = 8
+OPT_STAR_HEAD_PASSES = 9
+GRID_OPT_BLOCK = 1
+# the tuning table's parameter names and their option ids
+TRI_OPTIONS = {"star_min_vertices": 1, "star_enable": 2, "star_give_up": 3, "graph": 4, "tail_threshold": 5,
+               "revert_tail": 6, "reloc_tail": 7, "big_grid_min": 8, "star_head_passes": 9}
 
 
 class MclError(RuntimeError):
@@ -128,6 +137,10 @@ SIGNATURES = {
     "mcl_grid_sorted_ptr": [_p],
     "mcl_grid_frnn_csr_noalloc": [_p, _i, _f, _f, _f, _f, _i, _i, _i, _f, _p, _p, _p, _i, _p, _p],
     "mcl_tri_set_option": [_p, _i, ctypes.c_longlong],
+    "mcl_tri_get_option": [_p, _i, ctypes.POINTER(ctypes.c_longlong)],
+    "mcl_grid_set_option": [_p, _i, ctypes.c_longlong],
+    "mcl_grid_get_option": [_p, _i, ctypes.POINTER(ctypes.c_longlong)],
+    "mcl_tuning_path": [],
     "mcl_tri_kernel_timer": [_p, _i],
     "mcl_tri_kernel_timer_read": [_p, _i, _p, _p],
     "mcl_grid_kernel_timer": [_p, _i],
@@ -156,7 +169,7 @@ SIGNATURES = {
     "mcl_version": [],
     "mcl_device_info": [_p, _i, _p],
 }
-RESTYPES = {"mcl_tri_pos_ptr": _p, "mcl_tri_target_ptr": _p, "mcl_grid_sorted_ptr": _p, "mcl_bgrid_sorted_ptr": _p,
+RESTYPES = {"mcl_tuning_path": ctypes.c_char_p, "mcl_tri_pos_ptr": _p, "mcl_tri_target_ptr": _p, "mcl_grid_sorted_ptr": _p, "mcl_bgrid_sorted_ptr": _p,
             "mcl_version": ctypes.c_char_p}
 
 

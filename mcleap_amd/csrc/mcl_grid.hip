@@ -38,6 +38,8 @@ struct GridParams {
     float minx, miny, csx, csy;
     int nx, ny, tot, nkeys;
     int morton, periodic;
+    int block = 256;  // threads per block of this grid's launches (64, 128 or 256)
+    int reach = 1;    // cells searched on each side of a point's cell (set per query)
 };
 
 __host__ __device__ __forceinline__ uint32_t spread16(uint32_t x) {
@@ -130,8 +132,9 @@ __global__ void __launch_bounds__(kT) k_frnn(GridParams g, const float2* __restr
         const float2 pi = pos[i];
         const int2 h = cell_of(g, pi);
         int c = 0, w = kPass ? off[i] : 0;
-        for (int ddx = -1; ddx <= 1; ddx++) {
-            for (int ddy = -1; ddy <= 1; ddy++) {
+        const int R = g.reach;
+        for (int ddx = -R; ddx <= R; ddx++) {
+            for (int ddy = -R; ddy <= R; ddy++) {
                 int x = h.x + ddx, y = h.y + ddy;
                 if (g.periodic) {
                     x = (x + g.nx) % g.nx;
@@ -169,8 +172,8 @@ __global__ void k_occupancy(const int* __restrict__ start, int nkeys, unsigned l
     if (nonempty) atomicAdd(out, nonempty);
 }
 
-int blocks(long long n) {
-    long long b = (n + kT - 1) / kT;
+int blocks(long long n, int block = kT) {
+    long long b = (n + block - 1) / block;
     return (int)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
 }
 
@@ -187,14 +190,14 @@ struct SortBufs {
 // cell (into perm), then the gather of the sorted positions and ind[i] = the
 // slot of point i.  Asynchronous on s; uses only the given buffers.
 int grid_sort(const GridParams& g, const float2* pos, const float2* vel, int n, const SortBufs& b, hipStream_t s) {
-    const int nk = g.nkeys;
+    const int nk = g.nkeys, B = g.block;
     hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, b.start, nk + 1);
     hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, b.fill, nk + 1);
-    hipLaunchKernelGGL(k_key_count, dim3(blocks(n)), dim3(kT), 0, s, g, pos, n, b.key, b.start);
+    hipLaunchKernelGGL(k_key_count, dim3(blocks(n, B)), dim3(B), 0, s, g, pos, n, b.key, b.start);
     MCL_TRY(scan_exclusive(b.start, b.start, nk, b.scan_tmp, s));
-    hipLaunchKernelGGL(k_slot, dim3(blocks(n)), dim3(kT), 0, s, b.key, n, b.start, b.fill, b.ind);
-    hipLaunchKernelGGL(k_cell_rank, dim3(blocks(n)), dim3(kT), 0, s, b.key, n, b.start, (const int*)b.ind, b.perm);
-    hipLaunchKernelGGL(k_gather, dim3(blocks(n)), dim3(kT), 0, s, b.perm, n, pos, vel, b.spos, b.svel, b.ind);
+    hipLaunchKernelGGL(k_slot, dim3(blocks(n, B)), dim3(B), 0, s, b.key, n, b.start, b.fill, b.ind);
+    hipLaunchKernelGGL(k_cell_rank, dim3(blocks(n, B)), dim3(B), 0, s, b.key, n, b.start, (const int*)b.ind, b.perm);
+    hipLaunchKernelGGL(k_gather, dim3(blocks(n, B)), dim3(B), 0, s, b.perm, n, pos, vel, b.spos, b.svel, b.ind);
     MCL_HIP(hipGetLastError());
     return MCL_OK;
 }
@@ -202,18 +205,29 @@ int grid_sort(const GridParams& g, const float2* pos, const float2* vel, int n, 
 // the count pass, its scan and (with d_nbrs) the fill pass of the CSR query
 // over sorted positions spos; total == NULL: no host read, writes past cap
 // dropped; else *total = the pair count after a synchronisation
-int frnn_csr(const GridParams& g, const float2* spos, int n, const int* start, float rad, int* off, int* nbrs,
+int frnn_csr(const GridParams& g0, const float2* spos, int n, const int* start, float rad, int* off, int* nbrs,
              int cap, long long* total, int* scan_tmp, hipStream_t s, KTimer* tc, KTimer* tf) {
     const float rad2 = rad * rad;
+    // cells searched on each side: one for the reference's row-major grid
+    // (gridCount2d.h:272-282, whatever the radius); with the Morton or
+    // periodic layouts as many as a radius longer than a cell needs
+    GridParams g = g0;
+    g.reach = 1;
+    if (g.morton || g.periodic) {
+        const double c = std::min(g.csx, g.csy);
+        g.reach = std::max(1, (int)std::ceil((double)rad / c * (1.0 - 1e-6)));
+        if (g.periodic && 2 * g.reach + 1 > std::min(g.nx, g.ny)) return MCL_ERR_INPUT;
+    }
+    const int B = g.block;
     if (tc) tc->mark(s);
-    hipLaunchKernelGGL(k_frnn<0>, dim3(blocks(n)), dim3(kT), 0, s, g, spos, n, start, rad2, off, (const int*)nullptr,
+    hipLaunchKernelGGL(k_frnn<0>, dim3(blocks(n, B)), dim3(B), 0, s, g, spos, n, start, rad2, off, (const int*)nullptr,
                        (int*)nullptr, 0);
     if (tc) tc->mark(s);
     MCL_TRY(scan_exclusive(off, off, n, scan_tmp, s));
     if (!total) {
         if (tf) tf->mark(s);
         if (nbrs && cap > 0)
-            hipLaunchKernelGGL(k_frnn<1>, dim3(blocks(n)), dim3(kT), 0, s, g, spos, n, start, rad2, (int*)nullptr,
+            hipLaunchKernelGGL(k_frnn<1>, dim3(blocks(n, B)), dim3(B), 0, s, g, spos, n, start, rad2, (int*)nullptr,
                                (const int*)off, nbrs, cap);
         if (tf) tf->mark(s);
         MCL_HIP(hipGetLastError());
@@ -224,7 +238,7 @@ int frnn_csr(const GridParams& g, const float2* spos, int n, const int* start, f
     MCL_HIP(hipStreamSynchronize(s));
     *total = tot;
     if (nbrs && cap > 0) {
-        hipLaunchKernelGGL(k_frnn<1>, dim3(blocks(n)), dim3(kT), 0, s, g, spos, n, start, rad2, (int*)nullptr,
+        hipLaunchKernelGGL(k_frnn<1>, dim3(blocks(n, B)), dim3(B), 0, s, g, spos, n, start, rad2, (int*)nullptr,
                            (const int*)off, nbrs, cap);
         MCL_HIP(hipGetLastError());
         MCL_HIP(hipStreamSynchronize(s));
@@ -298,6 +312,7 @@ extern "C" int mcl_grid_create(int nump, float minx, float miny, float csx, floa
     if (!ok) { grid_free(G); delete G; return MCL_ERR_HIP; }
     hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, 0, G->start, nk + 1);
     if (hipDeviceSynchronize() != hipSuccess) { grid_free(G); delete G; return MCL_ERR_HIP; }
+    tune_grid(G, G->g.periodic, nump);
     *out = G;
     return MCL_OK;
 }
@@ -450,6 +465,25 @@ extern "C" int mcl_grid_frnn_csr_noalloc(const void* d_pos, int nump, float minx
     if (d_perm) MCL_HIP(hipMemcpyAsync(d_perm, b.perm, sizeof(int) * nump, hipMemcpyDeviceToDevice, s));
     return frnn_csr(g, b.spos, nump, b.start, rad, (int*)d_offsets, (int*)d_nbrs, cap, total, b.scan_tmp, s, nullptr,
                     nullptr);
+}
+
+extern "C" int mcl_grid_set_option(mcl_grid_t* G, int option, long long value) {
+    if (!G) return MCL_ERR_INPUT;
+    if (option == MCL_GRID_OPT_BLOCK) {
+        if (value != 64 && value != 128 && value != 256) return MCL_ERR_INPUT;
+        G->g.block = (int)value;
+        return MCL_OK;
+    }
+    return MCL_ERR_INPUT;
+}
+
+extern "C" int mcl_grid_get_option(const mcl_grid_t* G, int option, long long* value) {
+    if (!G || !value) return MCL_ERR_INPUT;
+    if (option == MCL_GRID_OPT_BLOCK) {
+        *value = G->g.block;
+        return MCL_OK;
+    }
+    return MCL_ERR_INPUT;
 }
 
 extern "C" int mcl_grid_mean_occupancy(mcl_grid_t* G, float* out, void* stream) {

@@ -85,6 +85,9 @@ struct DMesh {
 // the flip rounds' round and stamp counters of a triangulation (read, or
 // set when set is true); saved in state files so that a resumed run repeats
 int tri_counters(mcl_tri_t* t, int* round, int* stamp, bool set);
+// the autotuner's table (mcl_tune.hip) applied to a new triangulation / grid
+void tune_tri(mcl_tri_t* t, int periodic, int ncorner, int nv);
+void tune_grid(mcl_grid_t* g, int periodic, int nump);
 // grid sort of pos + the asynchronous CSR FRNN query, no host reads
 int grid_enqueue_frnn(mcl_grid_t* g, const float2* pos, float rad, int* off, int* nbrs, int cap, hipStream_t s);
 

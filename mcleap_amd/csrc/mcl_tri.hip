@@ -1623,7 +1623,7 @@ __global__ void __launch_bounds__(kThreads) k_frnn_graph(DMesh m, float rr, int 
 // =========================================================================
 constexpr int kPathProbe = 64;  // first interval between measurements of the slower step path
 constexpr int kBigBlocks = 256;   // workgroups of the rounds that run the big exact stage
-constexpr int kBigGridMin = 2048;  // queue length from which those rounds run on kBigBlocks workgroups
+
 
 struct mcl_tri {
     DMesh m;
@@ -1692,6 +1692,7 @@ struct mcl_tri {
     hipGraphExec_t g_post[2] = {nullptr, nullptr};  // timestep tail (ring CSR + grid FRNN) per star_cur
     const void* post_key[8] = {};           // the buffers g_post was captured with
     bool capturing = false;                 // kernel timers are not recorded under capture
+    bool captured = false;                  // a graph was captured during the current step
     hipStream_t cap_stream = nullptr;       // stream the graphs are captured on (the caller's may be the null stream)
     // one ring
     int* deg = nullptr;
@@ -1703,6 +1704,8 @@ struct mcl_tri {
     int tail_threshold = 8192;  // queue length below which one workgroup finishes the rounds
     int revert_tail = 1024;     // bad-triangle count below which one workgroup finishes the take-backs
     int reloc_tail = 1024;      // relocation list length below which one workgroup finishes the rounds
+    int big_grid_min = 2048;    // big-exact queue length from which those rounds run on kBigBlocks workgroups
+    int star_head_passes = 4;   // star passes enqueued (and captured) before the first host read
     int round = 1;         // global round counter (claims, edge stamps)
     int stamp = 1;         // global stamp for vertex / triangle marks
     int grid_blocks = 1024;
@@ -1907,6 +1910,7 @@ extern "C" int mcl_tri_create(const float* pos, int npos, const int* v2, int nv,
     if (rc != MCL_OK) return rc;
     rc = upload_aos(t, pos, v2, he4, t1);
     if (rc != MCL_OK) { tri_free(t); delete t; return rc; }
+    tune_tri(t, periodic, ncorner, nv);
     *out = t;
     return MCL_OK;
 }
@@ -2342,7 +2346,7 @@ static int run_rounds(mcl_tri* t, bool all, int variant, int max_rounds, hipStre
     // single-workgroup tail takes the rest
     for (; r < cap; r++) {
         MCL_TRY(read_counts(t, s));
-        if (t->h_cnt[kCnCand + t->round % 3] < kBigGridMin) break;
+        if (t->h_cnt[kCnCand + t->round % 3] < t->big_grid_min) break;
         const int R = t->round;
         const int stat = kKind == 0 ? kStFlips : kStUntangleFlips;
         hipLaunchKernelGGL((k_round_test<kKind, false, true, true>), dim3(kBigBlocks), dim3(kThreads), 0, s, m,
@@ -2856,7 +2860,8 @@ static int ensure_mesh(mcl_tri* t, hipStream_t s) {
 // Returns MCL_OK with *done = false when the stars could not be settled; the
 // positions are then back at the start of the step and the mesh untouched.
 //
-// The step's first kStarHeadPasses star passes (the move, the first pass over
+// The step's first star passes (MCL_OPT_STAR_HEAD_PASSES of them, default 4:
+// the move, the first pass over
 // every vertex with its consistency check and give-up test, and retries of the
 // listed vertices) are enqueued without any host read: every list length
 // stays on the device, kernels over an empty list return at once, and the
@@ -2865,7 +2870,6 @@ static int ensure_mesh(mcl_tri* t, hipStream_t s) {
 // (its parameters copied to the device first).  The host then reads the
 // counters once; the rare step that has not settled continues eagerly, one
 // read per pass.
-constexpr int kStarHeadPasses = kStarSyncFrom + 1;
 
 namespace {
 // launch geometry of the star kernels
@@ -2979,7 +2983,7 @@ void star_head(mcl_tri* t, hipStream_t s, int cur, int nxt, const mcl_step_param
     star_check(t, s, nxt, true, false, 1);
     hipLaunchKernelGGL(k_star_give_up, dim3(1), dim3(64), 0, s, t->s_aux,
                        (int)std::min<long long>((long long)t->star_give_up * m.nv / 100, m.nv));
-    for (int r = 1; r < kStarHeadPasses; r++) star_retry(t, s, cur, nxt, r);
+    for (int r = 1; r < t->star_head_passes; r++) star_retry(t, s, cur, nxt, r);
     star_corner_fans(t, s, cur, nxt);
 }
 }  // namespace
@@ -3001,6 +3005,7 @@ static int capture_graph(mcl_tri* t, hipGraphExec_t* out, F&& enqueue) {
     hipGraph_t g = nullptr;
     MCL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     t->capturing = true;
+    t->captured = true;
     enqueue(s);
     t->capturing = false;
     const hipError_t e = hipStreamEndCapture(s, &g);
@@ -3052,8 +3057,8 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         return MCL_OK;
     };
     bool settled = false;
-    for (int r = kStarHeadPasses - 1; r < kStarPasses; r++) {
-        if (r >= kStarHeadPasses) {
+    for (int r = t->star_head_passes - 1; r < kStarPasses; r++) {
+        if (r >= t->star_head_passes) {
             star_retry(t, s, cur, nxt, r);
             star_corner_fans(t, s, cur, nxt);
             MCL_HIP(hipGetLastError());
@@ -3172,6 +3177,7 @@ static int kinetic_step(mcl_tri* t, const mcl_step_params_t* p, const float2* di
             }
         }
         MCL_HIP(hipStreamSynchronize(s));
+        t->captured = false;
         auto t0 = std::chrono::steady_clock::now();
         int rc;
         double flip_ms = -1.0;
@@ -3185,6 +3191,7 @@ static int kinetic_step(mcl_tri* t, const mcl_step_params_t* p, const float2* di
         if (rc != MCL_OK) return rc;
         MCL_HIP(hipStreamSynchronize(s));
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (t->captured) return MCL_OK;  // a step that captured a graph is not a measurement of its path
         if (probe) {  // the probed path stays slower: probe half as often
             const bool keeps = ms > t->path_ms[1 - path];
             t->path_probe = keeps ? std::min(2 * t->path_probe, 4096) : kPathProbe;
@@ -3302,6 +3309,7 @@ static int build_impl(const void* d_xy, const int* counts, int ninst, int period
     if (target) (void)hipFree(target);
     if (offc) (void)hipFree(offc);
     if (rc != MCL_OK) { tri_free(t); delete t; return rc; }
+    tune_tri(t, periodic, (int)base, t->m.nv);
     *out = t;
     return MCL_OK;
 }
@@ -3512,8 +3520,45 @@ extern "C" int mcl_tri_set_option(mcl_tri_t* t, int option, long long value) {
             if (value < 0 || value > 1) return MCL_ERR_INPUT;
             t->graph = (int)value;
             return MCL_OK;
+        case MCL_OPT_TAIL_THRESHOLD:
+            if (value < 0 || value > (1 << 24)) return MCL_ERR_INPUT;
+            t->tail_threshold = (int)value;
+            return MCL_OK;
+        case MCL_OPT_REVERT_TAIL:
+            if (value < 0 || value > (1 << 24)) return MCL_ERR_INPUT;
+            t->revert_tail = (int)value;
+            return MCL_OK;
+        case MCL_OPT_RELOC_TAIL:
+            if (value < 0 || value > (1 << 24)) return MCL_ERR_INPUT;
+            t->reloc_tail = (int)value;
+            return MCL_OK;
+        case MCL_OPT_BIG_GRID_MIN:
+            if (value < 1 || value > (1 << 30)) return MCL_ERR_INPUT;
+            t->big_grid_min = (int)value;
+            return MCL_OK;
+        case MCL_OPT_STAR_HEAD_PASSES:
+            if (value < 1 || value > kStarPasses) return MCL_ERR_INPUT;
+            t->star_head_passes = (int)value;
+            drop_graphs(t);
+            return MCL_OK;
         default:
             return MCL_ERR_INPUT;
+    }
+}
+
+extern "C" int mcl_tri_get_option(const mcl_tri_t* t, int option, long long* value) {
+    if (!t || !value) return MCL_ERR_INPUT;
+    switch (option) {
+        case MCL_OPT_STAR_MIN_VERTICES: *value = t->star_min_vertices; return MCL_OK;
+        case MCL_OPT_STAR_ENABLE: *value = t->star_enable; return MCL_OK;
+        case MCL_OPT_STAR_GIVE_UP: *value = t->star_give_up; return MCL_OK;
+        case MCL_OPT_GRAPH: *value = t->graph; return MCL_OK;
+        case MCL_OPT_TAIL_THRESHOLD: *value = t->tail_threshold; return MCL_OK;
+        case MCL_OPT_REVERT_TAIL: *value = t->revert_tail; return MCL_OK;
+        case MCL_OPT_RELOC_TAIL: *value = t->reloc_tail; return MCL_OK;
+        case MCL_OPT_BIG_GRID_MIN: *value = t->big_grid_min; return MCL_OK;
+        case MCL_OPT_STAR_HEAD_PASSES: *value = t->star_head_passes; return MCL_OK;
+        default: return MCL_ERR_INPUT;
     }
 }
 

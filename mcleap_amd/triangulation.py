@@ -443,6 +443,15 @@ class DeviceTriangulation:
 
     untangle2 = untangle  # Device_Triangulation.h:461-495
 
+    def get_option(self, option):
+        v = ctypes.c_longlong()
+        check(lib().mcl_tri_get_option(self._h, int(option), ctypes.byref(v)), "mcl_tri_get_option")
+        return v.value
+
+    def options(self):
+        """every tunable option by its tuning-table name"""
+        return {name: self.get_option(o) for name, o in _lib.TRI_OPTIONS.items()}
+
     def move_vertices(self, ids, d, mode):
         """mcl_tri_move_vertices: ids (host ints) move by d (host (n, 2)
         float32) with mode MOVE_*; returns the status (0 when every listed
@@ -680,6 +689,15 @@ class GridCount2d:
         j = torch.cat([j, torch.arange(n, device="cuda")])
         d = pos[j] - pos[i]
         f(i, j, d, torch.sqrt((d.double() ** 2).sum(1)))
+
+    def set_block(self, block):
+        """threads per block of this grid's sort and query kernels (64, 128, 256)"""
+        check(lib().mcl_grid_set_option(self._g, _lib.GRID_OPT_BLOCK, int(block)), "mcl_grid_set_option")
+
+    def block(self):
+        v = ctypes.c_longlong()
+        check(lib().mcl_grid_get_option(self._g, _lib.GRID_OPT_BLOCK, ctypes.byref(v)), "mcl_grid_get_option")
+        return v.value
 
     def build(self, src_pos):
         """counting sort of src_pos into the grid's own sorted copy (src untouched, async)"""

@@ -115,3 +115,41 @@ def test_graph_fallback_equals_eager(oracle):
         assert np.array_equal(oracle.canonical_triangles(a["he"], a["t"]), oracle.canonical_triangles(b["he"], b["t"]))
         assert_at_targets(graph.dt, sg, "fallback step %d" % k)
     assert fallbacks >= 1
+
+
+def test_tuning_table_applied_at_create(tmp_path):
+    """an entry of the autotuner's table sets the options of a triangulation
+    of its class and size at creation (and grid_block of a grid), and leaves
+    other classes at the defaults"""
+    import json
+    import os
+    import subprocess
+    import sys
+    table = {"entries": [{"domain": "torus", "log2_nv": 14, "graph": 1, "star_head_passes": 3, "grid_block": 128},
+                         {"domain": "box", "log2_nv": 14, "tail_threshold": 4096, "reloc_tail": 256}]}
+    path = tmp_path / "t.json"
+    path.write_text(json.dumps(table))
+    code = r"""
+import json, numpy as np, mcleap_amd as M
+from tests.conftest import uniform_points
+out = {}
+for periodic in (True, False):
+    H = M.HostTriangulation(periodic=periodic)
+    H.addDelaunayPoints(uniform_points(16000, 3))
+    dt = M.DeviceTriangulation(H)
+    out["torus" if periodic else "box"] = dt.options()
+g = M.GridCount2d(16000, (0.0, 0.0), (0.01, 0.01), (100, 100), M.GRID_MORTON | M.GRID_PERIODIC)
+out["grid_block_torus"] = g.block()
+g2 = M.GridCount2d(16000, (0.0, 0.0), (0.01, 0.01), (100, 100), M.GRID_MORTON)
+out["grid_block_box"] = g2.block()
+print(json.dumps(out))
+"""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=root,
+                       env=dict(os.environ, MCL_TUNING=str(path)), timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["torus"]["graph"] == 1 and out["torus"]["star_head_passes"] == 3
+    assert out["torus"]["tail_threshold"] == 8192
+    assert out["box"]["tail_threshold"] == 4096 and out["box"]["reloc_tail"] == 256 and out["box"]["graph"] == 0
+    assert out["grid_block_torus"] == 128 and out["grid_block_box"] == 256

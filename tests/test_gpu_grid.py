@@ -339,3 +339,25 @@ def test_c3_density_frnn(oracle):
     assert np.array_equal(got, _pairs_oracle(counts, nbr))
     bc, bn = O.brute_frnn(pts, float(r), cap=1536)
     assert np.array_equal(got, _pairs_oracle(bc, bn))
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("block", [64, 256])
+def test_half_radius_cells_equal_brute_force(oracle, periodic, block):
+    """cells of half the radius (the tuner's frnn_cells_per_radius = 2: a
+    5 x 5 search) and any block size give exactly the brute-force pairs"""
+    import torch
+    from mcleap_amd.workloads import grid_shape
+    O = oracle
+    rng = np.random.default_rng(23)
+    xy = rng.random((20000, 2)).astype(np.float32)
+    r = 0.015
+    nc, cs = grid_shape(r / 2, "torus" if periodic else "box")
+    g = M.GridCount2d(len(xy), (0.0, 0.0), (cs, cs), (nc, nc), M.GRID_MORTON | (M.GRID_PERIODIC if periodic else 0))
+    g.set_block(block)
+    assert g.block() == block
+    pos = torch.from_numpy(xy.copy()).cuda()
+    g.update(pos)
+    got = _pairs_gpu(g, pos, r)
+    bc, bn = O.brute_frnn(xy, r, periodic=periodic, cap=128)
+    assert np.array_equal(got, _pairs_oracle(bc, bn))

@@ -194,3 +194,29 @@ def test_move_flipflop_degenerate_link(oracle):
     assert H.moveFlipflopDelaunay(w, mid - H.m_pos[w])
     assert np.array_equal(H.m_pos[w], mid)
     _dt_equal(O, H, "after the on-edge insertion")
+
+
+def test_tuning_table_loaded_from_env(tmp_path):
+    """the autotuner's table is read at load (MCL_TUNING): a well-formed file
+    is reported by mcl_tuning_path, a missing one or MCL_TUNING=off is not,
+    and the Python lookup picks the nearest entry of the same domain"""
+    import json
+    import subprocess
+    import sys
+    from mcleap_amd import tuning as T
+    table = {"entries": [{"domain": "torus", "log2_nv": 20, "graph": 1, "star_head_passes": 3, "grid_block": 128,
+                          "frnn_cells_per_radius": 2},
+                         {"domain": "box", "log2_nv": 14, "tail_threshold": 4096}]}
+    path = tmp_path / "t.json"
+    path.write_text(json.dumps(table))
+    code = "import mcleap_amd as M; print(M.lib().mcl_tuning_path().decode())"
+    for env, want in ((str(path), str(path)), ("off", ""), (str(tmp_path / "none.json"), "")):
+        out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT,
+                             env=dict(os.environ, MCL_TUNING=env), timeout=120)
+        assert out.returncode == 0, out.stderr
+        assert out.stdout.strip() == want, (env, out.stdout)
+    assert T.lookup("torus", 1 << 20, str(path))["star_head_passes"] == 3
+    assert T.lookup("torus", 1 << 22, str(path)) is not None and T.lookup("torus", 1 << 23, str(path)) is None
+    assert T.cells_per_radius("torus", 1 << 20, str(path)) == 2
+    assert T.cells_per_radius("box", 16000, str(path)) == 1
+    assert T.lookup("ensemble", 1 << 20, str(path)) is None
