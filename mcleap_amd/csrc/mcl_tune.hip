@@ -66,6 +66,10 @@ bool read_entry(const std::string& s, size_t& i, Entry* e) {
         if (i < s.size() && s[i] == '"') {
             if (!read_string(s, i, &sval)) return false;
             if (key == "domain") e->domain = sval == "torus" ? 1 : (sval == "ensemble" ? 2 : (sval == "box" ? 0 : -1));
+        } else if (s.compare(i, 4, "true") == 0 || s.compare(i, 4, "null") == 0) {
+            i += 4;  // record fields such as tuned_set_kept; no parameter is boolean
+        } else if (s.compare(i, 5, "false") == 0) {
+            i += 5;
         } else {
             char* end = nullptr;
             const double v = strtod(s.c_str() + i, &end);
