@@ -6,9 +6,10 @@ A step is one pass of the hot path over the whole point set: move every
 point, restore the exact Delaunay triangulation (on the torus: per-vertex
 star recomputation with consistency checks; in the box: repair flips /
 take-backs / relocation / Lawson flips), one-ring CSR, and the uniform-grid
-FRNN rebuild + query (CSR), issued as one mcl_tri_timestep call (with
---graph 1, the default, the star step's head and the tail are hipGraph
-replays).  Inputs live in HBM before the timed region; the K timed steps are
+FRNN rebuild + query (CSR), issued as one mcl_tri_timestep call, with the
+library's launch options (MCL_OPT_GRAPH from its tuning table, eager
+launches by default; --graph 1 replays the star step's head and the tail
+from hipGraphs).  Inputs live in HBM before the timed region; the K timed steps are
 bracketed by torch.cuda.synchronize().  Then --timer-steps further steps run
 with eager launches and per-kernel HIP event timers, for the roofline.  After the timed
 steps the final mesh is read through the half-edge arrays and checked (every
@@ -91,8 +92,9 @@ def _args():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the c5 sub-record of the default c2 run")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--no-pred-check", action="store_true")
-    ap.add_argument("--graph", type=int, default=1, choices=(0, 1),
-                    help="replay each timestep's star head and tail from hipGraphs (MCL_OPT_GRAPH)")
+    ap.add_argument("--graph", type=int, default=-1, choices=(-1, 0, 1),
+                    help="replay each timestep's star head and tail from hipGraphs (MCL_OPT_GRAPH; "
+                         "-1: the library's setting)")
     ap.add_argument("--timer-steps", type=int, default=10,
                     help="eager steps timed after the run with per-kernel event timers (the roofline figures)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -369,7 +371,9 @@ def run_workload(a, M, torch, workload, steps, warmup, plain_steps, cpu_seconds,
             if g_ is not None:
                 g_.kernel_timer(on)
 
-    dt.set_option(M.OPT_GRAPH, a.graph)
+    if a.graph >= 0:
+        dt.set_option(M.OPT_GRAPH, a.graph)
+    graph = bool(dt.get_option(M.OPT_GRAPH))
     k = 0
     for _ in range(warmup):
         step(k)
@@ -384,7 +388,7 @@ def run_workload(a, M, torch, workload, steps, warmup, plain_steps, cpu_seconds,
     el = time.perf_counter() - t0
     ms_per_step = 1e3 * el / steps
     value = n * steps / el
-    print("[bench] %s timed %d steps: %.3f ms/step%s" % (workload, steps, ms_per_step, " (graphs)" if a.graph else ""),
+    print("[bench] %s timed %d steps: %.3f ms/step%s" % (workload, steps, ms_per_step, " (graphs)" if graph else ""),
           file=sys.stderr, flush=True)
     # per-kernel GPU times: further steps launched eagerly with event timers
     dt.set_option(M.OPT_GRAPH, 0)
@@ -524,7 +528,7 @@ def run_workload(a, M, torch, workload, steps, warmup, plain_steps, cpu_seconds,
                           "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / PEAK_HBM_GBS, "kbar": kbar},
         "kernels_ms_per_launch": {k_: (v_[0] / max(v_[1], 1)) for k_, v_ in kern.items()},
         "kernels_ms_per_step": {k_: v_[0] / tsteps for k_, v_ in kern.items()},
-        "graph": bool(a.graph),
+        "graph": graph,
         "eager_ms_per_step": eager_ms,
         "plain_value": plain,
         "plain_sample": plain_sample,

@@ -123,17 +123,34 @@ __global__ void __launch_bounds__(kT) k_gather(const int* __restrict__ perm, int
     }
 }
 
-// pass 0 counts, pass 1 writes
+// pass 0 counts, pass 1 writes.  The rows of a wave's 64 consecutive points
+// are one contiguous range of the output; in pass 1 a wave whose range fits
+// kWaveRow ints gathers it in LDS and writes it out with contiguous stores
+// (each thread storing its own short row directly writes partial cache lines,
+// several times the bytes of the list); a longer range is written directly.
+constexpr int kWaveRow = 1024;
 template <int kPass>
 __global__ void __launch_bounds__(kT) k_frnn(GridParams g, const float2* __restrict__ pos, int n,
                                              const int* __restrict__ start, float rad2, int* __restrict__ cnt,
                                              const int* __restrict__ off, int* __restrict__ nbr, int cap) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const float2 pi = pos[i];
+    __shared__ int s_row[kPass ? kT / 64 : 1][kPass ? kWaveRow : 1];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // block-uniform loop: every thread meets the barriers
+    for (int bb = blockIdx.x * blockDim.x; bb < n; bb += gridDim.x * blockDim.x) {
+        const int i = bb + threadIdx.x;
+        const bool act = i < n;
+        const int wb = bb + (threadIdx.x & ~63);  // this wave's first point
+        int w0 = 0, wn = 0;
+        if (kPass && wb < n) {
+            w0 = off[wb];
+            wn = off[min(wb + 64, n)] - w0;
+        }
+        const bool staged = kPass && wn <= kWaveRow;
+        const float2 pi = act ? pos[i] : make_float2(0.0f, 0.0f);
         const int2 h = cell_of(g, pi);
-        int c = 0, w = kPass ? off[i] : 0;
+        int c = 0, w = (kPass && act) ? off[i] : 0;
         const int R = g.reach;
-        for (int ddx = -R; ddx <= R; ddx++) {
+        for (int ddx = -R; act && ddx <= R; ddx++) {
             for (int ddy = -R; ddy <= R; ddy++) {
                 int x = h.x + ddx, y = h.y + ddy;
                 if (g.periodic) {
@@ -154,14 +171,24 @@ __global__ void __launch_bounds__(kT) k_frnn(GridParams g, const float2* __restr
                     if (g.periodic) { dx = min_image_f(dx); dy = min_image_f(dy); }
                     const float r2 = __fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy));
                     if (r2 <= rad2) {
-                        if (kPass && w < cap) nbr[w] = j;
+                        if (kPass) {
+                            if (staged) s_row[wv][w - w0] = j;
+                            else if (w < cap) nbr[w] = j;
+                        }
                         w++;
                         c++;
                     }
                 }
             }
         }
-        if (!kPass) cnt[i] = c;
+        if (!kPass && act) cnt[i] = c;
+        if (kPass) {
+            __syncthreads();
+            if (staged)
+                for (int k = lane; k < wn; k += 64)
+                    if (w0 + k < cap) nbr[w0 + k] = s_row[wv][k];
+            __syncthreads();
+        }
     }
 }
 
@@ -563,39 +590,64 @@ __global__ void __launch_bounds__(kT) k_bkey_count(BGridParams b, const float2* 
     }
 }
 
-// pass 0 counts, pass 1 writes (sorted slots; perm[s] = point index - v0)
+// pass 0 counts, pass 1 writes (sorted slots; perm[s] = point index - v0);
+// pass 1 stages a wave's rows in LDS when they fit (as k_frnn)
 template <int kPass>
 __global__ void __launch_bounds__(kT) k_bfrnn(BGridParams b, const float2* __restrict__ spos,
                                               const int* __restrict__ perm, const int* __restrict__ start,
                                               int* __restrict__ cnt, const int* __restrict__ off, int* __restrict__ nbr,
                                               int cap) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
-        const float2 pi = spos[i];
-        const int t = bgrid_inst(b, b.v0 + perm[i]);
-        const int nc = b.nc[t], cb = b.cbase[t];
-        const float rad2 = b.rad2[t];
-        const int2 h = bgrid_cell(b.cs[t], nc, pi);
-        int c = 0, w = kPass ? off[i] : 0;
-        // keys are row-major within an instance, so the up to 3 cells of one
-        // row are one contiguous range of the sorted points (same order as
-        // cell by cell)
-        const int x0 = max(h.x - 1, 0), x1 = min(h.x + 1, nc - 1);
-        for (int y = max(h.y - 1, 0); y <= min(h.y + 1, nc - 1); y++) {
-            const int k = cb + y * nc;
-            const int e = start[k + x1 + 1];
-            for (int j = start[k + x0]; j < e; j++) {
-                if (j == i) continue;
-                const float2 pj = spos[j];
-                const float dx = __fsub_rn(pj.x, pi.x), dy = __fsub_rn(pj.y, pi.y);
-                const float r2 = __fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy));
-                if (r2 <= rad2) {
-                    if (kPass && w < cap) nbr[w] = j;
-                    w++;
-                    c++;
+    __shared__ int s_row[kPass ? kT / 64 : 1][kPass ? kWaveRow : 1];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // block-uniform loop: every thread meets the barriers
+    for (int bb = blockIdx.x * blockDim.x; bb < b.n; bb += gridDim.x * blockDim.x) {
+        const int i = bb + threadIdx.x;
+        const bool act = i < b.n;
+        const int wb = bb + (threadIdx.x & ~63);
+        int w0 = 0, wn = 0;
+        if (kPass && wb < b.n) {
+            w0 = off[wb];
+            wn = off[min(wb + 64, b.n)] - w0;
+        }
+        const bool staged = kPass && wn <= kWaveRow;
+        int c = 0, w = (kPass && act) ? off[i] : 0;
+        if (act) {
+            const float2 pi = spos[i];
+            const int t = bgrid_inst(b, b.v0 + perm[i]);
+            const int nc = b.nc[t], cb = b.cbase[t];
+            const float rad2 = b.rad2[t];
+            const int2 h = bgrid_cell(b.cs[t], nc, pi);
+            // keys are row-major within an instance, so the up to 3 cells of one
+            // row are one contiguous range of the sorted points (same order as
+            // cell by cell)
+            const int x0 = max(h.x - 1, 0), x1 = min(h.x + 1, nc - 1);
+            for (int y = max(h.y - 1, 0); y <= min(h.y + 1, nc - 1); y++) {
+                const int k = cb + y * nc;
+                const int e = start[k + x1 + 1];
+                for (int j = start[k + x0]; j < e; j++) {
+                    if (j == i) continue;
+                    const float2 pj = spos[j];
+                    const float dx = __fsub_rn(pj.x, pi.x), dy = __fsub_rn(pj.y, pi.y);
+                    const float r2 = __fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy));
+                    if (r2 <= rad2) {
+                        if (kPass) {
+                            if (staged) s_row[wv][w - w0] = j;
+                            else if (w < cap) nbr[w] = j;
+                        }
+                        w++;
+                        c++;
+                    }
                 }
             }
         }
-        if (!kPass) cnt[i] = c;
+        if (!kPass && act) cnt[i] = c;
+        if (kPass) {
+            __syncthreads();
+            if (staged)
+                for (int k = lane; k < wn; k += 64)
+                    if (w0 + k < cap) nbr[w0 + k] = s_row[wv][k];
+            __syncthreads();
+        }
     }
 }
 

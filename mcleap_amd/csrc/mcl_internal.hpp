@@ -174,6 +174,9 @@ struct KTimer {
 // (all block-uniform); falls back to the plain grid-stride split when the
 // grid is not a multiple of 8.
 constexpr int kXcds = 8;
+// ints of LDS per wave for staging a wave's CSR one-ring rows
+constexpr int kRingWave = 1024;
+
 __device__ __forceinline__ void xcd_split(int n, int per_block, int& first, int& end, int& stride) {
     const int nb = gridDim.x;
     if (nb % kXcds != 0) {

@@ -1110,17 +1110,42 @@ __global__ void __launch_bounds__(256) k_star_face3(DMesh m) {
 __global__ void __launch_bounds__(256) k_star_ring_deg(int nv, const int* __restrict__ deg, int* __restrict__ off) {
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) off[v] = deg[v];
 }
+// The rows of a wave's 64 consecutive vertices are one contiguous range of
+// the output: a range of at most kRingWave ints is gathered in LDS and
+// written with contiguous stores (rows stored thread by thread write partial
+// cache lines, several times the bytes of the list); a longer one (corner
+// fans) directly.
 __global__ void __launch_bounds__(256) k_star_ring_csr(int nv, int ncorner, const int* __restrict__ deg,
                                                        const int* __restrict__ nbr, const int* __restrict__ off,
                                                        int* __restrict__ out, int cap) {
-    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
-        const int d = deg[v];
-        const int k0 = off[v];
-        const int* row = nbr + star_row(v, nv, ncorner);
-        for (int i = 0; i < d; i++) {
-            const int k = k0 + i;
-            if (k < cap) out[k] = row[i == 0 ? 0 : d - i];
+    __shared__ int s_row[4][kRingWave];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // block-uniform loop: every thread meets the barriers
+    for (int bb = blockIdx.x * blockDim.x; bb < nv; bb += gridDim.x * blockDim.x) {
+        const int v = bb + threadIdx.x;
+        const int wb = bb + (threadIdx.x & ~63);
+        int w0 = 0, wn = 0;
+        if (wb < nv) {
+            w0 = off[wb];
+            wn = off[min(wb + 64, nv)] - w0;
         }
+        const bool staged = wn <= kRingWave;
+        if (v < nv) {
+            const int d = deg[v];
+            const int k0 = off[v];
+            const int* row = nbr + star_row(v, nv, ncorner);
+            for (int i = 0; i < d; i++) {
+                const int k = k0 + i;
+                const int x = row[i == 0 ? 0 : d - i];
+                if (staged) s_row[wv][k - w0] = x;
+                else if (k < cap) out[k] = x;
+            }
+        }
+        __syncthreads();
+        if (staged)
+            for (int k = lane; k < wn; k += 64)
+                if (w0 + k < cap) out[w0 + k] = s_row[wv][k];
+        __syncthreads();
     }
 }
 

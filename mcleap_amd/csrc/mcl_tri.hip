@@ -1439,18 +1439,40 @@ __global__ void __launch_bounds__(kThreads) k_degree(DMesh m, int* __restrict__ 
         deg[v] = c;
     }
 }
+// (rows of a wave's vertices gathered in LDS and written with contiguous
+// stores when they fit, as k_star_ring_csr)
 __global__ void __launch_bounds__(kThreads) k_ring_csr(DMesh m, const int* __restrict__ off, int* __restrict__ nbr, int cap) {
+    __shared__ int s_row[kThreads / 64][kRingWave];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int first, last, stride;
     xcd_split(m.nv, blockDim.x, first, last, stride);
+    // block-uniform loop: every thread meets the barriers
     for (int base = first; base < last; base += stride) {
         const int v = base + threadIdx.x;
-        if (v >= last) break;
-        int h0 = m.v_he[v], h = h0, k = off[v], end = off[v + 1];
-        do {
-            if (k < cap && k < end) nbr[k] = m.he_v[h ^ 1];
-            k++;
-            h = m.he_next[h ^ 1];
-        } while (h != h0 && k < end);
+        const int wb = base + (threadIdx.x & ~63);
+        int w0 = 0, wn = 0;
+        if (wb < last) {
+            w0 = off[wb];
+            wn = off[min(wb + 64, last)] - w0;
+        }
+        const bool staged = wn <= kRingWave;
+        if (v < last) {
+            int h0 = m.v_he[v], h = h0, k = off[v], end = off[v + 1];
+            do {
+                const int x = m.he_v[h ^ 1];
+                if (k < end) {
+                    if (staged) s_row[wv][k - w0] = x;
+                    else if (k < cap) nbr[k] = x;
+                }
+                k++;
+                h = m.he_next[h ^ 1];
+            } while (h != h0 && k < end);
+        }
+        __syncthreads();
+        if (staged)
+            for (int k = lane; k < wn; k += 64)
+                if (w0 + k < cap) nbr[w0 + k] = s_row[wv][k];
+        __syncthreads();
     }
 }
 // oneRing<K> (layout 0) / oneRing2<K> (layout 1): kernels.cuh:54-121
@@ -1625,6 +1647,18 @@ constexpr int kPathProbe = 64;  // first interval between measurements of the sl
 constexpr int kBigBlocks = 256;   // workgroups of the rounds that run the big exact stage
 
 
+// the tail of a timestep (mcl_tri_timestep): CSR one-ring and grid FRNN query
+struct TailReq {
+    int* ring_off;
+    int* ring_nbrs;
+    int ring_cap;
+    mcl_grid_t* g;
+    float rad;
+    int* frnn_off;
+    int* frnn_nbrs;
+    int frnn_cap;
+};
+
 struct mcl_tri {
     DMesh m;
     // flip rounds
@@ -1689,11 +1723,18 @@ struct mcl_tri {
     mcl_step_params_t* h_params = nullptr;  // pinned staging copy of them
     int graph = 0;                          // MCL_OPT_GRAPH: star steps (and timesteps) replayed from hipGraphs
     hipGraphExec_t g_head[2] = {nullptr, nullptr};  // star-step head per star_cur
-    hipGraphExec_t g_post[2] = {nullptr, nullptr};  // timestep tail (ring CSR + grid FRNN) per star_cur
+    hipGraphExec_t g_post[3] = {nullptr, nullptr, nullptr};  // timestep tail: ring CSR per star_cur, grid FRNN
     const void* post_key[8] = {};           // the buffers g_post was captured with
     bool capturing = false;                 // kernel timers are not recorded under capture
     bool captured = false;                  // a graph was captured during the current step
     hipStream_t cap_stream = nullptr;       // stream the graphs are captured on (the caller's may be the null stream)
+    int* h_aux = nullptr;                   // pinned copy of s_aux, read after a star step's passes
+    hipEvent_t ev_step = nullptr;           // start of a timed kinetic step (path choice)
+    hipEvent_t ev_aux = nullptr;            // after a star step's last counter copy: the end of its own work
+    hipEvent_t ev_end = nullptr;            // end of a timed flip-based step
+    bool star_end_event = false;            // the current step settled by stars: ev_aux ends it
+    const TailReq* tail_req = nullptr;      // the calling timestep's tail (its FRNN part enqueued at the star read)
+    bool frnn_done = false;                 // ... and that FRNN query holds the step's final positions
     // one ring
     int* deg = nullptr;
     int* scan_tmp = nullptr;
@@ -1743,11 +1784,14 @@ static int tri_free(mcl_tri* t) {
                     t->c_bucket, t->c_count, t->d_stamp, t->d_params};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < 2; k++)
         if (t->g_head[k]) (void)hipGraphExecDestroy(t->g_head[k]);
+    for (int k = 0; k < 3; k++)
         if (t->g_post[k]) (void)hipGraphExecDestroy(t->g_post[k]);
-    }
     if (t->h_params) (void)hipHostFree(t->h_params);
+    if (t->h_aux) (void)hipHostFree(t->h_aux);
+    for (hipEvent_t e : {t->ev_step, t->ev_aux, t->ev_end})
+        if (e) (void)hipEventDestroy(e);
     if (t->cap_stream) (void)hipStreamDestroy(t->cap_stream);
     if (t->h_cnt) (void)hipHostFree(t->h_cnt);
     if (t->h_stats) (void)hipHostFree(t->h_stats);
@@ -1859,6 +1903,9 @@ int tri_make(int nv, int nhe, int nt, int periodic, int ncorner, mcl_tri** out) 
 #undef A
     if (rc == MCL_OK && hipHostMalloc((void**)&t->h_cnt, sizeof(int) * kCnCount) != hipSuccess) rc = MCL_ERR_HIP;
     if (rc == MCL_OK && hipHostMalloc((void**)&t->h_params, sizeof(mcl_step_params_t)) != hipSuccess) rc = MCL_ERR_HIP;
+    if (rc == MCL_OK && hipHostMalloc((void**)&t->h_aux, sizeof(int) * kStarAux) != hipSuccess) rc = MCL_ERR_HIP;
+    for (hipEvent_t* e : {&t->ev_step, &t->ev_aux, &t->ev_end})
+        if (rc == MCL_OK && hipEventCreate(e) != hipSuccess) rc = MCL_ERR_HIP;
     if (rc == MCL_OK &&
         hipHostMalloc((void**)&t->h_stats, sizeof(unsigned long long) * kStCount * kStatShards) != hipSuccess)
         rc = MCL_ERR_HIP;
@@ -2991,8 +3038,11 @@ void star_head(mcl_tri* t, hipStream_t s, int cur, int nxt, const mcl_step_param
 static void drop_graphs(mcl_tri* t) {
     for (int k = 0; k < 2; k++) {
         if (t->g_head[k]) (void)hipGraphExecDestroy(t->g_head[k]);
+        t->g_head[k] = nullptr;
+    }
+    for (int k = 0; k < 3; k++) {
         if (t->g_post[k]) (void)hipGraphExecDestroy(t->g_post[k]);
-        t->g_head[k] = t->g_post[k] = nullptr;
+        t->g_post[k] = nullptr;
     }
 }
 
@@ -3019,6 +3069,44 @@ static int capture_graph(mcl_tri* t, hipGraphExec_t* out, F&& enqueue) {
     return MCL_OK;
 }
 
+static int enqueue_ring_csr(mcl_tri* t, hipStream_t s, int c, bool from_stars, int* off, int* nbrs, int cap);
+
+// Enqueues a part of a timestep's tail on s: the grid's counting sort of the
+// positions with its CSR FRNN query (part 0), or the CSR one-ring (part 1),
+// from the stars in buffer c (from_stars) or from the half-edges.  With
+// MCL_OPT_GRAPH (the one-ring only from stars): one launch of a graph
+// captured per part (and star buffer), captured again when the caller's
+// buffers change.  Only device work, no host read: running a part again over
+// the same positions and stars gives the same arrays.
+static int launch_tail(mcl_tri* t, const TailReq& q, hipStream_t s, int part, int c, bool from_stars) {
+    auto enqueue = [&](hipStream_t cs) -> int {
+        if (part == 0 && q.g) MCL_TRY(grid_enqueue_frnn(q.g, t->m.pos, q.rad, q.frnn_off, q.frnn_nbrs, q.frnn_cap, cs));
+        if (part == 1 && q.ring_off) MCL_TRY(enqueue_ring_csr(t, cs, c, from_stars, q.ring_off, q.ring_nbrs, q.ring_cap));
+        return MCL_OK;
+    };
+    if ((part == 0 && !q.g) || (part == 1 && !q.ring_off)) return MCL_OK;
+    if (!t->graph || (part == 1 && !from_stars)) return enqueue(s);
+    float radv = q.rad;
+    const void* key[8] = {q.ring_off, q.ring_nbrs, (const void*)(intptr_t)q.ring_cap, q.g,
+                          (const void*)(intptr_t)*(const int*)&radv, q.frnn_off, q.frnn_nbrs,
+                          (const void*)(intptr_t)q.frnn_cap};
+    if (memcmp(key, t->post_key, sizeof(key)) != 0) {
+        for (int k = 0; k < 3; k++) {
+            if (t->g_post[k]) (void)hipGraphExecDestroy(t->g_post[k]);
+            t->g_post[k] = nullptr;
+        }
+        memcpy(t->post_key, key, sizeof(key));
+    }
+    hipGraphExec_t* ge = &t->g_post[part == 0 ? 2 : c];
+    if (!*ge) {
+        int trc = MCL_OK;
+        MCL_TRY(capture_graph(t, ge, [&](hipStream_t cs) { trc = enqueue(cs); }));
+        MCL_TRY(trc);
+    }
+    MCL_HIP(hipGraphLaunch(*ge, s));
+    return MCL_OK;
+}
+
 static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp, mcl_stats_t* st, hipStream_t s,
                      bool* done) {
     DMesh& m = t->m;
@@ -3042,13 +3130,32 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     }
     MCL_HIP(hipGetLastError());
     int aux[kStarAux];
-    // the counters, and the statistics with them (one synchronisation)
-    auto read_aux = [&](int pass) -> int {
-        MCL_HIP(hipMemcpyAsync(aux, t->s_aux, sizeof(aux), hipMemcpyDeviceToHost, s));
+    // the counters, and the statistics with them (one wait).  At the first
+    // read the FRNN part of the calling timestep's tail is enqueued behind
+    // the copies before the host waits for them, so that the GPU runs it
+    // while the host takes its decision.  It reads the positions only, which
+    // later passes change only by sending a vertex found on top of another
+    // back to its previous position (counted in the stalled and duplicate
+    // statistics); if one does, or the step falls back, it runs again after
+    // the step (frnn_done false).
+    unsigned long long moved_back = 0;
+    auto read_aux = [&](int pass, bool first) -> int {
+        MCL_HIP(hipMemcpyAsync(t->h_aux, t->s_aux, sizeof(aux), hipMemcpyDeviceToHost, s));
         MCL_HIP(hipMemcpyAsync(t->h_stats, t->stats, sizeof(unsigned long long) * kStCount * kStatShards,
                                hipMemcpyDeviceToHost, s));
-        MCL_HIP(hipStreamSynchronize(s));
+        MCL_HIP(hipEventRecord(t->ev_aux, s));
+        if (first && t->tail_req && t->tail_req->g) {
+            MCL_TRY(launch_tail(t, *t->tail_req, s, 0, nxt, true));
+            t->frnn_done = true;
+        }
+        MCL_HIP(hipEventSynchronize(t->ev_aux));
+        memcpy(aux, t->h_aux, sizeof(aux));
         t->h_stats_fresh = true;
+        unsigned long long mb = 0;
+        for (int j = 0; j < kStatShards; j++)
+            mb += t->h_stats[(size_t)kStStalled * kStatShards + j] + t->h_stats[(size_t)kStRelDup * kStatShards + j];
+        if (first) moved_back = mb;
+        else if (mb != moved_back) t->frnn_done = false;
         if (trace)
             fprintf(stderr,
                     "  [stars pass %d] inconsistent %d (failed stars so far %d: candidates %d undecided %d open %d "
@@ -3063,7 +3170,7 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
             star_corner_fans(t, s, cur, nxt);
             MCL_HIP(hipGetLastError());
         }
-        MCL_TRY(read_aux(r));
+        MCL_TRY(read_aux(r, r == t->star_head_passes - 1));
         if (aux[1] == 0) {
             long long sum_deg, sum_corner = 0;
             memcpy(&sum_deg, aux + 2, sizeof(sum_deg));
@@ -3076,6 +3183,7 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
             break;
         }
     }
+    if (!settled) t->frnn_done = false;
     if (!settled && trace && aux[1] > 0) {
         // the first vertices still listed: id, position, star degree (-1: failed)
         const int k = std::min(aux[1], 12);
@@ -3104,6 +3212,7 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     t->star_cur = nxt;
     t->stars_valid = true;
     t->mesh_stale = true;
+    t->star_end_event = true;  // nothing of the step's own work follows its last counter read
     if (validate_on()) MCL_TRY(ensure_mesh(t, s));
     if (st) {
         long long retried;
@@ -3176,9 +3285,15 @@ static int kinetic_step(mcl_tri* t, const mcl_step_params_t* p, const float2* di
                 t->path_since = 0;
             }
         }
-        MCL_HIP(hipStreamSynchronize(s));
+        // The path's time on the stream, between events: from the point the
+        // GPU reaches the step (after the previous step's work) to the end of
+        // the path's own work (a settled star step: its last counter read,
+        // before any timestep tail it enqueued; the flip-based step: its
+        // end), host round trips inside the step included.  No host wait for
+        // the previous work.
         t->captured = false;
-        auto t0 = std::chrono::steady_clock::now();
+        t->star_end_event = false;
+        MCL_HIP(hipEventRecord(t->ev_step, s));
         int rc;
         double flip_ms = -1.0;
         if (path == 1) {
@@ -3189,8 +3304,15 @@ static int kinetic_step(mcl_tri* t, const mcl_step_params_t* p, const float2* di
             rc = step_impl(t, p, disp, st, stream);
         }
         if (rc != MCL_OK) return rc;
-        MCL_HIP(hipStreamSynchronize(s));
-        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        hipEvent_t end = t->ev_aux;
+        if (!t->star_end_event) {
+            end = t->ev_end;
+            MCL_HIP(hipEventRecord(end, s));
+        }
+        MCL_HIP(hipEventSynchronize(end));
+        float ev_ms = 0.0f;
+        MCL_HIP(hipEventElapsedTime(&ev_ms, t->ev_step, end));
+        const double ms = ev_ms;
         if (t->captured) return MCL_OK;  // a step that captured a graph is not a measurement of its path
         if (probe) {  // the probed path stays slower: probe half as often
             const bool keeps = ms > t->path_ms[1 - path];
@@ -3365,16 +3487,15 @@ extern "C" int mcl_tri_one_ring(mcl_tri_t* t, int K, int layout, void* d_ring, i
 }
 
 // the CSR one-ring, asynchronous (cap >= nhe: every half-edge leaves one
-// vertex), from the stars when they are current, else from the half-edges
-static int enqueue_ring_csr(mcl_tri* t, hipStream_t s, int* off, int* nbrs, int cap) {
-    const int c = t->star_cur;
-    if (t->stars_valid)
+// vertex), from the stars in buffer c when from_stars, else from the half-edges
+static int enqueue_ring_csr(mcl_tri* t, hipStream_t s, int c, bool from_stars, int* off, int* nbrs, int cap) {
+    if (from_stars)
         hipLaunchKernelGGL(k_star_ring_deg, dim3(blocks_for(t->m.nv)), dim3(256), 0, s, t->m.nv,
                            (const int*)t->star_deg[c], off);
     else
         hipLaunchKernelGGL(k_degree, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, off);
     MCL_TRY(scan_exclusive(off, off, t->m.nv, t->scan_tmp, s));
-    if (t->stars_valid)
+    if (from_stars)
         hipLaunchKernelGGL(k_star_ring_csr, dim3(blocks_for(t->m.nv)), dim3(256), 0, s, t->m.nv, t->m.ncorner,
                            (const int*)t->star_deg[c], (const int*)t->star_nbr[c], (const int*)off, nbrs, cap);
     else
@@ -3388,38 +3509,20 @@ extern "C" int mcl_tri_timestep(mcl_tri_t* t, const mcl_step_params_t* p, void* 
                                 int frnn_cap, mcl_stats_t* st, void* stream) {
     if (!t || !p || (d_ring_off && (!d_ring_nbrs || ring_cap < t->m.nhe)) || (g && !d_frnn_off)) return MCL_ERR_INPUT;
     hipStream_t s = (hipStream_t)stream;
+    const TailReq q{(int*)d_ring_off, (int*)d_ring_nbrs, ring_cap, g, rad, (int*)d_frnn_off, (int*)d_frnn_nbrs,
+                    frnn_cap};
+    const bool want = d_ring_off || g;
+    // a star step enqueues the FRNN part right after its counter read, before
+    // it waits for that read (see star_step); frnn_done: it need not run again
+    t->tail_req = want ? &q : nullptr;
+    t->frnn_done = false;
     int rc = mcl_tri_step(t, p, st, stream);
+    t->tail_req = nullptr;
     if (rc != MCL_OK && rc != MCL_ERR_UNRESOLVED) return rc;
-    // the tail: mesh one-ring and grid FRNN of the new positions
-    if (!d_ring_off && !g) return rc;
+    if (!want) return rc;
+    if (!t->frnn_done) MCL_TRY(launch_tail(t, q, s, 0, t->star_cur, t->stars_valid));
     if (!t->stars_valid) MCL_TRY(ensure_mesh(t, s));
-    auto tail = [&](hipStream_t cs) -> int {
-        if (d_ring_off) MCL_TRY(enqueue_ring_csr(t, cs, (int*)d_ring_off, (int*)d_ring_nbrs, ring_cap));
-        if (g) MCL_TRY(grid_enqueue_frnn(g, t->m.pos, rad, (int*)d_frnn_off, (int*)d_frnn_nbrs, frnn_cap, cs));
-        return MCL_OK;
-    };
-    if (t->graph && t->stars_valid) {
-        float radv = rad;
-        const void* key[8] = {d_ring_off, d_ring_nbrs, (const void*)(intptr_t)ring_cap, g,
-                              (const void*)(intptr_t)*(const int*)&radv, d_frnn_off, d_frnn_nbrs,
-                              (const void*)(intptr_t)frnn_cap};
-        if (memcmp(key, t->post_key, sizeof(key)) != 0) {
-            for (int k = 0; k < 2; k++) {
-                if (t->g_post[k]) (void)hipGraphExecDestroy(t->g_post[k]);
-                t->g_post[k] = nullptr;
-            }
-            memcpy(t->post_key, key, sizeof(key));
-        }
-        const int c = t->star_cur;
-        if (!t->g_post[c]) {
-            int trc = MCL_OK;
-            MCL_TRY(capture_graph(t, &t->g_post[c], [&](hipStream_t cs) { trc = tail(cs); }));
-            MCL_TRY(trc);
-        }
-        MCL_HIP(hipGraphLaunch(t->g_post[c], s));
-        return rc;
-    }
-    MCL_TRY(tail(s));
+    MCL_TRY(launch_tail(t, q, s, 1, t->star_cur, t->stars_valid));
     return rc;
 }
 
