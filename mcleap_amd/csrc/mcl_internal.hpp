@@ -122,6 +122,7 @@ enum CountSlot : int {
     kCnErr = 14,      // error flags
     kCnMisc = 15,
     kCnTouched = 16,  // vertices of quads flipped by repair rounds
+    kCnDefer = 17,    // relocations deferred past the Lawson rounds
     kCnCount = 32
 };
 

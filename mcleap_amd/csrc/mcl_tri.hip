@@ -1028,7 +1028,8 @@ MCL_HD int reloc_core(const DMesh& m, int v, const int* star, int deg0, int tt, 
 
 // phase 3: winners (all claims held) remove and re-insert their vertex.
 // loc: {relocated, stalled, walk left the mesh or retries spent, star above
-// kMaxStar, removal stuck, put back, duplicate targets}
+// kMaxStar (counted by the driver, see reloc_deferred), removal stuck, put
+// back, duplicate targets}
 constexpr int kRelocRetries = 64;  // waits and walk retries per vertex and step
 template <bool kF32>
 __device__ __forceinline__ void reloc_apply_body(const DMesh& m, const int* __restrict__ vlist, int n, int round,
@@ -1050,7 +1051,8 @@ __device__ __forceinline__ void reloc_apply_body(const DMesh& m, const int* __re
             }
             continue;
         }
-        if (tt == -4) { loc[1]++; loc[3]++; continue; }  // star larger than kMaxStar: stays put
+        if (tt == -4) continue;  // star larger than kMaxStar: stays put for this phase (counted by the
+                                 // driver, which may relocate it again after Lawson rounds)
         if (tt < 0) { loc[1]++; loc[2]++; continue; }  // target outside the mesh: stays put
         const unsigned long long key = claim_key(round, v);
         int star[kMaxStar];
@@ -1146,6 +1148,21 @@ __global__ void __launch_bounds__(kThreads) k_reloc_init(const int* __restrict__
         dst[i] = v;
         r_wait[v] = 0;
     }
+}
+
+// the vertices of list (n) whose relocation met a star above kMaxStar (r_tt
+// -4) into out, counted in *count
+__global__ void __launch_bounds__(kThreads) k_reloc_deferred(const int* __restrict__ list, int n,
+                                                             const int* __restrict__ r_tt, int* __restrict__ out,
+                                                             int* __restrict__ count) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int v = list[i];
+        if (r_tt[v] == -4) out[agg_inc(count)] = v;
+    }
+}
+// adds v to statistics slot k (its first shard)
+__global__ void k_stat_add(unsigned long long* __restrict__ stats, int k, unsigned long long v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) stats[(size_t)k * kStatShards] += v;
 }
 
 // ------------------------------------------------------------ flip rounds
@@ -1692,6 +1709,7 @@ struct mcl_tri {
     int* r_tt = nullptr;               // nv: relocation target triangle
     int* r_tt2 = nullptr;              // nv: second triangle when the target is on an edge
     int* r_wait = nullptr;             // nv: relocation retries (waits, walks) in this step
+    int* r_defer = nullptr;            // nv: relocations deferred past the Lawson rounds
     int* tail_out = nullptr;           // 8: results of the single-workgroup loops
     // star recomputation (torus kinetic step)
     int* star_deg[2] = {nullptr, nullptr};  // nv each
@@ -1778,7 +1796,7 @@ static int tri_free(mcl_tri* t) {
     void* ptrs[] = {t->m.pos, t->m.he_next, t->m.he_v, t->m.he_t, t->m.he_op, t->m.v_he, t->m.v_pidx, t->m.t_he,
                     t->claim, t->e_stamp, t->cand[0], t->cand[1], t->wants, t->cnt, t->rounds_done, t->stats,
                     t->scalar, t->acc, t->tgt, t->v_mv, t->v_rev, t->v_fexp, t->t_stamp, t->bad[0], t->bad[1],
-                    t->rev, t->allrev, t->pend[0], t->pend[1], t->deg, t->scan_tmp, t->v_touch, t->t_rep, t->t_rep_e, t->e_undo, t->undo, t->stuck, t->touched, t->r_tt, t->r_tt2, t->r_wait, t->v_soft, t->tail_out,
+                    t->rev, t->allrev, t->pend[0], t->pend[1], t->deg, t->scan_tmp, t->v_touch, t->t_rep, t->t_rep_e, t->e_undo, t->undo, t->stuck, t->touched, t->r_tt, t->r_tt2, t->r_wait, t->r_defer, t->v_soft, t->tail_out,
                     t->star_deg[0], t->star_deg[1], t->star_nbr[0], t->star_nbr[1], t->s_ecnt, t->s_tcnt,
                     t->s_mark, t->s_list, t->s_aux, t->star_tdeg, t->star_tnbr, t->s_rlist, t->s_heavy,
                     t->c_bucket, t->c_count, t->d_stamp, t->d_params};
@@ -1889,7 +1907,7 @@ int tri_make(int nv, int nhe, int nt, int periodic, int ncorner, mcl_tri** out) 
     A(t->bad[0], nt); A(t->bad[1], nt); A(t->rev, nv); A(t->allrev, nv); A(t->pend[0], nv); A(t->pend[1], nv);
     A(t->deg, nv + 1); A(t->scan_tmp, scan_scratch_ints(nv + 1) + 16);
     A(t->v_touch, nv); A(t->t_rep, nt); A(t->t_rep_e, nt); A(t->e_undo, m.ne); A(t->undo, m.ne);
-    A(t->stuck, nt); A(t->touched, nv); A(t->r_tt, nv); A(t->r_tt2, nv); A(t->r_wait, nv); A(t->v_soft, nv); A(t->tail_out, 8);
+    A(t->stuck, nt); A(t->touched, nv); A(t->r_tt, nv); A(t->r_tt2, nv); A(t->r_wait, nv); A(t->r_defer, nv); A(t->v_soft, nv); A(t->tail_out, 8);
     if (periodic || m.ncorner > 0) {
         A(t->star_deg[0], nv); A(t->star_deg[1], nv);
         // vertex rows, then the corners' fans (star_row); star_tnbr also holds
@@ -2176,6 +2194,8 @@ __global__ void __launch_bounds__(kThreads) k_targets_listed(DMesh m, const int*
 
 static int read_counts(mcl_tri* t, hipStream_t s);
 static int relocate_rounds(mcl_tri* t, hipStream_t s, bool f32, int n, int* slot, int* rounds);
+static int reloc_deferred(mcl_tri* t, hipStream_t s, bool f32, int variant, const int* list, int n, bool lawson,
+                          long long* rounds, int* reloc_rounds);
 template <int kKind>
 static int run_rounds(mcl_tri* t, bool all, int variant, int max_rounds, hipStream_t s, long long* rounds_out);
 static void stats_begin(mcl_tri* t, hipStream_t s);
@@ -2209,9 +2229,12 @@ extern "C" int mcl_tri_move_vertices(mcl_tri_t* t, const int* ids, const float* 
     int rc = MCL_OK;
     long long rounds = 0;
     if (mode < 2) {
+        // the list again in allrev: relocate_rounds consumes pend[0] and pend[1]
+        MCL_HIP(hipMemcpyAsync(t->allrev, t->pend[0], sizeof(int) * n, hipMemcpyDeviceToDevice, s));
         int slot = 0, rr = 0;
         rc = relocate_rounds(t, s, true, n, &slot, &rr);
         if (rc == MCL_OK && mode == 1) rc = run_rounds<0>(t, true, 0, 0, s, &rounds);
+        if (rc == MCL_OK) rc = reloc_deferred(t, s, true, 0, t->allrev, n, mode == 1, &rounds, &rr);
     } else {
         rc = run_rounds<1>(t, true, 1, 0, s, &rounds);
         if (rc == MCL_ERR_ROUND_CAP) rc = MCL_OK;
@@ -2772,6 +2795,53 @@ static int relocate_rounds(mcl_tri* t, hipStream_t s, bool f32, int n, int* slot
     return n == 0 ? MCL_OK : MCL_ERR_ROUND_CAP;
 }
 
+// The vertices of the relocation list `list` (n, in allrev or r_defer)
+// whose relocation met a star above kMaxStar.  Relocation re-inserts
+// vertices without Delaunay flips, so a vertex opposite an edge on which many
+// points are inserted (points clamped onto a side of the box, say) gains an
+// edge per point; after Lawson rounds (lawson) its degree is an ordinary
+// Delaunay one again, and it is relocated again (then Lawson rounds again),
+// at most kDeferPasses times.  What is still deferred after that, or
+// without Lawson rounds, counts as stalled.
+constexpr int kDeferPasses = 4;
+static int reloc_deferred(mcl_tri* t, hipStream_t s, bool f32, int variant, const int* list, int n, bool lawson,
+                          long long* rounds, int* reloc_rounds) {
+    const int* in = list;
+    for (int pass = 0; n > 0; pass++) {
+        int* out = in == t->r_defer ? t->allrev : t->r_defer;
+        hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->cnt + kCnDefer, 1, 0);
+        hipLaunchKernelGGL(k_reloc_deferred, dim3(blocks_for(n)), dim3(kThreads), 0, s, in, n, (const int*)t->r_tt,
+                           out, t->cnt + kCnDefer);
+        MCL_HIP(hipGetLastError());
+        MCL_TRY(read_counts(t, s));
+        const int nd = t->h_cnt[kCnDefer];
+        if (nd == 0) return MCL_OK;
+        if (trace_on()) fprintf(stderr, "  [reloc deferred %d] %d vertices\n", pass, nd);
+        if (!lawson || pass == kDeferPasses) {
+            hipLaunchKernelGGL(k_stat_add, dim3(1), dim3(64), 0, s, t->stats, (int)kStRelocFailed,
+                               (unsigned long long)nd);
+            hipLaunchKernelGGL(k_stat_add, dim3(1), dim3(64), 0, s, t->stats, (int)kStRelStar,
+                               (unsigned long long)nd);
+            MCL_HIP(hipGetLastError());
+            return MCL_OK;
+        }
+        hipLaunchKernelGGL(k_reloc_init, dim3(blocks_for(nd)), dim3(kThreads), 0, s, (const int*)out, nd, t->pend[0],
+                           t->r_wait);
+        MCL_HIP(hipGetLastError());
+        int slot = 0, rr = 0;
+        MCL_TRY(relocate_rounds(t, s, f32, nd, &slot, &rr));
+        MCL_TRY(validate(t, s, "deferred relocation rounds"));
+        *reloc_rounds += rr;
+        long long lr = 0;
+        MCL_TRY(run_rounds<0>(t, true, variant, 0, s, &lr));
+        MCL_TRY(validate(t, s, "Lawson rounds after deferred relocations"));
+        *rounds += lr;
+        in = out;
+        n = nd;
+    }
+    return MCL_OK;
+}
+
 // One kinetic step:
 //   move      -- every non-corner vertex goes to its target
 //   check     -- triangles that are not strictly CCW
@@ -2837,6 +2907,7 @@ static int step_impl(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     long long rounds = 0;
     if (rc == MCL_OK) rc = run_rounds<0>(t, true, variant, 0, s, &rounds);
     if (rc == MCL_OK) rc = validate(t, s, "Lawson rounds");
+    if (rc == MCL_OK && nrev > 0) rc = reloc_deferred(t, s, f32, variant, t->allrev, nrev, true, &rounds, &reloc_rounds);
     if (trace) {
         unsigned long long h[kStCount];
         if (stats_read(t, s, h) == MCL_OK)

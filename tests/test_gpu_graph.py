@@ -153,3 +153,57 @@ print(json.dumps(out))
     assert out["torus"]["tail_threshold"] == 8192
     assert out["box"]["tail_threshold"] == 4096 and out["box"]["reloc_tail"] == 256 and out["box"]["graph"] == 0
     assert out["grid_block_torus"] == 128 and out["grid_block_box"] == 256
+
+
+@pytest.mark.parametrize("periodic,head,graph", [(True, 1, False), (True, 4, False), (True, 1, True), (False, 1, False)])
+def test_timestep_tail_equals_fresh_queries(oracle, periodic, head, graph):
+    """mcl_tri_timestep enqueues the grid FRNN part of its tail at the star
+    step's counter read, before the stars have settled, and the one-ring after
+    it (again after the step when the positions changed later).  After every
+    step the ring and FRNN arrays it wrote must equal queries made afresh on
+    the resulting mesh and positions.  One head pass makes most steps settle
+    only after that read (the path that enqueues the one-ring late and may
+    repeat the FRNN); the box case (moves of four mean spacings) clamps
+    points onto the sides and corners of the unit square, where exact
+    duplicates send vertices back, and its first step defers relocations
+    whose star passed 64 triangles in the relocation phase's mesh (relocated
+    again after the Lawson rounds: no vertex may stall)."""
+    import torch
+    n = 20000
+    H = M.HostTriangulation(periodic=periodic)
+    H.addDelaunayPoints(uniform_points(n, 46))
+    rad = float(np.float32(2.0 / np.sqrt(n)))
+    r = _Run(H, graph, periodic, rad)
+    r.dt.set_option(M.OPT_STAR_HEAD_PASSES, head)
+    nv = r.dt.m_pos_size
+    nc, cs = grid_shape(rad, "torus" if periodic else "box")
+    fresh = M.GridCount2d(nv, (0.0, 0.0), (cs, cs), (nc, nc), M.GRID_MORTON | (M.GRID_PERIODIC if periodic else 0))
+    off2 = torch.zeros(nv + 1, dtype=torch.int32, device="cuda")
+    nbr2 = torch.zeros(nv * 64, dtype=torch.int32, device="cuda")
+    roff2 = torch.zeros(nv + 1, dtype=torch.int32, device="cuda")
+    rnbr2 = torch.zeros(r.dt.m_he_size, dtype=torch.int32, device="cuda")
+    motion = (M.MOTION_SHEAR, 2e-4 * np.sqrt(2 ** 20 / n), 1e-3) if periodic else (M.MOTION_GAUSS, 3e-2, 0.0)
+    late = 0
+    for k in range(6):
+        st = r.step(motion[0], motion[1], motion[2], k)
+        late += st["iterations"] > head
+        torch.cuda.synchronize()
+        fresh.build(r.dt.pos_tensor())
+        tot = fresh.frnn_csr(fresh.sorted_ptr(), rad, off2, nbr2)
+        assert torch.equal(r.frnn[0], off2), "step %d: FRNN offsets differ" % k
+        assert torch.equal(r.frnn[1][:tot], nbr2[:tot]), "step %d: FRNN lists differ" % k
+        tr = r.dt.one_ring_csr(roff2, rnbr2)
+        assert torch.equal(r.ring[0], roff2), "step %d: ring offsets differ" % k
+        assert torch.equal(r.ring[1][:tr], rnbr2[:tr]), "step %d: rings differ" % k
+        assert_at_targets(r.dt, st, "step %d" % k)
+    pos, v, he, t = r.dt.download()
+    assert O_check(oracle, pos, v, he, t, periodic)
+    if head == 1 and periodic:
+        assert late >= 1, "no step settled after the first read"
+
+
+def O_check(O, pos, v, he, t, periodic):
+    rc, msg = O.check_mesh(pos, v, he, t, periodic=periodic, require_ccw=True)
+    assert rc == 0, msg
+    assert O.count_non_delaunay(pos, he, periodic=periodic) == 0
+    return True
