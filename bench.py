@@ -122,6 +122,20 @@ def _cpu_record(value, sample):
             "sample": sample + "; host CPU %s (%d cores online), 1 thread used" % (model, ncpu)}
 
 
+def _keep_distinct(old, new):
+    """the moved points with exact coincidences resolved as the GPU step does:
+    of the points on one spot the smallest index keeps it, the others stay
+    at their previous positions (repeated while that makes new coincidences)"""
+    new = new.copy()
+    for _ in range(8):
+        _, first, inv = np.unique(new.view(np.uint64).ravel(), return_index=True, return_inverse=True)
+        back = first[inv.ravel()] != np.arange(len(new))
+        if not back.any():
+            break
+        new[back] = old[back]
+    return new
+
+
 def cpu_baseline(pos0, cfg, periodic, seconds):
     """The oracle (restated HostTriangulation: walk, f1to3, delonizeVertex,
     exact predicates) rebuilding the exact Delaunay triangulation of the moved
@@ -149,7 +163,7 @@ def cpu_baseline(pos0, cfg, periodic, seconds):
             new = (pos + d.astype(np.float32)) % 1.0
         else:
             new = np.clip(pos + d.astype(np.float32), 0.0, 1.0 - 2.0 ** -24)
-        pos = np.ascontiguousarray(new, np.float32)
+        pos = _keep_distinct(pos, np.ascontiguousarray(new, np.float32))
         if periodic:
             k = L.orc_torus_delaunay(pos.ctypes.data_as(ctypes.c_void_p), n, float(margin),
                                      out.ctypes.data_as(ctypes.c_void_p))
