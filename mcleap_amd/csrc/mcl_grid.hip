@@ -76,6 +76,13 @@ __device__ __forceinline__ int key_of(const GridParams& g, int x, int y) {
 __global__ void k_zero(int* p, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
+// two arrays of n ints each, one launch
+__global__ void k_zero2(int* __restrict__ p, int* __restrict__ q, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        p[i] = 0;
+        q[i] = 0;
+    }
+}
 
 __global__ void __launch_bounds__(kT) k_key_count(GridParams g, const float2* __restrict__ pos, int n,
                                                   int* __restrict__ key, int* __restrict__ count) {
@@ -218,8 +225,7 @@ struct SortBufs {
 // slot of point i.  Asynchronous on s; uses only the given buffers.
 int grid_sort(const GridParams& g, const float2* pos, const float2* vel, int n, const SortBufs& b, hipStream_t s) {
     const int nk = g.nkeys, B = g.block;
-    hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, b.start, nk + 1);
-    hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, b.fill, nk + 1);
+    hipLaunchKernelGGL(k_zero2, dim3(blocks(nk + 1)), dim3(kT), 0, s, b.start, b.fill, nk + 1);
     hipLaunchKernelGGL(k_key_count, dim3(blocks(n, B)), dim3(B), 0, s, g, pos, n, b.key, b.start);
     MCL_TRY(scan_exclusive(b.start, b.start, nk, b.scan_tmp, s));
     hipLaunchKernelGGL(k_slot, dim3(blocks(n, B)), dim3(B), 0, s, b.key, n, b.start, b.fill, b.ind);
@@ -744,8 +750,7 @@ extern "C" int mcl_bgrid_build(mcl_bgrid_t* G, const void* d_pos, void* stream) 
     const int n = G->n, nk = G->nkeys;
     const BGridParams b = G->params();
     G->timers[2].mark(s);
-    hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, G->start, nk + 1);
-    hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, G->fill, nk + 1);
+    hipLaunchKernelGGL(k_zero2, dim3(blocks(nk + 1)), dim3(kT), 0, s, G->start, G->fill, nk + 1);
     hipLaunchKernelGGL(k_bkey_count, dim3(blocks(n)), dim3(kT), 0, s, b, (const float2*)d_pos, G->key, G->start);
     int rc = scan_exclusive(G->start, G->start, nk, G->scan_tmp, s);
     if (rc != MCL_OK) return rc;

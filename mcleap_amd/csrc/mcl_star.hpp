@@ -220,7 +220,9 @@ __device__ __forceinline__ int star_orient_f32(float4 A, float4 B) {
 // inside).  acc: the positions at the start of the step (of vertices found
 // on one point, all but the smallest id go back there, counted as
 // duplicates when that vertex is at its own target tgt, else as stalled;
-// null: they just fail).
+// null: they just fail).  reset (may be null): a counter this launch zeroes
+// for a later kernel.  give_up >= 0: a list longer than that is not
+// processed; the list count *nlist_dev is set to 0 and *gave_up to 1.
 //
 // kL lanes handle one vertex (64: one per wave, up to 128 candidates; 32: two
 // per wave, up to 64 candidates, which holds nearly every 2-ring and halves
@@ -239,7 +241,8 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                                               int* __restrict__ deg_out, int* __restrict__ nbr_out,
                                               int* __restrict__ fail, int* __restrict__ heavy,
                                               const float2* __restrict__ acc, const float2* __restrict__ tgt,
-                                              unsigned long long* __restrict__ stats) {
+                                              unsigned long long* __restrict__ stats, int* __restrict__ reset,
+                                              int give_up, int* __restrict__ gave_up) {
     static_assert(kL == 32 || kL == 64, "lanes per vertex");
     constexpr int kV = 256 / kL;                    // vertices per block iteration
     constexpr int kC = kL == 64 ? kCandMax : 64;    // candidates per vertex
@@ -253,7 +256,17 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
     const int g = threadIdx.x / kL, lane = threadIdx.x % kL;
     const int gbase = (threadIdx.x & 63) - lane;  // first wave lane of this vertex's group
     const unsigned long long gmask = kL == 64 ? ~0ull : (0xffffffffull << gbase);
-    const int n = rows ? *nrows : (vlist ? (nlist_dev ? *nlist_dev : nlist) : m.nv);
+    int n = rows ? *nrows : (vlist ? (nlist_dev ? *nlist_dev : nlist) : m.nv);
+    if (give_up >= 0 && n > give_up) {
+        // every block reads the count before or after block 0 clears it, and
+        // either way takes it as 0
+        n = 0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            *const_cast<int*>(nlist_dev) = 0;
+            *gave_up = 1;
+        }
+    }
+    if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;
     const bool periodic = m.periodic != 0;
     const int max_depth = depth < 2 ? 2 : (depth > kStarMaxDepth ? kStarMaxDepth : depth);
     unsigned loc[4] = {0, 0, 0, 0};
@@ -765,15 +778,17 @@ __device__ __forceinline__ void append_finish(AppendBuf& ab, int* count, int* __
 // with mark_nbrs set, with their star neighbours and any inconsistency marks
 // v, otherwise only the side of a conflict that loses the incircle test
 // (star_inconsistent with `decide`).  Without a list the
-// degree sum of the valid stars is added to *sum_deg.
+// degree sum of the valid stars is added to *sum_deg.  reset (may be null):
+// a counter this launch zeroes for a later kernel.
 __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restrict__ vlist, const int* __restrict__ nlist,
                                                     const int* __restrict__ deg, const int* __restrict__ nbr,
                                                     const float2* __restrict__ pos, int periodic, int ncorner,
                                                     int mark_nbrs, int* __restrict__ mark, const int* __restrict__ stamp_base,
                                                     int stamp_off, int* __restrict__ list, int* __restrict__ count,
-                                                    long long* __restrict__ sum_deg) {
+                                                    long long* __restrict__ sum_deg, int* __restrict__ reset) {
     __shared__ AppendBuf ab;
     const int stamp = *stamp_base + stamp_off;
+    if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;  // a counter zeroed for a later kernel
     append_init(ab);
     const int max_tile = (int)blockDim.x * (mark_nbrs ? kStarCap + 1 : 1);
     long long my_sum = 0;
@@ -819,16 +834,18 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
 // re-checking each recomputed vertex with its previous star neighbours (the
 // only vertices whose consistency the new stars can change: a vertex that
 // has a recomputed vertex in its star without being in that vertex's
-// previous star was inconsistent, hence recomputed itself).
+// previous star was inconsistent, hence recomputed itself).  reset (may be
+// null): a counter this launch zeroes for a later kernel.
 __global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vlist, const int* __restrict__ nlist_dev,
                                                       const int* __restrict__ tdeg, const int* __restrict__ tnbr,
                                                       int* __restrict__ deg, int* __restrict__ nbr,
                                                       int* __restrict__ mark, const int* __restrict__ stamp_base,
                                                       int stamp_off, int* __restrict__ rlist,
                                                       int* __restrict__ rcount, long long* __restrict__ sum_deg,
-                                                      long long* __restrict__ retried) {
+                                                      long long* __restrict__ retried, int* __restrict__ reset) {
     __shared__ AppendBuf ab;
     const int stamp = *stamp_base + stamp_off;
+    if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;  // a counter zeroed for a later kernel
     append_init(ab);
     const int nlist = *nlist_dev;
     if (blockIdx.x == 0 && threadIdx.x == 0 && nlist) {
@@ -867,21 +884,9 @@ __global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vl
     if (delta) atomicAdd((unsigned long long*)sum_deg, (unsigned long long)delta);
 }
 
-// After the first check: when more than limit vertices are inconsistent
-// (motion the star step cannot follow in a few passes), empty the list and
-// flag the step as given up (aux[19]); later passes then do nothing.
 // the mark stamps of one star step: *base + 1 .. *base + kStampSpan - 1
+// (the base advanced by the step's prologue)
 constexpr int kStampSpan = 64;
-__global__ void k_stamp_advance(int* __restrict__ base) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *base += kStampSpan;
-}
-
-__global__ void k_star_give_up(int* __restrict__ aux, int limit) {
-    if (blockIdx.x == 0 && threadIdx.x == 0 && aux[1] > limit) {
-        aux[1] = 0;
-        aux[19] = 1;
-    }
-}
 
 // Corner fans from their neighbours' stars.  For a non-corner x with corner
 // c at position i of its star, the triangles (x, c, s_i+1) and (x, s_i-1, c)

@@ -217,11 +217,33 @@ constexpr int kMotionTargetHalf = 1001;  // internal: half way from the current 
 // position (acc), the target (tgt) and moves the vertex to its target
 // (pdev: the parameters in device memory, read at run time, as a captured
 // graph needs; null: the by-value pv)
+// What a star step's head needs done before its passes, done by block 0 of
+// the move instead of separate launches: zero the statistics shards, the
+// star counters and the heavy-list count, and advance the check stamps.
+// (Empty for the flip-based step.)
+struct StepPrologue {
+    unsigned long long* stats;
+    int nstats;
+    int* aux;
+    int naux;
+    int* heavy;
+    int* stamp;
+    int stamp_span;
+};
+
 __global__ void __launch_bounds__(kThreads) k_move_motion(DMesh m, float2* __restrict__ acc, float2* __restrict__ tgt,
                                                           int* __restrict__ v_mv, int* __restrict__ v_fexp, int stamp,
                                                           mcl_step_params_t pv, const mcl_step_params_t* __restrict__ pdev,
-                                                          const float2* __restrict__ disp) {
+                                                          const float2* __restrict__ disp, StepPrologue pr) {
     const mcl_step_params_t p = pdev ? *pdev : pv;
+    if (blockIdx.x == 0) {
+        for (int i = threadIdx.x; i < pr.nstats; i += blockDim.x) pr.stats[i] = 0;
+        for (int i = threadIdx.x; i < pr.naux; i += blockDim.x) pr.aux[i] = 0;
+        if (threadIdx.x == 0) {
+            if (pr.heavy) *pr.heavy = 0;
+            if (pr.stamp) *pr.stamp += pr.stamp_span;
+        }
+    }
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
         float2 q = m.pos[v];
         acc[v] = q;
@@ -2872,7 +2894,7 @@ static int step_impl(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     const int mv_stamp = ++t->stamp;
     t->timers[2].mark(s);
     hipLaunchKernelGGL(k_move_motion, dim3(blocks_for(m.nv)), dim3(kThreads), 0, s, m, t->acc, t->tgt, t->v_mv,
-                       t->v_fexp, mv_stamp, *p, (const mcl_step_params_t*)nullptr, disp);
+                       t->v_fexp, mv_stamp, *p, (const mcl_step_params_t*)nullptr, disp, StepPrologue{});
     t->timers[2].mark(s);
     MCL_TRY(zero_slots(t, s, kCnBad, 11));  // bad, rev, move, pend, err, misc, touched
     t->timers[1].mark(s);
@@ -3008,65 +3030,87 @@ struct StarGrid {
 // (odeg, onbr): the filter-only kernel, two vertices per wave, then the full
 // kernel on the vertices it could not finish (more than 64 candidates, or a
 // test the filter could not decide)
+// Counters of the star passes (s_aux, and the heavy-list count s_heavy[0])
+// are zeroed by a kernel that runs before their producer and after their
+// last reader, not by launches of their own: the step's prologue (in the
+// move) zeroes all of them, each check zeroes s_heavy for the next pass's
+// stars, each retry's filter kernel zeroes the re-check count [8] for its
+// scatter, and each scatter zeroes the list count its check fills.  The list
+// count alternates between [1] (the first check, and the checks of even
+// retries) and [20] (odd retries), so that a pass's list is not zeroed while
+// that pass still reads its count.
+__host__ __device__ constexpr int star_list_slot(int r) { return (r & 1) ? 20 : 1; }
+
+// stars of the listed vertices (all without a list; count *nlist_dev) into
+// (odeg, onbr): the filter-only kernel, two vertices per wave, then the full
+// kernel on the vertices it could not finish (more than 64 candidates, or a
+// test the filter could not decide).  reset, give_up: as k_star's.
 void star_pass_stars(mcl_tri* t, hipStream_t s, int cur, const int* vlist, const int* nlist_dev, int depth,
-                     const int* bdeg, const int* bnbr, int* odeg, int* onbr) {
+                     const int* bdeg, const int* bnbr, int* odeg, int* onbr, int* reset, int give_up) {
     DMesh& m = t->m;
     const StarGrid G{t};
-    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_heavy, 1, 0);
     if (!vlist)  // first pass: 2-rings read off the previous stars directly
         hipLaunchKernelGGL((k_star<true, true, 32, true>), G.full(m.nv, 8), dim3(256), 0, s, m, vlist, 0, nlist_dev,
                            (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
                            (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy,
-                           (const float2*)t->acc, (const float2*)t->tgt, t->stats);
+                           (const float2*)t->acc, (const float2*)t->tgt, t->stats, reset, give_up, t->s_aux + 19);
     else
         hipLaunchKernelGGL((k_star<true, true, 32, false>), G.list(), dim3(256), 0, s, m, vlist, 0, nlist_dev,
                            (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
                            (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy,
-                           (const float2*)t->acc, (const float2*)t->tgt, t->stats);
+                           (const float2*)t->acc, (const float2*)t->tgt, t->stats, reset, give_up, t->s_aux + 19);
     hipLaunchKernelGGL((k_star<true, false, 64, false>), dim3(256), dim3(256), 0, s, m, vlist, 0, nlist_dev,
                        (const int*)(t->s_heavy + 1), (const int*)t->s_heavy, depth, (const int*)t->star_deg[cur],
                        (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, (int*)nullptr,
-                       (const float2*)t->acc, (const float2*)t->tgt, t->stats);
+                       (const float2*)t->acc, (const float2*)t->tgt, t->stats, (int*)nullptr, -1, (int*)nullptr);
 }
 
-// s_aux: [0] failed stars, [1] listed vertices, [2..3] degree sum of the
-// valid stars, [4..7] failures by reason, [8] vertices to re-check,
-// [10] vertices handed to the full kernel, [12..13] vertices retried,
-// [14] retry passes with a non-empty list, [16..18] corner fans, [19] gave up
+// s_aux: [0] failed stars, [1] / [20] listed vertices (star_list_slot),
+// [2..3] degree sum of the valid stars, [4..7] failures by reason, [8]
+// vertices to re-check, [10] vertices handed to the full kernel, [12..13]
+// vertices retried, [14] retry passes with a non-empty list, [16..18] corner
+// fans, [19] gave up
 //
-// The first check covers every vertex and lists the inconsistent ones; later
-// ones cover the re-check list.  mark_nbrs: list the star neighbours of the
-// inconsistent vertices too (otherwise each conflict lists only the side that
-// loses its incircle test).  stamp_off: this check's offset from the step's
-// stamp base.
-void star_check(mcl_tri* t, hipStream_t s, int nxt, bool first, bool mark_nbrs, int stamp_off) {
+// The first check (r = 0) covers every vertex and lists the inconsistent
+// ones; later ones (retry r) cover the re-check list.  mark_nbrs: list the
+// star neighbours of the inconsistent vertices too (otherwise each conflict
+// lists only the side that loses its incircle test).  stamp_off: this
+// check's offset from the step's stamp base.
+void star_check(mcl_tri* t, hipStream_t s, int nxt, int r, bool mark_nbrs, int stamp_off) {
     DMesh& m = t->m;
     const StarGrid G{t};
-    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 1, 1, 0);
+    const bool first = r == 0;
     // the first check: 4 tiles of 256 vertices per block, its appends
     // flushed once per block
     hipLaunchKernelGGL(k_star_check, first ? G.full(m.nv, 1024) : G.list(), dim3(256), 0, s, m.nv,
                        first ? (const int*)nullptr : (const int*)t->s_rlist,
                        first ? (const int*)nullptr : (const int*)(t->s_aux + 8), (const int*)t->star_deg[nxt],
                        (const int*)t->star_nbr[nxt], (const float2*)m.pos, m.periodic, m.ncorner, mark_nbrs ? 1 : 0,
-                       t->s_mark, (const int*)t->d_stamp, stamp_off, t->s_list, t->s_aux + 1,
-                       (long long*)(t->s_aux + 2));
+                       t->s_mark, (const int*)t->d_stamp, stamp_off, t->s_list, t->s_aux + star_list_slot(r),
+                       (long long*)(t->s_aux + 2), t->s_heavy);
+}
+
+// the most vertices the first check may list before the step gives up on the
+// stars (MCL_OPT_STAR_GIVE_UP percent of them)
+static int star_give_up_limit(const mcl_tri* t) {
+    return (int)std::min<long long>((long long)t->star_give_up * t->m.nv / 100, t->m.nv);
 }
 
 // retry pass r (1 <= r < kStarPasses): recompute the listed vertices from the
 // 2-rings in the previous mesh and in the stars found so far (a vertex whose
 // star failed, having left its 2-ring, searches one ring deeper each pass),
-// copy them in and re-check
+// copy them in and re-check.  The first retry applies the give-up limit to
+// the first check's list.
 void star_retry(mcl_tri* t, hipStream_t s, int cur, int nxt, int r) {
     const StarGrid G{t};
-    star_pass_stars(t, s, cur, t->s_list, t->s_aux + 1, 2 + r, t->star_deg[nxt], t->star_nbr[nxt], t->star_tdeg,
-                    t->star_tnbr);
-    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux + 8, 1, 0);
-    hipLaunchKernelGGL(k_star_scatter, G.list(), dim3(256), 0, s, (const int*)t->s_list, (const int*)(t->s_aux + 1),
+    int* in_count = t->s_aux + star_list_slot(r - 1);
+    star_pass_stars(t, s, cur, t->s_list, in_count, 2 + r, t->star_deg[nxt], t->star_nbr[nxt], t->star_tdeg,
+                    t->star_tnbr, t->s_aux + 8, r == 1 ? star_give_up_limit(t) : -1);
+    hipLaunchKernelGGL(k_star_scatter, G.list(), dim3(256), 0, s, (const int*)t->s_list, (const int*)in_count,
                        (const int*)t->star_tdeg, (const int*)t->star_tnbr, t->star_deg[nxt], t->star_nbr[nxt],
                        t->s_mark, (const int*)t->d_stamp, 2 * r, t->s_rlist, t->s_aux + 8,
-                       (long long*)(t->s_aux + 2), (long long*)(t->s_aux + 12));
-    star_check(t, s, nxt, false, r >= kStarDecidePasses, 2 * r + 1);
+                       (long long*)(t->s_aux + 2), (long long*)(t->s_aux + 12), t->s_aux + star_list_slot(r));
+    star_check(t, s, nxt, r, r >= kStarDecidePasses, 2 * r + 1);
 }
 
 // corner fans from this step's stars (box meshes), into aux[16..18]
@@ -3089,18 +3133,18 @@ void star_corner_fans(mcl_tri* t, hipStream_t s, int cur, int nxt) {
 void star_head(mcl_tri* t, hipStream_t s, int cur, int nxt, const mcl_step_params_t* p,
                const mcl_step_params_t* pdev, const float2* disp, int mv_stamp) {
     DMesh& m = t->m;
-    hipLaunchKernelGGL(k_stamp_advance, dim3(1), dim3(64), 0, s, t->d_stamp);
+    // the statistics, the star counters and the heavy-list count start at
+    // zero, and the check stamps move on (the move's block 0)
+    const StepPrologue pr{t->stats, (int)(kStCount * kStatShards), t->s_aux, kStarAux, t->s_heavy, t->d_stamp,
+                          kStampSpan};
     t->timers[2].mark(s);
     hipLaunchKernelGGL(k_move_motion, dim3(blocks_for(m.nv)), dim3(kThreads), 0, s, m, t->acc, t->tgt, t->v_mv,
-                       t->v_fexp, mv_stamp, *p, pdev, disp);
+                       t->v_fexp, mv_stamp, *p, pdev, disp, pr);
     t->timers[2].mark(s);
-    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->s_aux, kStarAux, 0);
     t->timers[5].mark(s);
-    star_pass_stars(t, s, cur, nullptr, nullptr, 2, nullptr, nullptr, t->star_deg[nxt], t->star_nbr[nxt]);
+    star_pass_stars(t, s, cur, nullptr, nullptr, 2, nullptr, nullptr, t->star_deg[nxt], t->star_nbr[nxt], nullptr, -1);
     t->timers[5].mark(s);
-    star_check(t, s, nxt, true, false, 1);
-    hipLaunchKernelGGL(k_star_give_up, dim3(1), dim3(64), 0, s, t->s_aux,
-                       (int)std::min<long long>((long long)t->star_give_up * m.nv / 100, m.nv));
+    star_check(t, s, nxt, 0, false, 1);
     for (int r = 1; r < t->star_head_passes; r++) star_retry(t, s, cur, nxt, r);
     star_corner_fans(t, s, cur, nxt);
 }
@@ -3231,10 +3275,11 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
             fprintf(stderr,
                     "  [stars pass %d] inconsistent %d (failed stars so far %d: candidates %d undecided %d open %d "
                     "outside %d; full-kernel vertices %d)\n",
-                    pass, aux[1], aux[0], aux[4], aux[5], aux[6], aux[7], aux[10]);
+                    pass, aux[star_list_slot(pass)], aux[0], aux[4], aux[5], aux[6], aux[7], aux[10]);
         return MCL_OK;
     };
     bool settled = false;
+    int listed = 0;
     for (int r = t->star_head_passes - 1; r < kStarPasses; r++) {
         if (r >= t->star_head_passes) {
             star_retry(t, s, cur, nxt, r);
@@ -3242,7 +3287,10 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
             MCL_HIP(hipGetLastError());
         }
         MCL_TRY(read_aux(r, r == t->star_head_passes - 1));
-        if (aux[1] == 0) {
+        listed = aux[star_list_slot(r)];
+        // the give-up limit (applied on the device by the first retry)
+        if (r == 0 && listed > star_give_up_limit(t)) break;
+        if (listed == 0) {
             long long sum_deg, sum_corner = 0;
             memcpy(&sum_deg, aux + 2, sizeof(sum_deg));
             if (m.ncorner) memcpy(&sum_corner, aux + 16, sizeof(sum_corner));
@@ -3255,9 +3303,9 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         }
     }
     if (!settled) t->frnn_done = false;
-    if (!settled && trace && aux[1] > 0) {
+    if (!settled && trace && listed > 0) {
         // the first vertices still listed: id, position, star degree (-1: failed)
-        const int k = std::min(aux[1], 12);
+        const int k = std::min(listed, 12);
         int ids[12];
         MCL_HIP(hipMemcpyAsync(ids, t->s_list, sizeof(int) * k, hipMemcpyDeviceToHost, s));
         MCL_HIP(hipStreamSynchronize(s));
@@ -3301,7 +3349,7 @@ static int star_or_fallback(mcl_tri* t, const mcl_step_params_t* p, const float2
                             bool* done, double* flip_ms = nullptr) {
     hipStream_t s = (hipStream_t)stream;
     auto t0 = std::chrono::steady_clock::now();
-    stats_begin(t, s);
+    t->h_stats_fresh = false;  // (the star head's prologue zeroes the statistics)
     mcl_stats_t tmp{};
     *done = false;
     int rc = star_step(t, p, disp, &tmp, s, done);
