@@ -92,15 +92,19 @@ def test_graph_timesteps_equal_eager(oracle, periodic):
     assert star_steps >= (6 if periodic else 4), star_steps
 
 
-def test_graph_fallback_equals_eager(oracle):
+@pytest.mark.parametrize("head", [4, 1])
+def test_graph_fallback_equals_eager(oracle, head):
     """a star step that gives up (the first check lists more than 1% of the
     vertices) falls back to the flip-based step in both modes, with equal
-    results"""
+    results; the limit is applied on the device by the first retry pass
+    (four head passes) or by the host at its first read (one head pass)"""
     n = 20000
     H = M.HostTriangulation(periodic=True)
     H.addDelaunayPoints(uniform_points(n, 45))
     rad = float(np.float32(2.0 / np.sqrt(n)))
     eager, graph = _Run(H, False, True, rad, give_up=1), _Run(H, True, True, rad, give_up=1)
+    for r in (eager, graph):
+        r.dt.set_option(M.OPT_STAR_HEAD_PASSES, head)
     fallbacks = 0
     for k in range(3):
         se = eager.step(M.MOTION_SHEAR, 1e-3, 1e-3, k)
