@@ -357,7 +357,9 @@ int mcl_grid_index_ptrs(mcl_grid_t* g, void** d_ind, void** d_perm, void** d_cel
  * r2 = dx*dx + dy*dy <= rad*rad (FP32, no contraction).  d_offsets has
  * nump+1 ints, d_nbrs cap ints.  *total = pair count (MCL_ERR_CAPACITY if > cap).
  * With total == NULL the call is asynchronous; entries past cap are dropped
- * and d_offsets[nump] still holds the full count. */
+ * and d_offsets[nump] still holds the full count.  The count pass leaves each
+ * slot's hits in the grid for the fill pass, so queries on one grid (this
+ * call, mcl_tri_timestep's tail) must not overlap in time. */
 int mcl_grid_frnn_csr(mcl_grid_t* g, const void* d_pos, float rad, void* d_offsets, void* d_nbrs,
                       int cap, long long* total, void* stream);
 /* the counting sort of update() without touching the source: the sorted

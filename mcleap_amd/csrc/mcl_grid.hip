@@ -135,11 +135,18 @@ __global__ void __launch_bounds__(kT) k_gather(const int* __restrict__ perm, int
 // kWaveRow ints gathers it in LDS and writes it out with contiguous stores
 // (each thread storing its own short row directly writes partial cache lines,
 // several times the bytes of the list); a longer range is written directly.
+// With hits given, pass 0 also records which candidates (the points of the
+// searched cells other than i, in scan order) are within the radius: bit k
+// for the k-th, bit 63 set when there are more than 63 (then pass 1 computes
+// the distances again); pass 1 then writes the recorded ones without reading
+// any position.
 constexpr int kWaveRow = 1024;
+constexpr unsigned long long kHitsOverflow = 1ull << 63;
 template <int kPass>
 __global__ void __launch_bounds__(kT) k_frnn(GridParams g, const float2* __restrict__ pos, int n,
                                              const int* __restrict__ start, float rad2, int* __restrict__ cnt,
-                                             const int* __restrict__ off, int* __restrict__ nbr, int cap) {
+                                             const int* __restrict__ off, int* __restrict__ nbr, int cap,
+                                             unsigned long long* __restrict__ hits) {
     __shared__ int s_row[kPass ? kT / 64 : 1][kPass ? kWaveRow : 1];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     // block-uniform loop: every thread meets the barriers
@@ -153,12 +160,19 @@ __global__ void __launch_bounds__(kT) k_frnn(GridParams g, const float2* __restr
             wn = off[min(wb + 64, n)] - w0;
         }
         const bool staged = kPass && wn <= kWaveRow;
+        unsigned long long hm = 0;  // pass 0: hits so far; pass 1: hits left to write
+        bool from_hits = false;
+        if (kPass && hits && act) {
+            hm = hits[i];
+            from_hits = !(hm & kHitsOverflow);
+        }
         const float2 pi = act ? pos[i] : make_float2(0.0f, 0.0f);
         const int2 h = cell_of(g, pi);
-        int c = 0, w = (kPass && act) ? off[i] : 0;
+        int c = 0, w = (kPass && act) ? off[i] : 0, nb = 0;
         const int R = g.reach;
         for (int ddx = -R; act && ddx <= R; ddx++) {
             for (int ddy = -R; ddy <= R; ddy++) {
+                if (from_hits && hm == 0) break;
                 int x = h.x + ddx, y = h.y + ddy;
                 if (g.periodic) {
                     x = (x + g.nx) % g.nx;
@@ -171,13 +185,28 @@ __global__ void __launch_bounds__(kT) k_frnn(GridParams g, const float2* __restr
                 }
                 const int k = key_of(g, x, y);
                 const int b = start[k], e = start[k + 1];
+                if (from_hits) {
+                    for (int j = b; j < e && hm; j++) {
+                        if (j == i) continue;
+                        if (hm & 1ull) {
+                            if (staged) s_row[wv][w - w0] = j;
+                            else if (w < cap) nbr[w] = j;
+                            w++;
+                        }
+                        hm >>= 1;
+                    }
+                    continue;
+                }
                 for (int j = b; j < e; j++) {
                     if (j == i) continue;
                     const float2 pj = pos[j];
                     float dx = __fsub_rn(pj.x, pi.x), dy = __fsub_rn(pj.y, pi.y);
                     if (g.periodic) { dx = min_image_f(dx); dy = min_image_f(dy); }
                     const float r2 = __fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy));
-                    if (r2 <= rad2) {
+                    const bool hit = r2 <= rad2;
+                    if (!kPass && hit && nb < 63) hm |= 1ull << nb;
+                    nb++;
+                    if (hit) {
                         if (kPass) {
                             if (staged) s_row[wv][w - w0] = j;
                             else if (w < cap) nbr[w] = j;
@@ -188,7 +217,10 @@ __global__ void __launch_bounds__(kT) k_frnn(GridParams g, const float2* __restr
                 }
             }
         }
-        if (!kPass && act) cnt[i] = c;
+        if (!kPass && act) {
+            cnt[i] = c;
+            if (hits) hits[i] = nb <= 63 ? hm : kHitsOverflow;
+        }
         if (kPass) {
             __syncthreads();
             if (staged)
@@ -239,7 +271,8 @@ int grid_sort(const GridParams& g, const float2* pos, const float2* vel, int n, 
 // over sorted positions spos; total == NULL: no host read, writes past cap
 // dropped; else *total = the pair count after a synchronisation
 int frnn_csr(const GridParams& g0, const float2* spos, int n, const int* start, float rad, int* off, int* nbrs,
-             int cap, long long* total, int* scan_tmp, hipStream_t s, KTimer* tc, KTimer* tf) {
+             int cap, long long* total, int* scan_tmp, hipStream_t s, KTimer* tc, KTimer* tf,
+             unsigned long long* hits = nullptr) {
     const float rad2 = rad * rad;
     // cells searched on each side: one for the reference's row-major grid
     // (gridCount2d.h:272-282, whatever the radius); with the Morton or
@@ -254,14 +287,14 @@ int frnn_csr(const GridParams& g0, const float2* spos, int n, const int* start, 
     const int B = g.block;
     if (tc) tc->mark(s);
     hipLaunchKernelGGL(k_frnn<0>, dim3(blocks(n, B)), dim3(B), 0, s, g, spos, n, start, rad2, off, (const int*)nullptr,
-                       (int*)nullptr, 0);
+                       (int*)nullptr, 0, hits);
     if (tc) tc->mark(s);
     MCL_TRY(scan_exclusive(off, off, n, scan_tmp, s));
     if (!total) {
         if (tf) tf->mark(s);
         if (nbrs && cap > 0)
             hipLaunchKernelGGL(k_frnn<1>, dim3(blocks(n, B)), dim3(B), 0, s, g, spos, n, start, rad2, (int*)nullptr,
-                               (const int*)off, nbrs, cap);
+                               (const int*)off, nbrs, cap, hits);
         if (tf) tf->mark(s);
         MCL_HIP(hipGetLastError());
         return MCL_OK;
@@ -272,7 +305,7 @@ int frnn_csr(const GridParams& g0, const float2* spos, int n, const int* start, 
     *total = tot;
     if (nbrs && cap > 0) {
         hipLaunchKernelGGL(k_frnn<1>, dim3(blocks(n, B)), dim3(B), 0, s, g, spos, n, start, rad2, (int*)nullptr,
-                           (const int*)off, nbrs, cap);
+                           (const int*)off, nbrs, cap, hits);
         MCL_HIP(hipGetLastError());
         MCL_HIP(hipStreamSynchronize(s));
     }
@@ -311,12 +344,14 @@ struct mcl_grid {
     float2* spos = nullptr;
     float2* svel = nullptr;
     unsigned long long* scalar = nullptr;
+    unsigned long long* hits = nullptr;  // nump: the FRNN count pass's hit masks, read by its fill pass
     mcl::KTimer timers[3];  // FRNN count, FRNN fill, build
     SortBufs bufs() const { return {key, start, fill, perm, ind, scan_tmp, spos, svel}; }
 };
 
 static void grid_free(mcl_grid* G) {
-    void* p[] = {G->key, G->start, G->fill, G->perm, G->ind, G->cnt, G->scan_tmp, G->spos, G->svel, G->scalar};
+    void* p[] = {G->key, G->start, G->fill, G->perm, G->ind, G->cnt, G->scan_tmp, G->spos, G->svel, G->scalar,
+                 G->hits};
     for (void* q : p)
         if (q) (void)hipFree(q);
 }
@@ -341,7 +376,8 @@ extern "C" int mcl_grid_create(int nump, float minx, float miny, float csx, floa
               hipMalloc(&G->scan_tmp, sizeof(int) * (scan_scratch_ints((nk > nump ? nk : nump) + 1) + 16)) == hipSuccess &&
               hipMalloc(&G->spos, sizeof(float2) * (nump + 1)) == hipSuccess &&
               hipMalloc(&G->svel, sizeof(float2) * (nump + 1)) == hipSuccess &&
-              hipMalloc(&G->scalar, sizeof(unsigned long long) * 2) == hipSuccess;
+              hipMalloc(&G->scalar, sizeof(unsigned long long) * 2) == hipSuccess &&
+              hipMalloc(&G->hits, sizeof(unsigned long long) * (nump + 1)) == hipSuccess;
     if (!ok) { grid_free(G); delete G; return MCL_ERR_HIP; }
     hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, 0, G->start, nk + 1);
     if (hipDeviceSynchronize() != hipSuccess) { grid_free(G); delete G; return MCL_ERR_HIP; }
@@ -399,7 +435,7 @@ extern "C" int mcl_grid_frnn_csr(mcl_grid_t* G, const void* d_pos, float rad, vo
                                  long long* total, void* stream) {
     if (!G || !d_pos || !d_offsets) return MCL_ERR_INPUT;
     return frnn_csr(G->g, (const float2*)d_pos, G->nump, G->start, rad, (int*)d_offsets, (int*)d_nbrs, cap, total,
-                    G->scan_tmp, (hipStream_t)stream, &G->timers[0], &G->timers[1]);
+                    G->scan_tmp, (hipStream_t)stream, &G->timers[0], &G->timers[1], G->hits);
 }
 
 // the counting sort of pos into the grid and the CSR query over the sorted
@@ -412,7 +448,7 @@ int mcl::grid_enqueue_frnn(mcl_grid_t* G, const float2* pos, float rad, int* off
     MCL_TRY(grid_sort(G->g, pos, nullptr, G->nump, G->bufs(), s));
     G->timers[2].mark(s);
     return frnn_csr(G->g, G->spos, G->nump, G->start, rad, off, nbrs, cap, nullptr, G->scan_tmp, s, &G->timers[0],
-                    &G->timers[1]);
+                    &G->timers[1], G->hits);
 }
 
 // ------------------------------------------------------------------------
