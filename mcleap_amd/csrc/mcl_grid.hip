@@ -149,8 +149,9 @@ __global__ void __launch_bounds__(kT) k_frnn(GridParams g, const float2* __restr
                                              unsigned long long* __restrict__ hits) {
     __shared__ int s_row[kPass ? kT / 64 : 1][kPass ? kWaveRow : 1];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    // block-uniform loop: every thread meets the barriers
-    for (int bb = blockIdx.x * blockDim.x; bb < n; bb += gridDim.x * blockDim.x) {
+    // block-uniform loop: every thread meets the barriers; each XCD takes one
+    // contiguous range of the sorted points
+    for (int bb = xcd_block() * blockDim.x; bb < n; bb += gridDim.x * blockDim.x) {
         const int i = bb + threadIdx.x;
         const bool act = i < n;
         const int wb = bb + (threadIdx.x & ~63);  // this wave's first point
@@ -641,8 +642,9 @@ __global__ void __launch_bounds__(kT) k_bfrnn(BGridParams b, const float2* __res
                                               int cap) {
     __shared__ int s_row[kPass ? kT / 64 : 1][kPass ? kWaveRow : 1];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    // block-uniform loop: every thread meets the barriers
-    for (int bb = blockIdx.x * blockDim.x; bb < b.n; bb += gridDim.x * blockDim.x) {
+    // block-uniform loop: every thread meets the barriers; each XCD takes one
+    // contiguous range of the sorted points
+    for (int bb = xcd_block() * blockDim.x; bb < b.n; bb += gridDim.x * blockDim.x) {
         const int i = bb + threadIdx.x;
         const bool act = i < b.n;
         const int wb = bb + (threadIdx.x & ~63);

@@ -175,6 +175,16 @@ struct KTimer {
 // (all block-uniform); falls back to the plain grid-stride split when the
 // grid is not a multiple of 8.
 constexpr int kXcds = 8;
+// Logical block index for a grid-stride loop: consecutive blocks are dealt
+// round-robin to the XCDs, so with gridDim.x a multiple of kXcds this maps the
+// blocks of one XCD to one contiguous range (its L2 then holds that part of
+// the data and its halo).
+__device__ __forceinline__ int xcd_block() {
+    const int nb = gridDim.x;
+    if (nb % kXcds != 0) return blockIdx.x;
+    return (blockIdx.x % kXcds) * (nb / kXcds) + blockIdx.x / kXcds;
+}
+
 // ints of LDS per wave for staging a wave's CSR one-ring rows
 constexpr int kRingWave = 1024;
 
