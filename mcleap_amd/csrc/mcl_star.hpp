@@ -223,6 +223,8 @@ __device__ __forceinline__ int star_orient_f32(float4 A, float4 B) {
 // null: they just fail).  reset (may be null): a counter this launch zeroes
 // for a later kernel.  give_up >= 0: a list longer than that is not
 // processed; the list count *nlist_dev is set to 0 and *gave_up to 1.
+// hint (may be null; retries): a further first-ring candidate per vertex
+// (-1: none), left by the consistency check.
 //
 // kL lanes handle one vertex (64: one per wave, up to 128 candidates; 32: two
 // per wave, up to 64 candidates, which holds nearly every 2-ring and halves
@@ -242,7 +244,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                                               int* __restrict__ fail, int* __restrict__ heavy,
                                               const float2* __restrict__ acc, const float2* __restrict__ tgt,
                                               unsigned long long* __restrict__ stats, int* __restrict__ reset,
-                                              int give_up, int* __restrict__ gave_up) {
+                                              int give_up, int* __restrict__ gave_up, const int* __restrict__ hint) {
     static_assert(kL == 32 || kL == 64, "lanes per vertex");
     constexpr int kV = 256 / kL;                    // vertices per block iteration
     constexpr int kC = kL == 64 ? kCandMax : 64;    // candidates per vertex
@@ -352,6 +354,11 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                 for (int i = lane; i < db && i < kStarCap; i += kL) {
                     const int ni = nbr_b[(size_t)v * kStarCap + i];
                     if ((unsigned)ni < (unsigned)m.nv) cand_insert<kH, kC>(tab, list, count, ni);
+                }
+                // a vertex whose star lists v while v's did not (left by a check)
+                if (hint && lane == 0) {
+                    const int hv = hint[v];
+                    if ((unsigned)hv < (unsigned)m.nv && hv != v) cand_insert<kH, kC>(tab, list, count, hv);
                 }
             }
             __syncthreads();
@@ -640,11 +647,13 @@ __device__ __forceinline__ int star_conflict(const float2* __restrict__ pos, int
 // (star_conflict), the other vertex being marked by its own check.  A vertex
 // of degree <= 8 whose neighbours have degree <= 8 (nearly all) is checked
 // with every load issued up front: its row, the neighbours' degrees, the
-// neighbours' first 8 entries.
+// neighbours' first 8 entries.  *missing: a neighbour whose star lacks v
+// (-1 if none found).
 __device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, const int* __restrict__ nbr, int v,
                                                   int d, bool decide, const float2* __restrict__ pos,
-                                                  bool periodic, int ncorner) {
+                                                  bool periodic, int ncorner, int* missing) {
     const int* row = nbr + (size_t)v * kStarCap;
+    *missing = -1;
     if (d <= 8) {
         const int4* rv = reinterpret_cast<const int4*>(row);
         const int4 q0 = rv[0], q1 = rv[1];
@@ -685,6 +694,7 @@ __device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, c
                 if (k < n && e[k] == v) pred = k > 0 ? e[k - 1] : last;
             pv[i] = pred;
             if (pred == w) continue;
+            if (pred < 0 && *missing < 0) *missing = r[i];
             if (!decide || pred < 0) bad = true;
             else conf |= 1u << i;
         }
@@ -718,6 +728,7 @@ __device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, c
         if (du < 3) return true;
         const int j = star_find(nbr, u, du, v);
         const int pred = j < 0 ? -1 : nbr[(size_t)u * kStarCap + (j == 0 ? du - 1 : j - 1)];
+        if (j < 0) *missing = u;
         if (pred != w && (!decide || pred < 0 || star_conflict(pos, v, u, w, pred, periodic) != 0)) return true;
         u = w;
     }
@@ -785,12 +796,13 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
                                                     const float2* __restrict__ pos, int periodic, int ncorner,
                                                     int mark_nbrs, int* __restrict__ mark, const int* __restrict__ stamp_base,
                                                     int stamp_off, int* __restrict__ list, int* __restrict__ count,
-                                                    long long* __restrict__ sum_deg, int* __restrict__ reset) {
+                                                    long long* __restrict__ sum_deg, int* __restrict__ reset,
+                                                    int* __restrict__ hint) {
     __shared__ AppendBuf ab;
     const int stamp = *stamp_base + stamp_off;
     if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;  // a counter zeroed for a later kernel
     append_init(ab);
-    const int max_tile = (int)blockDim.x * (mark_nbrs ? kStarCap + 1 : 1);
+    const int max_tile = (int)blockDim.x * (mark_nbrs ? kStarCap + 1 : 2);
     long long my_sum = 0;
     const int n = vlist ? *nlist : nv;
     int first, end, stride;
@@ -803,12 +815,18 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
         const int d = active ? deg[v] : 3;
         const int* row = nbr + (size_t)v * kStarCap;
         bool bad = active && d < 3;
+        int missing = -1;
         if (active && !bad) {
             if (!vlist) my_sum += d;
-            bad = star_inconsistent(deg, nbr, v, d, !mark_nbrs, pos, periodic != 0, ncorner);
+            bad = star_inconsistent(deg, nbr, v, d, !mark_nbrs, pos, periodic != 0, ncorner, &missing);
         }
-        // claim v and its neighbours once per stamp, then append the claimed ones
-        unsigned claimed = 0;  // bit 0: v, bit 1 + i: neighbour i
+        // claim v and its neighbours once per stamp, then append the claimed
+        // ones.  With hint given, a neighbour u whose star lacks v is listed
+        // too, with v left as a candidate for it: when v has moved into u's star from
+        // beyond u's 2-ring, and u's other neighbours (recomputed without v
+        // as well) agree with u, no other check lists u, and u's 2-ring in the
+        // new stars does not reach v either.
+        unsigned claimed = 0;  // bit 0: v, bit 1 + i: neighbour i, bit 31: the one missing v
         int nclaim = 0;
         if (bad) {
             if (atomicExch(&mark[v], stamp) != stamp) { claimed |= 1u; nclaim++; }
@@ -817,11 +835,16 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
                     claimed |= 2u << i;
                     nclaim++;
                 }
+            if (hint && missing >= ncorner) {
+                hint[missing] = v;
+                if (atomicExch(&mark[missing], stamp) != stamp) { claimed |= 1u << 31; nclaim++; }
+            }
         }
         int at = append_reserve(ab, nclaim);
         if (claimed & 1u) ab.buf[at++] = v;
         for (int i = 0; i < d && i < kStarCap; i++)
             if (claimed & (2u << i)) ab.buf[at++] = row[i];
+        if (claimed & (1u << 31)) ab.buf[at++] = missing;
         append_tile_end(ab, max_tile, count, list);
     }
     append_finish(ab, count, list);

@@ -229,6 +229,7 @@ struct StepPrologue {
     int* heavy;
     int* stamp;
     int stamp_span;
+    int* hint;  // per vertex, set to -1 (the checks' candidate hints hold for one step)
 };
 
 __global__ void __launch_bounds__(kThreads) k_move_motion(DMesh m, float2* __restrict__ acc, float2* __restrict__ tgt,
@@ -245,6 +246,7 @@ __global__ void __launch_bounds__(kThreads) k_move_motion(DMesh m, float2* __res
         }
     }
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m.nv; v += gridDim.x * blockDim.x) {
+        if (pr.hint) pr.hint[v] = -1;
         float2 q = m.pos[v];
         acc[v] = q;
         if (is_corner(q)) { tgt[v] = q; continue; }
@@ -1756,6 +1758,7 @@ struct mcl_tri {
     int* s_rlist = nullptr;                 // nv: vertices to re-check
     int* s_aux = nullptr;                   // kStarAux counters (see star_step)
     int* s_heavy = nullptr;                 // nv + 1: count, then rows for the full star kernel
+    int* s_hint = nullptr;                  // nv: a further candidate per vertex, left by the checks
     int* c_bucket = nullptr;                // ncorner * kCornerCap * 3: corner fan entries
     int* c_count = nullptr;                 // ncorner
     int* d_stamp = nullptr;                 // 1: base of the star checks' mark stamps (advanced on the device)
@@ -1820,7 +1823,7 @@ static int tri_free(mcl_tri* t) {
                     t->scalar, t->acc, t->tgt, t->v_mv, t->v_rev, t->v_fexp, t->t_stamp, t->bad[0], t->bad[1],
                     t->rev, t->allrev, t->pend[0], t->pend[1], t->deg, t->scan_tmp, t->v_touch, t->t_rep, t->t_rep_e, t->e_undo, t->undo, t->stuck, t->touched, t->r_tt, t->r_tt2, t->r_wait, t->r_defer, t->v_soft, t->tail_out,
                     t->star_deg[0], t->star_deg[1], t->star_nbr[0], t->star_nbr[1], t->s_ecnt, t->s_tcnt,
-                    t->s_mark, t->s_list, t->s_aux, t->star_tdeg, t->star_tnbr, t->s_rlist, t->s_heavy,
+                    t->s_mark, t->s_list, t->s_aux, t->star_tdeg, t->star_tnbr, t->s_rlist, t->s_heavy, t->s_hint,
                     t->c_bucket, t->c_count, t->d_stamp, t->d_params};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -1937,7 +1940,7 @@ int tri_make(int nv, int nhe, int nt, int periodic, int ncorner, mcl_tri** out) 
         A(t->star_nbr[0], star_rows_size(nv, m.ncorner)); A(t->star_nbr[1], star_rows_size(nv, m.ncorner));
         A(t->star_tdeg, nv); A(t->star_tnbr, star_rows_size(nv, m.ncorner));
         A(t->s_ecnt, nv + 1); A(t->s_tcnt, nv + 1); A(t->s_mark, nv); A(t->s_list, nv); A(t->s_aux, kStarAux);
-        A(t->s_rlist, nv); A(t->s_heavy, nv + 1); A(t->d_stamp, 1); A(t->d_params, 1);
+        A(t->s_rlist, nv); A(t->s_heavy, nv + 1); A(t->s_hint, nv); A(t->d_stamp, 1); A(t->d_params, 1);
         if (m.ncorner > 0) { A(t->c_bucket, (size_t)m.ncorner * kCornerCap * 3); A(t->c_count, m.ncorner); }
     }
 #undef A
@@ -3053,16 +3056,19 @@ void star_pass_stars(mcl_tri* t, hipStream_t s, int cur, const int* vlist, const
         hipLaunchKernelGGL((k_star<true, true, 32, true>), G.full(m.nv, 8), dim3(256), 0, s, m, vlist, 0, nlist_dev,
                            (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
                            (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy,
-                           (const float2*)t->acc, (const float2*)t->tgt, t->stats, reset, give_up, t->s_aux + 19);
+                           (const float2*)t->acc, (const float2*)t->tgt, t->stats, reset, give_up, t->s_aux + 19,
+                           (const int*)nullptr);
     else
         hipLaunchKernelGGL((k_star<true, true, 32, false>), G.list(), dim3(256), 0, s, m, vlist, 0, nlist_dev,
                            (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
                            (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy,
-                           (const float2*)t->acc, (const float2*)t->tgt, t->stats, reset, give_up, t->s_aux + 19);
+                           (const float2*)t->acc, (const float2*)t->tgt, t->stats, reset, give_up, t->s_aux + 19,
+                           (const int*)t->s_hint);
     hipLaunchKernelGGL((k_star<true, false, 64, false>), dim3(256), dim3(256), 0, s, m, vlist, 0, nlist_dev,
                        (const int*)(t->s_heavy + 1), (const int*)t->s_heavy, depth, (const int*)t->star_deg[cur],
                        (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, (int*)nullptr,
-                       (const float2*)t->acc, (const float2*)t->tgt, t->stats, (int*)nullptr, -1, (int*)nullptr);
+                       (const float2*)t->acc, (const float2*)t->tgt, t->stats, (int*)nullptr, -1, (int*)nullptr,
+                       vlist ? (const int*)t->s_hint : (const int*)nullptr);
 }
 
 // s_aux: [0] failed stars, [1] / [20] listed vertices (star_list_slot),
@@ -3072,7 +3078,10 @@ void star_pass_stars(mcl_tri* t, hipStream_t s, int cur, const int* vlist, const
 // fans, [19] gave up
 //
 // The first check (r = 0) covers every vertex and lists the inconsistent
-// ones; later ones (retry r) cover the re-check list.  mark_nbrs: list the
+// ones; later ones (retry r) cover the re-check list, and also list a
+// neighbour whose star lacks the vertex, with the vertex as its candidate
+// (only there: at the first check most such pairs settle without it, and
+// listing them all costs a third more recomputations).  mark_nbrs: list the
 // star neighbours of the inconsistent vertices too (otherwise each conflict
 // lists only the side that loses its incircle test).  stamp_off: this
 // check's offset from the step's stamp base.
@@ -3087,7 +3096,7 @@ void star_check(mcl_tri* t, hipStream_t s, int nxt, int r, bool mark_nbrs, int s
                        first ? (const int*)nullptr : (const int*)(t->s_aux + 8), (const int*)t->star_deg[nxt],
                        (const int*)t->star_nbr[nxt], (const float2*)m.pos, m.periodic, m.ncorner, mark_nbrs ? 1 : 0,
                        t->s_mark, (const int*)t->d_stamp, stamp_off, t->s_list, t->s_aux + star_list_slot(r),
-                       (long long*)(t->s_aux + 2), t->s_heavy);
+                       (long long*)(t->s_aux + 2), t->s_heavy, first ? (int*)nullptr : t->s_hint);
 }
 
 // the most vertices the first check may list before the step gives up on the
@@ -3136,7 +3145,7 @@ void star_head(mcl_tri* t, hipStream_t s, int cur, int nxt, const mcl_step_param
     // the statistics, the star counters and the heavy-list count start at
     // zero, and the check stamps move on (the move's block 0)
     const StepPrologue pr{t->stats, (int)(kStCount * kStatShards), t->s_aux, kStarAux, t->s_heavy, t->d_stamp,
-                          kStampSpan};
+                          kStampSpan, t->s_hint};
     t->timers[2].mark(s);
     hipLaunchKernelGGL(k_move_motion, dim3(blocks_for(m.nv)), dim3(kThreads), 0, s, m, t->acc, t->tgt, t->v_mv,
                        t->v_fexp, mv_stamp, *p, pdev, disp, pr);
@@ -3374,8 +3383,9 @@ static int star_or_fallback(mcl_tri* t, const mcl_step_params_t* p, const float2
 }
 
 // Path choice.  MCL_OPT_STAR_ENABLE = 2 (default): each step takes the path
-// whose last step was faster (host clock over the whole step, a star step
-// that fell back counting its attempt and the flip-based step after it); the
+// that measured faster (time on the stream over the step's own work, a star
+// step that fell back counting its attempt and the flip-based step after it;
+// the star path's time averaged over its steps, the flip path's last); the
 // other path is measured again after kPathProbe x (slow / fast) steps, so
 // that probing costs at most 1 / kPathProbe of the run, and twice as late
 // each time the measurement keeps the choice (up to 4096 steps); a change in
@@ -3437,7 +3447,13 @@ static int kinetic_step(mcl_tri* t, const mcl_step_params_t* p, const float2* di
             const bool keeps = ms > t->path_ms[1 - path];
             t->path_probe = keeps ? std::min(2 * t->path_probe, 4096) : kPathProbe;
         }
-        t->path_ms[path] = ms;
+        // the star path's cost is averaged over its steps (weight 0.2 per
+        // step): a rare step that falls back to flips (attempt + flip step)
+        // does not turn the choice for the next ~64 x (slow / fast) steps,
+        // while a star step that keeps falling back soon costs its attempt
+        // plus the flip step
+        if (path == 1 && t->path_ms[1] > 0.0) t->path_ms[1] = 0.8 * t->path_ms[1] + 0.2 * ms;
+        else t->path_ms[path] = ms;
         if (flip_ms >= 0.0) t->path_ms[0] = flip_ms;  // a fallback is also a flip-step measurement
         if (t->path_ms[0] > 0.0 && t->path_ms[1] > 0.0) {
             // a probe costs (slow - fast); after kPathProbe * slow / fast steps

@@ -67,3 +67,23 @@ def test_c3_clustered_4m(oracle):
     dt = M.DeviceTriangulation.from_points(xy)
     _steps(dt, "c3", 2)
     _check(oracle, dt, False)
+
+
+def test_c2_star_steps_settle_for_240_steps():
+    """The 1M-point torus workload by star recomputation for 240 steps: every
+    step settles without falling back to flips.  Step 215 of this run once
+    did not: a vertex moved into the triangle of three vertices beyond its
+    neighbours' 2-rings, whose stars (computed without it) agreed with each
+    other, so only that vertex was ever listed; the check now also lists a
+    neighbour whose star lacks the vertex, with the vertex as a candidate."""
+    c = CONFIGS["c2"]
+    dt = M.DeviceTriangulation.from_points(initial_points("c2"), periodic=True)
+    dt.set_option(M.OPT_STAR_ENABLE, 1)
+    fallbacks = []
+    for k in range(240):
+        st = dt.step(c["motion"], c["sigma"], c["amp"], seed=c["seed"], step=k)
+        if st["fallbacks"]:
+            fallbacks.append(k)
+    assert fallbacks == [], fallbacks
+    assert_at_targets(dt, st, "step 239")
+    assert dt.count_non_delaunay() == 0 and dt.count_inverted() == 0
