@@ -11,11 +11,13 @@
 // candidate (the farthest inverted point, always a hull vertex); the turn test
 // orient(a', b', c') has the sign of incircle(a, b, c, v) (divide the rows of
 // the lifted determinant by |p - v|^2), so every decision is the library's
-// exact incircle with the symbolic tie rule.  One wavefront handles one
-// vertex: each lane finds, for the candidates it owns (two at most), the most
-// clockwise other candidate as seen from it (its hull successor when it is a
-// hull vertex), and the star is then traced through those successors from the
-// nearest candidate.
+// exact incircle with the symbolic tie rule.  A group of lanes (a half wave
+// or a whole one) handles one vertex.  With at most one candidate per lane the
+// hull is found by elimination in angle order and certified with the filter
+// (k_star); otherwise, or when the certification fails, each lane finds, for
+// the candidates it owns (two at most), the most clockwise other candidate as
+// seen from it (its hull successor when it is a hull vertex), and the star is
+// traced through those successors from the nearest candidate.
 //
 // If every triangle (v, s_i, s_i+1) of every star also appears in the stars of
 // its other two vertices, the stars form a triangulation whose every edge is
@@ -202,6 +204,56 @@ __device__ __forceinline__ int star_orient_f32(float4 A, float4 B) {
     return 0;
 }
 
+// Lane masks of one group of kL lanes (a half wave or a whole one) for the
+// hull elimination: the group's bits of a wave ballot, the nearest set bit
+// cyclically before and after a lane (the lane itself when none is set).
+template <int kL> struct HullMask;
+template <> struct HullMask<32> {
+    using type = unsigned;
+    static __device__ __forceinline__ unsigned group(unsigned long long b, int gbase) { return (unsigned)(b >> gbase); }
+    static __device__ __forceinline__ int popc(unsigned m) { return __popc(m); }
+    static __device__ __forceinline__ int lowest(unsigned m) { return __ffs(m) - 1; }
+    static __device__ __forceinline__ int before(unsigned m, unsigned lo, int lane) {
+        const unsigned b = m & lo;
+        return m ? (b ? 31 - __clz(b) : 31 - __clz(m)) : lane;
+    }
+    static __device__ __forceinline__ int after(unsigned m, unsigned hi, int lane) {
+        const unsigned a = m & hi;
+        return m ? (a ? __ffs(a) - 1 : __ffs(m) - 1) : lane;
+    }
+};
+template <> struct HullMask<64> {
+    using type = unsigned long long;
+    static __device__ __forceinline__ unsigned long long group(unsigned long long b, int) { return b; }
+    static __device__ __forceinline__ int popc(unsigned long long m) { return __popcll(m); }
+    static __device__ __forceinline__ int lowest(unsigned long long m) { return __ffsll((long long)m) - 1; }
+    static __device__ __forceinline__ int before(unsigned long long m, unsigned long long lo, int lane) {
+        const unsigned long long b = m & lo;
+        return m ? (b ? 63 - __clzll(b) : 63 - __clzll(m)) : lane;
+    }
+    static __device__ __forceinline__ int after(unsigned long long m, unsigned long long hi, int lane) {
+        const unsigned long long a = m & hi;
+        return m ? (a ? __ffsll((long long)a) - 1 : __ffsll((long long)m) - 1) : lane;
+    }
+};
+
+// incircle(a, b, c, v) from the filter rows A, B, C of a - v, b - v, c - v in
+// the successor scan's evaluation order (minors of the x, y columns, expanded
+// along the lifts with FMAs): det has the sign of orient(a', b', c') of the
+// inverted points, and is that sign exactly when sure (kStarIccBound covers
+// this order: about 11 eps of the permanent from input, product, difference
+// and FMA roundings).
+__device__ __forceinline__ bool star_icc_sure(float4 A, float4 B, float4 C, float& det) {
+    const float axby = A.x * B.y, bxay = B.x * A.y;
+    const float bxcy = B.x * C.y, cxby = C.x * B.y;
+    const float cxay = C.x * A.y, axcy = A.x * C.y;
+    const float m1 = bxcy - cxby, m2 = cxay - axcy, m3 = axby - bxay;
+    const float s1 = fabsf(bxcy) + fabsf(cxby), s2 = fabsf(cxay) + fabsf(axcy), s3 = fabsf(axby) + fabsf(bxay);
+    det = fmaf(A.z, m1, fmaf(B.z, m2, C.z * m3));
+    const float perm = fmaf(A.z, s1, fmaf(B.z, s2, C.z * s3));
+    return (perm > 7.9e-31f) & (fabsf(det) > kStarIccBound * perm);
+}
+
 // Recomputes the stars of the listed vertices (all vertices when vlist is
 // null) with the current positions.  Candidates: the 2-ring of v in the star
 // arrays deg_a/nbr_a (the previous mesh).  With deg_b/nbr_b (the stars found
@@ -254,6 +306,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
     __shared__ float2 s_pos[kV][kC];
     __shared__ float4 s_row[kV][kC];  // (x, y, lift) of candidate - v, candidate id (bits)
     __shared__ int s_succ[kV][kC];
+    __shared__ __align__(16) int s_key[kV][kL];  // angle keys of the hull elimination
     __shared__ int s_cnt[kV];
     const int g = threadIdx.x / kL, lane = threadIdx.x % kL;
     const int gbase = (threadIdx.x & 63) - lane;  // first wave lane of this vertex's group
@@ -457,15 +510,144 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
         const int start = best;
         if (start >= kc && ok) { ok = false; why = 4; }
         const int id_start = ok ? __float_as_int(s_row[g][start].w) : -1;
+        // ---- the hull by elimination (filter kernel, at most 32 candidates).
+        // Lane r takes the candidate of rank r in angle around v (the
+        // inversion keeps directions from v).  Rounds drop every remaining
+        // candidate whose inverted point is not a strictly convex corner of
+        // the chain of the remaining ones: it then lies in the triangle of the
+        // origin and its two chain neighbours, inside the hull.  The rounds
+        // use plain FP32 on inverted points and prove nothing.  What they leave
+        // is certified with the filter: every corner strictly convex
+        // (incircle), every wedge (v, h_i, h_i+1) strictly CCW (orient) with
+        // one wedge crossing the ray y = 0, x > 0 (one turn around v), and
+        // every other candidate strictly inside the triangle of the origin and
+        // the corners of its wedge.  The corners are then the hull of the
+        // inverted candidates and every other candidate lies strictly inside
+        // it: the star the successor scan below traces, written from the same
+        // start.  A vertex that fails any of this takes the scan.
+        bool hull_done = false;
+        int h_pos = -1, h_deg = 0, h_id = -1;
+        if (kFast || kL == 64) {
+            using Mask = typename HullMask<kL>::type;
+            const bool try_hull = ok && kc <= kL;
+            const int kc_h = try_hull ? kc : 0;
+            // own candidate's key: a pseudo-angle in [-1, 3) as a monotone
+            // int, the slot in its low bits (unique keys)
+            constexpr int kSlotBits = kL == 64 ? 6 : 5;
+            int key = 0x7fffffff;
+            if (lane < kc_h) {
+                const float4 R = s_row[g][lane];
+                const float q = R.y * __builtin_amdgcn_rcpf(fabsf(R.x) + fabsf(R.y));
+                const int b = __float_as_int(R.x >= 0.0f ? q : 2.0f - q);
+                key = ((b >= 0 ? b : b ^ 0x7fffffff) & ~(kL - 1)) | lane;
+            }
+            static_assert((1 << kSlotBits) == kL, "slot bits");
+            s_key[g][lane] = key;
+            // (LDS operations of one wave complete in order; the fences keep
+            // the compiler from moving the reads above the write)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            int kc_w = kc_h;
+            if (kL == 32) kc_w = max(kc_w, __shfl_xor(kc_w, 32));
+            kc_w = __builtin_amdgcn_readfirstlane(kc_w);
+            int rank = 0;
+            for (int i = 0; i < kc_w; i += 4) {
+                const int4 k4 = *reinterpret_cast<const int4*>(&s_key[g][i]);
+                rank += (k4.x < key) + (k4.y < key) + (k4.z < key) + (k4.w < key);
+            }
+            if (lane >= kc_h) rank = lane;
+            // forward permute: lane gbase + rank receives this lane's slot
+            const int slot = __builtin_amdgcn_ds_permute((gbase + rank) << 2, lane);
+            const bool in = lane < kc_h;
+            float4 R = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            float ix = 0.0f, iy = 0.0f;
+            if (in) {
+                R = s_row[g][slot];
+                const float rz = __builtin_amdgcn_rcpf(R.z);
+                ix = R.x * rz;
+                iy = R.y * rz;
+            }
+            const int myid = in ? __float_as_int(R.w) : -1;
+            // a second copy of a candidate (same id, next rank) starts dropped
+            const int pid = __shfl(myid, gbase + (lane > 0 ? lane - 1 : 0));
+            bool alive = in && !(lane > 0 && pid == myid);
+            const Mask lo_mask = (Mask(1) << lane) - Mask(1), hi_mask = ~((Mask(2) << lane) - Mask(1));
+            for (int round = 0; round < 64; round++) {
+                const Mask M = HullMask<kL>::group(__ballot(alive), gbase);
+                const int pr = HullMask<kL>::before(M, lo_mask, lane);
+                const int nr = HullMask<kL>::after(M, hi_mask, lane);
+                const float px = __shfl(ix, gbase + pr), py = __shfl(iy, gbase + pr);
+                const float nx = __shfl(ix, gbase + nr), ny = __shfl(iy, gbase + nr);
+                // reflex (or straight, or not a number): this point lies left
+                // of prev -> next, on the side of the origin
+                const bool drop = alive && !((nx - px) * (iy - py) - (ny - py) * (ix - px) < 0.0f);
+                if (!__ballot(drop)) break;
+                alive = alive && !drop;
+            }
+            // certification, one filter test (and one or two orientations) per lane
+            const Mask M = HullMask<kL>::group(__ballot(alive), gbase);
+            const int h = HullMask<kL>::popc(M);
+            const int pr = HullMask<kL>::before(M, lo_mask, lane);
+            const int nr = HullMask<kL>::after(M, hi_mask, lane);
+            const int ps = __shfl(slot, gbase + pr), ns = __shfl(slot, gbase + nr);
+            bool good = !in, turn = false;
+            unsigned tests = 0;
+            if (in && h >= 3) {
+                const float4 P = s_row[g][ps], N = s_row[g][ns];
+                float det;
+                if (alive) {
+                    // a corner: strictly convex, and (v, this, next) strictly CCW
+                    good = star_icc_sure(P, R, N, det) && det > 0.0f && star_orient_f32(R, N) > 0;
+                    turn = R.y < 0.0f && N.y >= 0.0f;
+                    tests = 1;
+                } else if (myid == __float_as_int(P.w) || myid == __float_as_int(N.w)) {
+                    good = true;  // a second copy of a corner
+                } else {
+                    // strictly inside the triangle (origin, P', N')
+                    good = star_icc_sure(P, N, R, det) && det > 0.0f && star_orient_f32(P, R) > 0 &&
+                           star_orient_f32(R, N) > 0;
+                    tests = 1;
+                }
+            }
+            const Mask gbad = HullMask<kL>::group(__ballot(!good), gbase);
+            const Mask gturn = HullMask<kL>::group(__ballot(alive && turn), gbase);
+            const Mask gstart = HullMask<kL>::group(__ballot(alive && slot == start), gbase);
+            hull_done = try_hull && h >= 3 && h <= kStarCap && gbad == 0 && HullMask<kL>::popc(gturn) == 1 &&
+                        gstart != 0;
+            if (hull_done) {
+                // place in the star, counted from the start
+                const int rs = HullMask<kL>::lowest(gstart);
+                int pos = HullMask<kL>::popc(M & lo_mask) - HullMask<kL>::popc(M & ((Mask(1) << rs) - Mask(1)));
+                if (pos < 0) pos += h;
+                h_pos = alive ? pos : -1;
+                h_deg = h;
+                h_id = myid;
+                loc[0] += tests;
+            }
+            // filter-kernel vertices that took the scan (one count per wave)
+            if (kFast) {
+                const unsigned long long fb = __ballot(lane == 0 && try_hull && !hull_done);
+                if (fb && (threadIdx.x & 63) == 0) atomicAdd(fail + 21, (int)__popcll(fb));
+            }
+        }
+        // the first pass (2-rings) leaves such a vertex to the full kernel:
+        // without the scan below it keeps fewer registers, more waves per SIMD
+        constexpr bool kScan = !(kFast && kL == 32 && kRing2);
+        if (!kScan && ok && !hull_done) {
+            ok = false;
+            why = 5;
+        }
         // ---- successors: for each owned candidate j, the most clockwise other
         // candidate as seen from j' (b beats a when orient(j', a', b') < 0,
         // i.e. incircle(j, a, b, v) < 0); -1 when a test was undecided
-        if (kFast) {
+        if (!kScan) {
+        } else if (kFast) {
             // Branch-free, with wave-uniform loops: both vertices of a wave
             // run to the larger candidate count, entries past a vertex's own
             // count (and the lane's own candidate) are masked out, and the
             // winner is kept by selects.
-            const int kc_g = ok ? kc : 0;
+            const int kc_g = (ok && !hull_done) ? kc : 0;
             int kc_w = kc_g;
             if (kL == 32) kc_w = max(kc_w, __shfl_xor(kc_w, 32));
             kc_w = __builtin_amdgcn_readfirstlane(kc_w);
@@ -515,7 +697,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                     s_succ[g][j] = undecided ? -1 : (to_start ? a + kC : a);
                 }
             }
-        } else if (ok) {
+        } else if (ok && !hull_done) {
             for (int j = lane; j < kc; j += kL) {
                 const float4 rj = s_row[g][j];
                 const int idj = __float_as_int(rj.w);
@@ -555,7 +737,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
         __syncthreads();
         // ---- trace the hull from the start through the successors
         int my_slot = -1, d = 0;
-        if (ok) {
+        if (ok && !hull_done) {
             int cur = start;
             bool closed = false;
             for (;;) {
@@ -573,7 +755,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
             }
         }
         // every triangle (v, s_i, s_i+1) strictly CCW (v inside the hull)
-        if (ok) {
+        if (ok && !hull_done) {
             const int nl = (lane + 1 < d) ? lane + 1 : 0;
             const int nslot = __shfl(my_slot, gbase + nl);
             bool bad = false, undecided = false;
@@ -593,7 +775,10 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
         }
         if (active) {
             const size_t row = (size_t)idx;
-            if (ok) {
+            if (hull_done) {
+                if (h_pos >= 0) nbr_out[row * kStarCap + h_pos] = h_id;
+                if (lane == 0) deg_out[row] = h_deg;
+            } else if (ok) {
                 if (lane < d) nbr_out[row * kStarCap + lane] = s_cand[g][my_slot];
                 if (lane == 0) deg_out[row] = d;
             } else if (lane == 0) {
