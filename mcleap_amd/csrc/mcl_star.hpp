@@ -171,6 +171,10 @@ __device__ __forceinline__ int star_find(const int* __restrict__ nbr, int u, int
 constexpr float kStarIccBound = 16.0f * 5.9604644775390625e-08f;  // the scan's incircle filter, x permanent
 
 constexpr int kStarMaxDepth = 5;  // deepest ring of the previous mesh searched
+// candidate sets of k_star
+constexpr int kCandHash = 0;   // rings de-duplicated in an LDS hash set (any depth)
+constexpr int kCandRing2 = 1;  // the 2-ring read off the previous (consistent) stars
+constexpr int kCandRaw = 2;    // old and new first rings with their new stars, copies kept
 
 // The FP32 stage of incircle(a, b, c, v) from precomputed rows (x, y, lift)
 // of a - v, b - v, c - v: the differences rounded to FP32 from their exact
@@ -286,7 +290,7 @@ __device__ __forceinline__ bool star_icc_sure(float4 A, float4 B, float4 C, floa
 // vertex (reason 5) for a later pass with the full cascade; this variant
 // carries none of the exact-arithmetic code and keeps a small register
 // footprint.
-template <bool kF32, bool kFast, int kL, bool kRing2>
+template <bool kF32, bool kFast, int kL, int kCand>
 __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ vlist, int nlist,
                                               const int* __restrict__ nlist_dev, const int* __restrict__ rows,
                                               const int* __restrict__ nrows, int depth,
@@ -339,7 +343,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
         int* list = s_cand[g];
         int* count = &s_cnt[g];
         int kc;
-        if (kRing2) {
+        if (kCand == kCandRing2) {
             // ---- candidates straight from the previous (consistent) stars:
             // v's star, then for each neighbour u_i the part of u_i's star
             // behind it.  In u_i's star v is followed by u_i-1 and preceded by
@@ -385,6 +389,110 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                 }
                 if (at < kC) list[at] = urow[q];
             }
+            __syncthreads();
+        } else if (kCand == kCandRaw) {
+            // ---- v's old and new first rings (and the check's hint), each
+            // with its star in this step's stars (a corner: the entries of its
+            // previous fan around v): the hash-set candidates, gathered without
+            // atomics.  Repeats are dropped twice, in the first ring and in the
+            // gathered list, by an LDS table indexed by a hash of the id: every
+            // entry writes its index to its slot, and after a barrier an entry
+            // whose slot holds another entry with the same id drops out (a slot
+            // shared by two ids only leaves repeats, which the hull elimination
+            // drops).  A vertex whose own star failed (it left its 2-ring) is
+            // left to the full kernel, which searches deeper rings.
+            constexpr int kBits = kH == 128 ? 7 : 8;
+            static_assert((1 << kBits) == kH, "hash table size");
+            const int da = active ? min(deg_a[v], kStarCap) : 0;
+            const int db0 = active ? deg_b[v] : 0;
+            const bool widen = active && db0 < 0;
+            const int db = widen ? 0 : min(db0, kStarCap);
+            int u = -1;
+            if (lane < da) u = nbr_a[(size_t)v * kStarCap + lane];
+            else if (lane < da + db) u = nbr_b[(size_t)v * kStarCap + lane - da];
+            else if (lane == da + db && hint && active) u = hint[v];
+            if ((unsigned)u >= (unsigned)m.nv || u == v) u = -1;
+            const unsigned hu = ((unsigned)u * 2654435761u) >> (32 - kBits);
+            if (u >= 0) tab[hu] = lane;
+            __syncthreads();
+            {
+                const int w = u >= 0 ? tab[hu] : lane;
+                const int uw = __shfl(u, gbase + w);
+                if (w != lane && uw == u) u = -1;
+            }
+            int du = 0, p = -1, cnt = 0;
+            const int* urow = nbr_b;
+            int e[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // the row's first 8 entries, loaded at once
+            const bool fan = u >= 0 && u < m.ncorner;
+            if (fan) {
+                du = deg_a[u];
+                urow = nbr_a + star_row(u, m.nv, m.ncorner);
+                p = (du >= 1 && du <= kCornerCap) ? row_find(urow, du, v) : -1;
+                cnt = 1 + (p >= 0 ? 2 * kCornerWindow + 3 : 0);
+            } else if (u >= 0) {
+                urow = nbr_b + (size_t)u * kStarCap;
+                const int4 q0 = reinterpret_cast<const int4*>(urow)[0];
+                const int4 q1 = reinterpret_cast<const int4*>(urow)[1];
+                du = min(deg_b[u], kStarCap);
+                e[0] = q0.x; e[1] = q0.y; e[2] = q0.z; e[3] = q0.w;
+                e[4] = q1.x; e[5] = q1.y; e[6] = q1.z; e[7] = q1.w;
+                cnt = 1 + (du > 0 ? du : 0);
+            }
+            int x = cnt;  // inclusive prefix over the group's lanes
+            for (int off = 1; off < kL; off <<= 1) {
+                const int y = __shfl_up(x, off);
+                if (lane >= off) x += y;
+            }
+            const int raw = __shfl(x, gbase + kL - 1);
+            int at = x - cnt;
+            if (cnt > 0 && at < kC) list[at] = u;
+            if (!fan && du <= 8) {
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    if (k < du && at + 1 + k < kC) list[at + 1 + k] = e[k];
+            } else {
+                for (int k = 1; k < cnt; k++) {
+                    const int q = fan ? ((p + k - kCornerWindow - 2) % du + du) % du : k - 1;
+                    if (at + k < kC) list[at + k] = urow[q];
+                }
+            }
+            __syncthreads();
+            // the gathered entries, two per lane: ids in range other than v,
+            // one copy of each
+            constexpr int kPer = kC / kL;
+            int c[kPer];
+            unsigned hc[kPer];
+#pragma unroll
+            for (int h = 0; h < kPer; h++) {
+                const int j = lane + kL * h;
+                c[h] = (j < raw && j < kC && !widen) ? list[j] : -1;
+                if ((unsigned)c[h] >= (unsigned)m.nv || c[h] == v) c[h] = -1;
+                hc[h] = ((unsigned)c[h] * 2654435761u) >> (32 - kBits);
+                if (c[h] >= 0) tab[hc[h]] = j;
+            }
+            __syncthreads();
+            unsigned long long keep[kPer];
+#pragma unroll
+            for (int h = 0; h < kPer; h++) {
+                const int j = lane + kL * h;
+                if (c[h] >= 0) {
+                    const int w = tab[hc[h]];
+                    if (w != j && list[w] == c[h]) c[h] = -1;
+                }
+                keep[h] = __ballot(c[h] >= 0) & gmask;
+            }
+            __syncthreads();
+            int before = 0;
+#pragma unroll
+            for (int h = 0; h < kPer; h++) {
+                const unsigned long long mine = keep[h] & ((1ull << (threadIdx.x & 63)) - 1ull);
+                if (c[h] >= 0) list[before + __popcll(mine)] = c[h];
+                before += __popcll(keep[h]);
+            }
+            // more gathered entries than the list holds, or a widening vertex:
+            // full kernel (no candidate count, so that no row is read from
+            // entries the squeeze did not write)
+            kc = (raw > kC || widen) ? 0 : before;
             __syncthreads();
         } else {
             // ---- candidates, de-duplicated in LDS, listed ring by ring
@@ -633,7 +741,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
         }
         // the first pass (2-rings) leaves such a vertex to the full kernel:
         // without the scan below it keeps fewer registers, more waves per SIMD
-        constexpr bool kScan = !(kFast && kL == 32 && kRing2);
+        constexpr bool kScan = !(kFast && kCand != kCandHash);
         if (!kScan && ok && !hull_done) {
             ok = false;
             why = 5;

@@ -1758,6 +1758,7 @@ struct mcl_tri {
     int* s_rlist = nullptr;                 // nv: vertices to re-check
     int* s_aux = nullptr;                   // kStarAux counters (see star_step)
     int* s_heavy = nullptr;                 // nv + 1: count, then rows for the full star kernel
+    int* s_heavy2 = nullptr;                // nv + 1: the first pass's second list (left by its wave-per-vertex kernel)
     int* s_hint = nullptr;                  // nv: a further candidate per vertex, left by the checks
     int* c_bucket = nullptr;                // ncorner * kCornerCap * 3: corner fan entries
     int* c_count = nullptr;                 // ncorner
@@ -1823,8 +1824,8 @@ static int tri_free(mcl_tri* t) {
                     t->scalar, t->acc, t->tgt, t->v_mv, t->v_rev, t->v_fexp, t->t_stamp, t->bad[0], t->bad[1],
                     t->rev, t->allrev, t->pend[0], t->pend[1], t->deg, t->scan_tmp, t->v_touch, t->t_rep, t->t_rep_e, t->e_undo, t->undo, t->stuck, t->touched, t->r_tt, t->r_tt2, t->r_wait, t->r_defer, t->v_soft, t->tail_out,
                     t->star_deg[0], t->star_deg[1], t->star_nbr[0], t->star_nbr[1], t->s_ecnt, t->s_tcnt,
-                    t->s_mark, t->s_list, t->s_aux, t->star_tdeg, t->star_tnbr, t->s_rlist, t->s_heavy, t->s_hint,
-                    t->c_bucket, t->c_count, t->d_stamp, t->d_params};
+                    t->s_mark, t->s_list, t->s_aux, t->star_tdeg, t->star_tnbr, t->s_rlist, t->s_heavy, t->s_heavy2,
+                    t->s_hint, t->c_bucket, t->c_count, t->d_stamp, t->d_params};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int k = 0; k < 2; k++)
@@ -1940,7 +1941,7 @@ int tri_make(int nv, int nhe, int nt, int periodic, int ncorner, mcl_tri** out) 
         A(t->star_nbr[0], star_rows_size(nv, m.ncorner)); A(t->star_nbr[1], star_rows_size(nv, m.ncorner));
         A(t->star_tdeg, nv); A(t->star_tnbr, star_rows_size(nv, m.ncorner));
         A(t->s_ecnt, nv + 1); A(t->s_tcnt, nv + 1); A(t->s_mark, nv); A(t->s_list, nv); A(t->s_aux, kStarAux);
-        A(t->s_rlist, nv); A(t->s_heavy, nv + 1); A(t->s_hint, nv); A(t->d_stamp, 1); A(t->d_params, 1);
+        A(t->s_rlist, nv); A(t->s_heavy, nv + 1); A(t->s_heavy2, nv + 1); A(t->s_hint, nv); A(t->d_stamp, 1); A(t->d_params, 1);
         if (m.ncorner > 0) { A(t->c_bucket, (size_t)m.ncorner * kCornerCap * 3); A(t->c_count, m.ncorner); }
     }
 #undef A
@@ -3045,27 +3046,41 @@ struct StarGrid {
 __host__ __device__ constexpr int star_list_slot(int r) { return (r & 1) ? 20 : 1; }
 
 // stars of the listed vertices (all without a list; count *nlist_dev) into
-// (odeg, onbr): the filter-only kernel, two vertices per wave, then the full
-// kernel on the vertices it could not finish (more than 64 candidates, or a
-// test the filter could not decide).  reset, give_up: as k_star's.
+// (odeg, onbr).  First pass (no list): the 2-ring filter kernel, two vertices
+// per wave and at most 32 candidates each; the vertices it leaves (more
+// candidates, or a hull it could not certify) go through the same kernel with
+// a whole wave per vertex and up to 64 candidates (its own list, whose count
+// the first launch zeroes), and what that leaves to the full kernel.  Retries:
+// the filter kernel over old and new first rings with their new stars, a wave
+// per vertex, then the full kernel on what it leaves (more than 64
+// candidates, a failed own star, a hull it could not certify).  reset,
+// give_up: as k_star's.
 void star_pass_stars(mcl_tri* t, hipStream_t s, int cur, const int* vlist, const int* nlist_dev, int depth,
                      const int* bdeg, const int* bnbr, int* odeg, int* onbr, int* reset, int give_up) {
     DMesh& m = t->m;
     const StarGrid G{t};
-    if (!vlist)  // first pass: 2-rings read off the previous stars directly
-        hipLaunchKernelGGL((k_star<true, true, 32, true>), G.full(m.nv, 8), dim3(256), 0, s, m, vlist, 0, nlist_dev,
-                           (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
+    const int* heavy_rows = t->s_heavy;
+    if (!vlist) {
+        hipLaunchKernelGGL((k_star<true, true, 32, kCandRing2>), G.full(m.nv, 8), dim3(256), 0, s, m, vlist, 0,
+                           nlist_dev, (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
                            (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy,
-                           (const float2*)t->acc, (const float2*)t->tgt, t->stats, reset, give_up, t->s_aux + 19,
-                           (const int*)nullptr);
-    else
-        hipLaunchKernelGGL((k_star<true, true, 32, false>), G.list(), dim3(256), 0, s, m, vlist, 0, nlist_dev,
+                           (const float2*)t->acc, (const float2*)t->tgt, t->stats, t->s_heavy2, give_up,
+                           t->s_aux + 19, (const int*)nullptr);
+        hipLaunchKernelGGL((k_star<true, true, 64, kCandRing2>), dim3(256), dim3(256), 0, s, m, vlist, 0, nlist_dev,
+                           (const int*)(t->s_heavy + 1), (const int*)t->s_heavy, depth,
+                           (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr,
+                           t->s_aux, t->s_heavy2, (const float2*)t->acc, (const float2*)t->tgt, t->stats,
+                           (int*)nullptr, -1, (int*)nullptr, (const int*)nullptr);
+        heavy_rows = t->s_heavy2;
+    } else {
+        hipLaunchKernelGGL((k_star<true, true, 64, kCandRaw>), G.list(), dim3(256), 0, s, m, vlist, 0, nlist_dev,
                            (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
                            (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy,
                            (const float2*)t->acc, (const float2*)t->tgt, t->stats, reset, give_up, t->s_aux + 19,
                            (const int*)t->s_hint);
-    hipLaunchKernelGGL((k_star<true, false, 64, false>), dim3(256), dim3(256), 0, s, m, vlist, 0, nlist_dev,
-                       (const int*)(t->s_heavy + 1), (const int*)t->s_heavy, depth, (const int*)t->star_deg[cur],
+    }
+    hipLaunchKernelGGL((k_star<true, false, 64, kCandHash>), dim3(256), dim3(256), 0, s, m, vlist, 0, nlist_dev,
+                       heavy_rows + 1, heavy_rows, depth, (const int*)t->star_deg[cur],
                        (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, (int*)nullptr,
                        (const float2*)t->acc, (const float2*)t->tgt, t->stats, (int*)nullptr, -1, (int*)nullptr,
                        vlist ? (const int*)t->s_hint : (const int*)nullptr);
