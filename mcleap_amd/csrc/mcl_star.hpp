@@ -933,37 +933,102 @@ __device__ __forceinline__ int star_conflict(const float2* __restrict__ pos, int
     return sg > 0 ? 1 : (sg < 0 ? 0 : -1);
 }
 
+// adds x (one value per thread, every thread of the wave calling) to *dst
+// with one atomic per wave: a counter that every thread of a large launch
+// adds to would otherwise serialise the launch on its address
+__device__ __forceinline__ void wave_add(long long x, unsigned long long* dst) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off);
+    if ((threadIdx.x & 63) == 0 && x) atomicAdd(dst, (unsigned long long)x);
+}
+
+// Block-buffered appends to a global list: items collect in an LDS buffer
+// and go out with one atomic reservation and a coalesced copy per flush (a
+// single counter serialises its atomics; one reservation per 256-thread tile
+// was a large part of the check at 1M vertices).
+constexpr int kAppendBuf = 8192;  // ints of LDS per block (the scatter's buffer)
+template <int kN = kAppendBuf>
+struct AppendBuf {
+    int buf[kN];
+    int fill, base;
+};
+template <int kN>
+__device__ __forceinline__ void append_init(AppendBuf<kN>& ab) {
+    if (threadIdx.x == 0) ab.fill = 0;
+    __syncthreads();
+}
+// this thread's offset for n items (a wave prefix sum, one LDS atomic per wave)
+template <int kN>
+__device__ __forceinline__ int append_reserve(AppendBuf<kN>& ab, int n) {
+    const int lane = threadIdx.x & 63;
+    int x = n;
+#pragma unroll
+    for (int dd = 1; dd < 64; dd <<= 1) {
+        const int y = __shfl_up(x, dd);
+        if (lane >= dd) x += y;
+    }
+    const int wtot = __shfl(x, 63);
+    int woff = 0;
+    if (lane == 63 && wtot) woff = atomicAdd(&ab.fill, wtot);
+    woff = __shfl(woff, 63);
+    return woff + x - n;
+}
+// writes the buffer out and empties it; every thread of the block, after a barrier
+template <int kN>
+__device__ __forceinline__ void append_flush(AppendBuf<kN>& ab, int* count, int* __restrict__ list) {
+    if (threadIdx.x == 0 && ab.fill) ab.base = atomicAdd(count, ab.fill);
+    __syncthreads();
+    const int nf = ab.fill, base = ab.base;
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) list[base + i] = ab.buf[i];
+    __syncthreads();
+    if (threadIdx.x == 0) ab.fill = 0;
+    __syncthreads();
+}
+// after a tile's items are written (and a barrier): flush when the next tile,
+// up to max_tile items, might not fit
+template <int kN>
+__device__ __forceinline__ void append_tile_end(AppendBuf<kN>& ab, int max_tile, int* count, int* __restrict__ list) {
+    __syncthreads();
+    if (ab.fill > kN - max_tile) append_flush(ab, count, list);
+}
+template <int kN>
+__device__ __forceinline__ void append_finish(AppendBuf<kN>& ab, int* count, int* __restrict__ list) {
+    __syncthreads();
+    if (ab.fill > 0) append_flush(ab, count, list);
+}
+
 // true when v's star (degree d >= 3) must be recomputed.  A triangle (v, u, w)
 // of v's star is consistent when w directly precedes v in u's star.  Without
 // `decide` any inconsistent triangle marks v; with it, v is marked only when
 // v is missing from u's star or v's triangle loses against u's (v, u, w')
 // (star_conflict), the other vertex being marked by its own check.  A vertex
 // of degree <= 8 whose neighbours have degree <= 8 (nearly all) is checked
-// with every load issued up front: its row, the neighbours' degrees, the
-// neighbours' first 8 entries.  *missing: a neighbour whose star lacks v
+// with every load issued up front: its row (q0, q1: its first 8 entries,
+// loaded by the caller with its degree), the neighbours' degrees and first 8
+// entries.  *missing: a neighbour whose star lacks v
 // (-1 if none found).
 __device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, const int* __restrict__ nbr, int v,
-                                                  int d, bool decide, const float2* __restrict__ pos,
-                                                  bool periodic, int ncorner, int* missing) {
+                                                  int d, int4 q0, int4 q1, bool decide,
+                                                  const float2* __restrict__ pos, bool periodic, int ncorner,
+                                                  int* missing) {
     const int* row = nbr + (size_t)v * kStarCap;
     *missing = -1;
     if (d <= 8) {
-        const int4* rv = reinterpret_cast<const int4*>(row);
-        const int4 q0 = rv[0], q1 = rv[1];
         const int r[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+        // the neighbours' degrees and first 8 entries, loaded together (a
+        // row does not wait for its degree)
         int du[8];
-#pragma unroll
-        for (int i = 0; i < 8; i++) du[i] = (i < d && r[i] >= ncorner) ? deg[r[i]] : 0;
         int4 lo[8], hi[8];
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            if (i < d && du[i] >= 3 && du[i] <= 8) {
+            du[i] = 0;
+            lo[i] = make_int4(-1, -1, -1, -1);
+            hi[i] = lo[i];
+            if (i < d && r[i] >= ncorner) {
                 const int4* ru = reinterpret_cast<const int4*>(nbr + (size_t)r[i] * kStarCap);
+                du[i] = deg[r[i]];
                 lo[i] = ru[0];
                 hi[i] = ru[1];
-            } else {
-                lo[i] = make_int4(-1, -1, -1, -1);
-                hi[i] = lo[i];
             }
         }
         bool bad = false, slow = false;
@@ -1028,70 +1093,24 @@ __device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, c
     return false;
 }
 
-// Block-buffered appends to a global list: items collect in an LDS buffer
-// and go out with one atomic reservation and a coalesced copy per flush (a
-// single counter serialises its atomics; one reservation per 256-thread tile
-// was a large part of the check at 1M vertices).
-constexpr int kAppendBuf = 8192;  // ints of LDS per block
-struct AppendBuf {
-    int buf[kAppendBuf];
-    int fill, base;
-};
-__device__ __forceinline__ void append_init(AppendBuf& ab) {
-    if (threadIdx.x == 0) ab.fill = 0;
-    __syncthreads();
-}
-// this thread's offset for n items (a wave prefix sum, one LDS atomic per wave)
-__device__ __forceinline__ int append_reserve(AppendBuf& ab, int n) {
-    const int lane = threadIdx.x & 63;
-    int x = n;
-#pragma unroll
-    for (int dd = 1; dd < 64; dd <<= 1) {
-        const int y = __shfl_up(x, dd);
-        if (lane >= dd) x += y;
-    }
-    const int wtot = __shfl(x, 63);
-    int woff = 0;
-    if (lane == 63 && wtot) woff = atomicAdd(&ab.fill, wtot);
-    woff = __shfl(woff, 63);
-    return woff + x - n;
-}
-// writes the buffer out and empties it; every thread of the block, after a barrier
-__device__ __forceinline__ void append_flush(AppendBuf& ab, int* count, int* __restrict__ list) {
-    if (threadIdx.x == 0 && ab.fill) ab.base = atomicAdd(count, ab.fill);
-    __syncthreads();
-    const int nf = ab.fill, base = ab.base;
-    for (int i = threadIdx.x; i < nf; i += blockDim.x) list[base + i] = ab.buf[i];
-    __syncthreads();
-    if (threadIdx.x == 0) ab.fill = 0;
-    __syncthreads();
-}
-// after a tile's items are written (and a barrier): flush when the next tile,
-// up to max_tile items, might not fit
-__device__ __forceinline__ void append_tile_end(AppendBuf& ab, int max_tile, int* count, int* __restrict__ list) {
-    __syncthreads();
-    if (ab.fill > kAppendBuf - max_tile) append_flush(ab, count, list);
-}
-__device__ __forceinline__ void append_finish(AppendBuf& ab, int* count, int* __restrict__ list) {
-    __syncthreads();
-    if (ab.fill > 0) append_flush(ab, count, list);
-}
-
-// Checks the *nlist listed vertices (all when vlist is null), one thread per
-// vertex.  Vertices that fail (or whose star failed) are appended to `list`;
+// k_star_check with one thread per vertex, for the first check of a pass
+// (every vertex): 256 vertices share each barrier and flush of the append
+// buffer, where k_star_check's lanes per vertex make the short lists of the
+// later checks faster.  Checks the *nlist listed vertices (all when vlist is
+// null).  Vertices that fail (or whose star failed) are appended to `list`;
 // with mark_nbrs set, with their star neighbours and any inconsistency marks
 // v, otherwise only the side of a conflict that loses the incircle test
 // (star_inconsistent with `decide`).  Without a list the
 // degree sum of the valid stars is added to *sum_deg.  reset (may be null):
 // a counter this launch zeroes for a later kernel.
-__global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restrict__ vlist, const int* __restrict__ nlist,
+__global__ void __launch_bounds__(256) k_star_check_all(int nv, const int* __restrict__ vlist, const int* __restrict__ nlist,
                                                     const int* __restrict__ deg, const int* __restrict__ nbr,
                                                     const float2* __restrict__ pos, int periodic, int ncorner,
                                                     int mark_nbrs, int* __restrict__ mark, const int* __restrict__ stamp_base,
                                                     int stamp_off, int* __restrict__ list, int* __restrict__ count,
                                                     long long* __restrict__ sum_deg, int* __restrict__ reset,
                                                     int* __restrict__ hint) {
-    __shared__ AppendBuf ab;
+    __shared__ AppendBuf<> ab;
     const int stamp = *stamp_base + stamp_off;
     if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;  // a counter zeroed for a later kernel
     append_init(ab);
@@ -1105,13 +1124,15 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
         const int v0 = idx < end ? (vlist ? vlist[idx] : idx) : 0;
         const bool active = idx < end && v0 >= ncorner;  // corner stars are derived, not checked
         const int v = active ? v0 : 0;
-        const int d = active ? deg[v] : 3;
         const int* row = nbr + (size_t)v * kStarCap;
+        // v's degree and first 8 entries, loaded together
+        const int4 q0 = reinterpret_cast<const int4*>(row)[0], q1 = reinterpret_cast<const int4*>(row)[1];
+        const int d = active ? deg[v] : 3;
         bool bad = active && d < 3;
         int missing = -1;
         if (active && !bad) {
             if (!vlist) my_sum += d;
-            bad = star_inconsistent(deg, nbr, v, d, !mark_nbrs, pos, periodic != 0, ncorner, &missing);
+            bad = star_inconsistent(deg, nbr, v, d, q0, q1, !mark_nbrs, pos, periodic != 0, ncorner, &missing);
         }
         // claim v and its neighbours once per stamp, then append the claimed
         // ones.  With hint given, a neighbour u whose star lacks v is listed
@@ -1141,7 +1162,126 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
         append_tile_end(ab, max_tile, count, list);
     }
     append_finish(ab, count, list);
-    if (my_sum) atomicAdd((unsigned long long*)sum_deg, (unsigned long long)my_sum);
+    wave_add(my_sum, (unsigned long long*)sum_deg);
+}
+
+// Checks the *nlist listed vertices (all when vlist is null), kStarCheckLanes
+// lanes per vertex: lane i takes the triangles (v, u_i, u_i+1) of v's star
+// for i = lane, lane + 8, ...  A triangle is consistent when u_i+1 directly
+// precedes v in u_i's star (corners are skipped: their rows are derived
+// later).  A vertex fails when its star failed (degree < 3), a neighbour's
+// star failed, a neighbour's star lacks v, or, with mark_nbrs set, on any
+// inconsistent triangle; without it (decide) a triangle whose neighbour's
+// star holds another vertex w' before v is decided by one FP32 incircle
+// test (star_conflict): only the side whose triangle is not Delaunay fails
+// (undecided: v fails), the other side failing in its own check.  Failed
+// vertices are appended to `list`, with mark_nbrs also their star
+// neighbours, each once per stamp.  With hint given, the first neighbour u
+// (in v's row) whose star lacks v is listed too, with hint[u] = v: when v has
+// moved into u's star from beyond u's 2-ring, and u's other neighbours
+// (recomputed without v as well) agree with u, no other check lists u, and
+// u's 2-ring in the new stars does not reach v either.  Without a list the
+// degree sum of the valid stars is added to *sum_deg.  reset (may be null):
+// a counter this launch zeroes for a later kernel.
+//
+// Each lane loads its neighbour's degree and first 12 entries at once (all
+// but a few stars), so no vertex's check waits on one load after another.
+constexpr int kStarCheckLanes = 8;
+constexpr int kStarCheckBuf = 2048;  // append buffer ints per block
+__global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restrict__ vlist, const int* __restrict__ nlist,
+                                                    const int* __restrict__ deg, const int* __restrict__ nbr,
+                                                    const float2* __restrict__ pos, int periodic, int ncorner,
+                                                    int mark_nbrs, int* __restrict__ mark, const int* __restrict__ stamp_base,
+                                                    int stamp_off, int* __restrict__ list, int* __restrict__ count,
+                                                    long long* __restrict__ sum_deg, int* __restrict__ reset,
+                                                    int* __restrict__ hint) {
+    constexpr int kCL = kStarCheckLanes, kCV = 256 / kCL;
+    __shared__ AppendBuf<kStarCheckBuf> ab;
+    const int stamp = *stamp_base + stamp_off;
+    if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;  // a counter zeroed for a later kernel
+    append_init(ab);
+    const int max_tile = kCV * (kStarCap + 2);  // v, its neighbours, the one missing v
+    const int g = threadIdx.x / kCL, lane = threadIdx.x % kCL;
+    const int gbase = (threadIdx.x & 63) - lane;
+    const unsigned long long gmask = 0xffull << gbase;
+    const bool decide = !mark_nbrs;
+    long long my_sum = 0;
+    const int n = vlist ? *nlist : nv;
+    int first, end, stride;
+    xcd_split(n, kCV, first, end, stride);
+    for (int base = first; base < end; base += stride) {
+        const int idx = base + g;
+        const int v0 = idx < end ? (vlist ? vlist[idx] : idx) : 0;
+        const bool active = idx < end && v0 >= ncorner;  // corner stars are derived, not checked
+        const int v = active ? v0 : 0;
+        const int* row = nbr + (size_t)v * kStarCap;
+        // v's degree with its first entries (u_lane, u_lane+1)
+        const int d = active ? deg[v] : 0;
+        const int u_first = row[lane], w_first = row[lane + 1];
+        const int r0 = __shfl(u_first, gbase);
+        if (!vlist && lane == 0 && active && d >= 3) my_sum += d;
+        bool bad = false;
+        int miss = kStarCap;  // smallest i whose neighbour's star lacks v
+        for (int i = lane; active && d >= 3 && i < d; i += kCL) {
+            const int u = i < kCL ? u_first : row[i];
+            const int w = i + 1 < d ? (i < kCL ? w_first : row[i + 1]) : r0;
+            if (u < ncorner) continue;
+            const int* urow = nbr + (size_t)u * kStarCap;
+            const int4 q0 = reinterpret_cast<const int4*>(urow)[0];
+            const int4 q1 = reinterpret_cast<const int4*>(urow)[1];
+            const int4 q2 = reinterpret_cast<const int4*>(urow)[2];
+            const int du = deg[u];
+            if (du < 3) {
+                bad = true;
+                continue;
+            }
+            int pred = -1;
+            if (du <= 12) {
+                const int e[12] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+                int last = e[0];
+#pragma unroll
+                for (int k = 1; k < 12; k++) last = (k == du - 1) ? e[k] : last;
+#pragma unroll
+                for (int k = 0; k < 12; k++)
+                    if (k < du && e[k] == v) pred = k > 0 ? e[k - 1] : last;
+            } else {
+                const int j = star_find(nbr, u, du, v);
+                pred = j < 0 ? -1 : urow[j == 0 ? du - 1 : j - 1];
+            }
+            if (pred == w) continue;
+            if (pred < 0) {
+                miss = min(miss, i);
+                bad = true;
+            } else if (!decide || star_conflict(pos, v, u, w, pred, periodic != 0) != 0) {
+                bad = true;
+            }
+        }
+        // the group's verdict, and the first neighbour lacking v
+        bad = active && ((d < 3) || (__ballot(bad) & gmask) != 0);
+#pragma unroll
+        for (int off = 1; off < kCL; off <<= 1) miss = min(miss, __shfl_xor(miss, off));
+        const int missing = (bad && miss < kStarCap) ? row[miss] : -1;
+        // claim v (lane 0), its neighbours (their lanes, with mark_nbrs) and
+        // the neighbour missing v (lane 0) once per stamp, then append them
+        int items[2 + (kStarCap + kCL - 1) / kCL];
+        int nclaim = 0;
+        if (bad) {
+            if (lane == 0 && atomicExch(&mark[v], stamp) != stamp) items[nclaim++] = v;
+            for (int i = lane; mark_nbrs && i < d; i += kCL) {
+                const int u = row[i];
+                if (u >= ncorner && atomicExch(&mark[u], stamp) != stamp) items[nclaim++] = u;
+            }
+            if (lane == 0 && hint && missing >= ncorner) {
+                hint[missing] = v;
+                if (atomicExch(&mark[missing], stamp) != stamp) items[nclaim++] = missing;
+            }
+        }
+        const int at = append_reserve(ab, nclaim);
+        for (int k = 0; k < nclaim; k++) ab.buf[at + k] = items[k];
+        append_tile_end(ab, max_tile, count, list);
+    }
+    append_finish(ab, count, list);
+    wave_add(my_sum, (unsigned long long*)sum_deg);
 }
 
 // Copies the recomputed stars (rows by list position, *nlist_dev of them) to
@@ -1159,7 +1299,7 @@ __global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vl
                                                       int stamp_off, int* __restrict__ rlist,
                                                       int* __restrict__ rcount, long long* __restrict__ sum_deg,
                                                       long long* __restrict__ retried, int* __restrict__ reset) {
-    __shared__ AppendBuf ab;
+    __shared__ AppendBuf<> ab;
     const int stamp = *stamp_base + stamp_off;
     if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;  // a counter zeroed for a later kernel
     append_init(ab);
@@ -1197,7 +1337,7 @@ __global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vl
         append_tile_end(ab, (int)blockDim.x * (kStarCap + 1), rcount, rlist);
     }
     append_finish(ab, rcount, rlist);
-    if (delta) atomicAdd((unsigned long long*)sum_deg, (unsigned long long)delta);
+    wave_add(delta, (unsigned long long*)sum_deg);
 }
 
 // the mark stamps of one star step: *base + 1 .. *base + kStampSpan - 1

@@ -3105,14 +3105,20 @@ void star_check(mcl_tri* t, hipStream_t s, int nxt, int r, bool mark_nbrs, int s
     DMesh& m = t->m;
     const StarGrid G{t};
     const bool first = r == 0;
-    // the first check: 4 tiles of 256 vertices per block, its appends
-    // flushed once per block
-    hipLaunchKernelGGL(k_star_check, first ? G.full(m.nv, 1024) : G.list(), dim3(256), 0, s, m.nv,
-                       first ? (const int*)nullptr : (const int*)t->s_rlist,
-                       first ? (const int*)nullptr : (const int*)(t->s_aux + 8), (const int*)t->star_deg[nxt],
-                       (const int*)t->star_nbr[nxt], (const float2*)m.pos, m.periodic, m.ncorner, mark_nbrs ? 1 : 0,
-                       t->s_mark, (const int*)t->d_stamp, stamp_off, t->s_list, t->s_aux + star_list_slot(r),
-                       (long long*)(t->s_aux + 2), t->s_heavy, first ? (int*)nullptr : t->s_hint);
+    // the first check (every vertex): one thread per vertex, 4 tiles of 256
+    // vertices per block; the later ones (lists): 8 lanes per vertex
+    if (first)
+        hipLaunchKernelGGL(k_star_check_all, G.full(m.nv, 1024), dim3(256), 0, s, m.nv, (const int*)nullptr,
+                           (const int*)nullptr, (const int*)t->star_deg[nxt], (const int*)t->star_nbr[nxt],
+                           (const float2*)m.pos, m.periodic, m.ncorner, mark_nbrs ? 1 : 0, t->s_mark,
+                           (const int*)t->d_stamp, stamp_off, t->s_list, t->s_aux + star_list_slot(r),
+                           (long long*)(t->s_aux + 2), t->s_heavy, (int*)nullptr);
+    else
+        hipLaunchKernelGGL(k_star_check, G.list(), dim3(256), 0, s, m.nv, (const int*)t->s_rlist,
+                           (const int*)(t->s_aux + 8), (const int*)t->star_deg[nxt], (const int*)t->star_nbr[nxt],
+                           (const float2*)m.pos, m.periodic, m.ncorner, mark_nbrs ? 1 : 0, t->s_mark,
+                           (const int*)t->d_stamp, stamp_off, t->s_list, t->s_aux + star_list_slot(r),
+                           (long long*)(t->s_aux + 2), t->s_heavy, t->s_hint);
 }
 
 // the most vertices the first check may list before the step gives up on the
