@@ -302,6 +302,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                                               unsigned long long* __restrict__ stats, int* __restrict__ reset,
                                               int give_up, int* __restrict__ gave_up, const int* __restrict__ hint) {
     static_assert(kL == 32 || kL == 64, "lanes per vertex");
+    static_assert(kCand != kCandRaw || kL == 64, "the raw gather takes a first ring of up to 49 entries, one per lane");
     constexpr int kV = 256 / kL;                    // vertices per block iteration
     constexpr int kC = kL == 64 ? kCandMax : 64;    // candidates per vertex
     constexpr int kH = 2 * kC;                      // hash slots per vertex
@@ -400,7 +401,8 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
             // whose slot holds another entry with the same id drops out (a slot
             // shared by two ids only leaves repeats, which the hull elimination
             // drops).  A vertex whose own star failed (it left its 2-ring) is
-            // left to the full kernel, which searches deeper rings.
+            // left to the full kernel, which searches deeper rings.  (One wave
+            // per vertex: at most 49 first-ring entries, one per lane.)
             constexpr int kBits = kH == 128 ? 7 : 8;
             static_assert((1 << kBits) == kH, "hash table size");
             const int da = active ? min(deg_a[v], kStarCap) : 0;
@@ -1314,12 +1316,23 @@ __global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vl
         const bool active = idx < nlist;
         const int v = active ? vlist[idx] : 0;
         const int od = active ? deg[v] : 0;
+        int* row = nbr + (size_t)v * kStarCap;
+        // claims: v, then the previous entries, the first 8 (nearly all) from
+        // two 16-byte loads with their 8 exchanges issued together
         unsigned claimed = 0;
         int nclaim = 0;
+        const int4 q0 = reinterpret_cast<const int4*>(row)[0], q1 = reinterpret_cast<const int4*>(row)[1];
+        const int e[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
         if (active) {
             if (atomicExch(&mark[v], stamp) != stamp) { claimed |= 1u; nclaim++; }
-            for (int i = 0; i < od; i++)
-                if (atomicExch(&mark[nbr[(size_t)v * kStarCap + i]], stamp) != stamp) {
+            int got[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) got[i] = i < od ? atomicExch(&mark[e[i]], stamp) : stamp;
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+                if (got[i] != stamp) { claimed |= 2u << i; nclaim++; }
+            for (int i = 8; i < od; i++)
+                if (atomicExch(&mark[row[i]], stamp) != stamp) {
                     claimed |= 2u << i;
                     nclaim++;
                 }
@@ -1327,11 +1340,11 @@ __global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vl
         int at = append_reserve(ab, nclaim);
         if (claimed & 1u) ab.buf[at++] = v;
         for (int i = 0; i < od && i < kStarCap; i++)
-            if (claimed & (2u << i)) ab.buf[at++] = nbr[(size_t)v * kStarCap + i];
+            if (claimed & (2u << i)) ab.buf[at++] = row[i];
         if (active) {
             const int nd = tdeg[idx];
             delta += (nd >= 3 ? nd : 0) - (od >= 3 ? od : 0);
-            for (int k = 0; k < nd; k++) nbr[(size_t)v * kStarCap + k] = tnbr[(size_t)idx * kStarCap + k];
+            for (int k = 0; k < nd; k++) row[k] = tnbr[(size_t)idx * kStarCap + k];
             deg[v] = nd;
         }
         append_tile_end(ab, (int)blockDim.x * (kStarCap + 1), rcount, rlist);
