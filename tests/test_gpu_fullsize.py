@@ -87,3 +87,31 @@ def test_c2_star_steps_settle_for_240_steps():
     assert fallbacks == [], fallbacks
     assert_at_targets(dt, st, "step 239")
     assert dt.count_non_delaunay() == 0 and dt.count_inverted() == 0
+
+
+def test_c5_ensemble_4096_full_size(oracle):
+    """The batched ensemble at its full size: 4096 box instances, n_i =
+    floor(exp U[ln 256, ln 65536]) points (47.6M in all), built on the GPU
+    (mcl_batch_build) and advanced by 3 shared steps of Gaussian moves with
+    sigma = 5e-4 (the path choice as the library makes it).  After every step
+    the union mesh has no inverted triangle and no edge failing the exact
+    incircle test on the device, and every vertex is at its target; after the
+    steps the smallest, the largest and 4 instances between them each equal,
+    as a canonical triangle set, the oracle's from-scratch exact Delaunay
+    triangulation of their downloaded positions."""
+    from mcleap_amd.workloads import ensemble_points
+    from tests.test_gpu_batch import _check_batch
+
+    pts = ensemble_points(4096, seed=CONFIGS["c5"]["seed"])
+    assert sum(len(p) for p in pts) > 40_000_000
+    dt = M.DeviceTriangulation.from_ensemble(pts)
+    B = dt.h_triangulation
+    for k in range(3):
+        st = dt.step(M.MOTION_GAUSS, CONFIGS["c5"]["sigma"], seed=CONFIGS["c5"]["seed"], step=k)
+        print("c5 step", k, {x: st[x] for x in ("rounds", "flips", "relocated", "stalled", "duplicates",
+                                               "fallbacks", "ms")}, flush=True)
+        assert dt.count_inverted() == 0 and dt.count_non_delaunay() == 0
+        assert_at_targets(dt, st, "c5 step %d" % k)
+    order = np.argsort(B.counts, kind="stable")
+    picked = [int(order[0]), int(order[-1])] + [int(order[j]) for j in (800, 1700, 2600, 3500)]
+    _check_batch(O=oracle, B=B, dt=dt, tag="c5 after 3 steps", instances=picked)
