@@ -178,8 +178,8 @@ constexpr int kCandRaw = 2;    // old and new first rings with their new stars, 
 
 // The FP32 stage of incircle(a, b, c, v) from precomputed rows (x, y, lift)
 // of a - v, b - v, c - v: the differences rounded to FP32 from their exact
-// values and lift = x * x + y * y in FP32, the same operations in the same
-// order as the filter inside incircle(), so the decision is identical.
+// values (star_diff) and lift = x * x + y * y in FP32, the operations and
+// bound of the filter inside incircle(), so a decided sign is the exact sign.
 // Returns +-1, or 0 when the filter cannot decide.
 __device__ __forceinline__ int star_incircle_f32(float4 A, float4 B, float4 C) {
 #pragma clang fp contract(off)
@@ -240,6 +240,19 @@ template <> struct HullMask<64> {
         return m ? (a ? __ffsll((long long)a) - 1 : __ffsll((long long)m) - 1) : lane;
     }
 };
+
+// b - a (minimum image when periodic) rounded to FP32: the FP32 subtraction
+// rounds the exact difference once, and is that value whenever no image shift
+// can be taken (|b - a| < 0.4999 is certainly below 0.5); otherwise the exact
+// expansion, shifted, is rounded (through FP64).  Either way the row carries a
+// relative error of at most about one FP32 rounding, which the filters'
+// bounds allow for.
+__device__ __forceinline__ float star_diff(float b, float a, bool periodic) {
+#pragma clang fp contract(off)
+    const float d = b - a;
+    if (!periodic || fabsf(d) < 0.4999f) return d;
+    return (float)dexp_value(diff_exact(b, a, 0, true));
+}
 
 // incircle(a, b, c, v) from the filter rows A, B, C of a - v, b - v, c - v in
 // the successor scan's evaluation order (minors of the x, y columns, expanded
@@ -552,8 +565,8 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
             for (int j = lane; active && j < kc && j < kC; j += kL) {
                 const float2 pj = m.pos[list[j]];
                 s_pos[g][j] = pj;
-                const float x = (float)dexp_value(diff_exact(pj.x, pv.x, 0, periodic));
-                const float y = (float)dexp_value(diff_exact(pj.y, pv.y, 0, periodic));
+                const float x = star_diff(pj.x, pv.x, periodic);
+                const float y = star_diff(pj.y, pv.y, periodic);
                 s_row[g][j] = make_float4(x, y, x * x + y * y, __int_as_float(list[j]));
             }
         }
