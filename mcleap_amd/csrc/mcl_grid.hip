@@ -55,12 +55,14 @@ __host__ __device__ __forceinline__ uint32_t spread16(uint32_t x) {
 __device__ __forceinline__ int2 cell_of(const GridParams& g, float2 p) {
     float qx = __fsub_rn(p.x, g.minx), qy = __fsub_rn(p.y, g.miny);
     int x = (int)__fdiv_rn(qx, g.csx), y = (int)__fdiv_rn(qy, g.csy);
-    x %= g.nx;
-    y %= g.ny;
+    // C remainder (a no-op inside the grid: the division runs only outside it)
+    if ((unsigned)x >= (unsigned)g.nx) x %= g.nx;
+    if ((unsigned)y >= (unsigned)g.ny) y %= g.ny;
     if (g.periodic || g.morton) {
         if (g.periodic) {
-            x = (x + g.nx) % g.nx;
-            y = (y + g.ny) % g.ny;
+            // (x + nx) % nx for x in (-nx, nx)
+            x += x < 0 ? g.nx : 0;
+            y += y < 0 ? g.ny : 0;
         } else {
             x = min(max(x, 0), g.nx - 1);
             y = min(max(y, 0), g.ny - 1);
@@ -176,8 +178,12 @@ __global__ void __launch_bounds__(kT) k_frnn(GridParams g, const float2* __restr
                 if (from_hits && hm == 0) break;
                 int x = h.x + ddx, y = h.y + ddy;
                 if (g.periodic) {
-                    x = (x + g.nx) % g.nx;
-                    y = (y + g.ny) % g.ny;
+                    // wrapped without a division: |ddx|, |ddy| <= reach, and
+                    // 2 * reach + 1 <= nx, ny (checked at the launch)
+                    x += x < 0 ? g.nx : 0;
+                    x -= x >= g.nx ? g.nx : 0;
+                    y += y < 0 ? g.ny : 0;
+                    y -= y >= g.ny ? g.ny : 0;
                 } else if (g.morton) {
                     if (x < 0 || x >= g.nx || y < 0 || y >= g.ny) continue;
                 } else {
