@@ -1618,14 +1618,28 @@ __global__ void __launch_bounds__(256) k_star_ring_csr(int nv, int ncorner, cons
         }
         const bool staged = wn <= kRingWave;
         if (v < nv) {
+            const int* row = nbr + star_row(v, nv, ncorner);
+            // the row's first 8 entries with the degree (rows are 16-byte aligned)
+            const int4 q0 = reinterpret_cast<const int4*>(row)[0], q1 = reinterpret_cast<const int4*>(row)[1];
             const int d = deg[v];
             const int k0 = off[v];
-            const int* row = nbr + star_row(v, nv, ncorner);
-            for (int i = 0; i < d; i++) {
-                const int k = k0 + i;
-                const int x = row[i == 0 ? 0 : d - i];
-                if (staged) s_row[wv][k - w0] = x;
-                else if (k < cap) out[k] = x;
+            if (d <= 8) {
+                // entry j goes to place 0 (j = 0) or d - j of the ring
+                const int e[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    if (j < d) {
+                        const int k = k0 + (j == 0 ? 0 : d - j);
+                        if (staged) s_row[wv][k - w0] = e[j];
+                        else if (k < cap) out[k] = e[j];
+                    }
+            } else {
+                for (int i = 0; i < d; i++) {
+                    const int k = k0 + i;
+                    const int x = row[i == 0 ? 0 : d - i];
+                    if (staged) s_row[wv][k - w0] = x;
+                    else if (k < cap) out[k] = x;
+                }
             }
         }
         __syncthreads();
