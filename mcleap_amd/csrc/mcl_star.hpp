@@ -1357,7 +1357,16 @@ __global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vl
         if (active) {
             const int nd = tdeg[idx];
             delta += (nd >= 3 ? nd : 0) - (od >= 3 ? od : 0);
-            for (int k = 0; k < nd; k++) row[k] = tnbr[(size_t)idx * kStarCap + k];
+            // the new row: 16-byte copies of its first 8 entries (entries past
+            // the degree are never read), the rest one by one
+            const int4* tr = reinterpret_cast<const int4*>(tnbr + (size_t)idx * kStarCap);
+            int4* rr = reinterpret_cast<int4*>(row);
+            if (nd > 0) {
+                const int4 t0 = tr[0], t1 = tr[1];
+                rr[0] = t0;
+                rr[1] = t1;
+            }
+            for (int k = 8; k < nd; k++) row[k] = tnbr[(size_t)idx * kStarCap + k];
             deg[v] = nd;
         }
         append_tile_end(ab, (int)blockDim.x * (kStarCap + 1), rcount, rlist);
