@@ -1016,60 +1016,77 @@ __device__ __forceinline__ void append_finish(AppendBuf<kN>& ab, int* count, int
 // of v's star is consistent when w directly precedes v in u's star.  Without
 // `decide` any inconsistent triangle marks v; with it, v is marked only when
 // v is missing from u's star or v's triangle loses against u's (v, u, w')
-// (star_conflict), the other vertex being marked by its own check.  A vertex
-// of degree <= 8 whose neighbours have degree <= 8 (nearly all) is checked
-// with every load issued up front: its row (q0, q1: its first 8 entries,
-// loaded by the caller with its degree), the neighbours' degrees and first 8
-// entries.  *missing: a neighbour whose star lacks v
-// (-1 if none found).
+// (star_conflict), the other vertex being marked by its own check.  The
+// triangles go in batches of 8 with every load of a batch issued up front:
+// the neighbours' degrees and first 12 entries (all but a few stars); a
+// neighbour with more entries is searched one vector at a time.  q0, q1: v's
+// first 8 entries, loaded by the caller with its degree.  *missing: a
+// neighbour whose star lacks v (-1 if none found).
 __device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, const int* __restrict__ nbr, int v,
                                                   int d, int4 q0, int4 q1, bool decide,
                                                   const float2* __restrict__ pos, bool periodic, int ncorner,
                                                   int* missing) {
     const int* row = nbr + (size_t)v * kStarCap;
+    const int4* rv = reinterpret_cast<const int4*>(row);
     *missing = -1;
-    if (d <= 8) {
-        const int r[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-        // the neighbours' degrees and first 8 entries, loaded together (a
-        // row does not wait for its degree)
+    bool bad = false;
+    const int first = q0.x;
+#pragma unroll 1
+    for (int i0 = 0; i0 < d && !bad; i0 += 8) {
+        int4 p0 = q0, p1 = q1;
+        if (i0 > 0) {
+            p0 = rv[i0 >> 2];
+            p1 = rv[(i0 >> 2) + 1];
+        }
+        const int r[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+        // the entry after the batch (the apex of its last triangle)
+        const int after = i0 + 8 < d ? row[i0 + 8] : first;
         int du[8];
-        int4 lo[8], hi[8];
+        int4 e0[8], e1[8], e2[8];
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            du[i] = 0;
-            lo[i] = make_int4(-1, -1, -1, -1);
-            hi[i] = lo[i];
-            if (i < d && r[i] >= ncorner) {
-                const int4* ru = reinterpret_cast<const int4*>(nbr + (size_t)r[i] * kStarCap);
-                du[i] = deg[r[i]];
-                lo[i] = ru[0];
-                hi[i] = ru[1];
+        for (int k = 0; k < 8; k++) {
+            du[k] = 0;
+            e0[k] = e1[k] = e2[k] = make_int4(-1, -1, -1, -1);
+            if (i0 + k < d && r[k] >= ncorner) {
+                const int4* ru = reinterpret_cast<const int4*>(nbr + (size_t)r[k] * kStarCap);
+                du[k] = deg[r[k]];
+                e0[k] = ru[0];
+                e1[k] = ru[1];
+                e2[k] = ru[2];
             }
         }
-        bool bad = false, slow = false;
         unsigned conf = 0;  // triangles to decide by star_conflict
         int pv[8];          // per triangle: the entry before v in u's star
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            pv[i] = -1;
-            if (i >= d || r[i] < ncorner) continue;  // corner rows are derived later, not looked up
-            const int w = (i + 1 < d) ? r[(i + 1) & 7] : r[0];
-            const int n = du[i];
-            if (n < 3) { bad = true; continue; }
-            if (n > 8) { slow = true; continue; }
-            const int e[8] = {lo[i].x, lo[i].y, lo[i].z, lo[i].w, hi[i].x, hi[i].y, hi[i].z, hi[i].w};
-            int last = e[0];
-#pragma unroll
-            for (int k = 1; k < 8; k++) last = (k == n - 1) ? e[k] : last;
+        for (int k = 0; k < 8; k++) {
+            pv[k] = -1;
+            const int i = i0 + k;
+            if (i >= d || r[k] < ncorner) continue;  // corner rows are derived later, not looked up
+            const int w = i + 1 < d ? (k < 7 ? r[k + 1] : after) : first;
+            const int n = du[k];
+            if (n < 3) {
+                bad = true;
+                continue;
+            }
             int pred = -1;
+            if (n <= 12) {
+                const int e[12] = {e0[k].x, e0[k].y, e0[k].z, e0[k].w, e1[k].x, e1[k].y,
+                                   e1[k].z, e1[k].w, e2[k].x, e2[k].y, e2[k].z, e2[k].w};
+                int last = e[0];
 #pragma unroll
-            for (int k = 0; k < 8; k++)
-                if (k < n && e[k] == v) pred = k > 0 ? e[k - 1] : last;
-            pv[i] = pred;
+                for (int c = 1; c < 12; c++) last = (c == n - 1) ? e[c] : last;
+#pragma unroll
+                for (int c = 0; c < 12; c++)
+                    if (c < n && e[c] == v) pred = c > 0 ? e[c - 1] : last;
+            } else {
+                const int j = star_find(nbr, r[k], n, v);
+                pred = j < 0 ? -1 : nbr[(size_t)r[k] * kStarCap + (j == 0 ? n - 1 : j - 1)];
+            }
+            pv[k] = pred;
             if (pred == w) continue;
-            if (pred < 0 && *missing < 0) *missing = r[i];
+            if (pred < 0 && *missing < 0) *missing = r[k];
             if (!decide || pred < 0) bad = true;
-            else conf |= 1u << i;
+            else conf |= 1u << k;
         }
         // the conflicts, one at a time (the test appears once in the code)
 #pragma unroll 1
@@ -1081,31 +1098,13 @@ __device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, c
             for (int k = 0; k < 8; k++)
                 if (k == i) {
                     u = r[k];
-                    w = (k + 1 < d) ? r[(k + 1) & 7] : r[0];
+                    w = i0 + k + 1 < d ? (k < 7 ? r[k + 1] : after) : first;
                     p = pv[k];
                 }
             if (star_conflict(pos, v, u, w, p, periodic) != 0) bad = true;
         }
-        if (bad || !slow) return bad;
     }
-    // general form: one triangle at a time
-    const int first = row[0];
-    int u = first;
-    for (int i = 0; i < d; i++) {
-        const int w = (i + 1 == d) ? first : row[i + 1];
-        if (u < ncorner) {
-            u = w;
-            continue;
-        }
-        const int du = deg[u];
-        if (du < 3) return true;
-        const int j = star_find(nbr, u, du, v);
-        const int pred = j < 0 ? -1 : nbr[(size_t)u * kStarCap + (j == 0 ? du - 1 : j - 1)];
-        if (j < 0) *missing = u;
-        if (pred != w && (!decide || pred < 0 || star_conflict(pos, v, u, w, pred, periodic) != 0)) return true;
-        u = w;
-    }
-    return false;
+    return bad;
 }
 
 // k_star_check with one thread per vertex, for the first check of a pass
