@@ -86,22 +86,28 @@ __global__ void k_zero2(int* __restrict__ p, int* __restrict__ q, int n) {
     }
 }
 
+// key and histogram; with slot given, each point's place among its cell's
+// points in atomic (arbitrary) order, the count before its own increment
 __global__ void __launch_bounds__(kT) k_key_count(GridParams g, const float2* __restrict__ pos, int n,
-                                                  int* __restrict__ key, int* __restrict__ count) {
+                                                  int* __restrict__ key, int* __restrict__ count,
+                                                  int* __restrict__ slot) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         int2 c = cell_of(g, pos[i]);
         int k = key_of(g, c.x, c.y);
         key[i] = k;
-        atomicAdd(&count[k], 1);
+        if (slot) slot[i] = atomicAdd(&count[k], 1);
+        else atomicAdd(&count[k], 1);
     }
 }
 
-// each point into a slot of its cell, in atomic (arbitrary) order
+// each point into a slot of its cell, in atomic (arbitrary) order: the place
+// its key pass took (slot given), else a second round of atomics on fill
 __global__ void __launch_bounds__(kT) k_slot(const int* __restrict__ key, int n, const int* __restrict__ start,
-                                             int* __restrict__ fill, int* __restrict__ perm) {
+                                             int* __restrict__ fill, const int* __restrict__ slot,
+                                             int* __restrict__ perm) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         int k = key[i];
-        perm[start[k] + atomicAdd(&fill[k], 1)] = i;
+        perm[start[k] + (slot ? slot[i] : atomicAdd(&fill[k], 1))] = i;
     }
 }
 
@@ -258,16 +264,22 @@ struct SortBufs {
     float2 *spos, *svel;
 };
 
-// Counting sort of n points by cell: key + histogram, exclusive scan of the
-// histogram, slots in atomic order (into ind), the stable order within each
-// cell (into perm), then the gather of the sorted positions and ind[i] = the
-// slot of point i.  Asynchronous on s; uses only the given buffers.
+// Counting sort of n points by cell: key + histogram (with each point's place
+// in its cell in atomic order), exclusive scan of the histogram, slots in
+// that order (into ind), the stable order within each cell (into perm), then
+// the gather of the sorted positions and ind[i] = the slot of point i.
+// Asynchronous on s; uses only the given buffers.
 int grid_sort(const GridParams& g, const float2* pos, const float2* vel, int n, const SortBufs& b, hipStream_t s) {
     const int nk = g.nkeys, B = g.block;
-    hipLaunchKernelGGL(k_zero2, dim3(blocks(nk + 1)), dim3(kT), 0, s, b.start, b.fill, nk + 1);
-    hipLaunchKernelGGL(k_key_count, dim3(blocks(n, B)), dim3(B), 0, s, g, pos, n, b.key, b.start);
+    // each point's place in its cell comes from the histogram's own atomics,
+    // kept in the sorted positions' buffer until the gather fills it (unless
+    // that buffer is the input: then a second round of atomics)
+    int* slot = (const void*)pos == (const void*)b.spos ? nullptr : reinterpret_cast<int*>(b.spos);
+    if (slot) hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, b.start, nk + 1);
+    else hipLaunchKernelGGL(k_zero2, dim3(blocks(nk + 1)), dim3(kT), 0, s, b.start, b.fill, nk + 1);
+    hipLaunchKernelGGL(k_key_count, dim3(blocks(n, B)), dim3(B), 0, s, g, pos, n, b.key, b.start, slot);
     MCL_TRY(scan_exclusive(b.start, b.start, nk, b.scan_tmp, s));
-    hipLaunchKernelGGL(k_slot, dim3(blocks(n, B)), dim3(B), 0, s, b.key, n, b.start, b.fill, b.ind);
+    hipLaunchKernelGGL(k_slot, dim3(blocks(n, B)), dim3(B), 0, s, b.key, n, b.start, b.fill, (const int*)slot, b.ind);
     hipLaunchKernelGGL(k_cell_rank, dim3(blocks(n, B)), dim3(B), 0, s, b.key, n, b.start, (const int*)b.ind, b.perm);
     hipLaunchKernelGGL(k_gather, dim3(blocks(n, B)), dim3(B), 0, s, b.perm, n, pos, vel, b.spos, b.svel, b.ind);
     MCL_HIP(hipGetLastError());
@@ -627,7 +639,8 @@ __device__ __forceinline__ int2 bgrid_cell(float cs, int nc, float2 p) {
 }
 
 __global__ void __launch_bounds__(kT) k_bkey_count(BGridParams b, const float2* __restrict__ pos,
-                                                   int* __restrict__ key, int* __restrict__ count) {
+                                                   int* __restrict__ key, int* __restrict__ count,
+                                                   int* __restrict__ slot) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
         const int v = b.v0 + i;
         const int t = bgrid_inst(b, v);
@@ -635,7 +648,8 @@ __global__ void __launch_bounds__(kT) k_bkey_count(BGridParams b, const float2* 
         const int2 c = bgrid_cell(b.cs[t], nc, pos[v]);
         const int k = b.cbase[t] + c.y * nc + c.x;
         key[i] = k;
-        atomicAdd(&count[k], 1);
+        if (slot) slot[i] = atomicAdd(&count[k], 1);
+        else atomicAdd(&count[k], 1);
     }
 }
 
@@ -794,11 +808,18 @@ extern "C" int mcl_bgrid_build(mcl_bgrid_t* G, const void* d_pos, void* stream) 
     const int n = G->n, nk = G->nkeys;
     const BGridParams b = G->params();
     G->timers[2].mark(s);
-    hipLaunchKernelGGL(k_zero2, dim3(blocks(nk + 1)), dim3(kT), 0, s, G->start, G->fill, nk + 1);
-    hipLaunchKernelGGL(k_bkey_count, dim3(blocks(n)), dim3(kT), 0, s, b, (const float2*)d_pos, G->key, G->start);
+    // each point's place in its cell from the histogram's atomics, kept in the
+    // sorted positions' buffer until the gather fills it (unless that buffer
+    // is the input: then a second round of atomics)
+    int* slot = (const float2*)d_pos + G->v0 == G->spos ? nullptr : reinterpret_cast<int*>(G->spos);
+    if (slot) hipLaunchKernelGGL(k_zero, dim3(blocks(nk + 1)), dim3(kT), 0, s, G->start, nk + 1);
+    else hipLaunchKernelGGL(k_zero2, dim3(blocks(nk + 1)), dim3(kT), 0, s, G->start, G->fill, nk + 1);
+    hipLaunchKernelGGL(k_bkey_count, dim3(blocks(n)), dim3(kT), 0, s, b, (const float2*)d_pos, G->key, G->start,
+                       slot);
     int rc = scan_exclusive(G->start, G->start, nk, G->scan_tmp, s);
     if (rc != MCL_OK) return rc;
-    hipLaunchKernelGGL(k_slot, dim3(blocks(n)), dim3(kT), 0, s, G->key, n, G->start, G->fill, G->ind);
+    hipLaunchKernelGGL(k_slot, dim3(blocks(n)), dim3(kT), 0, s, G->key, n, G->start, G->fill, (const int*)slot,
+                       G->ind);
     hipLaunchKernelGGL(k_cell_rank, dim3(blocks(n)), dim3(kT), 0, s, G->key, n, G->start, (const int*)G->ind,
                        G->perm);
     hipLaunchKernelGGL(k_gather, dim3(blocks(n)), dim3(kT), 0, s, G->perm, n, (const float2*)d_pos + G->v0,
