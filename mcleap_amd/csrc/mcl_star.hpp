@@ -1599,13 +1599,10 @@ __global__ void __launch_bounds__(256) k_star_face3(DMesh m) {
     }
 }
 
-// one-ring CSR straight from the stars, in the order of the half-edge walk
-// (clockwise from v.he, which the rebuild sets to v -> first star entry):
-// first entry, then the rest of the star backwards
-__global__ void __launch_bounds__(256) k_star_ring_deg(int nv, const int* __restrict__ deg, int* __restrict__ off) {
-    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) off[v] = deg[v];
-}
-// The rows of a wave's 64 consecutive vertices are one contiguous range of
+// One-ring CSR straight from the stars (offsets: the scan of the star
+// degrees), in the order of the half-edge walk (clockwise from v.he, which
+// the rebuild sets to v -> first star entry): first entry, then the rest of
+// the star backwards.  The rows of a wave's 64 consecutive vertices are one contiguous range of
 // the output: a range of at most kRingWave ints is gathered in LDS and
 // written with contiguous stores (rows stored thread by thread write partial
 // cache lines, several times the bytes of the list); a longer one (corner

@@ -3646,12 +3646,11 @@ extern "C" int mcl_tri_one_ring(mcl_tri_t* t, int K, int layout, void* d_ring, i
 // the CSR one-ring, asynchronous (cap >= nhe: every half-edge leaves one
 // vertex), from the stars in buffer c when from_stars, else from the half-edges
 static int enqueue_ring_csr(mcl_tri* t, hipStream_t s, int c, bool from_stars, int* off, int* nbrs, int cap) {
-    if (from_stars)
-        hipLaunchKernelGGL(k_star_ring_deg, dim3(blocks_for(t->m.nv)), dim3(256), 0, s, t->m.nv,
-                           (const int*)t->star_deg[c], off);
-    else
+    // offsets: the scan of the star degrees (from the stars), or of the degrees
+    // counted on the half-edges
+    if (!from_stars)
         hipLaunchKernelGGL(k_degree, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, off);
-    MCL_TRY(scan_exclusive(off, off, t->m.nv, t->scan_tmp, s));
+    MCL_TRY(scan_exclusive(from_stars ? (const int*)t->star_deg[c] : off, off, t->m.nv, t->scan_tmp, s));
     if (from_stars)
         hipLaunchKernelGGL(k_star_ring_csr, dim3(blocks_for(t->m.nv)), dim3(256), 0, s, t->m.nv, t->m.ncorner,
                            (const int*)t->star_deg[c], (const int*)t->star_nbr[c], (const int*)off, nbrs, cap);
@@ -3690,12 +3689,11 @@ extern "C" int mcl_tri_one_ring_csr(mcl_tri_t* t, void* d_offsets, void* d_nbrs,
     // with current stars the ring comes from them (same order as the walk)
     const bool from_stars = t->stars_valid;
     const int c = t->star_cur;
-    if (from_stars)
-        hipLaunchKernelGGL(k_star_ring_deg, dim3(blocks_for(t->m.nv)), dim3(256), 0, s, t->m.nv,
-                           (const int*)t->star_deg[c], off);
-    else
+    // offsets: the scan of the star degrees (from the stars), or of the degrees
+    // counted on the half-edges
+    if (!from_stars)
         hipLaunchKernelGGL(k_degree, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m, off);
-    MCL_TRY(scan_exclusive(off, off, t->m.nv, t->scan_tmp, s));
+    MCL_TRY(scan_exclusive(from_stars ? (const int*)t->star_deg[c] : off, off, t->m.nv, t->scan_tmp, s));
     auto fill = [&]() {
         if (from_stars)
             hipLaunchKernelGGL(k_star_ring_csr, dim3(blocks_for(t->m.nv)), dim3(256), 0, s, t->m.nv, t->m.ncorner,
