@@ -957,60 +957,6 @@ __device__ __forceinline__ void wave_add(long long x, unsigned long long* dst) {
     if ((threadIdx.x & 63) == 0 && x) atomicAdd(dst, (unsigned long long)x);
 }
 
-// Block-buffered appends to a global list: items collect in an LDS buffer
-// and go out with one atomic reservation and a coalesced copy per flush (a
-// single counter serialises its atomics; one reservation per 256-thread tile
-// was a large part of the check at 1M vertices).
-constexpr int kAppendBuf = 8192;  // ints of LDS per block (the scatter's buffer)
-template <int kN = kAppendBuf>
-struct AppendBuf {
-    int buf[kN];
-    int fill, base;
-};
-template <int kN>
-__device__ __forceinline__ void append_init(AppendBuf<kN>& ab) {
-    if (threadIdx.x == 0) ab.fill = 0;
-    __syncthreads();
-}
-// this thread's offset for n items (a wave prefix sum, one LDS atomic per wave)
-template <int kN>
-__device__ __forceinline__ int append_reserve(AppendBuf<kN>& ab, int n) {
-    const int lane = threadIdx.x & 63;
-    int x = n;
-#pragma unroll
-    for (int dd = 1; dd < 64; dd <<= 1) {
-        const int y = __shfl_up(x, dd);
-        if (lane >= dd) x += y;
-    }
-    const int wtot = __shfl(x, 63);
-    int woff = 0;
-    if (lane == 63 && wtot) woff = atomicAdd(&ab.fill, wtot);
-    woff = __shfl(woff, 63);
-    return woff + x - n;
-}
-// writes the buffer out and empties it; every thread of the block, after a barrier
-template <int kN>
-__device__ __forceinline__ void append_flush(AppendBuf<kN>& ab, int* count, int* __restrict__ list) {
-    if (threadIdx.x == 0 && ab.fill) ab.base = atomicAdd(count, ab.fill);
-    __syncthreads();
-    const int nf = ab.fill, base = ab.base;
-    for (int i = threadIdx.x; i < nf; i += blockDim.x) list[base + i] = ab.buf[i];
-    __syncthreads();
-    if (threadIdx.x == 0) ab.fill = 0;
-    __syncthreads();
-}
-// after a tile's items are written (and a barrier): flush when the next tile,
-// up to max_tile items, might not fit
-template <int kN>
-__device__ __forceinline__ void append_tile_end(AppendBuf<kN>& ab, int max_tile, int* count, int* __restrict__ list) {
-    __syncthreads();
-    if (ab.fill > kN - max_tile) append_flush(ab, count, list);
-}
-template <int kN>
-__device__ __forceinline__ void append_finish(AppendBuf<kN>& ab, int* count, int* __restrict__ list) {
-    __syncthreads();
-    if (ab.fill > 0) append_flush(ab, count, list);
-}
 
 // true when v's star (degree d >= 3) must be recomputed.  A triangle (v, u, w)
 // of v's star is consistent when w directly precedes v in u's star.  Without

@@ -106,53 +106,6 @@ __device__ __forceinline__ void flush_stats(const unsigned (&loc)[N], unsigned l
                   (unsigned long long)acc[threadIdx.x]);
 }
 
-// Appends, for one tile of the block (one item per thread), the items with
-// pred set to the list counted by *ctr: a wave ballot, one LDS atomic per
-// wave and ONE global atomic per block tile (instead of one per wave), so a
-// heavily appended list does not serialise on its counter.  Every thread of
-// the block must make the same number of calls.  Returns the item's slot, or
-// -1 when pred is false.  s_tile: 2 ints of LDS.
-__device__ __forceinline__ int block_append(int* ctr, bool pred, int* s_tile) {
-    const unsigned long long mask = __ballot(pred);
-    const int lane = __lane_id();
-    const int wcount = __popcll(mask);
-    const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-    int woff = 0;
-    if (lane == 0 && wcount) woff = atomicAdd(&s_tile[0], wcount);
-    woff = __shfl(woff, 0);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int tot = s_tile[0];
-        s_tile[1] = tot ? atomicAdd(ctr, tot) : 0;
-        s_tile[0] = 0;
-    }
-    __syncthreads();
-    return pred ? s_tile[1] + woff + rank : -1;
-}
-
-// Like block_append for `count` (0..n) entries per thread: returns the first
-// of this thread's `count` consecutive slots.  Same uniform-call rule.
-__device__ __forceinline__ int block_reserve(int* ctr, int count, int* s_tile) {
-    const int lane = __lane_id();
-    int x = count;  // inclusive prefix sum over the wave
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(x, d);
-        if (lane >= d) x += y;
-    }
-    const int wtot = __shfl(x, 63);
-    int woff = 0;
-    if (lane == 63 && wtot) woff = atomicAdd(&s_tile[0], wtot);
-    woff = __shfl(woff, 63);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int tot = s_tile[0];
-        s_tile[1] = tot ? atomicAdd(ctr, tot) : 0;
-        s_tile[0] = 0;
-    }
-    __syncthreads();
-    return s_tile[1] + woff + (x - count);
-}
 
 // loc[st]++ with a run-time st would put loc in scratch memory (a memory
 // round trip per predicate); constant indices keep the counters in registers
@@ -302,9 +255,8 @@ __global__ void k_integrate(DMesh m, const float* __restrict__ dt, const float2*
 template <bool kF32>
 __global__ void __launch_bounds__(kThreads) k_check_all(DMesh m, int* __restrict__ bad, int* __restrict__ cnt, int slot,
                                                         unsigned long long* __restrict__ stats) {
-    __shared__ int s_tile[2];
-    if (threadIdx.x == 0) s_tile[0] = 0;
-    __syncthreads();
+    __shared__ AppendBuf<kListBuf> ab;
+    append_init(ab);
     unsigned loc[4] = {0, 0, 0, 0};
     int first, end, stride;
     xcd_split(m.nt, blockDim.x, first, end, stride);
@@ -318,9 +270,11 @@ __global__ void __launch_bounds__(kThreads) k_check_all(DMesh m, int* __restrict
             bump(loc, st);
             is_bad = s <= 0;
         }
-        const int k = block_append(&cnt[slot], is_bad, s_tile);
-        if (is_bad && bad) bad[k] = t;
+        const int at = append_reserve(ab, is_bad ? 1 : 0);
+        if (is_bad) ab.buf[at] = t;
+        append_tile_end(ab, (int)blockDim.x, &cnt[slot], bad);
     }
+    append_finish(ab, &cnt[slot], bad);
     flush_stats(loc, stats, kStOrient0);
 }
 
@@ -1204,9 +1158,8 @@ __global__ void __launch_bounds__(kThreads) k_round_test(DMesh m, const int* __r
     const int r3 = round % 3;
     const int n = kAll ? n_all : cnt[kCnCand + r3];
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt[kCnCand + (r3 + 2) % 3] = 0;
-    __shared__ int s_tile[2];
-    if (threadIdx.x == 0) s_tile[0] = 0;
-    __syncthreads();
+    __shared__ AppendBuf<kListBuf> ab;
+    append_init(ab);
     unsigned loc[4] = {0, 0, 0, 0};
     int first, end, stride;
     xcd_split(n, blockDim.x, first, end, stride);
@@ -1284,9 +1237,11 @@ __global__ void __launch_bounds__(kThreads) k_round_test(DMesh m, const int* __r
             atomicMax(&claim[tt.y], key);
         }
         } while (false);
-        const int k = block_append(&cnt[kCnWants + r3], want, s_tile);
-        if (want) wants[k] = e;
+        const int at = append_reserve(ab, want ? 1 : 0);
+        if (want) ab.buf[at] = e;
+        append_tile_end(ab, (int)blockDim.x, &cnt[kCnWants + r3], wants);
     }
+    append_finish(ab, &cnt[kCnWants + r3], wants);
     flush_stats(loc, stats, kKind == 0 ? kStIncircle0 : kStOrient0);
 }
 
@@ -1303,9 +1258,8 @@ __global__ void __launch_bounds__(kThreads) k_round_flip(DMesh m, int* __restric
     const int n = cnt[kCnWants + r3];
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt[kCnWants + (r3 + 1) % 3] = 0;
     const int slot = kCnCand + (r3 + 1) % 3, stamp = round + 1;
-    __shared__ int s_tile[2];
-    if (threadIdx.x == 0) s_tile[0] = 0;
-    __syncthreads();
+    __shared__ AppendBuf<kListBuf> ab;
+    append_init(ab);
     unsigned loc[1] = {0};
     for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const int i = base + threadIdx.x;
@@ -1330,12 +1284,14 @@ __global__ void __launch_bounds__(kThreads) k_round_flip(DMesh m, int* __restric
             if (q3 >= 0 && atomicExch(&e_stamp[q3], stamp) == stamp) q3 = -1;
         }
         const int nq = (q0 >= 0) + (q1 >= 0) + (q2 >= 0) + (q3 >= 0);
-        int p = block_reserve(&cnt[slot], nq, s_tile);
-        if (q0 >= 0) cand_next[p++] = q0;
-        if (q1 >= 0) cand_next[p++] = q1;
-        if (q2 >= 0) cand_next[p++] = q2;
-        if (q3 >= 0) cand_next[p++] = q3;
+        int p = append_reserve(ab, nq);
+        if (q0 >= 0) ab.buf[p++] = q0;
+        if (q1 >= 0) ab.buf[p++] = q1;
+        if (q2 >= 0) ab.buf[p++] = q2;
+        if (q3 >= 0) ab.buf[p++] = q3;
+        append_tile_end(ab, 4 * (int)blockDim.x, &cnt[slot], cand_next);
     }
+    append_finish(ab, &cnt[slot], cand_next);
     flush_stats(loc, stats, stat_slot);
 }
 
