@@ -1735,6 +1735,11 @@ struct mcl_tri {
     bool star_end_event = false;            // the current step settled by stars: ev_aux ends it
     const TailReq* tail_req = nullptr;      // the calling timestep's tail (its FRNN part enqueued at the star read)
     bool frnn_done = false;                 // ... and that FRNN query holds the step's final positions
+    bool ring_done = false;                 // ... and its one-ring part ran on the step's final stars
+    hipStream_t side_stream = nullptr;      // a star step's FRNN part, beside its retry passes
+    hipEvent_t ev_fork = nullptr;           // caller's stream -> side stream (after the first star check)
+    hipEvent_t ev_join = nullptr;           // side stream -> caller's stream (end of the FRNN part)
+    bool side_pending = false;              // the caller's stream has not yet waited for ev_join
     // one ring
     int* deg = nullptr;
     int* scan_tmp = nullptr;
@@ -1790,9 +1795,10 @@ static int tri_free(mcl_tri* t) {
         if (t->g_post[k]) (void)hipGraphExecDestroy(t->g_post[k]);
     if (t->h_params) (void)hipHostFree(t->h_params);
     if (t->h_aux) (void)hipHostFree(t->h_aux);
-    for (hipEvent_t e : {t->ev_step, t->ev_aux, t->ev_end})
+    for (hipEvent_t e : {t->ev_step, t->ev_aux, t->ev_end, t->ev_fork, t->ev_join})
         if (e) (void)hipEventDestroy(e);
     if (t->cap_stream) (void)hipStreamDestroy(t->cap_stream);
+    if (t->side_stream) (void)hipStreamDestroy(t->side_stream);
     if (t->h_cnt) (void)hipHostFree(t->h_cnt);
     if (t->h_stats) (void)hipHostFree(t->h_stats);
     return MCL_OK;
@@ -1906,6 +1912,8 @@ int tri_make(int nv, int nhe, int nt, int periodic, int ncorner, mcl_tri** out) 
     if (rc == MCL_OK && hipHostMalloc((void**)&t->h_aux, sizeof(int) * kStarAux) != hipSuccess) rc = MCL_ERR_HIP;
     for (hipEvent_t* e : {&t->ev_step, &t->ev_aux, &t->ev_end})
         if (rc == MCL_OK && hipEventCreate(e) != hipSuccess) rc = MCL_ERR_HIP;
+    for (hipEvent_t* e : {&t->ev_fork, &t->ev_join})
+        if (rc == MCL_OK && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) rc = MCL_ERR_HIP;
     if (rc == MCL_OK &&
         hipHostMalloc((void**)&t->h_stats, sizeof(unsigned long long) * kStCount * kStatShards) != hipSuccess)
         rc = MCL_ERR_HIP;
@@ -2971,6 +2979,8 @@ static int ensure_mesh(mcl_tri* t, hipStream_t s) {
 // counters once; the rare step that has not settled continues eagerly, one
 // read per pass.
 
+static int launch_tail(mcl_tri* t, const TailReq& q, hipStream_t s, int part, int c, bool from_stars);
+
 namespace {
 // launch geometry of the star kernels
 struct StarGrid {
@@ -3114,11 +3124,29 @@ void star_corner_fans(mcl_tri* t, hipStream_t s, int cur, int nxt) {
                        (long long*)(t->s_aux + 16));
 }
 
+// The FRNN part of the calling timestep's tail (grid sort and CSR query) on a
+// second stream, once the first check has run: it reads the positions only,
+// and runs beside the retry passes, a few launches over short lists that
+// leave most of the GPU idle, instead of after them.  The caller's stream
+// waits for it before the timestep returns (join_side).  Eager steps only
+// (a captured head keeps the tail on the caller's stream).
+int fork_side(mcl_tri* t, hipStream_t s, int nxt) {
+    if (t->capturing || t->graph || !t->tail_req || !t->tail_req->g) return MCL_OK;
+    if (!t->side_stream) MCL_HIP(hipStreamCreateWithFlags(&t->side_stream, hipStreamNonBlocking));
+    MCL_HIP(hipEventRecord(t->ev_fork, s));
+    MCL_HIP(hipStreamWaitEvent(t->side_stream, t->ev_fork, 0));
+    t->side_pending = true;
+    MCL_TRY(launch_tail(t, *t->tail_req, t->side_stream, 0, nxt, true));
+    MCL_HIP(hipEventRecord(t->ev_join, t->side_stream));
+    t->frnn_done = true;
+    return MCL_OK;
+}
+
 // the head of a star step (see above); pdev: the step parameters in device
 // memory (graph), else p by value.  Ends with the corner fans, so that the
 // counters read after it describe a complete set of stars.
-void star_head(mcl_tri* t, hipStream_t s, int cur, int nxt, const mcl_step_params_t* p,
-               const mcl_step_params_t* pdev, const float2* disp, int mv_stamp) {
+int star_head(mcl_tri* t, hipStream_t s, int cur, int nxt, const mcl_step_params_t* p,
+              const mcl_step_params_t* pdev, const float2* disp, int mv_stamp) {
     DMesh& m = t->m;
     // the statistics, the star counters and the heavy-list count start at
     // zero, and the check stamps move on (the move's block 0)
@@ -3132,10 +3160,20 @@ void star_head(mcl_tri* t, hipStream_t s, int cur, int nxt, const mcl_step_param
     star_pass_stars(t, s, cur, nullptr, nullptr, 2, nullptr, nullptr, t->star_deg[nxt], t->star_nbr[nxt], nullptr, -1);
     t->timers[5].mark(s);
     star_check(t, s, nxt, 0, false, 1);
+    MCL_TRY(fork_side(t, s, nxt));
     for (int r = 1; r < t->star_head_passes; r++) star_retry(t, s, cur, nxt, r);
     star_corner_fans(t, s, cur, nxt);
+    return MCL_OK;
 }
 }  // namespace
+
+// the caller's stream waits for the side stream's work (if any is pending)
+static int join_side(mcl_tri* t, hipStream_t s) {
+    if (!t->side_pending) return MCL_OK;
+    t->side_pending = false;
+    MCL_HIP(hipStreamWaitEvent(s, t->ev_join, 0));
+    return MCL_OK;
+}
 
 static void drop_graphs(mcl_tri* t) {
     for (int k = 0; k < 2; k++) {
@@ -3221,14 +3259,14 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
             // v_mv marks matter only to the flip-based step, which writes its own
             const int mv_stamp = ++t->stamp;
             MCL_TRY(capture_graph(t, &t->g_head[cur], [&](hipStream_t cs) {
-                star_head(t, cs, cur, nxt, p, t->d_params, nullptr, mv_stamp);
+                (void)star_head(t, cs, cur, nxt, p, t->d_params, nullptr, mv_stamp);
             }));
         }
         *t->h_params = *p;
         MCL_HIP(hipMemcpyAsync(t->d_params, t->h_params, sizeof(mcl_step_params_t), hipMemcpyHostToDevice, s));
         MCL_HIP(hipGraphLaunch(t->g_head[cur], s));
     } else {
-        star_head(t, s, cur, nxt, p, nullptr, disp, ++t->stamp);
+        MCL_TRY(star_head(t, s, cur, nxt, p, nullptr, disp, ++t->stamp));
     }
     MCL_HIP(hipGetLastError());
     int aux[kStarAux];
@@ -3246,9 +3284,17 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         MCL_HIP(hipMemcpyAsync(t->h_stats, t->stats, sizeof(unsigned long long) * kStCount * kStatShards,
                                hipMemcpyDeviceToHost, s));
         MCL_HIP(hipEventRecord(t->ev_aux, s));
-        if (first && t->tail_req && t->tail_req->g) {
+        if (first && t->tail_req && t->tail_req->g && !t->frnn_done) {
             MCL_TRY(launch_tail(t, *t->tail_req, s, 0, nxt, true));
             t->frnn_done = true;
+        }
+        // with the FRNN part on the side stream, the one-ring part on the
+        // stars as they stand (final when this read finds the step settled,
+        // else run again after the step), so that the GPU has work while the
+        // host waits
+        if (first && t->tail_req && t->tail_req->ring_off && t->side_pending) {
+            MCL_TRY(launch_tail(t, *t->tail_req, s, 1, nxt, true));
+            t->ring_done = true;
         }
         MCL_HIP(hipEventSynchronize(t->ev_aux));
         memcpy(aux, t->h_aux, sizeof(aux));
@@ -3256,8 +3302,12 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         unsigned long long mb = 0;
         for (int j = 0; j < kStatShards; j++)
             mb += t->h_stats[(size_t)kStStalled * kStatShards + j] + t->h_stats[(size_t)kStRelDup * kStatShards + j];
-        if (first) moved_back = mb;
-        else if (mb != moved_back) t->frnn_done = false;
+        // the FRNN part holds the final positions unless a vertex was sent
+        // back to its previous position after it started: after the first
+        // read (caller's stream), or at any point of the step (side stream,
+        // which starts during the head)
+        if (first) moved_back = t->side_pending ? 0 : mb;
+        if (mb != moved_back) t->frnn_done = false;
         if (trace)
             fprintf(stderr,
                     "  [stars pass %d] inconsistent %d (failed stars so far %d: candidates %d undecided %d open %d "
@@ -3269,6 +3319,7 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     int listed = 0;
     for (int r = t->star_head_passes - 1; r < kStarPasses; r++) {
         if (r >= t->star_head_passes) {
+            t->ring_done = false;  // the stars change
             star_retry(t, s, cur, nxt, r);
             star_corner_fans(t, s, cur, nxt);
             MCL_HIP(hipGetLastError());
@@ -3289,7 +3340,7 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
             break;
         }
     }
-    if (!settled) t->frnn_done = false;
+    if (!settled) t->frnn_done = t->ring_done = false;
     if (!settled && trace && listed > 0) {
         // the first vertices still listed: id, position, star degree (-1: failed)
         const int k = std::min(listed, 12);
@@ -3627,14 +3678,17 @@ extern "C" int mcl_tri_timestep(mcl_tri_t* t, const mcl_step_params_t* p, void* 
     // a star step enqueues the FRNN part right after its counter read, before
     // it waits for that read (see star_step); frnn_done: it need not run again
     t->tail_req = want ? &q : nullptr;
-    t->frnn_done = false;
+    t->frnn_done = t->ring_done = false;
     int rc = mcl_tri_step(t, p, st, stream);
     t->tail_req = nullptr;
+    // the FRNN part run on the side stream (if any) is part of this call's work
+    const int jrc = join_side(t, s);
     if (rc != MCL_OK && rc != MCL_ERR_UNRESOLVED) return rc;
+    MCL_TRY(jrc);
     if (!want) return rc;
     if (!t->frnn_done) MCL_TRY(launch_tail(t, q, s, 0, t->star_cur, t->stars_valid));
     if (!t->stars_valid) MCL_TRY(ensure_mesh(t, s));
-    MCL_TRY(launch_tail(t, q, s, 1, t->star_cur, t->stars_valid));
+    if (!t->ring_done || !t->stars_valid) MCL_TRY(launch_tail(t, q, s, 1, t->star_cur, t->stars_valid));
     return rc;
 }
 

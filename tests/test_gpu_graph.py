@@ -161,9 +161,11 @@ print(json.dumps(out))
 
 @pytest.mark.parametrize("periodic,head,graph", [(True, 1, False), (True, 4, False), (True, 1, True), (False, 1, False)])
 def test_timestep_tail_equals_fresh_queries(oracle, periodic, head, graph):
-    """mcl_tri_timestep enqueues the grid FRNN part of its tail at the star
-    step's counter read, before the stars have settled, and the one-ring after
-    it (again after the step when the positions changed later).  After every
+    """mcl_tri_timestep runs the grid FRNN part of its tail on a second stream
+    from the star step's first check on (with graphs: on the caller's stream
+    at the counter read), before the stars have settled, and enqueues the
+    one-ring at the counter read (each again after the step when the
+    positions or the stars changed later).  After every
     step the ring and FRNN arrays it wrote must equal queries made afresh on
     the resulting mesh and positions.  One head pass makes most steps settle
     only after that read (the path that enqueues the one-ring late and may
