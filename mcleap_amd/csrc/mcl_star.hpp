@@ -319,9 +319,13 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
     constexpr int kV = 256 / kL;                    // vertices per block iteration
     constexpr int kC = kL == 64 ? kCandMax : 64;    // candidates per vertex
     constexpr int kH = 2 * kC;                      // hash slots per vertex
+    // the successor scan (below) is compiled into the full kernel and the
+    // hash-gather filter kernel; the others hand what the hull elimination
+    // leaves to a later kernel
+    constexpr bool kScan = !(kFast && kCand != kCandHash);
     __shared__ int s_tab[kV][kH];
     __shared__ int s_cand[kV][kC];
-    __shared__ float2 s_pos[kV][kC];
+    __shared__ float2 s_pos[kV][kFast ? 1 : kC];  // the candidates' positions (exact predicates)
     __shared__ float4 s_row[kV][kC];  // (x, y, lift) of candidate - v, candidate id (bits)
     __shared__ int s_succ[kV][kC];
     __shared__ __align__(16) int s_key[kV][kL];  // angle keys of the hull elimination
@@ -558,13 +562,16 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
             }
             kc = hi;
         }
-        // positions of the candidates and their filter rows relative to v
+        // positions of the candidates and their filter rows relative to v; a
+        // candidate on top of v (filter kernels: found here)
+        bool same = false;
         {
 #pragma clang fp contract(off)
             const float2 pv = m.pos[v];
             for (int j = lane; active && j < kc && j < kC; j += kL) {
                 const float2 pj = m.pos[list[j]];
-                s_pos[g][j] = pj;
+                if (kFast) same |= pj.x == pv.x && pj.y == pv.y;
+                else s_pos[g][j] = pj;
                 const float x = star_diff(pj.x, pv.x, periodic);
                 const float y = star_diff(pj.y, pv.y, periodic);
                 s_row[g][j] = make_float4(x, y, x * x + y * y, __int_as_float(list[j]));
@@ -575,20 +582,21 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
         int why = ok ? 0 : 4;  // failure reason slot in fail[] (0: none)
         const float2 pv = m.pos[v];
         // ---- start: the nearest candidate (largest inverted radius), always
-        // a hull vertex.  The full kernel takes exact squared distances; the
-        // filter kernel the FP32 lift of the filter rows, which can pick a
-        // near-tie that is not on the hull: the trace then does not close and
-        // the vertex goes to the full kernel.  A candidate on top of v makes
-        // the inversion meaningless.
+        // a hull vertex, where the scan starts its trace and the star is
+        // written from (kernels without the scan write it from the first
+        // corner in angle order: the star's rotation is arbitrary).  The full
+        // kernel takes exact squared distances; the filter kernel the FP32
+        // lift of the filter rows, which can pick a near-tie that is not on
+        // the hull: the trace then does not close and the vertex goes to the
+        // full kernel.  A candidate on top of v makes the inversion
+        // meaningless.
         int best = kCandMax;
-        bool same = false;
         int same_lo = 0x7fffffff;  // smallest id of a candidate on top of v (full kernel)
-        if (kFast) {
+        if (kFast && !kScan) {
+        } else if (kFast) {
             float dist = 3.4e38f;
             for (int j = lane; ok && j < kc; j += kL) {
                 const float4 rj = s_row[g][j];
-                const float2 pj = s_pos[g][j];
-                same |= pj.x == pv.x && pj.y == pv.y;
                 if (rj.z < dist) { dist = rj.z; best = j; }
             }
             for (int off = kL / 2; off >= 1; off >>= 1) {
@@ -631,8 +639,8 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
             }
         }
         const int start = best;
-        if (start >= kc && ok) { ok = false; why = 4; }
-        const int id_start = ok ? __float_as_int(s_row[g][start].w) : -1;
+        if (kScan && start >= kc && ok) { ok = false; why = 4; }
+        const int id_start = (kScan && ok) ? __float_as_int(s_row[g][start].w) : -1;
         // ---- the hull by elimination (filter kernel, at most 32 candidates).
         // Lane r takes the candidate of rank r in angle around v (the
         // inversion keeps directions from v).  Rounds drop every remaining
@@ -735,7 +743,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
             }
             const Mask gbad = HullMask<kL>::group(__ballot(!good), gbase);
             const Mask gturn = HullMask<kL>::group(__ballot(alive && turn), gbase);
-            const Mask gstart = HullMask<kL>::group(__ballot(alive && slot == start), gbase);
+            const Mask gstart = kScan ? HullMask<kL>::group(__ballot(alive && slot == start), gbase) : M;
             hull_done = try_hull && h >= 3 && h <= kStarCap && gbad == 0 && HullMask<kL>::popc(gturn) == 1 &&
                         gstart != 0;
             if (hull_done) {
@@ -756,7 +764,6 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
         }
         // the first pass (2-rings) leaves such a vertex to the full kernel:
         // without the scan below it keeps fewer registers, more waves per SIMD
-        constexpr bool kScan = !(kFast && kCand != kCandHash);
         if (!kScan && ok && !hull_done) {
             ok = false;
             why = 5;
