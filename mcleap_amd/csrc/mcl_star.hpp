@@ -1064,24 +1064,27 @@ __device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, c
 // (every vertex): 256 vertices share each barrier and flush of the append
 // buffer, where k_star_check's lanes per vertex make the short lists of the
 // later checks faster.  Checks the *nlist listed vertices (all when vlist is
-// null).  Vertices that fail (or whose star failed) are appended to `list`;
-// with mark_nbrs set, with their star neighbours and any inconsistency marks
-// v, otherwise only the side of a conflict that loses the incircle test
-// (star_inconsistent with `decide`).  Without a list the
-// degree sum of the valid stars is added to *sum_deg.  reset (may be null):
-// a counter this launch zeroes for a later kernel.
+// null).  Vertices that fail (or whose star failed) are appended to `list`:
+// of a conflict only the side that loses the incircle test
+// (star_inconsistent with `decide`), each vertex at most twice per tile (v,
+// and a neighbour whose star lacks v, with hint), which the append buffer
+// holds.  Without a list the degree sum of the valid stars is added to
+// *sum_deg.  reset (may be null): a counter this launch zeroes for a later
+// kernel.
 __global__ void __launch_bounds__(256) k_star_check_all(int nv, const int* __restrict__ vlist, const int* __restrict__ nlist,
                                                     const int* __restrict__ deg, const int* __restrict__ nbr,
                                                     const float2* __restrict__ pos, int periodic, int ncorner,
-                                                    int mark_nbrs, int* __restrict__ mark, const int* __restrict__ stamp_base,
+                                                    int* __restrict__ mark, const int* __restrict__ stamp_base,
                                                     int stamp_off, int* __restrict__ list, int* __restrict__ count,
                                                     long long* __restrict__ sum_deg, int* __restrict__ reset,
                                                     int* __restrict__ hint) {
-    __shared__ AppendBuf<> ab;
+    // (a quarter of the LDS of the default buffer, for more waves per SIMD:
+    // in the first check some 8% of the vertices fail)
+    __shared__ AppendBuf<2048> ab;
     const int stamp = *stamp_base + stamp_off;
     if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;  // a counter zeroed for a later kernel
     append_init(ab);
-    const int max_tile = (int)blockDim.x * (mark_nbrs ? kStarCap + 1 : 2);
+    const int max_tile = (int)blockDim.x * 2;
     long long my_sum = 0;
     const int n = vlist ? *nlist : nv;
     int first, end, stride;
@@ -1099,33 +1102,25 @@ __global__ void __launch_bounds__(256) k_star_check_all(int nv, const int* __res
         int missing = -1;
         if (active && !bad) {
             if (!vlist) my_sum += d;
-            bad = star_inconsistent(deg, nbr, v, d, q0, q1, !mark_nbrs, pos, periodic != 0, ncorner, &missing);
+            bad = star_inconsistent(deg, nbr, v, d, q0, q1, true, pos, periodic != 0, ncorner, &missing);
         }
-        // claim v and its neighbours once per stamp, then append the claimed
-        // ones.  With hint given, a neighbour u whose star lacks v is listed
-        // too, with v left as a candidate for it: when v has moved into u's star from
-        // beyond u's 2-ring, and u's other neighbours (recomputed without v
-        // as well) agree with u, no other check lists u, and u's 2-ring in the
-        // new stars does not reach v either.
-        unsigned claimed = 0;  // bit 0: v, bit 1 + i: neighbour i, bit 31: the one missing v
-        int nclaim = 0;
+        // claim v once per stamp, then append it.  With hint given, a
+        // neighbour u whose star lacks v is listed too, with v left as a
+        // candidate for it: when v has moved into u's star from beyond u's
+        // 2-ring, and u's other neighbours (recomputed without v as well)
+        // agree with u, no other check lists u, and u's 2-ring in the new
+        // stars does not reach v either.
+        bool claim_v = false, claim_m = false;
         if (bad) {
-            if (atomicExch(&mark[v], stamp) != stamp) { claimed |= 1u; nclaim++; }
-            for (int i = 0; mark_nbrs && i < d; i++)
-                if (row[i] >= ncorner && atomicExch(&mark[row[i]], stamp) != stamp) {
-                    claimed |= 2u << i;
-                    nclaim++;
-                }
+            claim_v = atomicExch(&mark[v], stamp) != stamp;
             if (hint && missing >= ncorner) {
                 hint[missing] = v;
-                if (atomicExch(&mark[missing], stamp) != stamp) { claimed |= 1u << 31; nclaim++; }
+                claim_m = atomicExch(&mark[missing], stamp) != stamp;
             }
         }
-        int at = append_reserve(ab, nclaim);
-        if (claimed & 1u) ab.buf[at++] = v;
-        for (int i = 0; i < d && i < kStarCap; i++)
-            if (claimed & (2u << i)) ab.buf[at++] = row[i];
-        if (claimed & (1u << 31)) ab.buf[at++] = missing;
+        int at = append_reserve(ab, (claim_v ? 1 : 0) + (claim_m ? 1 : 0));
+        if (claim_v) ab.buf[at++] = v;
+        if (claim_m) ab.buf[at++] = missing;
         append_tile_end(ab, max_tile, count, list);
     }
     append_finish(ab, count, list);

@@ -3076,7 +3076,7 @@ void star_check(mcl_tri* t, hipStream_t s, int nxt, int r, bool mark_nbrs, int s
     if (first)
         hipLaunchKernelGGL(k_star_check_all, G.full(m.nv, 1024), dim3(256), 0, s, m.nv, (const int*)nullptr,
                            (const int*)nullptr, (const int*)t->star_deg[nxt], (const int*)t->star_nbr[nxt],
-                           (const float2*)m.pos, m.periodic, m.ncorner, mark_nbrs ? 1 : 0, t->s_mark,
+                           (const float2*)m.pos, m.periodic, m.ncorner, t->s_mark,
                            (const int*)t->d_stamp, stamp_off, t->s_list, t->s_aux + star_list_slot(r),
                            (long long*)(t->s_aux + 2), t->s_heavy, (int*)nullptr);
     else
