@@ -368,16 +368,13 @@ def run_workload(a, M, torch, workload, steps, warmup, plain_steps, cpu_seconds,
         f_nbr = torch.zeros(f_cap, dtype=torch.int32, device=dev)
 
     def step(k, variant=a.variant):
+        # the whole timestep in one call: kinetic step, CSR one-ring, grid sort + FRNN
         if bgrid is None:
-            # the whole timestep in one call: kinetic step, CSR one-ring, grid sort + FRNN
             return dt.timestep(cfg["motion"], cfg["sigma"], cfg["amp"], seed=cfg["seed"], step=k, variant=variant,
                                ring=(ring_off, ring_nbr), grid=grid, rad=radius or 0.0,
                                frnn=(f_off, f_nbr) if grid is not None else None)
-        st = dt.step(cfg["motion"], cfg["sigma"], cfg["amp"], seed=cfg["seed"], step=k, variant=variant)
-        dt.one_ring_csr_async(ring_off, ring_nbr)
-        bgrid.build(pos_t)
-        bgrid.frnn_csr_async(f_off, f_nbr)
-        return st
+        return dt.timestep(cfg["motion"], cfg["sigma"], cfg["amp"], seed=cfg["seed"], step=k, variant=variant,
+                           ring=(ring_off, ring_nbr), bgrid=bgrid, frnn=(f_off, f_nbr))
 
     def timers(on):
         dt.kernel_timer(on)

@@ -292,6 +292,10 @@ void* mcl_tri_target_ptr(mcl_tri_t* t);
  * FRNN query into d_frnn_off / d_frnn_nbrs (as mcl_grid_build +
  * mcl_grid_frnn_csr with total = NULL: writes past frnn_cap dropped,
  * d_frnn_off[nump] the full count) when g is given (g sized for every vertex).
+ * The grid part runs on a second stream of the library's, on the targets
+ * (mcl_tri_target_ptr), beside the step (again after it, on `stream`, when a
+ * vertex did not reach its target); `stream` waits for it before the call
+ * returns, so every output is ordered on `stream` as if enqueued there.
  * With MCL_OPT_GRAPH the tail is one graph launch.  Status as mcl_tri_step. */
 int mcl_tri_timestep(mcl_tri_t* t, const mcl_step_params_t* p, void* d_ring_off, void* d_ring_nbrs, int ring_cap,
                      mcl_grid_t* g, float rad, void* d_frnn_off, void* d_frnn_nbrs, int frnn_cap, mcl_stats_t* st,
@@ -394,6 +398,8 @@ int mcl_grid_mean_occupancy(mcl_grid_t* g, float* out, void* stream);
  *   65536) may restore the triangulation by per-vertex star recomputation
  *   (torus, and box meshes with their corners first); smaller ones, where
  *   the number of launches dominates, use the flip-based step.  0 = always.
+ *   The same bound decides whether mcl_tri_timestep runs its grid part on a
+ *   second stream beside the step (at least this many vertices).
  * MCL_OPT_STAR_ENABLE: 0 = never use the star recomputation; 1 = every step
  *   (after a step where it did not settle, skipped for 1, 2, 4 ... 64 steps);
  *   2 (default) = the path whose last step was faster, the slower one timed
@@ -489,6 +495,14 @@ int mcl_bgrid_frnn_csr(mcl_bgrid_t* g, void* d_offsets, void* d_nbrs, int cap, l
 /* HIP-event timers: 0 FRNN count, 1 FRNN fill, 2 build */
 int mcl_bgrid_kernel_timer(mcl_bgrid_t* g, int enable);
 int mcl_bgrid_kernel_timer_read(mcl_bgrid_t* g, int which, double* total_ms, long long* launches);
+/* mcl_tri_timestep for an ensemble (no counterpart in the reference, whose
+ * driver loop steps one triangulation, moving_device_mesh.cu:109-243): the
+ * step, the CSR one-ring, and the batched grids' sort of the new positions
+ * with their CSR FRNN query (as mcl_bgrid_build + mcl_bgrid_frnn_csr with
+ * total = NULL) when bg is given; bg's points must be vertices of t. */
+int mcl_tri_timestep_batch(mcl_tri_t* t, const mcl_step_params_t* p, void* d_ring_off, void* d_ring_nbrs,
+                           int ring_cap, mcl_bgrid_t* bg, void* d_frnn_off, void* d_frnn_nbrs, int frnn_cap,
+                           mcl_stats_t* st, void* stream);
 
 /* ----------------------------------------------------------------- misc */
 /* library build string and the device it runs on */

@@ -124,6 +124,7 @@ SIGNATURES = {
     "mcl_tri_count_non_delaunay": [_p, _p, _p],
     "mcl_tri_step": [_p, _p, _p, _p],
     "mcl_tri_timestep": [_p, _p, _p, _p, _i, _p, _f, _p, _p, _i, _p, _p],
+    "mcl_tri_timestep_batch": [_p, _p, _p, _p, _i, _p, _p, _p, _i, _p, _p],
     "mcl_tri_one_ring": [_p, _i, _i, _p, ctypes.POINTER(_i), _p],
     "mcl_tri_one_ring_csr": [_p, _p, _p, _i, _p, _p],
     "mcl_tri_closest": [_p, _p, _p, _p, _p],

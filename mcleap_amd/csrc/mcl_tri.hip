@@ -1654,6 +1654,8 @@ struct TailReq {
     int* frnn_off;
     int* frnn_nbrs;
     int frnn_cap;
+    mcl_bgrid_t* bg = nullptr;  // an ensemble's per-instance grids (instead of g)
+    bool has_frnn() const { return g || bg; }
 };
 
 struct mcl_tri {
@@ -1740,6 +1742,8 @@ struct mcl_tri {
     hipEvent_t ev_fork = nullptr;           // caller's stream -> side stream (after the first star check)
     hipEvent_t ev_join = nullptr;           // side stream -> caller's stream (end of the FRNN part)
     bool side_pending = false;              // the caller's stream has not yet waited for ev_join
+    bool frnn_on_targets = false;           // a flip-based step's FRNN part ran on the targets (side stream)
+    int* d_offt = nullptr;                  // 8: vertices off their targets (k_off_target)
     // one ring
     int* deg = nullptr;
     int* scan_tmp = nullptr;
@@ -1786,7 +1790,7 @@ static int tri_free(mcl_tri* t) {
                     t->rev, t->allrev, t->pend[0], t->pend[1], t->deg, t->scan_tmp, t->v_touch, t->t_rep, t->t_rep_e, t->e_undo, t->undo, t->stuck, t->touched, t->r_tt, t->r_tt2, t->r_wait, t->r_defer, t->v_soft, t->tail_out,
                     t->star_deg[0], t->star_deg[1], t->star_nbr[0], t->star_nbr[1], t->s_ecnt, t->s_tcnt,
                     t->s_mark, t->s_list, t->s_aux, t->star_tdeg, t->star_tnbr, t->s_rlist, t->s_heavy, t->s_heavy2,
-                    t->s_hint, t->c_bucket, t->c_count, t->d_stamp, t->d_params};
+                    t->s_hint, t->c_bucket, t->c_count, t->d_stamp, t->d_params, t->d_offt};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int k = 0; k < 2; k++)
@@ -1895,7 +1899,7 @@ int tri_make(int nv, int nhe, int nt, int periodic, int ncorner, mcl_tri** out) 
     A(t->bad[0], nt); A(t->bad[1], nt); A(t->rev, nv); A(t->allrev, nv); A(t->pend[0], nv); A(t->pend[1], nv);
     A(t->deg, nv + 1); A(t->scan_tmp, scan_scratch_ints(nv + 1) + 16);
     A(t->v_touch, nv); A(t->t_rep, nt); A(t->t_rep_e, nt); A(t->e_undo, m.ne); A(t->undo, m.ne);
-    A(t->stuck, nt); A(t->touched, nv); A(t->r_tt, nv); A(t->r_tt2, nv); A(t->r_wait, nv); A(t->r_defer, nv); A(t->v_soft, nv); A(t->tail_out, 8);
+    A(t->stuck, nt); A(t->touched, nv); A(t->r_tt, nv); A(t->r_tt2, nv); A(t->r_wait, nv); A(t->r_defer, nv); A(t->v_soft, nv); A(t->tail_out, 8); A(t->d_offt, 8);
     if (periodic || m.ncorner > 0) {
         A(t->star_deg[0], nv); A(t->star_deg[1], nv);
         // vertex rows, then the corners' fans (star_row); star_tnbr also holds
@@ -2848,6 +2852,8 @@ static int reloc_deferred(mcl_tri* t, hipStream_t s, bool f32, int variant, cons
 //   relocate  -- vertices sent back are removed from their stars and
 //                re-inserted at their targets
 //   Lawson    -- flip rounds to the exact Delaunay triangulation
+static int fork_side(mcl_tri* t, hipStream_t s, int nxt);
+
 static int step_impl(mcl_tri* t, const mcl_step_params_t* p, const float2* disp, mcl_stats_t* st, void* stream) {
     t->stars_valid = false;  // flips and relocations change the topology
     hipStream_t s = (hipStream_t)stream;
@@ -2864,6 +2870,10 @@ static int step_impl(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     hipLaunchKernelGGL(k_move_motion, dim3(blocks_for(m.nv)), dim3(kThreads), 0, s, m, t->acc, t->tgt, t->v_mv,
                        t->v_fexp, mv_stamp, *p, (const mcl_step_params_t*)nullptr, disp, StepPrologue{});
     t->timers[2].mark(s);
+    // the timestep's FRNN part beside the whole step, on the targets (where
+    // every vertex ends unless one stalls: checked after the step)
+    MCL_TRY(fork_side(t, s, t->star_cur));
+    if (t->side_pending) t->frnn_on_targets = true;
     MCL_TRY(zero_slots(t, s, kCnBad, 11));  // bad, rev, move, pend, err, misc, touched
     t->timers[1].mark(s);
     if (f32)
@@ -2979,7 +2989,8 @@ static int ensure_mesh(mcl_tri* t, hipStream_t s) {
 // counters once; the rare step that has not settled continues eagerly, one
 // read per pass.
 
-static int launch_tail(mcl_tri* t, const TailReq& q, hipStream_t s, int part, int c, bool from_stars);
+static int launch_tail(mcl_tri* t, const TailReq& q, hipStream_t s, int part, int c, bool from_stars,
+                       const float2* pos = nullptr);
 
 namespace {
 // launch geometry of the star kernels
@@ -3124,24 +3135,6 @@ void star_corner_fans(mcl_tri* t, hipStream_t s, int cur, int nxt) {
                        (long long*)(t->s_aux + 16));
 }
 
-// The FRNN part of the calling timestep's tail (grid sort and CSR query) on a
-// second stream, once the first check has run: it reads the positions only,
-// and runs beside the retry passes, a few launches over short lists that
-// leave most of the GPU idle, instead of after them.  The caller's stream
-// waits for it before the timestep returns (join_side).  Eager steps only
-// (a captured head keeps the tail on the caller's stream).
-int fork_side(mcl_tri* t, hipStream_t s, int nxt) {
-    if (t->capturing || t->graph || !t->tail_req || !t->tail_req->g) return MCL_OK;
-    if (!t->side_stream) MCL_HIP(hipStreamCreateWithFlags(&t->side_stream, hipStreamNonBlocking));
-    MCL_HIP(hipEventRecord(t->ev_fork, s));
-    MCL_HIP(hipStreamWaitEvent(t->side_stream, t->ev_fork, 0));
-    t->side_pending = true;
-    MCL_TRY(launch_tail(t, *t->tail_req, t->side_stream, 0, nxt, true));
-    MCL_HIP(hipEventRecord(t->ev_join, t->side_stream));
-    t->frnn_done = true;
-    return MCL_OK;
-}
-
 // the head of a star step (see above); pdev: the step parameters in device
 // memory (graph), else p by value.  Ends with the corner fans, so that the
 // counters read after it describe a complete set of stars.
@@ -3166,6 +3159,31 @@ int star_head(mcl_tri* t, hipStream_t s, int cur, int nxt, const mcl_step_params
     return MCL_OK;
 }
 }  // namespace
+
+// The FRNN part of the calling timestep's tail (grid sort and CSR query) on a
+// second stream, on the targets the move kernel wrote (where every vertex
+// ends the step unless it stalls or is left on a duplicate target): from the
+// first check of a star step on, or beside the whole of a flip-based step.
+// It reads the targets only, and runs beside launches over short lists and
+// host round trips that leave most of the GPU idle, instead of after them.
+// The caller's stream waits for it before the timestep returns (join_side);
+// it runs again after the step when a vertex is not at its target.  Eager
+// steps only (a captured head keeps the tail on the caller's stream), and
+// meshes of MCL_OPT_STAR_MIN_VERTICES or more (below, where launches and
+// host round trips dominate, the query takes less time than the fork, the
+// join and the check of the targets).
+static int fork_side(mcl_tri* t, hipStream_t s, int nxt) {
+    if (t->capturing || t->graph || !t->tail_req || !t->tail_req->has_frnn() || t->m.nv < t->star_min_vertices)
+        return MCL_OK;
+    if (!t->side_stream) MCL_HIP(hipStreamCreateWithFlags(&t->side_stream, hipStreamNonBlocking));
+    MCL_HIP(hipEventRecord(t->ev_fork, s));
+    MCL_HIP(hipStreamWaitEvent(t->side_stream, t->ev_fork, 0));
+    t->side_pending = true;
+    MCL_TRY(launch_tail(t, *t->tail_req, t->side_stream, 0, nxt, true, t->tgt));
+    MCL_HIP(hipEventRecord(t->ev_join, t->side_stream));
+    t->frnn_done = true;
+    return MCL_OK;
+}
 
 // the caller's stream waits for the side stream's work (if any is pending)
 static int join_side(mcl_tri* t, hipStream_t s) {
@@ -3218,16 +3236,23 @@ static int enqueue_ring_csr(mcl_tri* t, hipStream_t s, int c, bool from_stars, i
 // captured per part (and star buffer), captured again when the caller's
 // buffers change.  Only device work, no host read: running a part again over
 // the same positions and stars gives the same arrays.
-static int launch_tail(mcl_tri* t, const TailReq& q, hipStream_t s, int part, int c, bool from_stars) {
+static int launch_tail(mcl_tri* t, const TailReq& q, hipStream_t s, int part, int c, bool from_stars,
+                       const float2* pos) {
+    if (!pos) pos = t->m.pos;
     auto enqueue = [&](hipStream_t cs) -> int {
-        if (part == 0 && q.g) MCL_TRY(grid_enqueue_frnn(q.g, t->m.pos, q.rad, q.frnn_off, q.frnn_nbrs, q.frnn_cap, cs));
+        if (part == 0 && q.g) MCL_TRY(grid_enqueue_frnn(q.g, pos, q.rad, q.frnn_off, q.frnn_nbrs, q.frnn_cap, cs));
+        if (part == 0 && q.bg) {
+            MCL_TRY(mcl_bgrid_build(q.bg, pos, cs));
+            MCL_TRY(mcl_bgrid_frnn_csr(q.bg, q.frnn_off, q.frnn_nbrs, q.frnn_cap, nullptr, cs));
+        }
         if (part == 1 && q.ring_off) MCL_TRY(enqueue_ring_csr(t, cs, c, from_stars, q.ring_off, q.ring_nbrs, q.ring_cap));
         return MCL_OK;
     };
-    if ((part == 0 && !q.g) || (part == 1 && !q.ring_off)) return MCL_OK;
+    if ((part == 0 && !q.has_frnn()) || (part == 1 && !q.ring_off)) return MCL_OK;
     if (!t->graph || (part == 1 && !from_stars)) return enqueue(s);
     float radv = q.rad;
-    const void* key[8] = {q.ring_off, q.ring_nbrs, (const void*)(intptr_t)q.ring_cap, q.g,
+    const void* key[8] = {q.ring_off, q.ring_nbrs, (const void*)(intptr_t)q.ring_cap,
+                          q.g ? (const void*)q.g : (const void*)q.bg,
                           (const void*)(intptr_t)*(const int*)&radv, q.frnn_off, q.frnn_nbrs,
                           (const void*)(intptr_t)q.frnn_cap};
     if (memcmp(key, t->post_key, sizeof(key)) != 0) {
@@ -3284,7 +3309,7 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         MCL_HIP(hipMemcpyAsync(t->h_stats, t->stats, sizeof(unsigned long long) * kStCount * kStatShards,
                                hipMemcpyDeviceToHost, s));
         MCL_HIP(hipEventRecord(t->ev_aux, s));
-        if (first && t->tail_req && t->tail_req->g && !t->frnn_done) {
+        if (first && t->tail_req && t->tail_req->has_frnn() && !t->frnn_done) {
             MCL_TRY(launch_tail(t, *t->tail_req, s, 0, nxt, true));
             t->frnn_done = true;
         }
@@ -3667,29 +3692,62 @@ static int enqueue_ring_csr(mcl_tri* t, hipStream_t s, int c, bool from_stars, i
     return MCL_OK;
 }
 
-extern "C" int mcl_tri_timestep(mcl_tri_t* t, const mcl_step_params_t* p, void* d_ring_off, void* d_ring_nbrs,
-                                int ring_cap, mcl_grid_t* g, float rad, void* d_frnn_off, void* d_frnn_nbrs,
-                                int frnn_cap, mcl_stats_t* st, void* stream) {
-    if (!t || !p || (d_ring_off && (!d_ring_nbrs || ring_cap < t->m.nhe)) || (g && !d_frnn_off)) return MCL_ERR_INPUT;
-    hipStream_t s = (hipStream_t)stream;
-    const TailReq q{(int*)d_ring_off, (int*)d_ring_nbrs, ring_cap, g, rad, (int*)d_frnn_off, (int*)d_frnn_nbrs,
-                    frnn_cap};
-    const bool want = d_ring_off || g;
-    // a star step enqueues the FRNN part right after its counter read, before
-    // it waits for that read (see star_step); frnn_done: it need not run again
+// the whole timestep with the tail q (see mcl_tri_timestep)
+static int timestep_impl(mcl_tri* t, const mcl_step_params_t* p, const TailReq& q, mcl_stats_t* st, hipStream_t s) {
+    const bool want = q.ring_off || q.has_frnn();
+    // a star step enqueues the FRNN part on the side stream after its first
+    // check (a flip-based step right after its move); frnn_done: it need not
+    // run again
     t->tail_req = want ? &q : nullptr;
     t->frnn_done = t->ring_done = false;
-    int rc = mcl_tri_step(t, p, st, stream);
+    t->frnn_on_targets = false;
+    int rc = mcl_tri_step(t, p, st, s);
     t->tail_req = nullptr;
     // the FRNN part run on the side stream (if any) is part of this call's work
     const int jrc = join_side(t, s);
     if (rc != MCL_OK && rc != MCL_ERR_UNRESOLVED) return rc;
     MCL_TRY(jrc);
     if (!want) return rc;
+    if (t->frnn_on_targets && t->frnn_done) {
+        // a flip-based step's FRNN part ran on the targets: it holds the final
+        // positions when every vertex is at its target
+        t->frnn_on_targets = false;
+        hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, s, t->d_offt, 6, 0);
+        hipLaunchKernelGGL(k_off_target, dim3(blocks_for(t->m.nv)), dim3(kThreads), 0, s, t->m,
+                           (const float2*)t->tgt, t->d_offt);
+        int off = 0;
+        MCL_HIP(hipMemcpyAsync(&off, t->d_offt, sizeof(int), hipMemcpyDeviceToHost, s));
+        MCL_HIP(hipStreamSynchronize(s));
+        if (off != 0 || rc != MCL_OK) t->frnn_done = false;
+    }
     if (!t->frnn_done) MCL_TRY(launch_tail(t, q, s, 0, t->star_cur, t->stars_valid));
     if (!t->stars_valid) MCL_TRY(ensure_mesh(t, s));
     if (!t->ring_done || !t->stars_valid) MCL_TRY(launch_tail(t, q, s, 1, t->star_cur, t->stars_valid));
     return rc;
+}
+
+extern "C" int mcl_tri_timestep(mcl_tri_t* t, const mcl_step_params_t* p, void* d_ring_off, void* d_ring_nbrs,
+                                int ring_cap, mcl_grid_t* g, float rad, void* d_frnn_off, void* d_frnn_nbrs,
+                                int frnn_cap, mcl_stats_t* st, void* stream) {
+    if (!t || !p || (d_ring_off && (!d_ring_nbrs || ring_cap < t->m.nhe)) || (g && !d_frnn_off)) return MCL_ERR_INPUT;
+    const TailReq q{(int*)d_ring_off, (int*)d_ring_nbrs, ring_cap, g, rad, (int*)d_frnn_off, (int*)d_frnn_nbrs,
+                    frnn_cap};
+    return timestep_impl(t, p, q, st, (hipStream_t)stream);
+}
+
+extern "C" int mcl_tri_timestep_batch(mcl_tri_t* t, const mcl_step_params_t* p, void* d_ring_off, void* d_ring_nbrs,
+                                      int ring_cap, mcl_bgrid_t* bg, void* d_frnn_off, void* d_frnn_nbrs,
+                                      int frnn_cap, mcl_stats_t* st, void* stream) {
+    if (!t || !p || (d_ring_off && (!d_ring_nbrs || ring_cap < t->m.nhe)) || (bg && !d_frnn_off)) return MCL_ERR_INPUT;
+    if (bg) {
+        int npoints = 0, first = 0, ncells = 0;
+        MCL_TRY(mcl_bgrid_sizes(bg, &npoints, &first, &ncells));
+        if (first + npoints > t->m.nv) return MCL_ERR_INPUT;  // its points must be vertices of this mesh
+    }
+    TailReq q{(int*)d_ring_off, (int*)d_ring_nbrs, ring_cap, nullptr, 0.0f, (int*)d_frnn_off, (int*)d_frnn_nbrs,
+              frnn_cap};
+    q.bg = bg;
+    return timestep_impl(t, p, q, st, (hipStream_t)stream);
 }
 
 extern "C" int mcl_tri_one_ring_csr(mcl_tri_t* t, void* d_offsets, void* d_nbrs, int cap, int* total, void* stream) {

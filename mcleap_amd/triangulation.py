@@ -503,15 +503,26 @@ class DeviceTriangulation:
         return self.last_stats
 
     def timestep(self, motion, sigma, amp=0.0, seed=1, step=0, variant=0, ring=None, grid=None, rad=0.0,
-                 frnn=None):
+                 frnn=None, bgrid=None):
         """one whole timestep (mcl_tri_timestep): the kinetic step, then
         asynchronously the CSR one-ring into ring = (offsets, nbrs) and the
         grid's sort of the new positions with its CSR FRNN query into
-        frnn = (offsets, nbrs) (grid: a GridCount2d over every vertex)"""
+        frnn = (offsets, nbrs) (grid: a GridCount2d over every vertex; or,
+        for an ensemble, bgrid: a GridCount2dBatch, mcl_tri_timestep_batch)"""
         p = _lib.StepParams(int(motion), float(sigma), float(amp), int(seed), int(step), int(variant), 0)
         st = _lib.Stats()
         ro, rn = ring if ring is not None else (None, None)
         fo, fn = frnn if frnn is not None else (None, None)
+        if bgrid is not None:
+            if grid is not None:
+                raise ValueError("grid or bgrid, not both")
+            rc = lib().mcl_tri_timestep_batch(self._h, ctypes.byref(p), _dev_ptr(ro), _dev_ptr(rn),
+                                              0 if rn is None else int(rn.numel()), bgrid._g, _dev_ptr(fo),
+                                              _dev_ptr(fn), 0 if fn is None else int(fn.numel()),
+                                              ctypes.byref(st), None)
+            self.last_stats = st.as_dict()
+            check(rc, "mcl_tri_timestep_batch")
+            return self.last_stats
         rc = lib().mcl_tri_timestep(self._h, ctypes.byref(p), _dev_ptr(ro), _dev_ptr(rn),
                                     0 if rn is None else int(rn.numel()), None if grid is None else grid._g,
                                     float(rad), _dev_ptr(fo), _dev_ptr(fn), 0 if fn is None else int(fn.numel()),

@@ -108,3 +108,51 @@ def test_batch_c5_sizes(oracle):
         assert dt.count_inverted() == 0 and dt.count_non_delaunay() == 0
         assert_at_targets(dt, st, "step %d" % k)
     _check_batch(O=oracle, B=B, dt=dt, tag="after 6 steps", instances=[0, 1, 17, 63])
+
+
+@pytest.mark.parametrize("star,sigma", [(True, 2e-3), (False, 2e-3), (False, 3e-2)])
+def test_batch_timestep_tail_equals_fresh_queries(oracle, star, sigma):
+    """mcl_tri_timestep_batch: the ensemble's step with its CSR one-ring and
+    the batched grids' FRNN query in one call (the FRNN part runs on a
+    second stream on the targets, beside the step, and again after it when a
+    vertex is not at its target: moves of sigma = 3e-2 clamp points onto the
+    sides and corners of the unit square, where exact duplicates stay
+    behind).  After every step the ring and FRNN arrays it wrote must equal
+    queries made afresh on the resulting mesh and positions."""
+    import torch
+    pts = ensemble_points(seed=11, sizes=batch_sizes())
+    B = M.HostTriangulationBatch(pts)
+    dt = M.DeviceTriangulation(B)
+    dt.set_option(M.OPT_STAR_MIN_VERTICES, 0)
+    dt.set_option(M.OPT_STAR_ENABLE, 1 if star else 0)
+    rad = np.minimum(np.array([2.0 / np.sqrt(max(len(p), 1)) for p in pts], np.float32), np.float32(1.0 / 3.0))
+    bg = M.GridCount2dBatch(B.vstart, rad)
+    fresh = M.GridCount2dBatch(B.vstart, rad)
+    nv, n = dt.m_pos_size, bg.numP
+    fresh.build(dt.pos_tensor())
+    cap = int(fresh.frnn_csr(torch.zeros(n + 1, dtype=torch.int32, device="cuda")) * 2) + 4096
+    fo = torch.zeros(n + 1, dtype=torch.int32, device="cuda")
+    fn = torch.zeros(cap, dtype=torch.int32, device="cuda")
+    fo2 = torch.zeros(n + 1, dtype=torch.int32, device="cuda")
+    fn2 = torch.zeros(cap, dtype=torch.int32, device="cuda")
+    ro = torch.zeros(nv + 1, dtype=torch.int32, device="cuda")
+    rn = torch.zeros(dt.m_he_size, dtype=torch.int32, device="cuda")
+    ro2 = torch.zeros(nv + 1, dtype=torch.int32, device="cuda")
+    rn2 = torch.zeros(dt.m_he_size, dtype=torch.int32, device="cuda")
+    dups = 0
+    for k in range(5):
+        st = dt.timestep(M.MOTION_GAUSS, sigma, seed=5, step=k, ring=(ro, rn), bgrid=bg, frnn=(fo, fn))
+        print("step", k, {x: st[x] for x in ("rounds", "fallbacks", "stalled", "duplicates")}, flush=True)
+        dups += st["duplicates"] + st["stalled"]
+        torch.cuda.synchronize()
+        fresh.build(dt.pos_tensor())
+        tot = fresh.frnn_csr(fo2, fn2)
+        assert tot <= cap
+        assert torch.equal(fo, fo2), "step %d: FRNN offsets differ" % k
+        assert torch.equal(fn[:tot], fn2[:tot]), "step %d: FRNN lists differ" % k
+        tr = dt.one_ring_csr(ro2, rn2)
+        assert torch.equal(ro, ro2), "step %d: ring offsets differ" % k
+        assert torch.equal(rn[:tr], rn2[:tr]), "step %d: rings differ" % k
+        assert_at_targets(dt, st, "step %d" % k)
+    _check_batch(oracle, B, dt, "after the timesteps")
+    print("vertices left behind over the steps:", dups)
