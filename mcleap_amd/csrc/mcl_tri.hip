@@ -474,18 +474,38 @@ __global__ void __launch_bounds__(kThreads) k_filter_bad(DMesh m, const int* __r
     const int n = cnt[slot_in];
     unsigned loc[4] = {0, 0, 0, 0};
     const int slot_c = kCnCand + round % 3;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int t = bad[i];
-        int a, b, c, st;
-        tri_verts(m, t, a, b, c);
-        if (orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st) > 0) { bump(loc, st); continue; }
-        bump(loc, st);
-        bad_out[agg_inc(&cnt[slot_out])] = t;
-        const int h0 = m.t_he[t], h1 = m.he_next[h0], h2 = m.he_next[h1];
-        const int es[3] = {h0 >> 1, h1 >> 1, h2 >> 1};
+    // both lists through LDS, one global reservation per flush (a wave-level
+    // atomic per list and wave serialises thousands of waves on two counters)
+    __shared__ AppendBuf<kListBuf> ab_bad, ab_cand;
+    append_init(ab_bad);
+    append_init(ab_cand);
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const int i = base + threadIdx.x;
+        bool keep = false;
+        int t = -1, es[3] = {-1, -1, -1};
+        if (i < n) {
+            t = bad[i];
+            int a, b, c, st;
+            tri_verts(m, t, a, b, c);
+            keep = orient<kF32>(m.pos[a], m.pos[b], m.pos[c], m.periodic, &st) <= 0;
+            bump(loc, st);
+            if (keep) {
+                const int h0 = m.t_he[t], h1 = m.he_next[h0], h2 = m.he_next[h1];
+                es[0] = h0 >> 1; es[1] = h1 >> 1; es[2] = h2 >> 1;
+                for (int k = 0; k < 3; k++)
+                    if (atomicExch(&e_stamp[es[k]], round) == round) es[k] = -1;
+            }
+        }
+        const int at = append_reserve(ab_bad, keep ? 1 : 0);
+        if (keep) ab_bad.buf[at] = t;
+        int p = append_reserve(ab_cand, (es[0] >= 0) + (es[1] >= 0) + (es[2] >= 0));
         for (int k = 0; k < 3; k++)
-            if (atomicExch(&e_stamp[es[k]], round) != round) cand[agg_inc(&cnt[slot_c])] = es[k];
+            if (es[k] >= 0) ab_cand.buf[p++] = es[k];
+        append_tile_end(ab_bad, (int)blockDim.x, &cnt[slot_out], bad_out);
+        append_tile_end(ab_cand, 3 * (int)blockDim.x, &cnt[slot_c], cand);
     }
+    append_finish(ab_bad, &cnt[slot_out], bad_out);
+    append_finish(ab_cand, &cnt[slot_c], cand);
     flush_stats(loc, stats, kStOrient0);
 }
 
@@ -501,24 +521,40 @@ __global__ void __launch_bounds__(kThreads) k_repair_flip(DMesh m, int* __restri
     const int r3 = round % 3;
     const int n = cnt[kCnWants + r3];
     unsigned loc[1] = {0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int e = wants[i];
-        const int2 tt = reinterpret_cast<const int2*>(m.he_t)[e];
-        const unsigned long long key = claim_key(round, e);
-        if (claim[tt.x] == key && claim[tt.y] == key) {
-            const int2 vv = reinterpret_cast<const int2*>(m.he_v)[e];
-            const int2 oo = reinterpret_cast<const int2*>(m.he_op)[e];
-            int h2, h3, h4, h5;
-            flip_soa(m, e, h2, h3, h4, h5);
-            t_rep[tt.x] = rep_stamp; t_rep[tt.y] = rep_stamp;
-            t_rep_e[tt.x] = e; t_rep_e[tt.y] = e;
-            const int qv[4] = {vv.x, vv.y, oo.x, oo.y};
-            for (int k = 0; k < 4; k++)
-                if (atomicExch(&v_touch[qv[k]], rep_stamp) != rep_stamp) touched[agg_inc(&cnt[slot_touched])] = qv[k];
-            agg_inc(&cnt[kCnErr]);
-            loc[0]++;
+    // the touched list through LDS and the flip count per block (wave-level
+    // atomics on one counter serialise thousands of waves)
+    __shared__ AppendBuf<kListBuf> ab;
+    __shared__ int s_flips;
+    if (threadIdx.x == 0) s_flips = 0;
+    append_init(ab);
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const int i = base + threadIdx.x;
+        int qv[4] = {-1, -1, -1, -1};
+        if (i < n) {
+            const int e = wants[i];
+            const int2 tt = reinterpret_cast<const int2*>(m.he_t)[e];
+            const unsigned long long key = claim_key(round, e);
+            if (claim[tt.x] == key && claim[tt.y] == key) {
+                const int2 vv = reinterpret_cast<const int2*>(m.he_v)[e];
+                const int2 oo = reinterpret_cast<const int2*>(m.he_op)[e];
+                int h2, h3, h4, h5;
+                flip_soa(m, e, h2, h3, h4, h5);
+                t_rep[tt.x] = rep_stamp; t_rep[tt.y] = rep_stamp;
+                t_rep_e[tt.x] = e; t_rep_e[tt.y] = e;
+                qv[0] = vv.x; qv[1] = vv.y; qv[2] = oo.x; qv[3] = oo.y;
+                for (int k = 0; k < 4; k++)
+                    if (atomicExch(&v_touch[qv[k]], rep_stamp) == rep_stamp) qv[k] = -1;
+                atomicAdd(&s_flips, 1);
+                loc[0]++;
+            }
         }
+        int p = append_reserve(ab, (qv[0] >= 0) + (qv[1] >= 0) + (qv[2] >= 0) + (qv[3] >= 0));
+        for (int k = 0; k < 4; k++)
+            if (qv[k] >= 0) ab.buf[p++] = qv[k];
+        append_tile_end(ab, 4 * (int)blockDim.x, &cnt[slot_touched], touched);
     }
+    append_finish(ab, &cnt[slot_touched], touched);
+    if (threadIdx.x == 0 && s_flips) atomicAdd(&cnt[kCnErr], s_flips);
     flush_stats(loc, stats, kStUntangleFlips);
 }
 
