@@ -1672,6 +1672,8 @@ __global__ void __launch_bounds__(kThreads) k_frnn_graph(DMesh m, float rr, int 
 }  // namespace mcl
 
 #include "mcl_star.hpp"
+// unsigned long longs after the statistics shards: the star counters (s_aux)
+constexpr int kStatsTail = (kStarAux + 1) / 2;
 
 // =========================================================================
 // host side
@@ -1750,7 +1752,7 @@ struct mcl_tri {
     int* s_mark = nullptr;                  // nv
     int* s_list = nullptr;                  // nv
     int* s_rlist = nullptr;                 // nv: vertices to re-check
-    int* s_aux = nullptr;                   // kStarAux counters (see star_step)
+    int* s_aux = nullptr;                   // kStarAux counters (see star_step), inside the stats allocation
     int* s_heavy = nullptr;                 // nv + 1: count, then rows for the full star kernel
     int* s_heavy2 = nullptr;                // nv + 1: the first pass's second list (left by its wave-per-vertex kernel)
     int* s_hint = nullptr;                  // nv: a further candidate per vertex, left by the checks
@@ -1766,7 +1768,6 @@ struct mcl_tri {
     bool capturing = false;                 // kernel timers are not recorded under capture
     bool captured = false;                  // a graph was captured during the current step
     hipStream_t cap_stream = nullptr;       // stream the graphs are captured on (the caller's may be the null stream)
-    int* h_aux = nullptr;                   // pinned copy of s_aux, read after a star step's passes
     hipEvent_t ev_step = nullptr;           // start of a timed kinetic step (path choice)
     hipEvent_t ev_aux = nullptr;            // after a star step's last counter copy: the end of its own work
     hipEvent_t ev_end = nullptr;            // end of a timed flip-based step
@@ -1825,7 +1826,7 @@ static int tri_free(mcl_tri* t) {
                     t->scalar, t->acc, t->tgt, t->v_mv, t->v_rev, t->v_fexp, t->t_stamp, t->bad[0], t->bad[1],
                     t->rev, t->allrev, t->pend[0], t->pend[1], t->deg, t->scan_tmp, t->v_touch, t->t_rep, t->t_rep_e, t->e_undo, t->undo, t->stuck, t->touched, t->r_tt, t->r_tt2, t->r_wait, t->r_defer, t->v_soft, t->tail_out,
                     t->star_deg[0], t->star_deg[1], t->star_nbr[0], t->star_nbr[1], t->s_ecnt, t->s_tcnt,
-                    t->s_mark, t->s_list, t->s_aux, t->star_tdeg, t->star_tnbr, t->s_rlist, t->s_heavy, t->s_heavy2,
+                    t->s_mark, t->s_list, t->star_tdeg, t->star_tnbr, t->s_rlist, t->s_heavy, t->s_heavy2,
                     t->s_hint, t->c_bucket, t->c_count, t->d_stamp, t->d_params, t->d_offt};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -1834,7 +1835,6 @@ static int tri_free(mcl_tri* t) {
     for (int k = 0; k < 3; k++)
         if (t->g_post[k]) (void)hipGraphExecDestroy(t->g_post[k]);
     if (t->h_params) (void)hipHostFree(t->h_params);
-    if (t->h_aux) (void)hipHostFree(t->h_aux);
     for (hipEvent_t e : {t->ev_step, t->ev_aux, t->ev_end, t->ev_fork, t->ev_join})
         if (e) (void)hipEventDestroy(e);
     if (t->cap_stream) (void)hipStreamDestroy(t->cap_stream);
@@ -1930,7 +1930,7 @@ int tri_make(int nv, int nhe, int nt, int periodic, int ncorner, mcl_tri** out) 
     A(m.pos, nv); A(m.he_next, nhe); A(m.he_v, nhe); A(m.he_t, nhe); A(m.he_op, nhe);
     A(m.v_he, nv); A(m.v_pidx, nv); A(m.t_he, nt);
     A(t->claim, nt); A(t->e_stamp, m.ne); A(t->cand[0], m.ne); A(t->cand[1], m.ne); A(t->wants, m.ne);
-    A(t->cnt, kCnCount); A(t->rounds_done, 1); A(t->stats, (size_t)kStCount * kStatShards); A(t->scalar, 4);
+    A(t->cnt, kCnCount); A(t->rounds_done, 1); A(t->stats, (size_t)kStCount * kStatShards + kStatsTail); A(t->scalar, 4);
     A(t->acc, nv); A(t->tgt, nv); A(t->v_mv, nv); A(t->v_rev, nv); A(t->v_fexp, nv); A(t->t_stamp, nt);
     A(t->bad[0], nt); A(t->bad[1], nt); A(t->rev, nv); A(t->allrev, nv); A(t->pend[0], nv); A(t->pend[1], nv);
     A(t->deg, nv + 1); A(t->scan_tmp, scan_scratch_ints(nv + 1) + 16);
@@ -1942,20 +1942,23 @@ int tri_make(int nv, int nhe, int nt, int periodic, int ncorner, mcl_tri** out) 
         // the rebuild's half-edge ids in the same layout
         A(t->star_nbr[0], star_rows_size(nv, m.ncorner)); A(t->star_nbr[1], star_rows_size(nv, m.ncorner));
         A(t->star_tdeg, nv); A(t->star_tnbr, star_rows_size(nv, m.ncorner));
-        A(t->s_ecnt, nv + 1); A(t->s_tcnt, nv + 1); A(t->s_mark, nv); A(t->s_list, nv); A(t->s_aux, kStarAux);
+        A(t->s_ecnt, nv + 1); A(t->s_tcnt, nv + 1); A(t->s_mark, nv); A(t->s_list, nv);
         A(t->s_rlist, nv); A(t->s_heavy, nv + 1); A(t->s_heavy2, nv + 1); A(t->s_hint, nv); A(t->d_stamp, 1); A(t->d_params, 1);
         if (m.ncorner > 0) { A(t->c_bucket, (size_t)m.ncorner * kCornerCap * 3); A(t->c_count, m.ncorner); }
     }
 #undef A
     if (rc == MCL_OK && hipHostMalloc((void**)&t->h_cnt, sizeof(int) * kCnCount) != hipSuccess) rc = MCL_ERR_HIP;
     if (rc == MCL_OK && hipHostMalloc((void**)&t->h_params, sizeof(mcl_step_params_t)) != hipSuccess) rc = MCL_ERR_HIP;
-    if (rc == MCL_OK && hipHostMalloc((void**)&t->h_aux, sizeof(int) * kStarAux) != hipSuccess) rc = MCL_ERR_HIP;
+    // the star counters live right after the statistics shards, so that one
+    // copy brings both to the host
+    if (rc == MCL_OK && t->star_deg[0]) t->s_aux = reinterpret_cast<int*>(t->stats + (size_t)kStCount * kStatShards);
     for (hipEvent_t* e : {&t->ev_step, &t->ev_aux, &t->ev_end})
         if (rc == MCL_OK && hipEventCreate(e) != hipSuccess) rc = MCL_ERR_HIP;
     for (hipEvent_t* e : {&t->ev_fork, &t->ev_join})
         if (rc == MCL_OK && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) rc = MCL_ERR_HIP;
     if (rc == MCL_OK &&
-        hipHostMalloc((void**)&t->h_stats, sizeof(unsigned long long) * kStCount * kStatShards) != hipSuccess)
+        hipHostMalloc((void**)&t->h_stats, sizeof(unsigned long long) * ((size_t)kStCount * kStatShards + kStatsTail)) !=
+            hipSuccess)
         rc = MCL_ERR_HIP;
     if (rc == MCL_OK) {
         hipLaunchKernelGGL(k_set_u64, dim3(blocks_for(nt)), dim3(kThreads), 0, 0, t->claim, nt, 0ull);
@@ -3341,8 +3344,8 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
     // the step (frnn_done false).
     unsigned long long moved_back = 0;
     auto read_aux = [&](int pass, bool first) -> int {
-        MCL_HIP(hipMemcpyAsync(t->h_aux, t->s_aux, sizeof(aux), hipMemcpyDeviceToHost, s));
-        MCL_HIP(hipMemcpyAsync(t->h_stats, t->stats, sizeof(unsigned long long) * kStCount * kStatShards,
+        // the statistics shards and, right after them, the star counters: one copy
+        MCL_HIP(hipMemcpyAsync(t->h_stats, t->stats, sizeof(unsigned long long) * ((size_t)kStCount * kStatShards + kStatsTail),
                                hipMemcpyDeviceToHost, s));
         MCL_HIP(hipEventRecord(t->ev_aux, s));
         if (first && t->tail_req && t->tail_req->has_frnn() && !t->frnn_done) {
@@ -3358,7 +3361,7 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
             t->ring_done = true;
         }
         MCL_HIP(hipEventSynchronize(t->ev_aux));
-        memcpy(aux, t->h_aux, sizeof(aux));
+        memcpy(aux, t->h_stats + (size_t)kStCount * kStatShards, sizeof(aux));
         t->h_stats_fresh = true;
         unsigned long long mb = 0;
         for (int j = 0; j < kStatShards; j++)
