@@ -756,10 +756,14 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                 h_id = myid;
                 loc[0] += tests;
             }
-            // filter-kernel vertices that took the scan (one count per wave)
+            // filter-kernel vertices that took the scan (one count per wave);
+            // the wave-per-vertex kernels also count the vertices with more
+            // candidates than lanes (fail[22]) and uncertified hulls (fail[23])
             if (kFast) {
                 const unsigned long long fb = __ballot(lane == 0 && try_hull && !hull_done);
                 if (fb && (threadIdx.x & 63) == 0) atomicAdd(fail + 21, (int)__popcll(fb));
+                if (kL == 64 && lane == 0 && ok && !try_hull) atomicAdd(fail + 22, 1);
+                if (kL == 64 && lane == 0 && try_hull && !hull_done) atomicAdd(fail + 23, 1);
             }
         }
         // the first pass (2-rings) leaves such a vertex to the full kernel:

@@ -3107,7 +3107,9 @@ void star_pass_stars(mcl_tri* t, hipStream_t s, int cur, const int* vlist, const
 // vertices to re-check, [10] vertices handed to the full kernel, [12..13]
 // vertices retried, [14] retry passes with a non-empty list, [16..18] corner
 // fans, [19] gave up, [21] filter-kernel vertices whose hull elimination
-// failed its certification (they took the successor scan)
+// failed its certification (they took the successor scan), [22] / [23]
+// vertices the wave-per-vertex filter kernels handed on (more candidates
+// than lanes / an uncertified hull)
 //
 // The first check (r = 0) covers every vertex and lists the inconsistent
 // ones; later ones (retry r) cover the re-check list, and also list a
@@ -3375,8 +3377,10 @@ static int star_step(mcl_tri* t, const mcl_step_params_t* p, const float2* disp,
         if (trace)
             fprintf(stderr,
                     "  [stars pass %d] inconsistent %d (failed stars so far %d: candidates %d undecided %d open %d "
-                    "outside %d; full-kernel vertices %d; filter-kernel vertices that took the successor scan %d)\n",
-                    pass, aux[star_list_slot(pass)], aux[0], aux[4], aux[5], aux[6], aux[7], aux[10], aux[21]);
+                    "outside %d; full-kernel vertices %d; filter-kernel vertices that took the successor scan %d; "
+                    "wave-per-vertex filter kernels: more candidates than lanes %d, uncertified hulls %d)\n",
+                    pass, aux[star_list_slot(pass)], aux[0], aux[4], aux[5], aux[6], aux[7], aux[10], aux[21], aux[22],
+                    aux[23]);
         return MCL_OK;
     };
     bool settled = false;
