@@ -271,6 +271,17 @@ __device__ __forceinline__ bool star_icc_sure(float4 A, float4 B, float4 C, floa
     return (perm > 7.9e-31f) & (fabsf(det) > kStarIccBound * perm);
 }
 
+// Orders the LDS accesses of one wave: k_star's per-vertex LDS data belong to
+// a group of 32 or 64 lanes, which lies inside one wave, and the LDS
+// operations of a wave execute in order; the fences keep the compiler from
+// moving accesses across.  (A block barrier would also make the block's
+// other waves, working on other vertices, wait for each other.)
+__device__ __forceinline__ void group_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Recomputes the stars of the listed vertices (all vertices when vlist is
 // null) with the current positions.  Candidates: the 2-ring of v in the star
 // arrays deg_a/nbr_a (the previous mesh).  With deg_b/nbr_b (the stars found
@@ -298,8 +309,10 @@ __device__ __forceinline__ bool star_icc_sure(float4 A, float4 B, float4 C, floa
 // kL lanes handle one vertex (64: one per wave, up to 128 candidates; 32: two
 // per wave, up to 64 candidates, which holds nearly every 2-ring and halves
 // the instructions per vertex).  The loop over vertices advances 256 / kL
-// vertices per block iteration so that every wave of a block meets the same
-// barriers.  kFast: FP32 filter only, a test it cannot decide fails the
+// vertices per block iteration; a vertex's LDS data stay inside its wave, so
+// its stages are ordered by wave-level syncs (group_sync), and each wave
+// moves on without waiting for the block's others.  kFast: FP32 filter only,
+// a test it cannot decide fails the
 // vertex (reason 5) for a later pass with the full cascade; this variant
 // carries none of the exact-arithmetic code and keeps a small register
 // footprint.
@@ -407,7 +420,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                 }
                 if (at < kC) list[at] = urow[q];
             }
-            __syncthreads();
+            group_sync();
         } else if (kCand == kCandRaw) {
             // ---- v's old and new first rings (and the check's hint), each
             // with its star in this step's stars (a corner: the entries of its
@@ -433,7 +446,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
             if ((unsigned)u >= (unsigned)m.nv || u == v) u = -1;
             const unsigned hu = ((unsigned)u * 2654435761u) >> (32 - kBits);
             if (u >= 0) tab[hu] = lane;
-            __syncthreads();
+            group_sync();
             {
                 const int w = u >= 0 ? tab[hu] : lane;
                 const int uw = __shfl(u, gbase + w);
@@ -475,7 +488,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                     if (at + k < kC) list[at + k] = urow[q];
                 }
             }
-            __syncthreads();
+            group_sync();
             // the gathered entries, two per lane: ids in range other than v,
             // one copy of each
             constexpr int kPer = kC / kL;
@@ -489,7 +502,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                 hc[h] = ((unsigned)c[h] * 2654435761u) >> (32 - kBits);
                 if (c[h] >= 0) tab[hc[h]] = j;
             }
-            __syncthreads();
+            group_sync();
             unsigned long long keep[kPer];
 #pragma unroll
             for (int h = 0; h < kPer; h++) {
@@ -500,7 +513,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                 }
                 keep[h] = __ballot(c[h] >= 0) & gmask;
             }
-            __syncthreads();
+            group_sync();
             int before = 0;
 #pragma unroll
             for (int h = 0; h < kPer; h++) {
@@ -512,12 +525,12 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
             // full kernel (no candidate count, so that no row is read from
             // entries the squeeze did not write)
             kc = (raw > kC || widen) ? 0 : before;
-            __syncthreads();
+            group_sync();
         } else {
             // ---- candidates, de-duplicated in LDS, listed ring by ring
             for (int k = 0; k < kH / kL; k++) tab[lane + kL * k] = -1;
             if (lane == 0) *count = 0;
-            __syncthreads();
+            group_sync();
             const int db = (active && deg_b) ? deg_b[v] : 0;
             // a vertex whose star failed (it left its 2-ring) widens through the
             // previous stars; a valid but inconsistent one expands its first ring
@@ -541,7 +554,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                     if ((unsigned)hv < (unsigned)m.nv && hv != v) cand_insert<kH, kC>(tab, list, count, hv);
                 }
             }
-            __syncthreads();
+            group_sync();
             // further rings: expand the entries the previous ring added
             int lo = 0, hi = *count;
             for (int dd = 2; dd <= max_depth; dd++) {
@@ -556,7 +569,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                             insert_star<kH, kC>(tab, list, count, deg_b, nbr_b, u, v, m.nv, m.ncorner);
                     }
                 }
-                __syncthreads();
+                group_sync();
                 lo = hi;
                 hi = *count;
             }
@@ -577,7 +590,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                 s_row[g][j] = make_float4(x, y, x * x + y * y, __int_as_float(list[j]));
             }
         }
-        __syncthreads();
+        group_sync();
         bool ok = active && kc >= 3 && kc <= kC;
         int why = ok ? 0 : 4;  // failure reason slot in fail[] (0: none)
         const float2 pv = m.pos[v];
@@ -674,11 +687,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
             }
             static_assert((1 << kSlotBits) == kL, "slot bits");
             s_key[g][lane] = key;
-            // (LDS operations of one wave complete in order; the fences keep
-            // the compiler from moving the reads above the write)
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            group_sync();
             int kc_w = kc_h;
             if (kL == 32) kc_w = max(kc_w, __shfl_xor(kc_w, 32));
             kc_w = __builtin_amdgcn_readfirstlane(kc_w);
@@ -868,7 +877,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                 s_succ[g][j] = (undecided || a < 0) ? -1 : (to_start ? a + kC : a);
             }
         }
-        __syncthreads();
+        group_sync();
         // ---- trace the hull from the start through the successors
         int my_slot = -1, d = 0;
         if (ok && !hull_done) {
