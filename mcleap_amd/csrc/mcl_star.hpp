@@ -316,8 +316,9 @@ __device__ __forceinline__ void group_sync() {
 // vertex (reason 5) for a later pass with the full cascade; this variant
 // carries none of the exact-arithmetic code and keeps a small register
 // footprint.
+constexpr int kStarBlock = 128;  // threads per block of k_star (two waves; 256 and 64 measured slower)
 template <bool kF32, bool kFast, int kL, int kCand>
-__global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ vlist, int nlist,
+__global__ void __launch_bounds__(kStarBlock) k_star(DMesh m, const int* __restrict__ vlist, int nlist,
                                               const int* __restrict__ nlist_dev, const int* __restrict__ rows,
                                               const int* __restrict__ nrows, int depth,
                                               const int* __restrict__ deg_a, const int* __restrict__ nbr_a,
@@ -329,7 +330,7 @@ __global__ void __launch_bounds__(256) k_star(DMesh m, const int* __restrict__ v
                                               int give_up, int* __restrict__ gave_up, const int* __restrict__ hint) {
     static_assert(kL == 32 || kL == 64, "lanes per vertex");
     static_assert(kCand != kCandRaw || kL == 64, "the raw gather takes a first ring of up to 49 entries, one per lane");
-    constexpr int kV = 256 / kL;                    // vertices per block iteration
+    constexpr int kV = kStarBlock / kL;             // vertices per block iteration
     constexpr int kC = kL == 64 ? kCandMax : 64;    // candidates per vertex
     constexpr int kH = 2 * kC;                      // hash slots per vertex
     // the successor scan (below) is compiled into the full kernel and the

@@ -3044,6 +3044,14 @@ struct StarGrid {
     }
     // list lengths stay on the device: grids sized for a long list
     dim3 list() const { return dim3((unsigned)(std::min<long long>(4LL * t->grid_blocks, 2048) & ~7LL)); }
+    // the same numbers of threads in blocks of kStarBlock (k_star)
+    dim3 star_full(long long nverts, int kv) const {
+        const long long scale = 256 / kStarBlock;
+        long long b = std::max<long long>(1, std::min<long long>((nverts + kv - 1) / kv, 4LL * scale * t->grid_blocks));
+        if (b >= 64) b &= ~7LL;
+        return dim3((unsigned)b);
+    }
+    dim3 star_list() const { return dim3(list().x * (256 / kStarBlock)); }
 };
 
 // stars of the listed vertices (all without a list; count *nlist_dev) into
@@ -3077,25 +3085,25 @@ void star_pass_stars(mcl_tri* t, hipStream_t s, int cur, const int* vlist, const
     const StarGrid G{t};
     const int* heavy_rows = t->s_heavy;
     if (!vlist) {
-        hipLaunchKernelGGL((k_star<true, true, 32, kCandRing2>), G.full(m.nv, 8), dim3(256), 0, s, m, vlist, 0,
+        hipLaunchKernelGGL((k_star<true, true, 32, kCandRing2>), G.star_full(m.nv, kStarBlock / 32), dim3(kStarBlock), 0, s, m, vlist, 0,
                            nlist_dev, (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
                            (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy,
                            (const float2*)t->acc, (const float2*)t->tgt, t->stats, t->s_heavy2, give_up,
                            t->s_aux + 19, (const int*)nullptr);
-        hipLaunchKernelGGL((k_star<true, true, 64, kCandRing2>), dim3(256), dim3(256), 0, s, m, vlist, 0, nlist_dev,
+        hipLaunchKernelGGL((k_star<true, true, 64, kCandRing2>), dim3(512), dim3(kStarBlock), 0, s, m, vlist, 0, nlist_dev,
                            (const int*)(t->s_heavy + 1), (const int*)t->s_heavy, depth,
                            (const int*)t->star_deg[cur], (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr,
                            t->s_aux, t->s_heavy2, (const float2*)t->acc, (const float2*)t->tgt, t->stats,
                            (int*)nullptr, -1, (int*)nullptr, (const int*)nullptr);
         heavy_rows = t->s_heavy2;
     } else {
-        hipLaunchKernelGGL((k_star<true, true, 64, kCandRaw>), G.list(), dim3(256), 0, s, m, vlist, 0, nlist_dev,
+        hipLaunchKernelGGL((k_star<true, true, 64, kCandRaw>), G.star_list(), dim3(kStarBlock), 0, s, m, vlist, 0, nlist_dev,
                            (const int*)nullptr, (const int*)nullptr, depth, (const int*)t->star_deg[cur],
                            (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, t->s_heavy,
                            (const float2*)t->acc, (const float2*)t->tgt, t->stats, reset, give_up, t->s_aux + 19,
                            (const int*)t->s_hint);
     }
-    hipLaunchKernelGGL((k_star<true, false, 64, kCandHash>), dim3(256), dim3(256), 0, s, m, vlist, 0, nlist_dev,
+    hipLaunchKernelGGL((k_star<true, false, 64, kCandHash>), dim3(512), dim3(kStarBlock), 0, s, m, vlist, 0, nlist_dev,
                        heavy_rows + 1, heavy_rows, depth, (const int*)t->star_deg[cur],
                        (const int*)t->star_nbr[cur], bdeg, bnbr, odeg, onbr, t->s_aux, (int*)nullptr,
                        (const float2*)t->acc, (const float2*)t->tgt, t->stats, (int*)nullptr, -1, (int*)nullptr,
