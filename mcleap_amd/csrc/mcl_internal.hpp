@@ -247,6 +247,53 @@ __device__ __forceinline__ void append_finish(AppendBuf<kN>& ab, int* count, int
     if (ab.fill > 0) append_flush(ab, count, list);
 }
 
+// The same buffer without barriers between tiles: a wave's items go to the
+// LDS buffer while it has room and the rest straight to the list (one global
+// reservation per wave for what does not fit), and the buffer is written out
+// once, by append_finish_nb.  Positions are handed out by one LDS atomic per
+// wave, so the waves of a block never wait for each other until the end.
+struct AppendSlot {
+    int at;     // buffer position of this thread's first item
+    int gbase;  // list index of the item at buffer position p >= kN: gbase + p
+};
+template <int kN>
+__device__ __forceinline__ AppendSlot append_reserve_nb(AppendBuf<kN>& ab, int n, int* count) {
+    const int lane = threadIdx.x & 63;
+    int x = n;
+#pragma unroll
+    for (int dd = 1; dd < 64; dd <<= 1) {
+        const int y = __shfl_up(x, dd);
+        if (lane >= dd) x += y;
+    }
+    const int wtot = __shfl(x, 63);
+    int old = 0;
+    if (lane == 63 && wtot) old = atomicAdd(&ab.fill, wtot);
+    old = __shfl(old, 63);
+    const int cut = old > kN ? old : kN;             // first position of this wave past the buffer
+    const int over = old + wtot > cut ? old + wtot - cut : 0;
+    int g = 0;
+    if (lane == 63 && over) g = atomicAdd(count, over);
+    g = __shfl(g, 63);
+    return AppendSlot{old + x - n, g - cut};
+}
+template <int kN>
+__device__ __forceinline__ void append_put(AppendBuf<kN>& ab, const AppendSlot& sl, int k, int v,
+                                           int* __restrict__ list) {
+    const int p = sl.at + k;
+    if (p < kN) ab.buf[p] = v;
+    else list[sl.gbase + p] = v;
+}
+// every thread of the block, once, after its last append_put
+template <int kN>
+__device__ __forceinline__ void append_finish_nb(AppendBuf<kN>& ab, int* count, int* __restrict__ list) {
+    __syncthreads();
+    const int nf = ab.fill < kN ? ab.fill : kN;
+    if (threadIdx.x == 0 && nf) ab.base = atomicAdd(count, nf);
+    __syncthreads();
+    const int base = ab.base;
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) list[base + i] = ab.buf[i];
+}
+
 __device__ __forceinline__ void xcd_split(int n, int per_block, int& first, int& end, int& stride) {
     const int nb = gridDim.x;
     if (nb % kXcds != 0) {

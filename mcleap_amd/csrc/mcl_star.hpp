@@ -1075,9 +1075,10 @@ __device__ __forceinline__ bool star_inconsistent(const int* __restrict__ deg, c
 }
 
 // k_star_check with one thread per vertex, for the first check of a pass
-// (every vertex): 256 vertices share each barrier and flush of the append
-// buffer, where k_star_check's lanes per vertex make the short lists of the
-// later checks faster.  Checks the *nlist listed vertices (all when vlist is
+// (every vertex): the listed vertices go through an LDS buffer written out
+// once per block (append_reserve_nb: the block's waves never wait for each
+// other before the end), where k_star_check's lanes per vertex make the
+// short lists of the later checks faster.  Checks the *nlist listed vertices (all when vlist is
 // null).  Vertices that fail (or whose star failed) are appended to `list`:
 // of a conflict only the side that loses the incircle test
 // (star_inconsistent with `decide`), each vertex at most twice per tile (v,
@@ -1098,7 +1099,6 @@ __global__ void __launch_bounds__(256) k_star_check_all(int nv, const int* __res
     const int stamp = *stamp_base + stamp_off;
     if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;  // a counter zeroed for a later kernel
     append_init(ab);
-    const int max_tile = (int)blockDim.x * 2;
     long long my_sum = 0;
     const int n = vlist ? *nlist : nv;
     int first, end, stride;
@@ -1132,12 +1132,11 @@ __global__ void __launch_bounds__(256) k_star_check_all(int nv, const int* __res
                 claim_m = atomicExch(&mark[missing], stamp) != stamp;
             }
         }
-        int at = append_reserve(ab, (claim_v ? 1 : 0) + (claim_m ? 1 : 0));
-        if (claim_v) ab.buf[at++] = v;
-        if (claim_m) ab.buf[at++] = missing;
-        append_tile_end(ab, max_tile, count, list);
+        const AppendSlot sl = append_reserve_nb(ab, (claim_v ? 1 : 0) + (claim_m ? 1 : 0), count);
+        if (claim_v) append_put(ab, sl, 0, v, list);
+        if (claim_m) append_put(ab, sl, claim_v ? 1 : 0, missing, list);
     }
-    append_finish(ab, count, list);
+    append_finish_nb(ab, count, list);
     wave_add(my_sum, (unsigned long long*)sum_deg);
 }
 
