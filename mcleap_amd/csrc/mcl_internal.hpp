@@ -252,6 +252,10 @@ __device__ __forceinline__ void append_finish(AppendBuf<kN>& ab, int* count, int
 // reservation per wave for what does not fit), and the buffer is written out
 // once, by append_finish_nb.  Positions are handed out by one LDS atomic per
 // wave, so the waves of a block never wait for each other until the end.
+// For blocks that append less than the buffer holds (a few tiles each, as
+// the star checks and scatter over their lists); a block that sweeps many
+// tiles fills it early, and every later wave then takes a global atomic on
+// the one counter (the flip rounds' grids keep the per-tile flush).
 struct AppendSlot {
     int at;     // buffer position of this thread's first item
     int gbase;  // list index of the item at buffer position p >= kN: gbase + p
@@ -276,14 +280,16 @@ __device__ __forceinline__ AppendSlot append_reserve_nb(AppendBuf<kN>& ab, int n
     g = __shfl(g, 63);
     return AppendSlot{old + x - n, g - cut};
 }
+// (list null: the items are only counted)
 template <int kN>
 __device__ __forceinline__ void append_put(AppendBuf<kN>& ab, const AppendSlot& sl, int k, int v,
                                            int* __restrict__ list) {
     const int p = sl.at + k;
     if (p < kN) ab.buf[p] = v;
-    else list[sl.gbase + p] = v;
+    else if (list) list[sl.gbase + p] = v;
 }
-// every thread of the block, once, after its last append_put
+// every thread of the block, once, after its last append_put (list null:
+// only the count is added)
 template <int kN>
 __device__ __forceinline__ void append_finish_nb(AppendBuf<kN>& ab, int* count, int* __restrict__ list) {
     __syncthreads();
@@ -291,7 +297,8 @@ __device__ __forceinline__ void append_finish_nb(AppendBuf<kN>& ab, int* count, 
     if (threadIdx.x == 0 && nf) ab.base = atomicAdd(count, nf);
     __syncthreads();
     const int base = ab.base;
-    for (int i = threadIdx.x; i < nf; i += blockDim.x) list[base + i] = ab.buf[i];
+    if (list)
+        for (int i = threadIdx.x; i < nf; i += blockDim.x) list[base + i] = ab.buf[i];
 }
 
 __device__ __forceinline__ void xcd_split(int n, int per_block, int& first, int& end, int& stride) {

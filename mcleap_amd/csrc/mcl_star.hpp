@@ -1175,7 +1175,6 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
     const int stamp = *stamp_base + stamp_off;
     if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;  // a counter zeroed for a later kernel
     append_init(ab);
-    const int max_tile = kCV * (kStarCap + 2);  // v, its neighbours, the one missing v
     const int g = threadIdx.x / kCL, lane = threadIdx.x % kCL;
     const int gbase = (threadIdx.x & 63) - lane;
     const unsigned long long gmask = 0xffull << gbase;
@@ -1251,11 +1250,10 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
                 if (atomicExch(&mark[missing], stamp) != stamp) items[nclaim++] = missing;
             }
         }
-        const int at = append_reserve(ab, nclaim);
-        for (int k = 0; k < nclaim; k++) ab.buf[at + k] = items[k];
-        append_tile_end(ab, max_tile, count, list);
+        const AppendSlot sl = append_reserve_nb(ab, nclaim, count);
+        for (int k = 0; k < nclaim; k++) append_put(ab, sl, k, items[k], list);
     }
-    append_finish(ab, count, list);
+    append_finish_nb(ab, count, list);
     wave_add(my_sum, (unsigned long long*)sum_deg);
 }
 
@@ -1274,7 +1272,7 @@ __global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vl
                                                       int stamp_off, int* __restrict__ rlist,
                                                       int* __restrict__ rcount, long long* __restrict__ sum_deg,
                                                       long long* __restrict__ retried, int* __restrict__ reset) {
-    __shared__ AppendBuf<> ab;
+    __shared__ AppendBuf<kListBuf> ab;  // (what does not fit goes straight to rlist)
     const int stamp = *stamp_base + stamp_off;
     if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;  // a counter zeroed for a later kernel
     append_init(ab);
@@ -1310,10 +1308,11 @@ __global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vl
                     nclaim++;
                 }
         }
-        int at = append_reserve(ab, nclaim);
-        if (claimed & 1u) ab.buf[at++] = v;
+        const AppendSlot sl = append_reserve_nb(ab, nclaim, rcount);
+        int at = 0;
+        if (claimed & 1u) append_put(ab, sl, at++, v, rlist);
         for (int i = 0; i < od && i < kStarCap; i++)
-            if (claimed & (2u << i)) ab.buf[at++] = row[i];
+            if (claimed & (2u << i)) append_put(ab, sl, at++, row[i], rlist);
         if (active) {
             const int nd = tdeg[idx];
             delta += (nd >= 3 ? nd : 0) - (od >= 3 ? od : 0);
@@ -1329,9 +1328,8 @@ __global__ void __launch_bounds__(256) k_star_scatter(const int* __restrict__ vl
             for (int k = 8; k < nd; k++) row[k] = tnbr[(size_t)idx * kStarCap + k];
             deg[v] = nd;
         }
-        append_tile_end(ab, (int)blockDim.x * (kStarCap + 1), rcount, rlist);
     }
-    append_finish(ab, rcount, rlist);
+    append_finish_nb(ab, rcount, rlist);
     wave_add(delta, (unsigned long long*)sum_deg);
 }
 
