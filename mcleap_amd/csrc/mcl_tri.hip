@@ -3131,10 +3131,11 @@ void star_check(mcl_tri* t, hipStream_t s, int nxt, int r, bool mark_nbrs, int s
     DMesh& m = t->m;
     const StarGrid G{t};
     const bool first = r == 0;
-    // the first check (every vertex): one thread per vertex, 4 tiles of 256
-    // vertices per block; the later ones (lists): 8 lanes per vertex
+    // the first check (every vertex): one thread per vertex, 8 tiles of 256
+    // vertices per block (1, 4 and 16 measured slower: 259, 126 and 163 us
+    // against 115 at 1M); the later ones (lists): 8 lanes per vertex
     if (first)
-        hipLaunchKernelGGL(k_star_check_all, G.full(m.nv, 1024), dim3(256), 0, s, m.nv, (const int*)nullptr,
+        hipLaunchKernelGGL(k_star_check_all, G.full(m.nv, 2048), dim3(256), 0, s, m.nv, (const int*)nullptr,
                            (const int*)nullptr, (const int*)t->star_deg[nxt], (const int*)t->star_nbr[nxt],
                            (const float2*)m.pos, m.periodic, m.ncorner, t->s_mark,
                            (const int*)t->d_stamp, stamp_off, t->s_list, t->s_aux + star_list_slot(r),
