@@ -3051,7 +3051,9 @@ struct StarGrid {
         if (b >= 64) b &= ~7LL;
         return dim3((unsigned)b);
     }
-    dim3 star_list() const { return dim3(list().x * (256 / kStarBlock)); }
+    // (twice the threads of list(): the retry's wave-per-vertex kernel, 162 ->
+    // 137 us over ~81k vertices at 1M; half the threads measured 172 us)
+    dim3 star_list() const { return dim3(list().x * (256 / kStarBlock) * 2); }
 };
 
 // stars of the listed vertices (all without a list; count *nlist_dev) into
