@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(kT) k_frnn(GridParams g, const float2* __restr
                                              unsigned long long* __restrict__ hits) {
     __shared__ int s_row[kPass ? kT / 64 : 1][kPass ? kWaveRow : 1];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    // block-uniform loop: every thread meets the barriers; each XCD takes one
+    // each wave stages its own rows (wave-level syncs); each XCD takes one
     // contiguous range of the sorted points
     for (int bb = xcd_block() * blockDim.x; bb < n; bb += gridDim.x * blockDim.x) {
         const int i = bb + threadIdx.x;
@@ -235,11 +235,11 @@ __global__ void __launch_bounds__(kT) k_frnn(GridParams g, const float2* __restr
             if (hits) hits[i] = nb <= 63 ? hm : kHitsOverflow;
         }
         if (kPass) {
-            __syncthreads();
+            wave_sync();
             if (staged)
                 for (int k = lane; k < wn; k += 64)
                     if (w0 + k < cap) nbr[w0 + k] = s_row[wv][k];
-            __syncthreads();
+            wave_sync();
         }
     }
 }
@@ -662,7 +662,7 @@ __global__ void __launch_bounds__(kT) k_bfrnn(BGridParams b, const float2* __res
                                               int cap) {
     __shared__ int s_row[kPass ? kT / 64 : 1][kPass ? kWaveRow : 1];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    // block-uniform loop: every thread meets the barriers; each XCD takes one
+    // each wave stages its own rows (wave-level syncs); each XCD takes one
     // contiguous range of the sorted points
     for (int bb = xcd_block() * blockDim.x; bb < b.n; bb += gridDim.x * blockDim.x) {
         const int i = bb + threadIdx.x;
@@ -706,11 +706,11 @@ __global__ void __launch_bounds__(kT) k_bfrnn(BGridParams b, const float2* __res
         }
         if (!kPass && act) cnt[i] = c;
         if (kPass) {
-            __syncthreads();
+            wave_sync();
             if (staged)
                 for (int k = lane; k < wn; k += 64)
                     if (w0 + k < cap) nbr[w0 + k] = s_row[wv][k];
-            __syncthreads();
+            wave_sync();
         }
     }
 }

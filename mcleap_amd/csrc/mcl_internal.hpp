@@ -193,6 +193,17 @@ constexpr int kRingWave = 1024;
 // single counter serialises its atomics: one reservation per block per
 // iteration of a grid-stride loop (thousands of blocks, hundreds of
 // iterations) costs milliseconds by itself.
+// Orders the LDS accesses of one wave where LDS data belong to one wave (a
+// wave's staged rows, a vertex's candidates): the LDS operations of a wave
+// execute in order, and the fences keep the compiler from moving accesses
+// across.  Unlike a block barrier it does not make the block's other waves
+// wait.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 constexpr int kAppendBuf = 8192;  // ints of LDS per block (the star checks' and scatter's buffer)
 constexpr int kListBuf = 2048;    // ints of LDS per block (the flip rounds' lists: up to 4 per thread a tile)
 template <int kN = kAppendBuf>

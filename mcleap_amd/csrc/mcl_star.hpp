@@ -276,11 +276,7 @@ __device__ __forceinline__ bool star_icc_sure(float4 A, float4 B, float4 C, floa
 // operations of a wave execute in order; the fences keep the compiler from
 // moving accesses across.  (A block barrier would also make the block's
 // other waves, working on other vertices, wait for each other.)
-__device__ __forceinline__ void group_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+__device__ __forceinline__ void group_sync() { wave_sync(); }
 
 // Recomputes the stars of the listed vertices (all vertices when vlist is
 // null) with the current positions.  Candidates: the 2-ring of v in the star
@@ -1571,7 +1567,7 @@ __global__ void __launch_bounds__(256) k_star_ring_csr(int nv, int ncorner, cons
                                                        int* __restrict__ out, int cap) {
     __shared__ int s_row[4][kRingWave];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    // block-uniform loop: every thread meets the barriers
+    // each wave stages its own rows (wave-level syncs)
     for (int bb = blockIdx.x * blockDim.x; bb < nv; bb += gridDim.x * blockDim.x) {
         const int v = bb + threadIdx.x;
         const int wb = bb + (threadIdx.x & ~63);
@@ -1606,11 +1602,11 @@ __global__ void __launch_bounds__(256) k_star_ring_csr(int nv, int ncorner, cons
                 }
             }
         }
-        __syncthreads();
+        wave_sync();
         if (staged)
             for (int k = lane; k < wn; k += 64)
                 if (w0 + k < cap) out[w0 + k] = s_row[wv][k];
-        __syncthreads();
+        wave_sync();
     }
 }
 

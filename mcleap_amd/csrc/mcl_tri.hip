@@ -1479,7 +1479,7 @@ __global__ void __launch_bounds__(kThreads) k_ring_csr(DMesh m, const int* __res
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int first, last, stride;
     xcd_split(m.nv, blockDim.x, first, last, stride);
-    // block-uniform loop: every thread meets the barriers
+    // each wave stages its own rows (wave-level syncs)
     for (int base = first; base < last; base += stride) {
         const int v = base + threadIdx.x;
         const int wb = base + (threadIdx.x & ~63);
@@ -1501,11 +1501,11 @@ __global__ void __launch_bounds__(kThreads) k_ring_csr(DMesh m, const int* __res
                 h = m.he_next[h ^ 1];
             } while (h != h0 && k < end);
         }
-        __syncthreads();
+        wave_sync();
         if (staged)
             for (int k = lane; k < wn; k += 64)
                 if (w0 + k < cap) nbr[w0 + k] = s_row[wv][k];
-        __syncthreads();
+        wave_sync();
     }
 }
 // oneRing<K> (layout 0) / oneRing2<K> (layout 1): kernels.cuh:54-121
