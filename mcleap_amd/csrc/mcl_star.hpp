@@ -1173,7 +1173,7 @@ __global__ void __launch_bounds__(256) k_star_check(int nv, const int* __restric
     append_init(ab);
     const int g = threadIdx.x / kCL, lane = threadIdx.x % kCL;
     const int gbase = (threadIdx.x & 63) - lane;
-    const unsigned long long gmask = 0xffull << gbase;
+    const unsigned long long gmask = ((1ull << kCL) - 1ull) << gbase;
     const bool decide = !mark_nbrs;
     long long my_sum = 0;
     const int n = vlist ? *nlist : nv;
